@@ -1,0 +1,1114 @@
+/*
+ * ceos_oracle.c -- CPU restatement of the CEOS element hot path (TEST INFRASTRUCTURE ONLY).
+ *
+ * The checker for libceos_gpu.so: tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg are the only users.  Every routine follows the reference file:line
+ * given in its comment, written with the same operation structure (dense B/G, D read from
+ * its upper triangle, Ke = sum_gp w detJ (B^T D B + G^T S G), CSR linear-search assembly)
+ * so that it is an independent check of the structure-exploiting GPU kernels.
+ *
+ * Parity pins (tests/test_oracle_*.py): closed forms and invariants for the shape functions,
+ * B/G, F and volume; finite-difference checks of every material tangent and of Ke = dFe/du;
+ * brute-force and closed-form CSR patterns; the reference's own golden data from
+ * Checker/Cases/{NonLinear_FEA_3D_Hexa8_StVenant, Linear_FEA_3D_Hexa8_GeneralizedHookesLaw}
+ * (Newton residual histories, Marc and CEOS displacements, Gauss-point stresses).
+ */
+#include "ceos_oracle.h"
+
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define NGP 8
+#define NPE 8
+#define NDE 24
+#define MAXMAT 64
+#define MAXSTATE 32
+
+/* ------------------------------------------------------------------------------------------
+ * small 3x3 helpers, Tools/MathTools/ModMathRoutines.f90:16-62 (det = rule of Sarrus in the
+ * same term order; inverse = adjugate / det)
+ * ---------------------------------------------------------------------------------------- */
+static double det3(const double M[3][3]) {
+    return M[0][0] * M[1][1] * M[2][2] + M[0][1] * M[1][2] * M[2][0] + M[0][2] * M[1][0] * M[2][1] -
+           M[0][2] * M[1][1] * M[2][0] - M[0][1] * M[1][0] * M[2][2] - M[0][0] * M[1][2] * M[2][1];
+}
+
+static void inv3(const double A[3][3], double B[3][3]) {
+    double d = det3(A);
+    B[0][0] = A[1][1] * A[2][2] - A[1][2] * A[2][1];
+    B[0][1] = A[0][2] * A[2][1] - A[0][1] * A[2][2];
+    B[0][2] = A[0][1] * A[1][2] - A[0][2] * A[1][1];
+    B[1][0] = A[1][2] * A[2][0] - A[1][0] * A[2][2];
+    B[1][1] = A[0][0] * A[2][2] - A[0][2] * A[2][0];
+    B[1][2] = A[0][2] * A[1][0] - A[0][0] * A[1][2];
+    B[2][0] = A[1][0] * A[2][1] - A[1][1] * A[2][0];
+    B[2][1] = A[0][1] * A[2][0] - A[0][0] * A[2][1];
+    B[2][2] = A[0][0] * A[1][1] - A[0][1] * A[1][0];
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) B[i][j] /= d;
+}
+
+static void matmul3(const double A[3][3], const double B[3][3], double C[3][3]) {
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) {
+            double s = 0.0;
+            for (int k = 0; k < 3; k++) s += A[i][k] * B[k][j];
+            C[i][j] = s;
+        }
+}
+
+static void transpose3(const double A[3][3], double B[3][3]) {
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) B[i][j] = A[j][i];
+}
+
+/* ------------------------------------------------------------------------------------------
+ * hex8, FEM/Elements/ElementLibrary/ModElementHexa8.f90:150-329
+ * node order (---,+--,++-,-+-,--+,+-+,+++,-++), Gauss points = node signs / sqrt(3), w = 1
+ * ---------------------------------------------------------------------------------------- */
+static const double HEX8_ID[8][3] = {{-1, -1, -1}, {1, -1, -1}, {1, 1, -1}, {-1, 1, -1},
+                                     {-1, -1, 1},  {1, -1, 1},  {1, 1, 1},  {-1, 1, 1}};
+
+void or_hex8_gauss(double nat[8][3], double w[8]) {
+    double x = 1.0 / sqrt(3.0);
+    for (int g = 0; g < 8; g++) {
+        for (int k = 0; k < 3; k++) nat[g][k] = HEX8_ID[g][k] * x;
+        w[g] = 1.0;
+    }
+}
+
+void or_hex8_shape(const double nat[3], double N[8]) {
+    for (int i = 0; i < 8; i++)
+        N[i] = (1.0 + HEX8_ID[i][0] * nat[0]) * (1.0 + HEX8_ID[i][1] * nat[1]) *
+               (1.0 + HEX8_ID[i][2] * nat[2]) / 8.0;
+}
+
+void or_hex8_dshape(const double nat[3], double dN[8][3]) {
+    double xi = nat[0], eta = nat[1], zeta = nat[2];
+    for (int i = 0; i < 8; i++) {
+        dN[i][0] = HEX8_ID[i][0] * (1.0 + HEX8_ID[i][1] * eta) * (1.0 + HEX8_ID[i][2] * zeta) / 8.0;
+        dN[i][1] = HEX8_ID[i][1] * (1.0 + HEX8_ID[i][0] * xi) * (1.0 + HEX8_ID[i][2] * zeta) / 8.0;
+        dN[i][2] = HEX8_ID[i][2] * (1.0 + HEX8_ID[i][0] * xi) * (1.0 + HEX8_ID[i][1] * eta) / 8.0;
+    }
+}
+
+/* Jacob(i,j) = sum_n dN(n,i) coord(n,j); gradN(n,:) = Jacob^-1 dN(n,:)
+ * FEM/Elements/ElementMethods/ModElement.f90:848-866 */
+static int grad_shape(const double coord[8][3], const double nat[3], double grad[8][3], double *detJ) {
+    double dN[8][3], Jac[3][3], Ji[3][3];
+    or_hex8_dshape(nat, dN);
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) {
+            double s = 0.0;
+            for (int n = 0; n < 8; n++) s += dN[n][i] * coord[n][j];
+            Jac[i][j] = s;
+        }
+    *detJ = det3(Jac);
+    if (*detJ <= 1.0e-13) return OR_ERR_DETJ;
+    inv3(Jac, Ji);
+    for (int n = 0; n < 8; n++)
+        for (int i = 0; i < 3; i++)
+            grad[n][i] = Ji[i][0] * dN[n][0] + Ji[i][1] * dN[n][1] + Ji[i][2] * dN[n][2];
+    return OR_OK;
+}
+
+/* B rows: e11,e22,e33,g12,g23,g13; G rows: du_i/dx_j, i-major.
+ * FEM/Elements/ElementMethods/ModElement.f90:867-888 */
+int or_matrix_bg(const double coord[8][3], const double nat[3], double B[6][24], double G[9][24],
+                 double *detJ) {
+    double g[8][3];
+    int st = grad_shape(coord, nat, g, detJ);
+    if (st) return st;
+    memset(B, 0, sizeof(double) * 6 * 24);
+    memset(G, 0, sizeof(double) * 9 * 24);
+    for (int n = 0; n < 8; n++) {
+        int c = 3 * n;
+        B[0][c + 0] = g[n][0];
+        B[1][c + 1] = g[n][1];
+        B[2][c + 2] = g[n][2];
+        B[3][c + 0] = g[n][1];
+        B[3][c + 1] = g[n][0];
+        B[4][c + 2] = g[n][1];
+        B[4][c + 1] = g[n][2];
+        B[5][c + 2] = g[n][0];
+        B[5][c + 0] = g[n][2];
+        for (int j = 0; j < 3; j++) {
+            G[0 + j][c + 0] = g[n][j];
+            G[3 + j][c + 1] = g[n][j];
+            G[6 + j][c + 2] = g[n][j];
+        }
+    }
+    return OR_OK;
+}
+
+/* F(i,j) = delta_ij + sum_n dN(n,j)/dX u_n,i, FEM/Elements/ElementMethods/ModElement.f90:1210-1300 */
+int or_deformation_gradient(const double X[8][3], const double nat[3], const double Ue[24],
+                            double F[3][3]) {
+    double g[8][3], detJ;
+    int st = grad_shape(X, nat, g, &detJ);
+    if (st) return st;
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) {
+            double s = 0.0;
+            for (int n = 0; n < 8; n++) s += g[n][j] * Ue[3 * n + i];
+            F[i][j] = s;
+        }
+    for (int i = 0; i < 3; i++) F[i][i] += 1.0;
+    return OR_OK;
+}
+
+/* ------------------------------------------------------------------------------------------
+ * Voigt helpers, Tools/MathTools/ModMathRoutines.f90:194-205, 532-660
+ * ---------------------------------------------------------------------------------------- */
+static void ball_v(const double a[6], const double b[6], double T[6][6]) {
+    for (int i = 0; i < 6; i++)
+        for (int j = 0; j < 6; j++) T[i][j] = a[i] * b[j];
+}
+
+static void square_v(const double A[6], const double B[6], double T[6][6]) {
+    T[0][0] = A[0] * B[0];
+    T[0][1] = A[3] * B[3];
+    T[0][2] = A[5] * B[5];
+    T[0][3] = A[0] * B[3] / 2 + A[3] * B[0] / 2;
+    T[0][4] = A[3] * B[5] / 2 + A[5] * B[3] / 2;
+    T[0][5] = A[0] * B[5] / 2 + A[5] * B[0] / 2;
+    T[1][0] = A[3] * B[3];
+    T[1][1] = A[1] * B[1];
+    T[1][2] = A[4] * B[4];
+    T[1][3] = A[3] * B[1] / 2 + A[1] * B[3] / 2;
+    T[1][4] = A[1] * B[4] / 2 + A[4] * B[1] / 2;
+    T[1][5] = A[3] * B[4] / 2 + A[4] * B[3] / 2;
+    T[2][0] = A[5] * B[5];
+    T[2][1] = A[4] * B[4];
+    T[2][2] = A[2] * B[2];
+    T[2][3] = A[5] * B[4] / 2 + A[4] * B[5] / 2;
+    T[2][4] = A[4] * B[2] / 2 + A[2] * B[4] / 2;
+    T[2][5] = A[5] * B[2] / 2 + A[2] * B[5] / 2;
+    T[3][0] = A[0] * B[3] / 2 + A[3] * B[0] / 2;
+    T[3][1] = A[3] * B[1] / 2 + A[1] * B[3] / 2;
+    T[3][2] = A[5] * B[4] / 2 + A[4] * B[5] / 2;
+    T[3][3] = A[0] * B[1] / 4 + A[3] * B[3] / 2 + A[1] * B[0] / 4;
+    T[3][4] = A[3] * B[4] / 4 + A[5] * B[1] / 4 + A[1] * B[5] / 4 + A[4] * B[3] / 4;
+    T[3][5] = A[0] * B[4] / 4 + A[5] * B[3] / 4 + A[3] * B[5] / 4 + A[4] * B[0] / 4;
+    T[4][0] = A[3] * B[5] / 2 + A[5] * B[3] / 2;
+    T[4][1] = A[1] * B[4] / 2 + A[4] * B[1] / 2;
+    T[4][2] = A[4] * B[2] / 2 + A[2] * B[4] / 2;
+    T[4][3] = A[3] * B[4] / 4 + A[5] * B[1] / 4 + A[1] * B[5] / 4 + A[4] * B[3] / 4;
+    T[4][4] = A[1] * B[2] / 4 + A[4] * B[4] / 2 + A[2] * B[1] / 4;
+    T[4][5] = A[3] * B[2] / 4 + A[5] * B[4] / 4 + A[4] * B[5] / 4 + A[2] * B[3] / 4;
+    T[5][0] = A[0] * B[5] / 2 + A[5] * B[0] / 2;
+    T[5][1] = A[3] * B[4] / 2 + A[4] * B[3] / 2;
+    T[5][2] = A[5] * B[2] / 2 + A[2] * B[5] / 2;
+    T[5][3] = A[0] * B[4] / 4 + A[5] * B[3] / 4 + A[3] * B[5] / 4 + A[4] * B[0] / 4;
+    T[5][4] = A[3] * B[2] / 4 + A[5] * B[4] / 4 + A[4] * B[5] / 4 + A[2] * B[3] / 4;
+    T[5][5] = A[0] * B[2] / 4 + A[5] * B[5] / 2 + A[2] * B[0] / 4;
+}
+
+static double inner_v(const double a[6], const double b[6]) {
+    return a[0] * b[0] + a[1] * b[1] + a[2] * b[2] + 2.0 * a[3] * b[3] + 2.0 * a[4] * b[4] +
+           2.0 * a[5] * b[5];
+}
+
+static void to_voigt(const double A[3][3], double v[6]) {
+    v[0] = A[0][0];
+    v[1] = A[1][1];
+    v[2] = A[2][2];
+    v[3] = A[0][1];
+    v[4] = A[1][2];
+    v[5] = A[0][2];
+}
+
+/* T = (1/J) A W_sym A^T with W read from its upper triangle (dsymm 'R','U'),
+ * Tools/MathTools/ModMathRoutines.f90:254-311 */
+static void push_forward_v(const double W[6][6], const double F[3][3], double T[6][6]) {
+    double J = det3(F), Ft[3][3], A[6][6], Ws[6][6], AW[6][6];
+    transpose3(F, Ft);
+#define FT(i, j) Ft[(i)-1][(j)-1]
+    A[0][0] = FT(1, 1) * FT(1, 1);
+    A[0][1] = FT(2, 1) * FT(2, 1);
+    A[0][2] = FT(3, 1) * FT(3, 1);
+    A[0][3] = 2.0 * FT(1, 1) * FT(2, 1);
+    A[0][4] = 2.0 * FT(2, 1) * FT(3, 1);
+    A[0][5] = 2.0 * FT(1, 1) * FT(3, 1);
+    A[1][0] = FT(1, 2) * FT(1, 2);
+    A[1][1] = FT(2, 2) * FT(2, 2);
+    A[1][2] = FT(3, 2) * FT(3, 2);
+    A[1][3] = 2.0 * FT(1, 2) * FT(2, 2);
+    A[1][4] = 2.0 * FT(2, 2) * FT(3, 2);
+    A[1][5] = 2.0 * FT(1, 2) * FT(3, 2);
+    A[2][0] = FT(1, 3) * FT(1, 3);
+    A[2][1] = FT(2, 3) * FT(2, 3);
+    A[2][2] = FT(3, 3) * FT(3, 3);
+    A[2][3] = 2.0 * FT(1, 3) * FT(2, 3);
+    A[2][4] = 2.0 * FT(2, 3) * FT(3, 3);
+    A[2][5] = 2.0 * FT(1, 3) * FT(3, 3);
+    A[3][0] = FT(1, 1) * FT(1, 2);
+    A[3][1] = FT(2, 1) * FT(2, 2);
+    A[3][2] = FT(3, 1) * FT(3, 2);
+    A[3][3] = FT(1, 1) * FT(2, 2) + FT(2, 1) * FT(1, 2);
+    A[3][4] = FT(2, 1) * FT(3, 2) + FT(3, 1) * FT(2, 2);
+    A[3][5] = FT(3, 1) * FT(1, 2) + FT(1, 1) * FT(3, 2);
+    A[4][0] = FT(1, 2) * FT(1, 3);
+    A[4][1] = FT(2, 2) * FT(2, 3);
+    A[4][2] = FT(3, 2) * FT(3, 3);
+    A[4][3] = FT(1, 2) * FT(2, 3) + FT(2, 2) * FT(1, 3);
+    A[4][4] = FT(2, 2) * FT(3, 3) + FT(3, 2) * FT(2, 3);
+    A[4][5] = FT(3, 2) * FT(1, 3) + FT(1, 2) * FT(3, 3);
+    A[5][0] = FT(1, 1) * FT(1, 3);
+    A[5][1] = FT(2, 1) * FT(2, 3);
+    A[5][2] = FT(3, 1) * FT(3, 3);
+    A[5][3] = FT(1, 3) * FT(2, 1) + FT(2, 3) * FT(1, 1);
+    A[5][4] = FT(2, 3) * FT(3, 1) + FT(3, 3) * FT(2, 1);
+    A[5][5] = FT(3, 3) * FT(1, 1) + FT(1, 3) * FT(3, 1);
+#undef FT
+    for (int i = 0; i < 6; i++)
+        for (int j = 0; j < 6; j++) Ws[i][j] = (i <= j) ? W[i][j] : W[j][i];
+    for (int i = 0; i < 6; i++)
+        for (int j = 0; j < 6; j++) {
+            double s = 0.0;
+            for (int k = 0; k < 6; k++) s += A[i][k] * Ws[k][j];
+            AW[i][j] = s;
+        }
+    for (int i = 0; i < 6; i++)
+        for (int j = 0; j < 6; j++) {
+            double s = 0.0;
+            for (int k = 0; k < 6; k++) s += AW[i][k] * A[j][k];
+            T[i][j] = s / J;
+        }
+}
+
+/* ------------------------------------------------------------------------------------------
+ * materials
+ * ---------------------------------------------------------------------------------------- */
+static void elastic_D(double E, double nu, double D[6][6]) {
+    /* ModGeneralizedHookesLaw .f90:526-560 and ModJ2Plasticity.f90:587-597 */
+    double cte = E / ((1.0 + nu) * (1.0 - 2.0 * nu));
+    memset(D, 0, sizeof(double) * 36);
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) D[i][j] = cte * ((i == j) ? (1.0 - nu) : nu);
+    for (int i = 3; i < 6; i++) D[i][i] = cte * (0.5 - nu);
+}
+
+static void small_strain(const double F[3][3], double e[6]) {
+    e[0] = F[0][0] - 1.0;
+    e[1] = F[1][1] - 1.0;
+    e[2] = F[2][2] - 1.0;
+    e[3] = F[0][1] + F[1][0];
+    e[4] = F[1][2] + F[2][1];
+    e[5] = F[0][2] + F[2][0];
+}
+
+/* model 1: ModGeneralizedHookesLaw .f90:475-520 */
+static void mat_ghl(const double *p, const double F[3][3], double s[6], double D[6][6]) {
+    double e[6];
+    small_strain(F, e);
+    elastic_D(p[0], p[1], D);
+    for (int i = 0; i < 6; i++) {
+        double acc = 0.0;
+        for (int j = 0; j < 6; j++) acc += D[i][j] * e[j];
+        s[i] = acc;
+    }
+}
+
+/* model 7: ModCompressibleNeoHookean.f90:422-546 */
+static void mat_cnh(const double *p, const double F[3][3], double s[6], double D[6][6]) {
+    double mu = p[0], lam = p[1], Ft[3][3], b[3][3];
+    transpose3(F, Ft);
+    matmul3(F, Ft, b);
+    double J = det3(F), lnJ = log(J);
+    double S[3][3];
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++)
+            S[i][j] = (mu / J) * (b[i][j] - (i == j ? 1.0 : 0.0)) + ((lam / J) * lnJ) * (i == j ? 1.0 : 0.0);
+    to_voigt(S, s);
+    double a = lam / J, c = (2.0 / J) * (mu - lam * lnJ);
+    memset(D, 0, sizeof(double) * 36);
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) D[i][j] = a;
+    for (int i = 0; i < 3; i++) D[i][i] += c;
+    for (int i = 3; i < 6; i++) D[i][i] = c * 0.5;
+}
+
+/* model 3: ModNeoHookean.f90:409-615 */
+static void mat_nh(const double *p, const double F[3][3], double s[6], double D[6][6]) {
+    double C10 = p[0], K = p[1], Ft[3][3], b[3][3];
+    transpose3(F, Ft);
+    matmul3(F, Ft, b);
+    double trb = b[0][0] + b[1][1] + b[2][2];
+    double J = det3(F);
+    double pr = 3.0 * K * pow(J, -2.0 / 3.0) * (pow(J, 1.0 / 3.0) - 1.0);
+    double S[3][3];
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++)
+            S[i][j] = 2.0 * C10 * pow(J, -5.0 / 3.0) * (b[i][j] - (trb / 3.0) * (i == j ? 1.0 : 0.0)) +
+                      pr * (i == j ? 1.0 : 0.0);
+    to_voigt(S, s);
+    /* tangent */
+    double C[3][3], Ci[3][3], CV[6], CiV[6];
+    matmul3(Ft, F, C);
+    inv3(C, Ci);
+    to_voigt(C, CV);
+    to_voigt(Ci, CiV);
+    double Sf[6] = {2.0 * C10, 2.0 * C10, 2.0 * C10, 0, 0, 0};
+    double ipSC = inner_v(Sf, CV);
+    double devSf[6], Siso[6];
+    for (int i = 0; i < 6; i++) devSf[i] = Sf[i] - (1.0 / 3.0) * ipSC * CiV[i];
+    double Jm23 = pow(J, -2.0 / 3.0);
+    for (int i = 0; i < 6; i++) Siso[i] = Jm23 * devSf[i];
+    double sq[6][6], bl[6][6], Pm[6][6], b1[6][6], b2[6][6], Cbar[6][6];
+    square_v(CiV, CiV, sq);
+    ball_v(CiV, CiV, bl);
+    for (int i = 0; i < 6; i++)
+        for (int j = 0; j < 6; j++) Pm[i][j] = sq[i][j] - (1.0 / 3.0) * bl[i][j];
+    double d2 = -K * pow(J, -5.0 / 3.0) * (pow(J, 1.0 / 3.0) - 2.0);
+    ball_v(Siso, CiV, b1);
+    ball_v(CiV, Siso, b2);
+    for (int i = 0; i < 6; i++)
+        for (int j = 0; j < 6; j++) {
+            double ciso = (2.0 / 3.0) * Jm23 * ipSC * Pm[i][j] - (2.0 / 3.0) * (b1[i][j] + b2[i][j]);
+            double cvol = J * (pr + J * d2) * bl[i][j] - 2.0 * J * pr * sq[i][j];
+            Cbar[i][j] = ciso + cvol;
+        }
+    push_forward_v(Cbar, F, D);
+}
+
+/* model 5: ModStVenantKirchhoff.f90:643-773 (tangent: upper triangle only) */
+static void mat_svk(const double *p, const double F[3][3], double s[6], double D[6][6]) {
+    double E = p[0], nu = p[1];
+    double lam = nu * E / ((1.0 + nu) * (1.0 - 2.0 * nu));
+    double mu = E / (2.0 * (1.0 + nu));
+    double Ft[3][3], C[3][3], Ee[3][3], S[3][3], tmp[3][3], sig[3][3];
+    transpose3(F, Ft);
+    matmul3(Ft, F, C);
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) Ee[i][j] = 0.5 * (C[i][j] - (i == j ? 1.0 : 0.0));
+    double trE = Ee[0][0] + Ee[1][1] + Ee[2][2];
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) S[i][j] = lam * trE * (i == j ? 1.0 : 0.0) + 2.0 * mu * Ee[i][j];
+    double J = det3(F);
+    matmul3(S, Ft, tmp);
+    matmul3(F, tmp, sig);
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) sig[i][j] = (1.0 / J) * sig[i][j];
+    to_voigt(sig, s);
+    double b[3][3];
+    matmul3(F, Ft, b);
+    static const int vi[6][2] = {{0, 0}, {1, 1}, {2, 2}, {0, 1}, {1, 2}, {0, 2}};
+    memset(D, 0, sizeof(double) * 36);
+    for (int I = 0; I < 6; I++)
+        for (int K = I; K < 6; K++) {
+            int i = vi[I][0], j = vi[I][1], k = vi[K][0], l = vi[K][1];
+            D[I][K] = (lam / J) * b[i][j] * b[k][l] + (mu / J) * (b[i][k] * b[j][l] + b[i][l] * b[j][k]);
+        }
+}
+
+/* model 2: ModJ2Plasticity.f90:385-633.
+ * state layout (nstate = 17): [0..5] plastic strain, [6] acc. plastic strain,
+ * [7..12] S_dev_t, [13] q_t, [14] DeltaGamma, [15] PlasticState flag, [16] spare.
+ * state_old: only [0..6] are read (the committed values). */
+#define J2_NSTATE 17
+static void mat_j2(const double *p, const double F[3][3], const double *old, double s[6],
+                   double *nw) {
+    double E = p[0], nu = p[1], sy = p[2], H = p[3];
+    double strain[6];
+    small_strain(F, strain);
+    double K = E / (3.0 * (1.0 - 2.0 * nu));
+    double G = E / (2.0 * (1.0 + nu));
+    double Eet[6], Eedev[6], Sdt[6];
+    for (int i = 0; i < 6; i++) Eet[i] = strain[i] - old[i];
+    double evol = Eet[0] + Eet[1] + Eet[2];
+    for (int i = 0; i < 6; i++) Eedev[i] = Eet[i] - evol * (i < 3 ? 1.0 : 0.0) / 3.0;
+    double ep_acc_t = old[6];
+    double pt = K * evol;
+    for (int i = 0; i < 6; i++) Sdt[i] = G * Eedev[i] * (i < 3 ? 2.0 : 1.0);
+    double qt = sqrt((3.0 / 2.0) * (Sdt[0] * Sdt[0] + Sdt[1] * Sdt[1] + Sdt[2] * Sdt[2] +
+                                    2.0 * Sdt[3] * Sdt[3] + 2.0 * Sdt[4] * Sdt[4] + 2.0 * Sdt[5] * Sdt[5]));
+    double phi = qt - (sy + H * ep_acc_t);
+    for (int i = 0; i < 6; i++) nw[7 + i] = Sdt[i];
+    nw[13] = qt;
+    nw[16] = 0.0;
+    if (phi <= 0.0) {
+        nw[15] = 0.0;
+        nw[6] = ep_acc_t;
+        for (int i = 0; i < 6; i++) nw[i] = old[i];
+        for (int i = 0; i < 6; i++) s[i] = Sdt[i] + pt * (i < 3 ? 1.0 : 0.0);
+        nw[14] = 0.0;
+    } else {
+        nw[15] = 1.0;
+        double dg = phi / (3.0 * G + H);
+        nw[14] = dg;
+        double Sd[6], Ee[6];
+        for (int i = 0; i < 6; i++) Sd[i] = (1.0 - dg * 3.0 * G / qt) * Sdt[i];
+        for (int i = 0; i < 6; i++) s[i] = Sd[i] + pt * (i < 3 ? 1.0 : 0.0);
+        for (int i = 0; i < 6; i++) Ee[i] = Sd[i] / (2.0 * G) + evol * (i < 3 ? 1.0 : 0.0) / 3.0;
+        nw[6] = old[6] + dg;
+        for (int i = 0; i < 6; i++) nw[i] = strain[i] - Ee[i];
+    }
+}
+
+static void mat_j2_tangent(const double *p, const double *st, double D[6][6]) {
+    double E = p[0], nu = p[1], H = p[3];
+    if (st[15] == 0.0) {
+        elastic_D(E, nu, D);
+        return;
+    }
+    double K = E / (3.0 * (1.0 - 2.0 * nu));
+    double G = E / (2.0 * (1.0 + nu));
+    const double *Sdt = st + 7;
+    double qt = st[13], dg = st[14];
+    double nrm = sqrt(Sdt[0] * Sdt[0] + Sdt[1] * Sdt[1] + Sdt[2] * Sdt[2] + 2.0 * Sdt[3] * Sdt[3] +
+                      2.0 * Sdt[4] * Sdt[4] + 2.0 * Sdt[5] * Sdt[5]);
+    double N[6];
+    for (int i = 0; i < 6; i++) N[i] = Sdt[i] / nrm;
+    double a = 2.0 * G * (1.0 - dg * 3.0 * G / qt);
+    double b = 6.0 * G * G * (dg / qt - 1.0 / (3.0 * G + H));
+    for (int i = 0; i < 6; i++)
+        for (int j = 0; j < 6; j++) {
+            double Is = (i == j) ? (i < 3 ? 1.0 : 0.5) : 0.0;
+            double I1 = (i < 3 && j < 3) ? 1.0 : 0.0;
+            double Id = Is - I1 / 3.0;
+            D[i][j] = a * Id + b * (N[i] * N[j]) + K * I1;
+        }
+}
+
+int or_model_nstate(int model) {
+    switch (model) {
+    case OR_MODEL_J2: return J2_NSTATE;
+    default: return 0;
+    }
+}
+
+static int mat_nprops(int model) {
+    switch (model) {
+    case OR_MODEL_GHL: return 2;
+    case OR_MODEL_J2: return 4;
+    case OR_MODEL_NEOHOOKEAN: return 2;
+    case OR_MODEL_SVK: return 2;
+    case OR_MODEL_COMP_NEOHOOKEAN: return 2;
+    default: return -1;
+    }
+}
+
+/* stress update (UpdateStressAndStateVariables) */
+static int mat_update(int model, const double *p, const double F[3][3], double time,
+                      const double *old, double s[6], double *nw) {
+    double Dtmp[6][6];
+    (void)time;
+    switch (model) {
+    case OR_MODEL_GHL: mat_ghl(p, F, s, Dtmp); return OR_OK;
+    case OR_MODEL_COMP_NEOHOOKEAN: mat_cnh(p, F, s, Dtmp); return OR_OK;
+    case OR_MODEL_NEOHOOKEAN: mat_nh(p, F, s, Dtmp); return OR_OK;
+    case OR_MODEL_SVK: mat_svk(p, F, s, Dtmp); return OR_OK;
+    case OR_MODEL_J2: mat_j2(p, F, old, s, nw); return OR_OK;
+    default: return OR_ERR_ARG;
+    }
+}
+
+/* tangent (GetTangentModulus), uses F and the state left by the last update */
+static void mat_tangent(int model, const double *p, const double F[3][3], const double *nw,
+                        double D[6][6]) {
+    double s[6];
+    switch (model) {
+    case OR_MODEL_GHL: mat_ghl(p, F, s, D); return;
+    case OR_MODEL_COMP_NEOHOOKEAN: mat_cnh(p, F, s, D); return;
+    case OR_MODEL_NEOHOOKEAN: mat_nh(p, F, s, D); return;
+    case OR_MODEL_SVK: mat_svk(p, F, s, D); return;
+    case OR_MODEL_J2: mat_j2_tangent(p, nw, D); return;
+    default: memset(D, 0, sizeof(double) * 36); return;
+    }
+}
+
+int or_material_point(int model, const double *props, const double Fv[9], double time,
+                      const double *state_old, double *sigma6, double *D36, double *state_new) {
+    double F[3][3];
+    double zero[MAXSTATE] = {0};
+    double nw[MAXSTATE] = {0};
+    memcpy(F, Fv, sizeof(F));
+    if (!state_old) state_old = zero;
+    int st = mat_update(model, props, F, time, state_old, sigma6, nw);
+    if (st) return st;
+    if (D36) mat_tangent(model, props, F, nw, (double(*)[6])D36);
+    if (state_new) memcpy(state_new, nw, sizeof(double) * or_model_nstate(model));
+    return OR_OK;
+}
+
+/* ------------------------------------------------------------------------------------------
+ * mesh
+ * ---------------------------------------------------------------------------------------- */
+typedef struct {
+    int model;
+    double p[16];
+    int set;
+} or_matdef;
+
+struct or_mesh {
+    int nn, ne, nl;
+    int nthreads;
+    double *X, *x;          /* [nn][3] reference / current */
+    int *conn;              /* [ne][8] 0-based */
+    int *mat;               /* [ne] material id */
+    or_matdef mats[MAXMAT]; /* indexed by material id */
+    double *F;              /* [ne][8][9] */
+    double *sig;            /* [ne][8][6] */
+    double *st_old, *st_new;/* [ne][8][MAXSTATE] */
+    double *vol, *volx;     /* [ne] */
+    double time;
+    int *failed;            /* [ne] 0/1 */
+    /* CSR upper (1-based) */
+    long nnz;
+    int *rowmap, *col;
+    /* element colouring for the threaded mode */
+    int ncolor;
+    int *color_ptr, *color_elems;
+};
+
+or_mesh *or_mesh_create(int nn, const double *X, int ne, const int *conn1, const int *mat_id) {
+    if (nn <= 0 || ne <= 0 || !X || !conn1) return NULL;
+    or_mesh *m = (or_mesh *)calloc(1, sizeof(or_mesh));
+    m->nn = nn;
+    m->ne = ne;
+    m->nl = 1;
+    m->nthreads = 1;
+    m->X = (double *)malloc(sizeof(double) * 3 * nn);
+    m->x = (double *)malloc(sizeof(double) * 3 * nn);
+    memcpy(m->X, X, sizeof(double) * 3 * nn);
+    memcpy(m->x, X, sizeof(double) * 3 * nn);
+    m->conn = (int *)malloc(sizeof(int) * 8 * ne);
+    m->mat = (int *)malloc(sizeof(int) * ne);
+    for (int e = 0; e < ne; e++) {
+        for (int a = 0; a < 8; a++) {
+            int n = conn1[8 * e + a] - 1;
+            if (n < 0 || n >= nn) {
+                or_mesh_destroy(m);
+                return NULL;
+            }
+            m->conn[8 * e + a] = n;
+        }
+        m->mat[e] = mat_id ? mat_id[e] : 1;
+        if (m->mat[e] < 0 || m->mat[e] >= MAXMAT) {
+            or_mesh_destroy(m);
+            return NULL;
+        }
+    }
+    m->F = (double *)calloc((size_t)ne * 8 * 9, sizeof(double));
+    for (long i = 0; i < (long)ne * 8; i++) m->F[9 * i] = m->F[9 * i + 4] = m->F[9 * i + 8] = 1.0;
+    m->sig = (double *)calloc((size_t)ne * 8 * 6, sizeof(double));
+    m->st_old = (double *)calloc((size_t)ne * 8 * MAXSTATE, sizeof(double));
+    m->st_new = (double *)calloc((size_t)ne * 8 * MAXSTATE, sizeof(double));
+    m->vol = (double *)calloc(ne, sizeof(double));
+    m->volx = (double *)calloc(ne, sizeof(double));
+    m->failed = (int *)calloc(ne, sizeof(int));
+    m->nnz = -1;
+    return m;
+}
+
+void or_mesh_destroy(or_mesh *m) {
+    if (!m) return;
+    free(m->X); free(m->x); free(m->conn); free(m->mat); free(m->F); free(m->sig);
+    free(m->st_old); free(m->st_new); free(m->vol); free(m->volx); free(m->failed);
+    free(m->rowmap); free(m->col); free(m->color_ptr); free(m->color_elems);
+    free(m);
+}
+
+int or_material_set(or_mesh *m, int mat_id, int model, const double *props, int nprops) {
+    if (!m || mat_id < 0 || mat_id >= MAXMAT) return OR_ERR_ARG;
+    int np = mat_nprops(model);
+    if (np < 0 || nprops < np || nprops > 16) return OR_ERR_ARG;
+    m->mats[mat_id].model = model;
+    memset(m->mats[mat_id].p, 0, sizeof(m->mats[mat_id].p));
+    memcpy(m->mats[mat_id].p, props, sizeof(double) * nprops);
+    m->mats[mat_id].set = 1;
+    return OR_OK;
+}
+
+void or_set_analysis(or_mesh *m, int nl_analysis) { m->nl = nl_analysis ? 1 : 0; }
+int or_ndof(const or_mesh *m) { return 3 * m->nn; }
+
+static void elem_coords(const or_mesh *m, int e, const double *src, double c[8][3]) {
+    for (int a = 0; a < 8; a++)
+        for (int k = 0; k < 3; k++) c[a][k] = src[3 * m->conn[8 * e + a] + k];
+}
+
+/* ElementVolume, FEM/Elements/ElementMethods/ModElement.f90:1313-1427 */
+static int elem_volume(const or_mesh *m, int e, double *V, double *VX) {
+    double nat[8][3], w[8], x[8][3], X[8][3];
+    or_hex8_gauss(nat, w);
+    elem_coords(m, e, m->x, x);
+    elem_coords(m, e, m->X, X);
+    *V = 0.0;
+    *VX = 0.0;
+    for (int g = 0; g < 8; g++) {
+        double dN[8][3], Jc[3][3], Jx[3][3];
+        or_hex8_dshape(nat[g], dN);
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 3; j++) {
+                double s = 0.0, sx = 0.0;
+                for (int n = 0; n < 8; n++) {
+                    s += dN[n][i] * x[n][j];
+                    sx += dN[n][i] * X[n][j];
+                }
+                Jc[i][j] = s;
+                Jx[i][j] = sx;
+            }
+        double dj = det3(Jc), djx = det3(Jx);
+        if (dj <= 1.0e-13) return OR_ERR_DETJ;
+        *VX += w[g] * djx;
+        *V += w[g] * dj;
+    }
+    return OR_OK;
+}
+
+void or_element_volume(or_mesh *m, int e, double *V, double *VX) { elem_volume(m, e, V, VX); }
+
+/* one element of SolveConstitutiveModel, FEM/Subroutines/SubSolveConstitutiveModel.f90:56-136 */
+static int solve_elem(or_mesh *m, int e, double time, const double *U) {
+    double nat[8][3], w[8], X[8][3], Ue[24], V, VX;
+    int st = OR_OK;
+    or_hex8_gauss(nat, w);
+    int sv = elem_volume(m, e, &V, &VX);
+    m->failed[e] = (sv != OR_OK);
+    m->vol[e] = V;
+    m->volx[e] = VX;
+    elem_coords(m, e, m->X, X);
+    for (int a = 0; a < 8; a++)
+        for (int k = 0; k < 3; k++) Ue[3 * a + k] = U[3 * m->conn[8 * e + a] + k];
+    const or_matdef *md = &m->mats[m->mat[e]];
+    for (int g = 0; g < 8; g++) {
+        double F[3][3];
+        long gi = (long)e * 8 + g;
+        int sf = or_deformation_gradient(X, nat[g], Ue, F);
+        if (sf) st = sf;
+        memcpy(&m->F[9 * gi], F, sizeof(F));
+        int su = mat_update(md->model, md->p, F, time, &m->st_old[MAXSTATE * gi], &m->sig[6 * gi],
+                            &m->st_new[MAXSTATE * gi]);
+        if (su) st = su;
+    }
+    if (sv) st = sv;
+    return st;
+}
+
+static void build_colors(or_mesh *m);
+
+int or_solve_constitutive(or_mesh *m, double time, const double *U) {
+    for (int i = 0; i < MAXMAT; i++) (void)i;
+    for (int e = 0; e < m->ne; e++)
+        if (!m->mats[m->mat[e]].set) return OR_ERR_ARG;
+    m->time = time;
+    int st = OR_OK;
+    if (m->nthreads <= 1) {
+        for (int e = 0; e < m->ne; e++) {
+            int s = solve_elem(m, e, time, U);
+            if (s && (st == OR_OK || s == OR_ERR_DETJ)) st = s;
+        }
+    } else {
+        int err = 0, errdet = 0;
+#pragma omp parallel for schedule(static) num_threads(m->nthreads) reduction(| : err, errdet)
+        for (int e = 0; e < m->ne; e++) {
+            int s = solve_elem(m, e, time, U);
+            if (s) err |= 1;
+            if (s == OR_ERR_DETJ) errdet |= 1;
+        }
+        if (errdet) st = OR_ERR_DETJ;
+        else if (err) st = OR_ERR_LOCAL_NR;
+    }
+    return st;
+}
+
+int or_failed_elements(const or_mesh *m, int *ids, int cap) {
+    int n = 0;
+    for (int e = 0; e < m->ne; e++)
+        if (m->failed[e]) {
+            if (ids && n < cap) ids[n] = e + 1;
+            n++;
+        }
+    return n;
+}
+
+/* ElementInternalForce, FEM/Elements/ElementMethods/ModElement.f90:484-562 */
+static int elem_fe(or_mesh *m, int e, double Fe[24]) {
+    double nat[8][3], w[8], x[8][3], B[6][24], G[9][24], detJ;
+    or_hex8_gauss(nat, w);
+    elem_coords(m, e, m->x, x);
+    memset(Fe, 0, sizeof(double) * 24);
+    for (int g = 0; g < 8; g++) {
+        if (or_matrix_bg(x, nat[g], B, G, &detJ)) return OR_ERR_DETJ;
+        const double *s = &m->sig[6 * ((long)e * 8 + g)];
+        double a = w[g] * detJ;
+        for (int j = 0; j < 24; j++) {
+            double acc = 0.0;
+            for (int i = 0; i < 6; i++) acc += B[i][j] * s[i];
+            Fe[j] += a * acc;
+        }
+    }
+    return OR_OK;
+}
+
+int or_element_internal_force(or_mesh *m, int e, double *Fe) { return elem_fe(m, e, Fe); }
+
+/* InternalForce, FEM/Subroutines/SubInternalForce.f90:49-97 */
+int or_internal_force(or_mesh *m, double *Fint) {
+    int ndof = 3 * m->nn, st = OR_OK;
+    memset(Fint, 0, sizeof(double) * ndof);
+    if (m->nthreads <= 1) {
+        for (int e = 0; e < m->ne; e++) {
+            double Fe[24];
+            if (elem_fe(m, e, Fe)) {
+                m->failed[e] = 1;
+                st = OR_ERR_DETJ;
+            }
+            for (int a = 0; a < 8; a++)
+                for (int k = 0; k < 3; k++) Fint[3 * m->conn[8 * e + a] + k] += Fe[3 * a + k];
+        }
+    } else {
+        build_colors(m);
+        int err = 0;
+        for (int c = 0; c < m->ncolor; c++) {
+#pragma omp parallel for schedule(static) num_threads(m->nthreads) reduction(| : err)
+            for (int i = m->color_ptr[c]; i < m->color_ptr[c + 1]; i++) {
+                int e = m->color_elems[i];
+                double Fe[24];
+                if (elem_fe(m, e, Fe)) {
+                    m->failed[e] = 1;
+                    err |= 1;
+                }
+                for (int a = 0; a < 8; a++)
+                    for (int k = 0; k < 3; k++) Fint[3 * m->conn[8 * e + a] + k] += Fe[3 * a + k];
+            }
+        }
+        if (err) st = OR_ERR_DETJ;
+    }
+    return st;
+}
+
+/* ElementStiffnessMatrix, FEM/Elements/ElementMethods/ModElement.f90:200-348.
+ * nl = 0: Ke += (B^T D B) w detJ with the full D (:293-308);
+ * nl = 1: DB = D_sym B, SG = S_sym G (dsymm 'L','U'), Ke += w detJ (B^T DB + G^T SG) (:310-348) */
+static int elem_ke(or_mesh *m, int e, double Ke[24][24]) {
+    double nat[8][3], w[8], x[8][3], B[6][24], G[9][24], detJ;
+    or_hex8_gauss(nat, w);
+    elem_coords(m, e, m->x, x);
+    memset(Ke, 0, sizeof(double) * 576);
+    const or_matdef *md = &m->mats[m->mat[e]];
+    for (int g = 0; g < 8; g++) {
+        long gi = (long)e * 8 + g;
+        double D[6][6], F[3][3];
+        memcpy(F, &m->F[9 * gi], sizeof(F));
+        mat_tangent(md->model, md->p, F, &m->st_new[MAXSTATE * gi], D);
+        if (or_matrix_bg(x, nat[g], B, G, &detJ)) return OR_ERR_DETJ;
+        double a = w[g] * detJ;
+        if (!m->nl) {
+            double BtD[24][6];
+            for (int i = 0; i < 24; i++)
+                for (int j = 0; j < 6; j++) {
+                    double s = 0.0;
+                    for (int k = 0; k < 6; k++) s += B[k][i] * D[k][j];
+                    BtD[i][j] = s;
+                }
+            for (int i = 0; i < 24; i++)
+                for (int j = 0; j < 24; j++) {
+                    double s = 0.0;
+                    for (int k = 0; k < 6; k++) s += BtD[i][k] * B[k][j];
+                    Ke[i][j] += s * a;
+                }
+            continue;
+        }
+        const double *sg = &m->sig[6 * gi];
+        double Ds[6][6], S[9][9], DB[6][24], SG[9][24];
+        for (int i = 0; i < 6; i++)
+            for (int j = 0; j < 6; j++) Ds[i][j] = (i <= j) ? D[i][j] : D[j][i];
+        /* GetMatrixOfStressesBase, ConstitutiveLibrary/ModConstitutiveModel.f90:292-310 */
+        memset(S, 0, sizeof(S));
+        for (int b = 0; b < 3; b++) {
+            int o = 3 * b;
+            S[o + 0][o + 0] = sg[0];
+            S[o + 1][o + 1] = sg[1];
+            S[o + 2][o + 2] = sg[2];
+            S[o + 0][o + 1] = S[o + 1][o + 0] = sg[3];
+            S[o + 1][o + 2] = S[o + 2][o + 1] = sg[4];
+            S[o + 0][o + 2] = S[o + 2][o + 0] = sg[5];
+        }
+        for (int i = 0; i < 6; i++)
+            for (int j = 0; j < 24; j++) {
+                double s = 0.0;
+                for (int k = 0; k < 6; k++) s += Ds[i][k] * B[k][j];
+                DB[i][j] = s;
+            }
+        for (int i = 0; i < 9; i++)
+            for (int j = 0; j < 24; j++) {
+                double s = 0.0;
+                for (int k = 0; k < 9; k++) s += S[i][k] * G[k][j];
+                SG[i][j] = s;
+            }
+        for (int i = 0; i < 24; i++)
+            for (int j = 0; j < 24; j++) {
+                double s = 0.0;
+                for (int k = 0; k < 6; k++) s += B[k][i] * DB[k][j];
+                Ke[i][j] += a * s;
+            }
+        for (int i = 0; i < 24; i++)
+            for (int j = 0; j < 24; j++) {
+                double s = 0.0;
+                for (int k = 0; k < 9; k++) s += G[k][i] * SG[k][j];
+                Ke[i][j] += a * s;
+            }
+    }
+    return OR_OK;
+}
+
+int or_element_stiffness(or_mesh *m, int e, double *Ke) { return elem_ke(m, e, (double(*)[24])Ke); }
+
+/* ------------------------------------------------------------------------------------------
+ * CSR pattern: each element's DOF clique is inserted into per-row sorted column lists
+ * (SparseMatrixSetArray + AddItems, ModSparseMatrixRoutines.f90:254-288 /
+ * ModSparseVectorRoutines.f90:222-312), then the upper triangle (col >= row) is kept with
+ * 1-based indices and RowMap(n+1) = nz+1 (ConvertToCoordinateFormatUpperTriangular :573-609).
+ * ---------------------------------------------------------------------------------------- */
+typedef struct {
+    int n, cap;
+    int *c;
+} row_list;
+
+static void row_insert(row_list *r, int c) {
+    int lo = 0, hi = r->n;
+    while (lo < hi) {
+        int mid = (lo + hi) / 2;
+        if (r->c[mid] < c) lo = mid + 1;
+        else hi = mid;
+    }
+    if (lo < r->n && r->c[lo] == c) return;
+    if (r->n == r->cap) {
+        r->cap = r->cap ? 2 * r->cap : 32;
+        r->c = (int *)realloc(r->c, sizeof(int) * r->cap);
+    }
+    memmove(r->c + lo + 1, r->c + lo, sizeof(int) * (r->n - lo));
+    r->c[lo] = c;
+    r->n++;
+}
+
+static int build_pattern(or_mesh *m) {
+    if (m->nnz >= 0) return OR_OK;
+    int ndof = 3 * m->nn;
+    row_list *rows = (row_list *)calloc(ndof, sizeof(row_list));
+    for (int e = 0; e < m->ne; e++) {
+        int gm[24];
+        for (int a = 0; a < 8; a++)
+            for (int k = 0; k < 3; k++) gm[3 * a + k] = 3 * m->conn[8 * e + a] + k + 1;
+        for (int i = 0; i < 24; i++)
+            for (int j = 0; j < 24; j++) row_insert(&rows[gm[i] - 1], gm[j]);
+    }
+    long nzfull = 0;
+    for (int r = 0; r < ndof; r++) nzfull += rows[r].n;
+    long nz = (nzfull - ndof) / 2 + ndof;
+    m->rowmap = (int *)malloc(sizeof(int) * (ndof + 1));
+    m->col = (int *)malloc(sizeof(int) * (nz > 0 ? nz : 1));
+    long n = 0;
+    int bad = 0;
+    for (int r = 0; r < ndof; r++) {
+        m->rowmap[r] = (int)(n + 1);
+        if (rows[r].n == 0) bad = 1; /* orphan DOF: reference formula for nz would be wrong */
+        for (int i = 0; i < rows[r].n; i++)
+            if (rows[r].c[i] >= r + 1) {
+                if (n < nz) m->col[n] = rows[r].c[i];
+                n++;
+            }
+        free(rows[r].c);
+    }
+    free(rows);
+    m->rowmap[ndof] = (int)(nz + 1);
+    m->nnz = nz;
+    if (bad || n != nz) return OR_ERR_ARG;
+    return OR_OK;
+}
+
+long or_csr_nnz(or_mesh *m) {
+    if (build_pattern(m)) return -1;
+    return m->nnz;
+}
+
+int or_csr_pattern(or_mesh *m, int *rowmap, int *col) {
+    int st = build_pattern(m);
+    if (st) return st;
+    memcpy(rowmap, m->rowmap, sizeof(int) * (3 * m->nn + 1));
+    memcpy(col, m->col, sizeof(int) * m->nnz);
+    return OR_OK;
+}
+
+/* AssembleGlobalMatrixUpperTriangular, FEM/Subroutines/SubAssembleGlobalMatrixUpperTriangular.f90:13-69 */
+static int assemble_upper(const or_mesh *m, const int gm[24], double Ke[24][24], double *Kval) {
+    for (int i = 0; i < 24; i++) {
+        int iG = gm[i];
+        for (int j = 0; j < 24; j++) {
+            int jG = gm[j];
+            if (jG >= iG) {
+                int found = 0;
+                for (int k = m->rowmap[iG - 1]; k <= m->rowmap[iG] - 1; k++)
+                    if (m->col[k - 1] == jG) {
+                        found = 1;
+                        Kval[k - 1] += Ke[i][j];
+                    }
+                if (!found) return OR_ERR_SLOT;
+            }
+        }
+    }
+    return OR_OK;
+}
+
+/* greedy element colouring (elements of one colour share no node) for the threaded mode */
+static void build_colors(or_mesh *m) {
+    if (m->color_ptr) return;
+    int ne = m->ne, nn = m->nn;
+    int *col = (int *)malloc(sizeof(int) * ne);
+    uint64_t *used = (uint64_t *)calloc(nn, sizeof(uint64_t));
+    int nc = 0;
+    for (int e = 0; e < ne; e++) {
+        uint64_t u = 0;
+        for (int a = 0; a < 8; a++) u |= used[m->conn[8 * e + a]];
+        int c = 0;
+        while (c < 64 && (u >> c) & 1) c++;
+        col[e] = c;
+        if (c + 1 > nc) nc = c + 1;
+        for (int a = 0; a < 8; a++) used[m->conn[8 * e + a]] |= (1ull << c);
+    }
+    m->ncolor = nc;
+    m->color_ptr = (int *)calloc(nc + 1, sizeof(int));
+    m->color_elems = (int *)malloc(sizeof(int) * ne);
+    for (int e = 0; e < ne; e++) m->color_ptr[col[e] + 1]++;
+    for (int c = 0; c < nc; c++) m->color_ptr[c + 1] += m->color_ptr[c];
+    int *fill = (int *)malloc(sizeof(int) * nc);
+    for (int c = 0; c < nc; c++) fill[c] = m->color_ptr[c];
+    for (int e = 0; e < ne; e++) m->color_elems[fill[col[e]]++] = e;
+    free(fill);
+    free(used);
+    free(col);
+}
+
+void or_set_threads(or_mesh *m, int nthreads) { m->nthreads = nthreads < 1 ? 1 : nthreads; }
+
+/* TangentStiffnessMatrix FEM branch, FEM/Subroutines/SubTangentStiffnessMatrix.f90:63, 294-310 */
+int or_tangent(or_mesh *m, double *Kval) {
+    int st = build_pattern(m);
+    if (st) return st;
+    memset(Kval, 0, sizeof(double) * m->nnz);
+    if (m->nthreads <= 1) {
+        for (int e = 0; e < m->ne; e++) {
+            double Ke[24][24];
+            int gm[24];
+            elem_ke(m, e, Ke);
+            for (int a = 0; a < 8; a++)
+                for (int k = 0; k < 3; k++) gm[3 * a + k] = 3 * m->conn[8 * e + a] + k + 1;
+            int s = assemble_upper(m, gm, Ke, Kval);
+            if (s) return s;
+        }
+    } else {
+        build_colors(m);
+        int err = 0;
+        for (int c = 0; c < m->ncolor; c++) {
+#pragma omp parallel for schedule(static) num_threads(m->nthreads) reduction(| : err)
+            for (int i = m->color_ptr[c]; i < m->color_ptr[c + 1]; i++) {
+                int e = m->color_elems[i];
+                double Ke[24][24];
+                int gm[24];
+                elem_ke(m, e, Ke);
+                for (int a = 0; a < 8; a++)
+                    for (int k = 0; k < 3; k++) gm[3 * a + k] = 3 * m->conn[8 * e + a] + k + 1;
+                if (assemble_upper(m, gm, Ke, Kval)) err |= 1;
+            }
+        }
+        if (err) return OR_ERR_SLOT;
+    }
+    return OR_OK;
+}
+
+/* UpdateMeshCoordinates, FEM/Subroutines/SubUpdateMeshCoordinates.f90:15-19 */
+void or_update_mesh(or_mesh *m, const double *U) {
+    for (long i = 0; i < 3L * m->nn; i++) m->x[i] = m->X[i] + U[i];
+}
+
+/* SwitchConvergedState loop, FEM/ModFEMAnalysis.f90:1349-1353 (J2: ModJ2Plasticity.f90:625-633) */
+void or_switch_converged_state(or_mesh *m) {
+    for (int e = 0; e < m->ne; e++) {
+        int model = m->mats[m->mat[e]].model;
+        if (model == OR_MODEL_J2) {
+            for (int g = 0; g < 8; g++) {
+                long gi = (long)e * 8 + g;
+                memcpy(&m->st_old[MAXSTATE * gi], &m->st_new[MAXSTATE * gi], sizeof(double) * 7);
+            }
+        }
+    }
+}
+
+void or_get_gp_stress(const or_mesh *m, double *out) {
+    memcpy(out, m->sig, sizeof(double) * 6 * 8 * (size_t)m->ne);
+}
+
+void or_get_gp_F(const or_mesh *m, double *out) {
+    memcpy(out, m->F, sizeof(double) * 9 * 8 * (size_t)m->ne);
+}
+
+void or_get_state(const or_mesh *m, int which, double *out) {
+    const double *src = which ? m->st_new : m->st_old;
+    for (long gi = 0; gi < (long)m->ne * 8; gi++)
+        memcpy(&out[J2_NSTATE * gi], &src[MAXSTATE * gi], sizeof(double) * J2_NSTATE);
+}
+
+void or_set_state(or_mesh *m, int which, const double *in) {
+    double *dst = which ? m->st_new : m->st_old;
+    for (long gi = 0; gi < (long)m->ne * 8; gi++)
+        memcpy(&dst[MAXSTATE * gi], &in[J2_NSTATE * gi], sizeof(double) * J2_NSTATE);
+}
+
+/* HomogenizeStress / HomogenizeDeformationGradient, MultiScale/ModMultiscaleAnalysis.f90:140-441.
+ * P = J sigma F^-T (Tools/ModContinuumMechanics.f90:125-129); 9-vectors column-major
+ * (Tools/MathTools/ModVoigtNotation.f90:70-86). Uses the element VolumeX stored by the last
+ * or_solve_constitutive. */
+int or_homogenize(or_mesh *m, double P[9], double Fh[9]) {
+    double nat[8][3], w[8];
+    or_hex8_gauss(nat, w);
+    double tv = 0.0;
+    for (int e = 0; e < m->ne; e++) tv += m->volx[e];
+    memset(P, 0, sizeof(double) * 9);
+    memset(Fh, 0, sizeof(double) * 9);
+    for (int e = 0; e < m->ne; e++) {
+        double X[8][3];
+        elem_coords(m, e, m->X, X);
+        for (int g = 0; g < 8; g++) {
+            double dN[8][3], Jx[3][3];
+            or_hex8_dshape(nat[g], dN);
+            for (int i = 0; i < 3; i++)
+                for (int j = 0; j < 3; j++) {
+                    double s = 0.0;
+                    for (int n = 0; n < 8; n++) s += dN[n][i] * X[n][j];
+                    Jx[i][j] = s;
+                }
+            double djx = det3(Jx);
+            long gi = (long)e * 8 + g;
+            double F[3][3], Fi[3][3], FiT[3][3], sg[3][3], Pt[3][3];
+            memcpy(F, &m->F[9 * gi], sizeof(F));
+            const double *s6 = &m->sig[6 * gi];
+            sg[0][0] = s6[0]; sg[1][1] = s6[1]; sg[2][2] = s6[2];
+            sg[0][1] = sg[1][0] = s6[3];
+            sg[1][2] = sg[2][1] = s6[4];
+            sg[0][2] = sg[2][0] = s6[5];
+            double J = det3(F);
+            double Ft[3][3];
+            transpose3(F, Ft);
+            inv3(Ft, FiT);
+            (void)Fi;
+            matmul3(sg, FiT, Pt);
+            for (int i = 0; i < 3; i++)
+                for (int j = 0; j < 3; j++) Pt[i][j] *= J;
+            double a = w[g] * djx;
+            for (int j = 0; j < 3; j++)
+                for (int i = 0; i < 3; i++) {
+                    P[3 * j + i] += (Pt[i][j] * a) / tv;
+                    Fh[3 * j + i] += (F[i][j] * a) / tv;
+                }
+        }
+    }
+    return OR_OK;
+}
