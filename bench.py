@@ -1,0 +1,186 @@
+"""Throughput of the hex8 F+K element path on one MI355X.
+
+Metric: elem/s assembled (F+K, hex8) -- one step = one full evaluation of the hot path over the
+80x80x80 perturbed hex8 cube (512,000 elements, 1,594,323 DOFs): constitutive update at all
+4,096,000 Gauss points, internal force, tangent stiffness, assembled into the 63,786,006-entry
+upper CSR (FP64), i.e. SolveConstitutiveModel + InternalForce + TangentStiffnessMatrix of the
+reference.  Inputs (U, Fint, K) are resident in HBM; the evaluation is the device-resident
+C-ABI entry ceos_evaluate_device.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+
+Prints one JSON line.  cpu_baseline: the CPU oracle (a plain-C restatement of the reference
+loops, OpenMP over element colours) timed on the host cores on a bounded sample of the same
+workload: the 32^3 corner sub-cube of the same perturbed mesh and displacement field.
+"""
+import argparse
+import json
+import os
+import re
+import sys
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(REPO, "ceos-eems_amd"))
+
+import numpy as np  # noqa: E402
+
+N_CELLS = 80
+MU, K_BULK = 1.0, 100.0
+LAM = K_BULK - 2.0 * MU / 3.0
+# algorithmic work per element and evaluation (structure-exploiting count, FP64 tangent):
+FLOP_PER_ELEM = 37.5e3
+BYTES_PER_ELEM = 1103.0
+PEAK_FP64_TFLOPS = 78.6
+PEAK_HBM_GBS = 8000.0
+PROFILE_PMC = os.path.join(REPO, "profiles", "element_kernel_c2_pmc.txt")
+
+
+def displacement(X):
+    """Step-1 state of the stretch test: u_x = 0.04 X (stretch 1.2 over 5 steps) plus a seeded
+    smooth transverse field, so that F is non-trivial at every Gauss point."""
+    U = np.zeros_like(X)
+    U[:, 0] = 0.04 * X[:, 0]
+    U[:, 1] = -0.006 * X[:, 1] + 0.002 * np.sin(3.1 * X[:, 0]) * X[:, 2]
+    U[:, 2] = -0.006 * X[:, 2] + 0.002 * np.cos(2.3 * X[:, 1]) * X[:, 0]
+    return U.reshape(-1)
+
+
+def read_traffic():
+    """HBM bytes per launch of the element kernel from the committed rocprofv3 --pmc summary."""
+    if not os.path.exists(PROFILE_PMC):
+        return None
+    m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", open(PROFILE_PMC).read())
+    return float(m.group(1)) if m else None
+
+
+def cpu_baseline(X, conn, U, n_sub=32, budget_s=12.0):
+    sys.path.insert(0, os.path.join(REPO, "tests"))
+    import oracle_lib as ol
+    m1 = N_CELLS + 1
+    # elements and nodes of the sub-cube i,j,k < n_sub (x fastest numbering)
+    ijk = np.stack(np.meshgrid(np.arange(n_sub), np.arange(n_sub), np.arange(n_sub), indexing="ij"), -1)
+    eid = (ijk[..., 2] + N_CELLS * (ijk[..., 1] + N_CELLS * ijk[..., 0])).ravel()
+    sub_conn = conn[eid]
+    nodes = np.unique(sub_conn)
+    remap = np.zeros(m1 ** 3 + 1, dtype=np.int32)
+    remap[nodes] = np.arange(1, len(nodes) + 1)
+    Xs = X[nodes - 1]
+    Us = U.reshape(-1, 3)[nodes - 1].reshape(-1)
+    threads = min(os.cpu_count() or 1, 16)
+    o = ol.OracleMesh(Xs, remap[sub_conn], threads=threads)
+    o.set_material(1, ol.MODEL_CNH, [MU, LAM])
+    o.update_mesh(Us)
+    o.csr()
+    n, t0 = 0, time.perf_counter()
+    while True:
+        o.solve_constitutive(0.2, Us)
+        o.internal_force()
+        o.tangent()
+        n += 1
+        if time.perf_counter() - t0 > budget_s:
+            break
+    dt = time.perf_counter() - t0
+    return {"value": n * len(sub_conn) / dt, "unit": "elem/s", "cores": threads, "kind": "port",
+            "sample": "%d evaluations (F+K, FP64) of the %d^3 corner sub-cube (%d hex8) of the same "
+                      "perturbed mesh and displacement field, CPU oracle with %d OpenMP threads"
+                      % (n, n_sub, len(sub_conn), threads)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--precision", choices=["fp64", "fp32k"], default="fp64")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: this build shards nothing across GPUs "
+              "(the element path is per-GPU; see the sharding design). Use --gpus 1.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    from ceos_amd import gpu, hiprt, meshgen
+
+    t_setup = time.perf_counter()
+    X, conn = meshgen.cube(N_CELLS, perturb=0.1)
+    U = displacement(X)
+    prec = gpu.K_FP64 if args.precision == "fp64" else gpu.K_FP32_STORED
+    mesh = gpu.CeosMesh(X, conn, precision=prec)
+    mesh.set_material(1, gpu.MODEL_CNH, [MU, LAM])
+    ncolor, nnz = mesh.info()
+    ne = len(conn)
+    dU = hiprt.DeviceArray.from_numpy(U)
+    dF = hiprt.DeviceArray(len(U))
+    dK = hiprt.DeviceArray(nnz, np.float64 if prec == gpu.K_FP64 else np.float32)
+    stream = hiprt.Stream()
+    sp = stream.ptr
+    mesh.update_mesh_device(dU.ptr, sp)
+    setup_s = time.perf_counter() - t_setup
+
+    def step():
+        mesh.evaluate_device(0.2, dU.ptr, dF.ptr, dK.ptr, sp)
+
+    for _ in range(args.warmup):
+        step()
+    stream.synchronize()
+    code, st = mesh.device_status()
+    if code != 0:
+        raise RuntimeError("evaluation status %d" % code)
+    e0, e1 = hiprt.Event(), hiprt.Event()
+    hiprt.synchronize()
+    t0 = time.perf_counter()
+    e0.record(stream)
+    for _ in range(args.steps):
+        step()
+    e1.record(stream)
+    e1.synchronize()
+    hiprt.synchronize()
+    wall = time.perf_counter() - t0
+    gpu_ms = e0.elapsed_ms(e1)
+    ms_per_step = gpu_ms / args.steps
+    value = ne * args.steps / (gpu_ms * 1e-3)
+
+    # dominant kernel: the fused element kernel, one launch per colour (ncolor launches / step)
+    launch_s = (gpu_ms * 1e-3) / (args.steps * ncolor)
+    elems_per_launch = ne / ncolor
+    flops_launch = FLOP_PER_ELEM * elems_per_launch
+    achieved_tf = flops_launch / launch_s / 1e12
+    bytes_launch = BYTES_PER_ELEM * elems_per_launch
+    achieved_gbs = bytes_launch / launch_s / 1e9
+    roofline = {"bound": "mfma", "achieved": round(achieved_tf, 3), "peak": PEAK_FP64_TFLOPS,
+                "unit": "TFLOP/s", "frac": round(achieved_tf / PEAK_FP64_TFLOPS, 4),
+                "traffic": read_traffic()}
+    out = {
+        "metric": "elem/s assembled (F+K, hex8)",
+        "value": value,
+        "unit": "elem/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if prec == gpu.K_FP64 else "f64 (K stored f32)",
+        "data": "synthetic",
+        "config": {"workload": "80^3 hex8 cube, interior nodes perturbed +-10% h, compressible "
+                               "Neo-Hookean mu=1 K=100, F+K evaluation at the step-1 state",
+                   "elements": ne, "dofs": 3 * len(X), "gauss_points": 8 * ne, "nnz_upper": nnz,
+                   "colors": ncolor, "tangent": args.precision, "parallelism": "single GPU"},
+        "roofline": roofline,
+        "roofline_hbm": {"achieved": round(achieved_gbs, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                         "frac": round(achieved_gbs / PEAK_HBM_GBS, 4),
+                         "algorithmic_bytes_per_elem": BYTES_PER_ELEM},
+        "wall_s_timed": wall,
+        "setup_s": setup_s,
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(X, conn, U)
+        out["speedup_vs_cpu_baseline"] = value / out["cpu_baseline"]["value"]
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,232 @@
+"""Python mirror of the reference element-layer interface, bound to libceos_gpu.so (C ABI).
+
+Method names follow the reference subroutines they replace (paths relative to SourceCode/):
+  solve_constitutive      SolveConstitutiveModel   FEM/Subroutines/SubSolveConstitutiveModel.f90:13
+  internal_force          InternalForce            FEM/Subroutines/SubInternalForce.f90:13
+  tangent                 TangentStiffnessMatrix   FEM/Subroutines/SubTangentStiffnessMatrix.f90:13
+  update_mesh             UpdateMeshCoordinates    FEM/Subroutines/SubUpdateMeshCoordinates.f90:1
+  switch_converged_state  SwitchConvergedState     FEM/ModFEMAnalysis.f90:1349-1353
+  homogenize              HomogenizeStress / HomogenizeDeformationGradient
+                                                   MultiScale/ModMultiscaleAnalysis.f90:140-441
+  csr                     AllocateKgSparseUpperTriangular  FEM/ModFEMAnalysis.f90:209-273
+Status codes mirror ClassStatus (Tools/ModStatus.f90): 0 ok, -1 detJ <= 1e-13, ...
+
+There is no CPU fallback: constructing a mesh without the built library or without a GPU raises.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(PKG, "lib", "libceos_gpu.so")
+
+OK, ERR_DETJ, ERR_LOCAL_NR, ERR_SLOT, ERR_ARG, ERR_HIP = 0, -1, -2, -3, -4, -5
+MODEL_GHL, MODEL_J2, MODEL_NH, MODEL_SVK, MODEL_CNH = 1, 2, 3, 5, 7
+K_FP64, K_FP32_STORED = 0, 1
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_fp = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+_ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+
+
+class CeosStatus(C.Structure):
+    _fields_ = [("error", C.c_int), ("error_id", C.c_int), ("n_failed", C.c_int),
+                ("description", C.c_char * 256)]
+
+
+class CeosError(RuntimeError):
+    def __init__(self, code, what):
+        super().__init__("%s failed with status %d" % (what, code))
+        self.code = code
+
+
+EXPORTS = [
+    "ceos_mesh_create", "ceos_mesh_destroy", "ceos_material_set", "ceos_set_analysis",
+    "ceos_set_precision", "ceos_csr_pattern", "ceos_solve_constitutive", "ceos_internal_force",
+    "ceos_tangent", "ceos_tangent_f32", "ceos_update_mesh", "ceos_switch_converged_state",
+    "ceos_failed_elements", "ceos_get_gp_stress", "ceos_get_state", "ceos_set_state",
+    "ceos_homogenize", "ceos_state_save", "ceos_state_load", "ceos_set_fused",
+    "ceos_evaluate_device", "ceos_update_mesh_device", "ceos_device_status", "ceos_mesh_info",
+]
+
+_lib = None
+
+
+def load_library(path=LIB_PATH):
+    """Load libceos_gpu.so and declare the C ABI. Raises if it is not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError("libceos_gpu.so is not built (%s); run __graft_entry__.build()" % path)
+    L = C.CDLL(path)
+    vp, i, d = C.c_void_p, C.c_int, C.c_double
+    L.ceos_mesh_create.argtypes = [i, _dp, i, i, _ip, _ip, C.POINTER(vp)]
+    L.ceos_mesh_destroy.argtypes = [vp]
+    L.ceos_mesh_destroy.restype = None
+    L.ceos_material_set.argtypes = [vp, i, i, _dp, i]
+    L.ceos_set_analysis.argtypes = [vp, i, i]
+    L.ceos_set_precision.argtypes = [vp, i]
+    L.ceos_set_fused.argtypes = [vp, i]
+    L.ceos_csr_pattern.argtypes = [vp, i, C.POINTER(i), C.POINTER(C.c_longlong), vp, vp]
+    L.ceos_solve_constitutive.argtypes = [vp, d, _dp, C.POINTER(CeosStatus)]
+    L.ceos_internal_force.argtypes = [vp, _dp, C.POINTER(CeosStatus)]
+    L.ceos_tangent.argtypes = [vp, _dp]
+    L.ceos_tangent_f32.argtypes = [vp, _fp]
+    L.ceos_update_mesh.argtypes = [vp, _dp]
+    L.ceos_switch_converged_state.argtypes = [vp]
+    L.ceos_failed_elements.argtypes = [vp, vp, i]
+    L.ceos_get_gp_stress.argtypes = [vp, _dp]
+    L.ceos_get_state.argtypes = [vp, i, _dp]
+    L.ceos_set_state.argtypes = [vp, i, _dp]
+    L.ceos_homogenize.argtypes = [vp, _dp, _dp]
+    L.ceos_state_save.argtypes = [vp, C.c_char_p]
+    L.ceos_state_load.argtypes = [vp, C.c_char_p]
+    L.ceos_evaluate_device.argtypes = [vp, d, vp, vp, vp, vp]
+    L.ceos_update_mesh_device.argtypes = [vp, vp, vp]
+    L.ceos_device_status.argtypes = [vp, C.POINTER(CeosStatus)]
+    L.ceos_mesh_info.argtypes = [vp, C.POINTER(i), C.POINTER(C.c_longlong)]
+    _lib = L
+    return L
+
+
+def _check(code, what):
+    if code != OK:
+        raise CeosError(code, what)
+
+
+class CeosMesh:
+    """One hex8 mesh on the GPU. X (nn,3), conn (ne,8) 1-based node ids, mat_id (ne,)."""
+
+    def __init__(self, X, conn1, mat_id=None, nl_analysis=True, precision=K_FP64, fused=True):
+        L = load_library()
+        X = np.ascontiguousarray(X, dtype=np.float64).reshape(-1, 3)
+        conn1 = np.ascontiguousarray(conn1, dtype=np.int32).reshape(-1, 8)
+        self.nn, self.ne = X.shape[0], conn1.shape[0]
+        self.ndof = 3 * self.nn
+        if mat_id is None:
+            mat_id = np.ones(self.ne, dtype=np.int32)
+        mat_id = np.ascontiguousarray(mat_id, dtype=np.int32)
+        h = C.c_void_p()
+        _check(L.ceos_mesh_create(self.nn, X.reshape(-1), self.ne, 8, conn1.reshape(-1), mat_id,
+                                  C.byref(h)), "ceos_mesh_create")
+        self._h = h
+        _check(L.ceos_set_analysis(self._h, 1 if nl_analysis else 0, 1), "ceos_set_analysis")
+        _check(L.ceos_set_precision(self._h, precision), "ceos_set_precision")
+        _check(L.ceos_set_fused(self._h, 1 if fused else 0), "ceos_set_fused")
+        self.precision = precision
+        self._csr = None
+        self.last_status = CeosStatus()
+
+    @property
+    def handle(self):
+        return self._h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            load_library().ceos_mesh_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_material(self, mat_id, model, props):
+        p = np.ascontiguousarray(props, dtype=np.float64)
+        _check(load_library().ceos_material_set(self._h, mat_id, model, p, len(p)), "ceos_material_set")
+
+    def info(self):
+        nc, nnz = C.c_int(), C.c_longlong()
+        _check(load_library().ceos_mesh_info(self._h, C.byref(nc), C.byref(nnz)), "ceos_mesh_info")
+        return nc.value, nnz.value
+
+    def csr(self, upper=True):
+        if upper and self._csr is not None:
+            return self._csr
+        L = load_library()
+        n, nnz = C.c_int(), C.c_longlong()
+        _check(L.ceos_csr_pattern(self._h, 1 if upper else 0, C.byref(n), C.byref(nnz), None, None),
+               "ceos_csr_pattern")
+        rm = np.zeros(n.value + 1, dtype=np.int32)
+        col = np.zeros(nnz.value, dtype=np.int32)
+        _check(L.ceos_csr_pattern(self._h, 1 if upper else 0, C.byref(n), C.byref(nnz),
+                                  rm.ctypes.data, col.ctypes.data), "ceos_csr_pattern")
+        if upper:
+            self._csr = (rm, col)
+        return rm, col
+
+    def solve_constitutive(self, time, U):
+        U = np.ascontiguousarray(U, dtype=np.float64)
+        return load_library().ceos_solve_constitutive(self._h, time, U, C.byref(self.last_status))
+
+    def internal_force(self):
+        F = np.zeros(self.ndof)
+        st = load_library().ceos_internal_force(self._h, F, C.byref(self.last_status))
+        return F, st
+
+    def tangent(self):
+        rm, col = self.csr()
+        L = load_library()
+        if self.precision == K_FP32_STORED:
+            K = np.zeros(len(col), dtype=np.float32)
+            _check(L.ceos_tangent_f32(self._h, K), "ceos_tangent_f32")
+            return K.astype(np.float64)
+        K = np.zeros(len(col))
+        _check(L.ceos_tangent(self._h, K), "ceos_tangent")
+        return K
+
+    def update_mesh(self, U):
+        _check(load_library().ceos_update_mesh(self._h, np.ascontiguousarray(U, dtype=np.float64)),
+               "ceos_update_mesh")
+
+    def switch_converged_state(self):
+        _check(load_library().ceos_switch_converged_state(self._h), "ceos_switch_converged_state")
+
+    def failed_elements(self):
+        L = load_library()
+        n = L.ceos_failed_elements(self._h, None, 0)
+        ids = np.zeros(max(n, 1), dtype=np.int32)
+        L.ceos_failed_elements(self._h, ids.ctypes.data, n)
+        return ids[:n]
+
+    def gp_stress(self):
+        out = np.zeros(self.ne * 48)
+        _check(load_library().ceos_get_gp_stress(self._h, out), "ceos_get_gp_stress")
+        return out.reshape(self.ne, 8, 6)
+
+    def state(self, which=1):
+        out = np.zeros(self.ne * 64)
+        _check(load_library().ceos_get_state(self._h, which, out), "ceos_get_state")
+        return out.reshape(self.ne, 8, 8)
+
+    def set_state(self, which, arr):
+        _check(load_library().ceos_set_state(self._h, which,
+                                             np.ascontiguousarray(arr, dtype=np.float64).reshape(-1)),
+               "ceos_set_state")
+
+    def homogenize(self):
+        P, F = np.zeros(9), np.zeros(9)
+        _check(load_library().ceos_homogenize(self._h, P, F), "ceos_homogenize")
+        return P, F
+
+    def save_state(self, path):
+        _check(load_library().ceos_state_save(self._h, path.encode()), "ceos_state_save")
+
+    def load_state(self, path):
+        _check(load_library().ceos_state_load(self._h, path.encode()), "ceos_state_load")
+
+    # device-resident path (pointers are integers, e.g. torch.Tensor.data_ptr())
+    def evaluate_device(self, time, dU, dFint, dK, stream=None):
+        _check(load_library().ceos_evaluate_device(self._h, time, dU, dFint, dK, stream),
+               "ceos_evaluate_device")
+
+    def update_mesh_device(self, dU, stream=None):
+        _check(load_library().ceos_update_mesh_device(self._h, dU, stream), "ceos_update_mesh_device")
+
+    def device_status(self):
+        st = CeosStatus()
+        code = load_library().ceos_device_status(self._h, C.byref(st))
+        return code, st

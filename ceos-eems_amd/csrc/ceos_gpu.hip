@@ -1,0 +1,810 @@
+// libceos_gpu.so -- host runtime and C ABI of the MI355X element hot path (see include/ceos_gpu.h).
+//
+// Setup (once per mesh, host C++):
+//   * node-blocked CSR pattern of the element cliques (the reference builds the same pattern
+//     with linked lists, FEM/ModFEMAnalysis.f90:209-273 and ModSparseMatrixRoutines.f90);
+//   * greedy element colouring (elements of one colour share no node), elements of a colour
+//     grouped by material model and kept in id order; device arrays use this slot order;
+//   * per element: the CSR rank byte and first-touch bit of its 28 off-diagonal node blocks
+//     and the first-touch bits of its 8 nodes (see element_kernel.hpp).
+// Evaluation: one launch of the fused element kernel per (colour, model) range, in stream order.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/ceos_gpu.h"
+#include "element_kernel.hpp"
+#include "homogenize_kernel.hpp"
+
+using namespace ceos;
+
+#define HIP_OK(expr)                                                                           \
+    do {                                                                                       \
+        hipError_t _e = (expr);                                                                \
+        if (_e != hipSuccess) {                                                                \
+            fprintf(stderr, "ceos_gpu: %s failed: %s (%s:%d)\n", #expr, hipGetErrorString(_e), \
+                    __FILE__, __LINE__);                                                       \
+            return CEOS_ERR_HIP;                                                               \
+        }                                                                                      \
+    } while (0)
+
+namespace {
+
+constexpr int kMaxMat = 64;
+constexpr int kMaxProps = 8;
+
+struct MatDef {
+    int model = 0;
+    double p[kMaxProps] = {0};
+    bool set = false;
+};
+
+struct LaunchRange {
+    int begin, end, model;
+};
+
+template <typename T>
+void dfree(T*& p) {
+    if (p) (void)hipFree((void*)p);
+    p = nullptr;
+}
+
+template <typename T>
+hipError_t dalloc(T*& p, size_t n) {
+    dfree(p);
+    return hipMalloc((void**)&p, sizeof(T) * (n ? n : 1));
+}
+
+void set_status(ceos_status* st, int code, int nfail, const char* msg) {
+    if (!st) return;
+    st->error = code != 0;
+    st->error_id = code;
+    st->n_failed = nfail;
+    snprintf(st->description, sizeof(st->description), "%s", msg ? msg : "");
+}
+
+bool model_supported(int m) {
+    return m == kGHL || m == kJ2 || m == kNH || m == kSVK || m == kCNH;
+}
+
+int model_nprops(int m) {
+    switch (m) {
+        case kJ2: return 4;
+        default: return 2;
+    }
+}
+
+}  // namespace
+
+struct ceos_handle {
+    int nn = 0, ne = 0;
+    int nl = 1, elem_tech = CEOS_ELEM_FULL_INTEGRATION, precision = CEOS_K_FP64;
+    bool fused = true;
+    int device = 0;
+    hipStream_t stream = nullptr;
+
+    std::vector<int> conn0;      // [ne][8] original order, 0-based
+    std::vector<int> mat_of;     // [ne] dense material index
+    std::vector<int> mat_ids;    // dense index -> user material id
+    MatDef mats[kMaxMat];
+    std::vector<double> X;       // [nn][3]
+
+    // pattern (host)
+    std::vector<int> up_ptr, up_nodes;  // upper neighbours per node (sorted)
+    std::vector<int> rowstart, nup;
+    long long nnz = 0;
+    std::vector<int> color;      // [ne]
+    int ncolor = 0;
+
+    // slot order
+    bool finalized = false;
+    std::vector<int> perm;       // slot -> element
+    std::vector<int> iperm;      // element -> slot
+    std::vector<LaunchRange> ranges;
+    bool has_j2 = false;
+
+    // device
+    double *dX = nullptr, *dx = nullptr, *dU = nullptr, *dFint = nullptr, *dK = nullptr;
+    float* dK32 = nullptr;
+    double *dsig = nullptr, *dst_old = nullptr, *dst_new = nullptr, *dprops = nullptr;
+    int *dconn = nullptr, *dmat = nullptr, *drowstart = nullptr, *dnup = nullptr;
+    uint32_t* dpairs = nullptr;
+    uint8_t *ddiag_first = nullptr, *delem_err = nullptr;
+    int* derr = nullptr;
+    double* dhpart = nullptr;  // homogenisation partial sums
+    int hpart_n = 0;
+
+    bool f_valid = false, k_valid = false, have_u = false;
+    double time = 0.0;
+};
+
+// ------------------------------------------------------------------------------------------
+// setup helpers
+// ------------------------------------------------------------------------------------------
+static int build_pattern(ceos_handle* h) {
+    const int nn = h->nn, ne = h->ne;
+    std::vector<int> deg(nn + 1, 0);
+    for (int e = 0; e < ne; ++e)
+        for (int a = 0; a < 8; ++a) deg[h->conn0[8 * e + a] + 1]++;
+    for (int i = 0; i < nn; ++i) deg[i + 1] += deg[i];
+    std::vector<int> ne_list(deg[nn]);
+    {
+        std::vector<int> fill(deg.begin(), deg.end() - 1);
+        for (int e = 0; e < ne; ++e)
+            for (int a = 0; a < 8; ++a) ne_list[fill[h->conn0[8 * e + a]]++] = e;
+    }
+    h->up_ptr.assign(nn + 1, 0);
+    h->up_nodes.clear();
+    h->up_nodes.reserve((size_t)nn * 14);
+    std::vector<int> nb;
+    for (int A = 0; A < nn; ++A) {
+        if (deg[A + 1] == deg[A]) return CEOS_ERR_ARG;  // node in no element
+        nb.clear();
+        for (int q = deg[A]; q < deg[A + 1]; ++q) {
+            const int e = ne_list[q];
+            for (int a = 0; a < 8; ++a) {
+                const int B = h->conn0[8 * e + a];
+                if (B > A) nb.push_back(B);
+            }
+        }
+        std::sort(nb.begin(), nb.end());
+        nb.erase(std::unique(nb.begin(), nb.end()), nb.end());
+        if (nb.size() > 127) return CEOS_ERR_ARG;  // rank must fit 7 bits
+        h->up_nodes.insert(h->up_nodes.end(), nb.begin(), nb.end());
+        h->up_ptr[A + 1] = (int)h->up_nodes.size();
+    }
+    h->rowstart.assign(nn, 0);
+    h->nup.assign(nn, 0);
+    long long acc = 0;
+    for (int A = 0; A < nn; ++A) {
+        const int nu = h->up_ptr[A + 1] - h->up_ptr[A];
+        h->nup[A] = nu;
+        h->rowstart[A] = (int)acc;
+        acc += 6 + 9LL * nu;
+        if (acc > 2147483000LL) return CEOS_ERR_ARG;  // int32 CSR as in the reference
+    }
+    h->nnz = acc;
+    return CEOS_OK;
+}
+
+static int build_colors(ceos_handle* h) {
+    const int ne = h->ne;
+    std::vector<uint64_t> used(h->nn, 0);
+    h->color.assign(ne, 0);
+    int nc = 0;
+    for (int e = 0; e < ne; ++e) {
+        uint64_t u = 0;
+        for (int a = 0; a < 8; ++a) u |= used[h->conn0[8 * e + a]];
+        int c = 0;
+        while (c < 64 && ((u >> c) & 1ull)) ++c;
+        if (c == 64) return CEOS_ERR_ARG;
+        h->color[e] = c;
+        nc = std::max(nc, c + 1);
+        for (int a = 0; a < 8; ++a) used[h->conn0[8 * e + a]] |= (1ull << c);
+    }
+    h->ncolor = nc;
+    return CEOS_OK;
+}
+
+static inline int rank_of(const ceos_handle* h, int A, int B) {  // A < B
+    const int* b = h->up_nodes.data() + h->up_ptr[A];
+    const int* e = h->up_nodes.data() + h->up_ptr[A + 1];
+    const int* p = std::lower_bound(b, e, B);
+    if (p == e || *p != B) return -1;
+    return (int)(p - b);
+}
+
+static void free_device(ceos_handle* h) {
+    dfree(h->dX); dfree(h->dx); dfree(h->dU); dfree(h->dFint); dfree(h->dK); dfree(h->dK32);
+    dfree(h->dsig); dfree(h->dst_old); dfree(h->dst_new); dfree(h->dprops);
+    dfree(h->dconn); dfree(h->dmat); dfree(h->drowstart); dfree(h->dnup);
+    dfree(h->dpairs); dfree(h->ddiag_first); dfree(h->delem_err); dfree(h->derr); dfree(h->dhpart);
+}
+
+static int upload_props(ceos_handle* h) {
+    std::vector<double> props((size_t)kMaxMat * kMaxProps, 0.0);
+    for (size_t i = 0; i < h->mat_ids.size(); ++i)
+        for (int k = 0; k < kMaxProps; ++k) props[i * kMaxProps + k] = h->mats[i].p[k];
+    HIP_OK(hipMemcpyAsync(h->dprops, props.data(), sizeof(double) * props.size(), hipMemcpyHostToDevice,
+                          h->stream));
+    HIP_OK(hipStreamSynchronize(h->stream));  // props is a temporary
+    return CEOS_OK;
+}
+
+// slot order, launch ranges, rank/first-touch tables, per-element device arrays
+static int finalize(ceos_handle* h) {
+    if (h->finalized) return CEOS_OK;
+    for (size_t i = 0; i < h->mat_ids.size(); ++i)
+        if (!h->mats[i].set) return CEOS_ERR_ARG;
+    const int ne = h->ne;
+    // group by (colour, model), id order inside a group
+    std::vector<int> models;
+    for (size_t i = 0; i < h->mat_ids.size(); ++i) models.push_back(h->mats[i].model);
+    std::sort(models.begin(), models.end());
+    models.erase(std::unique(models.begin(), models.end()), models.end());
+    h->has_j2 = std::find(models.begin(), models.end(), (int)kJ2) != models.end();
+    h->perm.clear();
+    h->ranges.clear();
+    {
+        std::vector<std::vector<int>> bucket((size_t)h->ncolor * models.size());
+        for (int e = 0; e < ne; ++e) {
+            const int m = h->mats[h->mat_of[e]].model;
+            const int mi = (int)(std::lower_bound(models.begin(), models.end(), m) - models.begin());
+            bucket[(size_t)h->color[e] * models.size() + mi].push_back(e);
+        }
+        for (int c = 0; c < h->ncolor; ++c)
+            for (size_t mi = 0; mi < models.size(); ++mi) {
+                auto& v = bucket[(size_t)c * models.size() + mi];
+                if (v.empty()) continue;
+                LaunchRange r{(int)h->perm.size(), 0, models[mi]};
+                h->perm.insert(h->perm.end(), v.begin(), v.end());
+                r.end = (int)h->perm.size();
+                h->ranges.push_back(r);
+            }
+    }
+    h->iperm.assign(ne, 0);
+    for (int s = 0; s < ne; ++s) h->iperm[h->perm[s]] = s;
+
+    // per slot tables in execution order; first-touch over node blocks
+    std::vector<int> conn_s((size_t)ne * 8), mat_s(ne);
+    std::vector<uint32_t> pairs((size_t)ne * 8, 0);
+    std::vector<uint8_t> dfirst(ne, 0);
+    std::vector<uint64_t> touched((h->nnz + 63) / 64, 0);
+    auto test_and_set = [&](long long id) {
+        const bool was = (touched[id >> 6] >> (id & 63)) & 1ull;
+        touched[id >> 6] |= (1ull << (id & 63));
+        return !was;
+    };
+    for (int s = 0; s < ne; ++s) {
+        const int e = h->perm[s];
+        const int* cn = &h->conn0[(size_t)8 * e];
+        for (int a = 0; a < 8; ++a) conn_s[(size_t)8 * s + a] = cn[a];
+        mat_s[s] = h->mat_of[e];
+        uint8_t df = 0;
+        for (int b = 0; b < 8; ++b)
+            if (test_and_set(h->rowstart[cn[b]])) df |= (uint8_t)(1u << b);
+        dfirst[s] = df;
+        for (int b = 0; b < 8; ++b) {
+            const int np = b < 4 ? 4 : 3;
+            uint32_t w = 0;
+            for (int j = 0; j < np; ++j) {
+                const int c = (b + j + 1) & 7;
+                const int A = std::min(cn[b], cn[c]), B = std::max(cn[b], cn[c]);
+                const int r = rank_of(h, A, B);
+                if (r < 0 || A == B) return CEOS_ERR_SLOT;
+                const long long id = (long long)h->rowstart[A] + 3 + 3LL * r;
+                const uint32_t first = test_and_set(id) ? 1u : 0u;
+                w |= ((uint32_t)r | (first << 7)) << (8 * j);
+            }
+            pairs[(size_t)8 * s + b] = w;
+        }
+    }
+
+    HIP_OK(dalloc(h->dconn, (size_t)ne * 8));
+    HIP_OK(dalloc(h->dmat, ne));
+    HIP_OK(dalloc(h->dpairs, (size_t)ne * 8));
+    HIP_OK(dalloc(h->ddiag_first, ne));
+    HIP_OK(dalloc(h->delem_err, ne));
+    HIP_OK(dalloc(h->dsig, (size_t)ne * 48));
+    HIP_OK(hipMemcpyAsync(h->dconn, conn_s.data(), sizeof(int) * conn_s.size(), hipMemcpyHostToDevice, h->stream));
+    HIP_OK(hipMemcpyAsync(h->dmat, mat_s.data(), sizeof(int) * ne, hipMemcpyHostToDevice, h->stream));
+    HIP_OK(hipMemcpyAsync(h->dpairs, pairs.data(), sizeof(uint32_t) * pairs.size(), hipMemcpyHostToDevice, h->stream));
+    HIP_OK(hipMemcpyAsync(h->ddiag_first, dfirst.data(), ne, hipMemcpyHostToDevice, h->stream));
+    HIP_OK(hipMemsetAsync(h->delem_err, 0, ne, h->stream));
+    HIP_OK(hipMemsetAsync(h->dsig, 0, sizeof(double) * (size_t)ne * 48, h->stream));
+    if (h->has_j2) {
+        HIP_OK(dalloc(h->dst_old, (size_t)ne * 8 * kJ2State));
+        HIP_OK(dalloc(h->dst_new, (size_t)ne * 8 * kJ2State));
+        HIP_OK(hipMemsetAsync(h->dst_old, 0, sizeof(double) * (size_t)ne * 8 * kJ2State, h->stream));
+        HIP_OK(hipMemsetAsync(h->dst_new, 0, sizeof(double) * (size_t)ne * 8 * kJ2State, h->stream));
+    }
+    HIP_OK(hipStreamSynchronize(h->stream));
+    h->finalized = true;
+    return CEOS_OK;
+}
+
+// ------------------------------------------------------------------------------------------
+// kernel dispatch
+// ------------------------------------------------------------------------------------------
+template <int MODEL, bool NL, bool WANT_K, typename KT>
+static void launch_t(const ElemParams& p, int b, int e, hipStream_t s) {
+    const int n = e - b;
+    const int grid = (n + kEPB - 1) / kEPB;
+    hipLaunchKernelGGL((k_element<MODEL, NL, WANT_K, KT>), dim3(grid), dim3(kEPB * 8), 0, s, p, b, e);
+}
+
+template <int MODEL, typename KT>
+static void launch_m(const ElemParams& p, int b, int e, bool nl, bool wk, hipStream_t s) {
+    if (nl) {
+        if (wk) launch_t<MODEL, true, true, KT>(p, b, e, s);
+        else launch_t<MODEL, true, false, KT>(p, b, e, s);
+    } else {
+        if (wk) launch_t<MODEL, false, true, KT>(p, b, e, s);
+        else launch_t<MODEL, false, false, KT>(p, b, e, s);
+    }
+}
+
+template <typename KT>
+static void launch_range(const ElemParams& p, const LaunchRange& r, bool nl, bool wk, hipStream_t s) {
+    switch (r.model) {
+        case kGHL: launch_m<kGHL, KT>(p, r.begin, r.end, nl, wk, s); break;
+        case kJ2: launch_m<kJ2, KT>(p, r.begin, r.end, nl, wk, s); break;
+        case kNH: launch_m<kNH, KT>(p, r.begin, r.end, nl, wk, s); break;
+        case kSVK: launch_m<kSVK, KT>(p, r.begin, r.end, nl, wk, s); break;
+        case kCNH: launch_m<kCNH, KT>(p, r.begin, r.end, nl, wk, s); break;
+        default: break;
+    }
+}
+
+// one evaluation: sigma/state for every GP, Fint, and (if K != null) the tangent
+static int evaluate(ceos_handle* h, const double* dU, double* dFint, void* dK, hipStream_t s) {
+    int st = finalize(h);
+    if (st) return st;
+    ElemParams p;
+    p.ne = h->ne;
+    p.conn = h->dconn;
+    p.mat = h->dmat;
+    p.props = h->dprops;
+    p.X = h->dX;
+    p.x = h->dx;
+    p.U = dU;
+    p.pairs = h->dpairs;
+    p.diag_first = h->ddiag_first;
+    p.rowstart = h->drowstart;
+    p.nup = h->dnup;
+    p.Fint = dFint;
+    p.K = dK;
+    p.sig = h->dsig;
+    p.st_old = h->dst_old;
+    p.st_new = h->dst_new;
+    p.elem_err = h->delem_err;
+    p.err_count = h->derr;
+    hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(1), 0, s, h->derr);
+    const bool wk = dK != nullptr;
+    for (const auto& r : h->ranges) {
+        if (h->precision == CEOS_K_FP32_STORED) launch_range<float>(p, r, h->nl != 0, wk, s);
+        else launch_range<double>(p, r, h->nl != 0, wk, s);
+    }
+    HIP_OK(hipGetLastError());
+    return CEOS_OK;
+}
+
+static int read_status(ceos_handle* h, ceos_status* st) {
+    int nerr = 0;
+    HIP_OK(hipMemcpyAsync(&nerr, h->derr, sizeof(int), hipMemcpyDeviceToHost, h->stream));
+    HIP_OK(hipStreamSynchronize(h->stream));
+    if (nerr) {
+        std::vector<uint8_t> flags(h->ne);
+        HIP_OK(hipMemcpy(flags.data(), h->delem_err, h->ne, hipMemcpyDeviceToHost));
+        int n = 0;
+        for (int s = 0; s < h->ne; ++s) n += flags[s] != 0;
+        set_status(st, CEOS_ERR_DETJ, n,
+                   "ElementVolume / DeformationGradient: Determinant of the Jacobian Matrix <= 0.0d0");
+        return CEOS_ERR_DETJ;
+    }
+    set_status(st, CEOS_OK, 0, "");
+    return CEOS_OK;
+}
+
+static int ensure_k_buffers(ceos_handle* h) {
+    if (h->precision == CEOS_K_FP32_STORED) {
+        if (!h->dK32) HIP_OK(dalloc(h->dK32, (size_t)h->nnz));
+    } else {
+        if (!h->dK) HIP_OK(dalloc(h->dK, (size_t)h->nnz));
+    }
+    return CEOS_OK;
+}
+
+static void* kbuf(ceos_handle* h) {
+    return h->precision == CEOS_K_FP32_STORED ? (void*)h->dK32 : (void*)h->dK;
+}
+
+// ------------------------------------------------------------------------------------------
+// C ABI
+// ------------------------------------------------------------------------------------------
+extern "C" {
+
+int ceos_mesh_create(int nn, const double* X, int ne, int npe, const int* conn, const int* mat_id,
+                     ceos_handle** out) {
+    if (!out) return CEOS_ERR_ARG;
+    *out = nullptr;
+    if (nn <= 0 || ne <= 0 || npe != 8 || !X || !conn) return CEOS_ERR_ARG;
+    ceos_handle* h = new ceos_handle();
+    h->nn = nn;
+    h->ne = ne;
+    h->X.assign(X, X + 3L * nn);
+    h->conn0.resize((size_t)8 * ne);
+    for (long i = 0; i < 8L * ne; ++i) {
+        const int n = conn[i] - 1;
+        if (n < 0 || n >= nn) { delete h; return CEOS_ERR_ARG; }
+        h->conn0[i] = n;
+    }
+    for (int e = 0; e < ne; ++e)
+        for (int a = 0; a < 8; ++a)
+            for (int b = a + 1; b < 8; ++b)
+                if (h->conn0[8 * e + a] == h->conn0[8 * e + b]) { delete h; return CEOS_ERR_ARG; }
+    {
+        std::map<int, int> dense;
+        for (int e = 0; e < ne; ++e) dense[mat_id ? mat_id[e] : 1] = 0;
+        if ((int)dense.size() > kMaxMat) { delete h; return CEOS_ERR_ARG; }
+        for (auto& kv : dense) { kv.second = (int)h->mat_ids.size(); h->mat_ids.push_back(kv.first); }
+        h->mat_of.resize(ne);
+        for (int e = 0; e < ne; ++e) h->mat_of[e] = dense[mat_id ? mat_id[e] : 1];
+    }
+    int st = build_pattern(h);
+    if (!st) st = build_colors(h);
+    if (st) { delete h; return st; }
+    if (hipGetDevice(&h->device) != hipSuccess ||
+        hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete h;
+        return CEOS_ERR_HIP;
+    }
+    auto fail = [&](int code) { free_device(h); (void)hipStreamDestroy(h->stream); delete h; return code; };
+    if (dalloc(h->dX, 3L * nn) || dalloc(h->dx, 3L * nn) || dalloc(h->dU, 3L * nn) ||
+        dalloc(h->dFint, 3L * nn) || dalloc(h->drowstart, nn) || dalloc(h->dnup, nn) ||
+        dalloc(h->dprops, (size_t)kMaxMat * kMaxProps) || dalloc(h->derr, 1))
+        return fail(CEOS_ERR_HIP);
+    if (hipMemcpy(h->dX, X, sizeof(double) * 3L * nn, hipMemcpyHostToDevice) ||
+        hipMemcpy(h->dx, X, sizeof(double) * 3L * nn, hipMemcpyHostToDevice) ||
+        hipMemset(h->dU, 0, sizeof(double) * 3L * nn) ||
+        hipMemcpy(h->drowstart, h->rowstart.data(), sizeof(int) * nn, hipMemcpyHostToDevice) ||
+        hipMemcpy(h->dnup, h->nup.data(), sizeof(int) * nn, hipMemcpyHostToDevice) ||
+        hipMemset(h->dprops, 0, sizeof(double) * kMaxMat * kMaxProps) ||
+        hipMemset(h->derr, 0, sizeof(int)))
+        return fail(CEOS_ERR_HIP);
+    *out = h;
+    return CEOS_OK;
+}
+
+void ceos_mesh_destroy(ceos_handle* h) {
+    if (!h) return;
+    (void)hipStreamSynchronize(h->stream);
+    free_device(h);
+    (void)hipStreamDestroy(h->stream);
+    delete h;
+}
+
+int ceos_material_set(ceos_handle* h, int mat_id, int model, const double* props, int nprops) {
+    if (!h || !props || !model_supported(model) || nprops < model_nprops(model) || nprops > kMaxProps)
+        return CEOS_ERR_ARG;
+    auto it = std::find(h->mat_ids.begin(), h->mat_ids.end(), mat_id);
+    if (it == h->mat_ids.end()) return CEOS_ERR_ARG;
+    const int i = (int)(it - h->mat_ids.begin());
+    MatDef& m = h->mats[i];
+    if (m.set && m.model != model) h->finalized = false;  // launch grouping changes
+    if (!m.set) h->finalized = false;
+    m.model = model;
+    for (int k = 0; k < kMaxProps; ++k) m.p[k] = k < nprops ? props[k] : 0.0;
+    m.set = true;
+    h->f_valid = h->k_valid = false;
+    return upload_props(h);
+}
+
+int ceos_set_analysis(ceos_handle* h, int nl_analysis, int elem_tech) {
+    if (!h || elem_tech != CEOS_ELEM_FULL_INTEGRATION) return CEOS_ERR_ARG;
+    h->nl = nl_analysis ? 1 : 0;
+    h->elem_tech = elem_tech;
+    h->f_valid = h->k_valid = false;
+    return CEOS_OK;
+}
+
+int ceos_set_precision(ceos_handle* h, int mode) {
+    if (!h || (mode != CEOS_K_FP64 && mode != CEOS_K_FP32_STORED)) return CEOS_ERR_ARG;
+    h->precision = mode;
+    h->k_valid = false;
+    return CEOS_OK;
+}
+
+int ceos_set_fused(ceos_handle* h, int fused) {
+    if (!h) return CEOS_ERR_ARG;
+    h->fused = fused != 0;
+    return CEOS_OK;
+}
+
+int ceos_mesh_info(ceos_handle* h, int* n_colors, long long* nnz_upper) {
+    if (!h) return CEOS_ERR_ARG;
+    if (n_colors) *n_colors = h->ncolor;
+    if (nnz_upper) *nnz_upper = h->nnz;
+    return CEOS_OK;
+}
+
+int ceos_csr_pattern(ceos_handle* h, int upper, int* n, long long* nnz, int* rowmap, int* col) {
+    if (!h) return CEOS_ERR_ARG;
+    const int nn = h->nn, ndof = 3 * nn;
+    if (upper) {
+        if (n) *n = ndof;
+        if (nnz) *nnz = h->nnz;
+        if (!rowmap && !col) return CEOS_OK;
+        if (!rowmap || !col) return CEOS_ERR_ARG;
+        long long k = 0;
+        for (int A = 0; A < nn; ++A) {
+            for (int i = 0; i < 3; ++i) {
+                const int r = 3 * A + i;
+                rowmap[r] = (int)(k + 1);
+                for (int c = r; c < 3 * A + 3; ++c) col[k++] = c + 1;
+                for (int q = h->up_ptr[A]; q < h->up_ptr[A + 1]; ++q) {
+                    const int B = h->up_nodes[q];
+                    for (int kk = 0; kk < 3; ++kk) col[k++] = 3 * B + kk + 1;
+                }
+            }
+        }
+        rowmap[ndof] = (int)(k + 1);
+        return k == h->nnz ? CEOS_OK : CEOS_ERR_SLOT;
+    }
+    // full pattern: every neighbour (lower and upper) of the node, 3 columns each
+    std::vector<int> lo_cnt(nn, 0);
+    for (int A = 0; A < nn; ++A)
+        for (int q = h->up_ptr[A]; q < h->up_ptr[A + 1]; ++q) lo_cnt[h->up_nodes[q]]++;
+    long long total = 0;
+    for (int A = 0; A < nn; ++A) total += 3LL * 3 * (1 + lo_cnt[A] + h->nup[A]);
+    if (n) *n = ndof;
+    if (nnz) *nnz = total;
+    if (!rowmap && !col) return CEOS_OK;
+    if (!rowmap || !col) return CEOS_ERR_ARG;
+    std::vector<std::vector<int>> lower(nn);
+    for (int A = 0; A < nn; ++A)
+        for (int q = h->up_ptr[A]; q < h->up_ptr[A + 1]; ++q) lower[h->up_nodes[q]].push_back(A);
+    long long k = 0;
+    for (int A = 0; A < nn; ++A) {
+        std::vector<int> nbs = lower[A];
+        std::sort(nbs.begin(), nbs.end());
+        nbs.push_back(A);
+        for (int q = h->up_ptr[A]; q < h->up_ptr[A + 1]; ++q) nbs.push_back(h->up_nodes[q]);
+        for (int i = 0; i < 3; ++i) {
+            rowmap[3 * A + i] = (int)(k + 1);
+            for (int B : nbs)
+                for (int kk = 0; kk < 3; ++kk) col[k++] = 3 * B + kk + 1;
+        }
+    }
+    rowmap[ndof] = (int)(k + 1);
+    return CEOS_OK;
+}
+
+int ceos_solve_constitutive(ceos_handle* h, double time, const double* U, ceos_status* st) {
+    if (!h || !U) return CEOS_ERR_ARG;
+    int rc = finalize(h);
+    if (rc) { set_status(st, rc, 0, "material not set or mesh setup failed"); return rc; }
+    HIP_OK(hipMemcpyAsync(h->dU, U, sizeof(double) * 3L * h->nn, hipMemcpyHostToDevice, h->stream));
+    h->have_u = true;
+    h->time = time;
+    void* K = nullptr;
+    if (h->fused) {
+        rc = ensure_k_buffers(h);
+        if (rc) return rc;
+        K = kbuf(h);
+    }
+    rc = evaluate(h, h->dU, h->dFint, K, h->stream);
+    if (rc) { set_status(st, rc, 0, "launch failed"); return rc; }
+    h->f_valid = true;
+    h->k_valid = K != nullptr;
+    return read_status(h, st);
+}
+
+int ceos_internal_force(ceos_handle* h, double* Fint, ceos_status* st) {
+    if (!h || !Fint || !h->have_u) return CEOS_ERR_ARG;
+    if (!h->f_valid) {
+        int rc = evaluate(h, h->dU, h->dFint, nullptr, h->stream);
+        if (rc) return rc;
+        h->f_valid = true;
+    }
+    HIP_OK(hipMemcpyAsync(Fint, h->dFint, sizeof(double) * 3L * h->nn, hipMemcpyDeviceToHost, h->stream));
+    return read_status(h, st);
+}
+
+static int tangent_common(ceos_handle* h) {
+    if (!h->have_u) return CEOS_ERR_ARG;
+    if (!h->k_valid) {
+        int rc = ensure_k_buffers(h);
+        if (rc) return rc;
+        rc = evaluate(h, h->dU, h->dFint, kbuf(h), h->stream);
+        if (rc) return rc;
+        h->k_valid = h->f_valid = true;
+    }
+    return CEOS_OK;
+}
+
+int ceos_tangent(ceos_handle* h, double* Kval) {
+    if (!h || !Kval) return CEOS_ERR_ARG;
+    if (h->precision != CEOS_K_FP64) return CEOS_ERR_ARG;
+    int rc = tangent_common(h);
+    if (rc) return rc;
+    HIP_OK(hipMemcpyAsync(Kval, h->dK, sizeof(double) * h->nnz, hipMemcpyDeviceToHost, h->stream));
+    HIP_OK(hipStreamSynchronize(h->stream));
+    return CEOS_OK;
+}
+
+int ceos_tangent_f32(ceos_handle* h, float* Kval) {
+    if (!h || !Kval) return CEOS_ERR_ARG;
+    if (h->precision != CEOS_K_FP32_STORED) return CEOS_ERR_ARG;
+    int rc = tangent_common(h);
+    if (rc) return rc;
+    HIP_OK(hipMemcpyAsync(Kval, h->dK32, sizeof(float) * h->nnz, hipMemcpyDeviceToHost, h->stream));
+    HIP_OK(hipStreamSynchronize(h->stream));
+    return CEOS_OK;
+}
+
+int ceos_update_mesh(ceos_handle* h, const double* U) {
+    if (!h || !U) return CEOS_ERR_ARG;
+    std::vector<double> x(3L * h->nn);
+    for (long i = 0; i < 3L * h->nn; ++i) x[i] = h->X[i] + U[i];
+    HIP_OK(hipMemcpyAsync(h->dx, x.data(), sizeof(double) * x.size(), hipMemcpyHostToDevice, h->stream));
+    HIP_OK(hipStreamSynchronize(h->stream));
+    h->f_valid = h->k_valid = false;
+    return CEOS_OK;
+}
+
+int ceos_switch_converged_state(ceos_handle* h) {
+    if (!h) return CEOS_ERR_ARG;
+    if (h->finalized && h->has_j2) {
+        HIP_OK(hipMemcpyAsync(h->dst_old, h->dst_new, sizeof(double) * (size_t)h->ne * 8 * 7,
+                              hipMemcpyDeviceToDevice, h->stream));
+        HIP_OK(hipStreamSynchronize(h->stream));
+    }
+    h->f_valid = h->k_valid = false;
+    return CEOS_OK;
+}
+
+int ceos_failed_elements(ceos_handle* h, int* ids, int cap) {
+    if (!h) return CEOS_ERR_ARG;
+    HIP_OK(hipStreamSynchronize(h->stream));
+    if (!h->finalized) return 0;
+    std::vector<uint8_t> flags(h->ne);
+    if (hipMemcpy(flags.data(), h->delem_err, h->ne, hipMemcpyDeviceToHost) != hipSuccess) return CEOS_ERR_HIP;
+    std::vector<int> out;
+    for (int s = 0; s < h->ne; ++s)
+        if (flags[s]) out.push_back(h->perm[s] + 1);
+    std::sort(out.begin(), out.end());
+    for (int i = 0; i < (int)out.size() && i < cap; ++i) ids[i] = out[i];
+    return (int)out.size();
+}
+
+int ceos_get_gp_stress(ceos_handle* h, double* sig) {
+    if (!h || !sig || !h->finalized) return CEOS_ERR_ARG;
+    HIP_OK(hipStreamSynchronize(h->stream));
+    const int ne = h->ne;
+    std::vector<double> d((size_t)ne * 48);
+    HIP_OK(hipMemcpy(d.data(), h->dsig, sizeof(double) * d.size(), hipMemcpyDeviceToHost));
+    for (int s = 0; s < ne; ++s) {
+        const int e = h->perm[s];
+        for (int g = 0; g < 8; ++g)
+            for (int k = 0; k < 6; ++k) sig[((size_t)e * 8 + g) * 6 + k] = d[((size_t)k * 8 + g) * ne + s];
+    }
+    return CEOS_OK;
+}
+
+int ceos_get_state(ceos_handle* h, int which, double* state) {
+    if (!h || !state) return CEOS_ERR_ARG;
+    HIP_OK(hipStreamSynchronize(h->stream));
+    int rc = finalize(h);
+    if (rc) return rc;
+    const int ne = h->ne;
+    if (!h->has_j2) {
+        memset(state, 0, sizeof(double) * (size_t)ne * 8 * kJ2State);
+        return CEOS_OK;
+    }
+    std::vector<double> d((size_t)ne * 8 * kJ2State);
+    HIP_OK(hipMemcpy(d.data(), which ? h->dst_new : h->dst_old, sizeof(double) * d.size(), hipMemcpyDeviceToHost));
+    for (int s = 0; s < ne; ++s) {
+        const int e = h->perm[s];
+        for (int g = 0; g < 8; ++g)
+            for (int v = 0; v < kJ2State; ++v)
+                state[((size_t)e * 8 + g) * kJ2State + v] = d[((size_t)v * 8 + g) * ne + s];
+    }
+    return CEOS_OK;
+}
+
+int ceos_set_state(ceos_handle* h, int which, const double* state) {
+    if (!h || !state) return CEOS_ERR_ARG;
+    HIP_OK(hipStreamSynchronize(h->stream));
+    int rc = finalize(h);
+    if (rc) return rc;
+    if (!h->has_j2) return CEOS_OK;
+    const int ne = h->ne;
+    std::vector<double> d((size_t)ne * 8 * kJ2State);
+    for (int s = 0; s < ne; ++s) {
+        const int e = h->perm[s];
+        for (int g = 0; g < 8; ++g)
+            for (int v = 0; v < kJ2State; ++v)
+                d[((size_t)v * 8 + g) * ne + s] = state[((size_t)e * 8 + g) * kJ2State + v];
+    }
+    HIP_OK(hipMemcpy(which ? h->dst_new : h->dst_old, d.data(), sizeof(double) * d.size(), hipMemcpyHostToDevice));
+    h->f_valid = h->k_valid = false;
+    return CEOS_OK;
+}
+
+int ceos_homogenize(ceos_handle* h, double P[9], double F[9]) {
+    if (!h || !P || !F || !h->have_u || !h->finalized) return CEOS_ERR_ARG;
+    const int nblk = (h->ne * 8 + kHomBlock - 1) / kHomBlock;
+    if (h->hpart_n < nblk) {
+        HIP_OK(dalloc(h->dhpart, (size_t)nblk * kHomVals));
+        h->hpart_n = nblk;
+    }
+    hipLaunchKernelGGL(k_homogenize_partial, dim3(nblk), dim3(kHomBlock), 0, h->stream, h->ne, h->dconn, h->dX,
+                       h->dU, h->dsig, h->dhpart);
+    HIP_OK(hipGetLastError());
+    std::vector<double> part((size_t)nblk * kHomVals);
+    HIP_OK(hipMemcpyAsync(part.data(), h->dhpart, sizeof(double) * part.size(), hipMemcpyDeviceToHost, h->stream));
+    HIP_OK(hipStreamSynchronize(h->stream));
+    double acc[kHomVals] = {0};
+    for (int b = 0; b < nblk; ++b)
+        for (int k = 0; k < kHomVals; ++k) acc[k] += part[(size_t)b * kHomVals + k];
+    const double V = acc[18];
+    for (int k = 0; k < 9; ++k) {
+        P[k] = acc[k] / V;
+        F[k] = acc[9 + k] / V;
+    }
+    return CEOS_OK;
+}
+
+int ceos_state_save(ceos_handle* h, const char* path) {
+    if (!h || !path) return CEOS_ERR_ARG;
+    int rc = finalize(h);
+    if (rc) return rc;
+    FILE* f = fopen(path, "wb");
+    if (!f) return CEOS_ERR_ARG;
+    const int hdr[4] = {0x43454f53, h->nn, h->ne, h->has_j2 ? 1 : 0};
+    std::vector<double> x(3L * h->nn);
+    bool ok = hipMemcpy(x.data(), h->dx, sizeof(double) * x.size(), hipMemcpyDeviceToHost) == hipSuccess;
+    ok = ok && fwrite(hdr, sizeof(hdr), 1, f) == 1 && fwrite(x.data(), sizeof(double), x.size(), f) == x.size();
+    if (ok && h->has_j2) {
+        std::vector<double> s((size_t)h->ne * 8 * kJ2State);
+        for (int which = 0; which < 2 && ok; ++which) {
+            ok = ceos_get_state(h, which, s.data()) == CEOS_OK &&
+                 fwrite(s.data(), sizeof(double), s.size(), f) == s.size();
+        }
+    }
+    fclose(f);
+    return ok ? CEOS_OK : CEOS_ERR_ARG;
+}
+
+int ceos_state_load(ceos_handle* h, const char* path) {
+    if (!h || !path) return CEOS_ERR_ARG;
+    int rc = finalize(h);
+    if (rc) return rc;
+    FILE* f = fopen(path, "rb");
+    if (!f) return CEOS_ERR_ARG;
+    int hdr[4];
+    bool ok = fread(hdr, sizeof(hdr), 1, f) == 1 && hdr[0] == 0x43454f53 && hdr[1] == h->nn && hdr[2] == h->ne &&
+              hdr[3] == (h->has_j2 ? 1 : 0);
+    std::vector<double> x(3L * h->nn);
+    ok = ok && fread(x.data(), sizeof(double), x.size(), f) == x.size();
+    if (ok) ok = hipMemcpy(h->dx, x.data(), sizeof(double) * x.size(), hipMemcpyHostToDevice) == hipSuccess;
+    if (ok && h->has_j2) {
+        std::vector<double> s((size_t)h->ne * 8 * kJ2State);
+        for (int which = 0; which < 2 && ok; ++which)
+            ok = fread(s.data(), sizeof(double), s.size(), f) == s.size() && ceos_set_state(h, which, s.data()) == CEOS_OK;
+    }
+    fclose(f);
+    h->f_valid = h->k_valid = false;
+    return ok ? CEOS_OK : CEOS_ERR_ARG;
+}
+
+int ceos_evaluate_device(ceos_handle* h, double time, const double* dU, double* dFint, void* dK, void* stream) {
+    if (!h || !dU || !dFint) return CEOS_ERR_ARG;
+    h->time = time;
+    hipStream_t s = stream ? (hipStream_t)stream : h->stream;
+    return evaluate(h, dU, dFint, dK, s);
+}
+
+int ceos_update_mesh_device(ceos_handle* h, const double* dU, void* stream) {
+    if (!h || !dU) return CEOS_ERR_ARG;
+    hipStream_t s = stream ? (hipStream_t)stream : h->stream;
+    const long n = 3L * h->nn;
+    hipLaunchKernelGGL(k_add, dim3((n + 255) / 256), dim3(256), 0, s, n, h->dX, dU, h->dx);
+    HIP_OK(hipGetLastError());
+    h->f_valid = h->k_valid = false;
+    return CEOS_OK;
+}
+
+int ceos_device_status(ceos_handle* h, ceos_status* st) {
+    if (!h) return CEOS_ERR_ARG;
+    return read_status(h, st);
+}
+
+}  // extern "C"

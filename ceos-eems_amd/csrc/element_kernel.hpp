@@ -1,0 +1,310 @@
+// Fused hex8 element kernel: constitutive update + internal force + tangent stiffness + CSR
+// scatter, one pass over the elements of one colour.
+//
+// Work of the reference loops it replaces (file:line relative to SourceCode/):
+//   SolveConstitutiveModel   FEM/Subroutines/SubSolveConstitutiveModel.f90:56-136
+//   InternalForce            FEM/Subroutines/SubInternalForce.f90:49-97 (+ ModElement.f90:484-562)
+//   TangentStiffnessMatrix   FEM/Subroutines/SubTangentStiffnessMatrix.f90:63,294-310
+//                            (+ ModElement.f90:200-348, SubAssembleGlobalMatrixUpperTriangular.f90)
+//
+// Mapping (wave64): 8 lanes per element, 8 elements per wavefront.
+//   phase 0: lane a gathers node a (X, x, u) into LDS.
+//   phase 1: lane g evaluates Gauss point g: grad_X N, F, grad_x N, detJ, sigma, D (upper),
+//            writes sigma (and the material state) to HBM and (grad_x N, w detJ sigma,
+//            w detJ D) to LDS.
+//   phase 2: lane b owns node b: Fe_b = sum_g B_b^T sigma w detJ, the diagonal 3x3 block
+//            (b,b) and the off-diagonal blocks (c,b), c = b+1..b+3 (and b+4 for b < 4) mod 8,
+//            K_cb = sum_g w detJ (B_c^T D B_b + (gradN_c . sigma gradN_b) I).
+//            All 36 node blocks of the element are computed exactly once.
+// Scatter is atomic-free: elements of one colour share no node, colours run in stream order,
+// and the first colour that touches a node block writes instead of adding (so neither K nor
+// Fint needs zeroing).  CSR slots come from the node-blocked structure of the pattern:
+//   row 3A+i starts at rowstart[A] + {0, 3+3 nup[A], 5+6 nup[A]}[i]; inside it the diagonal
+//   block columns come first, then 3 columns per upper neighbour in ascending node order, so
+//   slot(3A+i, 3B+k) = rowptr(A,i) + (3-i) + 3 rank_A(B) + k for A < B.
+// rank and the first-touch bit of each off-diagonal block are 1 byte per (element, lane, j).
+#pragma once
+#include <stdint.h>
+
+#include "device_math.hpp"
+
+namespace ceos {
+
+constexpr int kEPB = 16;              // elements per workgroup (128 threads = 2 waves)
+constexpr int kLdsPerElem = 408;      // doubles: gradN 8x8x3 + sigma 8x6 + D 8x21
+constexpr int kOffSig = 192;
+constexpr int kOffD = 240;
+constexpr int kJ2State = 8;           // [0..5] plastic strain, [6] acc. plastic strain, [7] flag
+
+struct ElemParams {
+    int ne;                        // elements (device slot order)
+    const int* conn;               // [ne][8] node ids, 0-based
+    const int* mat;                // [ne] dense material index
+    const double* props;           // [nmat][8]
+    const double* X;               // [nn][3] reference coordinates
+    const double* x;               // [nn][3] current coordinates
+    const double* U;               // [nn][3] total displacement
+    const uint32_t* pairs;         // [ne][8]: byte j-1 = rank | first<<7 of block (b+j, b)
+    const uint8_t* diag_first;     // [ne]: bit b = first writer of node b (diag block, Fint)
+    const int* rowstart;           // [nn] slot of (3A, 3A)
+    const int* nup;                // [nn] upper neighbour count
+    double* Fint;                  // [3 nn]
+    void* K;                       // [nnz] double or float
+    double* sig;                   // [6][8][ne]
+    const double* st_old;          // [kJ2State][8][ne]
+    double* st_new;                // [kJ2State][8][ne]
+    uint8_t* elem_err;             // [ne]
+    int* err_count;
+};
+
+template <typename KT>
+__device__ __forceinline__ void kstore(KT* K, long s, double v, bool first) {
+    if (first) K[s] = (KT)v;
+    else K[s] += (KT)v;
+}
+
+__device__ __forceinline__ long rowptr(int rs, int nu, int i) {
+    return (long)rs + (i == 0 ? 0 : (i == 1 ? 3 + 3 * nu : 5 + 6 * nu));
+}
+
+template <int MODEL>
+__device__ __forceinline__ bool material(const double* p, const double F[3][3], const double old[7],
+                                         double s[6], double D[21], double nw[7]) {
+    if constexpr (MODEL == kGHL) { mat_ghl(p, F, s, D); return false; }
+    else if constexpr (MODEL == kCNH) { mat_cnh(p, F, s, D); return false; }
+    else if constexpr (MODEL == kNH) { mat_nh(p, F, s, D); return false; }
+    else if constexpr (MODEL == kSVK) { mat_svk(p, F, s, D); return false; }
+    else { return mat_j2(p, F, old, s, D, nw); }
+}
+
+template <int MODEL, bool NL, bool WANT_K, typename KT>
+__global__ __launch_bounds__(128) void k_element(ElemParams p, int e_begin, int e_end) {
+    __shared__ double smem[kEPB * kLdsPerElem];
+    __shared__ int snode[kEPB][8];
+    const int lane = threadIdx.x & 7;
+    const int le = threadIdx.x >> 3;
+    const int e = e_begin + blockIdx.x * kEPB + le;
+    const bool active = e < e_end;
+    double* sm = smem + le * kLdsPerElem;
+
+    // ---------------- phase 0: gather node lane -----------------
+    int node = 0;
+    if (active) node = p.conn[(long)e * 8 + lane];
+    snode[le][lane] = node;
+    {
+        double v[9];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            v[k] = active ? p.X[3L * node + k] : 0.0;
+            v[3 + k] = active ? p.x[3L * node + k] : 0.0;
+            v[6 + k] = active ? p.U[3L * node + k] : 0.0;
+        }
+#pragma unroll
+        for (int k = 0; k < 9; ++k) sm[lane * 9 + k] = v[k];
+    }
+    __syncthreads();
+
+    // ---------------- phase 1: Gauss point g = lane -----------------
+    const int g = lane;
+    double gx[8][3];
+    double s[6], D[21];
+    double wdet;
+    bool bad;
+    {
+        double dN[8][3];
+        hex8_dshape_gp(g, dN);
+        double c[8][3];
+#pragma unroll
+        for (int a = 0; a < 8; ++a)
+#pragma unroll
+            for (int k = 0; k < 3; ++k) c[a][k] = sm[a * 9 + k];
+        double gX[8][3];
+        const double detX = hex8_grad(dN, c, gX);
+        double F[3][3];
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                double acc = 0.0;
+#pragma unroll
+                for (int a = 0; a < 8; ++a) acc += gX[a][j] * sm[a * 9 + 6 + i];
+                F[i][j] = acc + (i == j ? 1.0 : 0.0);
+            }
+#pragma unroll
+        for (int a = 0; a < 8; ++a)
+#pragma unroll
+            for (int k = 0; k < 3; ++k) c[a][k] = sm[a * 9 + 3 + k];
+        const double detx = hex8_grad(dN, c, gx);
+        bad = active && !(detx > kDetTol && detX > kDetTol);
+        wdet = detx;  // Gauss weight 1 (ModElementHexa8.f90:326)
+
+        double old[7] = {0, 0, 0, 0, 0, 0, 0}, nw[7];
+        const int mi = active ? p.mat[e] : 0;
+        const double* pr = p.props + 8 * mi;
+        if constexpr (MODEL == kJ2) {
+            if (active) {
+#pragma unroll
+                for (int v = 0; v < 7; ++v) old[v] = p.st_old[((long)v * 8 + g) * p.ne + e];
+            }
+        }
+        const bool plastic = material<MODEL>(pr, F, old, s, D, nw);
+        if (active) {
+#pragma unroll
+            for (int k = 0; k < 6; ++k) p.sig[((long)k * 8 + g) * p.ne + e] = s[k];
+            if constexpr (MODEL == kJ2) {
+#pragma unroll
+                for (int v = 0; v < 7; ++v) p.st_new[((long)v * 8 + g) * p.ne + e] = nw[v];
+                p.st_new[((long)7 * 8 + g) * p.ne + e] = plastic ? 1.0 : 0.0;
+            }
+        }
+        (void)plastic;
+    }
+    {
+        // per-element OR of the 8 Gauss-point detJ checks (8 lanes of one element)
+        const unsigned long long m = __ballot(bad ? 1 : 0);
+        const unsigned bits = (unsigned)(m >> (threadIdx.x & 56)) & 0xffu;  // this element's 8 lanes
+        if (active && lane == 0) {
+            p.elem_err[e] = bits ? 1 : 0;
+            if (bits) atomicAdd(p.err_count, 1);
+        }
+    }
+    __syncthreads();  // node data no longer needed: reuse the LDS slice for GP data
+#pragma unroll
+    for (int a = 0; a < 8; ++a)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) sm[g * 24 + a * 3 + k] = gx[a][k];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) sm[kOffSig + g * 6 + k] = s[k] * wdet;
+    if constexpr (WANT_K) {
+#pragma unroll
+        for (int k = 0; k < 21; ++k) sm[kOffD + g * 21 + k] = D[k] * wdet;
+    }
+    __syncthreads();
+
+    // ---------------- phase 2: node b = lane -----------------
+    const int b = lane;
+    const int npart = (b < 4) ? 4 : 3;
+    double fe0 = 0.0, fe1 = 0.0, fe2 = 0.0;
+    double kd[6];
+    double ko[4][9];
+#pragma unroll
+    for (int i = 0; i < 6; ++i) kd[i] = 0.0;
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+        for (int i = 0; i < 9; ++i) ko[j][i] = 0.0;
+
+#pragma unroll 1
+    for (int gp = 0; gp < 8; ++gp) {
+        const double* Gp = sm + gp * 24;
+        const double* Sp = sm + kOffSig + gp * 6;
+        const double b0 = Gp[b * 3 + 0], b1 = Gp[b * 3 + 1], b2 = Gp[b * 3 + 2];
+        const double S0 = Sp[0], S1 = Sp[1], S2 = Sp[2], S3 = Sp[3], S4 = Sp[4], S5 = Sp[5];
+        const double sg0 = S0 * b0 + S3 * b1 + S5 * b2;
+        const double sg1 = S3 * b0 + S1 * b1 + S4 * b2;
+        const double sg2 = S5 * b0 + S4 * b1 + S2 * b2;
+        fe0 += sg0;
+        fe1 += sg1;
+        fe2 += sg2;
+        if constexpr (WANT_K) {
+            const double* Dp = sm + kOffD + gp * 21;
+            double Dl[21];
+#pragma unroll
+            for (int k = 0; k < 21; ++k) Dl[k] = Dp[k];
+#define DD(I, J) Dl[(I) <= (J) ? up_idx(I, J) : up_idx(J, I)]
+            double DB[6][3];
+#pragma unroll
+            for (int I = 0; I < 6; ++I) {
+                DB[I][0] = DD(I, 0) * b0 + DD(I, 3) * b1 + DD(I, 5) * b2;
+                DB[I][1] = DD(I, 1) * b1 + DD(I, 3) * b0 + DD(I, 4) * b2;
+                DB[I][2] = DD(I, 2) * b2 + DD(I, 4) * b1 + DD(I, 5) * b0;
+            }
+#undef DD
+            // diagonal block, upper part (k >= i)
+            {
+                const double geo = NL ? (b0 * sg0 + b1 * sg1 + b2 * sg2) : 0.0;
+                kd[0] += b0 * DB[0][0] + b1 * DB[3][0] + b2 * DB[5][0] + geo;
+                kd[1] += b0 * DB[0][1] + b1 * DB[3][1] + b2 * DB[5][1];
+                kd[2] += b0 * DB[0][2] + b1 * DB[3][2] + b2 * DB[5][2];
+                kd[3] += b1 * DB[1][1] + b0 * DB[3][1] + b2 * DB[4][1] + geo;
+                kd[4] += b1 * DB[1][2] + b0 * DB[3][2] + b2 * DB[4][2];
+                kd[5] += b2 * DB[2][2] + b1 * DB[4][2] + b0 * DB[5][2] + geo;
+            }
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                if (j < npart) {
+                    const int c = (b + j + 1) & 7;
+                    const double c0 = Gp[c * 3 + 0], c1 = Gp[c * 3 + 1], c2 = Gp[c * 3 + 2];
+                    const double geo = NL ? (c0 * sg0 + c1 * sg1 + c2 * sg2) : 0.0;
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) {
+                        ko[j][0 * 3 + k] += c0 * DB[0][k] + c1 * DB[3][k] + c2 * DB[5][k];
+                        ko[j][1 * 3 + k] += c1 * DB[1][k] + c0 * DB[3][k] + c2 * DB[4][k];
+                        ko[j][2 * 3 + k] += c2 * DB[2][k] + c1 * DB[4][k] + c0 * DB[5][k];
+                    }
+                    ko[j][0] += geo;
+                    ko[j][4] += geo;
+                    ko[j][8] += geo;
+                }
+            }
+        }
+    }
+    if (!active) return;
+
+    // ---------------- scatter -----------------
+    const int B = node;
+    const bool dfirst = (p.diag_first[e] >> b) & 1;
+    if (dfirst) {
+        p.Fint[3L * B + 0] = fe0;
+        p.Fint[3L * B + 1] = fe1;
+        p.Fint[3L * B + 2] = fe2;
+    } else {
+        p.Fint[3L * B + 0] += fe0;
+        p.Fint[3L * B + 1] += fe1;
+        p.Fint[3L * B + 2] += fe2;
+    }
+    if constexpr (WANT_K) {
+        KT* K = (KT*)p.K;
+        const int rsB = p.rowstart[B], nuB = p.nup[B];
+        {
+            const long r0 = rowptr(rsB, nuB, 0), r1 = rowptr(rsB, nuB, 1), r2 = rowptr(rsB, nuB, 2);
+            kstore(K, r0 + 0, kd[0], dfirst);
+            kstore(K, r0 + 1, kd[1], dfirst);
+            kstore(K, r0 + 2, kd[2], dfirst);
+            kstore(K, r1 + 0, kd[3], dfirst);
+            kstore(K, r1 + 1, kd[4], dfirst);
+            kstore(K, r2 + 0, kd[5], dfirst);
+        }
+        const uint32_t pw = p.pairs[(long)e * 8 + b];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            if (j < npart) {
+                const int c = (b + j + 1) & 7;
+                const int C = snode[le][c];
+                const uint32_t byte = (pw >> (8 * j)) & 0xffu;
+                const int rank = (int)(byte & 0x7fu);
+                const bool first = (byte >> 7) & 1u;
+                if (C < B) {
+                    // rows of C: K(3C+i, 3B+k) = K_cb(i,k)
+                    const int rsC = p.rowstart[C], nuC = p.nup[C];
+#pragma unroll
+                    for (int i = 0; i < 3; ++i) {
+                        const long base = rowptr(rsC, nuC, i) + (3 - i) + 3 * rank;
+#pragma unroll
+                        for (int k = 0; k < 3; ++k) kstore(K, base + k, ko[j][i * 3 + k], first);
+                    }
+                } else {
+                    // rows of B: K(3B+k, 3C+i) = K_cb(i,k)
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) {
+                        const long base = rowptr(rsB, nuB, k) + (3 - k) + 3 * rank;
+#pragma unroll
+                        for (int i = 0; i < 3; ++i) kstore(K, base + i, ko[j][i * 3 + k], first);
+                    }
+                }
+            }
+        }
+    }
+}
+
+}  // namespace ceos

@@ -43,13 +43,8 @@ def sym_from_upper(rowmap, col, val, n):
 def sym_upper_matvec(rowmap, col, val, x):
     """y = K x with K symmetric stored as its upper triangle (mkl_dcsrsymv 'U')."""
     n = len(rowmap) - 1
-    rows = np.repeat(np.arange(n), np.diff(rowmap))
-    cols = col.astype(np.int64) - 1
-    y = np.zeros(n)
-    np.add.at(y, rows, val * x[cols])
-    off = cols != rows
-    np.add.at(y, cols[off], val[off] * x[rows[off]])
-    return y
+    U = sp.csr_matrix((val, col.astype(np.int64) - 1, rowmap.astype(np.int64) - 1), shape=(n, n))
+    return U @ x + U.T @ x - U.diagonal() * x
 
 
 class BoundaryConditions:
