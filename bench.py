@@ -94,6 +94,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--precision", choices=["fp64", "fp32k"], default="fp64")
+    ap.add_argument("--slabs", type=int, default=0, help="launch schedule (0 = automatic)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build shards nothing across GPUs "
@@ -109,6 +110,7 @@ def main():
     prec = gpu.K_FP64 if args.precision == "fp64" else gpu.K_FP32_STORED
     mesh = gpu.CeosMesh(X, conn, precision=prec)
     mesh.set_material(1, gpu.MODEL_CNH, [MU, LAM])
+    mesh.set_schedule(args.slabs)
     ncolor, nnz = mesh.info()
     ne = len(conn)
     dU = hiprt.DeviceArray.from_numpy(U)
@@ -168,7 +170,8 @@ def main():
         "config": {"workload": "80^3 hex8 cube, interior nodes perturbed +-10% h, compressible "
                                "Neo-Hookean mu=1 K=100, F+K evaluation at the step-1 state",
                    "elements": ne, "dofs": 3 * len(X), "gauss_points": 8 * ne, "nnz_upper": nnz,
-                   "colors": ncolor, "tangent": args.precision, "parallelism": "single GPU"},
+                   "colors": ncolor, "slabs": args.slabs, "tangent": args.precision,
+                   "parallelism": "single GPU"},
         "roofline": roofline,
         "roofline_hbm": {"achieved": round(achieved_gbs, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                          "frac": round(achieved_gbs / PEAK_HBM_GBS, 4),

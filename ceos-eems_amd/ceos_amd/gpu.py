@@ -48,6 +48,7 @@ EXPORTS = [
     "ceos_failed_elements", "ceos_get_gp_stress", "ceos_get_state", "ceos_set_state",
     "ceos_homogenize", "ceos_state_save", "ceos_state_load", "ceos_set_fused",
     "ceos_evaluate_device", "ceos_update_mesh_device", "ceos_device_status", "ceos_mesh_info",
+    "ceos_set_schedule",
 ]
 
 _lib = None
@@ -87,6 +88,7 @@ def load_library(path=LIB_PATH):
     L.ceos_update_mesh_device.argtypes = [vp, vp, vp]
     L.ceos_device_status.argtypes = [vp, C.POINTER(CeosStatus)]
     L.ceos_mesh_info.argtypes = [vp, C.POINTER(i), C.POINTER(C.c_longlong)]
+    L.ceos_set_schedule.argtypes = [vp, i]
     _lib = L
     return L
 
@@ -137,6 +139,9 @@ class CeosMesh:
     def set_material(self, mat_id, model, props):
         p = np.ascontiguousarray(props, dtype=np.float64)
         _check(load_library().ceos_material_set(self._h, mat_id, model, p, len(p)), "ceos_material_set")
+
+    def set_schedule(self, slabs):
+        _check(load_library().ceos_set_schedule(self._h, slabs), "ceos_set_schedule")
 
     def info(self):
         nc, nnz = C.c_int(), C.c_longlong()

@@ -108,6 +108,8 @@ struct ceos_handle {
     std::vector<int> iperm;      // element -> slot
     std::vector<LaunchRange> ranges;
     bool has_j2 = false;
+    int slabs = 0;           // 0 = automatic
+    int nslabs_used = 1;
 
     // device
     double *dX = nullptr, *dx = nullptr, *dU = nullptr, *dFint = nullptr, *dK = nullptr;
@@ -223,6 +225,24 @@ static int finalize(ceos_handle* h) {
     for (size_t i = 0; i < h->mat_ids.size(); ++i)
         if (!h->mats[i].set) return CEOS_ERR_ARG;
     const int ne = h->ne;
+    // J2 state of a previous slot order, kept in element order across the rebuild
+    std::vector<double> keep_old, keep_new;
+    if (h->dst_old && !h->perm.empty()) {
+        const size_t n = (size_t)ne * 8 * kJ2State;
+        std::vector<double> a(n), b(n);
+        HIP_OK(hipStreamSynchronize(h->stream));
+        HIP_OK(hipMemcpy(a.data(), h->dst_old, sizeof(double) * n, hipMemcpyDeviceToHost));
+        HIP_OK(hipMemcpy(b.data(), h->dst_new, sizeof(double) * n, hipMemcpyDeviceToHost));
+        keep_old.resize(n);
+        keep_new.resize(n);
+        for (int sl = 0; sl < ne; ++sl) {
+            const int e = h->perm[sl];
+            for (int q = 0; q < 8 * kJ2State; ++q) {
+                keep_old[(size_t)q * ne + e] = a[(size_t)q * ne + sl];
+                keep_new[(size_t)q * ne + e] = b[(size_t)q * ne + sl];
+            }
+        }
+    }
     // group by (colour, model), id order inside a group
     std::vector<int> models;
     for (size_t i = 0; i < h->mat_ids.size(); ++i) models.push_back(h->mats[i].model);
@@ -238,15 +258,22 @@ static int finalize(ceos_handle* h) {
             const int mi = (int)(std::lower_bound(models.begin(), models.end(), m) - models.begin());
             bucket[(size_t)h->color[e] * models.size() + mi].push_back(e);
         }
-        for (int c = 0; c < h->ncolor; ++c)
-            for (size_t mi = 0; mi < models.size(); ++mi) {
-                auto& v = bucket[(size_t)c * models.size() + mi];
-                if (v.empty()) continue;
-                LaunchRange r{(int)h->perm.size(), 0, models[mi]};
-                h->perm.insert(h->perm.end(), v.begin(), v.end());
-                r.end = (int)h->perm.size();
-                h->ranges.push_back(r);
-            }
+        // execution order: for each chunk of the id-ordered colour lists, every colour (and
+        // model) in turn; one chunk (the default) is plain colour order
+        int S = h->slabs;
+        if (S <= 0) S = 1;
+        h->nslabs_used = S;
+        for (int sl = 0; sl < S; ++sl)
+            for (int c = 0; c < h->ncolor; ++c)
+                for (size_t mi = 0; mi < models.size(); ++mi) {
+                    auto& v = bucket[(size_t)c * models.size() + mi];
+                    const size_t b0 = v.size() * sl / S, b1 = v.size() * (sl + 1) / S;
+                    if (b1 <= b0) continue;
+                    LaunchRange r{(int)h->perm.size(), 0, models[mi]};
+                    h->perm.insert(h->perm.end(), v.begin() + b0, v.begin() + b1);
+                    r.end = (int)h->perm.size();
+                    h->ranges.push_back(r);
+                }
     }
     h->iperm.assign(ne, 0);
     for (int s = 0; s < ne; ++s) h->iperm[h->perm[s]] = s;
@@ -299,10 +326,27 @@ static int finalize(ceos_handle* h) {
     HIP_OK(hipMemsetAsync(h->delem_err, 0, ne, h->stream));
     HIP_OK(hipMemsetAsync(h->dsig, 0, sizeof(double) * (size_t)ne * 48, h->stream));
     if (h->has_j2) {
-        HIP_OK(dalloc(h->dst_old, (size_t)ne * 8 * kJ2State));
-        HIP_OK(dalloc(h->dst_new, (size_t)ne * 8 * kJ2State));
-        HIP_OK(hipMemsetAsync(h->dst_old, 0, sizeof(double) * (size_t)ne * 8 * kJ2State, h->stream));
-        HIP_OK(hipMemsetAsync(h->dst_new, 0, sizeof(double) * (size_t)ne * 8 * kJ2State, h->stream));
+        const size_t n = (size_t)ne * 8 * kJ2State;
+        HIP_OK(dalloc(h->dst_old, n));
+        HIP_OK(dalloc(h->dst_new, n));
+        if (!keep_old.empty()) {
+            std::vector<double> a(n), b(n);
+            for (int sl = 0; sl < ne; ++sl) {
+                const int e = h->perm[sl];
+                for (int q = 0; q < 8 * kJ2State; ++q) {
+                    a[(size_t)q * ne + sl] = keep_old[(size_t)q * ne + e];
+                    b[(size_t)q * ne + sl] = keep_new[(size_t)q * ne + e];
+                }
+            }
+            HIP_OK(hipMemcpy(h->dst_old, a.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+            HIP_OK(hipMemcpy(h->dst_new, b.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+        } else {
+            HIP_OK(hipMemsetAsync(h->dst_old, 0, sizeof(double) * n, h->stream));
+            HIP_OK(hipMemsetAsync(h->dst_new, 0, sizeof(double) * n, h->stream));
+        }
+    } else {
+        dfree(h->dst_old);
+        dfree(h->dst_new);
     }
     HIP_OK(hipStreamSynchronize(h->stream));
     h->finalized = true;
@@ -504,6 +548,14 @@ int ceos_set_precision(ceos_handle* h, int mode) {
 int ceos_set_fused(ceos_handle* h, int fused) {
     if (!h) return CEOS_ERR_ARG;
     h->fused = fused != 0;
+    return CEOS_OK;
+}
+
+int ceos_set_schedule(ceos_handle* h, int slabs) {
+    if (!h || slabs < 0 || slabs > 4096) return CEOS_ERR_ARG;
+    h->slabs = slabs;
+    h->finalized = false;
+    h->f_valid = h->k_valid = false;
     return CEOS_OK;
 }
 

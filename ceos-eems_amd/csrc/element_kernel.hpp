@@ -57,12 +57,6 @@ struct ElemParams {
     int* err_count;
 };
 
-template <typename KT>
-__device__ __forceinline__ void kstore(KT* K, long s, double v, bool first) {
-    if (first) K[s] = (KT)v;
-    else K[s] += (KT)v;
-}
-
 __device__ __forceinline__ long rowptr(int rs, int nu, int i) {
     return (long)rs + (i == 0 ? 0 : (i == 1 ? 3 + 3 * nu : 5 + 6 * nu));
 }
@@ -252,58 +246,72 @@ __global__ __launch_bounds__(128) void k_element(ElemParams p, int e_begin, int 
     if (!active) return;
 
     // ---------------- scatter -----------------
+    // Every read of K / Fint is issued before any write: the slots of one lane are distinct,
+    // but the compiler cannot prove it, so interleaved read-add-write sequences would wait on
+    // one memory round trip each.
     const int B = node;
     const bool dfirst = (p.diag_first[e] >> b) & 1;
-    if (dfirst) {
-        p.Fint[3L * B + 0] = fe0;
-        p.Fint[3L * B + 1] = fe1;
-        p.Fint[3L * B + 2] = fe2;
-    } else {
-        p.Fint[3L * B + 0] += fe0;
-        p.Fint[3L * B + 1] += fe1;
-        p.Fint[3L * B + 2] += fe2;
+    {
+        double f0 = 0.0, f1 = 0.0, f2 = 0.0;
+        if (!dfirst) {
+            f0 = p.Fint[3L * B + 0];
+            f1 = p.Fint[3L * B + 1];
+            f2 = p.Fint[3L * B + 2];
+        }
+        p.Fint[3L * B + 0] = f0 + fe0;
+        p.Fint[3L * B + 1] = f1 + fe1;
+        p.Fint[3L * B + 2] = f2 + fe2;
     }
     if constexpr (WANT_K) {
         KT* K = (KT*)p.K;
         const int rsB = p.rowstart[B], nuB = p.nup[B];
-        {
-            const long r0 = rowptr(rsB, nuB, 0), r1 = rowptr(rsB, nuB, 1), r2 = rowptr(rsB, nuB, 2);
-            kstore(K, r0 + 0, kd[0], dfirst);
-            kstore(K, r0 + 1, kd[1], dfirst);
-            kstore(K, r0 + 2, kd[2], dfirst);
-            kstore(K, r1 + 0, kd[3], dfirst);
-            kstore(K, r1 + 1, kd[4], dfirst);
-            kstore(K, r2 + 0, kd[5], dfirst);
-        }
         const uint32_t pw = p.pairs[(long)e * 8 + b];
+        int slot[42];
+        double val[42];
+        bool rd[42];   // slot needs the old value (not the first writer)
+        bool wr[42];   // slot is written by this lane
+        {
+            const int r0 = (int)rowptr(rsB, nuB, 0), r1 = (int)rowptr(rsB, nuB, 1), r2 = (int)rowptr(rsB, nuB, 2);
+            const int ds[6] = {r0, r0 + 1, r0 + 2, r1, r1 + 1, r2};
+#pragma unroll
+            for (int q = 0; q < 6; ++q) {
+                slot[q] = ds[q];
+                val[q] = kd[q];
+                rd[q] = !dfirst;
+                wr[q] = true;
+            }
+        }
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-            if (j < npart) {
-                const int c = (b + j + 1) & 7;
-                const int C = snode[le][c];
-                const uint32_t byte = (pw >> (8 * j)) & 0xffu;
-                const int rank = (int)(byte & 0x7fu);
-                const bool first = (byte >> 7) & 1u;
-                if (C < B) {
-                    // rows of C: K(3C+i, 3B+k) = K_cb(i,k)
-                    const int rsC = p.rowstart[C], nuC = p.nup[C];
+            const bool valid = j < npart;
+            const int c = (b + j + 1) & 7;
+            const int C = snode[le][c];
+            const uint32_t byte = (pw >> (8 * j)) & 0xffu;
+            const int rank = (int)(byte & 0x7fu);
+            const bool first = (byte >> 7) & 1u;
+            const bool lowC = C < B;
+            const int rsC = lowC ? p.rowstart[C] : rsB;
+            const int nuC = lowC ? p.nup[C] : nuB;
+            // C < B: rows of C, K(3C+i, 3B+k) = K_cb(i,k); else rows of B, K(3B+k, 3C+i) = K_cb(i,k)
 #pragma unroll
-                    for (int i = 0; i < 3; ++i) {
-                        const long base = rowptr(rsC, nuC, i) + (3 - i) + 3 * rank;
+            for (int r = 0; r < 3; ++r) {
+                const int base = (int)rowptr(rsC, nuC, r) + (3 - r) + 3 * rank;
 #pragma unroll
-                        for (int k = 0; k < 3; ++k) kstore(K, base + k, ko[j][i * 3 + k], first);
-                    }
-                } else {
-                    // rows of B: K(3B+k, 3C+i) = K_cb(i,k)
-#pragma unroll
-                    for (int k = 0; k < 3; ++k) {
-                        const long base = rowptr(rsB, nuB, k) + (3 - k) + 3 * rank;
-#pragma unroll
-                        for (int i = 0; i < 3; ++i) kstore(K, base + i, ko[j][i * 3 + k], first);
-                    }
+                for (int t = 0; t < 3; ++t) {
+                    const int q = 6 + 9 * j + 3 * r + t;
+                    slot[q] = base + t;
+                    val[q] = lowC ? ko[j][r * 3 + t] : ko[j][t * 3 + r];
+                    rd[q] = valid && !first;
+                    wr[q] = valid;
                 }
             }
         }
+        double old[42];
+#pragma unroll
+        for (int q = 0; q < 42; ++q) old[q] = rd[q] ? (double)K[slot[q]] : 0.0;
+#pragma unroll
+        for (int q = 0; q < 42; ++q)
+            if (wr[q]) K[slot[q]] = (KT)(old[q] + val[q]);
     }
 }
 

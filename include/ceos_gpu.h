@@ -129,6 +129,12 @@ int ceos_evaluate_device(ceos_handle *h, double time, const double *dU, double *
 int ceos_update_mesh_device(ceos_handle *h, const double *dU, void *stream);
 int ceos_device_status(ceos_handle *h, ceos_status *st);
 
+/* Execution schedule of the element launches: the id-ordered element list of every colour is
+ * cut into `slabs` chunks and chunk s of all colours runs before chunk s+1 (0 or 1 = plain
+ * colour order, the default).  Changing it rebuilds the per-element tables at the next
+ * evaluation; results agree to rounding for every schedule. */
+int ceos_set_schedule(ceos_handle *h, int slabs);
+
 /* Mesh facts after setup: number of element colours and the nnz of the upper pattern. */
 int ceos_mesh_info(ceos_handle *h, int *n_colors, long long *nnz_upper);
 

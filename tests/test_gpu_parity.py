@@ -352,3 +352,31 @@ def test_full_size_c2_properties():
         t[k::3] = 1.0
         y = fd.sym_upper_matvec(rm, col, K, t)
         assert np.max(np.abs(y)) < 1e-10 * np.max(np.abs(K))
+
+
+def test_schedule_does_not_change_results():
+    """Any launch schedule (slab count) gives the oracle's Fint and K to rounding, and a J2
+    state committed under one schedule survives a switch to another."""
+    X, conn = meshgen.cube(8, perturb=0.1, seed=9)
+    U = np.random.default_rng(9).normal(scale=0.02, size=3 * len(X))
+    o = ol.OracleMesh(X, conn)
+    o.set_material(1, ol.MODEL_CNH, [1.0, 99.0])
+    o.update_mesh(U)
+    assert o.solve_constitutive(0.0, U) == 0
+    fo, ko = o.internal_force()[0], o.tangent()
+    for slabs in (1, 3, 7):
+        g = gpu.CeosMesh(X, conn)
+        g.set_material(1, ol.MODEL_CNH, [1.0, 99.0])
+        g.set_schedule(slabs)
+        g.update_mesh(U)
+        assert g.solve_constitutive(0.0, U) == 0
+        assert rel(g.internal_force()[0], fo) < RTOL and rel(g.tangent(), ko) < RTOL
+    g = gpu.CeosMesh(X, conn, nl_analysis=False)
+    g.set_material(1, ol.MODEL_J2, [210000.0, 0.3, 250.0, 1000.0])
+    Ue = 0.05 * U
+    assert g.solve_constitutive(0.0, Ue) == 0
+    g.switch_converged_state()
+    s0 = g.state(0)
+    assert s0[..., 6].max() > 0
+    g.set_schedule(5)
+    assert np.array_equal(g.state(0), s0)
