@@ -22,7 +22,8 @@ PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB_PATH = os.path.join(PKG, "lib", "libceos_gpu.so")
 
 OK, ERR_DETJ, ERR_LOCAL_NR, ERR_SLOT, ERR_ARG, ERR_HIP = 0, -1, -2, -3, -4, -5
-MODEL_GHL, MODEL_J2, MODEL_NH, MODEL_SVK, MODEL_CNH = 1, 2, 3, 5, 7
+MODEL_GHL, MODEL_J2, MODEL_NH, MODEL_SVK, MODEL_CNH, MODEL_VISCO = 1, 2, 3, 5, 7, 12
+STATE_STRIDE = 20
 K_FP64, K_FP32_STORED = 0, 1
 
 _dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
@@ -203,9 +204,9 @@ class CeosMesh:
         return out.reshape(self.ne, 8, 6)
 
     def state(self, which=1):
-        out = np.zeros(self.ne * 64)
+        out = np.zeros(self.ne * 8 * STATE_STRIDE)
         _check(load_library().ceos_get_state(self._h, which, out), "ceos_get_state")
-        return out.reshape(self.ne, 8, 8)
+        return out.reshape(self.ne, 8, STATE_STRIDE)
 
     def set_state(self, which, arr):
         _check(load_library().ceos_set_state(self._h, which,

@@ -70,12 +70,13 @@ void set_status(ceos_status* st, int code, int nfail, const char* msg) {
 }
 
 bool model_supported(int m) {
-    return m == kGHL || m == kJ2 || m == kNH || m == kSVK || m == kCNH;
+    return m == kGHL || m == kJ2 || m == kNH || m == kSVK || m == kCNH || m == kVISCO;
 }
 
 int model_nprops(int m) {
     switch (m) {
         case kJ2: return 4;
+        case kVISCO: return 6;
         default: return 2;
     }
 }
@@ -107,7 +108,7 @@ struct ceos_handle {
     std::vector<int> perm;       // slot -> element
     std::vector<int> iperm;      // element -> slot
     std::vector<LaunchRange> ranges;
-    bool has_j2 = false;
+    bool has_state = false;
     int slabs = 0;           // 0 = automatic
     int nslabs_used = 1;
 
@@ -228,7 +229,7 @@ static int finalize(ceos_handle* h) {
     // J2 state of a previous slot order, kept in element order across the rebuild
     std::vector<double> keep_old, keep_new;
     if (h->dst_old && !h->perm.empty()) {
-        const size_t n = (size_t)ne * 8 * kJ2State;
+        const size_t n = (size_t)ne * 8 * kStateStride;
         std::vector<double> a(n), b(n);
         HIP_OK(hipStreamSynchronize(h->stream));
         HIP_OK(hipMemcpy(a.data(), h->dst_old, sizeof(double) * n, hipMemcpyDeviceToHost));
@@ -237,7 +238,7 @@ static int finalize(ceos_handle* h) {
         keep_new.resize(n);
         for (int sl = 0; sl < ne; ++sl) {
             const int e = h->perm[sl];
-            for (int q = 0; q < 8 * kJ2State; ++q) {
+            for (int q = 0; q < 8 * kStateStride; ++q) {
                 keep_old[(size_t)q * ne + e] = a[(size_t)q * ne + sl];
                 keep_new[(size_t)q * ne + e] = b[(size_t)q * ne + sl];
             }
@@ -248,7 +249,8 @@ static int finalize(ceos_handle* h) {
     for (size_t i = 0; i < h->mat_ids.size(); ++i) models.push_back(h->mats[i].model);
     std::sort(models.begin(), models.end());
     models.erase(std::unique(models.begin(), models.end()), models.end());
-    h->has_j2 = std::find(models.begin(), models.end(), (int)kJ2) != models.end();
+    h->has_state = std::find(models.begin(), models.end(), (int)kJ2) != models.end() ||
+                   std::find(models.begin(), models.end(), (int)kVISCO) != models.end();
     h->perm.clear();
     h->ranges.clear();
     {
@@ -325,15 +327,15 @@ static int finalize(ceos_handle* h) {
     HIP_OK(hipMemcpyAsync(h->ddiag_first, dfirst.data(), ne, hipMemcpyHostToDevice, h->stream));
     HIP_OK(hipMemsetAsync(h->delem_err, 0, ne, h->stream));
     HIP_OK(hipMemsetAsync(h->dsig, 0, sizeof(double) * (size_t)ne * 48, h->stream));
-    if (h->has_j2) {
-        const size_t n = (size_t)ne * 8 * kJ2State;
+    if (h->has_state) {
+        const size_t n = (size_t)ne * 8 * kStateStride;
         HIP_OK(dalloc(h->dst_old, n));
         HIP_OK(dalloc(h->dst_new, n));
         if (!keep_old.empty()) {
             std::vector<double> a(n), b(n);
             for (int sl = 0; sl < ne; ++sl) {
                 const int e = h->perm[sl];
-                for (int q = 0; q < 8 * kJ2State; ++q) {
+                for (int q = 0; q < 8 * kStateStride; ++q) {
                     a[(size_t)q * ne + sl] = keep_old[(size_t)q * ne + e];
                     b[(size_t)q * ne + sl] = keep_new[(size_t)q * ne + e];
                 }
@@ -341,8 +343,15 @@ static int finalize(ceos_handle* h) {
             HIP_OK(hipMemcpy(h->dst_old, a.data(), sizeof(double) * n, hipMemcpyHostToDevice));
             HIP_OK(hipMemcpy(h->dst_new, b.data(), sizeof(double) * n, hipMemcpyHostToDevice));
         } else {
-            HIP_OK(hipMemsetAsync(h->dst_old, 0, sizeof(double) * n, h->stream));
-            HIP_OK(hipMemsetAsync(h->dst_new, 0, sizeof(double) * n, h->stream));
+            // initial state: zero, viscous deformation gradient Fv = I
+            // (ModJ2Plasticity.f90:151-154, ModViscoelasticMatrix.f90:120-133)
+            std::vector<double> a(n, 0.0);
+            for (int sl = 0; sl < ne; ++sl)
+                if (h->mats[h->mat_of[h->perm[sl]]].model == kVISCO)
+                    for (int g = 0; g < 8; ++g)
+                        for (int v : {0, 4, 8}) a[((size_t)v * 8 + g) * ne + sl] = 1.0;
+            HIP_OK(hipMemcpy(h->dst_old, a.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+            HIP_OK(hipMemcpy(h->dst_new, a.data(), sizeof(double) * n, hipMemcpyHostToDevice));
         }
     } else {
         dfree(h->dst_old);
@@ -382,6 +391,7 @@ static void launch_range(const ElemParams& p, const LaunchRange& r, bool nl, boo
         case kNH: launch_m<kNH, KT>(p, r.begin, r.end, nl, wk, s); break;
         case kSVK: launch_m<kSVK, KT>(p, r.begin, r.end, nl, wk, s); break;
         case kCNH: launch_m<kCNH, KT>(p, r.begin, r.end, nl, wk, s); break;
+        case kVISCO: launch_m<kVISCO, KT>(p, r.begin, r.end, nl, wk, s); break;
         default: break;
     }
 }
@@ -409,6 +419,7 @@ static int evaluate(ceos_handle* h, const double* dU, double* dFint, void* dK, h
     p.st_new = h->dst_new;
     p.elem_err = h->delem_err;
     p.err_count = h->derr;
+    p.time = h->time;
     hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(1), 0, s, h->derr);
     const bool wk = dK != nullptr;
     for (const auto& r : h->ranges) {
@@ -420,17 +431,21 @@ static int evaluate(ceos_handle* h, const double* dU, double* dFint, void* dK, h
 }
 
 static int read_status(ceos_handle* h, ceos_status* st) {
-    int nerr = 0;
-    HIP_OK(hipMemcpyAsync(&nerr, h->derr, sizeof(int), hipMemcpyDeviceToHost, h->stream));
+    int nerr[2] = {0, 0};
+    HIP_OK(hipMemcpyAsync(nerr, h->derr, sizeof(nerr), hipMemcpyDeviceToHost, h->stream));
     HIP_OK(hipStreamSynchronize(h->stream));
-    if (nerr) {
+    if (nerr[0] || nerr[1]) {
         std::vector<uint8_t> flags(h->ne);
         HIP_OK(hipMemcpy(flags.data(), h->delem_err, h->ne, hipMemcpyDeviceToHost));
         int n = 0;
         for (int s = 0; s < h->ne; ++s) n += flags[s] != 0;
-        set_status(st, CEOS_ERR_DETJ, n,
-                   "ElementVolume / DeformationGradient: Determinant of the Jacobian Matrix <= 0.0d0");
-        return CEOS_ERR_DETJ;
+        if (nerr[0]) {
+            set_status(st, CEOS_ERR_DETJ, n,
+                       "ElementVolume / DeformationGradient: Determinant of the Jacobian Matrix <= 0.0d0");
+            return CEOS_ERR_DETJ;
+        }
+        set_status(st, CEOS_ERR_LOCAL_NR, n, "Max Iteration in Local Newton-Raphson - Viscoelastic Matrix Model");
+        return CEOS_ERR_LOCAL_NR;
     }
     set_status(st, CEOS_OK, 0, "");
     return CEOS_OK;
@@ -492,7 +507,7 @@ int ceos_mesh_create(int nn, const double* X, int ne, int npe, const int* conn, 
     auto fail = [&](int code) { free_device(h); (void)hipStreamDestroy(h->stream); delete h; return code; };
     if (dalloc(h->dX, 3L * nn) || dalloc(h->dx, 3L * nn) || dalloc(h->dU, 3L * nn) ||
         dalloc(h->dFint, 3L * nn) || dalloc(h->drowstart, nn) || dalloc(h->dnup, nn) ||
-        dalloc(h->dprops, (size_t)kMaxMat * kMaxProps) || dalloc(h->derr, 1))
+        dalloc(h->dprops, (size_t)kMaxMat * kMaxProps) || dalloc(h->derr, 2))
         return fail(CEOS_ERR_HIP);
     if (hipMemcpy(h->dX, X, sizeof(double) * 3L * nn, hipMemcpyHostToDevice) ||
         hipMemcpy(h->dx, X, sizeof(double) * 3L * nn, hipMemcpyHostToDevice) ||
@@ -500,7 +515,7 @@ int ceos_mesh_create(int nn, const double* X, int ne, int npe, const int* conn, 
         hipMemcpy(h->drowstart, h->rowstart.data(), sizeof(int) * nn, hipMemcpyHostToDevice) ||
         hipMemcpy(h->dnup, h->nup.data(), sizeof(int) * nn, hipMemcpyHostToDevice) ||
         hipMemset(h->dprops, 0, sizeof(double) * kMaxMat * kMaxProps) ||
-        hipMemset(h->derr, 0, sizeof(int)))
+        hipMemset(h->derr, 0, 2 * sizeof(int)))
         return fail(CEOS_ERR_HIP);
     *out = h;
     return CEOS_OK;
@@ -693,8 +708,8 @@ int ceos_update_mesh(ceos_handle* h, const double* U) {
 
 int ceos_switch_converged_state(ceos_handle* h) {
     if (!h) return CEOS_ERR_ARG;
-    if (h->finalized && h->has_j2) {
-        HIP_OK(hipMemcpyAsync(h->dst_old, h->dst_new, sizeof(double) * (size_t)h->ne * 8 * 7,
+    if (h->finalized && h->has_state) {
+        HIP_OK(hipMemcpyAsync(h->dst_old, h->dst_new, sizeof(double) * (size_t)h->ne * 8 * kStateStride,
                               hipMemcpyDeviceToDevice, h->stream));
         HIP_OK(hipStreamSynchronize(h->stream));
     }
@@ -736,17 +751,17 @@ int ceos_get_state(ceos_handle* h, int which, double* state) {
     int rc = finalize(h);
     if (rc) return rc;
     const int ne = h->ne;
-    if (!h->has_j2) {
-        memset(state, 0, sizeof(double) * (size_t)ne * 8 * kJ2State);
+    if (!h->has_state) {
+        memset(state, 0, sizeof(double) * (size_t)ne * 8 * kStateStride);
         return CEOS_OK;
     }
-    std::vector<double> d((size_t)ne * 8 * kJ2State);
+    std::vector<double> d((size_t)ne * 8 * kStateStride);
     HIP_OK(hipMemcpy(d.data(), which ? h->dst_new : h->dst_old, sizeof(double) * d.size(), hipMemcpyDeviceToHost));
     for (int s = 0; s < ne; ++s) {
         const int e = h->perm[s];
         for (int g = 0; g < 8; ++g)
-            for (int v = 0; v < kJ2State; ++v)
-                state[((size_t)e * 8 + g) * kJ2State + v] = d[((size_t)v * 8 + g) * ne + s];
+            for (int v = 0; v < kStateStride; ++v)
+                state[((size_t)e * 8 + g) * kStateStride + v] = d[((size_t)v * 8 + g) * ne + s];
     }
     return CEOS_OK;
 }
@@ -756,14 +771,14 @@ int ceos_set_state(ceos_handle* h, int which, const double* state) {
     HIP_OK(hipStreamSynchronize(h->stream));
     int rc = finalize(h);
     if (rc) return rc;
-    if (!h->has_j2) return CEOS_OK;
+    if (!h->has_state) return CEOS_OK;
     const int ne = h->ne;
-    std::vector<double> d((size_t)ne * 8 * kJ2State);
+    std::vector<double> d((size_t)ne * 8 * kStateStride);
     for (int s = 0; s < ne; ++s) {
         const int e = h->perm[s];
         for (int g = 0; g < 8; ++g)
-            for (int v = 0; v < kJ2State; ++v)
-                d[((size_t)v * 8 + g) * ne + s] = state[((size_t)e * 8 + g) * kJ2State + v];
+            for (int v = 0; v < kStateStride; ++v)
+                d[((size_t)v * 8 + g) * ne + s] = state[((size_t)e * 8 + g) * kStateStride + v];
     }
     HIP_OK(hipMemcpy(which ? h->dst_new : h->dst_old, d.data(), sizeof(double) * d.size(), hipMemcpyHostToDevice));
     h->f_valid = h->k_valid = false;
@@ -800,12 +815,12 @@ int ceos_state_save(ceos_handle* h, const char* path) {
     if (rc) return rc;
     FILE* f = fopen(path, "wb");
     if (!f) return CEOS_ERR_ARG;
-    const int hdr[4] = {0x43454f53, h->nn, h->ne, h->has_j2 ? 1 : 0};
+    const int hdr[4] = {0x43454f53, h->nn, h->ne, h->has_state ? 1 : 0};
     std::vector<double> x(3L * h->nn);
     bool ok = hipMemcpy(x.data(), h->dx, sizeof(double) * x.size(), hipMemcpyDeviceToHost) == hipSuccess;
     ok = ok && fwrite(hdr, sizeof(hdr), 1, f) == 1 && fwrite(x.data(), sizeof(double), x.size(), f) == x.size();
-    if (ok && h->has_j2) {
-        std::vector<double> s((size_t)h->ne * 8 * kJ2State);
+    if (ok && h->has_state) {
+        std::vector<double> s((size_t)h->ne * 8 * kStateStride);
         for (int which = 0; which < 2 && ok; ++which) {
             ok = ceos_get_state(h, which, s.data()) == CEOS_OK &&
                  fwrite(s.data(), sizeof(double), s.size(), f) == s.size();
@@ -823,12 +838,12 @@ int ceos_state_load(ceos_handle* h, const char* path) {
     if (!f) return CEOS_ERR_ARG;
     int hdr[4];
     bool ok = fread(hdr, sizeof(hdr), 1, f) == 1 && hdr[0] == 0x43454f53 && hdr[1] == h->nn && hdr[2] == h->ne &&
-              hdr[3] == (h->has_j2 ? 1 : 0);
+              hdr[3] == (h->has_state ? 1 : 0);
     std::vector<double> x(3L * h->nn);
     ok = ok && fread(x.data(), sizeof(double), x.size(), f) == x.size();
     if (ok) ok = hipMemcpy(h->dx, x.data(), sizeof(double) * x.size(), hipMemcpyHostToDevice) == hipSuccess;
-    if (ok && h->has_j2) {
-        std::vector<double> s((size_t)h->ne * 8 * kJ2State);
+    if (ok && h->has_state) {
+        std::vector<double> s((size_t)h->ne * 8 * kStateStride);
         for (int which = 0; which < 2 && ok; ++which)
             ok = fread(s.data(), sizeof(double), s.size(), f) == s.size() && ceos_set_state(h, which, s.data()) == CEOS_OK;
     }

@@ -27,6 +27,7 @@
 #include <stdint.h>
 
 #include "device_math.hpp"
+#include "visco.hpp"
 
 namespace ceos {
 
@@ -35,6 +36,7 @@ constexpr int kLdsPerElem = 408;      // doubles: gradN 8x8x3 + sigma 8x6 + D 8x
 constexpr int kOffSig = 192;
 constexpr int kOffD = 240;
 constexpr int kJ2State = 8;           // [0..5] plastic strain, [6] acc. plastic strain, [7] flag
+constexpr int kStateStride = 20;      // state rows per Gauss point in the device arrays (max of models)
 
 struct ElemParams {
     int ne;                        // elements (device slot order)
@@ -51,10 +53,11 @@ struct ElemParams {
     double* Fint;                  // [3 nn]
     void* K;                       // [nnz] double or float
     double* sig;                   // [6][8][ne]
-    const double* st_old;          // [kJ2State][8][ne]
-    double* st_new;                // [kJ2State][8][ne]
-    uint8_t* elem_err;             // [ne]
-    int* err_count;
+    const double* st_old;          // [nstate][8][ne] committed state
+    double* st_new;                // [nstate][8][ne] state of the last update
+    uint8_t* elem_err;             // [ne]: bit 0 detJ <= 1e-13, bit 1 local Newton failed
+    int* err_count;                // [0] elements with detJ failures, [1] local Newton failures
+    double time;                   // analysis time of the evaluation (viscous time step)
 };
 
 __device__ __forceinline__ long rowptr(int rs, int nu, int i) {
@@ -62,12 +65,20 @@ __device__ __forceinline__ long rowptr(int rs, int nu, int i) {
 }
 
 template <int MODEL>
-__device__ __forceinline__ bool material(const double* p, const double F[3][3], const double old[7],
-                                         double s[6], double D[21], double nw[7]) {
+constexpr int model_nstate() {
+    return MODEL == kJ2 ? 7 : (MODEL == kVISCO ? kViscoState : 1);
+}
+
+// returns the J2 PlasticState flag, or for the viscoelastic model whether its local Newton
+// converged; the other models return false
+template <int MODEL>
+__device__ __forceinline__ bool material(const double* p, const double F[3][3], double time, const double* old,
+                                         double s[6], double D[21], double* nw) {
     if constexpr (MODEL == kGHL) { mat_ghl(p, F, s, D); return false; }
     else if constexpr (MODEL == kCNH) { mat_cnh(p, F, s, D); return false; }
     else if constexpr (MODEL == kNH) { mat_nh(p, F, s, D); return false; }
     else if constexpr (MODEL == kSVK) { mat_svk(p, F, s, D); return false; }
+    else if constexpr (MODEL == kVISCO) { return mat_visco(p, F, time, old, s, D, nw); }
     else { return mat_j2(p, F, old, s, D, nw); }
 }
 
@@ -104,6 +115,7 @@ __global__ __launch_bounds__(128) void k_element(ElemParams p, int e_begin, int 
     double s[6], D[21];
     double wdet;
     bool bad;
+    bool nrfail = false;
     {
         double dN[8][3];
         hex8_dshape_gp(g, dN);
@@ -132,34 +144,41 @@ __global__ __launch_bounds__(128) void k_element(ElemParams p, int e_begin, int 
         bad = active && !(detx > kDetTol && detX > kDetTol);
         wdet = detx;  // Gauss weight 1 (ModElementHexa8.f90:326)
 
-        double old[7] = {0, 0, 0, 0, 0, 0, 0}, nw[7];
+        constexpr int NS = model_nstate<MODEL>();
+        double old[NS], nw[NS];
+#pragma unroll
+        for (int v = 0; v < NS; ++v) old[v] = 0.0;
+        if constexpr (MODEL == kVISCO) { old[0] = old[4] = old[8] = 1.0; }
         const int mi = active ? p.mat[e] : 0;
         const double* pr = p.props + 8 * mi;
-        if constexpr (MODEL == kJ2) {
+        if constexpr (MODEL == kJ2 || MODEL == kVISCO) {
             if (active) {
 #pragma unroll
-                for (int v = 0; v < 7; ++v) old[v] = p.st_old[((long)v * 8 + g) * p.ne + e];
+                for (int v = 0; v < NS; ++v) old[v] = p.st_old[((long)v * 8 + g) * p.ne + e];
             }
         }
-        const bool plastic = material<MODEL>(pr, F, old, s, D, nw);
+        const bool flag = material<MODEL>(pr, F, p.time, old, s, D, nw);
         if (active) {
 #pragma unroll
             for (int k = 0; k < 6; ++k) p.sig[((long)k * 8 + g) * p.ne + e] = s[k];
-            if constexpr (MODEL == kJ2) {
+            if constexpr (MODEL == kJ2 || MODEL == kVISCO) {
 #pragma unroll
-                for (int v = 0; v < 7; ++v) p.st_new[((long)v * 8 + g) * p.ne + e] = nw[v];
-                p.st_new[((long)7 * 8 + g) * p.ne + e] = plastic ? 1.0 : 0.0;
+                for (int v = 0; v < NS; ++v) p.st_new[((long)v * 8 + g) * p.ne + e] = nw[v];
             }
+            if constexpr (MODEL == kJ2) p.st_new[((long)7 * 8 + g) * p.ne + e] = flag ? 1.0 : 0.0;
         }
-        (void)plastic;
+        if constexpr (MODEL == kVISCO) nrfail = active && !flag;
     }
     {
-        // per-element OR of the 8 Gauss-point detJ checks (8 lanes of one element)
+        // per-element OR of the 8 Gauss-point checks (8 lanes of one element)
         const unsigned long long m = __ballot(bad ? 1 : 0);
+        const unsigned long long mn = __ballot(nrfail ? 1 : 0);
         const unsigned bits = (unsigned)(m >> (threadIdx.x & 56)) & 0xffu;  // this element's 8 lanes
+        const unsigned nbits = (unsigned)(mn >> (threadIdx.x & 56)) & 0xffu;
         if (active && lane == 0) {
-            p.elem_err[e] = bits ? 1 : 0;
+            p.elem_err[e] = (bits ? 1 : 0) | (nbits ? 2 : 0);
             if (bits) atomicAdd(p.err_count, 1);
+            if (nbits) atomicAdd(p.err_count + 1, 1);
         }
     }
     __syncthreads();  // node data no longer needed: reuse the LDS slice for GP data

@@ -85,6 +85,6 @@ __global__ void k_add(long n, const double* __restrict__ a, const double* __rest
     if (i < n) c[i] = a[i] + b[i];
 }
 
-__global__ void k_zero_int(int* p) { *p = 0; }
+__global__ void k_zero_int(int* p) { p[0] = 0; p[1] = 0; }
 
 }  // namespace ceos

@@ -32,6 +32,13 @@ extern "C" {
 #define CEOS_MODEL_NEO_HOOKEAN 3            /* props: C10, bulk modulus */
 #define CEOS_MODEL_ST_VENANT_KIRCHHOFF 5    /* props: E, nu */
 #define CEOS_MODEL_COMPRESSIBLE_NEO_HOOKEAN 7 /* props: mu, lambda */
+#define CEOS_MODEL_VISCOELASTIC_MATRIX 12   /* props: K_inf, Mu_inf, Lambda_inf, K_e, Mu_e, Ni_v */
+
+/* Gauss-point state rows exchanged by ceos_get_state / ceos_set_state (per Gauss point):
+ *   J2:   [0..5] plastic strain (engineering shear), [6] accumulated plastic strain,
+ *         [7] PlasticState flag of the last update
+ *   visco: [0..8] Fv (column-major), [9..17] dv (column-major), [18] gamma, [19] time */
+#define CEOS_STATE_STRIDE 20
 
 /* element technology, FEM/Modules/ModAnalysis.f90 (only full integration is provided) */
 #define CEOS_ELEM_FULL_INTEGRATION 1
@@ -101,8 +108,8 @@ int ceos_failed_elements(ceos_handle *h, int *ids, int cap);
 /* Gauss-point Cauchy stress of the last evaluation, [ne][8][6] (ClassConstitutiveModel%Stress). */
 int ceos_get_gp_stress(ceos_handle *h, double *sig);
 
-/* Committed (which = 0) or last-update (which = 1) J2 state, [ne][8][8]:
- * plastic strain (6, engineering shear), accumulated plastic strain, plastic flag. */
+/* Committed (which = 0) or last-update (which = 1) Gauss-point state, [ne][8][CEOS_STATE_STRIDE]
+ * (zero for models without state). */
 int ceos_get_state(ceos_handle *h, int which, double *state);
 int ceos_set_state(ceos_handle *h, int which, const double *state);
 

@@ -476,9 +476,398 @@ static void mat_j2_tangent(const double *p, const double *st, double D[6][6]) {
         }
 }
 
+
+/* ------------------------------------------------------------------------------------------
+ * model 12: viscoelastic matrix, ConstitutiveModelsLibrary/ModViscoelasticMatrix.f90:276-1028.
+ * Second-order tensors as column-major 9-vectors p = i + 3j (Tools/MathTools/ModVoigtNotation.f90:
+ * Tensor2ToVoigt :70-86); fourth-order tensors as 9x9 matrices T[p][q] = T_ijkl.
+ * props: K_inf, Mu_inf, Lambda_inf, K_e, Mu_e, Ni_v (K_inf and K_e are read but not used by the
+ * reference either).
+ * state (nstate 20): [0..8] Fv, [9..17] dv, [18] gamma, [19] time of the state; the initial
+ * state is Fv = I, dv = 0, gamma = 0, time 0 (the reference never initialises Time_old,
+ * ModViscoelasticMatrix.f90:47,329).
+ * ---------------------------------------------------------------------------------------- */
+#define VISCO_NSTATE 20
+
+static void v9_eye(double I[9]) {
+    memset(I, 0, sizeof(double) * 9);
+    I[0] = I[4] = I[8] = 1.0;
+}
+
+static void v9_trans(const double *A, double *T) { /* TransposeT2Voigt :394-409 */
+    T[0] = A[0]; T[1] = A[3]; T[2] = A[6];
+    T[3] = A[1]; T[4] = A[4]; T[5] = A[7];
+    T[6] = A[2]; T[7] = A[5]; T[8] = A[8];
+}
+
+static double v9_det(const double *A) { /* DeterminantT2Voigt :381-390 */
+    return A[0] * A[4] * A[8] - A[0] * A[7] * A[5] + A[1] * A[5] * A[6] - A[1] * A[3] * A[8] +
+           A[2] * A[3] * A[7] - A[2] * A[4] * A[6];
+}
+
+static void v9_inv(const double *A, double *B) { /* InverseT2Voigt :413-430 */
+    double d = v9_det(A);
+    B[0] = A[4] * A[8] - A[7] * A[5];
+    B[1] = A[2] * A[7] - A[1] * A[8];
+    B[2] = -A[2] * A[4] + A[1] * A[5];
+    B[3] = A[5] * A[6] - A[3] * A[8];
+    B[4] = -A[2] * A[6] + A[0] * A[8];
+    B[5] = A[2] * A[3] - A[0] * A[5];
+    B[6] = A[3] * A[7] - A[4] * A[6];
+    B[7] = A[1] * A[6] - A[0] * A[7];
+    B[8] = -A[1] * A[3] + A[0] * A[4];
+    for (int k = 0; k < 9; k++) B[k] /= d;
+}
+
+static void v9_mul(const double *A, const double *B, double *C) { /* SingleContractionT2T2Voigt :434-449 */
+    double R[9];
+    for (int j = 0; j < 3; j++)
+        for (int i = 0; i < 3; i++) {
+            double s = 0.0;
+            for (int k = 0; k < 3; k++) s += A[i + 3 * k] * B[k + 3 * j];
+            R[i + 3 * j] = s;
+        }
+    memcpy(C, R, sizeof(R));
+}
+
+static void t4_ball(const double *A, const double *B, double T[9][9]) { /* BallVoigt: A_ij B_kl */
+    for (int p = 0; p < 9; p++)
+        for (int q = 0; q < 9; q++) T[p][q] = A[p] * B[q];
+}
+
+static void t4_square(const double *A, const double *B, double T[9][9]) { /* SquareVoigt: A_ik B_jl */
+    for (int j = 0; j < 3; j++)
+        for (int i = 0; i < 3; i++)
+            for (int l = 0; l < 3; l++)
+                for (int k = 0; k < 3; k++) T[i + 3 * j][k + 3 * l] = A[i + 3 * k] * B[j + 3 * l];
+}
+
+static int sw9(int p) { return (p % 3) * 3 + p / 3; } /* (i,j) -> (j,i) */
+
+static void t4_rsym(const double A[9][9], double R[9][9]) { /* RightSymmetrizationT4Voigt */
+    double T[9][9];
+    for (int p = 0; p < 9; p++)
+        for (int q = 0; q < 9; q++) T[p][q] = (q == sw9(q)) ? A[p][q] : 0.5 * A[p][q] + 0.5 * A[p][sw9(q)];
+    memcpy(R, T, sizeof(T));
+}
+
+static void t4_rlsym(const double A[9][9], double R[9][9]) { /* RigthLeftSymmetrizationT4Voigt */
+    double T[9][9];
+    t4_rsym(A, T);
+    double L[9][9];
+    for (int p = 0; p < 9; p++)
+        for (int q = 0; q < 9; q++) L[p][q] = (p == sw9(p)) ? T[p][q] : 0.5 * T[p][q] + 0.5 * T[sw9(p)][q];
+    memcpy(R, L, sizeof(L));
+}
+
+static void t4_mul(const double A[9][9], const double B[9][9], double C[9][9]) { /* DoubleContractionT4T4Voigt */
+    double T[9][9];
+    for (int p = 0; p < 9; p++)
+        for (int q = 0; q < 9; q++) {
+            double s = 0.0;
+            for (int r = 0; r < 9; r++) s += A[p][r] * B[r][q];
+            T[p][q] = s;
+        }
+    memcpy(C, T, sizeof(T));
+}
+
+static void t4_isym(double T[9][9]) { /* IdentityT4SymVoigt: (d_ik d_jl + d_il d_jk)/2 */
+    for (int p = 0; p < 9; p++)
+        for (int q = 0; q < 9; q++) T[p][q] = 0.5 * ((p == q) + (q == sw9(p)));
+}
+
+static const int SYM6[6] = {0, 4, 8, 3, 7, 6}; /* Tensor4VoigtToTensor4VoigtSym :324-367 */
+
+static void t4_to_sym(const double A[9][9], double T[6][6]) {
+    for (int a = 0; a < 6; a++)
+        for (int b = 0; b < 6; b++) T[a][b] = A[SYM6[a]][SYM6[b]];
+}
+
+static void v9_to_sym(const double *A, double *T) { /* Tensor2VoigtToTensor2VoigtSym :147-160 */
+    T[0] = A[0]; T[1] = A[4]; T[2] = A[8]; T[3] = A[3]; T[4] = A[7]; T[5] = A[2];
+}
+
+static void sym_to_v9(const double *A, double *T) { /* Tensor2VoigtSymToTensor2Voigt :164-181 */
+    T[0] = A[0]; T[1] = A[3]; T[2] = A[5];
+    T[3] = A[3]; T[4] = A[1]; T[5] = A[4];
+    T[6] = A[5]; T[7] = A[4]; T[8] = A[2];
+}
+
+/* Gaussian elimination with partial pivoting (the dgetrf/dgetrs solve of
+ * SolveLinearSystemLU / SolveMultipleLinearSystemLU, ModMathRoutines.f90:1036-1120). */
+static int lu_solve(int n, double *A, int nrhs, double *B) {
+    int piv[16];
+    for (int k = 0; k < n; k++) {
+        int pmax = k;
+        for (int i = k + 1; i < n; i++)
+            if (fabs(A[i * n + k]) > fabs(A[pmax * n + k])) pmax = i;
+        piv[k] = pmax;
+        if (A[pmax * n + k] == 0.0) return -1;
+        if (pmax != k)
+            for (int j = 0; j < n; j++) {
+                double t = A[k * n + j]; A[k * n + j] = A[pmax * n + j]; A[pmax * n + j] = t;
+            }
+        for (int i = k + 1; i < n; i++) {
+            A[i * n + k] /= A[k * n + k];
+            for (int j = k + 1; j < n; j++) A[i * n + j] -= A[i * n + k] * A[k * n + j];
+        }
+    }
+    for (int r = 0; r < nrhs; r++) {
+        double x[16];
+        for (int i = 0; i < n; i++) x[i] = B[i * nrhs + r];
+        for (int k = 0; k < n; k++)
+            if (piv[k] != k) { double t = x[k]; x[k] = x[piv[k]]; x[piv[k]] = t; }
+        for (int i = 0; i < n; i++)
+            for (int j = 0; j < i; j++) x[i] -= A[i * n + j] * x[j];
+        for (int i = n - 1; i >= 0; i--) {
+            for (int j = i + 1; j < n; j++) x[i] -= A[i * n + j] * x[j];
+            x[i] /= A[i * n + i];
+        }
+        for (int i = 0; i < n; i++) B[i * nrhs + r] = x[i];
+    }
+    return 0;
+}
+
+typedef struct {
+    double Fvm[9], FvFo[9], Jvm, Cem[9], CemInv[9], Sem[9], Mem[9], Dem[9][9];
+} visco_kin;
+
+/* intermediate quantities of one local iteration, ModViscoelasticMatrix.f90:452-471 */
+static void visco_kin_eval(const double *F, const double *Fvm_old, const double *dvm, double dt,
+                           double Mu_e, visco_kin *k) {
+    double I[9], A[9], Ai[9], Foi[9], Fvi[9], Fem[9], FemT[9], sq[9][9];
+    v9_eye(I);
+    for (int q = 0; q < 9; q++) A[q] = I[q] - dt * dvm[q];
+    v9_inv(A, Ai);
+    v9_mul(Ai, Fvm_old, k->Fvm);
+    k->Jvm = v9_det(k->Fvm);
+    v9_inv(Fvm_old, Foi);
+    v9_mul(k->Fvm, Foi, k->FvFo);
+    v9_inv(k->Fvm, Fvi);
+    v9_mul(F, Fvi, Fem);
+    v9_trans(Fem, FemT);
+    v9_mul(FemT, Fem, k->Cem);
+    v9_inv(k->Cem, k->CemInv);
+    for (int q = 0; q < 9; q++) k->Sem[q] = Mu_e * (I[q] - k->CemInv[q]);
+    t4_square(k->CemInv, k->CemInv, sq);
+    t4_rsym(sq, k->Dem);
+    for (int p = 0; p < 9; p++)
+        for (int q = 0; q < 9; q++) k->Dem[p][q] *= 2.0 * Mu_e;
+    v9_mul(k->Cem, k->Sem, k->Mem);
+}
+
+/* Residual_MATRIX, ModViscoelasticMatrix.f90:573-623 */
+static void visco_residual(const visco_kin *k, double gama, const double *dvm, double dt, double Ni_v,
+                           double Rm[7]) {
+    double FvFoT[9], Dp[9], DpT[9], DL[9];
+    v9_trans(k->FvFo, FvFoT);
+    v9_mul(FvFoT, k->Mem, Dp);
+    for (int q = 0; q < 9; q++) Dp[q] *= -dt;
+    v9_trans(Dp, DpT);
+    for (int q = 0; q < 9; q++) Dp[q] = 0.5 * (Dp[q] + DpT[q]);
+    for (int q = 0; q < 9; q++) Dp[q] += dt * (Ni_v * dvm[q]);
+    for (int q = 0; q < 9; q++) DL[q] = Dp[q] + gama * (dt * k->Jvm * k->FvFo[q]);
+    v9_to_sym(DL, Rm);
+    Rm[6] = k->Jvm - 1.0;
+}
+
+/* Jacobian_MATRIX, ModViscoelasticMatrix.f90:627-734; also returns DSem_Ddvm */
+static void visco_jacobian(const visco_kin *k, const double *Fvm_old, double gama, double dt,
+                           double Ni_v, double DSem[9][9], double Km[7][7]) {
+    double I[9], CF[9], B[9][9], DCem[9][9], T[9][9], T2[9][9], DMem[9][9];
+    v9_eye(I);
+    v9_mul(k->Cem, k->FvFo, CF);
+    t4_square(CF, I, B);
+    t4_rlsym(B, DCem);
+    for (int p = 0; p < 9; p++)
+        for (int q = 0; q < 9; q++) DCem[p][q] *= -2.0 * dt;
+    t4_mul(k->Dem, B, T);
+    t4_rsym(T, DSem);
+    for (int p = 0; p < 9; p++)
+        for (int q = 0; q < 9; q++) DSem[p][q] *= -dt;
+    t4_square(I, k->Sem, T);
+    t4_mul(T, DCem, T);
+    t4_square(k->Cem, I, T2);
+    t4_mul(T2, DSem, T2);
+    for (int p = 0; p < 9; p++)
+        for (int q = 0; q < 9; q++) DMem[p][q] = T[p][q] + T2[p][q];
+    double Foi[9], Aux1[9], Aux2[9], tmp[9], FvFoT[9], D2H[9][9];
+    v9_inv(Fvm_old, Foi);
+    v9_trans(Foi, Aux1);
+    v9_mul(Aux1, k->FvFo, Aux1);
+    v9_mul(k->Fvm, k->Mem, tmp);
+    v9_trans(tmp, Aux2);
+    t4_square(Aux1, Aux2, T);
+    v9_trans(k->FvFo, FvFoT);
+    t4_square(FvFoT, I, T2);
+    t4_mul(T2, DMem, T2);
+    for (int p = 0; p < 9; p++)
+        for (int q = 0; q < 9; q++) D2H[p][q] = pow(dt, 2.0) * T[p][q] + dt * T2[p][q];
+    t4_rlsym(D2H, D2H);
+    double Is[9][9], D2J[9][9], D2L[9][9];
+    t4_isym(Is);
+    t4_ball(k->FvFo, k->FvFo, T);
+    t4_square(k->FvFo, k->FvFo, T2);
+    for (int p = 0; p < 9; p++)
+        for (int q = 0; q < 9; q++) D2J[p][q] = T[p][q] + T2[p][q];
+    t4_rlsym(D2J, D2J);
+    for (int p = 0; p < 9; p++)
+        for (int q = 0; q < 9; q++) {
+            double d2pinc = -D2H[p][q] + dt * (Ni_v * Is[p][q]);
+            D2L[p][q] = d2pinc + gama * (pow(dt, 2.0) * k->Jvm * D2J[p][q]);
+        }
+    double D2Lg[9], auxs[6], Ks[6][6];
+    for (int q = 0; q < 9; q++) D2Lg[q] = dt * k->Jvm * k->FvFo[q];
+    memset(Km, 0, sizeof(double) * 49);
+    t4_to_sym(D2L, Ks);
+    for (int a = 0; a < 6; a++)
+        for (int b = 0; b < 6; b++) Km[a][b] = Ks[a][b];
+    v9_to_sym(D2Lg, auxs);
+    for (int j = 0; j < 6; j++) Km[j][6] = Km[6][j] = auxs[j];
+    /* Km(1:6,1:6) = Km(1:6,1:6) * IdentityT4SymInvVoigtSym; Km(7,1:6) likewise */
+    for (int a = 0; a < 7; a++)
+        for (int b = 3; b < 6; b++) Km[a][b] *= 2.0;
+}
+
+/* UpdateStressAndStateVariables_ViscoelasticMatrix_3D + Local_Newton_Raphson_MATRIX
+ * (ModViscoelasticMatrix.f90:276-569).  F9: column-major deformation gradient. */
+static int visco_update(const double *p, const double *F9, double time, const double *old,
+                        double s[6], double *nw) {
+    double Mu_inf = p[1], Lambda_inf = p[2], Mu_e = p[4], Ni_v = p[5];
+    const double *Fvm_old = old, *gama_old = old + 18;
+    double dt = time - old[19];
+    double dvm[9] = {0}, X[7], I[9];
+    double gama = *gama_old;
+    v9_eye(I);
+    v9_to_sym(dvm, X);
+    X[6] = gama;
+    visco_kin k;
+    int ok = 0;
+    for (int it = 1; it <= 10; it++) {
+        visco_kin_eval(F9, Fvm_old, dvm, dt, Mu_e, &k);
+        double Rm[7];
+        visco_residual(&k, gama, dvm, dt, Ni_v, Rm);
+        double nrm = 0.0;
+        for (int q = 0; q < 7; q++) nrm += Rm[q] * Rm[q];
+        nrm = sqrt(nrm);
+        if (nrm < 1.0e-5) { ok = 1; break; }
+        double DSem[9][9], Km[7][7], dX[7];
+        visco_jacobian(&k, Fvm_old, gama, dt, Ni_v, DSem, Km);
+        for (int q = 0; q < 7; q++) dX[q] = -Rm[q];
+        if (lu_solve(7, &Km[0][0], 1, dX)) break;
+        for (int q = 0; q < 7; q++) X[q] += dX[q];
+        sym_to_v9(X, dvm);
+        gama = X[6];
+    }
+    memcpy(nw, k.Fvm, sizeof(double) * 9);
+    memcpy(nw + 9, dvm, sizeof(double) * 9);
+    nw[18] = gama;
+    nw[19] = time;
+    /* stress, ModViscoelasticMatrix.f90:364-400 */
+    double Ft[9], C[9], Ci[9], Sinf[9], Fvi[9], FviT[9], aux[9], Sm[9];
+    double J = v9_det(F9);
+    v9_trans(F9, Ft);
+    v9_mul(Ft, F9, C);
+    v9_inv(C, Ci);
+    for (int q = 0; q < 9; q++) Sinf[q] = Mu_inf * (I[q] - Ci[q]) + Lambda_inf * log(J) * Ci[q];
+    v9_inv(k.Fvm, Fvi);
+    v9_mul(Fvi, k.Sem, aux);
+    v9_trans(Fvi, FviT);
+    v9_mul(aux, FviT, aux);
+    for (int q = 0; q < 9; q++) Sm[q] = Sinf[q] + aux[q];
+    double FS[9], sg[9];
+    v9_mul(F9, Sm, FS);
+    v9_mul(FS, Ft, sg);
+    for (int q = 0; q < 9; q++) sg[q] *= 1.0 / J;
+    s[0] = sg[0]; s[1] = sg[4]; s[2] = sg[8]; s[3] = sg[3]; s[4] = sg[7]; s[5] = sg[6];
+    return ok ? OR_OK : OR_ERR_LOCAL_NR;
+}
+
+/* GetTangentModulus_ViscoelasticMatrix_3D, ModViscoelasticMatrix.f90:745-998 */
+static void visco_tangent(const double *p, const double *F9, const double *old, const double *nw,
+                          const double F[3][3], double D[6][6]) {
+    double Mu_inf = p[1], Lambda_inf = p[2], Mu_e = p[4], Ni_v = p[5];
+    const double *Fvm_old = old, *dvm = nw + 9;
+    double gama = nw[18];
+    double dt = nw[19] - old[19];
+    double I[9], Ft[9], C[9], Ci[9];
+    v9_eye(I);
+    double J = v9_det(F9);
+    v9_trans(F9, Ft);
+    v9_mul(Ft, F9, C);
+    v9_inv(C, Ci);
+    visco_kin k;
+    visco_kin_eval(F9, Fvm_old, dvm, dt, Mu_e, &k);
+    double Fvi[9], FviT[9], FiSq[9][9], FiTSq[9][9], Dinf[9][9], T[9][9], T2[9][9];
+    v9_inv(k.Fvm, Fvi);
+    v9_trans(Fvi, FviT);
+    t4_square(Fvi, Fvi, FiSq);
+    t4_square(FviT, FviT, FiTSq);
+    t4_ball(Ci, Ci, Dinf);
+    t4_square(Ci, Ci, T);
+    t4_rsym(T, T);
+    for (int a = 0; a < 9; a++)
+        for (int b = 0; b < 9; b++)
+            Dinf[a][b] = Lambda_inf * Dinf[a][b] + 2.0 * (Mu_inf - Lambda_inf * log(J)) * T[a][b];
+    double DSem[9][9], Km[7][7];
+    visco_jacobian(&k, Fvm_old, gama, dt, Ni_v, DSem, Km);
+    double W[9][9], DSmDC[9][9], Z[9][9], DSmDdv[9][9], DCemDC[9][9], DSemDC[9][9], DMemDC[9][9];
+    t4_mul(FiSq, k.Dem, W);
+    t4_mul(W, FiTSq, W);
+    t4_rlsym(W, T);
+    for (int a = 0; a < 9; a++)
+        for (int b = 0; b < 9; b++) DSmDC[a][b] = 0.5 * (Dinf[a][b] + T[a][b]);
+    double Foi[9], FviS[9];
+    v9_inv(Fvm_old, Foi);
+    v9_mul(Fvi, k.Sem, FviS);
+    t4_square(Foi, FviS, Z);
+    for (int a = 0; a < 9; a++)
+        for (int b = 0; b < 9; b++) Z[a][b] *= -2.0 * dt;
+    t4_mul(FiSq, DSem, T);
+    for (int a = 0; a < 9; a++)
+        for (int b = 0; b < 9; b++) Z[a][b] += T[a][b];
+    t4_rlsym(Z, DSmDdv);
+    t4_rsym(FiTSq, DCemDC);
+    t4_mul(k.Dem, DCemDC, DSemDC);
+    for (int a = 0; a < 9; a++)
+        for (int b = 0; b < 9; b++) DSemDC[a][b] *= 0.5;
+    t4_square(I, k.Sem, T);
+    t4_mul(T, DCemDC, DMemDC);
+    t4_square(k.Cem, I, T);
+    t4_mul(T, DSemDC, T2);
+    for (int a = 0; a < 9; a++)
+        for (int b = 0; b < 9; b++) DMemDC[a][b] += T2[a][b];
+    double FvFoT[9], G[9][9];
+    v9_trans(k.FvFo, FvFoT);
+    t4_square(FvFoT, I, T);
+    t4_mul(T, DMemDC, G);
+    t4_rlsym(G, G);
+    for (int a = 0; a < 9; a++)
+        for (int b = 0; b < 9; b++) G[a][b] *= -dt;
+    double Gs[6][6], rhs[7 * 6];
+    t4_to_sym(G, Gs);
+    memset(rhs, 0, sizeof(rhs));
+    for (int a = 0; a < 6; a++)
+        for (int b = 0; b < 6; b++) rhs[a * 6 + b] = -Gs[a][b];
+    lu_solve(7, &Km[0][0], 6, rhs);
+    double dd[6][6], Ds[6][6], Dm[6][6], DC[6][6];
+    for (int a = 0; a < 6; a++)
+        for (int b = 0; b < 6; b++) dd[a][b] = rhs[a * 6 + b] * (a >= 3 ? 2.0 : 1.0);
+    t4_to_sym(DSmDdv, Ds);
+    t4_to_sym(DSmDC, DC);
+    for (int a = 0; a < 6; a++)
+        for (int b = 0; b < 6; b++) {
+            double acc = 0.0;
+            for (int c = 0; c < 6; c++) acc += Ds[a][c] * dd[c][b];
+            Dm[a][b] = 2.0 * (DC[a][b] + acc);
+        }
+    push_forward_v(Dm, F, D);
+}
+
 int or_model_nstate(int model) {
     switch (model) {
     case OR_MODEL_J2: return J2_NSTATE;
+    case OR_MODEL_VISCO_MATRIX: return VISCO_NSTATE;
     default: return 0;
     }
 }
@@ -490,6 +879,7 @@ static int mat_nprops(int model) {
     case OR_MODEL_NEOHOOKEAN: return 2;
     case OR_MODEL_SVK: return 2;
     case OR_MODEL_COMP_NEOHOOKEAN: return 2;
+    case OR_MODEL_VISCO_MATRIX: return 6;
     default: return -1;
     }
 }
@@ -498,22 +888,34 @@ static int mat_nprops(int model) {
 static int mat_update(int model, const double *p, const double F[3][3], double time,
                       const double *old, double s[6], double *nw) {
     double Dtmp[6][6];
-    (void)time;
     switch (model) {
     case OR_MODEL_GHL: mat_ghl(p, F, s, Dtmp); return OR_OK;
     case OR_MODEL_COMP_NEOHOOKEAN: mat_cnh(p, F, s, Dtmp); return OR_OK;
     case OR_MODEL_NEOHOOKEAN: mat_nh(p, F, s, Dtmp); return OR_OK;
     case OR_MODEL_SVK: mat_svk(p, F, s, Dtmp); return OR_OK;
     case OR_MODEL_J2: mat_j2(p, F, old, s, nw); return OR_OK;
+    case OR_MODEL_VISCO_MATRIX: {
+        double F9[9];
+        for (int j = 0; j < 3; j++)
+            for (int i = 0; i < 3; i++) F9[i + 3 * j] = F[i][j];
+        return visco_update(p, F9, time, old, s, nw);
+    }
     default: return OR_ERR_ARG;
     }
 }
 
 /* tangent (GetTangentModulus), uses F and the state left by the last update */
-static void mat_tangent(int model, const double *p, const double F[3][3], const double *nw,
-                        double D[6][6]) {
+static void mat_tangent(int model, const double *p, const double F[3][3], const double *old,
+                        const double *nw, double D[6][6]) {
     double s[6];
     switch (model) {
+    case OR_MODEL_VISCO_MATRIX: {
+        double F9[9];
+        for (int j = 0; j < 3; j++)
+            for (int i = 0; i < 3; i++) F9[i + 3 * j] = F[i][j];
+        visco_tangent(p, F9, old, nw, F, D);
+        return;
+    }
     case OR_MODEL_GHL: mat_ghl(p, F, s, D); return;
     case OR_MODEL_COMP_NEOHOOKEAN: mat_cnh(p, F, s, D); return;
     case OR_MODEL_NEOHOOKEAN: mat_nh(p, F, s, D); return;
@@ -529,10 +931,11 @@ int or_material_point(int model, const double *props, const double Fv[9], double
     double zero[MAXSTATE] = {0};
     double nw[MAXSTATE] = {0};
     memcpy(F, Fv, sizeof(F));
+    if (model == OR_MODEL_VISCO_MATRIX) zero[0] = zero[4] = zero[8] = 1.0; /* Fv = I */
     if (!state_old) state_old = zero;
     int st = mat_update(model, props, F, time, state_old, sigma6, nw);
     if (st) return st;
-    if (D36) mat_tangent(model, props, F, nw, (double(*)[6])D36);
+    if (D36) mat_tangent(model, props, F, state_old, nw, (double(*)[6])D36);
     if (state_new) memcpy(state_new, nw, sizeof(double) * or_model_nstate(model));
     return OR_OK;
 }
@@ -623,6 +1026,16 @@ int or_material_set(or_mesh *m, int mat_id, int model, const double *props, int 
     memset(m->mats[mat_id].p, 0, sizeof(m->mats[mat_id].p));
     memcpy(m->mats[mat_id].p, props, sizeof(double) * nprops);
     m->mats[mat_id].set = 1;
+    if (model == OR_MODEL_VISCO_MATRIX) /* initial viscous state Fv = I (ModViscoelasticMatrix.f90:120-133) */
+        for (int e = 0; e < m->ne; e++)
+            if (m->mat[e] == mat_id)
+                for (int g = 0; g < 8; g++) {
+                    long gi = (long)e * 8 + g;
+                    memset(&m->st_old[MAXSTATE * gi], 0, sizeof(double) * MAXSTATE);
+                    memset(&m->st_new[MAXSTATE * gi], 0, sizeof(double) * MAXSTATE);
+                    m->st_old[MAXSTATE * gi + 0] = m->st_old[MAXSTATE * gi + 4] = m->st_old[MAXSTATE * gi + 8] = 1.0;
+                    m->st_new[MAXSTATE * gi + 0] = m->st_new[MAXSTATE * gi + 4] = m->st_new[MAXSTATE * gi + 8] = 1.0;
+                }
     return OR_OK;
 }
 
@@ -798,7 +1211,7 @@ static int elem_ke(or_mesh *m, int e, double Ke[24][24]) {
         long gi = (long)e * 8 + g;
         double D[6][6], F[3][3];
         memcpy(F, &m->F[9 * gi], sizeof(F));
-        mat_tangent(md->model, md->p, F, &m->st_new[MAXSTATE * gi], D);
+        mat_tangent(md->model, md->p, F, &m->st_old[MAXSTATE * gi], &m->st_new[MAXSTATE * gi], D);
         if (or_matrix_bg(x, nat[g], B, G, &detJ)) return OR_ERR_DETJ;
         double a = w[g] * detJ;
         if (!m->nl) {
@@ -1038,6 +1451,11 @@ void or_switch_converged_state(or_mesh *m) {
                 long gi = (long)e * 8 + g;
                 memcpy(&m->st_old[MAXSTATE * gi], &m->st_new[MAXSTATE * gi], sizeof(double) * 7);
             }
+        } else if (model == OR_MODEL_VISCO_MATRIX) { /* ModViscoelasticMatrix.f90:1005-1028 */
+            for (int g = 0; g < 8; g++) {
+                long gi = (long)e * 8 + g;
+                memcpy(&m->st_old[MAXSTATE * gi], &m->st_new[MAXSTATE * gi], sizeof(double) * VISCO_NSTATE);
+            }
         }
     }
 }
@@ -1053,13 +1471,13 @@ void or_get_gp_F(const or_mesh *m, double *out) {
 void or_get_state(const or_mesh *m, int which, double *out) {
     const double *src = which ? m->st_new : m->st_old;
     for (long gi = 0; gi < (long)m->ne * 8; gi++)
-        memcpy(&out[J2_NSTATE * gi], &src[MAXSTATE * gi], sizeof(double) * J2_NSTATE);
+        memcpy(&out[OR_STATE_STRIDE * gi], &src[MAXSTATE * gi], sizeof(double) * OR_STATE_STRIDE);
 }
 
 void or_set_state(or_mesh *m, int which, const double *in) {
     double *dst = which ? m->st_new : m->st_old;
     for (long gi = 0; gi < (long)m->ne * 8; gi++)
-        memcpy(&dst[MAXSTATE * gi], &in[J2_NSTATE * gi], sizeof(double) * J2_NSTATE);
+        memcpy(&dst[MAXSTATE * gi], &in[OR_STATE_STRIDE * gi], sizeof(double) * OR_STATE_STRIDE);
 }
 
 /* HomogenizeStress / HomogenizeDeformationGradient, MultiScale/ModMultiscaleAnalysis.f90:140-441.

@@ -93,7 +93,8 @@ void or_switch_converged_state(or_mesh *m);
 int or_failed_elements(const or_mesh *m, int *ids, int cap);
 void or_get_gp_stress(const or_mesh *m, double *out);      /* [ne][8][6] */
 void or_get_gp_F(const or_mesh *m, double *out);           /* [ne][8][9] */
-void or_get_state(const or_mesh *m, int which, double *out); /* which 0=old 1=new; [ne][8][nstate] */
+#define OR_STATE_STRIDE 20
+void or_get_state(const or_mesh *m, int which, double *out); /* which 0=old 1=new; [ne][8][20] */
 void or_set_state(or_mesh *m, int which, const double *in);
 int or_element_stiffness(or_mesh *m, int e, double *Ke);   /* 24x24 row-major, e 0-based */
 int or_element_internal_force(or_mesh *m, int e, double *Fe);

@@ -17,6 +17,7 @@ _dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 _ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
 
 MODEL_GHL, MODEL_J2, MODEL_NH, MODEL_SVK, MODEL_CNH, MODEL_VISCO = 1, 2, 3, 5, 7, 12
+STATE_STRIDE = 20  # per Gauss point in or_get_state / or_set_state
 
 _lib = None
 
@@ -179,9 +180,9 @@ class OracleMesh:
         return out.reshape(self.ne, 8, 3, 3)
 
     def state(self, which=1):
-        out = np.zeros(self.ne * 8 * 17)
+        out = np.zeros(self.ne * 8 * STATE_STRIDE)
         lib().or_get_state(self._h, which, out)
-        return out.reshape(self.ne, 8, 17)
+        return out.reshape(self.ne, 8, STATE_STRIDE)
 
     def set_state(self, which, arr):
         lib().or_set_state(self._h, which, np.ascontiguousarray(arr, dtype=np.float64).reshape(-1))

@@ -380,3 +380,53 @@ def test_schedule_does_not_change_results():
     assert s0[..., 6].max() > 0
     g.set_schedule(5)
     assert np.array_equal(g.state(0), s0)
+
+
+VISCO = [50.0, 1.0, 10.0, 20.0, 2.0, 5.0]  # K_inf, Mu_inf, Lambda_inf, K_e, Mu_e, Ni_v
+
+
+def test_visco_parity_after_committed_step():
+    X, conn = meshgen.cube(3, perturb=0.1, seed=12)
+    g, o = pair(X, conn, ol.MODEL_VISCO, VISCO, True)
+    rng = np.random.default_rng(12)
+    U1 = rng.normal(scale=0.02, size=3 * len(X))
+    U2 = U1 + rng.normal(scale=0.02, size=3 * len(X))
+    for m in (g, o):
+        m.update_mesh(U1)
+        assert m.solve_constitutive(0.1, U1) == 0
+        m.switch_converged_state()
+        m.update_mesh(U2)
+        assert m.solve_constitutive(0.35, U2) == 0
+    assert rel(g.internal_force()[0], o.internal_force()[0]) < RTOL
+    assert rel(g.tangent(), o.tangent()) < 1e-11
+    assert rel(g.gp_stress(), o.gp_stress()) < RTOL
+    sg, so = g.state(1), o.state(1)
+    assert rel(sg, so) < 1e-11
+    assert np.all(sg[..., 19] == 0.35)
+
+
+def test_visco_relaxation_history_matches_oracle():
+    """Stretch to 1.1 in 2 steps, then hold for 4 steps (relaxation): same Newton histories."""
+    X, conn = meshgen.cube(3)
+    g, o = pair(X, conn, ol.MODEL_VISCO, VISCO, True)
+    fixed, presc = meshgen.stretch_bc(X)
+    table = [(0.0, 0.0), (1.0, 0.1), (3.0, 0.1)]
+    bc = fd.BoundaryConditions(3 * len(X), fixed, [(int(d), (lambda t: fd.interp_table(table, t))) for d in presc])
+    hs = [fd.quasi_static(m, bc, [(0.0, 1.0, 2), (1.0, 3.0, 4)], nl=True, tol=1e-9, maxiter=15) for m in (g, o)]
+    assert [h[4] for h in hs[0]] == [h[4] for h in hs[1]]
+    for a, b in zip(*hs):
+        assert rel(a[3], b[3]) < 1e-10
+    assert rel(g.gp_stress(), o.gp_stress()) < 1e-10
+
+
+def test_visco_local_newton_failure_status():
+    """A NaN viscous modulus (set by the host) keeps the local residual from converging: status
+    -2, as the reference's 'Max Iteration in Local Newton-Raphson' error."""
+    X, conn = meshgen.cube(2)
+    p = list(VISCO)
+    p[4] = float("nan")
+    g, o = pair(X, conn, ol.MODEL_VISCO, p, True)
+    U = np.random.default_rng(1).normal(scale=0.01, size=3 * len(X))
+    assert o.solve_constitutive(0.1, U) == -2
+    assert g.solve_constitutive(0.1, U) == gpu.ERR_LOCAL_NR
+    assert g.last_status.error == 1 and g.last_status.n_failed == len(conn)

@@ -119,3 +119,83 @@ def test_j2_consistent_tangent_fd(e11, shear):
         assert stp[15] == stm[15] == st[15]
         Dfd[:, J] = (sp - sm) / (2 * h)
     assert np.linalg.norm(D - Dfd) / np.linalg.norm(D) < 1e-6
+
+
+VISCO = [50.0, 1.0, 10.0, 20.0, 2.0, 5.0]  # K_inf, Mu_inf, Lambda_inf, K_e, Mu_e, Ni_v
+
+
+def _virgin_visco_state():
+    st = np.zeros(20)
+    st[[0, 4, 8]] = 1.0  # Fv = I, dv = 0, gamma = 0, time 0
+    return st
+
+
+def test_visco_without_viscous_branch_is_compressible_neo_hookean():
+    """Mu_e = 0: S = Mu_inf (I - C^-1) + Lambda_inf lnJ C^-1, the compressible Neo-Hookean law."""
+    p = list(VISCO)
+    p[4] = 0.0
+    F = np.eye(3) + np.array([[0.1, 0.03, 0.0], [0.02, -0.05, 0.01], [0.0, 0.04, 0.07]])
+    s, D, _ = ol.material_point(ol.MODEL_VISCO, p, F, time=0.3, state_old=_virgin_visco_state())
+    sc, Dc, _ = ol.material_point(ol.MODEL_CNH, [p[1], p[2]], F)
+    assert np.allclose(s, sc, rtol=1e-13, atol=1e-13)
+    assert np.allclose(np.triu(D), np.triu(Dc), rtol=1e-11, atol=1e-11)
+
+
+def test_visco_instantaneous_response():
+    """dt = 0 from the virgin state: no viscous flow, the response is compressible Neo-Hookean
+    with shear modulus Mu_inf + Mu_e."""
+    F = np.eye(3) + np.array([[0.08, 0.02, 0.0], [0.01, -0.04, 0.02], [0.03, 0.0, 0.05]])
+    s, _, st = ol.material_point(ol.MODEL_VISCO, VISCO, F, time=0.0, state_old=_virgin_visco_state())
+    sc, _, _ = ol.material_point(ol.MODEL_CNH, [VISCO[1] + VISCO[4], VISCO[2]], F)
+    assert np.allclose(s, sc, rtol=1e-12, atol=1e-12)
+    assert np.allclose(st[:9], [1, 0, 0, 0, 1, 0, 0, 0, 1], atol=1e-14)
+
+
+def test_visco_relaxes_towards_equilibrium():
+    """Held isochoric deformation, growing time step: the stress decays from the instantaneous
+    value towards the long-term branch (Mu_inf) and stays between the two bounds (viscous flow
+    is isochoric, Jv = 1, so a volume-preserving F relaxes completely)."""
+    F = np.diag([1.1, 1.1 ** -0.5, 1.1 ** -0.5])
+    s_inst, _, _ = ol.material_point(ol.MODEL_CNH, [VISCO[1] + VISCO[4], VISCO[2]], F)
+    s_inf, _, _ = ol.material_point(ol.MODEL_CNH, [VISCO[1], VISCO[2]], F)
+    prev = s_inst[0]
+    for dt in (0.01, 0.1, 1.0, 10.0, 100.0):
+        s, _, _ = ol.material_point(ol.MODEL_VISCO, VISCO, F, time=dt, state_old=_virgin_visco_state())
+        assert s_inf[0] - 1e-9 <= s[0] <= prev + 1e-12
+        prev = s[0]
+    assert abs(prev - s_inf[0]) < 0.05 * abs(s_inst[0] - s_inf[0])
+
+
+def test_visco_element_tangent_is_consistent():
+    """Ke = dFe/du (finite differences) after a committed viscous step; the algorithmic tangent
+    assumes an exactly converged local Newton (tolerance 1e-5), hence the 1e-6 bar."""
+    from ceos_amd import meshgen
+    X, conn = meshgen.cube(1)
+    rng = np.random.default_rng(11)
+    X = X + rng.uniform(-0.1, 0.1, size=X.shape)
+    m = ol.OracleMesh(X, conn)
+    m.set_material(1, ol.MODEL_VISCO, VISCO)
+    gdof = (3 * (conn[0][:, None] - 1) + np.arange(3)[None, :]).reshape(24)
+    U1 = rng.normal(scale=0.03, size=24)
+    m.update_mesh(U1)
+    assert m.solve_constitutive(0.1, U1) == 0
+    m.switch_converged_state()
+    U0 = U1 + rng.normal(scale=0.03, size=24)
+
+    def fe(U):
+        m.update_mesh(U)
+        assert m.solve_constitutive(0.25, U) == 0
+        return m.element_internal_force(0)
+
+    fe(U0)
+    Ke = m.element_stiffness(0)
+    h = 1e-6
+    Kfd = np.zeros((24, 24))
+    for j in range(24):
+        dp, dm = U0.copy(), U0.copy()
+        dp[gdof[j]] += h
+        dm[gdof[j]] -= h
+        Kfd[:, j] = (fe(dp) - fe(dm)) / (2 * h)
+    assert np.linalg.norm(Ke - Kfd) / np.linalg.norm(Ke) < 1e-6
+    st = m.state(1)
+    assert np.all(st[0, :, 19] == 0.25) and np.any(np.abs(st[0, :, 18]) > 0)
