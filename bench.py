@@ -95,6 +95,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--precision", choices=["fp64", "fp32k"], default="fp64")
     ap.add_argument("--slabs", type=int, default=0, help="launch schedule (0 = automatic)")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="copy 1 GiB device-to-device before the timed region: the reference kernel "
+                         "for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE to bytes")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build shards nothing across GPUs "
@@ -123,6 +126,16 @@ def main():
 
     def step():
         mesh.evaluate_device(0.2, dU.ptr, dF.ptr, dK.ptr, sp)
+
+    if args.calibrate_copy:
+        n = 1 << 27  # 1 GiB of doubles
+        a, b = hiprt.DeviceArray(n), hiprt.DeviceArray(n)
+        hiprt.check(hiprt.hip().hipMemset(a.ptr, 0, a.nbytes), "hipMemset")
+        hiprt.synchronize()
+        hiprt.check(hiprt.hip().hipMemcpy(b.ptr, a.ptr, a.nbytes, hiprt.D2D), "hipMemcpy D2D")
+        hiprt.synchronize()
+        a.free()
+        b.free()
 
     for _ in range(args.warmup):
         step()

@@ -25,6 +25,7 @@ OK, ERR_DETJ, ERR_LOCAL_NR, ERR_SLOT, ERR_ARG, ERR_HIP = 0, -1, -2, -3, -4, -5
 MODEL_GHL, MODEL_J2, MODEL_NH, MODEL_SVK, MODEL_CNH, MODEL_VISCO = 1, 2, 3, 5, 7, 12
 STATE_STRIDE = 20
 K_FP64, K_FP32_STORED = 0, 1
+ASSEMBLY_GATHER, ASSEMBLY_COLORED = 0, 1
 
 _dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 _fp = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
@@ -49,7 +50,7 @@ EXPORTS = [
     "ceos_failed_elements", "ceos_get_gp_stress", "ceos_get_state", "ceos_set_state",
     "ceos_homogenize", "ceos_state_save", "ceos_state_load", "ceos_set_fused",
     "ceos_evaluate_device", "ceos_update_mesh_device", "ceos_device_status", "ceos_mesh_info",
-    "ceos_set_schedule",
+    "ceos_set_schedule", "ceos_set_assembly",
 ]
 
 _lib = None
@@ -90,6 +91,7 @@ def load_library(path=LIB_PATH):
     L.ceos_device_status.argtypes = [vp, C.POINTER(CeosStatus)]
     L.ceos_mesh_info.argtypes = [vp, C.POINTER(i), C.POINTER(C.c_longlong)]
     L.ceos_set_schedule.argtypes = [vp, i]
+    L.ceos_set_assembly.argtypes = [vp, i]
     _lib = L
     return L
 
@@ -102,7 +104,8 @@ def _check(code, what):
 class CeosMesh:
     """One hex8 mesh on the GPU. X (nn,3), conn (ne,8) 1-based node ids, mat_id (ne,)."""
 
-    def __init__(self, X, conn1, mat_id=None, nl_analysis=True, precision=K_FP64, fused=True):
+    def __init__(self, X, conn1, mat_id=None, nl_analysis=True, precision=K_FP64, fused=True,
+                 assembly=ASSEMBLY_GATHER):
         L = load_library()
         X = np.ascontiguousarray(X, dtype=np.float64).reshape(-1, 3)
         conn1 = np.ascontiguousarray(conn1, dtype=np.int32).reshape(-1, 8)
@@ -118,6 +121,7 @@ class CeosMesh:
         _check(L.ceos_set_analysis(self._h, 1 if nl_analysis else 0, 1), "ceos_set_analysis")
         _check(L.ceos_set_precision(self._h, precision), "ceos_set_precision")
         _check(L.ceos_set_fused(self._h, 1 if fused else 0), "ceos_set_fused")
+        _check(L.ceos_set_assembly(self._h, assembly), "ceos_set_assembly")
         self.precision = precision
         self._csr = None
         self.last_status = CeosStatus()
@@ -140,6 +144,9 @@ class CeosMesh:
     def set_material(self, mat_id, model, props):
         p = np.ascontiguousarray(props, dtype=np.float64)
         _check(load_library().ceos_material_set(self._h, mat_id, model, p, len(p)), "ceos_material_set")
+
+    def set_assembly(self, mode):
+        _check(load_library().ceos_set_assembly(self._h, mode), "ceos_set_assembly")
 
     def set_schedule(self, slabs):
         _check(load_library().ceos_set_schedule(self._h, slabs), "ceos_set_schedule")

@@ -1,0 +1,98 @@
+// Gather assembly of the upper CSR tangent and of Fint from element records.
+//
+// Replaces the scatter of AssembleGlobalMatrixUpperTriangular / InternalForce
+// (FEM/Subroutines/SubAssembleGlobalMatrixUpperTriangular.f90:13-69, SubInternalForce.f90:83)
+// by its transpose: every CSR row segment owns its output and sums the element blocks that
+// contribute to it, in element order, through a contribution map built once per mesh.
+//
+// Work unit = one node block of the upper pattern (the diagonal block of node A, or the block
+// (A, B) of its q-th upper neighbour B), enumerated in CSR order: block blk belongs to node
+// blknode[blk] and is its q = blk - blkstart[A]-th block.  The thread sums the contributing
+// element blocks and writes its 3 pieces of the rows 3A..3A+2; consecutive threads write
+// consecutive pieces of the same rows.
+// Contribution word: bits 0..30 = offset of the 3x3 block (or of the 6-value diagonal block)
+// in the record array, bit 31 = stored transposed (the element computed block (c,b) with
+// global node C > B, whose row-major values are the columns of CSR block (B, C)).
+#pragma once
+#include <stdint.h>
+
+namespace ceos {
+
+struct GatherParams {
+    int nn;
+    long long nblocks;
+    const int* blknode;      // [nblocks] node A of block
+    const int* rowstart;     // [nn] slot of (3A, 3A)
+    const int* nup;          // [nn]
+    const int* blkstart;     // [nn] global block index of the diagonal block of A
+    const int* cptr;         // [nblocks+1] contribution ranges per block
+    const uint32_t* cword;   // [36 ne]
+    const double* rec;       // element records
+    double* Fint;            // [3 nn]
+    void* K;                 // [nnz]
+};
+
+template <typename KT>
+__global__ __launch_bounds__(256) void k_gather_blocks(GatherParams p) {
+    const long long blk = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (blk >= p.nblocks) return;
+    const int c0 = p.cptr[blk], c1 = p.cptr[blk + 1];
+    const int A = p.blknode[blk];
+    const int q = (int)(blk - p.blkstart[A]);
+    const int nu = p.nup[A];
+    const long r0 = p.rowstart[A], r1 = r0 + 3 + 3 * nu, r2 = r1 + 2 + 3 * nu;
+    KT* K = (KT*)p.K;
+    if (q == 0) {
+        // diagonal block, stored upper 00 01 02 11 12 22, preceded by the lane's Fe
+        double a[6] = {0, 0, 0, 0, 0, 0}, f[3] = {0, 0, 0};
+        for (int c = c0; c < c1; ++c) {
+            const double* d = p.rec + (long)(p.cword[c] & 0x7fffffffu);
+#pragma unroll
+            for (int k = 0; k < 6; ++k) a[k] += d[k];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) f[k] += d[k - 3];
+        }
+#pragma unroll
+        for (int k = 0; k < 3; ++k) p.Fint[3L * A + k] = f[k];
+        if (K) {
+            K[r0] = (KT)a[0]; K[r0 + 1] = (KT)a[1]; K[r0 + 2] = (KT)a[2];
+            K[r1] = (KT)a[3]; K[r1 + 1] = (KT)a[4];
+            K[r2] = (KT)a[5];
+        }
+        return;
+    }
+    double a[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int c = c0; c < c1; ++c) {
+        const uint32_t w = p.cword[c];
+        const double* d = p.rec + (long)(w & 0x7fffffffu);
+        double v[9];
+#pragma unroll
+        for (int k = 0; k < 9; ++k) v[k] = d[k];
+        if (w >> 31) {
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+#pragma unroll
+                for (int k = 0; k < 3; ++k) a[3 * i + k] += v[3 * k + i];
+        } else {
+#pragma unroll
+            for (int k = 0; k < 9; ++k) a[k] += v[k];
+        }
+    }
+    const long o = 3L * (q - 1);
+    K[r0 + 3 + o] = (KT)a[0]; K[r0 + 4 + o] = (KT)a[1]; K[r0 + 5 + o] = (KT)a[2];
+    K[r1 + 2 + o] = (KT)a[3]; K[r1 + 3 + o] = (KT)a[4]; K[r1 + 4 + o] = (KT)a[5];
+    K[r2 + 1 + o] = (KT)a[6]; K[r2 + 2 + o] = (KT)a[7]; K[r2 + 3 + o] = (KT)a[8];
+}
+
+// residual-only gather: Fint of every node from the Fe parts of its diagonal contributions
+__global__ __launch_bounds__(256) void k_gather_fint(GatherParams p) {
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    if (t >= 3 * p.nn) return;
+    const int A = t / 3, i = t - 3 * A;
+    const int blk = p.blkstart[A];
+    double f = 0.0;
+    for (int c = p.cptr[blk]; c < p.cptr[blk + 1]; ++c) f += p.rec[(long)(p.cword[c] & 0x7fffffffu) - 3 + i];
+    p.Fint[t] = f;
+}
+
+}  // namespace ceos

@@ -21,6 +21,7 @@
 #include "../../include/ceos_gpu.h"
 #include "element_kernel.hpp"
 #include "homogenize_kernel.hpp"
+#include "assembly_kernel.hpp"
 
 using namespace ceos;
 
@@ -111,6 +112,8 @@ struct ceos_handle {
     bool has_state = false;
     int slabs = 0;           // 0 = automatic
     int nslabs_used = 1;
+    int assembly = CEOS_ASSEMBLY_GATHER;
+    long long nblocks = 0;   // node blocks of the upper pattern (gather assembly)
 
     // device
     double *dX = nullptr, *dx = nullptr, *dU = nullptr, *dFint = nullptr, *dK = nullptr;
@@ -121,6 +124,9 @@ struct ceos_handle {
     uint8_t *ddiag_first = nullptr, *delem_err = nullptr;
     int* derr = nullptr;
     double* dhpart = nullptr;  // homogenisation partial sums
+    double* dscratch = nullptr;  // element records (gather assembly)
+    int *dblknode = nullptr, *dblkstart = nullptr, *dcptr = nullptr;
+    uint32_t* dcword = nullptr;
     int hpart_n = 0;
 
     bool f_valid = false, k_valid = false, have_u = false;
@@ -208,6 +214,7 @@ static void free_device(ceos_handle* h) {
     dfree(h->dsig); dfree(h->dst_old); dfree(h->dst_new); dfree(h->dprops);
     dfree(h->dconn); dfree(h->dmat); dfree(h->drowstart); dfree(h->dnup);
     dfree(h->dpairs); dfree(h->ddiag_first); dfree(h->delem_err); dfree(h->derr); dfree(h->dhpart);
+    dfree(h->dscratch); dfree(h->dblknode); dfree(h->dblkstart); dfree(h->dcptr); dfree(h->dcword);
 }
 
 static int upload_props(ceos_handle* h) {
@@ -217,6 +224,76 @@ static int upload_props(ceos_handle* h) {
     HIP_OK(hipMemcpyAsync(h->dprops, props.data(), sizeof(double) * props.size(), hipMemcpyHostToDevice,
                           h->stream));
     HIP_OK(hipStreamSynchronize(h->stream));  // props is a temporary
+    return CEOS_OK;
+}
+
+// contribution map of the gather assembly (see assembly_kernel.hpp), element-id order inside
+// every block, and the element record buffer
+static int build_gather_map(ceos_handle* h) {
+    const int nn = h->nn, ne = h->ne;
+    if ((long long)ne * kRecord >= (1LL << 31)) return CEOS_ERR_ARG;
+    std::vector<int> blkstart(nn);
+    long long nb = 0;
+    for (int A = 0; A < nn; ++A) {
+        blkstart[A] = (int)nb;
+        nb += 1 + h->nup[A];
+    }
+    if (nb >= (1LL << 31)) return CEOS_ERR_ARG;
+    std::vector<int> blknode(nb);
+    for (int A = 0; A < nn; ++A)
+        for (int q = 0; q <= h->nup[A]; ++q) blknode[blkstart[A] + q] = A;
+    h->nblocks = nb;
+    std::vector<int> cptr(nb + 1, 0);
+    std::vector<uint32_t> cword((size_t)ne * 36);
+    // pass 0 counts, pass 1 fills
+    std::vector<int> fill;
+    for (int pass = 0; pass < 2; ++pass) {
+        if (pass == 1) {
+            for (long long b = 0; b < nb; ++b) cptr[b + 1] += cptr[b];
+            fill.assign(cptr.begin(), cptr.end() - 1);
+        }
+        for (int e = 0; e < ne; ++e) {
+            const int* cn = &h->conn0[(size_t)8 * e];
+            const long long base = (long long)h->iperm[e] * kRecord;
+            for (int b = 0; b < 8; ++b) {
+                const int B = cn[b];
+                const long long lb = base + (long long)b * kLaneRec;
+                const int np = b < 4 ? 4 : 3;
+                int blks[5];
+                uint32_t words[5];
+                blks[0] = blkstart[B];
+                words[0] = (uint32_t)(lb + 3);
+                for (int j = 0; j < np; ++j) {
+                    const int C = cn[(b + j + 1) & 7];
+                    const uint32_t off = (uint32_t)(lb + 9 + 9 * j);
+                    if (C < B) {
+                        const int r = rank_of(h, C, B);
+                        if (r < 0) return CEOS_ERR_SLOT;
+                        blks[1 + j] = blkstart[C] + 1 + r;
+                        words[1 + j] = off;
+                    } else {
+                        const int r = rank_of(h, B, C);
+                        if (r < 0) return CEOS_ERR_SLOT;
+                        blks[1 + j] = blkstart[B] + 1 + r;
+                        words[1 + j] = off | 0x80000000u;
+                    }
+                }
+                for (int k = 0; k <= np; ++k) {
+                    if (pass == 0) cptr[blks[k] + 1]++;
+                    else cword[fill[blks[k]]++] = words[k];
+                }
+            }
+        }
+    }
+    HIP_OK(dalloc(h->dblkstart, nn));
+    HIP_OK(dalloc(h->dblknode, nb));
+    HIP_OK(dalloc(h->dcptr, nb + 1));
+    HIP_OK(dalloc(h->dcword, (size_t)ne * 36));
+    HIP_OK(dalloc(h->dscratch, (size_t)ne * kRecord));
+    HIP_OK(hipMemcpy(h->dblkstart, blkstart.data(), sizeof(int) * nn, hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(h->dblknode, blknode.data(), sizeof(int) * nb, hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(h->dcptr, cptr.data(), sizeof(int) * (nb + 1), hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(h->dcword, cword.data(), sizeof(uint32_t) * cword.size(), hipMemcpyHostToDevice));
     return CEOS_OK;
 }
 
@@ -260,13 +337,23 @@ static int finalize(ceos_handle* h) {
             const int mi = (int)(std::lower_bound(models.begin(), models.end(), m) - models.begin());
             bucket[(size_t)h->color[e] * models.size() + mi].push_back(e);
         }
+        if (h->assembly == CEOS_ASSEMBLY_GATHER) {
+            // no write conflicts: one launch per model, elements in id order
+            for (auto& v : bucket) v.clear();
+            for (int e = 0; e < ne; ++e) {
+                const int m = h->mats[h->mat_of[e]].model;
+                const int mi = (int)(std::lower_bound(models.begin(), models.end(), m) - models.begin());
+                bucket[mi].push_back(e);
+            }
+        }
+        const int ncol = h->assembly == CEOS_ASSEMBLY_GATHER ? 1 : h->ncolor;
         // execution order: for each chunk of the id-ordered colour lists, every colour (and
         // model) in turn; one chunk (the default) is plain colour order
         int S = h->slabs;
         if (S <= 0) S = 1;
         h->nslabs_used = S;
         for (int sl = 0; sl < S; ++sl)
-            for (int c = 0; c < h->ncolor; ++c)
+            for (int c = 0; c < ncol; ++c)
                 for (size_t mi = 0; mi < models.size(); ++mi) {
                     auto& v = bucket[(size_t)c * models.size() + mi];
                     const size_t b0 = v.size() * sl / S, b1 = v.size() * (sl + 1) / S;
@@ -280,11 +367,17 @@ static int finalize(ceos_handle* h) {
     h->iperm.assign(ne, 0);
     for (int s = 0; s < ne; ++s) h->iperm[h->perm[s]] = s;
 
+    if (h->assembly == CEOS_ASSEMBLY_GATHER) {
+        int rc = build_gather_map(h);
+        if (rc) return rc;
+    }
+
     // per slot tables in execution order; first-touch over node blocks
     std::vector<int> conn_s((size_t)ne * 8), mat_s(ne);
-    std::vector<uint32_t> pairs((size_t)ne * 8, 0);
-    std::vector<uint8_t> dfirst(ne, 0);
-    std::vector<uint64_t> touched((h->nnz + 63) / 64, 0);
+    const bool colored = h->assembly == CEOS_ASSEMBLY_COLORED;
+    std::vector<uint32_t> pairs(colored ? (size_t)ne * 8 : 0, 0);
+    std::vector<uint8_t> dfirst(colored ? ne : 0, 0);
+    std::vector<uint64_t> touched(colored ? (h->nnz + 63) / 64 : 0, 0);
     auto test_and_set = [&](long long id) {
         const bool was = (touched[id >> 6] >> (id & 63)) & 1ull;
         touched[id >> 6] |= (1ull << (id & 63));
@@ -295,6 +388,7 @@ static int finalize(ceos_handle* h) {
         const int* cn = &h->conn0[(size_t)8 * e];
         for (int a = 0; a < 8; ++a) conn_s[(size_t)8 * s + a] = cn[a];
         mat_s[s] = h->mat_of[e];
+        if (!colored) continue;
         uint8_t df = 0;
         for (int b = 0; b < 8; ++b)
             if (test_and_set(h->rowstart[cn[b]])) df |= (uint8_t)(1u << b);
@@ -317,14 +411,22 @@ static int finalize(ceos_handle* h) {
 
     HIP_OK(dalloc(h->dconn, (size_t)ne * 8));
     HIP_OK(dalloc(h->dmat, ne));
-    HIP_OK(dalloc(h->dpairs, (size_t)ne * 8));
-    HIP_OK(dalloc(h->ddiag_first, ne));
+    if (colored) {
+        HIP_OK(dalloc(h->dpairs, (size_t)ne * 8));
+        HIP_OK(dalloc(h->ddiag_first, ne));
+    } else {
+        dfree(h->dpairs);
+        dfree(h->ddiag_first);
+    }
     HIP_OK(dalloc(h->delem_err, ne));
     HIP_OK(dalloc(h->dsig, (size_t)ne * 48));
     HIP_OK(hipMemcpyAsync(h->dconn, conn_s.data(), sizeof(int) * conn_s.size(), hipMemcpyHostToDevice, h->stream));
     HIP_OK(hipMemcpyAsync(h->dmat, mat_s.data(), sizeof(int) * ne, hipMemcpyHostToDevice, h->stream));
-    HIP_OK(hipMemcpyAsync(h->dpairs, pairs.data(), sizeof(uint32_t) * pairs.size(), hipMemcpyHostToDevice, h->stream));
-    HIP_OK(hipMemcpyAsync(h->ddiag_first, dfirst.data(), ne, hipMemcpyHostToDevice, h->stream));
+    if (colored) {
+        HIP_OK(hipMemcpyAsync(h->dpairs, pairs.data(), sizeof(uint32_t) * pairs.size(), hipMemcpyHostToDevice,
+                              h->stream));
+        HIP_OK(hipMemcpyAsync(h->ddiag_first, dfirst.data(), ne, hipMemcpyHostToDevice, h->stream));
+    }
     HIP_OK(hipMemsetAsync(h->delem_err, 0, ne, h->stream));
     HIP_OK(hipMemsetAsync(h->dsig, 0, sizeof(double) * (size_t)ne * 48, h->stream));
     if (h->has_state) {
@@ -365,33 +467,33 @@ static int finalize(ceos_handle* h) {
 // ------------------------------------------------------------------------------------------
 // kernel dispatch
 // ------------------------------------------------------------------------------------------
-template <int MODEL, bool NL, bool WANT_K, typename KT>
+template <int MODEL, bool NL, bool WANT_K, typename KT, bool G>
 static void launch_t(const ElemParams& p, int b, int e, hipStream_t s) {
     const int n = e - b;
     const int grid = (n + kEPB - 1) / kEPB;
-    hipLaunchKernelGGL((k_element<MODEL, NL, WANT_K, KT>), dim3(grid), dim3(kEPB * 8), 0, s, p, b, e);
+    hipLaunchKernelGGL((k_element<MODEL, NL, WANT_K, KT, G>), dim3(grid), dim3(kEPB * 8), 0, s, p, b, e);
 }
 
-template <int MODEL, typename KT>
+template <int MODEL, typename KT, bool G>
 static void launch_m(const ElemParams& p, int b, int e, bool nl, bool wk, hipStream_t s) {
     if (nl) {
-        if (wk) launch_t<MODEL, true, true, KT>(p, b, e, s);
-        else launch_t<MODEL, true, false, KT>(p, b, e, s);
+        if (wk) launch_t<MODEL, true, true, KT, G>(p, b, e, s);
+        else launch_t<MODEL, true, false, KT, G>(p, b, e, s);
     } else {
-        if (wk) launch_t<MODEL, false, true, KT>(p, b, e, s);
-        else launch_t<MODEL, false, false, KT>(p, b, e, s);
+        if (wk) launch_t<MODEL, false, true, KT, G>(p, b, e, s);
+        else launch_t<MODEL, false, false, KT, G>(p, b, e, s);
     }
 }
 
-template <typename KT>
+template <typename KT, bool G>
 static void launch_range(const ElemParams& p, const LaunchRange& r, bool nl, bool wk, hipStream_t s) {
     switch (r.model) {
-        case kGHL: launch_m<kGHL, KT>(p, r.begin, r.end, nl, wk, s); break;
-        case kJ2: launch_m<kJ2, KT>(p, r.begin, r.end, nl, wk, s); break;
-        case kNH: launch_m<kNH, KT>(p, r.begin, r.end, nl, wk, s); break;
-        case kSVK: launch_m<kSVK, KT>(p, r.begin, r.end, nl, wk, s); break;
-        case kCNH: launch_m<kCNH, KT>(p, r.begin, r.end, nl, wk, s); break;
-        case kVISCO: launch_m<kVISCO, KT>(p, r.begin, r.end, nl, wk, s); break;
+        case kGHL: launch_m<kGHL, KT, G>(p, r.begin, r.end, nl, wk, s); break;
+        case kJ2: launch_m<kJ2, KT, G>(p, r.begin, r.end, nl, wk, s); break;
+        case kNH: launch_m<kNH, KT, G>(p, r.begin, r.end, nl, wk, s); break;
+        case kSVK: launch_m<kSVK, KT, G>(p, r.begin, r.end, nl, wk, s); break;
+        case kCNH: launch_m<kCNH, KT, G>(p, r.begin, r.end, nl, wk, s); break;
+        case kVISCO: launch_m<kVISCO, KT, G>(p, r.begin, r.end, nl, wk, s); break;
         default: break;
     }
 }
@@ -421,10 +523,38 @@ static int evaluate(ceos_handle* h, const double* dU, double* dFint, void* dK, h
     p.err_count = h->derr;
     p.time = h->time;
     hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(1), 0, s, h->derr);
+    p.scratch = h->dscratch;
     const bool wk = dK != nullptr;
+    const bool f32 = h->precision == CEOS_K_FP32_STORED;
+    const bool gather = h->assembly == CEOS_ASSEMBLY_GATHER;
     for (const auto& r : h->ranges) {
-        if (h->precision == CEOS_K_FP32_STORED) launch_range<float>(p, r, h->nl != 0, wk, s);
-        else launch_range<double>(p, r, h->nl != 0, wk, s);
+        if (gather) {
+            launch_range<double, true>(p, r, h->nl != 0, wk, s);  // records are FP64 either way
+        } else {
+            if (f32) launch_range<float, false>(p, r, h->nl != 0, wk, s);
+            else launch_range<double, false>(p, r, h->nl != 0, wk, s);
+        }
+    }
+    if (gather) {
+        GatherParams g;
+        g.nn = h->nn;
+        g.nblocks = h->nblocks;
+        g.blknode = h->dblknode;
+        g.rowstart = h->drowstart;
+        g.nup = h->dnup;
+        g.blkstart = h->dblkstart;
+        g.cptr = h->dcptr;
+        g.cword = h->dcword;
+        g.rec = h->dscratch;
+        g.Fint = dFint;
+        g.K = dK;
+        if (wk) {
+            const unsigned nbk = (unsigned)((h->nblocks + 255) / 256);
+            if (f32) hipLaunchKernelGGL(k_gather_blocks<float>, dim3(nbk), dim3(256), 0, s, g);
+            else hipLaunchKernelGGL(k_gather_blocks<double>, dim3(nbk), dim3(256), 0, s, g);
+        } else {
+            hipLaunchKernelGGL(k_gather_fint, dim3((3 * h->nn + 255) / 256), dim3(256), 0, s, g);
+        }
     }
     HIP_OK(hipGetLastError());
     return CEOS_OK;
@@ -563,6 +693,16 @@ int ceos_set_precision(ceos_handle* h, int mode) {
 int ceos_set_fused(ceos_handle* h, int fused) {
     if (!h) return CEOS_ERR_ARG;
     h->fused = fused != 0;
+    return CEOS_OK;
+}
+
+int ceos_set_assembly(ceos_handle* h, int mode) {
+    if (!h || (mode != CEOS_ASSEMBLY_GATHER && mode != CEOS_ASSEMBLY_COLORED)) return CEOS_ERR_ARG;
+    if (mode != h->assembly) {
+        h->assembly = mode;
+        h->finalized = false;
+        h->f_valid = h->k_valid = false;
+    }
     return CEOS_OK;
 }
 

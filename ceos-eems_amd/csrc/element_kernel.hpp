@@ -37,6 +37,11 @@ constexpr int kOffSig = 192;
 constexpr int kOffD = 240;
 constexpr int kJ2State = 8;           // [0..5] plastic strain, [6] acc. plastic strain, [7] flag
 constexpr int kStateStride = 20;      // state rows per Gauss point in the device arrays (max of models)
+// Element record of the gather assembly: lane b owns doubles [45 b, 45 b + 45):
+//   [0..2] Fe_b, [3..8] diagonal block (b,b) upper (00 01 02 11 12 22),
+//   [9 + 9 j .. 17 + 9 j] block (c,b), c = (b + j + 1) mod 8, rows = dofs of c, row-major.
+constexpr int kLaneRec = 45;
+constexpr int kRecord = 8 * kLaneRec;
 
 struct ElemParams {
     int ne;                        // elements (device slot order)
@@ -53,6 +58,7 @@ struct ElemParams {
     double* Fint;                  // [3 nn]
     void* K;                       // [nnz] double or float
     double* sig;                   // [6][8][ne]
+    double* scratch;               // [ne][kRecord] element records (gather assembly)
     const double* st_old;          // [nstate][8][ne] committed state
     double* st_new;                // [nstate][8][ne] state of the last update
     uint8_t* elem_err;             // [ne]: bit 0 detJ <= 1e-13, bit 1 local Newton failed
@@ -82,7 +88,7 @@ __device__ __forceinline__ bool material(const double* p, const double F[3][3], 
     else { return mat_j2(p, F, old, s, D, nw); }
 }
 
-template <int MODEL, bool NL, bool WANT_K, typename KT>
+template <int MODEL, bool NL, bool WANT_K, typename KT, bool GATHER>
 __global__ __launch_bounds__(128) void k_element(ElemParams p, int e_begin, int e_end) {
     __shared__ double smem[kEPB * kLdsPerElem];
     __shared__ int snode[kEPB][8];
@@ -261,6 +267,31 @@ __global__ __launch_bounds__(128) void k_element(ElemParams p, int e_begin, int 
                 }
             }
         }
+    }
+    if constexpr (GATHER) {
+        // stage the element record in LDS (the GP data are no longer read), then write the
+        // records of the block's consecutive elements with unit-stride stores
+        __syncthreads();
+        double* rec = sm + b * kLaneRec;
+        rec[0] = fe0;
+        rec[1] = fe1;
+        rec[2] = fe2;
+#pragma unroll
+        for (int q = 0; q < 6; ++q) rec[3 + q] = kd[q];
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+            for (int q = 0; q < 9; ++q) rec[9 + 9 * j + q] = (j < npart) ? ko[j][q] : 0.0;
+        __syncthreads();
+        const int e0 = e_begin + blockIdx.x * kEPB;
+        const int nel = min(kEPB, e_end - e0);
+        const int total = nel * kRecord;
+        double* dst = p.scratch + (long)e0 * kRecord;
+        for (int g2 = threadIdx.x; g2 < total; g2 += kEPB * 8) {
+            const int el = g2 / kRecord, r = g2 - el * kRecord;
+            dst[g2] = smem[el * kLdsPerElem + r];
+        }
+        return;
     }
     if (!active) return;
 

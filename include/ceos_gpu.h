@@ -142,6 +142,15 @@ int ceos_device_status(ceos_handle *h, ceos_status *st);
  * evaluation; results agree to rounding for every schedule. */
 int ceos_set_schedule(ceos_handle *h, int slabs);
 
+/* Assembly of Fint and the CSR tangent (both atomic-free, deterministic):
+ * CEOS_ASSEMBLY_GATHER (default): the element kernel writes per-element records and a second
+ *   kernel sums them into each CSR row segment through a precomputed contribution map (element
+ *   order, as the reference's loop); every K entry is written once, row-contiguously.
+ * CEOS_ASSEMBLY_COLORED: one launch per element colour scatters into K and Fint in place. */
+#define CEOS_ASSEMBLY_GATHER 0
+#define CEOS_ASSEMBLY_COLORED 1
+int ceos_set_assembly(ceos_handle *h, int mode);
+
 /* Mesh facts after setup: number of element colours and the nnz of the upper pattern. */
 int ceos_mesh_info(ceos_handle *h, int *n_colors, long long *nnz_upper);
 

@@ -29,8 +29,8 @@ MATS = {
 }
 
 
-def pair(X, conn, model, props, nl, mat_id=None):
-    g = gpu.CeosMesh(X, conn, mat_id=mat_id, nl_analysis=nl)
+def pair(X, conn, model, props, nl, mat_id=None, assembly=gpu.ASSEMBLY_GATHER):
+    g = gpu.CeosMesh(X, conn, mat_id=mat_id, nl_analysis=nl, assembly=assembly)
     o = ol.OracleMesh(X, conn, mat_id=mat_id, nl_analysis=nl)
     ids = [1] if mat_id is None else sorted(set(int(m) for m in mat_id))
     for i in ids:
@@ -77,12 +77,13 @@ def test_csr_pattern_bit_exact(mesh):
     assert np.array_equal(cf - 1, np.nonzero(A)[1])
 
 
+@pytest.mark.parametrize("assembly", [gpu.ASSEMBLY_GATHER, gpu.ASSEMBLY_COLORED])
 @pytest.mark.parametrize("name", list(MATS))
 @pytest.mark.parametrize("mesh", ["cube4", "scrambled4"])
-def test_fint_k_sigma_parity(name, mesh):
+def test_fint_k_sigma_parity(name, mesh, assembly):
     model, props, nl = MATS[name]
     X, conn = meshgen.cube(4, perturb=0.1, seed=2) if mesh == "cube4" else scrambled()
-    g, o = pair(X, conn, model, props, nl)
+    g, o = pair(X, conn, model, props, nl, assembly=assembly)
     rng = np.random.default_rng(7)
     scale = 0.03 if nl else (0.0004 if name == "j2" else 0.004)  # J2: mixed elastic/plastic
     U = rng.normal(scale=scale, size=3 * len(X))
@@ -234,6 +235,25 @@ def test_j2_cyclic_history_matches_oracle():
     sg, so = g.state(0), o.state(0)
     assert rel(sg[..., :7], so[..., :7]) < 1e-10
     assert sg[..., 6].max() > 0.0
+
+
+@pytest.mark.parametrize("assembly", [gpu.ASSEMBLY_GATHER, gpu.ASSEMBLY_COLORED])
+def test_assemblies_agree_and_residual_only_path(assembly):
+    """Both assemblies give the oracle's K and Fint; the residual-only pass (tangent not
+    requested) gives the same Fint."""
+    X, conn = meshgen.cube(5, perturb=0.1, seed=21)
+    U = np.random.default_rng(21).normal(scale=0.02, size=3 * len(X))
+    g, o = pair(X, conn, ol.MODEL_NH, [0.5, 100.0], True, assembly=assembly)
+    for m in (g, o):
+        m.update_mesh(U)
+        assert m.solve_constitutive(0.0, U) == 0
+    fo = o.internal_force()[0]
+    assert rel(g.internal_force()[0], fo) < RTOL and rel(g.tangent(), o.tangent()) < RTOL
+    r = gpu.CeosMesh(X, conn, fused=False, assembly=assembly)
+    r.set_material(1, ol.MODEL_NH, [0.5, 100.0])
+    r.update_mesh(U)
+    assert r.solve_constitutive(0.0, U) == 0
+    assert rel(r.internal_force()[0], fo) < RTOL
 
 
 def test_fp32_stored_tangent_inexact_newton():
