@@ -50,7 +50,7 @@ EXPORTS = [
     "ceos_failed_elements", "ceos_get_gp_stress", "ceos_get_state", "ceos_set_state",
     "ceos_homogenize", "ceos_state_save", "ceos_state_load", "ceos_set_fused",
     "ceos_evaluate_device", "ceos_update_mesh_device", "ceos_device_status", "ceos_mesh_info",
-    "ceos_set_schedule", "ceos_set_assembly",
+    "ceos_set_schedule", "ceos_set_assembly", "ceos_set_tangent_mode",
 ]
 
 _lib = None
@@ -92,6 +92,7 @@ def load_library(path=LIB_PATH):
     L.ceos_mesh_info.argtypes = [vp, C.POINTER(i), C.POINTER(C.c_longlong)]
     L.ceos_set_schedule.argtypes = [vp, i]
     L.ceos_set_assembly.argtypes = [vp, i]
+    L.ceos_set_tangent_mode.argtypes = [vp, i]
     _lib = L
     return L
 
@@ -144,6 +145,9 @@ class CeosMesh:
     def set_material(self, mat_id, model, props):
         p = np.ascontiguousarray(props, dtype=np.float64)
         _check(load_library().ceos_material_set(self._h, mat_id, model, p, len(p)), "ceos_material_set")
+
+    def set_tangent_mode(self, numerical):
+        _check(load_library().ceos_set_tangent_mode(self._h, 1 if numerical else 0), "ceos_set_tangent_mode")
 
     def set_assembly(self, mode):
         _check(load_library().ceos_set_assembly(self._h, mode), "ceos_set_assembly")

@@ -113,6 +113,7 @@ struct ceos_handle {
     int slabs = 0;           // 0 = automatic
     int nslabs_used = 1;
     int assembly = CEOS_ASSEMBLY_GATHER;
+    int numerical_tangent = 0;
     long long nblocks = 0;   // node blocks of the upper pattern (gather assembly)
 
     // device
@@ -522,6 +523,7 @@ static int evaluate(ceos_handle* h, const double* dU, double* dFint, void* dK, h
     p.elem_err = h->delem_err;
     p.err_count = h->derr;
     p.time = h->time;
+    p.numerical_tangent = h->numerical_tangent;
     hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(1), 0, s, h->derr);
     p.scratch = h->dscratch;
     const bool wk = dK != nullptr;
@@ -693,6 +695,13 @@ int ceos_set_precision(ceos_handle* h, int mode) {
 int ceos_set_fused(ceos_handle* h, int fused) {
     if (!h) return CEOS_ERR_ARG;
     h->fused = fused != 0;
+    return CEOS_OK;
+}
+
+int ceos_set_tangent_mode(ceos_handle* h, int mode) {
+    if (!h || (mode != CEOS_TANGENT_ANALYTIC && mode != CEOS_TANGENT_NUMERICAL)) return CEOS_ERR_ARG;
+    h->numerical_tangent = mode == CEOS_TANGENT_NUMERICAL;
+    h->k_valid = false;
     return CEOS_OK;
 }
 

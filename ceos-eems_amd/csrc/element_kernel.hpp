@@ -64,6 +64,7 @@ struct ElemParams {
     uint8_t* elem_err;             // [ne]: bit 0 detJ <= 1e-13, bit 1 local Newton failed
     int* err_count;                // [0] elements with detJ failures, [1] local Newton failures
     double time;                   // analysis time of the evaluation (viscous time step)
+    int numerical_tangent;         // 1: D from tangent_numerical instead of the model's own
 };
 
 __device__ __forceinline__ long rowptr(int rs, int nu, int i) {
@@ -86,6 +87,75 @@ __device__ __forceinline__ bool material(const double* p, const double F[3][3], 
     else if constexpr (MODEL == kSVK) { mat_svk(p, F, s, D); return false; }
     else if constexpr (MODEL == kVISCO) { return mat_visco(p, F, time, old, s, D, nw); }
     else { return mat_j2(p, F, old, s, D, nw); }
+}
+
+// GetTangentModulusBase (ConstitutiveLibrary/ModConstitutiveModel.f90:104-164): central
+// differences (h = 1e-8) of the first Piola stress P = J sigma F^-T over the 9 components of F,
+// A[(i,J)][(k,L)] = dP_kL/dF_iJ (column-major pairs), converted to the material modulus
+// C_IJKL = sum_ik Finv_Ii Finv_Kk (A[(i,J)][(k,L)] - delta_ik S_JL)
+// (First_Elasticity_Modulus_To_Second_Voigt, Tools/MathTools/ModMathRoutines.f90:315) and
+// pushed forward.  The model is evaluated 18 times from the same committed state.
+__device__ __forceinline__ void first_piola(const double s[6], const double F[3][3], double P[3][3]) {
+    const double J = det3(F);
+    double Fi[3][3];
+    inv3(F, J, Fi);
+    const double sg[3][3] = {{s[0], s[3], s[5]}, {s[3], s[1], s[4]}, {s[5], s[4], s[2]}};
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) P[i][j] = J * (sg[i][0] * Fi[j][0] + sg[i][1] * Fi[j][1] + sg[i][2] * Fi[j][2]);
+}
+
+template <int MODEL>
+__device__ __noinline__ void tangent_numerical(const double* pr, const double F0[3][3], double time,
+                                               const double* old, const double s0[6], double D[21]) {
+    constexpr int NS = model_nstate<MODEL>();
+    const double h = 1.0e-8;
+    double A[9][9];
+    double F[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) F[i][j] = F0[i][j];
+#pragma unroll 1
+    for (int ij = 0; ij < 9; ++ij) {
+        const int i = ij % 3, j = ij / 3;
+        double P[2][3][3];
+#pragma unroll 1
+        for (int sgn = 0; sgn < 2; ++sgn) {
+            F[i][j] = F0[i][j] + (sgn == 0 ? h : -h);
+            double s[6], Dt[21], nw[NS];
+            material<MODEL>(pr, F, time, old, s, Dt, nw);
+            first_piola(s, F, P[sgn]);
+        }
+        F[i][j] = F0[i][j];
+#pragma unroll
+        for (int l = 0; l < 3; ++l)
+#pragma unroll
+            for (int k = 0; k < 3; ++k) A[ij][k + 3 * l] = (P[0][k][l] - P[1][k][l]) / (2.0 * h);
+    }
+    double P0[3][3], V[3][3], S[3][3];
+    first_piola(s0, F0, P0);
+    inv3(F0, det3(F0), V);
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int b = 0; b < 3; ++b) S[a][b] = V[a][0] * P0[0][b] + V[a][1] * P0[1][b] + V[a][2] * P0[2][b];
+    const int vi[6] = {0, 1, 2, 0, 1, 0}, vj[6] = {0, 1, 2, 1, 2, 2};
+    double C[21];
+#pragma unroll 1
+    for (int a = 0; a < 6; ++a)
+        for (int b = a; b < 6; ++b) {
+            const int I = vi[a], J = vj[a], K = vi[b], L = vj[b];
+            double acc = 0.0;
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+#pragma unroll
+                for (int k = 0; k < 3; ++k)
+                    acc += V[I][i] * V[K][k] * (A[i + 3 * J][k + 3 * L] - (i == k ? S[J][L] : 0.0));
+            C[up_idx(a, b)] = acc;
+        }
+    push_forward_up(C, F0, det3(F0), D);
 }
 
 template <int MODEL, bool NL, bool WANT_K, typename KT, bool GATHER>
@@ -164,6 +234,9 @@ __global__ __launch_bounds__(128) void k_element(ElemParams p, int e_begin, int 
             }
         }
         const bool flag = material<MODEL>(pr, F, p.time, old, s, D, nw);
+        if constexpr (WANT_K) {
+            if (p.numerical_tangent) tangent_numerical<MODEL>(pr, F, p.time, old, s, D);
+        }
         if (active) {
 #pragma unroll
             for (int k = 0; k < 6; ++k) p.sig[((long)k * 8 + g) * p.ne + e] = s[k];

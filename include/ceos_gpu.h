@@ -151,6 +151,13 @@ int ceos_set_schedule(ceos_handle *h, int slabs);
 #define CEOS_ASSEMBLY_COLORED 1
 int ceos_set_assembly(ceos_handle *h, int mode);
 
+/* Tangent modulus used in the stiffness: the model's own (default) or the central-difference
+ * tangent of GetTangentModulusBase (ConstitutiveLibrary/ModConstitutiveModel.f90:104-164), the
+ * reference's fallback for models without an analytic tangent. */
+#define CEOS_TANGENT_ANALYTIC 0
+#define CEOS_TANGENT_NUMERICAL 1
+int ceos_set_tangent_mode(ceos_handle *h, int mode);
+
 /* Mesh facts after setup: number of element colours and the nnz of the upper pattern. */
 int ceos_mesh_info(ceos_handle *h, int *n_colors, long long *nnz_upper);
 

@@ -864,6 +864,62 @@ static void visco_tangent(const double *p, const double *F9, const double *old, 
     push_forward_v(Dm, F, D);
 }
 
+
+/* GetTangentModulusBase, ConstitutiveLibrary/ModConstitutiveModel.f90:104-164: central
+ * differences (h = 1e-8) of the first Piola stress over the 9 components of F, converted to the
+ * material modulus (First_Elasticity_Modulus_To_Second_Voigt, Tools/MathTools/ModMathRoutines.f90:315)
+ * and pushed forward.  A[(i,J)][(k,L)] = dP_kL / dF_iJ (column-major 9-indices);
+ * C_IJKL = sum_ik Finv_Ii Finv_Kk (A[(i,J)][(k,L)] - delta_ik S_JL). */
+static void first_piola(const double sig[6], const double F[3][3], double P[3][3]) {
+    double sg[3][3] = {{sig[0], sig[3], sig[5]}, {sig[3], sig[1], sig[4]}, {sig[5], sig[4], sig[2]}};
+    double Ft[3][3], FiT[3][3];
+    double J = det3(F);
+    transpose3(F, Ft);
+    inv3(Ft, FiT);
+    matmul3(sg, FiT, P);
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) P[i][j] *= J;
+}
+
+static int mat_update(int model, const double *p, const double F[3][3], double time,
+                      const double *old, double s[6], double *nw);
+
+static void mat_tangent_numerical(int model, const double *p, const double F0[3][3], double time,
+                                  const double *old, const double sig0[6], double D[6][6]) {
+    const double h = 1.0e-8;
+    double A[9][9], F[3][3], s[6], nw[MAXSTATE], Pf[3][3], Pb[3][3];
+    memcpy(F, F0, sizeof(F));
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) {
+            F[i][j] = F0[i][j] + h;
+            mat_update(model, p, F, time, old, s, nw);
+            first_piola(s, F, Pf);
+            F[i][j] = F0[i][j] - h;
+            mat_update(model, p, F, time, old, s, nw);
+            first_piola(s, F, Pb);
+            for (int l = 0; l < 3; l++)
+                for (int k = 0; k < 3; k++) A[i + 3 * j][k + 3 * l] = (Pf[k][l] - Pb[k][l]) / (2.0 * h);
+            F[i][j] = F0[i][j];
+        }
+    /* S = F^-1 (J sigma) F^-T from the stress of the unperturbed update */
+    double P0[3][3], V[3][3], S[3][3];
+    first_piola(sig0, F0, P0);
+    inv3(F0, V);
+    matmul3(V, P0, S);
+    static const int vi[6] = {0, 1, 2, 0, 1, 0}, vj[6] = {0, 1, 2, 1, 2, 2};
+    double C[6][6];
+    for (int a = 0; a < 6; a++)
+        for (int b = 0; b < 6; b++) {
+            int I = vi[a], J = vj[a], K = vi[b], L = vj[b];
+            double acc = 0.0;
+            for (int i = 0; i < 3; i++)
+                for (int k = 0; k < 3; k++)
+                    acc += V[I][i] * V[K][k] * (A[i + 3 * J][k + 3 * L] - (i == k ? S[J][L] : 0.0));
+            C[a][b] = acc;
+        }
+    push_forward_v(C, F0, D);
+}
+
 int or_model_nstate(int model) {
     switch (model) {
     case OR_MODEL_J2: return J2_NSTATE;
@@ -951,6 +1007,7 @@ typedef struct {
 
 struct or_mesh {
     int nn, ne, nl;
+    int numerical_tangent;
     int nthreads;
     double *X, *x;          /* [nn][3] reference / current */
     int *conn;              /* [ne][8] 0-based */
@@ -1040,6 +1097,7 @@ int or_material_set(or_mesh *m, int mat_id, int model, const double *props, int 
 }
 
 void or_set_analysis(or_mesh *m, int nl_analysis) { m->nl = nl_analysis ? 1 : 0; }
+void or_set_tangent_mode(or_mesh *m, int numerical) { m->numerical_tangent = numerical ? 1 : 0; }
 int or_ndof(const or_mesh *m) { return 3 * m->nn; }
 
 static void elem_coords(const or_mesh *m, int e, const double *src, double c[8][3]) {
@@ -1211,7 +1269,10 @@ static int elem_ke(or_mesh *m, int e, double Ke[24][24]) {
         long gi = (long)e * 8 + g;
         double D[6][6], F[3][3];
         memcpy(F, &m->F[9 * gi], sizeof(F));
-        mat_tangent(md->model, md->p, F, &m->st_old[MAXSTATE * gi], &m->st_new[MAXSTATE * gi], D);
+        if (m->numerical_tangent)
+            mat_tangent_numerical(md->model, md->p, F, m->time, &m->st_old[MAXSTATE * gi], &m->sig[6 * gi], D);
+        else
+            mat_tangent(md->model, md->p, F, &m->st_old[MAXSTATE * gi], &m->st_new[MAXSTATE * gi], D);
         if (or_matrix_bg(x, nat[g], B, G, &detJ)) return OR_ERR_DETJ;
         double a = w[g] * detJ;
         if (!m->nl) {

@@ -26,6 +26,7 @@
  *     5  St.Venant-Kirchhoff            ConstitutiveModelsLibrary/ModStVenantKirchhoff.f90:643-773
  *     7  compressible Neo-Hookean       ConstitutiveModelsLibrary/ModCompressibleNeoHookean.f90:422-546
  *     12 viscoelastic matrix            ConstitutiveModelsLibrary/ModViscoelasticMatrix.f90:276-1028
+ *   numerical tangent                   ConstitutiveLibrary/ModConstitutiveModel.f90:104-164
  *   Voigt push-forward                  Tools/MathTools/ModMathRoutines.f90:254-311
  *   homogenization                      MultiScale/ModMultiscaleAnalysis.f90:140-441
  *
@@ -81,6 +82,8 @@ void or_mesh_destroy(or_mesh *m);
  * visco: see ceos_oracle.c (or_visco_*) */
 int or_material_set(or_mesh *m, int mat_id, int model, const double *props, int nprops);
 void or_set_analysis(or_mesh *m, int nl_analysis);
+/* 0: the model's own tangent; 1: central-difference tangent of GetTangentModulusBase */
+void or_set_tangent_mode(or_mesh *m, int numerical);
 void or_set_threads(or_mesh *m, int nthreads);
 int or_ndof(const or_mesh *m);
 long or_csr_nnz(or_mesh *m);

@@ -38,6 +38,7 @@ def lib():
         L.or_material_set.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp, C.c_int]
         L.or_set_analysis.argtypes = [C.c_void_p, C.c_int]
         L.or_set_threads.argtypes = [C.c_void_p, C.c_int]
+        L.or_set_tangent_mode.argtypes = [C.c_void_p, C.c_int]
         L.or_csr_nnz.restype = C.c_long
         L.or_csr_nnz.argtypes = [C.c_void_p]
         L.or_csr_pattern.argtypes = [C.c_void_p, _ip, _ip]
@@ -121,6 +122,9 @@ class OracleMesh:
         st = lib().or_material_set(self._h, mat_id, model, p, len(p))
         if st:
             raise ValueError("or_material_set status %d" % st)
+
+    def set_tangent_mode(self, numerical):
+        lib().or_set_tangent_mode(self._h, 1 if numerical else 0)
 
     def set_threads(self, n):
         lib().or_set_threads(self._h, n)

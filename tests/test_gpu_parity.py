@@ -450,3 +450,30 @@ def test_visco_local_newton_failure_status():
     assert o.solve_constitutive(0.1, U) == -2
     assert g.solve_constitutive(0.1, U) == gpu.ERR_LOCAL_NR
     assert g.last_status.error == 1 and g.last_status.n_failed == len(conn)
+
+
+@pytest.mark.parametrize("name", ["cnh", "nh", "svk", "visco"])
+def test_numerical_tangent_matches_oracle(name):
+    """Central-difference tangent (GetTangentModulusBase, h = 1e-8) on both sides: the two
+    differ by the rounding of the perturbed stresses, amplified by 1/h, hence the 1e-6 bar;
+    both stay within 1e-5 of the analytic tangent."""
+    if name == "visco":
+        model, props = ol.MODEL_VISCO, VISCO
+    else:
+        model, props, _ = MATS[name]
+    X, conn = meshgen.cube(3, perturb=0.1, seed=4)
+    g, o = pair(X, conn, model, props, True)
+    rng = np.random.default_rng(4)
+    U = rng.normal(scale=0.02, size=3 * len(X))
+    for m in (g, o):
+        m.update_mesh(U)
+        assert m.solve_constitutive(0.2, U) == 0
+    Ka = g.tangent()
+    g.set_tangent_mode(True)
+    o.set_tangent_mode(True)
+    Kg, Ko = g.tangent(), o.tangent()
+    assert rel(Kg, Ko) < 1e-6, rel(Kg, Ko)
+    assert rel(Kg, Ka) < 1e-5, rel(Kg, Ka)
+    assert rel(Ko, Ka) < 1e-5
+    g.set_tangent_mode(False)
+    assert np.array_equal(g.tangent(), Ka)
