@@ -51,6 +51,7 @@ EXPORTS = [
     "ceos_homogenize", "ceos_state_save", "ceos_state_load", "ceos_set_fused",
     "ceos_evaluate_device", "ceos_update_mesh_device", "ceos_device_status", "ceos_mesh_info",
     "ceos_set_schedule", "ceos_set_assembly", "ceos_set_tangent_mode",
+    "ceos_timing_start", "ceos_timing_stop",
 ]
 
 _lib = None
@@ -93,6 +94,9 @@ def load_library(path=LIB_PATH):
     L.ceos_set_schedule.argtypes = [vp, i]
     L.ceos_set_assembly.argtypes = [vp, i]
     L.ceos_set_tangent_mode.argtypes = [vp, i]
+    L.ceos_timing_start.argtypes = [vp, i]
+    L.ceos_timing_stop.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                   C.POINTER(C.c_int), C.POINTER(C.c_longlong)]
     _lib = L
     return L
 
@@ -148,6 +152,17 @@ class CeosMesh:
 
     def set_tangent_mode(self, numerical):
         _check(load_library().ceos_set_tangent_mode(self._h, 1 if numerical else 0), "ceos_set_tangent_mode")
+
+    def timing_start(self, max_evaluations):
+        _check(load_library().ceos_timing_start(self._h, max_evaluations), "ceos_timing_start")
+
+    def timing_stop(self):
+        """(element_ms, assembly_ms, n_evaluations, element_launches) summed over the
+        evaluations timed since timing_start."""
+        te, ta, n, nl = C.c_double(), C.c_double(), C.c_int(), C.c_longlong()
+        _check(load_library().ceos_timing_stop(self._h, C.byref(te), C.byref(ta), C.byref(n), C.byref(nl)),
+               "ceos_timing_stop")
+        return te.value, ta.value, n.value, nl.value
 
     def set_assembly(self, mode):
         _check(load_library().ceos_set_assembly(self._h, mode), "ceos_set_assembly")

@@ -132,6 +132,12 @@ struct ceos_handle {
 
     bool f_valid = false, k_valid = false, have_u = false;
     double time = 0.0;
+
+    // phase timing: 3 events per evaluation (start, element kernels done, assembly done)
+    std::vector<hipEvent_t> tev;
+    int timing_cap = 0, timing_n = 0;
+    bool timing_on = false;
+    long long timing_launches = 0;
 };
 
 // ------------------------------------------------------------------------------------------
@@ -468,33 +474,35 @@ static int finalize(ceos_handle* h) {
 // ------------------------------------------------------------------------------------------
 // kernel dispatch
 // ------------------------------------------------------------------------------------------
-template <int MODEL, bool NL, bool WANT_K, typename KT, bool G>
+template <int MODEL, bool NL, bool WANT_K, typename KT, bool G, bool NUMT = false>
 static void launch_t(const ElemParams& p, int b, int e, hipStream_t s) {
     const int n = e - b;
     const int grid = (n + kEPB - 1) / kEPB;
-    hipLaunchKernelGGL((k_element<MODEL, NL, WANT_K, KT, G>), dim3(grid), dim3(kEPB * 8), 0, s, p, b, e);
+    hipLaunchKernelGGL((k_element<MODEL, NL, WANT_K, KT, G, NUMT>), dim3(grid), dim3(kEPB * 8), 0, s, p, b, e);
 }
 
 template <int MODEL, typename KT, bool G>
-static void launch_m(const ElemParams& p, int b, int e, bool nl, bool wk, hipStream_t s) {
+static void launch_m(const ElemParams& p, int b, int e, bool nl, bool wk, bool numt, hipStream_t s) {
     if (nl) {
-        if (wk) launch_t<MODEL, true, true, KT, G>(p, b, e, s);
+        if (wk && numt) launch_t<MODEL, true, true, KT, G, true>(p, b, e, s);
+        else if (wk) launch_t<MODEL, true, true, KT, G>(p, b, e, s);
         else launch_t<MODEL, true, false, KT, G>(p, b, e, s);
     } else {
-        if (wk) launch_t<MODEL, false, true, KT, G>(p, b, e, s);
+        if (wk && numt) launch_t<MODEL, false, true, KT, G, true>(p, b, e, s);
+        else if (wk) launch_t<MODEL, false, true, KT, G>(p, b, e, s);
         else launch_t<MODEL, false, false, KT, G>(p, b, e, s);
     }
 }
 
 template <typename KT, bool G>
-static void launch_range(const ElemParams& p, const LaunchRange& r, bool nl, bool wk, hipStream_t s) {
+static void launch_range(const ElemParams& p, const LaunchRange& r, bool nl, bool wk, bool numt, hipStream_t s) {
     switch (r.model) {
-        case kGHL: launch_m<kGHL, KT, G>(p, r.begin, r.end, nl, wk, s); break;
-        case kJ2: launch_m<kJ2, KT, G>(p, r.begin, r.end, nl, wk, s); break;
-        case kNH: launch_m<kNH, KT, G>(p, r.begin, r.end, nl, wk, s); break;
-        case kSVK: launch_m<kSVK, KT, G>(p, r.begin, r.end, nl, wk, s); break;
-        case kCNH: launch_m<kCNH, KT, G>(p, r.begin, r.end, nl, wk, s); break;
-        case kVISCO: launch_m<kVISCO, KT, G>(p, r.begin, r.end, nl, wk, s); break;
+        case kGHL: launch_m<kGHL, KT, G>(p, r.begin, r.end, nl, wk, numt, s); break;
+        case kJ2: launch_m<kJ2, KT, G>(p, r.begin, r.end, nl, wk, numt, s); break;
+        case kNH: launch_m<kNH, KT, G>(p, r.begin, r.end, nl, wk, numt, s); break;
+        case kSVK: launch_m<kSVK, KT, G>(p, r.begin, r.end, nl, wk, numt, s); break;
+        case kCNH: launch_m<kCNH, KT, G>(p, r.begin, r.end, nl, wk, numt, s); break;
+        case kVISCO: launch_m<kVISCO, KT, G>(p, r.begin, r.end, nl, wk, numt, s); break;
         default: break;
     }
 }
@@ -523,20 +531,23 @@ static int evaluate(ceos_handle* h, const double* dU, double* dFint, void* dK, h
     p.elem_err = h->delem_err;
     p.err_count = h->derr;
     p.time = h->time;
-    p.numerical_tangent = h->numerical_tangent;
     hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(1), 0, s, h->derr);
     p.scratch = h->dscratch;
+    hipEvent_t* ev = (h->timing_on && h->timing_n < h->timing_cap) ? &h->tev[3 * h->timing_n] : nullptr;
+    if (ev) HIP_OK(hipEventRecord(ev[0], s));
     const bool wk = dK != nullptr;
     const bool f32 = h->precision == CEOS_K_FP32_STORED;
     const bool gather = h->assembly == CEOS_ASSEMBLY_GATHER;
+    const bool numt = h->numerical_tangent != 0;
     for (const auto& r : h->ranges) {
         if (gather) {
-            launch_range<double, true>(p, r, h->nl != 0, wk, s);  // records are FP64 either way
+            launch_range<double, true>(p, r, h->nl != 0, wk, numt, s);  // records are FP64 either way
         } else {
-            if (f32) launch_range<float, false>(p, r, h->nl != 0, wk, s);
-            else launch_range<double, false>(p, r, h->nl != 0, wk, s);
+            if (f32) launch_range<float, false>(p, r, h->nl != 0, wk, numt, s);
+            else launch_range<double, false>(p, r, h->nl != 0, wk, numt, s);
         }
     }
+    if (ev) HIP_OK(hipEventRecord(ev[1], s));
     if (gather) {
         GatherParams g;
         g.nn = h->nn;
@@ -557,6 +568,11 @@ static int evaluate(ceos_handle* h, const double* dU, double* dFint, void* dK, h
         } else {
             hipLaunchKernelGGL(k_gather_fint, dim3((3 * h->nn + 255) / 256), dim3(256), 0, s, g);
         }
+    }
+    if (ev) {
+        HIP_OK(hipEventRecord(ev[2], s));
+        h->timing_n++;
+        h->timing_launches += (long long)h->ranges.size();
     }
     HIP_OK(hipGetLastError());
     return CEOS_OK;
@@ -653,9 +669,18 @@ int ceos_mesh_create(int nn, const double* X, int ne, int npe, const int* conn, 
     return CEOS_OK;
 }
 
+static void timing_free(ceos_handle* h) {
+    for (hipEvent_t e : h->tev) (void)hipEventDestroy(e);
+    h->tev.clear();
+    h->timing_cap = h->timing_n = 0;
+    h->timing_on = false;
+    h->timing_launches = 0;
+}
+
 void ceos_mesh_destroy(ceos_handle* h) {
     if (!h) return;
     (void)hipStreamSynchronize(h->stream);
+    timing_free(h);
     free_device(h);
     (void)hipStreamDestroy(h->stream);
     delete h;
@@ -1015,6 +1040,37 @@ int ceos_update_mesh_device(ceos_handle* h, const double* dU, void* stream) {
     hipLaunchKernelGGL(k_add, dim3((n + 255) / 256), dim3(256), 0, s, n, h->dX, dU, h->dx);
     HIP_OK(hipGetLastError());
     h->f_valid = h->k_valid = false;
+    return CEOS_OK;
+}
+
+int ceos_timing_start(ceos_handle* h, int max_evaluations) {
+    if (!h || max_evaluations <= 0 || max_evaluations > (1 << 20)) return CEOS_ERR_ARG;
+    HIP_OK(hipStreamSynchronize(h->stream));
+    timing_free(h);
+    h->tev.resize(3L * max_evaluations, nullptr);
+    for (auto& e : h->tev) HIP_OK(hipEventCreate(&e));
+    h->timing_cap = max_evaluations;
+    h->timing_on = true;
+    return CEOS_OK;
+}
+
+int ceos_timing_stop(ceos_handle* h, double* element_ms, double* assembly_ms, int* n_evaluations,
+                     long long* element_launches) {
+    if (!h || !h->timing_on) return CEOS_ERR_ARG;
+    double te = 0.0, ta = 0.0;
+    for (int i = 0; i < h->timing_n; ++i) {
+        float a = 0.f, b = 0.f;
+        HIP_OK(hipEventSynchronize(h->tev[3 * i + 2]));
+        HIP_OK(hipEventElapsedTime(&a, h->tev[3 * i], h->tev[3 * i + 1]));
+        HIP_OK(hipEventElapsedTime(&b, h->tev[3 * i + 1], h->tev[3 * i + 2]));
+        te += a;
+        ta += b;
+    }
+    if (element_ms) *element_ms = te;
+    if (assembly_ms) *assembly_ms = ta;
+    if (n_evaluations) *n_evaluations = h->timing_n;
+    if (element_launches) *element_launches = h->timing_launches;
+    timing_free(h);
     return CEOS_OK;
 }
 

@@ -64,7 +64,6 @@ struct ElemParams {
     uint8_t* elem_err;             // [ne]: bit 0 detJ <= 1e-13, bit 1 local Newton failed
     int* err_count;                // [0] elements with detJ failures, [1] local Newton failures
     double time;                   // analysis time of the evaluation (viscous time step)
-    int numerical_tangent;         // 1: D from tangent_numerical instead of the model's own
 };
 
 __device__ __forceinline__ long rowptr(int rs, int nu, int i) {
@@ -158,7 +157,9 @@ __device__ __noinline__ void tangent_numerical(const double* pr, const double F0
     push_forward_up(C, F0, det3(F0), D);
 }
 
-template <int MODEL, bool NL, bool WANT_K, typename KT, bool GATHER>
+// NUMT: tangent modulus by central differences (a separate instantiation, so that the default
+// kernel carries no call and keeps its register allocation)
+template <int MODEL, bool NL, bool WANT_K, typename KT, bool GATHER, bool NUMT = false>
 __global__ __launch_bounds__(128) void k_element(ElemParams p, int e_begin, int e_end) {
     __shared__ double smem[kEPB * kLdsPerElem];
     __shared__ int snode[kEPB][8];
@@ -234,9 +235,7 @@ __global__ __launch_bounds__(128) void k_element(ElemParams p, int e_begin, int 
             }
         }
         const bool flag = material<MODEL>(pr, F, p.time, old, s, D, nw);
-        if constexpr (WANT_K) {
-            if (p.numerical_tangent) tangent_numerical<MODEL>(pr, F, p.time, old, s, D);
-        }
+        if constexpr (WANT_K && NUMT) tangent_numerical<MODEL>(pr, F, p.time, old, s, D);
         if (active) {
 #pragma unroll
             for (int k = 0; k < 6; ++k) p.sig[((long)k * 8 + g) * p.ne + e] = s[k];

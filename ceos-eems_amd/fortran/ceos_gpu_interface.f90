@@ -17,6 +17,8 @@ module ceos_gpu_interface
     integer(c_int), parameter :: CEOS_OK = 0, CEOS_ERR_DETJ = -1, CEOS_ERR_LOCAL_NR = -2, &
                                  CEOS_ERR_SLOT = -3, CEOS_ERR_ARG = -4, CEOS_ERR_HIP = -5
     integer(c_int), parameter :: CEOS_K_FP64 = 0, CEOS_K_FP32_STORED = 1
+    integer(c_int), parameter :: CEOS_TANGENT_ANALYTIC = 0, CEOS_TANGENT_NUMERICAL = 1
+    integer(c_int), parameter :: CEOS_STATE_STRIDE = 20
 
     ! mirrors ClassStatus (Tools/ModStatus.f90:3-16)
     type, bind(C) :: ceos_status
@@ -87,6 +89,32 @@ module ceos_gpu_interface
             import :: c_int, c_double, c_ptr
             type(c_ptr), value :: h
             real(c_double), intent(out) :: Kval(*)
+        end function
+
+        integer(c_int) function ceos_tangent_f32(h, Kval) bind(C, name="ceos_tangent_f32")
+            import :: c_int, c_float, c_ptr
+            type(c_ptr), value :: h
+            real(c_float), intent(out) :: Kval(*)
+        end function
+
+        integer(c_int) function ceos_set_tangent_mode(h, mode) bind(C, name="ceos_set_tangent_mode")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: h
+            integer(c_int), value :: mode
+        end function
+
+        integer(c_int) function ceos_get_state(h, which, state) bind(C, name="ceos_get_state")
+            import :: c_int, c_double, c_ptr
+            type(c_ptr), value :: h
+            integer(c_int), value :: which
+            real(c_double), intent(out) :: state(*)
+        end function
+
+        integer(c_int) function ceos_set_state(h, which, state) bind(C, name="ceos_set_state")
+            import :: c_int, c_double, c_ptr
+            type(c_ptr), value :: h
+            integer(c_int), value :: which
+            real(c_double), intent(in) :: state(*)
         end function
 
         integer(c_int) function ceos_update_mesh(h, U) bind(C, name="ceos_update_mesh")

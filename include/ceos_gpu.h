@@ -136,6 +136,15 @@ int ceos_evaluate_device(ceos_handle *h, double time, const double *dU, double *
 int ceos_update_mesh_device(ceos_handle *h, const double *dU, void *stream);
 int ceos_device_status(ceos_handle *h, ceos_status *st);
 
+/* Phase timing of the next evaluations (instrumentation, no reference counterpart): after
+ * ceos_timing_start, each evaluation (up to max_evaluations) records three HIP events on the
+ * stream it runs on -- before the element kernels, between them and the assembly kernel, after
+ * the assembly.  ceos_timing_stop waits for the last event and returns the summed element and
+ * assembly milliseconds, the number of evaluations timed and of element-kernel launches. */
+int ceos_timing_start(ceos_handle *h, int max_evaluations);
+int ceos_timing_stop(ceos_handle *h, double *element_ms, double *assembly_ms, int *n_evaluations,
+                     long long *element_launches);
+
 /* Execution schedule of the element launches: the id-ordered element list of every colour is
  * cut into `slabs` chunks and chunk s of all colours runs before chunk s+1 (0 or 1 = plain
  * colour order, the default).  Changing it rebuilds the per-element tables at the next
