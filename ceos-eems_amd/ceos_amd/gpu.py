@@ -52,6 +52,8 @@ EXPORTS = [
     "ceos_evaluate_device", "ceos_update_mesh_device", "ceos_device_status", "ceos_mesh_info",
     "ceos_set_schedule", "ceos_set_assembly", "ceos_set_tangent_mode",
     "ceos_timing_start", "ceos_timing_stop",
+    "ceos_rve_batch_create", "ceos_rve_batch_destroy", "ceos_rve_set_solver", "ceos_rve_reset",
+    "ceos_rve_solve", "ceos_rve_solve_device", "ceos_rve_get_fluctuation",
 ]
 
 _lib = None
@@ -97,6 +99,14 @@ def load_library(path=LIB_PATH):
     L.ceos_timing_start.argtypes = [vp, i]
     L.ceos_timing_stop.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_double),
                                    C.POINTER(C.c_int), C.POINTER(C.c_longlong)]
+    L.ceos_rve_batch_create.argtypes = [i, _ip, vp, i, _dp, i, C.POINTER(vp)]
+    L.ceos_rve_batch_destroy.argtypes = [vp]
+    L.ceos_rve_batch_destroy.restype = None
+    L.ceos_rve_set_solver.argtypes = [vp, d, d, i, d, i, i]
+    L.ceos_rve_reset.argtypes = [vp]
+    L.ceos_rve_solve.argtypes = [vp, _dp, _dp, vp, vp]
+    L.ceos_rve_solve_device.argtypes = [vp, vp, vp, vp, vp, vp]
+    L.ceos_rve_get_fluctuation.argtypes = [vp, i, _dp]
     _lib = L
     return L
 
@@ -262,3 +272,64 @@ class CeosMesh:
         st = CeosStatus()
         code = load_library().ceos_device_status(self._h, C.byref(st))
         return code, st
+
+
+class RveBatch:
+    """Batch of periodic voxel RVEs solved on the GPU (ceos_rve_*).
+
+    sides (n_rve,) even in 4..16; phase: uint8 voxel phases, RVEs concatenated (x fastest);
+    props: (2, nprops) constants of the matrix (phase 0) and inclusion (phase 1) for `model`.
+    solve(Fbar) takes (n_rve, 3, 3) macro F and returns (P (n_rve, 3, 3), A (n_rve, 9, 9) or
+    None, info (n_rve, 4)); A[r, 3J+i, 3L+k] = dP_iJ/dF_kL.  Raises CeosError if an RVE did not
+    converge (info[:, 0] = -1 marks it)."""
+
+    def __init__(self, sides, phase, model, props):
+        L = load_library()
+        self.sides = np.ascontiguousarray(sides, dtype=np.int32)
+        self.n = len(self.sides)
+        ph = np.ascontiguousarray(phase, dtype=np.uint8)
+        if ph.size != int(np.sum(self.sides.astype(np.int64) ** 3)):
+            raise ValueError("phase has %d voxels, sides need %d" % (ph.size, np.sum(self.sides.astype(np.int64) ** 3)))
+        pr = np.ascontiguousarray(props, dtype=np.float64).reshape(2, -1)
+        h = C.c_void_p()
+        _check(L.ceos_rve_batch_create(self.n, self.sides, ph.ctypes.data, int(model), pr.reshape(-1),
+                                       pr.shape[1], C.byref(h)), "ceos_rve_batch_create")
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            load_library().ceos_rve_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_solver(self, newton_rtol=1e-10, newton_atol=1e-13, newton_maxit=12, pcg_rtol=1e-10,
+                   pcg_maxit=5000, max_cutbacks=20):
+        _check(load_library().ceos_rve_set_solver(self._h, newton_rtol, newton_atol, newton_maxit, pcg_rtol,
+                                                  pcg_maxit, max_cutbacks), "ceos_rve_set_solver")
+
+    def reset(self):
+        _check(load_library().ceos_rve_reset(self._h), "ceos_rve_reset")
+
+    def solve(self, Fbar, tangent=False, check=True):
+        F = np.asarray(Fbar, dtype=np.float64).reshape(self.n, 3, 3)
+        Fc = np.ascontiguousarray(F.transpose(0, 2, 1).reshape(self.n, 9))  # column-major 9-vectors
+        P = np.zeros((self.n, 9))
+        A = np.zeros((self.n, 81)) if tangent else None
+        info = np.zeros((self.n, 4), dtype=np.int32)
+        code = load_library().ceos_rve_solve(self._h, Fc.reshape(-1), P.reshape(-1),
+                                             A.ctypes.data if tangent else None, info.ctypes.data)
+        if check:
+            _check(code, "ceos_rve_solve")
+        Pm = P.reshape(self.n, 3, 3).transpose(0, 2, 1)
+        return Pm, (A.reshape(self.n, 9, 9) if tangent else None), info
+
+    def fluctuation(self, r):
+        n = int(self.sides[r])
+        w = np.zeros(3 * n ** 3)
+        _check(load_library().ceos_rve_get_fluctuation(self._h, r, w), "ceos_rve_get_fluctuation")
+        return w

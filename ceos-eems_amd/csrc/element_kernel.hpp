@@ -157,6 +157,82 @@ __device__ __noinline__ void tangent_numerical(const double* pr, const double F0
     push_forward_up(C, F0, det3(F0), D);
 }
 
+// Phase 2 of the element evaluation, lane = node b, from the Gauss-point data in LDS (sm: gradN
+// [8][8][3], w detJ sigma [8][6] at kOffSig, w detJ D (upper, 21) [8] at kOffD):
+//   Fe_b = sum_g B_b^T sigma, the upper diagonal block K_bb (00 01 02 11 12 22) and the blocks
+//   K_cb (rows = dofs of c, cols = dofs of b) for c = (b + j + 1) mod 8, j < 4 (b < 4) or j < 3,
+//   K_cb = sum_g B_c^T D B_b + (gradN_c . sigma gradN_b) I (the geometric part for NL).
+template <bool NL, bool WANT_K>
+__device__ __forceinline__ void node_blocks(const double* sm, int b, double& fe0, double& fe1, double& fe2,
+                                            double kd[6], double ko[4][9]) {
+    const int npart = (b < 4) ? 4 : 3;
+    fe0 = 0.0;
+    fe1 = 0.0;
+    fe2 = 0.0;
+#pragma unroll
+    for (int i = 0; i < 6; ++i) kd[i] = 0.0;
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+        for (int i = 0; i < 9; ++i) ko[j][i] = 0.0;
+
+#pragma unroll 1
+    for (int gp = 0; gp < 8; ++gp) {
+        const double* Gp = sm + gp * 24;
+        const double* Sp = sm + kOffSig + gp * 6;
+        const double b0 = Gp[b * 3 + 0], b1 = Gp[b * 3 + 1], b2 = Gp[b * 3 + 2];
+        const double S0 = Sp[0], S1 = Sp[1], S2 = Sp[2], S3 = Sp[3], S4 = Sp[4], S5 = Sp[5];
+        const double sg0 = S0 * b0 + S3 * b1 + S5 * b2;
+        const double sg1 = S3 * b0 + S1 * b1 + S4 * b2;
+        const double sg2 = S5 * b0 + S4 * b1 + S2 * b2;
+        fe0 += sg0;
+        fe1 += sg1;
+        fe2 += sg2;
+        if constexpr (WANT_K) {
+            const double* Dp = sm + kOffD + gp * 21;
+            double Dl[21];
+#pragma unroll
+            for (int k = 0; k < 21; ++k) Dl[k] = Dp[k];
+#define DD(I, J) Dl[(I) <= (J) ? up_idx(I, J) : up_idx(J, I)]
+            double DB[6][3];
+#pragma unroll
+            for (int I = 0; I < 6; ++I) {
+                DB[I][0] = DD(I, 0) * b0 + DD(I, 3) * b1 + DD(I, 5) * b2;
+                DB[I][1] = DD(I, 1) * b1 + DD(I, 3) * b0 + DD(I, 4) * b2;
+                DB[I][2] = DD(I, 2) * b2 + DD(I, 4) * b1 + DD(I, 5) * b0;
+            }
+#undef DD
+            // diagonal block, upper part (k >= i)
+            {
+                const double geo = NL ? (b0 * sg0 + b1 * sg1 + b2 * sg2) : 0.0;
+                kd[0] += b0 * DB[0][0] + b1 * DB[3][0] + b2 * DB[5][0] + geo;
+                kd[1] += b0 * DB[0][1] + b1 * DB[3][1] + b2 * DB[5][1];
+                kd[2] += b0 * DB[0][2] + b1 * DB[3][2] + b2 * DB[5][2];
+                kd[3] += b1 * DB[1][1] + b0 * DB[3][1] + b2 * DB[4][1] + geo;
+                kd[4] += b1 * DB[1][2] + b0 * DB[3][2] + b2 * DB[4][2];
+                kd[5] += b2 * DB[2][2] + b1 * DB[4][2] + b0 * DB[5][2] + geo;
+            }
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                if (j < npart) {
+                    const int c = (b + j + 1) & 7;
+                    const double c0 = Gp[c * 3 + 0], c1 = Gp[c * 3 + 1], c2 = Gp[c * 3 + 2];
+                    const double geo = NL ? (c0 * sg0 + c1 * sg1 + c2 * sg2) : 0.0;
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) {
+                        ko[j][0 * 3 + k] += c0 * DB[0][k] + c1 * DB[3][k] + c2 * DB[5][k];
+                        ko[j][1 * 3 + k] += c1 * DB[1][k] + c0 * DB[3][k] + c2 * DB[4][k];
+                        ko[j][2 * 3 + k] += c2 * DB[2][k] + c1 * DB[4][k] + c0 * DB[5][k];
+                    }
+                    ko[j][0] += geo;
+                    ko[j][4] += geo;
+                    ko[j][8] += geo;
+                }
+            }
+        }
+    }
+}
+
 // NUMT: tangent modulus by central differences (a separate instantiation, so that the default
 // kernel carries no call and keeps its register allocation)
 template <int MODEL, bool NL, bool WANT_K, typename KT, bool GATHER, bool NUMT = false>
@@ -275,71 +351,10 @@ __global__ __launch_bounds__(128) void k_element(ElemParams p, int e_begin, int 
     // ---------------- phase 2: node b = lane -----------------
     const int b = lane;
     const int npart = (b < 4) ? 4 : 3;
-    double fe0 = 0.0, fe1 = 0.0, fe2 = 0.0;
+    double fe0, fe1, fe2;
     double kd[6];
     double ko[4][9];
-#pragma unroll
-    for (int i = 0; i < 6; ++i) kd[i] = 0.0;
-#pragma unroll
-    for (int j = 0; j < 4; ++j)
-#pragma unroll
-        for (int i = 0; i < 9; ++i) ko[j][i] = 0.0;
-
-#pragma unroll 1
-    for (int gp = 0; gp < 8; ++gp) {
-        const double* Gp = sm + gp * 24;
-        const double* Sp = sm + kOffSig + gp * 6;
-        const double b0 = Gp[b * 3 + 0], b1 = Gp[b * 3 + 1], b2 = Gp[b * 3 + 2];
-        const double S0 = Sp[0], S1 = Sp[1], S2 = Sp[2], S3 = Sp[3], S4 = Sp[4], S5 = Sp[5];
-        const double sg0 = S0 * b0 + S3 * b1 + S5 * b2;
-        const double sg1 = S3 * b0 + S1 * b1 + S4 * b2;
-        const double sg2 = S5 * b0 + S4 * b1 + S2 * b2;
-        fe0 += sg0;
-        fe1 += sg1;
-        fe2 += sg2;
-        if constexpr (WANT_K) {
-            const double* Dp = sm + kOffD + gp * 21;
-            double Dl[21];
-#pragma unroll
-            for (int k = 0; k < 21; ++k) Dl[k] = Dp[k];
-#define DD(I, J) Dl[(I) <= (J) ? up_idx(I, J) : up_idx(J, I)]
-            double DB[6][3];
-#pragma unroll
-            for (int I = 0; I < 6; ++I) {
-                DB[I][0] = DD(I, 0) * b0 + DD(I, 3) * b1 + DD(I, 5) * b2;
-                DB[I][1] = DD(I, 1) * b1 + DD(I, 3) * b0 + DD(I, 4) * b2;
-                DB[I][2] = DD(I, 2) * b2 + DD(I, 4) * b1 + DD(I, 5) * b0;
-            }
-#undef DD
-            // diagonal block, upper part (k >= i)
-            {
-                const double geo = NL ? (b0 * sg0 + b1 * sg1 + b2 * sg2) : 0.0;
-                kd[0] += b0 * DB[0][0] + b1 * DB[3][0] + b2 * DB[5][0] + geo;
-                kd[1] += b0 * DB[0][1] + b1 * DB[3][1] + b2 * DB[5][1];
-                kd[2] += b0 * DB[0][2] + b1 * DB[3][2] + b2 * DB[5][2];
-                kd[3] += b1 * DB[1][1] + b0 * DB[3][1] + b2 * DB[4][1] + geo;
-                kd[4] += b1 * DB[1][2] + b0 * DB[3][2] + b2 * DB[4][2];
-                kd[5] += b2 * DB[2][2] + b1 * DB[4][2] + b0 * DB[5][2] + geo;
-            }
-#pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                if (j < npart) {
-                    const int c = (b + j + 1) & 7;
-                    const double c0 = Gp[c * 3 + 0], c1 = Gp[c * 3 + 1], c2 = Gp[c * 3 + 2];
-                    const double geo = NL ? (c0 * sg0 + c1 * sg1 + c2 * sg2) : 0.0;
-#pragma unroll
-                    for (int k = 0; k < 3; ++k) {
-                        ko[j][0 * 3 + k] += c0 * DB[0][k] + c1 * DB[3][k] + c2 * DB[5][k];
-                        ko[j][1 * 3 + k] += c1 * DB[1][k] + c0 * DB[3][k] + c2 * DB[4][k];
-                        ko[j][2 * 3 + k] += c2 * DB[2][k] + c1 * DB[4][k] + c0 * DB[5][k];
-                    }
-                    ko[j][0] += geo;
-                    ko[j][4] += geo;
-                    ko[j][8] += geo;
-                }
-            }
-        }
-    }
+    node_blocks<NL, WANT_K>(sm, b, fe0, fe1, fe2, kd, ko);
     if constexpr (GATHER) {
         // stage the element record in LDS (the GP data are no longer read), then write the
         // records of the block's consecutive elements with unit-stride stores

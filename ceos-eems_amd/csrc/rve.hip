@@ -1,0 +1,275 @@
+// Host side of the batched periodic RVE solver (C ABI in include/ceos_gpu.h).
+//
+// A batch holds n_rve voxel RVEs of sides 4..16 (even), their phase arrays, the converged state
+// of each (macro F and fluctuation) and the per-workgroup scratch of the persistent solver.
+// ceos_rve_solve runs every RVE to equilibrium at its new macro F in one launch of
+// k_rve_solve (rve_kernel.hpp); queue order is by decreasing voxel count (longest first), so
+// the last RVEs to start are the cheapest.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <numeric>
+#include <vector>
+
+#include "ceos_gpu.h"
+#include "rve_kernel.hpp"
+
+using namespace ceos;
+
+#define RVE_HIP_OK(x)                                     \
+    do {                                                  \
+        if ((x) != hipSuccess) return CEOS_ERR_HIP;       \
+    } while (0)
+
+struct ceos_rve_batch {
+    int n_rve = 0;
+    int model = 0;
+    int slots = 0;
+    long long max_nodes = 0, total_nodes = 0;
+    std::vector<int> side;
+    hipStream_t stream = nullptr;
+    double newton_rtol = 1e-10, newton_atol = 1e-13, pcg_rtol = 1e-10;
+    int newton_maxit = 12, pcg_maxit = 5000, max_cutbacks = 20;
+    // device
+    int *dorder = nullptr, *dside = nullptr, *dinfo = nullptr, *dqueue = nullptr;
+    long long *dnode_off = nullptr, *dphase_off = nullptr;
+    uint8_t* dphase = nullptr;
+    double *dprops = nullptr, *dFbar = nullptr, *dFconv = nullptr, *dw = nullptr, *dP = nullptr, *dA = nullptr;
+    double *dM = nullptr, *dV = nullptr;
+};
+
+namespace {
+
+template <typename T>
+hipError_t rmalloc(T*& p, size_t n) {
+    return hipMalloc((void**)&p, sizeof(T) * (n ? n : 1));
+}
+
+template <typename T>
+void rfree(T*& p) {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+}
+
+void free_all(ceos_rve_batch* b) {
+    rfree(b->dorder); rfree(b->dside); rfree(b->dinfo); rfree(b->dqueue);
+    rfree(b->dnode_off); rfree(b->dphase_off); rfree(b->dphase);
+    rfree(b->dprops); rfree(b->dFbar); rfree(b->dFconv); rfree(b->dw); rfree(b->dP); rfree(b->dA);
+    rfree(b->dM); rfree(b->dV);
+}
+
+__global__ void k_rve_reset(int n_rve, long long nodes3, double* Fconv, double* w) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < 9LL * n_rve) {
+        const int k = (int)(t % 9);
+        Fconv[t] = (k == 0 || k == 4 || k == 8) ? 1.0 : 0.0;
+    }
+    if (t < nodes3) w[t] = 0.0;
+}
+
+__global__ void k_zero_queue(int* q) { q[0] = 0; }
+
+template <int MODEL>
+void launch(const RveParams& p, int grid, bool tangent, hipStream_t s) {
+    hipLaunchKernelGGL(k_rve_solve<MODEL>, dim3(grid), dim3(kRveThreads), 0, s, p);
+    if (tangent) {
+        hipLaunchKernelGGL(k_zero_queue, dim3(1), dim3(1), 0, s, p.queue);
+        hipLaunchKernelGGL(k_rve_tangent<MODEL>, dim3(grid), dim3(kRveThreads), 0, s, p);
+    }
+}
+
+int reset(ceos_rve_batch* b) {
+    const long long n = std::max<long long>(9LL * b->n_rve, 3 * b->total_nodes);
+    hipLaunchKernelGGL(k_rve_reset, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, b->stream, b->n_rve,
+                       3 * b->total_nodes, b->dFconv, b->dw);
+    RVE_HIP_OK(hipGetLastError());
+    RVE_HIP_OK(hipStreamSynchronize(b->stream));
+    return CEOS_OK;
+}
+
+int run(ceos_rve_batch* b, bool want_A, hipStream_t s) {
+    RveParams p;
+    p.n_rve = b->n_rve;
+    p.order = b->dorder;
+    p.side = b->dside;
+    p.node_off = b->dnode_off;
+    p.phase_off = b->dphase_off;
+    p.phase = b->dphase;
+    p.props = b->dprops;
+    p.Fbar = b->dFbar;
+    p.Fconv = b->dFconv;
+    p.w = b->dw;
+    p.P = b->dP;
+    p.A = want_A ? b->dA : nullptr;
+    p.info = b->dinfo;
+    p.Mbuf = b->dM;
+    p.Vbuf = b->dV;
+    p.max_nodes = b->max_nodes;
+    p.queue = b->dqueue;
+    p.newton_rtol = b->newton_rtol;
+    p.newton_atol = b->newton_atol;
+    p.pcg_rtol = b->pcg_rtol;
+    p.newton_maxit = b->newton_maxit;
+    p.pcg_maxit = b->pcg_maxit;
+    p.max_cutbacks = b->max_cutbacks;
+    hipLaunchKernelGGL(k_zero_queue, dim3(1), dim3(1), 0, s, b->dqueue);
+    const int grid = std::min(b->n_rve, b->slots);
+    switch (b->model) {
+        case kNH: launch<kNH>(p, grid, want_A, s); break;
+        case kCNH: launch<kCNH>(p, grid, want_A, s); break;
+        case kSVK: launch<kSVK>(p, grid, want_A, s); break;
+        default: return CEOS_ERR_ARG;
+    }
+    RVE_HIP_OK(hipGetLastError());
+    return CEOS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ceos_rve_batch_create(int n_rve, const int* side, const unsigned char* phase, int model,
+                          const double* props, int nprops, ceos_rve_batch** out) {
+    if (!out) return CEOS_ERR_ARG;
+    *out = nullptr;
+    if (n_rve <= 0 || !side || !phase || !props || nprops <= 0 || nprops > 8) return CEOS_ERR_ARG;
+    if (model != kNH && model != kCNH && model != kSVK) return CEOS_ERR_ARG;
+    for (int i = 0; i < n_rve; ++i)
+        if (side[i] < 4 || side[i] > kRveMaxSide || (side[i] & 1)) return CEOS_ERR_ARG;
+    ceos_rve_batch* b = new ceos_rve_batch();
+    b->n_rve = n_rve;
+    b->model = model;
+    b->side.assign(side, side + n_rve);
+    std::vector<long long> node_off(n_rve), phase_off(n_rve);
+    long long tot = 0;
+    for (int i = 0; i < n_rve; ++i) {
+        const long long nn = (long long)side[i] * side[i] * side[i];
+        node_off[i] = tot;
+        phase_off[i] = tot;
+        tot += nn;
+        b->max_nodes = std::max(b->max_nodes, nn);
+    }
+    b->total_nodes = tot;
+    for (long long k = 0; k < tot; ++k)
+        if (phase[k] > 1) {
+            delete b;
+            return CEOS_ERR_ARG;
+        }
+    std::vector<int> order(n_rve);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int a, int c) { return side[a] > side[c]; });
+    std::vector<double> pr(16, 0.0);
+    for (int ph = 0; ph < 2; ++ph)
+        for (int k = 0; k < nprops; ++k) pr[8 * ph + k] = props[nprops * ph + k];
+
+    int dev = 0, cus = 0;
+    int rc = CEOS_OK;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0 ||
+        hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete b;
+        return CEOS_ERR_HIP;
+    }
+    b->slots = cus;  // ~107 KB of LDS per workgroup: one per CU
+    const long long slots = std::min<long long>(b->slots, n_rve);
+    bool okm =
+        rmalloc(b->dorder, n_rve) == hipSuccess && rmalloc(b->dside, n_rve) == hipSuccess &&
+        rmalloc(b->dinfo, 4LL * n_rve) == hipSuccess && rmalloc(b->dqueue, 4) == hipSuccess &&
+        rmalloc(b->dnode_off, n_rve) == hipSuccess && rmalloc(b->dphase_off, n_rve) == hipSuccess &&
+        rmalloc(b->dphase, tot) == hipSuccess && rmalloc(b->dprops, 16) == hipSuccess &&
+        rmalloc(b->dFbar, 9LL * n_rve) == hipSuccess && rmalloc(b->dFconv, 9LL * n_rve) == hipSuccess &&
+        rmalloc(b->dw, 3 * tot) == hipSuccess && rmalloc(b->dP, 9LL * n_rve) == hipSuccess &&
+        rmalloc(b->dA, 81LL * n_rve) == hipSuccess &&
+        rmalloc(b->dM, (size_t)(243 * b->max_nodes * slots)) == hipSuccess &&
+        rmalloc(b->dV, (size_t)(kRveVecPerNode * b->max_nodes * slots)) == hipSuccess;
+    if (!okm) rc = CEOS_ERR_HIP;
+    if (rc == CEOS_OK) {
+        bool okc =
+            hipMemcpy(b->dorder, order.data(), sizeof(int) * n_rve, hipMemcpyHostToDevice) == hipSuccess &&
+            hipMemcpy(b->dside, side, sizeof(int) * n_rve, hipMemcpyHostToDevice) == hipSuccess &&
+            hipMemcpy(b->dnode_off, node_off.data(), sizeof(long long) * n_rve, hipMemcpyHostToDevice) == hipSuccess &&
+            hipMemcpy(b->dphase_off, phase_off.data(), sizeof(long long) * n_rve, hipMemcpyHostToDevice) == hipSuccess &&
+            hipMemcpy(b->dphase, phase, (size_t)tot, hipMemcpyHostToDevice) == hipSuccess &&
+            hipMemcpy(b->dprops, pr.data(), sizeof(double) * 16, hipMemcpyHostToDevice) == hipSuccess;
+        if (!okc) rc = CEOS_ERR_HIP;
+    }
+    if (rc == CEOS_OK) rc = reset(b);
+    if (rc != CEOS_OK) {
+        free_all(b);
+        if (b->stream) (void)hipStreamDestroy(b->stream);
+        delete b;
+        return rc;
+    }
+    *out = b;
+    return CEOS_OK;
+}
+
+void ceos_rve_batch_destroy(ceos_rve_batch* b) {
+    if (!b) return;
+    (void)hipStreamSynchronize(b->stream);
+    free_all(b);
+    (void)hipStreamDestroy(b->stream);
+    delete b;
+}
+
+int ceos_rve_set_solver(ceos_rve_batch* b, double newton_rtol, double newton_atol, int newton_maxit,
+                        double pcg_rtol, int pcg_maxit, int max_cutbacks) {
+    if (!b || !(newton_rtol > 0.0) || !(newton_atol >= 0.0) || newton_maxit < 1 || newton_maxit > 100 ||
+        !(pcg_rtol > 0.0) || pcg_maxit < 1 || max_cutbacks < 0 || max_cutbacks > 60)
+        return CEOS_ERR_ARG;
+    b->newton_rtol = newton_rtol;
+    b->newton_atol = newton_atol;
+    b->newton_maxit = newton_maxit;
+    b->pcg_rtol = pcg_rtol;
+    b->pcg_maxit = pcg_maxit;
+    b->max_cutbacks = max_cutbacks;
+    return CEOS_OK;
+}
+
+int ceos_rve_reset(ceos_rve_batch* b) {
+    if (!b) return CEOS_ERR_ARG;
+    return reset(b);
+}
+
+int ceos_rve_solve(ceos_rve_batch* b, const double* Fbar, double* P, double* A, int* info) {
+    if (!b || !Fbar || !P) return CEOS_ERR_ARG;
+    const int n = b->n_rve;
+    RVE_HIP_OK(hipMemcpyAsync(b->dFbar, Fbar, sizeof(double) * 9 * n, hipMemcpyHostToDevice, b->stream));
+    int rc = run(b, A != nullptr, b->stream);
+    if (rc) return rc;
+    RVE_HIP_OK(hipMemcpyAsync(P, b->dP, sizeof(double) * 9 * n, hipMemcpyDeviceToHost, b->stream));
+    if (A) RVE_HIP_OK(hipMemcpyAsync(A, b->dA, sizeof(double) * 81 * n, hipMemcpyDeviceToHost, b->stream));
+    std::vector<int> inf(4 * (size_t)n);
+    RVE_HIP_OK(hipMemcpyAsync(inf.data(), b->dinfo, sizeof(int) * 4 * n, hipMemcpyDeviceToHost, b->stream));
+    RVE_HIP_OK(hipStreamSynchronize(b->stream));
+    if (info) std::memcpy(info, inf.data(), sizeof(int) * 4 * n);
+    for (int i = 0; i < n; ++i)
+        if (inf[4 * i] != 0) return CEOS_ERR_LOCAL_NR;
+    return CEOS_OK;
+}
+
+int ceos_rve_solve_device(ceos_rve_batch* b, const double* dFbar, double* dP, double* dA, int* dinfo, void* stream) {
+    if (!b || !dFbar || !dP) return CEOS_ERR_ARG;
+    hipStream_t s = stream ? (hipStream_t)stream : b->stream;
+    const int n = b->n_rve;
+    RVE_HIP_OK(hipMemcpyAsync(b->dFbar, dFbar, sizeof(double) * 9 * n, hipMemcpyDeviceToDevice, s));
+    int rc = run(b, dA != nullptr, s);
+    if (rc) return rc;
+    RVE_HIP_OK(hipMemcpyAsync(dP, b->dP, sizeof(double) * 9 * n, hipMemcpyDeviceToDevice, s));
+    if (dA) RVE_HIP_OK(hipMemcpyAsync(dA, b->dA, sizeof(double) * 81 * n, hipMemcpyDeviceToDevice, s));
+    if (dinfo) RVE_HIP_OK(hipMemcpyAsync(dinfo, b->dinfo, sizeof(int) * 4 * n, hipMemcpyDeviceToDevice, s));
+    return CEOS_OK;
+}
+
+int ceos_rve_get_fluctuation(ceos_rve_batch* b, int rve, double* w) {
+    if (!b || !w || rve < 0 || rve >= b->n_rve) return CEOS_ERR_ARG;
+    long long off = 0;
+    for (int i = 0; i < rve; ++i) off += (long long)b->side[i] * b->side[i] * b->side[i];
+    const long long nn = (long long)b->side[rve] * b->side[rve] * b->side[rve];
+    RVE_HIP_OK(hipStreamSynchronize(b->stream));
+    RVE_HIP_OK(hipMemcpy(w, b->dw + 3 * off, sizeof(double) * 3 * nn, hipMemcpyDeviceToHost));
+    return CEOS_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,754 @@
+// Batched periodic RVE solver: every RVE of a batch solved to equilibrium under its macro
+// deformation gradient in ONE persistent launch, one workgroup per RVE at a time.
+//
+// Work of the reference it replaces (file:line relative to SourceCode/):
+//   periodic system of equations, K_red = T^T K T, R_red = T^T R, expansion of the fluctuation
+//                      FEM/Modules/ModMultiscalePeriodicFEMSoE.f90:77-222
+//   affine part U = (F - I) Y and the fixed corner
+//                      MultiScale/ModMultiscaleBoundaryConditions.f90:200-289, 632-725
+//   Newton (max-abs norm, relative test)  Tools/MathTools/NonlinearSolver/ModNewtonRaphsonFull.f90:74-163
+//   golden-ratio cut-back of the load      FEM/ModFEMAnalysis.f90:1270-1340
+//   homogenized first Piola stress        MultiScale/ModMultiscaleAnalysis.f90:140-329
+//   the PARDISO solve is replaced by block-Jacobi preconditioned CG.
+//
+// Formulation on the torus.  The periodic voxel mesh of side n has n^3 nodes (the master nodes
+// of the reference's T): node (i,j,k) = i + n (j + n k); element (i,j,k) joins the nodes
+// ((i,j,k) + delta_a) mod n, delta_a the hex8 corner offsets.  Unknown: the fluctuation w of
+// every torus node, the displacement of element corner a being U_a = (F - I) Y_a + w(node_a)
+// with Y_a the unwrapped corner position (i,j,k) + delta_a times h = 1/n.  Assembling the
+// element blocks on the torus is T^T K T; node 0 (the 8 cube corners) has w = 0 (identity rows).
+//
+// Storage of the reduced tangent, per RVE being solved: the 27 neighbour blocks of every node,
+// M[(o*9 + 3a + b) * nn + i] = K(3i+a, 3 nbr(i,o)+b), o = (dx+1) + 3(dy+1) + 9(dz+1).
+// The assembly colours the voxels by coordinate parity (8 colours, n even), so the elements of
+// one colour share no node and their read-add-write of M and R needs no atomics.
+//
+// Workgroup of 256 threads = 4 waves.  Assembly: 32 elements per round, 8 lanes each, the same
+// three phases as the element kernel (nodes -> LDS; lane = Gauss point; lane = node).
+// CG: p lives in LDS (n <= 16: 98 KB), everything else in the workgroup's slice of HBM scratch;
+// thread t handles nodes t, t+256, ...; dot products reduce over the workgroup.
+#pragma once
+#include "element_kernel.hpp"
+
+namespace ceos {
+
+constexpr int kRveThreads = 256;
+constexpr int kRveElemPerRound = 32;
+constexpr int kRveMaxSide = 16;
+constexpr int kRveLds = kRveElemPerRound * kLdsPerElem;  // 13056 doubles >= 3 * 16^3
+constexpr int kRveVecPerNode = 51;  // w 3, R 3, x 3, r 3, q 3, z 3, Minv 6, B 27
+static_assert(kRveLds >= 3 * kRveMaxSide * kRveMaxSide * kRveMaxSide, "p must fit in LDS");
+
+struct RveParams {
+    int n_rve;
+    const int* order;             // RVE ids in queue order (largest first)
+    const int* side;              // [n_rve]
+    const long long* node_off;    // [n_rve] first node of each RVE in w
+    const long long* phase_off;   // [n_rve] first voxel of each RVE in phase
+    const uint8_t* phase;         // voxel phase (0 matrix, 1 inclusion), x fastest
+    const double* props;          // [2][8] material constants per phase
+    const double* Fbar;           // [n_rve][9] macro F, column-major (3J + i)
+    double* Fconv;                // [n_rve][9] last converged macro F
+    double* w;                    // [nodes][3] converged fluctuation
+    double* P;                    // [n_rve][9] homogenized first Piola stress, column-major
+    double* A;                    // [n_rve][81] dP/dF ([3J+i][3L+k]) or nullptr
+    int* info;                    // [n_rve][4] status, Newton iterations, CG iterations, cut-backs
+    double* Mbuf;                 // [slots][243 * max_nodes]
+    double* Vbuf;                 // [slots][kRveVecPerNode * max_nodes]
+    long long max_nodes;
+    int* queue;                   // next queue position (zeroed before the launch)
+    double newton_rtol, newton_atol, pcg_rtol;
+    int newton_maxit, pcg_maxit, max_cutbacks;
+};
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+// workgroup sums of N values; every thread gets the totals (red: >= 4 N doubles of LDS)
+template <int N>
+__device__ __forceinline__ void wg_sum(double (&v)[N], double* red) {
+#pragma unroll
+    for (int k = 0; k < N; ++k) v[k] = wave_sum(v[k]);
+    __syncthreads();  // the previous reduction's readers are done with red
+    if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+        for (int k = 0; k < N; ++k) red[(threadIdx.x >> 6) * N + k] = v[k];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < N; ++k) v[k] = (red[k] + red[N + k]) + (red[2 * N + k] + red[3 * N + k]);
+}
+
+__device__ __forceinline__ double wg_max(double v, double* red) {
+    v = wave_max(v);
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    return fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+}
+
+// hex8 corner offsets (---,+--,++-,-+-,--+,+-+,+++,-++)
+__device__ __forceinline__ int corner_dx(int a) { return ((a + 1) >> 1) & 1; }
+__device__ __forceinline__ int corner_dy(int a) { return (a >> 1) & 1; }
+__device__ __forceinline__ int corner_dz(int a) { return (a >> 2) & 1; }
+__device__ __forceinline__ int nbr_offset(int a, int b) {  // o of node b seen from node a
+    return (corner_dx(b) - corner_dx(a) + 1) + 3 * (corner_dy(b) - corner_dy(a) + 1) +
+           9 * (corner_dz(b) - corner_dz(a) + 1);
+}
+
+struct RveCtx {
+    int n, nn;
+    double h;
+    const uint8_t* phase;
+    double* wconv;  // converged fluctuation of this RVE (global state)
+    double* M;
+    double *wc, *R, *x, *r, *q, *z, *Minv, *B;
+};
+
+// Element pass over the whole RVE at the macro gradient Fa: zero and assemble M (if WANT_K) and
+// R.  Returns true if some element has detJ <= 1e-13.
+template <int MODEL, bool WANT_K>
+__device__ bool rve_assemble(const RveCtx& c, const RveParams& p, const double* Fa, double* lds,
+                             int (*snode)[8]) {
+    const int tid = threadIdx.x;
+    const long long nm = 243LL * c.nn;
+    if constexpr (WANT_K) {
+        for (long long k = tid; k < nm; k += kRveThreads) c.M[k] = 0.0;
+    }
+    for (int k = tid; k < 3 * c.nn; k += kRveThreads) c.R[k] = 0.0;
+    bool bad = false;
+    __syncthreads();
+    const int n = c.n, half = n >> 1, ncol = half * half * half;
+    const int lane = tid & 7, le = tid >> 3;
+    double* sm = lds + le * kLdsPerElem;
+    for (int color = 0; color < 8; ++color) {
+        const int cx = color & 1, cy = (color >> 1) & 1, cz = color >> 2;
+        for (int base = 0; base < ncol; base += kRveElemPerRound) {
+            const int el = base + le;
+            const bool active = el < ncol;
+            const int ex = 2 * (el % half) + cx, ey = 2 * ((el / half) % half) + cy, ez = 2 * (el / (half * half)) + cz;
+            // phase 0: corner a = lane
+            {
+                const int gx = ex + corner_dx(lane), gy = ey + corner_dy(lane), gz = ez + corner_dz(lane);
+                const int node = active ? ((gx % n) + n * ((gy % n) + n * (gz % n))) : 0;
+                snode[le][lane] = node;
+                const double Y[3] = {gx * c.h, gy * c.h, gz * c.h};
+#pragma unroll
+                for (int i = 0; i < 3; ++i) {
+                    double u = active ? c.wc[3 * node + i] : 0.0;
+#pragma unroll
+                    for (int J = 0; J < 3; ++J) u += (Fa[3 * i + J] - (i == J ? 1.0 : 0.0)) * Y[J];
+                    sm[lane * 9 + i] = Y[i];
+                    sm[lane * 9 + 3 + i] = Y[i] + u;
+                    sm[lane * 9 + 6 + i] = u;
+                }
+            }
+            __syncthreads();
+            // phase 1: Gauss point g = lane
+            const int g = lane;
+            double gxs[8][3];
+            double s[6], D[21];
+            double wdet;
+            {
+                double dN[8][3];
+                hex8_dshape_gp(g, dN);
+                double cc[8][3];
+#pragma unroll
+                for (int a = 0; a < 8; ++a)
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) cc[a][k] = sm[a * 9 + k];
+                double gX[8][3];
+                const double detX = hex8_grad(dN, cc, gX);
+                double F[3][3];
+#pragma unroll
+                for (int i = 0; i < 3; ++i)
+#pragma unroll
+                    for (int j = 0; j < 3; ++j) {
+                        double acc = 0.0;
+#pragma unroll
+                        for (int a = 0; a < 8; ++a) acc += gX[a][j] * sm[a * 9 + 6 + i];
+                        F[i][j] = acc + (i == j ? 1.0 : 0.0);
+                    }
+#pragma unroll
+                for (int a = 0; a < 8; ++a)
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) cc[a][k] = sm[a * 9 + 3 + k];
+                const double detx = hex8_grad(dN, cc, gxs);
+                bad |= active && !(detx > kDetTol && detX > kDetTol);
+                wdet = detx;
+                const int ph = active ? c.phase[ex + n * (ey + n * ez)] : 0;
+                double nw[1];
+                material<MODEL>(p.props + 8 * ph, F, 0.0, nullptr, s, D, nw);
+                if (!active) {
+#pragma unroll
+                    for (int k = 0; k < 6; ++k) s[k] = 0.0;
+#pragma unroll
+                    for (int k = 0; k < 21; ++k) D[k] = 0.0;
+                    wdet = 0.0;
+                }
+            }
+            __syncthreads();  // node data no longer needed
+#pragma unroll
+            for (int a = 0; a < 8; ++a)
+#pragma unroll
+                for (int k = 0; k < 3; ++k) sm[g * 24 + a * 3 + k] = gxs[a][k];
+#pragma unroll
+            for (int k = 0; k < 6; ++k) sm[kOffSig + g * 6 + k] = s[k] * wdet;
+            if constexpr (WANT_K) {
+#pragma unroll
+                for (int k = 0; k < 21; ++k) sm[kOffD + g * 21 + k] = D[k] * wdet;
+            }
+            __syncthreads();
+            // phase 2: node b = lane; read-add-write of R and M (no other element of the colour
+            // shares a node; each block slot is written by exactly one lane of the element)
+            {
+                const int b = lane;
+                const int npart = (b < 4) ? 4 : 3;
+                double fe0, fe1, fe2, kd[6], ko[4][9];
+                node_blocks<true, WANT_K>(sm, b, fe0, fe1, fe2, kd, ko);
+                if (active) {
+                    const int nb = snode[le][b];
+                    double r0 = c.R[3 * nb], r1 = c.R[3 * nb + 1], r2 = c.R[3 * nb + 2];
+                    c.R[3 * nb] = r0 + fe0;
+                    c.R[3 * nb + 1] = r1 + fe1;
+                    c.R[3 * nb + 2] = r2 + fe2;
+                    if constexpr (WANT_K) {
+                        const int nn = c.nn;
+                        double* M = c.M;
+                        {
+                            const double full[9] = {kd[0], kd[1], kd[2], kd[1], kd[3], kd[4], kd[2], kd[4], kd[5]};
+                            double old[9];
+#pragma unroll
+                            for (int k = 0; k < 9; ++k) old[k] = M[(13 * 9 + k) * nn + nb];
+#pragma unroll
+                            for (int k = 0; k < 9; ++k) M[(13 * 9 + k) * nn + nb] = old[k] + full[k];
+                        }
+                        // pair (c, b): K_cb into the row of c, K_bc = K_cb^T into the row of b
+#pragma unroll
+                        for (int j = 0; j < 4; ++j) {
+                            if (j < npart) {
+                                const int cn = (b + j + 1) & 7;
+                                const int nc = snode[le][cn];
+                                const int oc = nbr_offset(cn, b) * 9 * nn + nc;
+                                const int ob = nbr_offset(b, cn) * 9 * nn + nb;
+                                double o1[9], o2[9];
+#pragma unroll
+                                for (int k = 0; k < 9; ++k) {
+                                    o1[k] = M[oc + k * nn];
+                                    o2[k] = M[ob + k * nn];
+                                }
+#pragma unroll
+                                for (int r = 0; r < 3; ++r)
+#pragma unroll
+                                    for (int t = 0; t < 3; ++t) {
+                                        M[oc + (3 * r + t) * nn] = o1[3 * r + t] + ko[j][3 * r + t];
+                                        M[ob + (3 * r + t) * nn] = o2[3 * r + t] + ko[j][3 * t + r];
+                                    }
+                            }
+                        }
+                    }
+                }
+            }
+            __syncthreads();  // LDS and this round's writes before the next round
+        }
+    }
+    return bad;
+}
+
+// y = M v for node i (27 neighbour blocks; v in LDS); node 0 is the fixed corner (identity row,
+// its column never multiplies a nonzero since v_0 = 0)
+__device__ __forceinline__ void rve_row(const RveCtx& c, int i, const double* v, double y[3]) {
+    const int n = c.n, nn = c.nn;
+    const int ix = i % n, iy = (i / n) % n, iz = i / (n * n);
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+#pragma unroll 1
+    for (int dz = -1; dz <= 1; ++dz) {
+        const int zz = iz + dz < 0 ? n - 1 : (iz + dz == n ? 0 : iz + dz);
+#pragma unroll
+        for (int dy = -1; dy <= 1; ++dy) {
+            const int yy = iy + dy < 0 ? n - 1 : (iy + dy == n ? 0 : iy + dy);
+#pragma unroll
+            for (int dx = -1; dx <= 1; ++dx) {
+                const int xx = ix + dx < 0 ? n - 1 : (ix + dx == n ? 0 : ix + dx);
+                const int j = xx + n * (yy + n * zz);
+                const int o = (dx + 1) + 3 * (dy + 1) + 9 * (dz + 1);
+                const double* m = c.M + (long long)o * 9 * nn + i;
+                const double v0 = v[3 * j], v1 = v[3 * j + 1], v2 = v[3 * j + 2];
+                a0 += m[0] * v0 + m[(long long)nn] * v1 + m[2LL * nn] * v2;
+                a1 += m[3LL * nn] * v0 + m[4LL * nn] * v1 + m[5LL * nn] * v2;
+                a2 += m[6LL * nn] * v0 + m[7LL * nn] * v1 + m[8LL * nn] * v2;
+            }
+        }
+    }
+    y[0] = a0;
+    y[1] = a1;
+    y[2] = a2;
+}
+
+// block-Jacobi: inverse of every node's diagonal block (symmetric, upper 6 values); node 0 -> 0
+__device__ void rve_precond(const RveCtx& c) {
+    const long long nn = c.nn;
+    for (int i = threadIdx.x; i < c.nn; i += kRveThreads) {
+        double Bm[3][3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+#pragma unroll
+            for (int b = 0; b < 3; ++b) Bm[a][b] = c.M[(13LL * 9 + 3 * a + b) * nn + i];
+        double Bi[3][3];
+        inv3(Bm, det3(Bm), Bi);
+        const bool fixed = i == 0;
+        double* mi = c.Minv + 6 * i;
+        mi[0] = fixed ? 0.0 : Bi[0][0];
+        mi[1] = fixed ? 0.0 : 0.5 * (Bi[0][1] + Bi[1][0]);
+        mi[2] = fixed ? 0.0 : 0.5 * (Bi[0][2] + Bi[2][0]);
+        mi[3] = fixed ? 0.0 : Bi[1][1];
+        mi[4] = fixed ? 0.0 : 0.5 * (Bi[1][2] + Bi[2][1]);
+        mi[5] = fixed ? 0.0 : Bi[2][2];
+    }
+}
+
+__device__ __forceinline__ void apply_minv(const double* mi, const double r[3], double z[3]) {
+    z[0] = mi[0] * r[0] + mi[1] * r[1] + mi[2] * r[2];
+    z[1] = mi[1] * r[0] + mi[3] * r[1] + mi[4] * r[2];
+    z[2] = mi[2] * r[0] + mi[4] * r[1] + mi[5] * r[2];
+}
+
+// preconditioned CG on the reduced system: M x = sign * rhs (node 0 fixed).  p in LDS.
+// Returns the iterations used.
+__device__ int rve_pcg(const RveCtx& c, const RveParams& p, const double* rhs, double sign, double* pl,
+                       double* red) {
+    const int tid = threadIdx.x;
+    double acc[2] = {0.0, 0.0};
+    for (int i = tid; i < c.nn; i += kRveThreads) {
+        double r[3], z[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) r[k] = i == 0 ? 0.0 : sign * rhs[3 * i + k];
+        apply_minv(c.Minv + 6 * i, r, z);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            c.x[3 * i + k] = 0.0;
+            c.r[3 * i + k] = r[k];
+            pl[3 * i + k] = z[k];
+            acc[0] += r[k] * z[k];
+            acc[1] += r[k] * r[k];
+        }
+    }
+    wg_sum<2>(acc, red);
+    double rz = acc[0];
+    const double tol2 = p.pcg_rtol * p.pcg_rtol * acc[1];
+    if (!(acc[1] > 0.0)) return 0;
+    int it = 0;
+    while (it < p.pcg_maxit) {
+        ++it;
+        __syncthreads();  // p complete
+        double pq[1] = {0.0};
+        for (int i = tid; i < c.nn; i += kRveThreads) {
+            double y[3];
+            if (i == 0) {
+                y[0] = y[1] = y[2] = 0.0;
+            } else {
+                rve_row(c, i, pl, y);
+            }
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                c.q[3 * i + k] = y[k];
+                pq[0] += pl[3 * i + k] * y[k];
+            }
+        }
+        wg_sum<1>(pq, red);
+        if (!(pq[0] > 0.0)) break;  // not positive definite along p
+        const double alpha = rz / pq[0];
+        double s2[2] = {0.0, 0.0};
+        for (int i = tid; i < c.nn; i += kRveThreads) {
+            double r[3], z[3];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                c.x[3 * i + k] += alpha * pl[3 * i + k];
+                r[k] = c.r[3 * i + k] - alpha * c.q[3 * i + k];
+                c.r[3 * i + k] = r[k];
+            }
+            apply_minv(c.Minv + 6 * i, r, z);
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                c.z[3 * i + k] = z[k];
+                s2[0] += r[k] * r[k];
+                s2[1] += r[k] * z[k];
+            }
+        }
+        wg_sum<2>(s2, red);
+        if (s2[0] <= tol2) break;
+        const double beta = s2[1] / rz;
+        rz = s2[1];
+        for (int i = tid; i < c.nn; i += kRveThreads) {
+#pragma unroll
+            for (int k = 0; k < 3; ++k) pl[3 * i + k] = c.z[3 * i + k] + beta * pl[3 * i + k];
+        }
+    }
+    __syncthreads();
+    return it;
+}
+
+// engineering strain (11,22,33,12,23,13) of grad_x Phi_I = e_i (x) g_J, I = 3J + i
+__device__ __forceinline__ void mode_strain(const double Fi[3][3], int I, double e[6]) {
+    const int J = I / 3, i = I % 3;
+    const double g0 = Fi[J][0], g1 = Fi[J][1], g2 = Fi[J][2];
+    e[0] = i == 0 ? g0 : 0.0;
+    e[1] = i == 1 ? g1 : 0.0;
+    e[2] = i == 2 ? g2 : 0.0;
+    e[3] = (i == 0 ? g1 : 0.0) + (i == 1 ? g0 : 0.0);
+    e[4] = (i == 1 ? g2 : 0.0) + (i == 2 ? g1 : 0.0);
+    e[5] = (i == 0 ? g2 : 0.0) + (i == 2 ? g0 : 0.0);
+}
+
+// Gauss-point sums at the converged state, per thread:
+//   Pacc: P = J sigma F^-T times the reference volume weight (column-major) and the reference
+//         volume (HomogenizeStress, MultiScale/ModMultiscaleAnalysis.f90:140-329);
+//   T1 (WANT_T): the homogenized tangent part Phi^T K Phi for the affine modes
+//         Phi_(iJ) = e_i Y_J: grad_x Phi_(iJ) = e_i (x) g_J with g_J row J of F^-1, so
+//         (Phi_I^T K Phi_K) = sum_gp w detJ [eps(I)^T D eps(K) + delta_ik g_J . sigma g_L],
+//         45 upper entries (I <= K, I = 3J + i).
+template <int MODEL, bool WANT_T>
+__device__ void rve_gp_pass(const RveCtx& c, const RveParams& p, const double* Fa, double* lds, double Pacc[10],
+                            double T1[45]) {
+    const int tid = threadIdx.x;
+    const int n = c.n, nel = n * n * n;
+    const int lane = tid & 7, le = tid >> 3;
+    double* sm = lds + le * kLdsPerElem;
+#pragma unroll
+    for (int k = 0; k < 45; ++k) T1[k] = 0.0;
+#pragma unroll
+    for (int k = 0; k < 10; ++k) Pacc[k] = 0.0;
+    for (int base = 0; base < nel; base += kRveElemPerRound) {
+        const int el = base + le;
+        const bool active = el < nel;
+        const int ex = el % n, ey = (el / n) % n, ez = el / (n * n);
+        {
+            const int gx = ex + corner_dx(lane), gy = ey + corner_dy(lane), gz = ez + corner_dz(lane);
+            const int node = active ? ((gx % n) + n * ((gy % n) + n * (gz % n))) : 0;
+            const double Y[3] = {gx * c.h, gy * c.h, gz * c.h};
+#pragma unroll
+            for (int i = 0; i < 3; ++i) {
+                double u = active ? c.wc[3 * node + i] : 0.0;
+#pragma unroll
+                for (int J = 0; J < 3; ++J) u += (Fa[3 * i + J] - (i == J ? 1.0 : 0.0)) * Y[J];
+                sm[lane * 9 + i] = Y[i];
+                sm[lane * 9 + 3 + i] = Y[i] + u;
+                sm[lane * 9 + 6 + i] = u;
+            }
+        }
+        __syncthreads();
+        if (active) {
+            const int g = lane;
+            double dN[8][3];
+            hex8_dshape_gp(g, dN);
+            double cc[8][3], gX[8][3], gx8[8][3];
+#pragma unroll
+            for (int a = 0; a < 8; ++a)
+#pragma unroll
+                for (int k = 0; k < 3; ++k) cc[a][k] = sm[a * 9 + k];
+            const double detX = hex8_grad(dN, cc, gX);
+            double F[3][3];
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+#pragma unroll
+                for (int j = 0; j < 3; ++j) {
+                    double acc = 0.0;
+#pragma unroll
+                    for (int a = 0; a < 8; ++a) acc += gX[a][j] * sm[a * 9 + 6 + i];
+                    F[i][j] = acc + (i == j ? 1.0 : 0.0);
+                }
+#pragma unroll
+            for (int a = 0; a < 8; ++a)
+#pragma unroll
+                for (int k = 0; k < 3; ++k) cc[a][k] = sm[a * 9 + 3 + k];
+            const double detx = hex8_grad(dN, cc, gx8);
+            const int ph = c.phase[ex + n * (ey + n * ez)];
+            double s[6], D[21], nw[1];
+            material<MODEL>(p.props + 8 * ph, F, 0.0, nullptr, s, D, nw);
+            {
+                double P3[3][3];
+                first_piola(s, F, P3);
+#pragma unroll
+                for (int J = 0; J < 3; ++J)
+#pragma unroll
+                    for (int i = 0; i < 3; ++i) Pacc[3 * J + i] += P3[i][J] * detX;
+                Pacc[9] += detX;
+            }
+            if constexpr (WANT_T) {
+            double Fi[3][3];
+            inv3(F, det3(F), Fi);
+            double gsg[3][3];  // g_J . sigma g_L
+            {
+                const double sg[3][3] = {{s[0], s[3], s[5]}, {s[3], s[1], s[4]}, {s[5], s[4], s[2]}};
+#pragma unroll
+                for (int J = 0; J < 3; ++J)
+#pragma unroll
+                    for (int L = 0; L < 3; ++L) {
+                        double acc = 0.0;
+#pragma unroll
+                        for (int a = 0; a < 3; ++a)
+#pragma unroll
+                            for (int b2 = 0; b2 < 3; ++b2) acc += Fi[J][a] * sg[a][b2] * Fi[L][b2];
+                        gsg[J][L] = acc;
+                    }
+            }
+            int k45 = 0;
+#pragma unroll
+            for (int I = 0; I < 9; ++I) {
+                double eI[6], De[6];
+                mode_strain(Fi, I, eI);
+#pragma unroll
+                for (int a = 0; a < 6; ++a) {
+                    double acc = 0.0;
+#pragma unroll
+                    for (int b2 = 0; b2 < 6; ++b2) acc += D[a <= b2 ? up_idx(a, b2) : up_idx(b2, a)] * eI[b2];
+                    De[a] = acc;
+                }
+#pragma unroll
+                for (int K = I; K < 9; ++K) {
+                    double eK[6];
+                    mode_strain(Fi, K, eK);
+                    double acc = 0.0;
+#pragma unroll
+                    for (int a = 0; a < 6; ++a) acc += De[a] * eK[a];
+                    if ((I % 3) == (K % 3)) acc += gsg[I / 3][K / 3];
+                    T1[k45++] += acc * detx;
+                }
+            }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// b_(kL) = T^T K Phi_(kL) for the 9 affine modes: row i of M against the mode's values at the
+// neighbour offsets (translation invariance of K removes the absolute position); node 0 -> 0.
+// B[(3L + k)][i][a] = sum_o M[o][a][k](i) d_L(o).
+__device__ void rve_affine_rhs(const RveCtx& c) {
+    const long long nn = c.nn;
+    for (int i = threadIdx.x; i < c.nn; i += kRveThreads) {
+        double bb[9][3];
+#pragma unroll
+        for (int m = 0; m < 9; ++m) bb[m][0] = bb[m][1] = bb[m][2] = 0.0;
+        if (i != 0) {
+            for (int o = 0; o < 27; ++o) {
+                const double d[3] = {((o % 3) - 1) * c.h, (((o / 3) % 3) - 1) * c.h, ((o / 9) - 1) * c.h};
+#pragma unroll
+                for (int a = 0; a < 3; ++a)
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) {
+                        const double m = c.M[((long long)o * 9 + 3 * a + k) * nn + i];
+#pragma unroll
+                        for (int L = 0; L < 3; ++L) bb[3 * L + k][a] += m * d[L];
+                    }
+            }
+        }
+#pragma unroll
+        for (int m = 0; m < 9; ++m)
+#pragma unroll
+            for (int a = 0; a < 3; ++a) c.B[(long long)m * 3 * nn + 3 * i + a] = bb[m][a];
+    }
+}
+
+// the queue position's RVE and the workgroup's scratch slice
+__device__ __forceinline__ RveCtx rve_ctx(const RveParams& p, int rid, long long slot) {
+    RveCtx c;
+    c.n = p.side[rid];
+    c.nn = c.n * c.n * c.n;
+    c.h = 1.0 / c.n;
+    c.phase = p.phase + p.phase_off[rid];
+    c.wconv = p.w + 3 * p.node_off[rid];
+    c.M = p.Mbuf + slot * 243 * p.max_nodes;
+    double* V = p.Vbuf + slot * kRveVecPerNode * p.max_nodes;
+    const long long mn = p.max_nodes;
+    c.wc = V;
+    c.R = V + 3 * mn;
+    c.x = V + 6 * mn;
+    c.r = V + 9 * mn;
+    c.q = V + 12 * mn;
+    c.z = V + 15 * mn;
+    c.Minv = V + 18 * mn;
+    c.B = V + 24 * mn;
+    return c;
+}
+
+template <int MODEL>
+__global__ __launch_bounds__(kRveThreads) void k_rve_solve(RveParams p) {
+    __shared__ double lds[kRveLds];
+    __shared__ double red[4 * 45];
+    __shared__ int snode[kRveElemPerRound][8];
+    __shared__ int sq;
+    __shared__ double sFa[9];
+    const int tid = threadIdx.x;
+    const long long slot = blockIdx.x;
+    const double gr = 1.6180339887498948482;  // (1 + sqrt 5) / 2, FEM/ModFEMAnalysis.f90:1172
+    for (;;) {
+        if (tid == 0) sq = atomicAdd(p.queue, 1);
+        __syncthreads();
+        const int qi = sq;
+        __syncthreads();
+        if (qi >= p.n_rve) break;
+        const int rid = p.order[qi];
+        RveCtx c = rve_ctx(p, rid, slot);
+        for (int k = tid; k < 3 * c.nn; k += kRveThreads) c.wc[k] = c.wconv[k];
+        __syncthreads();
+
+        double alpha_min = 0.0, alpha = 1.0;
+        int cut = 0, newton_total = 0, pcg_total = 0;
+        bool done = false, failed = false;
+        for (int guard = 0; guard < 64 && !done && !failed; ++guard) {
+            // macro F of the substep, row-major in LDS (the converged F of the RVE is only
+            // updated after the loop)
+            if (tid < 9) {
+                const int i = tid / 3, J = tid % 3;
+                const double f0 = p.Fconv[9 * rid + 3 * J + i], ft = p.Fbar[9 * rid + 3 * J + i];
+                sFa[tid] = alpha == 1.0 ? ft : f0 + alpha * (ft - f0);
+            }
+            __syncthreads();
+            // Newton on the reduced system
+            bool ok = false;
+            double nr0 = 0.0;
+            for (int it = 0; it <= p.newton_maxit; ++it) {
+                const bool bad = rve_assemble<MODEL, true>(c, p, sFa, lds, snode);
+                double rmax = 0.0;
+                for (int i = tid; i < c.nn; i += kRveThreads) {
+                    if (i == 0) continue;
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) rmax = fmax(rmax, fabs(c.R[3 * i + k]));
+                }
+                double flags[1] = {bad ? 1.0 : 0.0};
+                wg_sum<1>(flags, red);
+                const double nr = wg_max(rmax, red);
+                if (flags[0] > 0.0 || !(nr <= 1.0e300)) break;
+                if (it == 0) nr0 = nr;
+                newton_total += it == 0 ? 0 : 1;
+                if (nr <= fmax(p.newton_rtol * nr0, p.newton_atol)) {
+                    ok = true;
+                    break;
+                }
+                if (it == p.newton_maxit) break;
+                rve_precond(c);
+                __syncthreads();
+                pcg_total += rve_pcg(c, p, c.R, -1.0, lds, red);
+                for (int k = tid; k < 3 * c.nn; k += kRveThreads) c.wc[k] += c.x[k];
+                __syncthreads();
+            }
+            if (!ok) {
+                for (int k = tid; k < 3 * c.nn; k += kRveThreads) c.wc[k] = c.wconv[k];
+                __syncthreads();
+                alpha = alpha_min + (1.0 - 1.0 / gr) * (alpha - alpha_min);
+                if (++cut > p.max_cutbacks) failed = true;
+                continue;
+            }
+            for (int k = tid; k < 3 * c.nn; k += kRveThreads) c.wconv[k] = c.wc[k];
+            __syncthreads();
+            if (alpha == 1.0) {
+                done = true;
+            } else {
+                const double aux = alpha_min;
+                alpha_min = alpha;
+                alpha = fmin(alpha + gr * (alpha - aux), 1.0);
+            }
+        }
+        if (done) {
+            double Pbar[10], T1[45];
+            rve_gp_pass<MODEL, false>(c, p, sFa, lds, Pbar, T1);
+            wg_sum<10>(Pbar, red);
+            if (tid < 9) {
+                p.P[9 * rid + tid] = Pbar[tid] / Pbar[9];
+                p.Fconv[9 * rid + tid] = p.Fbar[9 * rid + tid];
+            }
+        }
+        if (tid == 0) {
+            p.info[4 * rid] = done ? 0 : -1;
+            p.info[4 * rid + 1] = newton_total;
+            p.info[4 * rid + 2] = pcg_total;
+            p.info[4 * rid + 3] = cut;
+        }
+        __syncthreads();
+    }
+}
+
+
+// Homogenized tangent dP/dF of every converged RVE by static condensation (second persistent
+// launch, after k_rve_solve): the reduced tangent at the converged state is assembled again,
+//   A = (Phi^T K Phi + sum_m b_m^T v_m) / V,  K_red v_m = -b_m,  b_m = T^T K Phi_m,
+// nine CG solves per RVE.  RVEs whose solve failed are skipped.
+template <int MODEL>
+__global__ __launch_bounds__(kRveThreads) void k_rve_tangent(RveParams p) {
+    __shared__ double lds[kRveLds];
+    __shared__ double red[4 * 45];
+    __shared__ int snode[kRveElemPerRound][8];
+    __shared__ int sq;
+    __shared__ double sFa[9];
+    const int tid = threadIdx.x;
+    const long long slot = blockIdx.x;
+    for (;;) {
+        if (tid == 0) sq = atomicAdd(p.queue, 1);
+        __syncthreads();
+        const int qi = sq;
+        __syncthreads();
+        if (qi >= p.n_rve) break;
+        const int rid = p.order[qi];
+        if (p.info[4 * rid] != 0) continue;
+        RveCtx c = rve_ctx(p, rid, slot);
+        for (int k = tid; k < 3 * c.nn; k += kRveThreads) c.wc[k] = c.wconv[k];
+        if (tid < 9) sFa[tid] = p.Fconv[9 * rid + 3 * (tid % 3) + tid / 3];
+        __syncthreads();
+        rve_assemble<MODEL, true>(c, p, sFa, lds, snode);
+        double Pbar[10], T1[45];
+        rve_gp_pass<MODEL, true>(c, p, sFa, lds, Pbar, T1);
+        wg_sum<10>(Pbar, red);
+        wg_sum<45>(T1, red);
+        const double V0 = Pbar[9];
+        rve_affine_rhs(c);
+        rve_precond(c);
+        __syncthreads();
+        const long long nn = c.nn;
+        int pcg_total = 0;
+        // condensed part b_k . v_m, v_m = K_red^-1 (-b_m), parked in A until the end
+        double* Aout = p.A + 81LL * rid;
+        for (int m = 0; m < 9; ++m) {
+            pcg_total += rve_pcg(c, p, c.B + (long long)m * 3 * nn, -1.0, lds, red);
+            double dots[9];
+#pragma unroll
+            for (int k = 0; k < 9; ++k) dots[k] = 0.0;
+            for (int i = tid; i < c.nn; i += kRveThreads) {
+#pragma unroll
+                for (int a = 0; a < 3; ++a) {
+                    const double v = c.x[3 * i + a];
+#pragma unroll
+                    for (int k = 0; k < 9; ++k) dots[k] += c.B[(long long)k * 3 * nn + 3 * i + a] * v;
+                }
+            }
+            wg_sum<9>(dots, red);
+            if (tid == 0) {
+#pragma unroll
+                for (int k = 0; k < 9; ++k) Aout[9 * k + m] = dots[k];
+            }
+        }
+        __syncthreads();
+        if (tid < 81) {
+            const int I = tid / 9, K = tid % 9;
+            const int lo = I < K ? I : K, hi = I < K ? K : I;
+            const int idx = lo * 9 - (lo * (lo - 1)) / 2 + (hi - lo);
+            double t1 = 0.0;
+#pragma unroll
+            for (int k = 0; k < 45; ++k) t1 = (k == idx) ? T1[k] : t1;
+            Aout[tid] = (t1 + Aout[tid]) / V0;
+        }
+        if (tid == 0) p.info[4 * rid + 2] += pcg_total;
+        __syncthreads();
+    }
+}
+
+}  // namespace ceos

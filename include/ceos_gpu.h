@@ -170,6 +170,55 @@ int ceos_set_tangent_mode(ceos_handle *h, int mode);
 /* Mesh facts after setup: number of element colours and the nnz of the upper pattern. */
 int ceos_mesh_info(ceos_handle *h, int *n_colors, long long *nnz_upper);
 
+/* ------------------------------------------------------------------------------------------
+ * Batched periodic RVEs (FE^2 micro problems).
+ *
+ * Replaces, for every RVE of a batch at once, the reference's periodic multiscale analysis of
+ * one macro step: the periodic system of equations with K_red = T^T K T and R_red = T^T R
+ * (FEM/Modules/ModMultiscalePeriodicFEMSoE.f90:77-222), the affine displacement U = (F - I) Y
+ * with the cube corners fixed (MultiScale/ModMultiscaleBoundaryConditions.f90:200-289,
+ * 632-725), its Newton loop (Tools/MathTools/NonlinearSolver/ModNewtonRaphsonFull.f90:74-163,
+ * max-abs residual, relative test), the golden-ratio cut-back (FEM/ModFEMAnalysis.f90:1270-1340)
+ * and the homogenized first Piola stress (MultiScale/ModMultiscaleAnalysis.f90:140-329).
+ * PARDISO is replaced by block-Jacobi preconditioned CG on the GPU.
+ *
+ * RVE i is the unit cube cut into side[i]^3 voxel hex8 elements (side even, 4..16); phase holds
+ * one byte per voxel (0 matrix, 1 inclusion), RVEs concatenated, voxel (x,y,z) at
+ * x + s (y + s z).  Both phases use `model` (3 Neo-Hookean, 7 compressible Neo-Hookean or
+ * 5 St.Venant-Kirchhoff) with props[nprops * phase + k].
+ * ------------------------------------------------------------------------------------------ */
+typedef struct ceos_rve_batch ceos_rve_batch;
+
+int ceos_rve_batch_create(int n_rve, const int *side, const unsigned char *phase, int model,
+                          const double *props, int nprops, ceos_rve_batch **b);
+void ceos_rve_batch_destroy(ceos_rve_batch *b);
+
+/* Newton: converged when max|R_red| <= max(newton_rtol * max|R_red at the predictor|,
+ * newton_atol), at most newton_maxit iterations per substep, at most max_cutbacks cut-backs;
+ * CG: ||r||_2 <= pcg_rtol ||b||_2, at most pcg_maxit iterations.
+ * Defaults: 1e-10, 1e-13, 12, 1e-10, 5000, 20. */
+int ceos_rve_set_solver(ceos_rve_batch *b, double newton_rtol, double newton_atol, int newton_maxit,
+                        double pcg_rtol, int pcg_maxit, int max_cutbacks);
+
+/* Every RVE back to its undeformed state (F = I, zero fluctuation). */
+int ceos_rve_reset(ceos_rve_batch *b);
+
+/* Equilibrium of every RVE at its macro F (Fbar: 9 doubles per RVE, column-major like the
+ * reference's Voigt 9-vectors, ModVoigtNotation.f90:70-86), reached from the RVE's last
+ * converged state with the cut-back substepping.  Outputs per RVE: P (9, column-major),
+ * A (81, A[9*(3J+i) + 3L+k] = dP_iJ/dF_kL by static condensation; NULL = not computed) and
+ * info (4 ints: status 0 / -1, Newton iterations, CG iterations, cut-backs).  Returns
+ * CEOS_ERR_LOCAL_NR if some RVE did not converge (its state is left at its last converged
+ * substep).  Host arrays; the call synchronises. */
+int ceos_rve_solve(ceos_rve_batch *b, const double *Fbar, double *P, double *A, int *info);
+
+/* Same with device arrays on a caller stream (NULL = the batch's stream), asynchronous. */
+int ceos_rve_solve_device(ceos_rve_batch *b, const double *dFbar, double *dP, double *dA, int *dinfo,
+                          void *stream);
+
+/* Converged fluctuation of RVE `rve` (3 side^3 doubles, torus node x + s (y + s z)). */
+int ceos_rve_get_fluctuation(ceos_rve_batch *b, int rve, double *w);
+
 #ifdef __cplusplus
 }
 #endif

@@ -75,12 +75,16 @@ class OracleRVE:
         return _sym_from_upper(rm, col, self.mesh.tangent(), 3 * len(self.X))
 
     def _reduced(self, K):
-        Kr = (self.T.T @ K @ self.T).tolil()
-        Kr[0:3, :] = 0.0
-        Kr[:, 0:3] = 0.0
-        for k in range(3):
-            Kr[k, k] = 1.0
-        return Kr.tocsc()
+        """T^T K T with the corner node's rows and columns replaced by the identity."""
+        Kr = (self.T.T @ K @ self.T).tocsr()
+        mask = np.ones(Kr.shape[0])
+        mask[0:3] = 0.0
+        Dm = sp.diags(mask)
+        return (Dm @ Kr @ Dm + sp.diags(1.0 - mask)).tocsc()
+
+    @staticmethod
+    def _solve(Kr, rhs):
+        return spla.spsolve(Kr, rhs, permc_spec="MMD_AT_PLUS_A")
 
     def newton(self, Fbar, rtol=1e-10, atol=1e-13, maxit=12):
         """Newton on the reduced system from the current fluctuation, residual at the Taylor
@@ -102,7 +106,7 @@ class OracleRVE:
                 return True, it, hist
             if it == maxit:
                 return False, it, hist
-            dw = spla.spsolve(self._reduced(self._Kfull()), -R)
+            dw = self._solve(self._reduced(self._Kfull()), -R)
             dw[0:3] = 0.0
             self.w += dw
         return False, maxit, hist
@@ -146,7 +150,7 @@ class OracleRVE:
         KPhi = K @ self.Phi
         b = self.T.T @ KPhi
         b[0:3, :] = 0.0
-        v = spla.spsolve(self._reduced(K), -b)
+        v = self._solve(self._reduced(K), -b)
         v = np.asarray(v).reshape(-1, 9)
         return (self.Phi.T @ KPhi + b.T @ v) / self.V
 

@@ -1,0 +1,129 @@
+"""Parity of the batched periodic RVE solver (ceos_rve_*, one persistent launch) with the CPU
+restatement of the reference's periodic analysis (tests/rve_oracle.py: full mesh, T^T K T,
+direct solves).
+
+Bar: homogenized P within 1e-9 relative and fluctuations within 1e-8 of the oracle's (both
+Newton loops stop at max|R| <= 1e-10 max|R_0|, the GPU's CG at 1e-10, so the two converged
+states differ by the solver tolerances, not by rounding); the condensed tangent within 1e-7.
+"""
+import numpy as np
+import pytest
+
+import oracle_lib as ol
+import rve_oracle as ro
+from ceos_amd import gpu, rve
+
+pytestmark = pytest.mark.gpu
+
+PROPS = np.array([rve.MATRIX_PROPS, rve.INCLUSION_PROPS])
+
+
+def _F(seed, scale=0.06):
+    rng = np.random.default_rng(seed)
+    return np.eye(3) + rng.uniform(-scale, scale, size=(3, 3))
+
+
+def _batch(sides, ids):
+    ph = np.concatenate([rve.phases(n, i) for n, i in zip(sides, ids)])
+    return gpu.RveBatch(sides, ph, gpu.MODEL_NH, PROPS), ph
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+def test_homogeneous_rve_exact():
+    b = gpu.RveBatch([4], np.zeros(64, dtype=np.uint8), gpu.MODEL_NH, PROPS)
+    F = _F(1)
+    P, _, info = b.solve(F[None])
+    print("info", info, flush=True)
+    s, _, _ = ol.material_point(ol.MODEL_NH, list(rve.MATRIX_PROPS), F)
+    sig = np.array([[s[0], s[3], s[5]], [s[3], s[1], s[4]], [s[5], s[4], s[2]]])
+    Pm = np.linalg.det(F) * sig @ np.linalg.inv(F).T
+    assert info[0, 0] == 0 and info[0, 1] == 0
+    assert np.allclose(P[0], Pm, rtol=0, atol=1e-12 * np.abs(Pm).max())
+    assert np.max(np.abs(b.fluctuation(0))) == 0.0
+
+
+def test_mixed_batch_matches_oracle():
+    sides = [6, 4, 8, 4, 6]
+    ids = [3, 4, 5, 6, 7]
+    b, ph = _batch(sides, ids)
+    Fs = np.stack([_F(10 + k) for k in range(len(sides))])
+    P, _, info = b.solve(Fs)
+    print("info", info.tolist(), flush=True)
+    assert np.all(info[:, 0] == 0)
+    off = 0
+    for k, (n, i) in enumerate(zip(sides, ids)):
+        o = ro.OracleRVE(n, ph[off:off + n ** 3])
+        off += n ** 3
+        Po, _, _, _ = o.solve(Fs[k])
+        Po = Po.reshape(3, 3).T
+        print(k, n, rel(P[k], Po), flush=True)
+        assert rel(P[k], Po) < 1e-9
+        assert np.max(np.abs(b.fluctuation(k) - o.w)) < 1e-8 * max(np.max(np.abs(o.w)), 1e-3)
+
+
+def test_condensed_tangent_matches_oracle():
+    sides = [4, 6]
+    b, ph = _batch(sides, [11, 12])
+    b.set_solver(newton_rtol=1e-12, pcg_rtol=1e-12)
+    Fs = np.stack([_F(20), _F(21)])
+    P, A, info = b.solve(Fs, tangent=True)
+    print("info", info.tolist(), flush=True)
+    off = 0
+    for k, n in enumerate(sides):
+        o = ro.OracleRVE(n, ph[off:off + n ** 3])
+        off += n ** 3
+        o.solve(Fs[k], rtol=1e-12)
+        Ao = o.tangent()
+        print(k, rel(A[k], Ao), flush=True)
+        assert rel(A[k], Ao) < 1e-7
+        assert np.allclose(A[k], A[k].T, atol=1e-8 * np.abs(A[k]).max())  # hyperelastic: major symmetry
+
+
+def test_incremental_solves_and_cut_back():
+    """Two macro increments from the converged state, then a large stretch reached by the
+    golden-ratio cut-back; the states match the oracle driven through the same increments."""
+    sides = [4, 6]
+    b, ph = _batch(sides, [7, 8])
+    os_ = [ro.OracleRVE(n, ph[o:o + n ** 3]) for n, o in zip(sides, [0, 64])]
+    F1 = np.stack([_F(30, 0.03), _F(31, 0.03)])
+    F2 = np.stack([np.diag([1.25, 0.8, 1.1]), np.diag([0.85, 1.2, 1.05])])
+    for Fs in (F1, F2):
+        P, _, info = b.solve(Fs)
+        print("info", info.tolist(), flush=True)
+        for k, o in enumerate(os_):
+            Po, _, _, _ = o.solve(Fs[k])
+            assert rel(P[k], Po.reshape(3, 3).T) < 1e-9
+    assert info[0, 3] >= 1  # diag(1.25, 0.8, 1.1) from a mild state needs a cut-back
+
+
+def test_queue_longer_than_the_machine():
+    """More RVEs than workgroup slots: every RVE converges and a sample matches the oracle."""
+    n_rve = 600
+    sides = np.full(n_rve, 4, dtype=np.int32)
+    sides[::7] = 6
+    ids = np.arange(n_rve)
+    b, ph = _batch(sides, ids)
+    rng = np.random.default_rng(5)
+    Fs = np.eye(3)[None] + rng.uniform(-0.04, 0.04, size=(n_rve, 3, 3))
+    P, _, info = b.solve(Fs)
+    assert np.all(info[:, 0] == 0)
+    offs = np.concatenate([[0], np.cumsum(sides.astype(np.int64) ** 3)])
+    for k in (0, 1, 7, 299, 599):
+        n = int(sides[k])
+        o = ro.OracleRVE(n, ph[offs[k]:offs[k] + n ** 3])
+        Po, _, _, _ = o.solve(Fs[k])
+        assert rel(P[k], Po.reshape(3, 3).T) < 1e-9
+
+
+@pytest.mark.parametrize("n", [12, 16])
+def test_workload_sizes_match_oracle(n):
+    b, ph = _batch([n], [40 + n])
+    F = _F(50 + n, 0.04)
+    P, _, info = b.solve(F[None])
+    print("info", info.tolist(), flush=True)
+    o = ro.OracleRVE(n, ph)
+    Po, _, _, _ = o.solve(F)
+    assert rel(P[0], Po.reshape(3, 3).T) < 1e-9
