@@ -53,7 +53,8 @@ EXPORTS = [
     "ceos_set_schedule", "ceos_set_assembly", "ceos_set_tangent_mode",
     "ceos_timing_start", "ceos_timing_stop",
     "ceos_rve_batch_create", "ceos_rve_batch_destroy", "ceos_rve_set_solver", "ceos_rve_reset",
-    "ceos_rve_solve", "ceos_rve_solve_device", "ceos_rve_get_fluctuation",
+    "ceos_rve_solve", "ceos_rve_solve_device", "ceos_rve_get_fluctuation", "ceos_rve_set_inexact",
+    "ceos_rve_profile",
 ]
 
 _lib = None
@@ -104,6 +105,8 @@ def load_library(path=LIB_PATH):
     L.ceos_rve_batch_destroy.restype = None
     L.ceos_rve_set_solver.argtypes = [vp, d, d, i, d, i, i]
     L.ceos_rve_reset.argtypes = [vp]
+    L.ceos_rve_set_inexact.argtypes = [vp, i]
+    L.ceos_rve_profile.argtypes = [vp, vp]
     L.ceos_rve_solve.argtypes = [vp, _dp, _dp, vp, vp]
     L.ceos_rve_solve_device.argtypes = [vp, vp, vp, vp, vp, vp]
     L.ceos_rve_get_fluctuation.argtypes = [vp, i, _dp]
@@ -312,6 +315,9 @@ class RveBatch:
         _check(load_library().ceos_rve_set_solver(self._h, newton_rtol, newton_atol, newton_maxit, pcg_rtol,
                                                   pcg_maxit, max_cutbacks), "ceos_rve_set_solver")
 
+    def set_inexact(self, on):
+        _check(load_library().ceos_rve_set_inexact(self._h, 1 if on else 0), "ceos_rve_set_inexact")
+
     def reset(self):
         _check(load_library().ceos_rve_reset(self._h), "ceos_rve_reset")
 
@@ -327,6 +333,12 @@ class RveBatch:
             _check(code, "ceos_rve_solve")
         Pm = P.reshape(self.n, 3, 3).transpose(0, 2, 1)
         return Pm, (A.reshape(self.n, 9, 9) if tangent else None), info
+
+    def profile(self):
+        """(n_rve, 2) seconds in the Newton assemblies and in CG since the last reset."""
+        t = np.zeros((self.n, 2), dtype=np.int64)
+        _check(load_library().ceos_rve_profile(self._h, t.ctypes.data), "ceos_rve_profile")
+        return t / 1.0e8
 
     def fluctuation(self, r):
         n = int(self.sides[r])

@@ -31,12 +31,14 @@ struct ceos_rve_batch {
     hipStream_t stream = nullptr;
     double newton_rtol = 1e-10, newton_atol = 1e-13, pcg_rtol = 1e-10;
     int newton_maxit = 12, pcg_maxit = 5000, max_cutbacks = 20;
+    int inexact = 1;
     // device
     int *dorder = nullptr, *dside = nullptr, *dinfo = nullptr, *dqueue = nullptr;
     long long *dnode_off = nullptr, *dphase_off = nullptr;
     uint8_t* dphase = nullptr;
     double *dprops = nullptr, *dFbar = nullptr, *dFconv = nullptr, *dw = nullptr, *dP = nullptr, *dA = nullptr;
     double *dM = nullptr, *dV = nullptr;
+    long long* dprof = nullptr;
 };
 
 namespace {
@@ -56,11 +58,12 @@ void free_all(ceos_rve_batch* b) {
     rfree(b->dorder); rfree(b->dside); rfree(b->dinfo); rfree(b->dqueue);
     rfree(b->dnode_off); rfree(b->dphase_off); rfree(b->dphase);
     rfree(b->dprops); rfree(b->dFbar); rfree(b->dFconv); rfree(b->dw); rfree(b->dP); rfree(b->dA);
-    rfree(b->dM); rfree(b->dV);
+    rfree(b->dM); rfree(b->dV); rfree(b->dprof);
 }
 
-__global__ void k_rve_reset(int n_rve, long long nodes3, double* Fconv, double* w) {
+__global__ void k_rve_reset(int n_rve, long long nodes3, double* Fconv, double* w, long long* prof) {
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < 2LL * n_rve) prof[t] = 0;
     if (t < 9LL * n_rve) {
         const int k = (int)(t % 9);
         Fconv[t] = (k == 0 || k == 4 || k == 8) ? 1.0 : 0.0;
@@ -82,7 +85,7 @@ void launch(const RveParams& p, int grid, bool tangent, hipStream_t s) {
 int reset(ceos_rve_batch* b) {
     const long long n = std::max<long long>(9LL * b->n_rve, 3 * b->total_nodes);
     hipLaunchKernelGGL(k_rve_reset, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, b->stream, b->n_rve,
-                       3 * b->total_nodes, b->dFconv, b->dw);
+                       3 * b->total_nodes, b->dFconv, b->dw, b->dprof);
     RVE_HIP_OK(hipGetLastError());
     RVE_HIP_OK(hipStreamSynchronize(b->stream));
     return CEOS_OK;
@@ -113,6 +116,8 @@ int run(ceos_rve_batch* b, bool want_A, hipStream_t s) {
     p.newton_maxit = b->newton_maxit;
     p.pcg_maxit = b->pcg_maxit;
     p.max_cutbacks = b->max_cutbacks;
+    p.inexact = b->inexact;
+    p.prof = b->dprof;
     hipLaunchKernelGGL(k_zero_queue, dim3(1), dim3(1), 0, s, b->dqueue);
     const int grid = std::min(b->n_rve, b->slots);
     switch (b->model) {
@@ -171,7 +176,7 @@ int ceos_rve_batch_create(int n_rve, const int* side, const unsigned char* phase
         delete b;
         return CEOS_ERR_HIP;
     }
-    b->slots = cus;  // ~107 KB of LDS per workgroup: one per CU
+    b->slots = cus;  // ~107 KB of LDS per 512-thread workgroup: one per CU
     const long long slots = std::min<long long>(b->slots, n_rve);
     bool okm =
         rmalloc(b->dorder, n_rve) == hipSuccess && rmalloc(b->dside, n_rve) == hipSuccess &&
@@ -180,7 +185,7 @@ int ceos_rve_batch_create(int n_rve, const int* side, const unsigned char* phase
         rmalloc(b->dphase, tot) == hipSuccess && rmalloc(b->dprops, 16) == hipSuccess &&
         rmalloc(b->dFbar, 9LL * n_rve) == hipSuccess && rmalloc(b->dFconv, 9LL * n_rve) == hipSuccess &&
         rmalloc(b->dw, 3 * tot) == hipSuccess && rmalloc(b->dP, 9LL * n_rve) == hipSuccess &&
-        rmalloc(b->dA, 81LL * n_rve) == hipSuccess &&
+        rmalloc(b->dA, 81LL * n_rve) == hipSuccess && rmalloc(b->dprof, 2LL * n_rve) == hipSuccess &&
         rmalloc(b->dM, (size_t)(243 * b->max_nodes * slots)) == hipSuccess &&
         rmalloc(b->dV, (size_t)(kRveVecPerNode * b->max_nodes * slots)) == hipSuccess;
     if (!okm) rc = CEOS_ERR_HIP;
@@ -227,6 +232,12 @@ int ceos_rve_set_solver(ceos_rve_batch* b, double newton_rtol, double newton_ato
     return CEOS_OK;
 }
 
+int ceos_rve_set_inexact(ceos_rve_batch* b, int inexact) {
+    if (!b || (inexact != 0 && inexact != 1)) return CEOS_ERR_ARG;
+    b->inexact = inexact;
+    return CEOS_OK;
+}
+
 int ceos_rve_reset(ceos_rve_batch* b) {
     if (!b) return CEOS_ERR_ARG;
     return reset(b);
@@ -259,6 +270,13 @@ int ceos_rve_solve_device(ceos_rve_batch* b, const double* dFbar, double* dP, do
     RVE_HIP_OK(hipMemcpyAsync(dP, b->dP, sizeof(double) * 9 * n, hipMemcpyDeviceToDevice, s));
     if (dA) RVE_HIP_OK(hipMemcpyAsync(dA, b->dA, sizeof(double) * 81 * n, hipMemcpyDeviceToDevice, s));
     if (dinfo) RVE_HIP_OK(hipMemcpyAsync(dinfo, b->dinfo, sizeof(int) * 4 * n, hipMemcpyDeviceToDevice, s));
+    return CEOS_OK;
+}
+
+int ceos_rve_profile(ceos_rve_batch* b, long long* ticks) {
+    if (!b || !ticks) return CEOS_ERR_ARG;
+    RVE_HIP_OK(hipStreamSynchronize(b->stream));
+    RVE_HIP_OK(hipMemcpy(ticks, b->dprof, sizeof(long long) * 2 * b->n_rve, hipMemcpyDeviceToHost));
     return CEOS_OK;
 }
 

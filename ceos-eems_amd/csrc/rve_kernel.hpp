@@ -33,7 +33,9 @@
 namespace ceos {
 
 constexpr int kRveThreads = 256;
-constexpr int kRveElemPerRound = 32;
+constexpr int kRveWaves = kRveThreads / 64;
+constexpr int kRveElemPerRound = 32;          // assembly: threads 0..255, 8 lanes per element
+constexpr int kRveAsmThreads = 8 * kRveElemPerRound;
 constexpr int kRveMaxSide = 16;
 constexpr int kRveLds = kRveElemPerRound * kLdsPerElem;  // 13056 doubles >= 3 * 16^3
 constexpr int kRveVecPerNode = 51;  // w 3, R 3, x 3, r 3, q 3, z 3, Minv 6, B 27
@@ -59,6 +61,8 @@ struct RveParams {
     int* queue;                   // next queue position (zeroed before the launch)
     double newton_rtol, newton_atol, pcg_rtol;
     int newton_maxit, pcg_maxit, max_cutbacks;
+    int inexact;                  // 1: CG tolerance follows the Newton residual (forcing term)
+    long long* prof;              // [n_rve][2] clock ticks in assembly, in CG (or nullptr)
 };
 
 __device__ __forceinline__ double wave_sum(double v) {
@@ -73,7 +77,8 @@ __device__ __forceinline__ double wave_max(double v) {
     return v;
 }
 
-// workgroup sums of N values; every thread gets the totals (red: >= 4 N doubles of LDS)
+// workgroup sums of N values; every thread gets the totals, summed in a fixed order
+// (red: >= kRveWaves N doubles of LDS)
 template <int N>
 __device__ __forceinline__ void wg_sum(double (&v)[N], double* red) {
 #pragma unroll
@@ -85,7 +90,12 @@ __device__ __forceinline__ void wg_sum(double (&v)[N], double* red) {
     }
     __syncthreads();
 #pragma unroll
-    for (int k = 0; k < N; ++k) v[k] = (red[k] + red[N + k]) + (red[2 * N + k] + red[3 * N + k]);
+    for (int k = 0; k < N; ++k) {
+        double t = 0.0;
+#pragma unroll
+        for (int w = 0; w < kRveWaves; ++w) t += red[w * N + k];
+        v[k] = t;
+    }
 }
 
 __device__ __forceinline__ double wg_max(double v, double* red) {
@@ -93,7 +103,10 @@ __device__ __forceinline__ double wg_max(double v, double* red) {
     __syncthreads();
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
     __syncthreads();
-    return fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+    double t = red[0];
+#pragma unroll
+    for (int w = 1; w < kRveWaves; ++w) t = fmax(t, red[w]);
+    return t;
 }
 
 // hex8 corner offsets (---,+--,++-,-+-,--+,+-+,+++,-++)
@@ -133,11 +146,12 @@ __device__ bool rve_assemble(const RveCtx& c, const RveParams& p, const double* 
     for (int color = 0; color < 8; ++color) {
         const int cx = color & 1, cy = (color >> 1) & 1, cz = color >> 2;
         for (int base = 0; base < ncol; base += kRveElemPerRound) {
+            const bool worker = tid < kRveAsmThreads;  // the other threads only pass the barriers
             const int el = base + le;
-            const bool active = el < ncol;
+            const bool active = worker && el < ncol;
             const int ex = 2 * (el % half) + cx, ey = 2 * ((el / half) % half) + cy, ez = 2 * (el / (half * half)) + cz;
             // phase 0: corner a = lane
-            {
+            if (worker) {
                 const int gx = ex + corner_dx(lane), gy = ey + corner_dy(lane), gz = ez + corner_dz(lane);
                 const int node = active ? ((gx % n) + n * ((gy % n) + n * (gz % n))) : 0;
                 snode[le][lane] = node;
@@ -157,8 +171,8 @@ __device__ bool rve_assemble(const RveCtx& c, const RveParams& p, const double* 
             const int g = lane;
             double gxs[8][3];
             double s[6], D[21];
-            double wdet;
-            {
+            double wdet = 0.0;
+            if (worker) {
                 double dN[8][3];
                 hex8_dshape_gp(g, dN);
                 double cc[8][3];
@@ -197,20 +211,22 @@ __device__ bool rve_assemble(const RveCtx& c, const RveParams& p, const double* 
                 }
             }
             __syncthreads();  // node data no longer needed
+            if (worker) {
 #pragma unroll
-            for (int a = 0; a < 8; ++a)
+                for (int a = 0; a < 8; ++a)
 #pragma unroll
-                for (int k = 0; k < 3; ++k) sm[g * 24 + a * 3 + k] = gxs[a][k];
+                    for (int k = 0; k < 3; ++k) sm[g * 24 + a * 3 + k] = gxs[a][k];
 #pragma unroll
-            for (int k = 0; k < 6; ++k) sm[kOffSig + g * 6 + k] = s[k] * wdet;
-            if constexpr (WANT_K) {
+                for (int k = 0; k < 6; ++k) sm[kOffSig + g * 6 + k] = s[k] * wdet;
+                if constexpr (WANT_K) {
 #pragma unroll
-                for (int k = 0; k < 21; ++k) sm[kOffD + g * 21 + k] = D[k] * wdet;
+                    for (int k = 0; k < 21; ++k) sm[kOffD + g * 21 + k] = D[k] * wdet;
+                }
             }
             __syncthreads();
             // phase 2: node b = lane; read-add-write of R and M (no other element of the colour
             // shares a node; each block slot is written by exactly one lane of the element)
-            {
+            if (worker) {
                 const int b = lane;
                 const int npart = (b < 4) ? 4 : 3;
                 double fe0, fe1, fe2, kd[6], ko[4][9];
@@ -281,11 +297,11 @@ __device__ __forceinline__ void rve_row(const RveCtx& c, int i, const double* v,
                 const int xx = ix + dx < 0 ? n - 1 : (ix + dx == n ? 0 : ix + dx);
                 const int j = xx + n * (yy + n * zz);
                 const int o = (dx + 1) + 3 * (dy + 1) + 9 * (dz + 1);
-                const double* m = c.M + (long long)o * 9 * nn + i;
+                const double* m = c.M + o * 9 * nn + i;
                 const double v0 = v[3 * j], v1 = v[3 * j + 1], v2 = v[3 * j + 2];
-                a0 += m[0] * v0 + m[(long long)nn] * v1 + m[2LL * nn] * v2;
-                a1 += m[3LL * nn] * v0 + m[4LL * nn] * v1 + m[5LL * nn] * v2;
-                a2 += m[6LL * nn] * v0 + m[7LL * nn] * v1 + m[8LL * nn] * v2;
+                a0 += m[0] * v0 + m[nn] * v1 + m[2 * nn] * v2;
+                a1 += m[3 * nn] * v0 + m[4 * nn] * v1 + m[5 * nn] * v2;
+                a2 += m[6 * nn] * v0 + m[7 * nn] * v1 + m[8 * nn] * v2;
             }
         }
     }
@@ -322,10 +338,11 @@ __device__ __forceinline__ void apply_minv(const double* mi, const double r[3], 
     z[2] = mi[2] * r[0] + mi[4] * r[1] + mi[5] * r[2];
 }
 
-// preconditioned CG on the reduced system: M x = sign * rhs (node 0 fixed).  p in LDS.
+// preconditioned CG on the reduced system: M x = sign * rhs (node 0 fixed) to
+// ||r|| <= rtol ||b||.  p lives in LDS, the other vectors in the workgroup's scratch.
 // Returns the iterations used.
-__device__ int rve_pcg(const RveCtx& c, const RveParams& p, const double* rhs, double sign, double* pl,
-                       double* red) {
+__device__ int rve_pcg(const RveCtx& c, const RveParams& p, const double* rhs, double sign, double rtol,
+                       double* pl, double* red) {
     const int tid = threadIdx.x;
     double acc[2] = {0.0, 0.0};
     for (int i = tid; i < c.nn; i += kRveThreads) {
@@ -344,7 +361,7 @@ __device__ int rve_pcg(const RveCtx& c, const RveParams& p, const double* rhs, d
     }
     wg_sum<2>(acc, red);
     double rz = acc[0];
-    const double tol2 = p.pcg_rtol * p.pcg_rtol * acc[1];
+    const double tol2 = rtol * rtol * acc[1];
     if (!(acc[1] > 0.0)) return 0;
     int it = 0;
     while (it < p.pcg_maxit) {
@@ -428,10 +445,11 @@ __device__ void rve_gp_pass(const RveCtx& c, const RveParams& p, const double* F
 #pragma unroll
     for (int k = 0; k < 10; ++k) Pacc[k] = 0.0;
     for (int base = 0; base < nel; base += kRveElemPerRound) {
+        const bool worker = tid < kRveAsmThreads;
         const int el = base + le;
-        const bool active = el < nel;
+        const bool active = worker && el < nel;
         const int ex = el % n, ey = (el / n) % n, ez = el / (n * n);
-        {
+        if (worker) {
             const int gx = ex + corner_dx(lane), gy = ey + corner_dy(lane), gz = ez + corner_dz(lane);
             const int node = active ? ((gx % n) + n * ((gy % n) + n * (gz % n))) : 0;
             const double Y[3] = {gx * c.h, gy * c.h, gz * c.h};
@@ -584,7 +602,7 @@ __device__ __forceinline__ RveCtx rve_ctx(const RveParams& p, int rid, long long
 template <int MODEL>
 __global__ __launch_bounds__(kRveThreads) void k_rve_solve(RveParams p) {
     __shared__ double lds[kRveLds];
-    __shared__ double red[4 * 45];
+    __shared__ double red[kRveWaves * 45];
     __shared__ int snode[kRveElemPerRound][8];
     __shared__ int sq;
     __shared__ double sFa[9];
@@ -618,7 +636,9 @@ __global__ __launch_bounds__(kRveThreads) void k_rve_solve(RveParams p) {
             bool ok = false;
             double nr0 = 0.0;
             for (int it = 0; it <= p.newton_maxit; ++it) {
+                const long long t0 = wall_clock64();
                 const bool bad = rve_assemble<MODEL, true>(c, p, sFa, lds, snode);
+                if (p.prof && tid == 0) p.prof[2 * rid] += wall_clock64() - t0;
                 double rmax = 0.0;
                 for (int i = tid; i < c.nn; i += kRveThreads) {
                     if (i == 0) continue;
@@ -638,7 +658,12 @@ __global__ __launch_bounds__(kRveThreads) void k_rve_solve(RveParams p) {
                 if (it == p.newton_maxit) break;
                 rve_precond(c);
                 __syncthreads();
-                pcg_total += rve_pcg(c, p, c.R, -1.0, lds, red);
+                // inexact Newton: the linear solve only needs to beat the nonlinear residual
+                // reduction still to come (forcing 0.1 ||R|| / ||R_0||, within [pcg_rtol, 1e-2])
+                const double eta = p.inexact ? fmin(1e-2, fmax(p.pcg_rtol, 0.1 * nr / nr0)) : p.pcg_rtol;
+                const long long t1 = wall_clock64();
+                pcg_total += rve_pcg(c, p, c.R, -1.0, eta, lds, red);
+                if (p.prof && tid == 0) p.prof[2 * rid + 1] += wall_clock64() - t1;
                 for (int k = tid; k < 3 * c.nn; k += kRveThreads) c.wc[k] += c.x[k];
                 __syncthreads();
             }
@@ -686,7 +711,7 @@ __global__ __launch_bounds__(kRveThreads) void k_rve_solve(RveParams p) {
 template <int MODEL>
 __global__ __launch_bounds__(kRveThreads) void k_rve_tangent(RveParams p) {
     __shared__ double lds[kRveLds];
-    __shared__ double red[4 * 45];
+    __shared__ double red[kRveWaves * 45];
     __shared__ int snode[kRveElemPerRound][8];
     __shared__ int sq;
     __shared__ double sFa[9];
@@ -718,7 +743,7 @@ __global__ __launch_bounds__(kRveThreads) void k_rve_tangent(RveParams p) {
         // condensed part b_k . v_m, v_m = K_red^-1 (-b_m), parked in A until the end
         double* Aout = p.A + 81LL * rid;
         for (int m = 0; m < 9; ++m) {
-            pcg_total += rve_pcg(c, p, c.B + (long long)m * 3 * nn, -1.0, lds, red);
+            pcg_total += rve_pcg(c, p, c.B + (long long)m * 3 * nn, -1.0, p.pcg_rtol, lds, red);
             double dots[9];
 #pragma unroll
             for (int k = 0; k < 9; ++k) dots[k] = 0.0;

@@ -200,6 +200,11 @@ void ceos_rve_batch_destroy(ceos_rve_batch *b);
 int ceos_rve_set_solver(ceos_rve_batch *b, double newton_rtol, double newton_atol, int newton_maxit,
                         double pcg_rtol, int pcg_maxit, int max_cutbacks);
 
+/* 1 (default): inexact Newton, the CG tolerance of a Newton step is
+ * min(1e-2, max(pcg_rtol, 0.1 max|R| / max|R_0|)); 0: every CG solve to pcg_rtol.  The Newton
+ * convergence test is the same either way; the tangent solves always use pcg_rtol. */
+int ceos_rve_set_inexact(ceos_rve_batch *b, int inexact);
+
 /* Every RVE back to its undeformed state (F = I, zero fluctuation). */
 int ceos_rve_reset(ceos_rve_batch *b);
 
@@ -215,6 +220,10 @@ int ceos_rve_solve(ceos_rve_batch *b, const double *Fbar, double *P, double *A, 
 /* Same with device arrays on a caller stream (NULL = the batch's stream), asynchronous. */
 int ceos_rve_solve_device(ceos_rve_batch *b, const double *dFbar, double *dP, double *dA, int *dinfo,
                           void *stream);
+
+/* Instrumentation (no reference counterpart): per RVE, the device wall-clock ticks (100 MHz)
+ * spent in the Newton assemblies and in the CG solves of the solves since the last reset. */
+int ceos_rve_profile(ceos_rve_batch *b, long long *ticks);
 
 /* Converged fluctuation of RVE `rve` (3 side^3 doubles, torus node x + s (y + s z)). */
 int ceos_rve_get_fluctuation(ceos_rve_batch *b, int rve, double *w);
