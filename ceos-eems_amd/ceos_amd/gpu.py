@@ -55,6 +55,8 @@ EXPORTS = [
     "ceos_rve_batch_create", "ceos_rve_batch_destroy", "ceos_rve_set_solver", "ceos_rve_reset",
     "ceos_rve_solve", "ceos_rve_solve_device", "ceos_rve_get_fluctuation", "ceos_rve_set_inexact",
     "ceos_rve_profile",
+    "ceos_mlp_create", "ceos_mlp_destroy", "ceos_mlp_num_params", "ceos_mlp_get_params",
+    "ceos_mlp_set_params", "ceos_mlp_gradients", "ceos_mlp_train_step", "ceos_mlp_forward",
 ]
 
 _lib = None
@@ -107,6 +109,17 @@ def load_library(path=LIB_PATH):
     L.ceos_rve_reset.argtypes = [vp]
     L.ceos_rve_set_inexact.argtypes = [vp, i]
     L.ceos_rve_profile.argtypes = [vp, vp]
+    f = C.c_float
+    L.ceos_mlp_create.argtypes = [i, i, i, i, C.c_ulonglong, C.POINTER(vp)]
+    L.ceos_mlp_destroy.argtypes = [vp]
+    L.ceos_mlp_destroy.restype = None
+    L.ceos_mlp_num_params.argtypes = [vp]
+    L.ceos_mlp_num_params.restype = C.c_longlong
+    L.ceos_mlp_get_params.argtypes = [vp, vp]
+    L.ceos_mlp_set_params.argtypes = [vp, vp]
+    L.ceos_mlp_gradients.argtypes = [vp, vp, vp, i, vp, C.POINTER(f)]
+    L.ceos_mlp_train_step.argtypes = [vp, vp, vp, i, f, C.POINTER(f)]
+    L.ceos_mlp_forward.argtypes = [vp, vp, i, vp, vp, vp]
     L.ceos_rve_solve.argtypes = [vp, _dp, _dp, vp, vp]
     L.ceos_rve_solve_device.argtypes = [vp, vp, vp, vp, vp, vp]
     L.ceos_rve_get_fluctuation.argtypes = [vp, i, _dp]
@@ -345,3 +358,54 @@ class RveBatch:
         w = np.zeros(3 * n ** 3)
         _check(load_library().ceos_rve_get_fluctuation(self._h, r, w), "ceos_rve_get_fluctuation")
         return w
+
+
+class Mlp:
+    """MLP n_in -> width x3 (SiLU) -> n_out on the GPU (ceos_mlp_*).  Arrays passed to the
+    device methods are hiprt.DeviceArray (float32); params() / set_params() use numpy."""
+
+    def __init__(self, n_in, width, n_out, max_batch, seed=1):
+        L = load_library()
+        h = C.c_void_p()
+        _check(L.ceos_mlp_create(n_in, width, n_out, max_batch, seed, C.byref(h)), "ceos_mlp_create")
+        self._h = h
+        self.n_in, self.width, self.n_out, self.max_batch = n_in, width, n_out, max_batch
+        self.nparams = L.ceos_mlp_num_params(h)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            load_library().ceos_mlp_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def params(self):
+        p = np.zeros(self.nparams, dtype=np.float32)
+        _check(load_library().ceos_mlp_get_params(self._h, p.ctypes.data), "ceos_mlp_get_params")
+        return p
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        assert p.size == self.nparams
+        _check(load_library().ceos_mlp_set_params(self._h, p.ctypes.data), "ceos_mlp_set_params")
+
+    def gradients(self, dX, dT, batch):
+        g = np.zeros(self.nparams, dtype=np.float32)
+        loss = C.c_float()
+        _check(load_library().ceos_mlp_gradients(self._h, dX.ptr, dT.ptr, batch, g.ctypes.data, C.byref(loss)),
+               "ceos_mlp_gradients")
+        return g, loss.value
+
+    def train_step(self, dX, dT, batch, lr, want_loss=False):
+        loss = C.c_float()
+        _check(load_library().ceos_mlp_train_step(self._h, dX.ptr, dT.ptr, batch, lr,
+                                                  C.byref(loss) if want_loss else None), "ceos_mlp_train_step")
+        return loss.value if want_loss else None
+
+    def forward(self, dX, n, dY, dJ=None, stream=None):
+        _check(load_library().ceos_mlp_forward(self._h, dX.ptr, n, dY.ptr, dJ.ptr if dJ is not None else None,
+                                               stream), "ceos_mlp_forward")

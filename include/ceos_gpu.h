@@ -228,6 +228,35 @@ int ceos_rve_profile(ceos_rve_batch *b, long long *ticks);
 /* Converged fluctuation of RVE `rve` (3 side^3 doubles, torus node x + s (y + s z)). */
 int ceos_rve_get_fluctuation(ceos_rve_batch *b, int rve, double *w);
 
+/* ------------------------------------------------------------------------------------------
+ * MLP surrogate of the homogenized RVE response (no counterpart in the reference, which has no
+ * concurrent FE^2 loop: it stands in for the batched RVE solve at macro points inside its
+ * training range).  n_in -> width -> width -> width -> n_out with SiLU; parameters in one FP32
+ * array [W1 b1 W2 b2 W3 b3 W4 b4], W row-major [out][in].  Device pointers throughout.
+ * ------------------------------------------------------------------------------------------ */
+typedef struct ceos_mlp ceos_mlp;
+
+/* Weights and biases uniform in +-1/sqrt(fan_in) from splitmix64(seed); batches up to max_batch. */
+int ceos_mlp_create(int n_in, int width, int n_out, int max_batch, unsigned long long seed, ceos_mlp **m);
+void ceos_mlp_destroy(ceos_mlp *m);
+long long ceos_mlp_num_params(const ceos_mlp *m);
+int ceos_mlp_get_params(ceos_mlp *m, float *host_params);
+/* Also resets the Adam moments. */
+int ceos_mlp_set_params(ceos_mlp *m, const float *host_params);
+
+/* Mean squared error over a batch (dX [batch][n_in], dT [batch][n_out]) and its gradient with
+ * respect to the parameters, BF16 MFMA GEMMs with FP32 accumulation (host_grads / loss may be
+ * NULL). */
+int ceos_mlp_gradients(ceos_mlp *m, const float *dX, const float *dT, int batch, float *host_grads, float *loss);
+
+/* One Adam step (beta 0.9 / 0.999, eps 1e-8) on the same gradient; loss (may be NULL) is the
+ * batch MSE before the step. */
+int ceos_mlp_train_step(ceos_mlp *m, const float *dX, const float *dT, int batch, float lr, float *loss);
+
+/* FP32 inference: dY [n][n_out]; dJ (may be NULL) the input Jacobian [n][n_out][n_in] by reverse
+ * mode.  stream NULL = the model's stream, synchronised before returning. */
+int ceos_mlp_forward(ceos_mlp *m, const float *dX, int n, float *dY, float *dJ, void *stream);
+
 #ifdef __cplusplus
 }
 #endif
