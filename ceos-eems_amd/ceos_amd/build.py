@@ -13,7 +13,7 @@ LIBDIR = os.path.join(PKG, "lib")
 LIB = os.path.join(LIBDIR, "libceos_gpu.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-SOURCES = ["ceos_gpu.hip", "rve.hip", "mlp.hip"]
+SOURCES = ["ceos_gpu.hip", "rve.hip", "mlp.hip", "surrogate.hip"]
 FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
     # packed FP32 instructions stay off (only ordinary scalar/float2 code is written)

@@ -1,0 +1,291 @@
+"""FE^2 on the GPU: a hex8 macro mesh whose Gauss points are periodic voxel RVEs.
+
+The reference drives one RVE analysis at a time through a prescribed macro F history
+(MultiScale/ModMultiscaleAnalysis.f90:445-470); it has no concurrent macro loop.  Here one macro
+evaluation (residual + tangent of the macro mesh) needs the homogenized response of every macro
+Gauss point, obtained either
+
+  plain      by the batched periodic RVE solver (ceos_rve_solve_device: every RVE to
+             equilibrium at its macro F, then its condensed tangent), or
+  surrogate  by the MLP surrogate (ceos_surrogate_eval_device), with the RVE solve as fallback
+             for points outside the training hull and as judge on a sampled fraction of points.
+
+Either way the points' S and dS/dE go to the macro element kernel as external material points
+(CEOS_MODEL_EXTERNAL) and the macro Fint and CSR tangent are assembled on the GPU.
+
+The surrogate's descriptor of a microstructure is its small-strain homogenized stiffness C_lin
+(the condensed tangent at F = I), computed once per RVE at setup (`descriptors`).
+"""
+import math
+import os
+import time
+
+import numpy as np
+
+from . import gpu, hiprt, meshgen, rve
+
+PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+SURROGATE_PATH = os.path.join(PKG, "data", "fe2_surrogate.npz")
+
+LAM_LO, LAM_HI = 0.8, 1.25     # principal stretches of the training distribution
+U_SCALE = 0.25                 # U features: (U - I) / U_SCALE
+E_REF = 0.1                    # output scaling: S = C_lin : E + (|E|^2 / E_REF^2) (t_mean + t_std y)
+N_IN, WIDTH, N_OUT = 27, 256, 6
+VOIGT = ((0, 0), (1, 1), (2, 2), (0, 1), (1, 2), (0, 2))
+PROPS = np.array([rve.MATRIX_PROPS, rve.INCLUSION_PROPS])
+
+
+# ------------------------------------------------------------------------------------------
+# stretch sampling (the surrogate's training distribution)
+# ------------------------------------------------------------------------------------------
+def haar_rotations(rng, n):
+    """n rotation matrices, uniform (Haar) on SO(3)."""
+    q, r = np.linalg.qr(rng.normal(size=(n, 3, 3)))
+    q = q * np.sign(np.diagonal(r, axis1=1, axis2=2))[:, None, :]
+    neg = np.linalg.det(q) < 0
+    q[neg, :, 0] *= -1.0
+    return q
+
+
+def random_stretch(rng, n, lo=LAM_LO, hi=LAM_HI):
+    """Symmetric right stretch tensors: principal stretches uniform in [lo, hi], random axes."""
+    lam = rng.uniform(lo, hi, size=(n, 3))
+    Q = haar_rotations(rng, n)
+    return np.einsum("nij,nj,nkj->nik", Q, lam, Q)
+
+
+def _reflect(x, lo, hi):
+    w = hi - lo
+    y = np.mod(x - lo, 2.0 * w)
+    return lo + np.where(y > w, 2.0 * w - y, y)
+
+
+def walk_stretch(rng, U, sig_lam=0.05, sig_rot=0.25, lo=LAM_LO, hi=LAM_HI):
+    """One step of a reflected random walk of stretch tensors (principal stretches stay in
+    [lo, hi]; the principal axes turn by a random small rotation)."""
+    lam, Q = np.linalg.eigh(U)
+    lam = _reflect(lam + rng.normal(scale=sig_lam, size=lam.shape), lo, hi)
+    ax = rng.normal(size=(len(U), 3))
+    ax /= np.linalg.norm(ax, axis=1, keepdims=True)
+    ang = rng.normal(scale=sig_rot, size=len(U))
+    K = np.zeros((len(U), 3, 3))
+    K[:, 0, 1], K[:, 0, 2], K[:, 1, 2] = -ax[:, 2], ax[:, 1], -ax[:, 0]
+    K -= K.transpose(0, 2, 1)
+    s, c = np.sin(ang)[:, None, None], np.cos(ang)[:, None, None]
+    R = np.eye(3)[None] + s * K + (1.0 - c) * (K @ K)  # Rodrigues
+    Q = R @ Q
+    return np.einsum("nij,nj,nkj->nik", Q, lam, Q)
+
+
+def green_lagrange(F):
+    F = np.asarray(F)
+    return 0.5 * (np.einsum("...ki,...kj->...ij", F, F) - np.eye(3))
+
+
+def to_voigt(T):
+    return np.stack([T[..., i, j] for i, j in VOIGT], axis=-1)
+
+
+def from_up21(c):
+    """(..., 21) upper Voigt -> (..., 6, 6) symmetric."""
+    c = np.asarray(c)
+    out = np.zeros(c.shape[:-1] + (6, 6))
+    k = 0
+    for a in range(6):
+        for b in range(a, 6):
+            out[..., a, b] = out[..., b, a] = c[..., k]
+            k += 1
+    return out
+
+
+def linear_part(Clin, F):
+    """C_lin : E (Voigt, engineering shear in E) for F (..., 3, 3)."""
+    E = green_lagrange(F)
+    Ee = to_voigt(E) * np.array([1, 1, 1, 2, 2, 2.0])
+    return np.einsum("...ab,...b->...a", from_up21(Clin), Ee)
+
+
+def stress_rel_error(S, Sref):
+    """Tensor-Frobenius relative error of Voigt stresses (shear counted twice)."""
+    w = np.array([1, 1, 1, 2, 2, 2.0])
+    d = np.sqrt(np.sum(w * (S - Sref) ** 2, axis=-1))
+    return d / np.maximum(np.sqrt(np.sum(w * Sref ** 2, axis=-1)), 1e-300)
+
+
+# ------------------------------------------------------------------------------------------
+# descriptors and surrogate loading
+# ------------------------------------------------------------------------------------------
+def descriptors(batch, pcg_rtol=1e-8):
+    """C_lin (n_rve, 21): the condensed tangent of every RVE at F = I, as upper Voigt dS/dE.
+    Leaves the batch reset (undeformed) and with the default solver settings."""
+    n = batch.n
+    batch.reset()
+    batch.set_solver(pcg_rtol=pcg_rtol)
+    try:
+        P, A, _ = batch.solve(np.repeat(np.eye(3)[None], n, 0), tangent=True)
+    finally:
+        batch.set_solver()
+    dF = hiprt.DeviceArray.from_numpy(np.tile(np.eye(3).reshape(9), n))
+    dP = hiprt.DeviceArray.from_numpy(np.ascontiguousarray(P.transpose(0, 2, 1).reshape(-1)))
+    dA = hiprt.DeviceArray.from_numpy(np.ascontiguousarray(A.reshape(-1)))
+    dext = hiprt.DeviceArray(n * gpu.EXT_PER_GP)
+    gpu.rve_to_external_device(n, dF.ptr, dP.ptr, dA.ptr, dext.ptr)
+    return dext.numpy().reshape(n, gpu.EXT_PER_GP)[:, 6:].copy()
+
+
+def surrogate_cfg(meta):
+    return {k: meta[k] for k in ("u_scale", "lam_lo", "lam_hi", "e_ref", "d_mean", "d_std", "d_lo", "d_hi",
+                                 "t_mean", "t_std")}
+
+
+def load_surrogate(path=SURROGATE_PATH, max_points=4096):
+    """(Mlp, Surrogate, meta) from a trained surrogate file (numpy .npz, no pickles)."""
+    with np.load(path, allow_pickle=False) as z:
+        meta = {k: z[k] for k in z.files}
+    width = int(meta["width"])
+    mlp = gpu.Mlp(N_IN, width, N_OUT, max_batch=max_points, seed=1)
+    mlp.set_params(meta["params"].astype(np.float32))
+    sur = gpu.Surrogate(mlp, surrogate_cfg(meta), max_points)
+    return mlp, sur, meta
+
+
+# ------------------------------------------------------------------------------------------
+# the macro problem
+# ------------------------------------------------------------------------------------------
+class MacroFE2:
+    """n^3 hex8 macro cube; Gauss point g of element e is RVE 8 e + g of a seeded batch
+    (sides drawn from `side_choices`, 20 % spherical inclusions, Neo-Hookean phases)."""
+
+    def __init__(self, n_macro=8, seed=rve.SEED, side_choices=rve.SIDES, surrogate=None, X=None, conn=None):
+        if X is None:
+            X, conn = meshgen.cube(n_macro)
+        self.X, self.conn = X, conn
+        self.mesh = gpu.CeosMesh(X, conn)
+        self.mesh.set_material(1, gpu.MODEL_EXT, [])
+        self.nn, self.ne, self.ndof = self.mesh.nn, self.mesh.ne, self.mesh.ndof
+        self.ngp = 8 * self.ne
+        self.sides, self.phases = rve.batch(self.ngp, seed, side_choices)
+        self.rves = gpu.RveBatch(self.sides, self.phases, gpu.MODEL_NH, PROPS)
+        rm, col = self.mesh.csr()
+        self.nnz = len(col)
+        D = hiprt.DeviceArray
+        self.dU = D(self.ndof)
+        self.dF = D(9 * self.ngp)
+        self.dP = D(9 * self.ngp)
+        self.dA = D(81 * self.ngp)
+        self.dinfo = D(4 * self.ngp, np.int32)
+        self.dext = D(gpu.EXT_PER_GP * self.ngp)
+        self.dFint = D(self.ndof)
+        self.dK = D(self.nnz)
+        self.dClin = D(21 * self.ngp)
+        self.dflags = D(self.ngp, np.int32)
+        self.stream = hiprt.Stream()
+        self.mesh.set_external(self.dext.ptr)
+        self.Clin = None
+        self.sur = surrogate  # (Mlp, Surrogate, meta) or None
+        self.judge_rng = np.random.default_rng(seed)
+
+    def setup_descriptors(self, pcg_rtol=1e-8):
+        t = time.perf_counter()
+        self.Clin = descriptors(self.rves, pcg_rtol)
+        self.dClin.upload(self.Clin.reshape(-1))
+        return time.perf_counter() - t
+
+    def _kinematics(self, U):
+        s = self.stream.ptr
+        self.dU.upload(np.ascontiguousarray(U, dtype=np.float64))
+        self.mesh.update_mesh_device(self.dU.ptr, s)
+        self.mesh.gp_deformation_gradient_device(self.dU.ptr, self.dF.ptr, s)
+
+    def _assemble(self):
+        self.mesh.evaluate_device(0.0, self.dU.ptr, self.dFint.ptr, self.dK.ptr, self.stream.ptr)
+
+    def _finish(self, stats):
+        self.stream.synchronize()
+        code, st = self.mesh.device_status()
+        if code != 0:
+            raise gpu.CeosError(code, "macro element evaluation")
+        return stats
+
+    def gp_F(self):
+        """Gauss-point F of the last evaluation, (ngp, 3, 3)."""
+        self.stream.synchronize()
+        return self.dF.numpy().reshape(self.ngp, 3, 3).transpose(0, 2, 1)
+
+    def ext(self):
+        """(ngp, 27): S (Voigt) and dS/dE (upper 21) of the last evaluation."""
+        self.stream.synchronize()
+        return self.dext.numpy().reshape(self.ngp, gpu.EXT_PER_GP)
+
+    def fint(self):
+        self.stream.synchronize()
+        return self.dFint.numpy()
+
+    def tangent(self):
+        self.stream.synchronize()
+        return self.dK.numpy()
+
+    def evaluate_rve(self, U):
+        """Plain macro evaluation: every RVE solved and condensed, then Fint and K."""
+        t0 = time.perf_counter()
+        s = self.stream.ptr
+        self._kinematics(U)
+        self.rves.solve_device(self.dF.ptr, self.dP.ptr, self.dA.ptr, self.dinfo.ptr, s)
+        gpu.rve_to_external_device(self.ngp, self.dF.ptr, self.dP.ptr, self.dA.ptr, self.dext.ptr, stream=s)
+        self._assemble()
+        self.stream.synchronize()
+        info = self.dinfo.numpy().reshape(-1, 4)
+        if np.any(info[:, 0] != 0):
+            raise gpu.CeosError(gpu.ERR_LOCAL_NR, "RVE solve (%d points)" % int(np.sum(info[:, 0] != 0)))
+        return self._finish({"mode": "rve", "seconds": time.perf_counter() - t0, "rve_solves": self.ngp,
+                             "newton_mean": float(info[:, 1].mean()), "cg_total": int(info[:, 2].sum())})
+
+    def evaluate_surrogate(self, U, judge_frac=0.01, judge_tol=0.02):
+        """Surrogate macro evaluation: MLP for points inside the training hull, RVE solve (with
+        tangent) for the others, RVE re-solve of ceil(judge_frac ngp) sampled inside points.
+        If a judged point misses by more than judge_tol (relative, Frobenius) the whole
+        evaluation falls back to the plain path."""
+        if self.sur is None or self.Clin is None:
+            raise RuntimeError("surrogate evaluation needs a surrogate and setup_descriptors()")
+        t0 = time.perf_counter()
+        s = self.stream.ptr
+        self._kinematics(U)
+        self.sur[1].eval_device(self.ngp, self.dF.ptr, self.dClin.ptr, self.dext.ptr, self.dflags.ptr, None, s)
+        self.stream.synchronize()
+        flags = self.dflags.numpy()
+        Fcm = self.dF.numpy().reshape(self.ngp, 9)
+        Fm = Fcm.reshape(self.ngp, 3, 3).transpose(0, 2, 1)
+        out = np.nonzero(flags)[0].astype(np.int32)
+        if len(out):
+            P, A, _ = self.rves.solve_subset(out, Fm[out], tangent=True)
+            m = len(out)
+            D = hiprt.DeviceArray.from_numpy
+            dsel, dFs = D(out), D(np.ascontiguousarray(Fcm[out].reshape(-1)))
+            dPs = D(np.ascontiguousarray(P.transpose(0, 2, 1).reshape(-1)))
+            dAs = D(np.ascontiguousarray(A.reshape(-1)))
+            gpu.rve_to_external_device(m, dFs.ptr, dPs.ptr, dAs.ptr, self.dext.ptr, dsel=dsel.ptr, stream=s)
+        # the macro assembly runs while the judged points are re-solved
+        self._assemble()
+        inside = np.nonzero(flags == 0)[0]
+        nj = min(len(inside), int(math.ceil(judge_frac * self.ngp))) if judge_frac > 0 else 0
+        err = np.zeros(0)
+        if nj:
+            jid = np.sort(self.judge_rng.choice(inside, size=nj, replace=False)).astype(np.int32)
+            Pj, _, _ = self.rves.solve_subset(jid, Fm[jid])
+            Sj = np.linalg.solve(Fm[jid], Pj)
+            S_rve = to_voigt(0.5 * (Sj + Sj.transpose(0, 2, 1)))
+            S_nn = self.ext()[jid, :6]
+            err = stress_rel_error(S_nn, S_rve)
+        stats = {"mode": "surrogate", "outside_hull": int(len(out)), "judged": int(nj),
+                 "judge_max_err": float(err.max()) if nj else 0.0,
+                 "judge_mean_err": float(err.mean()) if nj else 0.0, "fallback": False}
+        if nj and err.max() > judge_tol:
+            r = self.evaluate_rve(U)
+            stats["fallback"] = True
+            stats["rve_solves"] = r["rve_solves"] + int(len(out)) + nj
+            stats["seconds"] = time.perf_counter() - t0
+            return stats
+        self._finish(stats)
+        stats["rve_solves"] = int(len(out)) + nj
+        stats["seconds"] = time.perf_counter() - t0
+        return stats

@@ -23,6 +23,8 @@ LIB_PATH = os.path.join(PKG, "lib", "libceos_gpu.so")
 
 OK, ERR_DETJ, ERR_LOCAL_NR, ERR_SLOT, ERR_ARG, ERR_HIP = 0, -1, -2, -3, -4, -5
 MODEL_GHL, MODEL_J2, MODEL_NH, MODEL_SVK, MODEL_CNH, MODEL_VISCO = 1, 2, 3, 5, 7, 12
+MODEL_EXT = 99  # FE^2 macro point: S and dS/dE per Gauss point from the caller
+EXT_PER_GP = 27
 STATE_STRIDE = 20
 K_FP64, K_FP32_STORED = 0, 1
 ASSEMBLY_GATHER, ASSEMBLY_COLORED = 0, 1
@@ -57,7 +59,17 @@ EXPORTS = [
     "ceos_rve_profile",
     "ceos_mlp_create", "ceos_mlp_destroy", "ceos_mlp_num_params", "ceos_mlp_get_params",
     "ceos_mlp_set_params", "ceos_mlp_gradients", "ceos_mlp_train_step", "ceos_mlp_forward",
+    "ceos_set_external_gp", "ceos_gp_deformation_gradient", "ceos_gp_deformation_gradient_device",
+    "ceos_rve_solve_subset", "ceos_rve_to_external_device", "ceos_mlp_shape",
+    "ceos_surrogate_create", "ceos_surrogate_destroy", "ceos_surrogate_features_device",
+    "ceos_surrogate_eval_device",
 ]
+
+
+class SurrogateCfg(C.Structure):
+    _fields_ = [("u_scale", C.c_double), ("lam_lo", C.c_double), ("lam_hi", C.c_double), ("e_ref", C.c_double),
+                ("d_mean", C.c_double * 21), ("d_std", C.c_double * 21), ("d_lo", C.c_double * 21),
+                ("d_hi", C.c_double * 21), ("t_mean", C.c_double * 6), ("t_std", C.c_double * 6)]
 
 _lib = None
 
@@ -123,6 +135,17 @@ def load_library(path=LIB_PATH):
     L.ceos_rve_solve.argtypes = [vp, _dp, _dp, vp, vp]
     L.ceos_rve_solve_device.argtypes = [vp, vp, vp, vp, vp, vp]
     L.ceos_rve_get_fluctuation.argtypes = [vp, i, _dp]
+    L.ceos_set_external_gp.argtypes = [vp, vp]
+    L.ceos_gp_deformation_gradient.argtypes = [vp, _dp, _dp]
+    L.ceos_gp_deformation_gradient_device.argtypes = [vp, vp, vp, vp]
+    L.ceos_rve_solve_subset.argtypes = [vp, i, _ip, _dp, _dp, vp, vp]
+    L.ceos_rve_to_external_device.argtypes = [i, vp, vp, vp, vp, vp, vp]
+    L.ceos_mlp_shape.argtypes = [vp, vp, vp, vp, vp]
+    L.ceos_surrogate_create.argtypes = [vp, C.POINTER(SurrogateCfg), i, C.POINTER(vp)]
+    L.ceos_surrogate_destroy.argtypes = [vp]
+    L.ceos_surrogate_destroy.restype = None
+    L.ceos_surrogate_features_device.argtypes = [vp, i, vp, vp, vp, vp, vp]
+    L.ceos_surrogate_eval_device.argtypes = [vp, i, vp, vp, vp, vp, vp, vp]
     _lib = L
     return L
 
@@ -281,6 +304,21 @@ class CeosMesh:
         _check(load_library().ceos_evaluate_device(self._h, time, dU, dFint, dK, stream),
                "ceos_evaluate_device")
 
+    def set_external(self, dext):
+        """Device pointer (int) of the [ne][8][27] external material points (MODEL_EXT)."""
+        _check(load_library().ceos_set_external_gp(self._h, dext), "ceos_set_external_gp")
+
+    def gp_deformation_gradient(self, U):
+        """F at every Gauss point, (ne, 8, 3, 3), element order."""
+        F = np.zeros(72 * self.ne)
+        _check(load_library().ceos_gp_deformation_gradient(self._h, np.ascontiguousarray(U, dtype=np.float64), F),
+               "ceos_gp_deformation_gradient")
+        return F.reshape(self.ne, 8, 3, 3).transpose(0, 1, 3, 2)
+
+    def gp_deformation_gradient_device(self, dU, dF, stream=None):
+        _check(load_library().ceos_gp_deformation_gradient_device(self._h, dU, dF, stream),
+               "ceos_gp_deformation_gradient_device")
+
     def update_mesh_device(self, dU, stream=None):
         _check(load_library().ceos_update_mesh_device(self._h, dU, stream), "ceos_update_mesh_device")
 
@@ -347,6 +385,28 @@ class RveBatch:
         Pm = P.reshape(self.n, 3, 3).transpose(0, 2, 1)
         return Pm, (A.reshape(self.n, 9, 9) if tangent else None), info
 
+    def solve_subset(self, sel, Fbar, tangent=False, check=True):
+        """Solve only the RVEs `sel` (distinct ids) at Fbar (len(sel), 3, 3); returns P, A, info for
+        those RVEs in the order of sel."""
+        sel = np.ascontiguousarray(sel, dtype=np.int32)
+        m = len(sel)
+        F = np.asarray(Fbar, dtype=np.float64).reshape(m, 3, 3)
+        Fc = np.ascontiguousarray(F.transpose(0, 2, 1).reshape(m, 9))
+        P = np.zeros((max(m, 1), 9))
+        A = np.zeros((max(m, 1), 81)) if tangent else None
+        info = np.zeros((max(m, 1), 4), dtype=np.int32)
+        code = load_library().ceos_rve_solve_subset(self._h, m, sel, Fc.reshape(-1) if m else np.zeros(9),
+                                                    P.reshape(-1), A.ctypes.data if tangent else None,
+                                                    info.ctypes.data)
+        if check:
+            _check(code, "ceos_rve_solve_subset")
+        Pm = P[:m].reshape(m, 3, 3).transpose(0, 2, 1)
+        return Pm, (A[:m].reshape(m, 9, 9) if tangent else None), info[:m]
+
+    def solve_device(self, dF, dP, dA=None, dinfo=None, stream=None):
+        """Device arrays (pointers): dF, dP 9 per RVE (column-major), dA 81 per RVE or None."""
+        _check(load_library().ceos_rve_solve_device(self._h, dF, dP, dA, dinfo, stream), "ceos_rve_solve_device")
+
     def profile(self):
         """(n_rve, 2) seconds in the Newton assemblies and in CG since the last reset."""
         t = np.zeros((self.n, 2), dtype=np.int64)
@@ -409,3 +469,49 @@ class Mlp:
     def forward(self, dX, n, dY, dJ=None, stream=None):
         _check(load_library().ceos_mlp_forward(self._h, dX.ptr, n, dY.ptr, dJ.ptr if dJ is not None else None,
                                                stream), "ceos_mlp_forward")
+
+
+def rve_to_external_device(n, dF, dP, dA, dext, dsel=None, stream=None):
+    """S = F^-1 P and dS/dE from dP/dF for n points into ext rows (dsel[k] or k)."""
+    _check(load_library().ceos_rve_to_external_device(n, dsel, dF, dP, dA, dext, stream),
+           "ceos_rve_to_external_device")
+
+
+class Surrogate:
+    """ceos_surrogate over an Mlp(27, width, 6): macro F + RVE descriptor -> S, dS/dE, hull flag.
+    cfg: dict with u_scale, lam_lo, lam_hi, e_ref, d_mean, d_std, d_lo, d_hi (21 each), t_mean,
+    t_std (6 each)."""
+
+    def __init__(self, mlp, cfg, max_points):
+        c = SurrogateCfg()
+        c.u_scale, c.lam_lo, c.lam_hi, c.e_ref = (float(cfg[k]) for k in ("u_scale", "lam_lo", "lam_hi", "e_ref"))
+        for k, n in (("d_mean", 21), ("d_std", 21), ("d_lo", 21), ("d_hi", 21), ("t_mean", 6), ("t_std", 6)):
+            v = np.asarray(cfg[k], dtype=np.float64).reshape(-1)
+            assert v.size == n, k
+            getattr(c, k)[:] = v.tolist()
+        self.cfg = c
+        self.mlp = mlp  # kept alive: the surrogate uses it
+        h = C.c_void_p()
+        _check(load_library().ceos_surrogate_create(mlp._h, C.byref(c), max_points, C.byref(h)),
+               "ceos_surrogate_create")
+        self._h = h
+        self.max_points = max_points
+
+    def close(self):
+        if getattr(self, "_h", None):
+            load_library().ceos_surrogate_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def features_device(self, n, dF, dClin, dX, dflags=None, stream=None):
+        _check(load_library().ceos_surrogate_features_device(self._h, n, dF, dClin, dX, dflags, stream),
+               "ceos_surrogate_features_device")
+
+    def eval_device(self, n, dF, dClin, dext, dflags=None, dP=None, stream=None):
+        _check(load_library().ceos_surrogate_eval_device(self._h, n, dF, dClin, dext, dflags, dP, stream),
+               "ceos_surrogate_eval_device")

@@ -80,6 +80,11 @@ class DeviceArray:
         check(hip().hipMemcpy(out.ctypes.data, self.ptr, self.nbytes, D2H), "hipMemcpy D2H")
         return out
 
+    def view(self, offset, n):
+        """Non-owning view of elements [offset, offset + n)."""
+        assert 0 <= offset and offset + n <= self.n
+        return DeviceView(self.ptr + offset * self.dtype.itemsize, n, self.dtype, self)
+
     def free(self):
         if self.ptr:
             hip().hipFree(self.ptr)
@@ -90,6 +95,24 @@ class DeviceArray:
             self.free()
         except Exception:
             pass
+
+
+class DeviceView:
+    """A window into a DeviceArray (keeps the owner alive; never frees)."""
+
+    def __init__(self, ptr, n, dtype, owner):
+        self.ptr, self.n, self.dtype, self._owner = ptr, n, np.dtype(dtype), owner
+        self.nbytes = self.n * self.dtype.itemsize
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        assert a.size == self.n
+        check(hip().hipMemcpy(self.ptr, a.ctypes.data, self.nbytes, H2D), "hipMemcpy H2D")
+
+    def numpy(self):
+        out = np.empty(self.n, dtype=self.dtype)
+        check(hip().hipMemcpy(out.ctypes.data, self.ptr, self.nbytes, D2H), "hipMemcpy D2H")
+        return out
 
 
 class Stream:

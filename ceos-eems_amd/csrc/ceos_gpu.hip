@@ -71,13 +71,14 @@ void set_status(ceos_status* st, int code, int nfail, const char* msg) {
 }
 
 bool model_supported(int m) {
-    return m == kGHL || m == kJ2 || m == kNH || m == kSVK || m == kCNH || m == kVISCO;
+    return m == kGHL || m == kJ2 || m == kNH || m == kSVK || m == kCNH || m == kVISCO || m == kEXT;
 }
 
 int model_nprops(int m) {
     switch (m) {
         case kJ2: return 4;
         case kVISCO: return 6;
+        case kEXT: return 0;
         default: return 2;
     }
 }
@@ -121,6 +122,9 @@ struct ceos_handle {
     float* dK32 = nullptr;
     double *dsig = nullptr, *dst_old = nullptr, *dst_new = nullptr, *dprops = nullptr;
     int *dconn = nullptr, *dmat = nullptr, *drowstart = nullptr, *dnup = nullptr;
+    int* dperm = nullptr;               // slot -> element (device copy of perm)
+    const double* dext = nullptr;       // caller's external material points (model kEXT)
+    bool has_ext = false;
     uint32_t* dpairs = nullptr;
     uint8_t *ddiag_first = nullptr, *delem_err = nullptr;
     int* derr = nullptr;
@@ -219,7 +223,7 @@ static inline int rank_of(const ceos_handle* h, int A, int B) {  // A < B
 static void free_device(ceos_handle* h) {
     dfree(h->dX); dfree(h->dx); dfree(h->dU); dfree(h->dFint); dfree(h->dK); dfree(h->dK32);
     dfree(h->dsig); dfree(h->dst_old); dfree(h->dst_new); dfree(h->dprops);
-    dfree(h->dconn); dfree(h->dmat); dfree(h->drowstart); dfree(h->dnup);
+    dfree(h->dconn); dfree(h->dmat); dfree(h->drowstart); dfree(h->dnup); dfree(h->dperm);
     dfree(h->dpairs); dfree(h->ddiag_first); dfree(h->delem_err); dfree(h->derr); dfree(h->dhpart);
     dfree(h->dscratch); dfree(h->dblknode); dfree(h->dblkstart); dfree(h->dcptr); dfree(h->dcword);
 }
@@ -335,6 +339,7 @@ static int finalize(ceos_handle* h) {
     models.erase(std::unique(models.begin(), models.end()), models.end());
     h->has_state = std::find(models.begin(), models.end(), (int)kJ2) != models.end() ||
                    std::find(models.begin(), models.end(), (int)kVISCO) != models.end();
+    h->has_ext = std::find(models.begin(), models.end(), (int)kEXT) != models.end();
     h->perm.clear();
     h->ranges.clear();
     {
@@ -418,6 +423,8 @@ static int finalize(ceos_handle* h) {
 
     HIP_OK(dalloc(h->dconn, (size_t)ne * 8));
     HIP_OK(dalloc(h->dmat, ne));
+    HIP_OK(dalloc(h->dperm, ne));
+    HIP_OK(hipMemcpyAsync(h->dperm, h->perm.data(), sizeof(int) * ne, hipMemcpyHostToDevice, h->stream));
     if (colored) {
         HIP_OK(dalloc(h->dpairs, (size_t)ne * 8));
         HIP_OK(dalloc(h->ddiag_first, ne));
@@ -503,6 +510,7 @@ static void launch_range(const ElemParams& p, const LaunchRange& r, bool nl, boo
         case kSVK: launch_m<kSVK, KT, G>(p, r.begin, r.end, nl, wk, numt, s); break;
         case kCNH: launch_m<kCNH, KT, G>(p, r.begin, r.end, nl, wk, numt, s); break;
         case kVISCO: launch_m<kVISCO, KT, G>(p, r.begin, r.end, nl, wk, numt, s); break;
+        case kEXT: launch_m<kEXT, KT, G>(p, r.begin, r.end, nl, wk, false, s); break;  // tangent is given
         default: break;
     }
 }
@@ -511,8 +519,11 @@ static void launch_range(const ElemParams& p, const LaunchRange& r, bool nl, boo
 static int evaluate(ceos_handle* h, const double* dU, double* dFint, void* dK, hipStream_t s) {
     int st = finalize(h);
     if (st) return st;
+    if (h->has_ext && !h->dext) return CEOS_ERR_ARG;  // ceos_set_external_gp was not called
     ElemParams p;
     p.ne = h->ne;
+    p.eid = h->dperm;
+    p.ext = h->dext;
     p.conn = h->dconn;
     p.mat = h->dmat;
     p.props = h->dprops;
@@ -687,7 +698,8 @@ void ceos_mesh_destroy(ceos_handle* h) {
 }
 
 int ceos_material_set(ceos_handle* h, int mat_id, int model, const double* props, int nprops) {
-    if (!h || !props || !model_supported(model) || nprops < model_nprops(model) || nprops > kMaxProps)
+    if (!h || (!props && nprops > 0) || !model_supported(model) || nprops < model_nprops(model) ||
+        nprops > kMaxProps)
         return CEOS_ERR_ARG;
     auto it = std::find(h->mat_ids.begin(), h->mat_ids.end(), mat_id);
     if (it == h->mat_ids.end()) return CEOS_ERR_ARG;
@@ -696,7 +708,7 @@ int ceos_material_set(ceos_handle* h, int mat_id, int model, const double* props
     if (m.set && m.model != model) h->finalized = false;  // launch grouping changes
     if (!m.set) h->finalized = false;
     m.model = model;
-    for (int k = 0; k < kMaxProps; ++k) m.p[k] = k < nprops ? props[k] : 0.0;
+    for (int k = 0; k < kMaxProps; ++k) m.p[k] = k < nprops && props ? props[k] : 0.0;
     m.set = true;
     h->f_valid = h->k_valid = false;
     return upload_props(h);
@@ -1077,6 +1089,46 @@ int ceos_timing_stop(ceos_handle* h, double* element_ms, double* assembly_ms, in
 int ceos_device_status(ceos_handle* h, ceos_status* st) {
     if (!h) return CEOS_ERR_ARG;
     return read_status(h, st);
+}
+
+int ceos_set_external_gp(ceos_handle* h, const double* dext) {
+    if (!h) return CEOS_ERR_ARG;
+    h->dext = dext;
+    h->f_valid = h->k_valid = false;
+    return CEOS_OK;
+}
+
+int ceos_gp_deformation_gradient_device(ceos_handle* h, const double* dU, double* dF, void* stream) {
+    if (!h || !dU || !dF) return CEOS_ERR_ARG;
+    int rc = finalize(h);
+    if (rc) return rc;
+    hipStream_t s = stream ? (hipStream_t)stream : h->stream;
+    const long n = 8L * h->ne;
+    hipLaunchKernelGGL(k_gp_deformation_gradient, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, h->ne,
+                       h->dconn, h->dperm, h->dX, dU, dF);
+    HIP_OK(hipGetLastError());
+    return CEOS_OK;
+}
+
+int ceos_gp_deformation_gradient(ceos_handle* h, const double* U, double* F) {
+    if (!h || !U || !F) return CEOS_ERR_ARG;
+    int rc = finalize(h);
+    if (rc) return rc;
+    double *dUt = nullptr, *dF = nullptr;  // temporaries: the handle's own U stays untouched
+    if (hipMalloc((void**)&dUt, sizeof(double) * 3L * h->nn) != hipSuccess) return CEOS_ERR_HIP;
+    if (hipMalloc((void**)&dF, sizeof(double) * 72L * h->ne) != hipSuccess) {
+        (void)hipFree(dUt);
+        return CEOS_ERR_HIP;
+    }
+    rc = CEOS_ERR_HIP;
+    if (hipMemcpyAsync(dUt, U, sizeof(double) * 3L * h->nn, hipMemcpyHostToDevice, h->stream) == hipSuccess &&
+        ceos_gp_deformation_gradient_device(h, dUt, dF, h->stream) == CEOS_OK &&
+        hipMemcpyAsync(F, dF, sizeof(double) * 72L * h->ne, hipMemcpyDeviceToHost, h->stream) == hipSuccess &&
+        hipStreamSynchronize(h->stream) == hipSuccess)
+        rc = CEOS_OK;
+    (void)hipFree(dUt);
+    (void)hipFree(dF);
+    return rc;
 }
 
 }  // extern "C"

@@ -88,8 +88,10 @@ __device__ __forceinline__ double hex8_grad(const double dN[8][3], const double 
 // ------------------------------------------------------------------------------------------
 // materials. props layout per model (as set through ceos_material_set):
 //   1 GHL: E, nu   2 J2: E, nu, sigma_y, H   3 NH: C10, K   5 SVK: E, nu   7 CNH: mu, lambda
+//   99 EXT: none (stress and tangent come per Gauss point from the caller, see mat_ext)
 // ------------------------------------------------------------------------------------------
-enum : int { kGHL = 1, kJ2 = 2, kNH = 3, kSVK = 5, kCNH = 7, kVISCO = 12 };
+enum : int { kGHL = 1, kJ2 = 2, kNH = 3, kSVK = 5, kCNH = 7, kVISCO = 12, kEXT = 99 };
+constexpr int kExtPerGp = 27;  // external material point: S (6) + dS/dE upper (21)
 
 __device__ __forceinline__ void elastic_up(double E, double nu, double D[21]) {
     // ModGeneralizedHookesLaw .f90:526-560, ModJ2Plasticity.f90:587-597
@@ -225,6 +227,27 @@ __device__ __forceinline__ void push_forward_up(const double W[21], const double
             for (int k = 0; k < 6; ++k) s += AW[i][k] * A[j][k];
             T[up_idx(i, j)] = s * rJ;
         }
+}
+
+// External material point (an FE^2 macro Gauss point whose response comes from an RVE solve or a
+// surrogate): q = second Piola-Kirchhoff stress S (Voigt 11,22,33,12,23,13) followed by the
+// upper triangle of its material tangent dS/dE (engineering shear).  The Cauchy stress
+// sigma = F S F^T / J and the spatial tangent follow by the push forward the hyperelastic models
+// use (ModNeoHookean.f90 calls Push_Forward_Voigt the same way).
+__device__ __forceinline__ void mat_ext(const double* q, const double F[3][3], double s[6], double D[21]) {
+    const double J = det3(F);
+    const double S[3][3] = {{q[0], q[3], q[5]}, {q[3], q[1], q[4]}, {q[5], q[4], q[2]}};
+    double FS[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) FS[i][j] = F[i][0] * S[0][j] + F[i][1] * S[1][j] + F[i][2] * S[2][j];
+    const double rJ = 1.0 / J;
+    const int vi[6] = {0, 1, 2, 0, 1, 0}, vj[6] = {0, 1, 2, 1, 2, 2};
+#pragma unroll
+    for (int a = 0; a < 6; ++a)
+        s[a] = (FS[vi[a]][0] * F[vj[a]][0] + FS[vi[a]][1] * F[vj[a]][1] + FS[vi[a]][2] * F[vj[a]][2]) * rJ;
+    push_forward_up(q + 6, F, J, D);
 }
 
 // model 3, ModNeoHookean.f90:409-615

@@ -64,6 +64,8 @@ struct ElemParams {
     uint8_t* elem_err;             // [ne]: bit 0 detJ <= 1e-13, bit 1 local Newton failed
     int* err_count;                // [0] elements with detJ failures, [1] local Newton failures
     double time;                   // analysis time of the evaluation (viscous time step)
+    const int* eid;                // [ne] slot -> element id (0-based)
+    const double* ext;             // [ne_elements][8][27] external material points (model kEXT)
 };
 
 __device__ __forceinline__ long rowptr(int rs, int nu, int i) {
@@ -72,7 +74,7 @@ __device__ __forceinline__ long rowptr(int rs, int nu, int i) {
 
 template <int MODEL>
 constexpr int model_nstate() {
-    return MODEL == kJ2 ? 7 : (MODEL == kVISCO ? kViscoState : 1);
+    return MODEL == kJ2 ? 7 : (MODEL == kVISCO ? kViscoState : (MODEL == kEXT ? kExtPerGp : 1));
 }
 
 // returns the J2 PlasticState flag, or for the viscoelastic model whether its local Newton
@@ -85,6 +87,7 @@ __device__ __forceinline__ bool material(const double* p, const double F[3][3], 
     else if constexpr (MODEL == kNH) { mat_nh(p, F, s, D); return false; }
     else if constexpr (MODEL == kSVK) { mat_svk(p, F, s, D); return false; }
     else if constexpr (MODEL == kVISCO) { return mat_visco(p, F, time, old, s, D, nw); }
+    else if constexpr (MODEL == kEXT) { mat_ext(old, F, s, D); return false; }
     else { return mat_j2(p, F, old, s, D, nw); }
 }
 
@@ -310,8 +313,15 @@ __global__ __launch_bounds__(128) void k_element(ElemParams p, int e_begin, int 
                 for (int v = 0; v < NS; ++v) old[v] = p.st_old[((long)v * 8 + g) * p.ne + e];
             }
         }
+        if constexpr (MODEL == kEXT) {
+            if (active) {
+                const double* q = p.ext + ((long)p.eid[e] * 8 + g) * kExtPerGp;
+#pragma unroll
+                for (int v = 0; v < NS; ++v) old[v] = q[v];
+            }
+        }
         const bool flag = material<MODEL>(pr, F, p.time, old, s, D, nw);
-        if constexpr (WANT_K && NUMT) tangent_numerical<MODEL>(pr, F, p.time, old, s, D);
+        if constexpr (WANT_K && NUMT && MODEL != kEXT) tangent_numerical<MODEL>(pr, F, p.time, old, s, D);
         if (active) {
 #pragma unroll
             for (int k = 0; k < 6; ++k) p.sig[((long)k * 8 + g) * p.ne + e] = s[k];

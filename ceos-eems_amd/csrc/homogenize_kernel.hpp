@@ -87,4 +87,40 @@ __global__ void k_add(long n, const double* __restrict__ a, const double* __rest
 
 __global__ void k_zero_int(int* p) { p[0] = 0; p[1] = 0; }
 
+// Deformation gradient at every Gauss point, F = I + sum_a u_a (x) grad_X N_a
+// (ModElement.f90:1210-1262, the arithmetic of the element kernel's phase 1), in element order:
+// F[(e 8 + g) 9 + 3J + i] = F_iJ (column-major 9-vectors, the reference's Voigt 9 layout).
+// One thread per (slot, Gauss point); eid maps the slot to its element.
+__global__ __launch_bounds__(256) void k_gp_deformation_gradient(int ne, const int* __restrict__ conn,
+                                                                 const int* __restrict__ eid,
+                                                                 const double* __restrict__ X,
+                                                                 const double* __restrict__ U,
+                                                                 double* __restrict__ Fout) {
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 8L * ne) return;
+    const int s = (int)(t >> 3), g = (int)(t & 7);
+    double dN[8][3], c[8][3], u[8][3], gX[8][3];
+    hex8_dshape_gp(g, dN);
+#pragma unroll
+    for (int a = 0; a < 8; ++a) {
+        const int n = conn[(long)s * 8 + a];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            c[a][k] = X[3L * n + k];
+            u[a][k] = U[3L * n + k];
+        }
+    }
+    (void)hex8_grad(dN, c, gX);
+    double* o = Fout + ((long)eid[s] * 8 + g) * 9;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            double acc = 0.0;
+#pragma unroll
+            for (int a = 0; a < 8; ++a) acc += gX[a][j] * u[a][i];
+            o[3 * j + i] = acc + (i == j ? 1.0 : 0.0);
+        }
+}
+
 }  // namespace ceos

@@ -428,6 +428,15 @@ void ceos_mlp_destroy(ceos_mlp* m) {
 
 long long ceos_mlp_num_params(const ceos_mlp* m) { return m ? m->nparams : -1; }
 
+int ceos_mlp_shape(const ceos_mlp* m, int* n_in, int* width, int* n_out, int* max_batch) {
+    if (!m) return CEOS_ERR_ARG;
+    if (n_in) *n_in = m->n_in;
+    if (width) *width = m->width;
+    if (n_out) *n_out = m->n_out;
+    if (max_batch) *max_batch = m->max_batch;
+    return CEOS_OK;
+}
+
 int ceos_mlp_get_params(ceos_mlp* m, float* p) {
     if (!m || !p) return CEOS_ERR_ARG;
     MLP_HIP_OK(hipStreamSynchronize(m->stream));

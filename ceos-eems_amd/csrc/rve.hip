@@ -39,6 +39,9 @@ struct ceos_rve_batch {
     double *dprops = nullptr, *dFbar = nullptr, *dFconv = nullptr, *dw = nullptr, *dP = nullptr, *dA = nullptr;
     double *dM = nullptr, *dV = nullptr;
     long long* dprof = nullptr;
+    // subset solves: selected ids in queue order and in caller order, compact F / P / A / info
+    int *dsel_q = nullptr, *dsel_u = nullptr, *dinfo_s = nullptr;
+    double *dF_s = nullptr, *dP_s = nullptr, *dA_s = nullptr;
 };
 
 namespace {
@@ -59,6 +62,7 @@ void free_all(ceos_rve_batch* b) {
     rfree(b->dnode_off); rfree(b->dphase_off); rfree(b->dphase);
     rfree(b->dprops); rfree(b->dFbar); rfree(b->dFconv); rfree(b->dw); rfree(b->dP); rfree(b->dA);
     rfree(b->dM); rfree(b->dV); rfree(b->dprof);
+    rfree(b->dsel_q); rfree(b->dsel_u); rfree(b->dinfo_s); rfree(b->dF_s); rfree(b->dP_s); rfree(b->dA_s);
 }
 
 __global__ void k_rve_reset(int n_rve, long long nodes3, double* Fconv, double* w, long long* prof) {
@@ -72,6 +76,24 @@ __global__ void k_rve_reset(int n_rve, long long nodes3, double* Fconv, double* 
 }
 
 __global__ void k_zero_queue(int* q) { q[0] = 0; }
+
+// macro F of the selected RVEs into their rows of Fbar
+__global__ void k_rve_scatter_F(int n_sel, const int* sel, const double* Fs, double* Fbar) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 9 * n_sel) return;
+    Fbar[9LL * sel[t / 9] + t % 9] = Fs[t];
+}
+
+// results of the selected RVEs into compact arrays (caller order)
+__global__ void k_rve_gather(int n_sel, const int* sel, const double* P, const double* A, const int* info,
+                             double* Ps, double* As, int* infos) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 81 * n_sel) return;
+    const int k = t / 81, j = t % 81, r = sel[k];
+    if (As) As[t] = A[81LL * r + j];
+    if (j < 9) Ps[9 * k + j] = P[9LL * r + j];
+    if (j < 4) infos[4 * k + j] = info[4LL * r + j];
+}
 
 template <int MODEL>
 void launch(const RveParams& p, int grid, bool tangent, hipStream_t s) {
@@ -91,10 +113,11 @@ int reset(ceos_rve_batch* b) {
     return CEOS_OK;
 }
 
-int run(ceos_rve_batch* b, bool want_A, hipStream_t s) {
+// one persistent launch over the queue `order` (n_queue RVE ids, largest first)
+int run(ceos_rve_batch* b, bool want_A, hipStream_t s, const int* order, int n_queue) {
     RveParams p;
-    p.n_rve = b->n_rve;
-    p.order = b->dorder;
+    p.n_rve = n_queue;
+    p.order = order;
     p.side = b->dside;
     p.node_off = b->dnode_off;
     p.phase_off = b->dphase_off;
@@ -119,7 +142,7 @@ int run(ceos_rve_batch* b, bool want_A, hipStream_t s) {
     p.inexact = b->inexact;
     p.prof = b->dprof;
     hipLaunchKernelGGL(k_zero_queue, dim3(1), dim3(1), 0, s, b->dqueue);
-    const int grid = std::min(b->n_rve, b->slots);
+    const int grid = std::min(n_queue, b->slots);
     switch (b->model) {
         case kNH: launch<kNH>(p, grid, want_A, s); break;
         case kCNH: launch<kCNH>(p, grid, want_A, s); break;
@@ -187,7 +210,10 @@ int ceos_rve_batch_create(int n_rve, const int* side, const unsigned char* phase
         rmalloc(b->dw, 3 * tot) == hipSuccess && rmalloc(b->dP, 9LL * n_rve) == hipSuccess &&
         rmalloc(b->dA, 81LL * n_rve) == hipSuccess && rmalloc(b->dprof, 2LL * n_rve) == hipSuccess &&
         rmalloc(b->dM, (size_t)(243 * b->max_nodes * slots)) == hipSuccess &&
-        rmalloc(b->dV, (size_t)(kRveVecPerNode * b->max_nodes * slots)) == hipSuccess;
+        rmalloc(b->dV, (size_t)(kRveVecPerNode * b->max_nodes * slots)) == hipSuccess &&
+        rmalloc(b->dsel_q, n_rve) == hipSuccess && rmalloc(b->dsel_u, n_rve) == hipSuccess &&
+        rmalloc(b->dinfo_s, 4LL * n_rve) == hipSuccess && rmalloc(b->dF_s, 9LL * n_rve) == hipSuccess &&
+        rmalloc(b->dP_s, 9LL * n_rve) == hipSuccess && rmalloc(b->dA_s, 81LL * n_rve) == hipSuccess;
     if (!okm) rc = CEOS_ERR_HIP;
     if (rc == CEOS_OK) {
         bool okc =
@@ -247,7 +273,7 @@ int ceos_rve_solve(ceos_rve_batch* b, const double* Fbar, double* P, double* A, 
     if (!b || !Fbar || !P) return CEOS_ERR_ARG;
     const int n = b->n_rve;
     RVE_HIP_OK(hipMemcpyAsync(b->dFbar, Fbar, sizeof(double) * 9 * n, hipMemcpyHostToDevice, b->stream));
-    int rc = run(b, A != nullptr, b->stream);
+    int rc = run(b, A != nullptr, b->stream, b->dorder, n);
     if (rc) return rc;
     RVE_HIP_OK(hipMemcpyAsync(P, b->dP, sizeof(double) * 9 * n, hipMemcpyDeviceToHost, b->stream));
     if (A) RVE_HIP_OK(hipMemcpyAsync(A, b->dA, sizeof(double) * 81 * n, hipMemcpyDeviceToHost, b->stream));
@@ -265,11 +291,44 @@ int ceos_rve_solve_device(ceos_rve_batch* b, const double* dFbar, double* dP, do
     hipStream_t s = stream ? (hipStream_t)stream : b->stream;
     const int n = b->n_rve;
     RVE_HIP_OK(hipMemcpyAsync(b->dFbar, dFbar, sizeof(double) * 9 * n, hipMemcpyDeviceToDevice, s));
-    int rc = run(b, dA != nullptr, s);
+    int rc = run(b, dA != nullptr, s, b->dorder, n);
     if (rc) return rc;
     RVE_HIP_OK(hipMemcpyAsync(dP, b->dP, sizeof(double) * 9 * n, hipMemcpyDeviceToDevice, s));
     if (dA) RVE_HIP_OK(hipMemcpyAsync(dA, b->dA, sizeof(double) * 81 * n, hipMemcpyDeviceToDevice, s));
     if (dinfo) RVE_HIP_OK(hipMemcpyAsync(dinfo, b->dinfo, sizeof(int) * 4 * n, hipMemcpyDeviceToDevice, s));
+    return CEOS_OK;
+}
+
+int ceos_rve_solve_subset(ceos_rve_batch* b, int n_sel, const int* sel, const double* Fsel, double* Psel,
+                          double* Asel, int* info_sel) {
+    if (!b || n_sel < 0 || n_sel > b->n_rve || (n_sel > 0 && (!sel || !Fsel || !Psel))) return CEOS_ERR_ARG;
+    if (n_sel == 0) return CEOS_OK;
+    std::vector<char> seen(b->n_rve, 0);
+    for (int k = 0; k < n_sel; ++k) {
+        if (sel[k] < 0 || sel[k] >= b->n_rve || seen[sel[k]]) return CEOS_ERR_ARG;  // a duplicate would race
+        seen[sel[k]] = 1;
+    }
+    std::vector<int> q(sel, sel + n_sel);
+    std::stable_sort(q.begin(), q.end(), [&](int a, int c) { return b->side[a] > b->side[c]; });
+    hipStream_t s = b->stream;
+    RVE_HIP_OK(hipMemcpyAsync(b->dsel_q, q.data(), sizeof(int) * n_sel, hipMemcpyHostToDevice, s));
+    RVE_HIP_OK(hipMemcpyAsync(b->dsel_u, sel, sizeof(int) * n_sel, hipMemcpyHostToDevice, s));
+    RVE_HIP_OK(hipMemcpyAsync(b->dF_s, Fsel, sizeof(double) * 9 * n_sel, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_rve_scatter_F, dim3((9 * n_sel + 255) / 256), dim3(256), 0, s, n_sel, b->dsel_u, b->dF_s,
+                       b->dFbar);
+    int rc = run(b, Asel != nullptr, s, b->dsel_q, n_sel);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_rve_gather, dim3((81 * n_sel + 255) / 256), dim3(256), 0, s, n_sel, b->dsel_u, b->dP,
+                       Asel ? b->dA : nullptr, b->dinfo, b->dP_s, Asel ? b->dA_s : nullptr, b->dinfo_s);
+    RVE_HIP_OK(hipGetLastError());
+    std::vector<int> inf(4 * (size_t)n_sel);
+    RVE_HIP_OK(hipMemcpyAsync(Psel, b->dP_s, sizeof(double) * 9 * n_sel, hipMemcpyDeviceToHost, s));
+    if (Asel) RVE_HIP_OK(hipMemcpyAsync(Asel, b->dA_s, sizeof(double) * 81 * n_sel, hipMemcpyDeviceToHost, s));
+    RVE_HIP_OK(hipMemcpyAsync(inf.data(), b->dinfo_s, sizeof(int) * 4 * n_sel, hipMemcpyDeviceToHost, s));
+    RVE_HIP_OK(hipStreamSynchronize(s));
+    if (info_sel) std::memcpy(info_sel, inf.data(), sizeof(int) * 4 * n_sel);
+    for (int k = 0; k < n_sel; ++k)
+        if (inf[4 * k] != 0) return CEOS_ERR_LOCAL_NR;
     return CEOS_OK;
 }
 

@@ -33,6 +33,9 @@ extern "C" {
 #define CEOS_MODEL_ST_VENANT_KIRCHHOFF 5    /* props: E, nu */
 #define CEOS_MODEL_COMPRESSIBLE_NEO_HOOKEAN 7 /* props: mu, lambda */
 #define CEOS_MODEL_VISCOELASTIC_MATRIX 12   /* props: K_inf, Mu_inf, Lambda_inf, K_e, Mu_e, Ni_v */
+/* No counterpart in the reference's library: an FE^2 macro Gauss point whose second Piola stress
+ * and material tangent come from the caller (ceos_set_external_gp); no props. */
+#define CEOS_MODEL_EXTERNAL 99
 
 /* Gauss-point state rows exchanged by ceos_get_state / ceos_set_state (per Gauss point):
  *   J2:   [0..5] plastic strain (engineering shear), [6] accumulated plastic strain,
@@ -170,6 +173,25 @@ int ceos_set_tangent_mode(ceos_handle *h, int mode);
 /* Mesh facts after setup: number of element colours and the nnz of the upper pattern. */
 int ceos_mesh_info(ceos_handle *h, int *n_colors, long long *nnz_upper);
 
+/* FE^2 macro meshes (elements of CEOS_MODEL_EXTERNAL).  The reference has no concurrent FE^2
+ * loop; these stand where a macro analysis would call the RVE analysis of
+ * MultiScale/ModMultiscaleAnalysis.f90 at every Gauss point.
+ *
+ * ceos_set_external_gp: device array [ne][8][27] in element order (0-based element, Gauss point
+ * in the hex8 order), per point the second Piola stress S (Voigt 11,22,33,12,23,13) then the
+ * upper triangle of dS/dE (21, engineering shear).  The element kernel forms sigma = F S F^T / J
+ * and pushes the tangent forward (Push_Forward_Voigt, Tools/MathTools/ModMathRoutines.f90:254-311).
+ * The pointer is kept until the next call; an evaluation with an external material and no array
+ * returns CEOS_ERR_ARG. */
+int ceos_set_external_gp(ceos_handle *h, const double *dext);
+
+/* Deformation gradient at every Gauss point, F = I + sum_a u_a (x) grad_X N_a
+ * (DeformationGradient, FEM/Elements/ElementMethods/ModElement.f90:1210-1300), the values the
+ * element kernel uses: F[(e 8 + g) 9 + 3J + i] = F_iJ.  Device arrays on a caller stream
+ * (NULL = the handle's stream), or host arrays (synchronous). */
+int ceos_gp_deformation_gradient_device(ceos_handle *h, const double *dU, double *dF, void *stream);
+int ceos_gp_deformation_gradient(ceos_handle *h, const double *U, double *F);
+
 /* ------------------------------------------------------------------------------------------
  * Batched periodic RVEs (FE^2 micro problems).
  *
@@ -228,6 +250,20 @@ int ceos_rve_profile(ceos_rve_batch *b, long long *ticks);
 /* Converged fluctuation of RVE `rve` (3 side^3 doubles, torus node x + s (y + s z)). */
 int ceos_rve_get_fluctuation(ceos_rve_batch *b, int rve, double *w);
 
+/* The same solve for the RVEs listed in sel only (distinct ids), each from its last converged
+ * state; Fsel / Psel / Asel (may be NULL) / info_sel hold one entry per listed RVE, in the order
+ * of sel.  Host arrays; the call synchronises.  The other RVEs keep their state. */
+int ceos_rve_solve_subset(ceos_rve_batch *b, int n_sel, const int *sel, const double *Fsel, double *Psel,
+                          double *Asel, int *info_sel);
+
+/* RVE results into external material points: for k < n, S = F^-1 P and
+ * dS_IJ/dE_KL = sum_ik Finv_Ii Finv_Kk (dP_iJ/dF_kL - delta_ik S_JL)
+ * (First_Elasticity_Modulus_To_Second_Voigt, Tools/MathTools/ModMathRoutines.f90:315), averaged
+ * over the index symmetries, written to ext row dsel[k] (dsel NULL: row k).  dF, dP: 9 per k
+ * (column-major), dA: 81 per k as ceos_rve_solve returns it.  Device arrays. */
+int ceos_rve_to_external_device(int n, const int *dsel, const double *dF, const double *dP, const double *dA,
+                                double *dext, void *stream);
+
 /* ------------------------------------------------------------------------------------------
  * MLP surrogate of the homogenized RVE response (no counterpart in the reference, which has no
  * concurrent FE^2 loop: it stands in for the batched RVE solve at macro points inside its
@@ -240,6 +276,7 @@ typedef struct ceos_mlp ceos_mlp;
 int ceos_mlp_create(int n_in, int width, int n_out, int max_batch, unsigned long long seed, ceos_mlp **m);
 void ceos_mlp_destroy(ceos_mlp *m);
 long long ceos_mlp_num_params(const ceos_mlp *m);
+int ceos_mlp_shape(const ceos_mlp *m, int *n_in, int *width, int *n_out, int *max_batch);
 int ceos_mlp_get_params(ceos_mlp *m, float *host_params);
 /* Also resets the Adam moments. */
 int ceos_mlp_set_params(ceos_mlp *m, const float *host_params);
@@ -256,6 +293,37 @@ int ceos_mlp_train_step(ceos_mlp *m, const float *dX, const float *dT, int batch
 /* FP32 inference: dY [n][n_out]; dJ (may be NULL) the input Jacobian [n][n_out][n_in] by reverse
  * mode.  stream NULL = the model's stream, synchronised before returning. */
 int ceos_mlp_forward(ceos_mlp *m, const float *dX, int n, float *dY, float *dJ, void *stream);
+
+/* ------------------------------------------------------------------------------------------
+ * Surrogate of the homogenized RVE response at FE^2 macro points (no counterpart in the
+ * reference).  Built on a ceos_mlp with n_in = 27, n_out = 6:
+ *   x = [(U - I)_Voigt / u_scale, (C_lin - d_mean) / d_std],  U = (F^T F)^(1/2)
+ *   S = C_lin : E + (|E|^2 / e_ref^2) (t_mean + t_std MLP(x)),  E = (F^T F - I) / 2
+ * C_lin (21 per point, upper Voigt) is the small-strain homogenized stiffness of the point's RVE
+ * (its condensed tangent at F = I).  dS/dE follows from the network's input Jacobian.  A point is
+ * inside the training hull when every principal stretch lies in [lam_lo, lam_hi] and every
+ * normalized descriptor component in [d_lo, d_hi]; flags[k] = 1 marks points outside.
+ * ------------------------------------------------------------------------------------------ */
+typedef struct ceos_surrogate ceos_surrogate;
+typedef struct {
+    double u_scale, lam_lo, lam_hi, e_ref;
+    double d_mean[21], d_std[21], d_lo[21], d_hi[21];
+    double t_mean[6], t_std[6];
+} ceos_surrogate_cfg;
+
+/* The surrogate uses mlp (not owned; it must outlive the surrogate) for up to max_points. */
+int ceos_surrogate_create(ceos_mlp *mlp, const ceos_surrogate_cfg *cfg, int max_points, ceos_surrogate **s);
+void ceos_surrogate_destroy(ceos_surrogate *s);
+
+/* Network inputs (dX [n][27] FP32) and hull flags (may be NULL) of n points. */
+int ceos_surrogate_features_device(ceos_surrogate *s, int n, const double *dF, const double *dClin, float *dX,
+                                   int *dflags, void *stream);
+
+/* External material points (dext [n][27], as ceos_set_external_gp takes them), hull flags (may
+ * be NULL) and first Piola stresses (dP, column-major 9 per point, may be NULL) of n points.
+ * Device arrays; stream NULL = synchronous. */
+int ceos_surrogate_eval_device(ceos_surrogate *s, int n, const double *dF, const double *dClin, double *dext,
+                               int *dflags, double *dP, void *stream);
 
 #ifdef __cplusplus
 }
