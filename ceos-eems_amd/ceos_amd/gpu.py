@@ -62,7 +62,7 @@ EXPORTS = [
     "ceos_set_external_gp", "ceos_gp_deformation_gradient", "ceos_gp_deformation_gradient_device",
     "ceos_rve_solve_subset", "ceos_rve_to_external_device", "ceos_mlp_shape",
     "ceos_surrogate_create", "ceos_surrogate_destroy", "ceos_surrogate_features_device",
-    "ceos_surrogate_eval_device",
+    "ceos_surrogate_eval_device", "ceos_rve_set_precision", "ceos_rve_set_tangent_rtol",
 ]
 
 
@@ -120,6 +120,8 @@ def load_library(path=LIB_PATH):
     L.ceos_rve_set_solver.argtypes = [vp, d, d, i, d, i, i]
     L.ceos_rve_reset.argtypes = [vp]
     L.ceos_rve_set_inexact.argtypes = [vp, i]
+    L.ceos_rve_set_precision.argtypes = [vp, i]
+    L.ceos_rve_set_tangent_rtol.argtypes = [vp, d]
     L.ceos_rve_profile.argtypes = [vp, vp]
     f = C.c_float
     L.ceos_mlp_create.argtypes = [i, i, i, i, C.c_ulonglong, C.POINTER(vp)]
@@ -371,6 +373,12 @@ class RveBatch:
 
     def reset(self):
         _check(load_library().ceos_rve_reset(self._h), "ceos_rve_reset")
+
+    def set_precision(self, mode):
+        _check(load_library().ceos_rve_set_precision(self._h, mode), "ceos_rve_set_precision")
+
+    def set_tangent_rtol(self, rtol):
+        _check(load_library().ceos_rve_set_tangent_rtol(self._h, rtol), "ceos_rve_set_tangent_rtol")
 
     def solve(self, Fbar, tangent=False, check=True):
         F = np.asarray(Fbar, dtype=np.float64).reshape(self.n, 3, 3)

@@ -32,6 +32,8 @@ struct ceos_rve_batch {
     double newton_rtol = 1e-10, newton_atol = 1e-13, pcg_rtol = 1e-10;
     int newton_maxit = 12, pcg_maxit = 5000, max_cutbacks = 20;
     int inexact = 1;
+    int precision = CEOS_K_FP64;  // storage of the reduced tangent
+    double tangent_rtol = -1.0;   // < 0: pcg_rtol
     // device
     int *dorder = nullptr, *dside = nullptr, *dinfo = nullptr, *dqueue = nullptr;
     long long *dnode_off = nullptr, *dphase_off = nullptr;
@@ -95,13 +97,19 @@ __global__ void k_rve_gather(int n_sel, const int* sel, const double* P, const d
     if (j < 4) infos[4 * k + j] = info[4LL * r + j];
 }
 
-template <int MODEL>
+template <int MODEL, typename MT>
 void launch(const RveParams& p, int grid, bool tangent, hipStream_t s) {
-    hipLaunchKernelGGL(k_rve_solve<MODEL>, dim3(grid), dim3(kRveThreads), 0, s, p);
+    hipLaunchKernelGGL((k_rve_solve<MODEL, MT>), dim3(grid), dim3(kRveThreads), 0, s, p);
     if (tangent) {
         hipLaunchKernelGGL(k_zero_queue, dim3(1), dim3(1), 0, s, p.queue);
-        hipLaunchKernelGGL(k_rve_tangent<MODEL>, dim3(grid), dim3(kRveThreads), 0, s, p);
+        hipLaunchKernelGGL((k_rve_tangent<MODEL, MT>), dim3(grid), dim3(kRveThreads), 0, s, p);
     }
+}
+
+template <int MODEL>
+void launch_p(const RveParams& p, int grid, bool tangent, bool f32, hipStream_t s) {
+    if (f32) launch<MODEL, float>(p, grid, tangent, s);
+    else launch<MODEL, double>(p, grid, tangent, s);
 }
 
 int reset(ceos_rve_batch* b) {
@@ -136,6 +144,7 @@ int run(ceos_rve_batch* b, bool want_A, hipStream_t s, const int* order, int n_q
     p.newton_rtol = b->newton_rtol;
     p.newton_atol = b->newton_atol;
     p.pcg_rtol = b->pcg_rtol;
+    p.tangent_rtol = b->tangent_rtol > 0.0 ? b->tangent_rtol : b->pcg_rtol;
     p.newton_maxit = b->newton_maxit;
     p.pcg_maxit = b->pcg_maxit;
     p.max_cutbacks = b->max_cutbacks;
@@ -143,10 +152,11 @@ int run(ceos_rve_batch* b, bool want_A, hipStream_t s, const int* order, int n_q
     p.prof = b->dprof;
     hipLaunchKernelGGL(k_zero_queue, dim3(1), dim3(1), 0, s, b->dqueue);
     const int grid = std::min(n_queue, b->slots);
+    const bool f32 = b->precision == CEOS_K_FP32_STORED;
     switch (b->model) {
-        case kNH: launch<kNH>(p, grid, want_A, s); break;
-        case kCNH: launch<kCNH>(p, grid, want_A, s); break;
-        case kSVK: launch<kSVK>(p, grid, want_A, s); break;
+        case kNH: launch_p<kNH>(p, grid, want_A, f32, s); break;
+        case kCNH: launch_p<kCNH>(p, grid, want_A, f32, s); break;
+        case kSVK: launch_p<kSVK>(p, grid, want_A, f32, s); break;
         default: return CEOS_ERR_ARG;
     }
     RVE_HIP_OK(hipGetLastError());
@@ -261,6 +271,18 @@ int ceos_rve_set_solver(ceos_rve_batch* b, double newton_rtol, double newton_ato
 int ceos_rve_set_inexact(ceos_rve_batch* b, int inexact) {
     if (!b || (inexact != 0 && inexact != 1)) return CEOS_ERR_ARG;
     b->inexact = inexact;
+    return CEOS_OK;
+}
+
+int ceos_rve_set_precision(ceos_rve_batch* b, int mode) {
+    if (!b || (mode != CEOS_K_FP64 && mode != CEOS_K_FP32_STORED)) return CEOS_ERR_ARG;
+    b->precision = mode;
+    return CEOS_OK;
+}
+
+int ceos_rve_set_tangent_rtol(ceos_rve_batch* b, double rtol) {
+    if (!b || !(rtol > 0.0 && rtol < 1.0)) return CEOS_ERR_ARG;
+    b->tangent_rtol = rtol;
     return CEOS_OK;
 }
 

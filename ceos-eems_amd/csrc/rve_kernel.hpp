@@ -55,11 +55,12 @@ struct RveParams {
     double* P;                    // [n_rve][9] homogenized first Piola stress, column-major
     double* A;                    // [n_rve][81] dP/dF ([3J+i][3L+k]) or nullptr
     int* info;                    // [n_rve][4] status, Newton iterations, CG iterations, cut-backs
-    double* Mbuf;                 // [slots][243 * max_nodes]
+    void* Mbuf;                   // [slots][243 * max_nodes] of MT (double-sized slots)
     double* Vbuf;                 // [slots][kRveVecPerNode * max_nodes]
     long long max_nodes;
     int* queue;                   // next queue position (zeroed before the launch)
     double newton_rtol, newton_atol, pcg_rtol;
+    double tangent_rtol;          // CG tolerance of the condensation solves
     int newton_maxit, pcg_maxit, max_cutbacks;
     int inexact;                  // 1: CG tolerance follows the Newton residual (forcing term)
     long long* prof;              // [n_rve][2] clock ticks in assembly, in CG (or nullptr)
@@ -118,19 +119,20 @@ __device__ __forceinline__ int nbr_offset(int a, int b) {  // o of node b seen f
            9 * (corner_dz(b) - corner_dz(a) + 1);
 }
 
+template <typename MT>
 struct RveCtx {
     int n, nn;
     double h;
     const uint8_t* phase;
     double* wconv;  // converged fluctuation of this RVE (global state)
-    double* M;
+    MT* M;          // reduced tangent: FP64, or FP32 storage read into FP64 products
     double *wc, *R, *x, *r, *q, *z, *Minv, *B;
 };
 
 // Element pass over the whole RVE at the macro gradient Fa: zero and assemble M (if WANT_K) and
 // R.  Returns true if some element has detJ <= 1e-13.
-template <int MODEL, bool WANT_K>
-__device__ bool rve_assemble(const RveCtx& c, const RveParams& p, const double* Fa, double* lds,
+template <int MODEL, bool WANT_K, typename MT>
+__device__ bool rve_assemble(const RveCtx<MT>& c, const RveParams& p, const double* Fa, double* lds,
                              int (*snode)[8]) {
     const int tid = threadIdx.x;
     const long long nm = 243LL * c.nn;
@@ -239,7 +241,7 @@ __device__ bool rve_assemble(const RveCtx& c, const RveParams& p, const double* 
                     c.R[3 * nb + 2] = r2 + fe2;
                     if constexpr (WANT_K) {
                         const int nn = c.nn;
-                        double* M = c.M;
+                        MT* M = c.M;
                         {
                             const double full[9] = {kd[0], kd[1], kd[2], kd[1], kd[3], kd[4], kd[2], kd[4], kd[5]};
                             double old[9];
@@ -281,27 +283,35 @@ __device__ bool rve_assemble(const RveCtx& c, const RveParams& p, const double* 
 }
 
 // y = M v for node i (27 neighbour blocks; v in LDS); node 0 is the fixed corner (identity row,
-// its column never multiplies a nonzero since v_0 = 0)
-__device__ __forceinline__ void rve_row(const RveCtx& c, int i, const double* v, double y[3]) {
+// its column never multiplies a nonzero since v_0 = 0).  The 81 tangent entries of one z-plane of
+// neighbours are loaded before any of them is used (the scheduling barrier keeps the loads
+// together), so a row costs three HBM round trips instead of one per block.
+template <typename MT>
+__device__ __forceinline__ void rve_row(const RveCtx<MT>& c, int i, const double* v, double y[3]) {
     const int n = c.n, nn = c.nn;
     const int ix = i % n, iy = (i / n) % n, iz = i / (n * n);
+    const int xs[3] = {ix == 0 ? n - 1 : ix - 1, ix, ix == n - 1 ? 0 : ix + 1};
+    const int ys[3] = {iy == 0 ? n - 1 : iy - 1, iy, iy == n - 1 ? 0 : iy + 1};
     double a0 = 0.0, a1 = 0.0, a2 = 0.0;
 #pragma unroll 1
     for (int dz = -1; dz <= 1; ++dz) {
         const int zz = iz + dz < 0 ? n - 1 : (iz + dz == n ? 0 : iz + dz);
+        // entries (o 9 + k), o = (dx+1) + 3 (dy+1) + 9 (dz+1): this plane is q = 9 ob + k, q < 81
+        const MT* mp = c.M + (long long)(81 * (dz + 1)) * nn + i;
+        double m[81];
 #pragma unroll
-        for (int dy = -1; dy <= 1; ++dy) {
-            const int yy = iy + dy < 0 ? n - 1 : (iy + dy == n ? 0 : iy + dy);
+        for (int q = 0; q < 81; ++q) m[q] = (double)mp[(long long)q * nn];
+        __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-            for (int dx = -1; dx <= 1; ++dx) {
-                const int xx = ix + dx < 0 ? n - 1 : (ix + dx == n ? 0 : ix + dx);
-                const int j = xx + n * (yy + n * zz);
-                const int o = (dx + 1) + 3 * (dy + 1) + 9 * (dz + 1);
-                const double* m = c.M + o * 9 * nn + i;
+        for (int by = 0; by < 3; ++by) {
+#pragma unroll
+            for (int bx = 0; bx < 3; ++bx) {
+                const int j = xs[bx] + n * (ys[by] + n * zz);
+                const int b = bx + 3 * by;
                 const double v0 = v[3 * j], v1 = v[3 * j + 1], v2 = v[3 * j + 2];
-                a0 += m[0] * v0 + m[nn] * v1 + m[2 * nn] * v2;
-                a1 += m[3 * nn] * v0 + m[4 * nn] * v1 + m[5 * nn] * v2;
-                a2 += m[6 * nn] * v0 + m[7 * nn] * v1 + m[8 * nn] * v2;
+                a0 += m[9 * b] * v0 + m[9 * b + 1] * v1 + m[9 * b + 2] * v2;
+                a1 += m[9 * b + 3] * v0 + m[9 * b + 4] * v1 + m[9 * b + 5] * v2;
+                a2 += m[9 * b + 6] * v0 + m[9 * b + 7] * v1 + m[9 * b + 8] * v2;
             }
         }
     }
@@ -311,7 +321,8 @@ __device__ __forceinline__ void rve_row(const RveCtx& c, int i, const double* v,
 }
 
 // block-Jacobi: inverse of every node's diagonal block (symmetric, upper 6 values); node 0 -> 0
-__device__ void rve_precond(const RveCtx& c) {
+template <typename MT>
+__device__ void rve_precond(const RveCtx<MT>& c) {
     const long long nn = c.nn;
     for (int i = threadIdx.x; i < c.nn; i += kRveThreads) {
         double Bm[3][3];
@@ -339,24 +350,47 @@ __device__ __forceinline__ void apply_minv(const double* mi, const double r[3], 
 }
 
 // preconditioned CG on the reduced system: M x = sign * rhs (node 0 fixed) to
-// ||r|| <= rtol ||b||.  p lives in LDS, the other vectors in the workgroup's scratch.
-// Returns the iterations used.
-__device__ int rve_pcg(const RveCtx& c, const RveParams& p, const double* rhs, double sign, double rtol,
+// ||r|| <= rtol ||b||.  p lives in LDS, the other vectors in the workgroup's scratch.  The vector
+// sweeps take kCgGroup nodes per thread at a time and load all their operands before computing
+// (one round trip per group).  Returns the iterations used.
+constexpr int kCgGroup = 4;
+
+template <typename MT>
+__device__ int rve_pcg(const RveCtx<MT>& c, const RveParams& p, const double* rhs, double sign, double rtol,
                        double* pl, double* red) {
     const int tid = threadIdx.x;
+    const int nn = c.nn;
+    constexpr int G = kCgGroup;
+    constexpr int S = kRveThreads * kCgGroup;
     double acc[2] = {0.0, 0.0};
-    for (int i = tid; i < c.nn; i += kRveThreads) {
-        double r[3], z[3];
+    for (int i0 = tid; i0 < nn; i0 += S) {
+        double r[G][3], mi[G][6];
 #pragma unroll
-        for (int k = 0; k < 3; ++k) r[k] = i == 0 ? 0.0 : sign * rhs[3 * i + k];
-        apply_minv(c.Minv + 6 * i, r, z);
+        for (int g = 0; g < G; ++g) {
+            const int i = i0 + g * kRveThreads;
+            const int ii = i < nn ? i : 0;
 #pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            c.x[3 * i + k] = 0.0;
-            c.r[3 * i + k] = r[k];
-            pl[3 * i + k] = z[k];
-            acc[0] += r[k] * z[k];
-            acc[1] += r[k] * r[k];
+            for (int k = 0; k < 3; ++k) r[g][k] = rhs[3 * ii + k];
+#pragma unroll
+            for (int k = 0; k < 6; ++k) mi[g][k] = c.Minv[6 * ii + k];
+        }
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            const int i = i0 + g * kRveThreads;
+            if (i >= nn) continue;
+            double rr[3], z[3];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) rr[k] = i == 0 ? 0.0 : sign * r[g][k];
+            apply_minv(mi[g], rr, z);
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                c.x[3 * i + k] = 0.0;
+                c.r[3 * i + k] = rr[k];
+                pl[3 * i + k] = z[k];
+                acc[0] += rr[k] * z[k];
+                acc[1] += rr[k] * rr[k];
+            }
         }
     }
     wg_sum<2>(acc, red);
@@ -368,7 +402,7 @@ __device__ int rve_pcg(const RveCtx& c, const RveParams& p, const double* rhs, d
         ++it;
         __syncthreads();  // p complete
         double pq[1] = {0.0};
-        for (int i = tid; i < c.nn; i += kRveThreads) {
+        for (int i = tid; i < nn; i += kRveThreads) {
             double y[3];
             if (i == 0) {
                 y[0] = y[1] = y[2] = 0.0;
@@ -385,29 +419,63 @@ __device__ int rve_pcg(const RveCtx& c, const RveParams& p, const double* rhs, d
         if (!(pq[0] > 0.0)) break;  // not positive definite along p
         const double alpha = rz / pq[0];
         double s2[2] = {0.0, 0.0};
-        for (int i = tid; i < c.nn; i += kRveThreads) {
-            double r[3], z[3];
+        for (int i0 = tid; i0 < nn; i0 += S) {
+            double xv[G][3], rv[G][3], qv[G][3], mi[G][6];
 #pragma unroll
-            for (int k = 0; k < 3; ++k) {
-                c.x[3 * i + k] += alpha * pl[3 * i + k];
-                r[k] = c.r[3 * i + k] - alpha * c.q[3 * i + k];
-                c.r[3 * i + k] = r[k];
+            for (int g = 0; g < G; ++g) {
+                const int i = i0 + g * kRveThreads;
+                const int ii = i < nn ? i : 0;
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    xv[g][k] = c.x[3 * ii + k];
+                    rv[g][k] = c.r[3 * ii + k];
+                    qv[g][k] = c.q[3 * ii + k];
+                }
+#pragma unroll
+                for (int k = 0; k < 6; ++k) mi[g][k] = c.Minv[6 * ii + k];
             }
-            apply_minv(c.Minv + 6 * i, r, z);
+            __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-            for (int k = 0; k < 3; ++k) {
-                c.z[3 * i + k] = z[k];
-                s2[0] += r[k] * r[k];
-                s2[1] += r[k] * z[k];
+            for (int g = 0; g < G; ++g) {
+                const int i = i0 + g * kRveThreads;
+                if (i >= nn) continue;
+                double r[3], z[3];
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    c.x[3 * i + k] = xv[g][k] + alpha * pl[3 * i + k];
+                    r[k] = rv[g][k] - alpha * qv[g][k];
+                    c.r[3 * i + k] = r[k];
+                }
+                apply_minv(mi[g], r, z);
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    c.z[3 * i + k] = z[k];
+                    s2[0] += r[k] * r[k];
+                    s2[1] += r[k] * z[k];
+                }
             }
         }
         wg_sum<2>(s2, red);
         if (s2[0] <= tol2) break;
         const double beta = s2[1] / rz;
         rz = s2[1];
-        for (int i = tid; i < c.nn; i += kRveThreads) {
+        for (int i0 = tid; i0 < nn; i0 += S) {
+            double zv[G][3];
 #pragma unroll
-            for (int k = 0; k < 3; ++k) pl[3 * i + k] = c.z[3 * i + k] + beta * pl[3 * i + k];
+            for (int g = 0; g < G; ++g) {
+                const int i = i0 + g * kRveThreads;
+                const int ii = i < nn ? i : 0;
+#pragma unroll
+                for (int k = 0; k < 3; ++k) zv[g][k] = c.z[3 * ii + k];
+            }
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int g = 0; g < G; ++g) {
+                const int i = i0 + g * kRveThreads;
+                if (i >= nn) continue;
+#pragma unroll
+                for (int k = 0; k < 3; ++k) pl[3 * i + k] = zv[g][k] + beta * pl[3 * i + k];
+            }
         }
     }
     __syncthreads();
@@ -433,8 +501,8 @@ __device__ __forceinline__ void mode_strain(const double Fi[3][3], int I, double
 //         Phi_(iJ) = e_i Y_J: grad_x Phi_(iJ) = e_i (x) g_J with g_J row J of F^-1, so
 //         (Phi_I^T K Phi_K) = sum_gp w detJ [eps(I)^T D eps(K) + delta_ik g_J . sigma g_L],
 //         45 upper entries (I <= K, I = 3J + i).
-template <int MODEL, bool WANT_T>
-__device__ void rve_gp_pass(const RveCtx& c, const RveParams& p, const double* Fa, double* lds, double Pacc[10],
+template <int MODEL, bool WANT_T, typename MT>
+__device__ void rve_gp_pass(const RveCtx<MT>& c, const RveParams& p, const double* Fa, double* lds, double Pacc[10],
                             double T1[45]) {
     const int tid = threadIdx.x;
     const int n = c.n, nel = n * n * n;
@@ -551,7 +619,8 @@ __device__ void rve_gp_pass(const RveCtx& c, const RveParams& p, const double* F
 // b_(kL) = T^T K Phi_(kL) for the 9 affine modes: row i of M against the mode's values at the
 // neighbour offsets (translation invariance of K removes the absolute position); node 0 -> 0.
 // B[(3L + k)][i][a] = sum_o M[o][a][k](i) d_L(o).
-__device__ void rve_affine_rhs(const RveCtx& c) {
+template <typename MT>
+__device__ void rve_affine_rhs(const RveCtx<MT>& c) {
     const long long nn = c.nn;
     for (int i = threadIdx.x; i < c.nn; i += kRveThreads) {
         double bb[9][3];
@@ -578,14 +647,15 @@ __device__ void rve_affine_rhs(const RveCtx& c) {
 }
 
 // the queue position's RVE and the workgroup's scratch slice
-__device__ __forceinline__ RveCtx rve_ctx(const RveParams& p, int rid, long long slot) {
-    RveCtx c;
+template <typename MT>
+__device__ __forceinline__ RveCtx<MT> rve_ctx(const RveParams& p, int rid, long long slot) {
+    RveCtx<MT> c;
     c.n = p.side[rid];
     c.nn = c.n * c.n * c.n;
     c.h = 1.0 / c.n;
     c.phase = p.phase + p.phase_off[rid];
     c.wconv = p.w + 3 * p.node_off[rid];
-    c.M = p.Mbuf + slot * 243 * p.max_nodes;
+    c.M = reinterpret_cast<MT*>(p.Mbuf) + slot * 243 * p.max_nodes;
     double* V = p.Vbuf + slot * kRveVecPerNode * p.max_nodes;
     const long long mn = p.max_nodes;
     c.wc = V;
@@ -599,7 +669,7 @@ __device__ __forceinline__ RveCtx rve_ctx(const RveParams& p, int rid, long long
     return c;
 }
 
-template <int MODEL>
+template <int MODEL, typename MT>
 __global__ __launch_bounds__(kRveThreads) void k_rve_solve(RveParams p) {
     __shared__ double lds[kRveLds];
     __shared__ double red[kRveWaves * 45];
@@ -616,7 +686,7 @@ __global__ __launch_bounds__(kRveThreads) void k_rve_solve(RveParams p) {
         __syncthreads();
         if (qi >= p.n_rve) break;
         const int rid = p.order[qi];
-        RveCtx c = rve_ctx(p, rid, slot);
+        RveCtx<MT> c = rve_ctx<MT>(p, rid, slot);
         for (int k = tid; k < 3 * c.nn; k += kRveThreads) c.wc[k] = c.wconv[k];
         __syncthreads();
 
@@ -708,7 +778,7 @@ __global__ __launch_bounds__(kRveThreads) void k_rve_solve(RveParams p) {
 // launch, after k_rve_solve): the reduced tangent at the converged state is assembled again,
 //   A = (Phi^T K Phi + sum_m b_m^T v_m) / V,  K_red v_m = -b_m,  b_m = T^T K Phi_m,
 // nine CG solves per RVE.  RVEs whose solve failed are skipped.
-template <int MODEL>
+template <int MODEL, typename MT>
 __global__ __launch_bounds__(kRveThreads) void k_rve_tangent(RveParams p) {
     __shared__ double lds[kRveLds];
     __shared__ double red[kRveWaves * 45];
@@ -725,7 +795,7 @@ __global__ __launch_bounds__(kRveThreads) void k_rve_tangent(RveParams p) {
         if (qi >= p.n_rve) break;
         const int rid = p.order[qi];
         if (p.info[4 * rid] != 0) continue;
-        RveCtx c = rve_ctx(p, rid, slot);
+        RveCtx<MT> c = rve_ctx<MT>(p, rid, slot);
         for (int k = tid; k < 3 * c.nn; k += kRveThreads) c.wc[k] = c.wconv[k];
         if (tid < 9) sFa[tid] = p.Fconv[9 * rid + 3 * (tid % 3) + tid / 3];
         __syncthreads();
@@ -743,7 +813,7 @@ __global__ __launch_bounds__(kRveThreads) void k_rve_tangent(RveParams p) {
         // condensed part b_k . v_m, v_m = K_red^-1 (-b_m), parked in A until the end
         double* Aout = p.A + 81LL * rid;
         for (int m = 0; m < 9; ++m) {
-            pcg_total += rve_pcg(c, p, c.B + (long long)m * 3 * nn, -1.0, p.pcg_rtol, lds, red);
+            pcg_total += rve_pcg(c, p, c.B + (long long)m * 3 * nn, -1.0, p.tangent_rtol, lds, red);
             double dots[9];
 #pragma unroll
             for (int k = 0; k < 9; ++k) dots[k] = 0.0;

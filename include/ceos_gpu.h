@@ -227,6 +227,16 @@ int ceos_rve_set_solver(ceos_rve_batch *b, double newton_rtol, double newton_ato
  * convergence test is the same either way; the tangent solves always use pcg_rtol. */
 int ceos_rve_set_inexact(ceos_rve_batch *b, int inexact);
 
+/* Storage of the reduced tangent in the CG solves (Newton steps and condensation):
+ * CEOS_K_FP64 (default) or CEOS_K_FP32_STORED, FP32 storage read into FP64 products and sums.
+ * Residuals, the Newton test, dot products and CG convergence tests stay FP64; the FP64 Newton
+ * residual corrects the perturbed steps (inexact Newton), so P converges to the same tolerance.
+ * The condensed tangent carries the storage rounding (about 1e-6 relative). */
+int ceos_rve_set_precision(ceos_rve_batch *b, int mode);
+
+/* CG tolerance of the nine condensation solves of the homogenized tangent (default: pcg_rtol). */
+int ceos_rve_set_tangent_rtol(ceos_rve_batch *b, double rtol);
+
 /* Every RVE back to its undeformed state (F = I, zero fluctuation). */
 int ceos_rve_reset(ceos_rve_batch *b);
 

@@ -127,3 +127,23 @@ def test_workload_sizes_match_oracle(n):
     o = ro.OracleRVE(n, ph)
     Po, _, _, _ = o.solve(F)
     assert rel(P[0], Po.reshape(3, 3).T) < 1e-9
+
+
+def test_fp32_stored_tangent_matches_fp64():
+    """FP32 storage of the reduced tangent: the Newton residual stays FP64, so P reaches the same
+    tolerance; the condensed tangent carries the storage rounding."""
+    sides = [6, 8]
+    out = []
+    for mode in (gpu.K_FP64, gpu.K_FP32_STORED):
+        b, _ = _batch(sides, [21, 22])
+        b.set_precision(mode)
+        Fs = np.stack([_F(60), _F(61)])
+        P, A, info = b.solve(Fs, tangent=True)
+        print(mode, info.tolist(), flush=True)
+        assert np.all(info[:, 0] == 0)
+        out.append((P, A))
+    (P64, A64), (P32, A32) = out
+    for k in range(len(sides)):
+        print(k, rel(P32[k], P64[k]), rel(A32[k], A64[k]), flush=True)
+        assert rel(P32[k], P64[k]) < 1e-9
+        assert rel(A32[k], A64[k]) < 1e-5
