@@ -7,11 +7,20 @@ upper CSR (FP64), i.e. SolveConstitutiveModel + InternalForce + TangentStiffness
 reference.  Inputs (U, Fint, K) are resident in HBM; the evaluation is the device-resident
 C-ABI entry ceos_evaluate_device.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-fe2]
 
 Prints one JSON line.  cpu_baseline: the CPU oracle (a plain-C restatement of the reference
 loops, OpenMP over element colours) timed on the host cores on a bounded sample of the same
 workload: the 32^3 corner sub-cube of the same perturbed mesh and displacement field.
+
+The line also carries the second metric, "fe2": FE^2 macro-step seconds, one macro Newton
+evaluation (Fint + K of the macro mesh) at all 4,096 macro Gauss points of an 8^3 hex8 macro cube
+whose Gauss points are periodic voxel RVEs (sides 8/12/16, 20 % inclusions, Neo-Hookean phases):
+plain = every RVE solved by the batched GPU solver and condensed, surrogate = the trained MLP
+(ceos-eems_amd/data/fe2_surrogate.npz) with RVE fallback outside its training hull and 1 % of
+the points re-solved as judges.  Both are timed at the first evaluation of a load step (RVEs
+start undeformed) and at a following Newton evaluation (RVEs start from that state); the
+surrogate's stresses are compared with the plain path's at every point.
 """
 import argparse
 import json
@@ -91,12 +100,127 @@ def cpu_baseline(X, conn, U, n_sub=32, budget_s=12.0):
                       % (n, n_sub, len(sub_conn), threads)}
 
 
+def macro_displacement(X):
+    """Macro field of the FE^2 workload: a homogeneous stretch-shear plus a smooth fluctuation
+    (principal stretches at the Gauss points within about [0.93, 1.09])."""
+    F0 = np.array([[1.06, 0.02, 0.0], [0.0, 0.97, 0.01], [0.0, 0.0, 0.98]])
+    x, y, z = X[:, 0], X[:, 1], X[:, 2]
+    U = X @ (F0 - np.eye(3)).T
+    U += 0.01 * np.stack([np.sin(np.pi * y) * np.sin(np.pi * z), np.sin(np.pi * z) * np.sin(np.pi * x),
+                          np.sin(np.pi * x) * np.sin(np.pi * y)], axis=1)
+    return U.reshape(-1)
+
+
+def newton_correction(X):
+    """A small smooth correction, standing in for the next macro Newton update."""
+    x, y, z = X[:, 0], X[:, 1], X[:, 2]
+    return 0.002 * np.stack([np.sin(2 * np.pi * x) * y, np.cos(np.pi * y) * z, x * y], axis=1).reshape(-1)
+
+
+RVE_CG_BYTES_PER_NODE = 243 * 4 + 240  # FP32-stored tangent row + the FP64 CG vector traffic
+
+
+def fe2_cpu_baseline(F, budget_note):
+    """One RVE per size class solved from the undeformed state and condensed by the CPU restatement
+    (sparse direct solves), extrapolated to the workload's 4,096 RVEs over the host cores."""
+    sys.path.insert(0, os.path.join(REPO, "tests"))
+    import rve_oracle as ro
+    from ceos_amd import rve
+    per = {}
+    for n in rve.SIDES:
+        o = ro.OracleRVE(n, rve.phases(n, 0))
+        t0 = time.perf_counter()
+        o.solve(F)
+        o.tangent()
+        per[n] = time.perf_counter() - t0
+    return per
+
+
+def fe2_bench(no_cpu):
+    from ceos_amd import fe2, gpu
+    t0 = time.perf_counter()
+    sur = fe2.load_surrogate(max_points=4096)
+    m = fe2.MacroFE2(n_macro=8, surrogate=sur)
+    m.rves.set_precision(gpu.K_FP32_STORED)
+    m.rves.set_tangent_rtol(1e-8)
+    t_build = time.perf_counter() - t0
+    t_desc = m.setup_descriptors()
+    U1 = macro_displacement(m.X)
+    U2 = U1 + newton_correction(m.X)
+    # surrogate path first, on undeformed RVEs (its judges and fallbacks start there)
+    m.rves.reset()
+    s1 = m.evaluate_surrogate(U1)
+    s2 = m.evaluate_surrogate(U2)
+    ext_s = m.ext()
+    f_s, k_s = m.fint(), m.tangent()
+    Fgp = m.gp_F()
+    m.rves.reset()
+    p1 = m.evaluate_rve(U1)
+    p2 = m.evaluate_rve(U2)
+    ext_p = m.ext()
+    f_p, k_p = m.fint(), m.tangent()
+    info = m.dinfo.numpy().reshape(-1, 4)
+    nodes = m.sides.astype(np.float64) ** 3
+    lam = np.sqrt(np.linalg.eigvalsh(np.einsum("nki,nkj->nij", Fgp, Fgp)))
+    err = fe2.stress_rel_error(ext_s[:, :6], ext_p[:, :6])
+    w = np.sqrt([1, 1, 1, 2, 2, 2.0])
+    agg = np.linalg.norm((ext_s[:, :6] - ext_p[:, :6]) * w) / np.linalg.norm(ext_p[:, :6] * w)
+    out = {
+        "metric": "FE2 macro-step s (one macro Newton evaluation at all macro Gauss points): NN surrogate vs RVE",
+        "unit": "s", "higher_is_better": False,
+        "config": {"workload": "8^3 hex8 macro cube, 4,096 Gauss points, each a periodic voxel RVE "
+                               "(sides 8/12/16 seeded, 20% spherical inclusions, Neo-Hookean matrix mu=1 "
+                               "K=50, inclusions 10x), macro stretch-shear field",
+                   "rves": int(m.ngp), "rve_nodes_total": int(nodes.sum()),
+                   "rve_tangent_storage": "fp32 (FP64 residual, dot products, convergence tests)",
+                   "rve_newton_rtol": 1e-10, "rve_cg_rtol": 1e-10, "rve_condensation_rtol": 1e-8,
+                   "macro_gp_stretch_range": [float(lam.min()), float(lam.max())],
+                   "surrogate": "MLP 27->256->256->256->6 (SiLU), FP32 inference, trained on %d RVE "
+                                "samples" % int(sur[2]["n_samples"])},
+        "plain_s": p2["seconds"], "surrogate_s": s2["seconds"],
+        "speedup": p2["seconds"] / s2["seconds"],
+        "plain_first_s": p1["seconds"], "surrogate_first_s": s1["seconds"],
+        "speedup_first": p1["seconds"] / s1["seconds"],
+        "surrogate_rve_solves": [s1["rve_solves"], s2["rve_solves"]],
+        "outside_hull": s2["outside_hull"], "judged": s2["judged"],
+        "judge_err": [s1["judge_err"], s2["judge_err"]],
+        "judge_point_max": max(s1["judge_point_max"], s2["judge_point_max"]),
+        "fallback": bool(s1["fallback"] or s2["fallback"]),
+        "stress_err_vs_rve": {"mean": float(err.mean()), "p99": float(np.percentile(err, 99)),
+                              "max": float(err.max()), "aggregate": float(agg)},
+        "fint_rel_diff": float(np.linalg.norm(f_s - f_p) / np.linalg.norm(f_p)),
+        "k_rel_diff": float(np.linalg.norm(k_s - k_p) / np.linalg.norm(k_p)),
+        "rve_cg_hbm": {"achieved": float(np.sum(info[:, 2] * nodes) * RVE_CG_BYTES_PER_NODE / p2["seconds"] / 1e9),
+                       "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                       "note": "algorithmic CG bytes (FP32 tangent rows + FP64 vectors) over the whole plain "
+                               "evaluation's wall time, a lower bound for the CG kernels alone"},
+        "setup_s": {"rve_batch_and_surrogate": t_build, "descriptors": t_desc},
+        "surrogate_test_err": {"mean": float(sur[2]["test_err_mean"]), "p99": float(sur[2]["test_err_p99"]),
+                               "max": float(sur[2]["test_err_max"]),
+                               "base_only_mean": float(sur[2]["test_err_base_mean"])},
+    }
+    if not no_cpu:
+        F = np.array([[1.06, 0.02, 0.0], [0.0, 0.97, 0.01], [0.0, 0.0, 0.98]])
+        per = fe2_cpu_baseline(F, "")
+        counts = {n: int(np.sum(m.sides == n)) for n in per}
+        cores = min(os.cpu_count() or 1, 16)
+        total = sum(counts[n] * per[n] for n in per) / cores
+        out["cpu_baseline"] = {"value": total, "unit": "s", "cores": cores, "kind": "port",
+                               "sample": "one RVE per side (8/12/16) solved from the undeformed state to the "
+                                         "macro stretch-shear F and condensed by the CPU restatement (sparse "
+                                         "direct solves, one core each: %s s); the total for the workload's "
+                                         "4,096 RVEs is extrapolated as %d busy cores"
+                                         % (", ".join("%.2f" % per[n] for n in sorted(per)), cores)}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fe2", action="store_true", help="skip the FE^2 macro-step metric")
     ap.add_argument("--precision", choices=["fp64", "fp32k"], default="fp64")
     ap.add_argument("--slabs", type=int, default=0, help="launch schedule (0 = automatic)")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -216,6 +340,9 @@ def main():
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(X, conn, U)
         out["speedup_vs_cpu_baseline"] = value / out["cpu_baseline"]["value"]
+    if not args.no_fe2:
+        del mesh, dU, dF, dK
+        out["fe2"] = fe2_bench(args.no_cpu)
     print(json.dumps(out), flush=True)
 
 

@@ -14,7 +14,9 @@ Either way the points' S and dS/dE go to the macro element kernel as external ma
 (CEOS_MODEL_EXTERNAL) and the macro Fint and CSR tangent are assembled on the GPU.
 
 The surrogate's descriptor of a microstructure is its small-strain homogenized stiffness C_lin
-(the condensed tangent at F = I), computed once per RVE at setup (`descriptors`).
+(the condensed tangent at F = I), computed once per RVE at setup (`descriptors`).  The surrogate
+predicts S as a base (Neo-Hookean with the Voigt-average moduli of C_lin plus the anisotropic
+rest of C_lin, `base_stress`) plus the network's residual.
 """
 import math
 import os
@@ -29,7 +31,7 @@ SURROGATE_PATH = os.path.join(PKG, "data", "fe2_surrogate.npz")
 
 LAM_LO, LAM_HI = 0.8, 1.25     # principal stretches of the training distribution
 U_SCALE = 0.25                 # U features: (U - I) / U_SCALE
-E_REF = 0.1                    # output scaling: S = C_lin : E + (|E|^2 / E_REF^2) (t_mean + t_std y)
+E_REF = 0.1                    # output scaling: S = S_base + (|E|^2 / E_REF^2) (t_mean + t_std y)
 N_IN, WIDTH, N_OUT = 27, 256, 6
 VOIGT = ((0, 0), (1, 1), (2, 2), (0, 1), (1, 2), (0, 2))
 PROPS = np.array([rve.MATRIX_PROPS, rve.INCLUSION_PROPS])
@@ -103,6 +105,36 @@ def linear_part(Clin, F):
     E = green_lagrange(F)
     Ee = to_voigt(E) * np.array([1, 1, 1, 2, 2, 2.0])
     return np.einsum("...ab,...b->...a", from_up21(Clin), Ee)
+
+
+def base_stress(Clin, F):
+    """The surrogate's base S (Voigt): the Neo-Hookean model 3 with the Voigt-average bulk and
+    shear moduli of C_lin, S = mu J^-2/3 (I - tr C / 3 C^-1) + J p C^-1, p = 3K J^-2/3 (J^1/3 - 1),
+    plus (C_lin - C_iso) : E; the stress ceos_surrogate_eval_device adds the network's residual to."""
+    Cl = from_up21(Clin)
+    d = np.diagonal(Cl, axis1=-2, axis2=-1)
+    K = (d[..., 0] + d[..., 1] + d[..., 2] + 2 * (Cl[..., 0, 1] + Cl[..., 0, 2] + Cl[..., 1, 2])) / 9.0
+    mu = ((d[..., 0] + d[..., 1] + d[..., 2]) - (Cl[..., 0, 1] + Cl[..., 0, 2] + Cl[..., 1, 2]) +
+          3 * (d[..., 3] + d[..., 4] + d[..., 5])) / 15.0
+    lam = K - 2 * mu / 3
+    Ciso = np.zeros(Cl.shape)
+    Ciso[..., :3, :3] = lam[..., None, None]
+    for a in range(3):
+        Ciso[..., a, a] += 2 * mu
+        Ciso[..., a + 3, a + 3] = mu
+    Cg = np.einsum("...ki,...kj->...ij", F, F)
+    J = np.linalg.det(F)
+    Ci = np.linalg.inv(Cg)
+    trC = np.trace(Cg, axis1=-2, axis2=-1)
+    p = 3.0 * K * J ** (-2.0 / 3.0) * (np.cbrt(J) - 1.0)
+    S = (mu * J ** (-2.0 / 3.0))[..., None, None] * (np.eye(3) - (trC / 3.0)[..., None, None] * Ci) + \
+        (J * p)[..., None, None] * Ci
+    return to_voigt(S) + linear_part(up21(Cl - Ciso), F)
+
+
+def up21(M):
+    """(..., 6, 6) -> (..., 21) upper Voigt."""
+    return np.stack([M[..., a, b] for a in range(6) for b in range(a, 6)], axis=-1)
 
 
 def stress_rel_error(S, Sref):
@@ -184,6 +216,7 @@ class MacroFE2:
         self.Clin = None
         self.sur = surrogate  # (Mlp, Surrogate, meta) or None
         self.judge_rng = np.random.default_rng(seed)
+        self.judge_ids = None
 
     def setup_descriptors(self, pcg_rtol=1e-8):
         t = time.perf_counter()
@@ -240,11 +273,19 @@ class MacroFE2:
         return self._finish({"mode": "rve", "seconds": time.perf_counter() - t0, "rve_solves": self.ngp,
                              "newton_mean": float(info[:, 1].mean()), "cg_total": int(info[:, 2].sum())})
 
-    def evaluate_surrogate(self, U, judge_frac=0.01, judge_tol=0.02):
-        """Surrogate macro evaluation: MLP for points inside the training hull, RVE solve (with
-        tangent) for the others, RVE re-solve of ceil(judge_frac ngp) sampled inside points.
-        If a judged point misses by more than judge_tol (relative, Frobenius) the whole
-        evaluation falls back to the plain path."""
+    def new_judge_sample(self):
+        """Draw the judged points again at the next surrogate evaluation (e.g. per load step);
+        until then the same points are re-solved, each from its own last converged state."""
+        self.judge_ids = None
+
+    def evaluate_surrogate(self, U, judge_frac=0.01, judge_tol=0.01, fallback_tangent="surrogate"):
+        """Surrogate macro evaluation: the MLP at points inside the training hull; outside it the
+        RVE solve gives the stress (with fallback_tangent="surrogate" the point keeps the
+        surrogate's tangent, an inexact macro Newton tangent; "rve" condenses the RVE tangent too);
+        ceil(judge_frac ngp) sampled inside points are re-solved as judges.  If the judged
+        stresses differ from the surrogate's by more than judge_tol norm-wise
+        (|S_nn - S_rve| / |S_rve| over the sample, tensor norms) the whole evaluation falls back
+        to the plain path."""
         if self.sur is None or self.Clin is None:
             raise RuntimeError("surrogate evaluation needs a surrogate and setup_descriptors()")
         t0 = time.perf_counter()
@@ -257,35 +298,41 @@ class MacroFE2:
         Fm = Fcm.reshape(self.ngp, 3, 3).transpose(0, 2, 1)
         out = np.nonzero(flags)[0].astype(np.int32)
         if len(out):
-            P, A, _ = self.rves.solve_subset(out, Fm[out], tangent=True)
+            with_t = fallback_tangent == "rve"
+            P, A, _ = self.rves.solve_subset(out, Fm[out], tangent=with_t)
             m = len(out)
             D = hiprt.DeviceArray.from_numpy
             dsel, dFs = D(out), D(np.ascontiguousarray(Fcm[out].reshape(-1)))
             dPs = D(np.ascontiguousarray(P.transpose(0, 2, 1).reshape(-1)))
-            dAs = D(np.ascontiguousarray(A.reshape(-1)))
-            gpu.rve_to_external_device(m, dFs.ptr, dPs.ptr, dAs.ptr, self.dext.ptr, dsel=dsel.ptr, stream=s)
+            dAs = D(np.ascontiguousarray(A.reshape(-1))) if with_t else None
+            gpu.rve_to_external_device(m, dFs.ptr, dPs.ptr, dAs.ptr if with_t else None, self.dext.ptr,
+                                       dsel=dsel.ptr, stream=s)
         # the macro assembly runs while the judged points are re-solved
         self._assemble()
         inside = np.nonzero(flags == 0)[0]
         nj = min(len(inside), int(math.ceil(judge_frac * self.ngp))) if judge_frac > 0 else 0
-        err = np.zeros(0)
+        if nj and (self.judge_ids is None or len(self.judge_ids) != nj):
+            self.judge_ids = np.sort(self.judge_rng.choice(inside, size=nj, replace=False)).astype(np.int32)
+        err, agg = np.zeros(0), 0.0
         if nj:
-            jid = np.sort(self.judge_rng.choice(inside, size=nj, replace=False)).astype(np.int32)
+            jid = self.judge_ids[flags[self.judge_ids] == 0]
             Pj, _, _ = self.rves.solve_subset(jid, Fm[jid])
             Sj = np.linalg.solve(Fm[jid], Pj)
             S_rve = to_voigt(0.5 * (Sj + Sj.transpose(0, 2, 1)))
             S_nn = self.ext()[jid, :6]
             err = stress_rel_error(S_nn, S_rve)
-        stats = {"mode": "surrogate", "outside_hull": int(len(out)), "judged": int(nj),
-                 "judge_max_err": float(err.max()) if nj else 0.0,
-                 "judge_mean_err": float(err.mean()) if nj else 0.0, "fallback": False}
-        if nj and err.max() > judge_tol:
+            w = np.sqrt([1, 1, 1, 2, 2, 2.0])
+            agg = float(np.linalg.norm((S_nn - S_rve) * w) / np.linalg.norm(S_rve * w))
+        stats = {"mode": "surrogate", "outside_hull": int(len(out)), "judged": int(len(err)),
+                 "judge_err": agg, "judge_point_max": float(err.max()) if len(err) else 0.0,
+                 "judge_point_mean": float(err.mean()) if len(err) else 0.0, "fallback": False}
+        if len(err) and agg > judge_tol:
             r = self.evaluate_rve(U)
             stats["fallback"] = True
-            stats["rve_solves"] = r["rve_solves"] + int(len(out)) + nj
+            stats["rve_solves"] = r["rve_solves"] + int(len(out)) + len(err)
             stats["seconds"] = time.perf_counter() - t0
             return stats
         self._finish(stats)
-        stats["rve_solves"] = int(len(out)) + nj
+        stats["rve_solves"] = int(len(out)) + len(err)
         stats["seconds"] = time.perf_counter() - t0
         return stats

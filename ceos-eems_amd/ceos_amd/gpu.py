@@ -63,6 +63,7 @@ EXPORTS = [
     "ceos_rve_solve_subset", "ceos_rve_to_external_device", "ceos_mlp_shape",
     "ceos_surrogate_create", "ceos_surrogate_destroy", "ceos_surrogate_features_device",
     "ceos_surrogate_eval_device", "ceos_rve_set_precision", "ceos_rve_set_tangent_rtol",
+    "ceos_mlp_gradients_weighted", "ceos_mlp_train_step_weighted",
 ]
 
 
@@ -133,6 +134,8 @@ def load_library(path=LIB_PATH):
     L.ceos_mlp_set_params.argtypes = [vp, vp]
     L.ceos_mlp_gradients.argtypes = [vp, vp, vp, i, vp, C.POINTER(f)]
     L.ceos_mlp_train_step.argtypes = [vp, vp, vp, i, f, C.POINTER(f)]
+    L.ceos_mlp_gradients_weighted.argtypes = [vp, vp, vp, vp, i, vp, C.POINTER(f)]
+    L.ceos_mlp_train_step_weighted.argtypes = [vp, vp, vp, vp, i, f, C.POINTER(f)]
     L.ceos_mlp_forward.argtypes = [vp, vp, i, vp, vp, vp]
     L.ceos_rve_solve.argtypes = [vp, _dp, _dp, vp, vp]
     L.ceos_rve_solve_device.argtypes = [vp, vp, vp, vp, vp, vp]
@@ -461,17 +464,20 @@ class Mlp:
         assert p.size == self.nparams
         _check(load_library().ceos_mlp_set_params(self._h, p.ctypes.data), "ceos_mlp_set_params")
 
-    def gradients(self, dX, dT, batch):
+    def gradients(self, dX, dT, batch, dW=None):
         g = np.zeros(self.nparams, dtype=np.float32)
         loss = C.c_float()
-        _check(load_library().ceos_mlp_gradients(self._h, dX.ptr, dT.ptr, batch, g.ctypes.data, C.byref(loss)),
-               "ceos_mlp_gradients")
+        _check(load_library().ceos_mlp_gradients_weighted(self._h, dX.ptr, dT.ptr, dW.ptr if dW is not None else None,
+                                                          batch, g.ctypes.data, C.byref(loss)),
+               "ceos_mlp_gradients_weighted")
         return g, loss.value
 
-    def train_step(self, dX, dT, batch, lr, want_loss=False):
+    def train_step(self, dX, dT, batch, lr, want_loss=False, dW=None):
         loss = C.c_float()
-        _check(load_library().ceos_mlp_train_step(self._h, dX.ptr, dT.ptr, batch, lr,
-                                                  C.byref(loss) if want_loss else None), "ceos_mlp_train_step")
+        _check(load_library().ceos_mlp_train_step_weighted(self._h, dX.ptr, dT.ptr,
+                                                           dW.ptr if dW is not None else None, batch, lr,
+                                                           C.byref(loss) if want_loss else None),
+               "ceos_mlp_train_step_weighted")
         return loss.value if want_loss else None
 
     def forward(self, dX, n, dY, dJ=None, stream=None):
@@ -480,7 +486,8 @@ class Mlp:
 
 
 def rve_to_external_device(n, dF, dP, dA, dext, dsel=None, stream=None):
-    """S = F^-1 P and dS/dE from dP/dF for n points into ext rows (dsel[k] or k)."""
+    """S = F^-1 P and dS/dE from dP/dF for n points into ext rows (dsel[k] or k); dA None writes
+    S only."""
     _check(load_library().ceos_rve_to_external_device(n, dsel, dF, dP, dA, dext, stream),
            "ceos_rve_to_external_device")
 

@@ -308,6 +308,42 @@ __device__ __forceinline__ void mat_nh(const double* p, const double F[3][3], do
     push_forward_up(Cbar, F, J, D);
 }
 
+// Model 3 in the reference configuration: second Piola stress and material tangent dS/dE (upper
+// Voigt, engineering shear) from C = F^T F and J = det F, the terms mat_nh forms before its push
+// forward: S = 2 C10 J^-2/3 (I - tr C / 3 C^-1) + J p C^-1, p = 3 K J^-2/3 (J^1/3 - 1).
+__device__ __forceinline__ void nh_material(double C10, double K, const double Cm[3][3], double J, double S[6],
+                                            double Ct[21]) {
+    double Ci[3][3];
+    inv3(Cm, det3(Cm), Ci);
+    const double CiV[6] = {Ci[0][0], Ci[1][1], Ci[2][2], Ci[0][1], Ci[1][2], Ci[0][2]};
+    const double J13 = pow(J, 1.0 / 3.0);
+    const double Jm23 = pow(J, -2.0 / 3.0);
+    const double Jm53 = pow(J, -5.0 / 3.0);
+    const double pr = 3.0 * K * Jm23 * (J13 - 1.0);
+    const double sf = 2.0 * C10;
+    const double ipSC = sf * (Cm[0][0] + Cm[1][1] + Cm[2][2]);
+    double Siso[6];
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+        Siso[i] = Jm23 * ((i < 3 ? sf : 0.0) - (1.0 / 3.0) * ipSC * CiV[i]);
+        S[i] = Siso[i] + J * pr * CiV[i];
+    }
+    double sq[21];
+    square_self_up(CiV, sq);
+    const double d2 = -K * Jm53 * (J13 - 2.0);
+    const double c1 = (2.0 / 3.0) * Jm23 * ipSC;
+    const double cv1 = J * (pr + J * d2), cv2 = 2.0 * J * pr;
+#pragma unroll
+    for (int i = 0; i < 6; ++i)
+#pragma unroll
+        for (int j = i; j < 6; ++j) {
+            const int k = up_idx(i, j);
+            const double bl = CiV[i] * CiV[j];
+            const double pm = sq[k] - (1.0 / 3.0) * bl;
+            Ct[k] = c1 * pm - (2.0 / 3.0) * (Siso[i] * CiV[j] + CiV[i] * Siso[j]) + cv1 * bl - cv2 * sq[k];
+        }
+}
+
 // model 5, ModStVenantKirchhoff.f90:643-773
 __device__ __forceinline__ void mat_svk(const double* p, const double F[3][3], double s[6], double D[21]) {
     const double E = p[0], nu = p[1];

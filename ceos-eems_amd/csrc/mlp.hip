@@ -169,15 +169,17 @@ __global__ void k_colsum(const float* dz, int rows, int cols, float* out) {
     out[c] = s;
 }
 
-// dY = 2 (Y - T) / (B n_out), per-block partial sums of (Y - T)^2
-__global__ void k_mse_grad(const float* y, const float* t, long long n, float scale, float* dy, float* part) {
+// dY = 2 W (Y - T) / (B n_out), per-block partial sums of W (Y - T)^2 (W = 1 without weights)
+__global__ void k_mse_grad(const float* y, const float* t, const float* w, long long n, float scale, float* dy,
+                           float* part) {
     __shared__ float red[256];
     const long long i = (long long)blockIdx.x * 256 + threadIdx.x;
     float e = 0.0f;
     if (i < n) {
         const float d = y[i] - t[i];
-        dy[i] = 2.0f * d * scale;
-        e = d * d;
+        const float wi = w ? w[i] : 1.0f;
+        dy[i] = 2.0f * wi * d * scale;
+        e = wi * d * d;
     }
     red[threadIdx.x] = e;
     __syncthreads();
@@ -317,11 +319,11 @@ void weight_grad(ceos_mlp* m, int l, const float* dz, const float* ain, int B, h
 }
 
 // forward + loss + backward into m->grads; loss partials in m->lossp
-int grads(ceos_mlp* m, const float* X, const float* T, int B, hipStream_t s) {
+int grads(ceos_mlp* m, const float* X, const float* T, const float* W, int B, hipStream_t s) {
     forward_train(m, X, B, s);
     const long long ny = (long long)B * m->n_out;
     const unsigned nb = (unsigned)((ny + 255) / 256);
-    hipLaunchKernelGGL(k_mse_grad, dim3(nb), dim3(256), 0, s, m->Y, T, ny, 1.0f / (float)ny, m->dY, m->lossp);
+    hipLaunchKernelGGL(k_mse_grad, dim3(nb), dim3(256), 0, s, m->Y, T, W, ny, 1.0f / (float)ny, m->dY, m->lossp);
     // layer 4
     weight_grad(m, 3, m->dY, m->A[2], B, s);
     // dZ3 = (dY W4) * silu'(Z3)
@@ -454,9 +456,10 @@ int ceos_mlp_set_params(ceos_mlp* m, const float* p) {
     return CEOS_OK;
 }
 
-int ceos_mlp_gradients(ceos_mlp* m, const float* dX, const float* dT, int batch, float* gout, float* loss) {
+int ceos_mlp_gradients_weighted(ceos_mlp* m, const float* dX, const float* dT, const float* dW, int batch,
+                                float* gout, float* loss) {
     if (!m || !dX || !dT || batch < 1 || batch > m->max_batch) return CEOS_ERR_ARG;
-    int rc = grads(m, dX, dT, batch, m->stream);
+    int rc = grads(m, dX, dT, dW, batch, m->stream);
     if (rc) return rc;
     if (gout) MLP_HIP_OK(hipMemcpyAsync(gout, m->grads, sizeof(float) * m->nparams, hipMemcpyDeviceToHost, m->stream));
     if (loss) return read_loss(m, batch, loss);
@@ -464,9 +467,14 @@ int ceos_mlp_gradients(ceos_mlp* m, const float* dX, const float* dT, int batch,
     return CEOS_OK;
 }
 
-int ceos_mlp_train_step(ceos_mlp* m, const float* dX, const float* dT, int batch, float lr, float* loss) {
+int ceos_mlp_gradients(ceos_mlp* m, const float* dX, const float* dT, int batch, float* gout, float* loss) {
+    return ceos_mlp_gradients_weighted(m, dX, dT, nullptr, batch, gout, loss);
+}
+
+int ceos_mlp_train_step_weighted(ceos_mlp* m, const float* dX, const float* dT, const float* dW, int batch, float lr,
+                                 float* loss) {
     if (!m || !dX || !dT || batch < 1 || batch > m->max_batch || !(lr > 0.0f)) return CEOS_ERR_ARG;
-    int rc = grads(m, dX, dT, batch, m->stream);
+    int rc = grads(m, dX, dT, dW, batch, m->stream);
     if (rc) return rc;
     m->step++;
     const float b1 = 0.9f, b2 = 0.999f;
@@ -476,6 +484,10 @@ int ceos_mlp_train_step(ceos_mlp* m, const float* dX, const float* dT, int batch
     MLP_HIP_OK(hipGetLastError());
     if (loss) return read_loss(m, batch, loss);
     return CEOS_OK;
+}
+
+int ceos_mlp_train_step(ceos_mlp* m, const float* dX, const float* dT, int batch, float lr, float* loss) {
+    return ceos_mlp_train_step_weighted(m, dX, dT, nullptr, batch, lr, loss);
 }
 
 int ceos_mlp_forward(ceos_mlp* m, const float* dX, int n, float* dY, float* dJ, void* stream) {

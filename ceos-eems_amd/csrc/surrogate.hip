@@ -12,10 +12,14 @@
 //   U      = (F^T F)^(1/2) by a cyclic Jacobi eigen-decomposition of C = F^T F
 //   x      = [(U - I)_Voigt / u_scale, (C_lin - d_mean) / d_std]          (n_in = 27, FP32)
 //   y      = MLP(x)                                                        (FP32, ceos_mlp_forward)
-//   S      = C_lin : E + (|E|^2 / e_ref^2) (t_mean + t_std y),  E = (C - I) / 2
-//   dS/dE  = C_lin + t (x) 2E / e_ref^2 + (|E|^2 / e_ref^2) t_std (dy/dx_U) dU/dE / u_scale
-// with dU/dE from U dU + dU U = 2 dE solved in the eigenbasis of U.  The linear term makes the
-// surrogate exact to first order at U = I; the network only carries the nonlinear part.
+//   S      = S_base + (|E|^2 / e_ref^2) (t_mean + t_std y),  E = (C - I) / 2
+//   S_base = S_NH(C; K_V, mu_V) + (C_lin - C_iso(K_V, mu_V)) : E
+//   dS/dE  = dS_base/dE + t (x) 2E / e_ref^2 + (|E|^2 / e_ref^2) t_std (dy/dx_U) dU/dE / u_scale
+// with dU/dE from U dU + dU U = 2 dE solved in the eigenbasis of U.  The base is the Neo-Hookean
+// model 3 with the Voigt-average bulk and shear moduli of C_lin plus the anisotropic rest of C_lin;
+// it linearizes to C_lin exactly, so the surrogate is exact to first order at U = I, and it
+// carries the volumetric nonlinearity (J-dependent pressure) of the Neo-Hookean phases, so the
+// network only learns a residual that is small against both the volumetric and the shear stress.
 // Training hull: the convex set {U : lam_lo <= eig(U) <= lam_hi} (the convex hull of the
 // training stretches, which are drawn over exactly that set) times the box of the normalized
 // descriptors seen in training.  Points outside are flagged for the RVE fallback.
@@ -178,6 +182,18 @@ __global__ __launch_bounds__(256) void k_sur_output(int n, SurCfg cfg, const dou
     for (int a = 0; a < 6; ++a)
 #pragma unroll
         for (int b = a; b < 6; ++b) Cl[a][b] = Cl[b][a] = cl[up_idx(a, b)];
+    // base: Neo-Hookean with the Voigt-average moduli of C_lin, plus C_lin's anisotropic rest : E
+    const double Kv = (Cl[0][0] + Cl[1][1] + Cl[2][2] + 2.0 * (Cl[0][1] + Cl[0][2] + Cl[1][2])) / 9.0;
+    const double mv = ((Cl[0][0] + Cl[1][1] + Cl[2][2]) - (Cl[0][1] + Cl[0][2] + Cl[1][2]) +
+                       3.0 * (Cl[3][3] + Cl[4][4] + Cl[5][5])) / 15.0;
+    const double lv = Kv - 2.0 * mv / 3.0;
+#pragma unroll
+    for (int a = 0; a < 6; ++a)
+#pragma unroll
+        for (int b = 0; b < 6; ++b)
+            Cl[a][b] -= (a < 3 && b < 3 ? lv : 0.0) + (a == b ? (a < 3 ? 2.0 * mv : mv) : 0.0);
+    double Snh[6], Cnh[21];
+    nh_material(0.5 * mv, Kv, C, det3(F), Snh, Cnh);
     double tt[6], S[6];
 #pragma unroll
     for (int a = 0; a < 6; ++a) {
@@ -185,7 +201,7 @@ __global__ __launch_bounds__(256) void k_sur_output(int n, SurCfg cfg, const dou
         double acc = 0.0;
 #pragma unroll
         for (int b = 0; b < 6; ++b) acc += Cl[a][b] * Ee[b];
-        S[a] = acc + phi * tt[a];
+        S[a] = Snh[a] + acc + phi * tt[a];
     }
     const double* o = aux + (long)kAux * t;
     const double lam[3] = {o[0], o[1], o[2]};
@@ -231,7 +247,7 @@ __global__ __launch_bounds__(256) void k_sur_output(int n, SurCfg cfg, const dou
             double acc = 0.0;
 #pragma unroll
             for (int c = 0; c < 6; ++c) acc += (double)J[kNin * a + c] * dx[c];
-            Ct[a][b] = Cl[a][b] + tt[a] * dphi + phi * cfg.t_std[a] * acc;
+            Ct[a][b] = Cnh[a <= b ? up_idx(a, b) : up_idx(b, a)] + Cl[a][b] + tt[a] * dphi + phi * cfg.t_std[a] * acc;
         }
     }
     double* e = ext + (long)kExtPerGp * t;
@@ -272,6 +288,7 @@ __global__ __launch_bounds__(256) void k_rve_to_ext(int n, const int* __restrict
         voigt_pair(a, I, Jc);
         e[a] = 0.5 * (S[I][Jc] + S[Jc][I]);
     }
+    if (!A) return;  // stress only: the point keeps the tangent it has
     const double* aa = A + 81L * t;  // aa[9 (3J + i) + 3L + k] = dP_iJ / dF_kL
     // C_IJKL = sum_ik Fi_Ii Fi_Kk (A_iJkL - delta_ik S_JL), the 21 upper Voigt entries after
     // averaging the 8 index permutations of the minor and major symmetries
@@ -400,7 +417,7 @@ int ceos_surrogate_eval_device(ceos_surrogate* s, int n, const double* dF, const
 
 int ceos_rve_to_external_device(int n, const int* dsel, const double* dF, const double* dP, const double* dA,
                                 double* dext, void* stream) {
-    if (n < 0 || (n > 0 && (!dF || !dP || !dA || !dext))) return CEOS_ERR_ARG;
+    if (n < 0 || (n > 0 && (!dF || !dP || !dext))) return CEOS_ERR_ARG;
     if (n == 0) return CEOS_OK;
     hipStream_t st = stream ? (hipStream_t)stream : hipStreamPerThread;
     hipLaunchKernelGGL(k_rve_to_ext, dim3((n + 255) / 256), dim3(256), 0, st, n, dsel, dF, dP, dA, dext);

@@ -270,7 +270,8 @@ int ceos_rve_solve_subset(ceos_rve_batch *b, int n_sel, const int *sel, const do
  * dS_IJ/dE_KL = sum_ik Finv_Ii Finv_Kk (dP_iJ/dF_kL - delta_ik S_JL)
  * (First_Elasticity_Modulus_To_Second_Voigt, Tools/MathTools/ModMathRoutines.f90:315), averaged
  * over the index symmetries, written to ext row dsel[k] (dsel NULL: row k).  dF, dP: 9 per k
- * (column-major), dA: 81 per k as ceos_rve_solve returns it.  Device arrays. */
+ * (column-major), dA: 81 per k as ceos_rve_solve returns it, or NULL: only S is written and the
+ * row keeps its tangent.  Device arrays. */
 int ceos_rve_to_external_device(int n, const int *dsel, const double *dF, const double *dP, const double *dA,
                                 double *dext, void *stream);
 
@@ -299,6 +300,14 @@ int ceos_mlp_gradients(ceos_mlp *m, const float *dX, const float *dT, int batch,
 /* One Adam step (beta 0.9 / 0.999, eps 1e-8) on the same gradient; loss (may be NULL) is the
  * batch MSE before the step. */
 int ceos_mlp_train_step(ceos_mlp *m, const float *dX, const float *dT, int batch, float lr, float *loss);
+
+/* The same two with a weighted mean squared error, sum W (Y - T)^2 / (batch n_out), dW [batch][n_out]
+ * (NULL = unweighted).  The surrogate is trained with weights that make the loss the squared
+ * relative error of the stress it predicts. */
+int ceos_mlp_gradients_weighted(ceos_mlp *m, const float *dX, const float *dT, const float *dW, int batch,
+                                float *host_grads, float *loss);
+int ceos_mlp_train_step_weighted(ceos_mlp *m, const float *dX, const float *dT, const float *dW, int batch, float lr,
+                                 float *loss);
 
 /* FP32 inference: dY [n][n_out]; dJ (may be NULL) the input Jacobian [n][n_out][n_in] by reverse
  * mode.  stream NULL = the model's stream, synchronised before returning. */

@@ -5,8 +5,11 @@
                              (Push_Forward_Voigt, Tools/MathTools/ModMathRoutines.f90:254-311)
   rve_to_ext(F, P, A)        S = F^-1 P, dS/dE from A = dP/dF
                              (First_Elasticity_Modulus_To_Second_Voigt, ModMathRoutines.f90:315)
+  nh_material(C10, K, F)     model 3 (Neo-Hookean) S and dS/dE in the reference configuration
+  base(Clin, F)              the surrogate's base: Neo-Hookean with the Voigt-average moduli of
+                             C_lin plus (C_lin - C_iso) : E
   surrogate(p, cfg, F, Clin) the surrogate's S, dS/dE and hull flag:
-                             S = C_lin:E + |E|^2/e_ref^2 (t_mean + t_std MLP(x)),
+                             S = base + |E|^2/e_ref^2 (t_mean + t_std MLP(x)),
                              x = [(U - I)/u_scale, (C_lin - d_mean)/d_std]
 The pieces are pinned in tests/test_fe2_oracle.py by what the mathematics fixes: the pull back of
 a pushed-forward tangent, central differences of P(F) and of S(E).
@@ -94,6 +97,55 @@ def rve_to_ext(F, P, A):
     return to_voigt(S), 0.5 * (Cv + Cv.T)
 
 
+def iso_moduli(Clin):
+    """Voigt-average bulk and shear moduli of a 6x6 stiffness (upper 21 given)."""
+    C = from_up21(Clin)
+    K = (C[0, 0] + C[1, 1] + C[2, 2] + 2 * (C[0, 1] + C[0, 2] + C[1, 2])) / 9.0
+    mu = ((C[0, 0] + C[1, 1] + C[2, 2]) - (C[0, 1] + C[0, 2] + C[1, 2]) + 3 * (C[3, 3] + C[4, 4] + C[5, 5])) / 15.0
+    return K, mu
+
+
+def iso_stiffness(K, mu):
+    lam = K - 2.0 * mu / 3.0
+    C = np.zeros((6, 6))
+    C[:3, :3] = lam
+    C[np.arange(3), np.arange(3)] += 2 * mu
+    C[np.arange(3, 6), np.arange(3, 6)] = mu
+    return C
+
+
+def nh_material(C10, K, F):
+    """Model 3 Neo-Hookean second Piola stress and tangent dS/dE (Voigt 6x6) at F, from the
+    closed form S = 2 C10 J^-2/3 (I - tr C/3 C^-1) + J p C^-1, p = 3K J^-2/3 (J^1/3 - 1) and its
+    derivative d/dE = 2 d/dC (dC^-1/dC = -C^-1 (x) C^-1 sym, dJ/dC = J C^-1 / 2)."""
+    Cg = F.T @ F
+    J = np.linalg.det(F)
+    Ci = np.linalg.inv(Cg)
+    I = np.eye(3)
+    trC = np.trace(Cg)
+    p = 3.0 * K * J ** (-2.0 / 3.0) * (J ** (1.0 / 3.0) - 1.0)
+    S = 2 * C10 * J ** (-2.0 / 3.0) * (I - trC / 3.0 * Ci) + J * p * Ci
+    # dS/dC (4th order), then dS/dE = 2 dS/dC
+    dCi = -0.5 * (np.einsum("ik,jl->ijkl", Ci, Ci) + np.einsum("il,jk->ijkl", Ci, Ci))
+    dJ = 0.5 * J * Ci                                   # dJ/dC_kl
+    dJm23 = -2.0 / 3.0 * J ** (-5.0 / 3.0) * dJ
+    dp = 3 * K * (-2.0 / 3.0 * J ** (-5.0 / 3.0) * (J ** (1.0 / 3.0) - 1.0) + J ** (-2.0 / 3.0) / 3.0 * J ** (-2.0 / 3.0)) * dJ
+    iso = I - trC / 3.0 * Ci
+    diso = -np.einsum("ij,kl->ijkl", Ci, I) / 3.0 - trC / 3.0 * dCi
+    dS = 2 * C10 * (np.einsum("ij,kl->ijkl", iso, dJm23) + J ** (-2.0 / 3.0) * diso) \
+        + np.einsum("ij,kl->ijkl", Ci, p * dJ + J * dp) + J * p * dCi
+    return to_voigt(S), tensor4_to_voigt(2.0 * dS)
+
+
+def base(Clin, F):
+    """The surrogate's base stress and tangent (Voigt): NH(K_V, mu_V) + (C_lin - C_iso) : E."""
+    K, mu = iso_moduli(Clin)
+    dC = from_up21(Clin) - iso_stiffness(K, mu)
+    E = 0.5 * (F.T @ F - np.eye(3))
+    S, C = nh_material(0.5 * mu, K, F)
+    return S + dC @ (to_voigt(E) * ENG), C + dC
+
+
 def stretch(F):
     """U = (F^T F)^(1/2), its eigenvalues and eigenvectors."""
     lam2, Q = np.linalg.eigh(F.T @ F)
@@ -117,7 +169,7 @@ def surrogate_stress(p, cfg, F, Clin, width):
     E = 0.5 * (F.T @ F - np.eye(3))
     phi = np.sum(E * E) / cfg["e_ref"] ** 2
     t = cfg["t_mean"] + cfg["t_std"] * y[0]
-    return from_up21(Clin) @ (to_voigt(E) * ENG) + phi * t, inside
+    return base(Clin, F)[0] + phi * t, inside
 
 
 def surrogate(p, cfg, F, Clin, width):
@@ -129,8 +181,8 @@ def surrogate(p, cfg, F, Clin, width):
     E = 0.5 * (F.T @ F - np.eye(3))
     phi = np.sum(E * E) / cfg["e_ref"] ** 2
     t = cfg["t_mean"] + cfg["t_std"] * y[0]
-    Cl = from_up21(Clin)
-    S = Cl @ (to_voigt(E) * ENG) + phi * t
+    Sb, Cb = base(Clin, F)
+    S = Sb + phi * t
     _, lam, Q = stretch(F)
     Ct = np.zeros((6, 6))
     for b, (K, L) in enumerate(VOIGT):
@@ -142,5 +194,5 @@ def surrogate(p, cfg, F, Clin, width):
         dU = Q @ dUp @ Q.T
         dx = to_voigt(dU) / cfg["u_scale"]
         dphi = 2.0 * E[K, L] / cfg["e_ref"] ** 2
-        Ct[:, b] = Cl[:, b] + t * dphi + phi * cfg["t_std"] * (Jx @ dx)
+        Ct[:, b] = Cb[:, b] + t * dphi + phi * cfg["t_std"] * (Jx @ dx)
     return S, 0.5 * (Ct + Ct.T), inside

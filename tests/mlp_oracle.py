@@ -44,14 +44,16 @@ def forward(p, X, n_in, width, n_out):
     return a @ W.T + b, Zs, As
 
 
-def loss_and_grad(p, X, T, n_in, width, n_out):
+def loss_and_grad(p, X, T, n_in, width, n_out, W=None):
+    """Loss sum W (Y - T)^2 / (batch n_out) (W = 1 when None) and its parameter gradient."""
     Y, Zs, As = forward(p, X, n_in, width, n_out)
     Ws = unpack(p, n_in, width, n_out)
     n = Y.size
     d = Y - T
-    loss = np.sum(d * d) / n
+    w = np.ones_like(d) if W is None else np.asarray(W, dtype=np.float64)
+    loss = np.sum(w * d * d) / n
     g = []
-    dz = 2.0 * d / n
+    dz = 2.0 * w * d / n
     for l in range(3, -1, -1):
         W, _ = Ws[l]
         g.append((dz.T @ As[l], dz.sum(axis=0)))

@@ -77,3 +77,23 @@ def test_adam_training_fits_a_smooth_map():
     # the trained parameters reproduce the loss in FP64
     lo, _ = mo.loss_and_grad(m.params().astype(np.float64), X, T, *DIMS)
     assert lo < l0 / 20
+
+
+def test_weighted_gradient_matches_fp64():
+    n = 1000
+    m = gpu.Mlp(*DIMS, max_batch=n, seed=4)
+    p = m.params()
+    X, T = _data(17, n)
+    W = np.random.default_rng(18).uniform(0.0, 5.0, size=T.shape).astype(np.float32)
+    g, loss = m.gradients(_dev(X), _dev(T), n, dW=_dev(W))
+    lo, go = mo.loss_and_grad(p.astype(np.float64), X, T, *DIMS, W=W)
+    print("weighted loss", loss, lo, flush=True)
+    assert abs(loss - lo) < 2e-2 * lo
+    e = np.linalg.norm(g - go) / np.linalg.norm(go)
+    assert e < 2e-2
+    # a weight of zero removes a sample: same gradient as the batch without it
+    W2 = np.ones_like(W)
+    W2[::2] = 0.0
+    g2, _ = m.gradients(_dev(X), _dev(T), n, dW=_dev(W2))
+    _, go2 = mo.loss_and_grad(p.astype(np.float64), X[1::2], T[1::2], *DIMS)
+    assert np.linalg.norm(2 * g2 - go2) / np.linalg.norm(go2) < 2e-2

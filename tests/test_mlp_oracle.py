@@ -44,3 +44,19 @@ def test_input_jacobian_fd():
         Xm[:, i] -= h
         fd = (mo.forward(p, Xp, *dims)[0] - mo.forward(p, Xm, *dims)[0]) / (2 * h)
         assert np.allclose(J[:, :, i], fd, atol=1e-8)
+
+
+def test_weighted_parameter_gradient_fd():
+    p, X, T, dims = _setup(3)
+    W = np.random.default_rng(4).uniform(0.0, 3.0, size=T.shape)
+    loss, g = mo.loss_and_grad(p, X, T, *dims, W=W)
+    Y = mo.forward(p, X, *dims)[0]
+    assert np.isclose(loss, np.sum(W * (Y - T) ** 2) / Y.size)
+    rng = np.random.default_rng(5)
+    h = 1e-6
+    for i in rng.choice(p.size, 40, replace=False):
+        pp, pm = p.copy(), p.copy()
+        pp[i] += h
+        pm[i] -= h
+        fd = (mo.loss_and_grad(pp, X, T, *dims, W=W)[0] - mo.loss_and_grad(pm, X, T, *dims, W=W)[0]) / (2 * h)
+        assert abs(fd - g[i]) < 1e-7 * max(1.0, abs(g[i])), (i, fd, g[i])
