@@ -7,7 +7,7 @@ upper CSR (FP64), i.e. SolveConstitutiveModel + InternalForce + TangentStiffness
 reference.  Inputs (U, Fint, K) are resident in HBM; the evaluation is the device-resident
 C-ABI entry ceos_evaluate_device.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-fe2]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-fe2 | --fe2-only]
 
 Prints one JSON line.  cpu_baseline: the CPU oracle (a plain-C restatement of the reference
 loops, OpenMP over element colours) timed on the host cores on a bounded sample of the same
@@ -221,6 +221,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fe2", action="store_true", help="skip the FE^2 macro-step metric")
+    ap.add_argument("--fe2-only", action="store_true",
+                    help="only the FE^2 macro-step metric (prints its object alone; for profiling it)")
     ap.add_argument("--precision", choices=["fp64", "fp32k"], default="fp64")
     ap.add_argument("--slabs", type=int, default=0, help="launch schedule (0 = automatic)")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -235,6 +237,9 @@ def main():
 
     from ceos_amd import gpu, hiprt, meshgen
 
+    if args.fe2_only:
+        print(json.dumps(fe2_bench(args.no_cpu)), flush=True)
+        return
     t_setup = time.perf_counter()
     X, conn = meshgen.cube(N_CELLS, perturb=0.1)
     U = displacement(X)

@@ -62,11 +62,21 @@ def _reflect(x, lo, hi):
     return lo + np.where(y > w, 2.0 * w - y, y)
 
 
-def walk_stretch(rng, U, sig_lam=0.05, sig_rot=0.25, lo=LAM_LO, hi=LAM_HI):
+def isochoric(rng, lam, sig_logj, lo=LAM_LO, hi=LAM_HI):
+    """Principal stretches rescaled to a volume ratio J = exp(N(0, sig_logj)), kept in [lo, hi]."""
+    J = np.prod(lam, axis=-1, keepdims=True)
+    target = np.exp(rng.normal(scale=sig_logj, size=J.shape))
+    return np.clip(lam * (target / J) ** (1.0 / 3.0), lo, hi)
+
+
+def walk_stretch(rng, U, sig_lam=0.05, sig_rot=0.25, lo=LAM_LO, hi=LAM_HI, sig_logj=None):
     """One step of a reflected random walk of stretch tensors (principal stretches stay in
-    [lo, hi]; the principal axes turn by a random small rotation)."""
+    [lo, hi]; the principal axes turn by a random small rotation).  With sig_logj the stretches
+    are rescaled to a near-isochoric volume ratio (log J ~ N(0, sig_logj))."""
     lam, Q = np.linalg.eigh(U)
     lam = _reflect(lam + rng.normal(scale=sig_lam, size=lam.shape), lo, hi)
+    if sig_logj is not None:
+        lam = isochoric(rng, lam, sig_logj, lo, hi)
     ax = rng.normal(size=(len(U), 3))
     ax /= np.linalg.norm(ax, axis=1, keepdims=True)
     ang = rng.normal(scale=sig_rot, size=len(U))
@@ -137,11 +147,12 @@ def up21(M):
     return np.stack([M[..., a, b] for a in range(6) for b in range(a, 6)], axis=-1)
 
 
-def stress_rel_error(S, Sref):
-    """Tensor-Frobenius relative error of Voigt stresses (shear counted twice)."""
+def stress_rel_error(S, Sref, floor=1e-300):
+    """Tensor-Frobenius relative error of Voigt stresses (shear counted twice); reference stresses
+    below `floor` count as `floor` (a ratio of round-off to round-off means nothing)."""
     w = np.array([1, 1, 1, 2, 2, 2.0])
     d = np.sqrt(np.sum(w * (S - Sref) ** 2, axis=-1))
-    return d / np.maximum(np.sqrt(np.sum(w * Sref ** 2, axis=-1)), 1e-300)
+    return d / np.maximum(np.sqrt(np.sum(w * Sref ** 2, axis=-1)), floor)
 
 
 # ------------------------------------------------------------------------------------------
@@ -222,6 +233,7 @@ class MacroFE2:
         t = time.perf_counter()
         self.Clin = descriptors(self.rves, pcg_rtol)
         self.dClin.upload(self.Clin.reshape(-1))
+        self.stress_floor = 1e-9 * float(np.abs(self.Clin).max())  # the stress of a 1e-9 strain
         return time.perf_counter() - t
 
     def _kinematics(self, U):
@@ -278,14 +290,17 @@ class MacroFE2:
         until then the same points are re-solved, each from its own last converged state."""
         self.judge_ids = None
 
-    def evaluate_surrogate(self, U, judge_frac=0.01, judge_tol=0.01, fallback_tangent="surrogate"):
+    def evaluate_surrogate(self, U, judge_frac=0.01, judge_tol=0.01, fallback_tangent="surrogate",
+                           judge_newton_rtol=1e-6):
         """Surrogate macro evaluation: the MLP at points inside the training hull; outside it the
         RVE solve gives the stress (with fallback_tangent="surrogate" the point keeps the
         surrogate's tangent, an inexact macro Newton tangent; "rve" condenses the RVE tangent too);
         ceil(judge_frac ngp) sampled inside points are re-solved as judges.  If the judged
-        stresses differ from the surrogate's by more than judge_tol norm-wise
-        (|S_nn - S_rve| / |S_rve| over the sample, tensor norms) the whole evaluation falls back
-        to the plain path."""
+        stresses differ from the surrogate's by more than judge_tol norm-wise (the RMS error of the
+        judged points over the RMS stress of all points, tensor norms: a sample estimate of
+        |S_nn - S_rve| / |S_rve| over all points) the whole evaluation falls back to the plain
+        path.  Judged RVEs are converged to judge_newton_rtol (relative Newton test),
+        far below the judge's tolerance."""
         if self.sur is None or self.Clin is None:
             raise RuntimeError("surrogate evaluation needs a surrogate and setup_descriptors()")
         t0 = time.perf_counter()
@@ -316,13 +331,22 @@ class MacroFE2:
         err, agg = np.zeros(0), 0.0
         if nj:
             jid = self.judge_ids[flags[self.judge_ids] == 0]
-            Pj, _, _ = self.rves.solve_subset(jid, Fm[jid])
+            self.rves.set_solver(newton_rtol=judge_newton_rtol)
+            try:
+                Pj, _, _ = self.rves.solve_subset(jid, Fm[jid])
+            finally:
+                self.rves.set_solver()
             Sj = np.linalg.solve(Fm[jid], Pj)
             S_rve = to_voigt(0.5 * (Sj + Sj.transpose(0, 2, 1)))
-            S_nn = self.ext()[jid, :6]
-            err = stress_rel_error(S_nn, S_rve)
-            w = np.sqrt([1, 1, 1, 2, 2, 2.0])
-            agg = float(np.linalg.norm((S_nn - S_rve) * w) / np.linalg.norm(S_rve * w))
+            S_all = self.ext()[:, :6]
+            S_nn = S_all[jid]
+            err = stress_rel_error(S_nn, S_rve, self.stress_floor)
+            # sample estimate of the norm-wise error over all points: RMS error of the judged points
+            # over the RMS stress of all points (a judged point with a tiny stress cannot blow it up)
+            w = np.array([1, 1, 1, 2, 2, 2.0])
+            num = np.mean(np.sum(w * (S_nn - S_rve) ** 2, axis=1))
+            den = np.mean(np.sum(w * S_all ** 2, axis=1))
+            agg = float(np.sqrt(num / max(den, self.stress_floor ** 2)))
         stats = {"mode": "surrogate", "outside_hull": int(len(out)), "judged": int(len(err)),
                  "judge_err": agg, "judge_point_max": float(err.max()) if len(err) else 0.0,
                  "judge_point_mean": float(err.mean()) if len(err) else 0.0, "fallback": False}

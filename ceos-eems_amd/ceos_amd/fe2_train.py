@@ -53,16 +53,19 @@ def S_from(U, P):
     return fe2.to_voigt(0.5 * (S + S.transpose(0, 2, 1)))
 
 
-def generate(n_rve, steps, seed, rng, sig_lam, sig_rot, lo=fe2.LAM_LO, hi=fe2.LAM_HI):
+def generate(n_rve, steps, seed, rng, sig_lam, sig_rot, lo=fe2.LAM_LO, hi=fe2.LAM_HI, sig_logj=None):
     b, sides = make_batch(n_rve, seed)
     t = time.time()
     Clin = fe2.descriptors(b, pcg_rtol=1e-8)
     log("descriptors of %d RVEs: %.1f s" % (n_rve, time.time() - t))
     U = fe2.random_stretch(rng, n_rve, lo, hi)
+    if sig_logj is not None:
+        lam, Q = np.linalg.eigh(U)
+        U = np.einsum("nij,nj,nkj->nik", Q, fe2.isochoric(rng, lam, sig_logj, lo, hi), Q)
     Us, ids, Ss = [], [], []
     for k in range(steps):
         if k:
-            U = fe2.walk_stretch(rng, U, sig_lam, sig_rot, lo, hi)
+            U = fe2.walk_stretch(rng, U, sig_lam, sig_rot, lo, hi, sig_logj)
         t = time.time()
         P, _, info = b.solve(U, check=False)
         ok = info[:, 0] == 0
@@ -74,10 +77,13 @@ def generate(n_rve, steps, seed, rng, sig_lam, sig_rot, lo=fe2.LAM_LO, hi=fe2.LA
     return Clin, sides, np.concatenate(Us), np.concatenate(ids), np.concatenate(Ss)
 
 
-def test_set(n_rve, seed, rng, lo=fe2.LAM_LO, hi=fe2.LAM_HI):
+def test_set(n_rve, seed, rng, lo=fe2.LAM_LO, hi=fe2.LAM_HI, sig_logj=None):
     b, sides = make_batch(n_rve, seed)
     Clin = fe2.descriptors(b, pcg_rtol=1e-8)
     U = fe2.random_stretch(rng, n_rve, lo, hi)
+    if sig_logj is not None:
+        lam, Q = np.linalg.eigh(U)
+        U = np.einsum("nij,nj,nkj->nik", Q, fe2.isochoric(rng, lam, sig_logj, lo, hi), Q)
     t = time.time()
     P, _, info = b.solve(U, check=False)
     ok = info[:, 0] == 0
@@ -154,9 +160,10 @@ def relative_weights(phi, t_std, S, clip=50.0):
     return np.minimum(W, clip).astype(np.float32)
 
 
-def train(X, T, epochs, batch, lr0, seed, W=None, log_every=25):
+def train(X, T, epochs, batch, lr0, seed, W=None, log_every=25, fp32=False, lr_floor=1e-2):
     n = len(X)
     mlp = gpu.Mlp(fe2.N_IN, fe2.WIDTH, fe2.N_OUT, max_batch=batch, seed=seed)
+    mlp.set_training_fp32(fp32)
     dX = hiprt.DeviceArray(n * fe2.N_IN, np.float32)
     dT = hiprt.DeviceArray(n * fe2.N_OUT, np.float32)
     dW = hiprt.DeviceArray(n * fe2.N_OUT, np.float32) if W is not None else None
@@ -172,7 +179,7 @@ def train(X, T, epochs, batch, lr0, seed, W=None, log_every=25):
         if dW is not None:
             dW.upload(W[perm])
         for b in range(nb):
-            lr = 1e-2 * lr0 + 0.5 * (lr0 - 1e-2 * lr0) * (1.0 + math.cos(math.pi * step / total))
+            lr = lr_floor * lr0 + 0.5 * (lr0 - lr_floor * lr0) * (1.0 + math.cos(math.pi * step / total))
             want = (b == nb - 1) and (ep % log_every == 0 or ep == epochs - 1)
             loss = mlp.train_step(dX.view(b * batch * fe2.N_IN, batch * fe2.N_IN),
                                   dT.view(b * batch * fe2.N_OUT, batch * fe2.N_OUT), batch, lr, want_loss=want,
@@ -200,10 +207,16 @@ def main(argv=None):
                     help="train on saved samples (repeatable: the sets are combined)")
     ap.add_argument("--lam-lo", type=float, default=fe2.LAM_LO, help="stretch range of the generated walks")
     ap.add_argument("--lam-hi", type=float, default=fe2.LAM_HI)
+    ap.add_argument("--isochoric", type=float, default=None,
+                    help="near-isochoric walks: log J ~ N(0, this) (nearly incompressible composites deform so)")
     ap.add_argument("--seed-offset", type=int, default=0,
                     help="offset of the training / test microstructure seeds (a new set of RVEs)")
     ap.add_argument("--margin", type=float, default=0.25,
                     help="descriptor hull: training range widened by this fraction on each side")
+    ap.add_argument("--train-fp32", action="store_true", help="FP32 training GEMMs (reference mode)")
+    ap.add_argument("--lr-floor", type=float, default=1e-2, help="final learning rate / initial")
+    ap.add_argument("--blend", type=float, default=None,
+                    help="loss weights (1 + blend W_rel) / (1 + blend), W_rel the relative-error weights")
     ap.add_argument("--loss", choices=["relative", "plain"], default="plain",
                     help="relative: weights that make the loss the squared relative stress error")
     a = ap.parse_args(argv)
@@ -227,8 +240,8 @@ def main(argv=None):
     else:
         t = time.time()
         Clin, sides, U, ids, S = generate(a.rves, a.steps, SEED_TRAIN + a.seed_offset, rng, a.sig_lam, a.sig_rot,
-                                          a.lam_lo, a.lam_hi)
-        Ct, st, Ut, St = test_set(a.test_rves, SEED_TEST + a.seed_offset, rng, a.lam_lo, a.lam_hi)
+                                          a.lam_lo, a.lam_hi, a.isochoric)
+        Ct, st, Ut, St = test_set(a.test_rves, SEED_TEST + a.seed_offset, rng, a.lam_lo, a.lam_hi, a.isochoric)
         t_gen = time.time() - t
         d = {"Clin": Clin, "sides": sides, "U": U, "ids": ids, "S": S, "Clin_test": Ct, "sides_test": st,
              "U_test": Ut, "S_test": St, "t_gen": t_gen, "lam_range": np.array([a.lam_lo, a.lam_hi])}
@@ -245,10 +258,13 @@ def main(argv=None):
     cfg["t_std"] = t_raw[keep].std(0)
     T = ((t_raw[keep] - cfg["t_mean"]) / cfg["t_std"]).astype(np.float32)
     W = relative_weights(e2[keep] / fe2.E_REF ** 2, cfg["t_std"], S[keep]) if a.loss == "relative" else None
+    if a.blend is not None:
+        Wr = relative_weights(e2[keep] / fe2.E_REF ** 2, cfg["t_std"], S[keep])
+        W = ((1.0 + a.blend * Wr) / (1.0 + a.blend)).astype(np.float32)
     feat_mlp = gpu.Mlp(fe2.N_IN, fe2.WIDTH, fe2.N_OUT, max_batch=a.batch, seed=1)
     X = features(feat_mlp, cfg, U[keep], Cp[keep], a.batch)
     feat_mlp.close()
-    mlp, t_train = train(X, T, a.epochs, a.batch, a.lr, a.seed, W)
+    mlp, t_train = train(X, T, a.epochs, a.batch, a.lr, a.seed, W, fp32=a.train_fp32, lr_floor=a.lr_floor)
     # held-out microstructures and stretches
     Ct, Ut, St = d["Clin_test"], d["U_test"], d["S_test"]
     S_nn, flags = evaluate(mlp, cfg, Ut, Ct, a.batch)

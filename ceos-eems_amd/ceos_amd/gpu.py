@@ -63,7 +63,7 @@ EXPORTS = [
     "ceos_rve_solve_subset", "ceos_rve_to_external_device", "ceos_mlp_shape",
     "ceos_surrogate_create", "ceos_surrogate_destroy", "ceos_surrogate_features_device",
     "ceos_surrogate_eval_device", "ceos_rve_set_precision", "ceos_rve_set_tangent_rtol",
-    "ceos_mlp_gradients_weighted", "ceos_mlp_train_step_weighted",
+    "ceos_mlp_gradients_weighted", "ceos_mlp_train_step_weighted", "ceos_mlp_set_training_precision",
 ]
 
 
@@ -136,6 +136,7 @@ def load_library(path=LIB_PATH):
     L.ceos_mlp_train_step.argtypes = [vp, vp, vp, i, f, C.POINTER(f)]
     L.ceos_mlp_gradients_weighted.argtypes = [vp, vp, vp, vp, i, vp, C.POINTER(f)]
     L.ceos_mlp_train_step_weighted.argtypes = [vp, vp, vp, vp, i, f, C.POINTER(f)]
+    L.ceos_mlp_set_training_precision.argtypes = [vp, i]
     L.ceos_mlp_forward.argtypes = [vp, vp, i, vp, vp, vp]
     L.ceos_rve_solve.argtypes = [vp, _dp, _dp, vp, vp]
     L.ceos_rve_solve_device.argtypes = [vp, vp, vp, vp, vp, vp]
@@ -453,6 +454,10 @@ class Mlp:
             self.close()
         except Exception:
             pass
+
+    def set_training_fp32(self, on):
+        _check(load_library().ceos_mlp_set_training_precision(self._h, 1 if on else 0),
+               "ceos_mlp_set_training_precision")
 
     def params(self):
         p = np.zeros(self.nparams, dtype=np.float32)

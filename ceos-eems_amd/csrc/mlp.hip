@@ -232,6 +232,7 @@ struct ceos_mlp {
     int din[4] = {0, 0, 0, 0}, dout[4] = {0, 0, 0, 0};
     hipStream_t stream = nullptr;
     long long step = 0;
+    bool train_fp32 = false;  // reference mode: the training GEMMs in FP32 (k_gemm_f32)
     float *params = nullptr, *grads = nullptr, *m1 = nullptr, *m2 = nullptr;
     // activations for max_batch (FP32): Z1..Z3, A1..A3, Y, dY, dZ (2 buffers), split partials
     float *Z[3] = {nullptr, nullptr, nullptr}, *A[3] = {nullptr, nullptr, nullptr};
@@ -293,16 +294,28 @@ Gemm fwd(const ceos_mlp* m, int l, int rows, const float* x, float* z) {
     return g;
 }
 
-// BF16 forward for training: keeps Z and A of the hidden layers, Y
+// training GEMM: BF16 MFMA (split-K partials at c + z M N), or FP32 in the reference mode (one split)
+template <int EPI>
+void run_train(const ceos_mlp* m, const Gemm& g, hipStream_t s, int splits = 1) {
+    if (m->train_fp32) {
+        Gemm h = g;
+        h.ksplit = g.K;
+        run_f32<EPI>(h, s);
+    } else {
+        run_bf16<EPI>(g, s, splits);
+    }
+}
+
+// forward for training: keeps Z and A of the hidden layers, Y
 void forward_train(ceos_mlp* m, const float* X, int B, hipStream_t s) {
     const float* in = X;
     for (int l = 0; l < 3; ++l) {
         Gemm g = fwd(m, l, B, in, m->Z[l]);
         g.act = m->A[l];
-        run_bf16<kEpiBiasSilu>(g, s);
+        run_train<kEpiBiasSilu>(m, g, s);
         in = m->A[l];
     }
-    run_bf16<kEpiBias>(fwd(m, 3, B, in, m->Y), s);
+    run_train<kEpiBias>(m, fwd(m, 3, B, in, m->Y), s);
 }
 
 // weight gradient dW_l = dZ^T A_in (split-K over the batch, deterministic reduction) and bias
@@ -310,8 +323,8 @@ void weight_grad(ceos_mlp* m, int l, const float* dz, const float* ain, int B, h
     const int no = m->dout[l], ni = m->din[l];
     Gemm g = gemm(no, ni, B, dz, 1, no, ain, ni, 1, m->part);
     g.ksplit = (((B + kSplits - 1) / kSplits) + 15) / 16 * 16;
-    const int splits = (B + g.ksplit - 1) / g.ksplit;
-    run_bf16<kEpiStore>(g, s, splits);
+    const int splits = m->train_fp32 ? 1 : (B + g.ksplit - 1) / g.ksplit;
+    run_train<kEpiStore>(m, g, s, splits);
     const long long n = (long long)no * ni;
     hipLaunchKernelGGL(k_reduce_splits, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, m->part, splits, n,
                        m->grads + m->off_w[l]);
@@ -334,7 +347,7 @@ int grads(ceos_mlp* m, const float* X, const float* T, const float* W, int B, hi
         Gemm g = gemm(B, m->din[l], m->dout[l], dcur, m->dout[l], 1, m->params + m->off_w[l], m->din[l], 1, dprev);
         g.z = m->Z[l - 1];
         g.zrows = B;
-        run_bf16<kEpiDSilu>(g, s);
+        run_train<kEpiDSilu>(m, g, s);
         weight_grad(m, l - 1, dprev, l - 1 == 0 ? X : m->A[l - 2], B, s);
         dcur = dprev;
     }
@@ -429,6 +442,12 @@ void ceos_mlp_destroy(ceos_mlp* m) {
 }
 
 long long ceos_mlp_num_params(const ceos_mlp* m) { return m ? m->nparams : -1; }
+
+int ceos_mlp_set_training_precision(ceos_mlp* m, int fp32) {
+    if (!m || (fp32 != 0 && fp32 != 1)) return CEOS_ERR_ARG;
+    m->train_fp32 = fp32 == 1;
+    return CEOS_OK;
+}
 
 int ceos_mlp_shape(const ceos_mlp* m, int* n_in, int* width, int* n_out, int* max_batch) {
     if (!m) return CEOS_ERR_ARG;

@@ -785,6 +785,9 @@ __global__ __launch_bounds__(kRveThreads) void k_rve_tangent(RveParams p) {
     __shared__ int snode[kRveElemPerRound][8];
     __shared__ int sq;
     __shared__ double sFa[9];
+    // Phi^T K Phi (45 upper), V0, condensed parts (81): kept in LDS, so that no register holds them
+    // through the nine CG solves (their 90 VGPRs would push the CG's batched loads into spills)
+    __shared__ double sT[45 + 1 + 81];
     const int tid = threadIdx.x;
     const long long slot = blockIdx.x;
     for (;;) {
@@ -800,17 +803,23 @@ __global__ __launch_bounds__(kRveThreads) void k_rve_tangent(RveParams p) {
         if (tid < 9) sFa[tid] = p.Fconv[9 * rid + 3 * (tid % 3) + tid / 3];
         __syncthreads();
         rve_assemble<MODEL, true>(c, p, sFa, lds, snode);
-        double Pbar[10], T1[45];
-        rve_gp_pass<MODEL, true>(c, p, sFa, lds, Pbar, T1);
-        wg_sum<10>(Pbar, red);
-        wg_sum<45>(T1, red);
-        const double V0 = Pbar[9];
+        {
+            double Pbar[10], T1[45];
+            rve_gp_pass<MODEL, true>(c, p, sFa, lds, Pbar, T1);
+            wg_sum<10>(Pbar, red);
+            wg_sum<45>(T1, red);
+            if (tid == 0) {
+#pragma unroll
+                for (int k = 0; k < 45; ++k) sT[k] = T1[k];
+                sT[45] = Pbar[9];
+            }
+        }
         rve_affine_rhs(c);
         rve_precond(c);
         __syncthreads();
         const long long nn = c.nn;
         int pcg_total = 0;
-        // condensed part b_k . v_m, v_m = K_red^-1 (-b_m), parked in A until the end
+        // condensed part b_k . v_m, v_m = K_red^-1 (-b_m)
         double* Aout = p.A + 81LL * rid;
         for (int m = 0; m < 9; ++m) {
             pcg_total += rve_pcg(c, p, c.B + (long long)m * 3 * nn, -1.0, p.tangent_rtol, lds, red);
@@ -828,7 +837,7 @@ __global__ __launch_bounds__(kRveThreads) void k_rve_tangent(RveParams p) {
             wg_sum<9>(dots, red);
             if (tid == 0) {
 #pragma unroll
-                for (int k = 0; k < 9; ++k) Aout[9 * k + m] = dots[k];
+                for (int k = 0; k < 9; ++k) sT[46 + 9 * k + m] = dots[k];
             }
         }
         __syncthreads();
@@ -836,10 +845,7 @@ __global__ __launch_bounds__(kRveThreads) void k_rve_tangent(RveParams p) {
             const int I = tid / 9, K = tid % 9;
             const int lo = I < K ? I : K, hi = I < K ? K : I;
             const int idx = lo * 9 - (lo * (lo - 1)) / 2 + (hi - lo);
-            double t1 = 0.0;
-#pragma unroll
-            for (int k = 0; k < 45; ++k) t1 = (k == idx) ? T1[k] : t1;
-            Aout[tid] = (t1 + Aout[tid]) / V0;
+            Aout[tid] = (sT[idx] + sT[46 + tid]) / sT[45];
         }
         if (tid == 0) p.info[4 * rid + 2] += pcg_total;
         __syncthreads();

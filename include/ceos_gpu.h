@@ -288,6 +288,9 @@ int ceos_mlp_create(int n_in, int width, int n_out, int max_batch, unsigned long
 void ceos_mlp_destroy(ceos_mlp *m);
 long long ceos_mlp_num_params(const ceos_mlp *m);
 int ceos_mlp_shape(const ceos_mlp *m, int *n_in, int *width, int *n_out, int *max_batch);
+/* 0 (default): training GEMMs on BF16 MFMA with FP32 accumulation; 1: FP32 GEMMs throughout (a
+ * reference mode for checking the BF16 path's training behaviour). */
+int ceos_mlp_set_training_precision(ceos_mlp *m, int fp32);
 int ceos_mlp_get_params(ceos_mlp *m, float *host_params);
 /* Also resets the Adam moments. */
 int ceos_mlp_set_params(ceos_mlp *m, const float *host_params);

@@ -74,3 +74,62 @@ def test_macro_evaluation_surrogate_vs_rve(surrogate):
     print("points: mean %.4f max %.4f  Fint %.4f  K %.4f" % (err.mean(), err.max(), fr, kr), flush=True)
     assert not s["fallback"] and s["judged"] == 1
     assert err.mean() < MEAN_ERR_BAR and fr < MEAN_ERR_BAR and kr < 0.05
+
+
+class _Fe2Backend:
+    """The quasi-static driver's backend interface over a MacroFE2 (plain or surrogate)."""
+
+    def __init__(self, m, mode):
+        self.m, self.mode, self.ndof = m, mode, m.ndof
+        self.evals = []
+
+    def csr(self):
+        return self.m.mesh.csr()
+
+    def solve_constitutive(self, time, U):
+        try:
+            st = self.m.evaluate_rve(U) if self.mode == "rve" else self.m.evaluate_surrogate(U, judge_tol=0.05)
+        except gpu.CeosError as e:
+            return e.code
+        self.evals.append(st)
+        return 0
+
+    def internal_force(self):
+        return self.m.fint(), 0
+
+    def tangent(self):
+        return self.m.tangent()
+
+    def update_mesh(self, U):
+        pass  # every evaluation moves the macro mesh to X + U itself
+
+    def switch_converged_state(self):
+        self.m.new_judge_sample()
+
+
+def test_macro_newton_solve_surrogate_vs_rve(surrogate):
+    """A macro load history solved to convergence twice (Newton, Dirichlet, the driver's sparse
+    direct solve): with every Gauss point an RVE, and with the surrogate.  Converged macro
+    displacements agree to the surrogate's accuracy; both Newton loops converge quadratically
+    (each path's tangent is consistent with its own stress)."""
+    import fe_driver as fd
+    out = {}
+    for mode in ("rve", "surrogate"):
+        m = fe2.MacroFE2(n_macro=2, side_choices=(8,), seed=SEED_HELD_OUT, surrogate=surrogate)
+        if mode == "surrogate":
+            m.setup_descriptors()
+            m.rves.reset()
+        X = m.X
+        fixed = [3 * i + k for i in np.nonzero(X[:, 0] < 1e-12)[0] for k in range(3)]
+        presc = [(3 * i, (lambda t: 0.06 * t)) for i in np.nonzero(X[:, 0] > 1 - 1e-12)[0]]
+        bc = fd.BoundaryConditions(m.ndof, fixed, presc)
+        be = _Fe2Backend(m, mode)
+        hist = fd.quasi_static(be, bc, [(0.0, 1.0, 2)], nl=True, tol=1e-6, maxiter=12)
+        out[mode] = (hist[-1][3], [h[4] for h in hist], be.evals)
+        print(mode, "iterations", out[mode][1], flush=True)
+    Ur, Us = out["rve"][0], out["surrogate"][0]
+    d = np.linalg.norm(Us - Ur) / np.linalg.norm(Ur)
+    print("converged macro U: surrogate vs RVE %.2e" % d, flush=True)
+    assert d < 1e-2
+    assert all(e["fallback"] is False for e in out["surrogate"][2]), out["surrogate"][2]
+    assert max(out["surrogate"][1]) <= max(out["rve"][1]) + 1

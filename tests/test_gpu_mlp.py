@@ -97,3 +97,16 @@ def test_weighted_gradient_matches_fp64():
     g2, _ = m.gradients(_dev(X), _dev(T), n, dW=_dev(W2))
     _, go2 = mo.loss_and_grad(p.astype(np.float64), X[1::2], T[1::2], *DIMS)
     assert np.linalg.norm(2 * g2 - go2) / np.linalg.norm(go2) < 2e-2
+
+
+def test_fp32_training_mode_gradient_matches_fp64():
+    """The reference training mode (FP32 GEMMs) reproduces the FP64 gradient to FP32 rounding."""
+    n = 512
+    m = gpu.Mlp(*DIMS, max_batch=n, seed=6)
+    m.set_training_fp32(True)
+    p = m.params()
+    X, T = _data(23, n)
+    g, loss = m.gradients(_dev(X), _dev(T), n)
+    lo, go = mo.loss_and_grad(p.astype(np.float64), X, T, *DIMS)
+    assert abs(loss - lo) < 1e-5 * lo
+    assert np.linalg.norm(g - go) / np.linalg.norm(go) < 1e-4
