@@ -225,6 +225,8 @@ def main():
                     help="only the FE^2 macro-step metric (prints its object alone; for profiling it)")
     ap.add_argument("--precision", choices=["fp64", "fp32k"], default="fp64")
     ap.add_argument("--slabs", type=int, default=0, help="launch schedule (0 = automatic)")
+    ap.add_argument("--pipeline", type=int, default=-1,
+                    help="element/gather pipeline chunks (-1 = the library's default, off; 0 = off)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="copy 1 GiB device-to-device before the timed region: the reference kernel "
                          "for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE to bytes")
@@ -248,6 +250,8 @@ def main():
     mesh = gpu.CeosMesh(X, conn, precision=prec)
     mesh.set_material(1, gpu.MODEL_CNH, [MU, LAM])
     mesh.set_schedule(args.slabs)
+    if args.pipeline >= 0:
+        mesh.set_pipeline(args.pipeline)
     ncolor, nnz = mesh.info()
     ne = len(conn)
     dU = hiprt.DeviceArray.from_numpy(U)

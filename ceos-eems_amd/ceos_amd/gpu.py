@@ -64,6 +64,7 @@ EXPORTS = [
     "ceos_surrogate_create", "ceos_surrogate_destroy", "ceos_surrogate_features_device",
     "ceos_surrogate_eval_device", "ceos_rve_set_precision", "ceos_rve_set_tangent_rtol",
     "ceos_mlp_gradients_weighted", "ceos_mlp_train_step_weighted", "ceos_mlp_set_training_precision",
+    "ceos_set_pipeline",
 ]
 
 
@@ -110,6 +111,7 @@ def load_library(path=LIB_PATH):
     L.ceos_device_status.argtypes = [vp, C.POINTER(CeosStatus)]
     L.ceos_mesh_info.argtypes = [vp, C.POINTER(i), C.POINTER(C.c_longlong)]
     L.ceos_set_schedule.argtypes = [vp, i]
+    L.ceos_set_pipeline.argtypes = [vp, i]
     L.ceos_set_assembly.argtypes = [vp, i]
     L.ceos_set_tangent_mode.argtypes = [vp, i]
     L.ceos_timing_start.argtypes = [vp, i]
@@ -221,6 +223,9 @@ class CeosMesh:
 
     def set_assembly(self, mode):
         _check(load_library().ceos_set_assembly(self._h, mode), "ceos_set_assembly")
+
+    def set_pipeline(self, chunks):
+        _check(load_library().ceos_set_pipeline(self._h, chunks), "ceos_set_pipeline")
 
     def set_schedule(self, slabs):
         _check(load_library().ceos_set_schedule(self._h, slabs), "ceos_set_schedule")

@@ -21,6 +21,7 @@ namespace ceos {
 struct GatherParams {
     int nn;
     long long nblocks;
+    long long blk0, blk1;    // the launch covers blocks [blk0, blk1)
     const int* blknode;      // [nblocks] node A of block
     const int* rowstart;     // [nn] slot of (3A, 3A)
     const int* nup;          // [nn]
@@ -34,8 +35,8 @@ struct GatherParams {
 
 template <typename KT>
 __global__ __launch_bounds__(256) void k_gather_blocks(GatherParams p) {
-    const long long blk = (long long)blockIdx.x * 256 + threadIdx.x;
-    if (blk >= p.nblocks) return;
+    const long long blk = p.blk0 + (long long)blockIdx.x * 256 + threadIdx.x;
+    if (blk >= p.blk1) return;
     const int c0 = p.cptr[blk], c1 = p.cptr[blk + 1];
     const int A = p.blknode[blk];
     const int q = (int)(blk - p.blkstart[A]);

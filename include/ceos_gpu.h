@@ -170,6 +170,15 @@ int ceos_set_assembly(ceos_handle *h, int mode);
 #define CEOS_TANGENT_NUMERICAL 1
 int ceos_set_tangent_mode(ceos_handle *h, int mode);
 
+/* Pipelined evaluation with the gather assembly (no reference counterpart: a schedule): the
+ * element launches are cut into `chunks` pieces of the slot order, and after each piece the
+ * gather of the CSR rows whose contributing elements are all done runs on a second stream of the
+ * handle, overlapping the next piece (FP64 compute against HBM traffic).  The caller's stream
+ * waits for that stream at the end of the evaluation.  Results are bitwise the same for every
+ * chunk count.  chunks 0 or 1 (default): no pipeline.  On the 80^3 cube the overlap does not pay:
+ * the element kernel streams 1.8 GB per evaluation itself, and the kernels contend for HBM. */
+int ceos_set_pipeline(ceos_handle *h, int chunks);
+
 /* Mesh facts after setup: number of element colours and the nnz of the upper pattern. */
 int ceos_mesh_info(ceos_handle *h, int *n_colors, long long *nnz_upper);
 
