@@ -144,19 +144,22 @@ def fe2_bench(no_cpu):
     m.rves.set_precision(gpu.K_FP32_STORED)
     m.rves.set_tangent_rtol(1e-8)
     t_build = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    tile, tile_s = m.rves.autotune(max_rves=256)
+    t_tune = time.perf_counter() - t0
     t_desc = m.setup_descriptors()
     U1 = macro_displacement(m.X)
     U2 = U1 + newton_correction(m.X)
     # surrogate path first, on undeformed RVEs (its judges and fallbacks start there)
     m.rves.reset()
-    s1 = m.evaluate_surrogate(U1)
-    s2 = m.evaluate_surrogate(U2)
+    s1 = m.evaluate_surrogate(U1, use_graph=True)
+    s2 = m.evaluate_surrogate(U2, use_graph=True)
     ext_s = m.ext()
     f_s, k_s = m.fint(), m.tangent()
     Fgp = m.gp_F()
     m.rves.reset()
-    p1 = m.evaluate_rve(U1)
-    p2 = m.evaluate_rve(U2)
+    p1 = m.evaluate_rve(U1, use_graph=True)
+    p2 = m.evaluate_rve(U2, use_graph=True)
     ext_p = m.ext()
     f_p, k_p = m.fint(), m.tangent()
     info = m.dinfo.numpy().reshape(-1, 4)
@@ -175,6 +178,8 @@ def fe2_bench(no_cpu):
                    "rve_tangent_storage": "fp32 (FP64 residual, dot products, convergence tests)",
                    "rve_newton_rtol": 1e-10, "rve_cg_rtol": 1e-10, "rve_condensation_rtol": 1e-8,
                    "macro_gp_stretch_range": [float(lam.min()), float(lam.max())],
+                   "rve_cg_tile": {"chosen": tile, "probe_s": {str(g): float(t) for g, t in zip((2, 4, 8), tile_s)}},
+                   "device_chains": "HIP graphs (captured at the first evaluation of each path)",
                    "surrogate": "MLP 27->256->256->256->6 (SiLU), FP32 inference, trained on %d RVE "
                                 "samples" % int(sur[2]["n_samples"])},
         "plain_s": p2["seconds"], "surrogate_s": s2["seconds"],
@@ -194,7 +199,7 @@ def fe2_bench(no_cpu):
                        "peak": PEAK_HBM_GBS, "unit": "GB/s",
                        "note": "algorithmic CG bytes (FP32 tangent rows + FP64 vectors) over the whole plain "
                                "evaluation's wall time, a lower bound for the CG kernels alone"},
-        "setup_s": {"rve_batch_and_surrogate": t_build, "descriptors": t_desc},
+        "setup_s": {"rve_batch_and_surrogate": t_build, "cg_tile_autotune": t_tune, "descriptors": t_desc},
         "surrogate_test_err": {"mean": float(sur[2]["test_err_mean"]), "p99": float(sur[2]["test_err_p99"]),
                                "max": float(sur[2]["test_err_max"]),
                                "base_only_mean": float(sur[2]["test_err_base_mean"])},

@@ -228,6 +228,11 @@ class MacroFE2:
         self.sur = surrogate  # (Mlp, Surrogate, meta) or None
         self.judge_rng = np.random.default_rng(seed)
         self.judge_ids = None
+        self.graphs = {}
+        # set up the mesh's device tables now: graph capture must find nothing left to allocate
+        self.dU.upload(np.zeros(self.ndof))
+        self.mesh.gp_deformation_gradient_device(self.dU.ptr, self.dF.ptr, self.stream.ptr)
+        self.stream.synchronize()
 
     def setup_descriptors(self, pcg_rtol=1e-8):
         t = time.perf_counter()
@@ -237,13 +242,42 @@ class MacroFE2:
         return time.perf_counter() - t
 
     def _kinematics(self, U):
-        s = self.stream.ptr
         self.dU.upload(np.ascontiguousarray(U, dtype=np.float64))
+
+    # device chains of one evaluation, all on self.stream: enqueue only (graph capturable)
+    def _chain_kinematics(self):
+        s = self.stream.ptr
         self.mesh.update_mesh_device(self.dU.ptr, s)
         self.mesh.gp_deformation_gradient_device(self.dU.ptr, self.dF.ptr, s)
 
-    def _assemble(self):
+    def _chain_rve(self):
+        s = self.stream.ptr
+        self._chain_kinematics()
+        self.rves.solve_device(self.dF.ptr, self.dP.ptr, self.dA.ptr, self.dinfo.ptr, s)
+        gpu.rve_to_external_device(self.ngp, self.dF.ptr, self.dP.ptr, self.dA.ptr, self.dext.ptr, stream=s)
+        self._chain_assemble()
+
+    def _chain_surrogate(self):
+        self._chain_kinematics()
+        self.sur[1].eval_device(self.ngp, self.dF.ptr, self.dClin.ptr, self.dext.ptr, self.dflags.ptr, None,
+                                self.stream.ptr)
+
+    def _chain_assemble(self):
         self.mesh.evaluate_device(0.0, self.dU.ptr, self.dFint.ptr, self.dK.ptr, self.stream.ptr)
+
+    def _run(self, key, use_graph):
+        """Enqueue a device chain, replaying its captured HIP graph when use_graph."""
+        fn = getattr(self, "_chain_" + key)
+        if not use_graph:
+            fn()
+            return
+        g = self.graphs.get(key)
+        if g is None:
+            g = self.graphs[key] = hiprt.Graph(self.stream, fn)
+        g.launch(self.stream)
+
+    def _assemble(self, use_graph=False):
+        self._run("assemble", use_graph)
 
     def _finish(self, stats):
         self.stream.synchronize()
@@ -270,14 +304,12 @@ class MacroFE2:
         self.stream.synchronize()
         return self.dK.numpy()
 
-    def evaluate_rve(self, U):
-        """Plain macro evaluation: every RVE solved and condensed, then Fint and K."""
+    def evaluate_rve(self, U, use_graph=False):
+        """Plain macro evaluation: every RVE solved and condensed, then Fint and K (the device chain
+        replayed from a HIP graph when use_graph)."""
         t0 = time.perf_counter()
-        s = self.stream.ptr
         self._kinematics(U)
-        self.rves.solve_device(self.dF.ptr, self.dP.ptr, self.dA.ptr, self.dinfo.ptr, s)
-        gpu.rve_to_external_device(self.ngp, self.dF.ptr, self.dP.ptr, self.dA.ptr, self.dext.ptr, stream=s)
-        self._assemble()
+        self._run("rve", use_graph)
         self.stream.synchronize()
         info = self.dinfo.numpy().reshape(-1, 4)
         if np.any(info[:, 0] != 0):
@@ -291,7 +323,7 @@ class MacroFE2:
         self.judge_ids = None
 
     def evaluate_surrogate(self, U, judge_frac=0.01, judge_tol=0.01, fallback_tangent="surrogate",
-                           judge_newton_rtol=1e-6):
+                           judge_newton_rtol=1e-6, use_graph=False):
         """Surrogate macro evaluation: the MLP at points inside the training hull; outside it the
         RVE solve gives the stress (with fallback_tangent="surrogate" the point keeps the
         surrogate's tangent, an inexact macro Newton tangent; "rve" condenses the RVE tangent too);
@@ -306,7 +338,7 @@ class MacroFE2:
         t0 = time.perf_counter()
         s = self.stream.ptr
         self._kinematics(U)
-        self.sur[1].eval_device(self.ngp, self.dF.ptr, self.dClin.ptr, self.dext.ptr, self.dflags.ptr, None, s)
+        self._run("surrogate", use_graph)
         self.stream.synchronize()
         flags = self.dflags.numpy()
         Fcm = self.dF.numpy().reshape(self.ngp, 9)
@@ -323,7 +355,7 @@ class MacroFE2:
             gpu.rve_to_external_device(m, dFs.ptr, dPs.ptr, dAs.ptr if with_t else None, self.dext.ptr,
                                        dsel=dsel.ptr, stream=s)
         # the macro assembly runs while the judged points are re-solved
-        self._assemble()
+        self._assemble(use_graph)
         inside = np.nonzero(flags == 0)[0]
         nj = min(len(inside), int(math.ceil(judge_frac * self.ngp))) if judge_frac > 0 else 0
         if nj and (self.judge_ids is None or len(self.judge_ids) != nj):
@@ -351,7 +383,7 @@ class MacroFE2:
                  "judge_err": agg, "judge_point_max": float(err.max()) if len(err) else 0.0,
                  "judge_point_mean": float(err.mean()) if len(err) else 0.0, "fallback": False}
         if len(err) and agg > judge_tol:
-            r = self.evaluate_rve(U)
+            r = self.evaluate_rve(U, use_graph)
             stats["fallback"] = True
             stats["rve_solves"] = r["rve_solves"] + int(len(out)) + len(err)
             stats["seconds"] = time.perf_counter() - t0

@@ -64,7 +64,7 @@ EXPORTS = [
     "ceos_surrogate_create", "ceos_surrogate_destroy", "ceos_surrogate_features_device",
     "ceos_surrogate_eval_device", "ceos_rve_set_precision", "ceos_rve_set_tangent_rtol",
     "ceos_mlp_gradients_weighted", "ceos_mlp_train_step_weighted", "ceos_mlp_set_training_precision",
-    "ceos_set_pipeline",
+    "ceos_set_pipeline", "ceos_rve_autotune", "ceos_rve_set_cg_group",
 ]
 
 
@@ -125,6 +125,8 @@ def load_library(path=LIB_PATH):
     L.ceos_rve_set_inexact.argtypes = [vp, i]
     L.ceos_rve_set_precision.argtypes = [vp, i]
     L.ceos_rve_set_tangent_rtol.argtypes = [vp, d]
+    L.ceos_rve_autotune.argtypes = [vp, i, C.POINTER(i), _dp]
+    L.ceos_rve_set_cg_group.argtypes = [vp, i]
     L.ceos_rve_profile.argtypes = [vp, vp]
     f = C.c_float
     L.ceos_mlp_create.argtypes = [i, i, i, i, C.c_ulonglong, C.POINTER(vp)]
@@ -385,6 +387,17 @@ class RveBatch:
 
     def set_precision(self, mode):
         _check(load_library().ceos_rve_set_precision(self._h, mode), "ceos_rve_set_precision")
+
+    def autotune(self, max_rves=256):
+        """Choose the CG sweep tile by timing the condensation of a sample; returns (tile, seconds
+        per candidate tile 2, 4, 8)."""
+        g = C.c_int()
+        t = np.zeros(3)
+        _check(load_library().ceos_rve_autotune(self._h, max_rves, C.byref(g), t), "ceos_rve_autotune")
+        return g.value, t
+
+    def set_cg_group(self, g):
+        _check(load_library().ceos_rve_set_cg_group(self._h, g), "ceos_rve_set_cg_group")
 
     def set_tangent_rtol(self, rtol):
         _check(load_library().ceos_rve_set_tangent_rtol(self._h, rtol), "ceos_rve_set_tangent_rtol")

@@ -37,6 +37,12 @@ def hip():
         L.hipGetDeviceCount.argtypes = [C.POINTER(C.c_int)]
         L.hipGetErrorString.restype = C.c_char_p
         L.hipGetErrorString.argtypes = [C.c_int]
+        L.hipStreamBeginCapture.argtypes = [vp, C.c_int]
+        L.hipStreamEndCapture.argtypes = [vp, C.POINTER(vp)]
+        L.hipGraphInstantiate.argtypes = [C.POINTER(vp), vp, vp, vp, C.c_size_t]
+        L.hipGraphLaunch.argtypes = [vp, vp]
+        L.hipGraphExecDestroy.argtypes = [vp]
+        L.hipGraphDestroy.argtypes = [vp]
         _hip = L
     return _hip
 
@@ -154,6 +160,38 @@ class Event:
         try:
             if self.ptr:
                 hip().hipEventDestroy(self.ptr)
+        except Exception:
+            pass
+
+
+CAPTURE_THREAD_LOCAL = 1
+
+
+class Graph:
+    """A HIP graph captured from the work `fn` enqueues on `stream` (fn must only enqueue: kernels,
+    device-to-device copies, event records; no memsets, no host synchronisation), replayed by
+    launch().  Replays see the same device pointers, so inputs are updated in place."""
+
+    def __init__(self, stream, fn):
+        L = hip()
+        check(L.hipStreamBeginCapture(stream.ptr, CAPTURE_THREAD_LOCAL), "hipStreamBeginCapture")
+        g = C.c_void_p()
+        try:
+            fn()
+        finally:
+            rc = L.hipStreamEndCapture(stream.ptr, C.byref(g))
+        check(rc, "hipStreamEndCapture")
+        e = C.c_void_p()
+        check(L.hipGraphInstantiate(C.byref(e), g, None, None, 0), "hipGraphInstantiate")
+        self.graph, self.exec = g.value, e.value
+
+    def launch(self, stream):
+        check(hip().hipGraphLaunch(self.exec, stream.ptr), "hipGraphLaunch")
+
+    def __del__(self):
+        try:
+            hip().hipGraphExecDestroy(self.exec)
+            hip().hipGraphDestroy(self.graph)
         except Exception:
             pass
 

@@ -33,6 +33,7 @@ struct ceos_rve_batch {
     int newton_maxit = 12, pcg_maxit = 5000, max_cutbacks = 20;
     int inexact = 1;
     int precision = CEOS_K_FP64;  // storage of the reduced tangent
+    int cg_group = kCgGroupDefault;  // nodes per thread and operand batch in the CG sweeps
     double tangent_rtol = -1.0;   // < 0: pcg_rtol
     // device
     int *dorder = nullptr, *dside = nullptr, *dinfo = nullptr, *dqueue = nullptr;
@@ -97,19 +98,27 @@ __global__ void k_rve_gather(int n_sel, const int* sel, const double* P, const d
     if (j < 4) infos[4 * k + j] = info[4LL * r + j];
 }
 
-template <int MODEL, typename MT>
-void launch(const RveParams& p, int grid, bool tangent, hipStream_t s) {
-    hipLaunchKernelGGL((k_rve_solve<MODEL, MT>), dim3(grid), dim3(kRveThreads), 0, s, p);
+// solve = false: the condensation only (the autotuner's probe)
+template <int MODEL, typename MT, int G>
+void launch(const RveParams& p, int grid, bool solve, bool tangent, hipStream_t s) {
+    if (solve) hipLaunchKernelGGL((k_rve_solve<MODEL, MT, G>), dim3(grid), dim3(kRveThreads), 0, s, p);
     if (tangent) {
         hipLaunchKernelGGL(k_zero_queue, dim3(1), dim3(1), 0, s, p.queue);
-        hipLaunchKernelGGL((k_rve_tangent<MODEL, MT>), dim3(grid), dim3(kRveThreads), 0, s, p);
+        hipLaunchKernelGGL((k_rve_tangent<MODEL, MT, G>), dim3(grid), dim3(kRveThreads), 0, s, p);
     }
 }
 
+template <int MODEL, typename MT>
+void launch_g(const RveParams& p, int grid, bool solve, bool tangent, int g, hipStream_t s) {
+    if (g == 2) launch<MODEL, MT, 2>(p, grid, solve, tangent, s);
+    else if (g == 8) launch<MODEL, MT, 8>(p, grid, solve, tangent, s);
+    else launch<MODEL, MT, 4>(p, grid, solve, tangent, s);
+}
+
 template <int MODEL>
-void launch_p(const RveParams& p, int grid, bool tangent, bool f32, hipStream_t s) {
-    if (f32) launch<MODEL, float>(p, grid, tangent, s);
-    else launch<MODEL, double>(p, grid, tangent, s);
+void launch_p(const RveParams& p, int grid, bool solve, bool tangent, bool f32, int g, hipStream_t s) {
+    if (f32) launch_g<MODEL, float>(p, grid, solve, tangent, g, s);
+    else launch_g<MODEL, double>(p, grid, solve, tangent, g, s);
 }
 
 int reset(ceos_rve_batch* b) {
@@ -121,8 +130,10 @@ int reset(ceos_rve_batch* b) {
     return CEOS_OK;
 }
 
-// one persistent launch over the queue `order` (n_queue RVE ids, largest first)
-int run(ceos_rve_batch* b, bool want_A, hipStream_t s, const int* order, int n_queue) {
+// one persistent launch over the queue `order` (n_queue RVE ids, largest first); solve = false
+// runs the condensation alone at the RVEs' current states
+int run(ceos_rve_batch* b, bool want_A, hipStream_t s, const int* order, int n_queue, bool solve = true,
+        int cg_group = 0) {
     RveParams p;
     p.n_rve = n_queue;
     p.order = order;
@@ -153,10 +164,11 @@ int run(ceos_rve_batch* b, bool want_A, hipStream_t s, const int* order, int n_q
     hipLaunchKernelGGL(k_zero_queue, dim3(1), dim3(1), 0, s, b->dqueue);
     const int grid = std::min(n_queue, b->slots);
     const bool f32 = b->precision == CEOS_K_FP32_STORED;
+    const int g = cg_group ? cg_group : b->cg_group;
     switch (b->model) {
-        case kNH: launch_p<kNH>(p, grid, want_A, f32, s); break;
-        case kCNH: launch_p<kCNH>(p, grid, want_A, f32, s); break;
-        case kSVK: launch_p<kSVK>(p, grid, want_A, f32, s); break;
+        case kNH: launch_p<kNH>(p, grid, solve, want_A, f32, g, s); break;
+        case kCNH: launch_p<kCNH>(p, grid, solve, want_A, f32, g, s); break;
+        case kSVK: launch_p<kSVK>(p, grid, solve, want_A, f32, g, s); break;
         default: return CEOS_ERR_ARG;
     }
     RVE_HIP_OK(hipGetLastError());
@@ -351,6 +363,63 @@ int ceos_rve_solve_subset(ceos_rve_batch* b, int n_sel, const int* sel, const do
     if (info_sel) std::memcpy(info_sel, inf.data(), sizeof(int) * 4 * n_sel);
     for (int k = 0; k < n_sel; ++k)
         if (inf[4 * k] != 0) return CEOS_ERR_LOCAL_NR;
+    return CEOS_OK;
+}
+
+int ceos_rve_autotune(ceos_rve_batch* b, int max_rves, int* chosen, double* seconds) {
+    if (!b || max_rves < 1) return CEOS_ERR_ARG;
+    // sample: up to max_rves RVEs, largest first (the CG of the largest sets the batch's time)
+    std::vector<int> q(b->n_rve);
+    std::iota(q.begin(), q.end(), 0);
+    std::stable_sort(q.begin(), q.end(), [&](int a, int c) { return b->side[a] > b->side[c]; });
+    const int n = std::min(max_rves, b->n_rve);
+    hipStream_t s = b->stream;
+    RVE_HIP_OK(hipMemcpyAsync(b->dsel_q, q.data(), sizeof(int) * n, hipMemcpyHostToDevice, s));
+    // the condensation needs converged states: mark the sample converged at its current state
+    // (the other RVEs keep their info)
+    std::vector<int> inf(4 * (size_t)b->n_rve);
+    RVE_HIP_OK(hipMemcpyAsync(inf.data(), b->dinfo, sizeof(int) * inf.size(), hipMemcpyDeviceToHost, s));
+    RVE_HIP_OK(hipStreamSynchronize(s));
+    for (int k = 0; k < n; ++k) inf[4 * (size_t)q[k]] = 0;
+    RVE_HIP_OK(hipMemcpyAsync(b->dinfo, inf.data(), sizeof(int) * inf.size(), hipMemcpyHostToDevice, s));
+    RVE_HIP_OK(hipStreamSynchronize(s));
+    hipEvent_t e0, e1;
+    RVE_HIP_OK(hipEventCreate(&e0));
+    RVE_HIP_OK(hipEventCreate(&e1));
+    // the probe only needs the time per CG iteration: a loose condensation tolerance keeps it short
+    // (iteration counts do not depend on the tile)
+    const double keep_rtol = b->tangent_rtol;
+    b->tangent_rtol = 1e-4;
+    int best = kCgGroupDefault;
+    double tbest = 1e300;
+    int rc = run(b, true, s, b->dsel_q, n, false, kCgGroupDefault);  // warm-up
+    for (int k = 0; k < 3 && rc == CEOS_OK; ++k) {
+        const int g = kCgGroups[k];
+        float ms = 0.0f;
+        if (hipEventRecord(e0, s) != hipSuccess) rc = CEOS_ERR_HIP;
+        if (rc == CEOS_OK) rc = run(b, true, s, b->dsel_q, n, false, g);
+        if (rc == CEOS_OK && (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                              hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+            rc = CEOS_ERR_HIP;
+        if (rc != CEOS_OK) break;
+        if (seconds) seconds[k] = 1e-3 * ms;
+        if (ms < tbest) {
+            tbest = ms;
+            best = g;
+        }
+    }
+    b->tangent_rtol = keep_rtol;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc != CEOS_OK) return rc;
+    b->cg_group = best;
+    if (chosen) *chosen = best;
+    return CEOS_OK;
+}
+
+int ceos_rve_set_cg_group(ceos_rve_batch* b, int g) {
+    if (!b || (g != 2 && g != 4 && g != 8)) return CEOS_ERR_ARG;
+    b->cg_group = g;
     return CEOS_OK;
 }
 

@@ -351,17 +351,18 @@ __device__ __forceinline__ void apply_minv(const double* mi, const double r[3], 
 
 // preconditioned CG on the reduced system: M x = sign * rhs (node 0 fixed) to
 // ||r|| <= rtol ||b||.  p lives in LDS, the other vectors in the workgroup's scratch.  The vector
-// sweeps take kCgGroup nodes per thread at a time and load all their operands before computing
-// (one round trip per group).  Returns the iterations used.
-constexpr int kCgGroup = 4;
+// sweeps take G nodes per thread at a time and load all their operands before computing (one
+// round trip per group); G is chosen among kCgGroups by ceos_rve_autotune.  Returns the
+// iterations used.
+constexpr int kCgGroups[3] = {2, 4, 8};
+constexpr int kCgGroupDefault = 4;
 
-template <typename MT>
+template <typename MT, int G>
 __device__ int rve_pcg(const RveCtx<MT>& c, const RveParams& p, const double* rhs, double sign, double rtol,
                        double* pl, double* red) {
     const int tid = threadIdx.x;
     const int nn = c.nn;
-    constexpr int G = kCgGroup;
-    constexpr int S = kRveThreads * kCgGroup;
+    constexpr int S = kRveThreads * G;
     double acc[2] = {0.0, 0.0};
     for (int i0 = tid; i0 < nn; i0 += S) {
         double r[G][3], mi[G][6];
@@ -669,7 +670,7 @@ __device__ __forceinline__ RveCtx<MT> rve_ctx(const RveParams& p, int rid, long 
     return c;
 }
 
-template <int MODEL, typename MT>
+template <int MODEL, typename MT, int G>
 __global__ __launch_bounds__(kRveThreads) void k_rve_solve(RveParams p) {
     __shared__ double lds[kRveLds];
     __shared__ double red[kRveWaves * 45];
@@ -732,7 +733,7 @@ __global__ __launch_bounds__(kRveThreads) void k_rve_solve(RveParams p) {
                 // reduction still to come (forcing 0.1 ||R|| / ||R_0||, within [pcg_rtol, 1e-2])
                 const double eta = p.inexact ? fmin(1e-2, fmax(p.pcg_rtol, 0.1 * nr / nr0)) : p.pcg_rtol;
                 const long long t1 = wall_clock64();
-                pcg_total += rve_pcg(c, p, c.R, -1.0, eta, lds, red);
+                pcg_total += rve_pcg<MT, G>(c, p, c.R, -1.0, eta, lds, red);
                 if (p.prof && tid == 0) p.prof[2 * rid + 1] += wall_clock64() - t1;
                 for (int k = tid; k < 3 * c.nn; k += kRveThreads) c.wc[k] += c.x[k];
                 __syncthreads();
@@ -778,7 +779,7 @@ __global__ __launch_bounds__(kRveThreads) void k_rve_solve(RveParams p) {
 // launch, after k_rve_solve): the reduced tangent at the converged state is assembled again,
 //   A = (Phi^T K Phi + sum_m b_m^T v_m) / V,  K_red v_m = -b_m,  b_m = T^T K Phi_m,
 // nine CG solves per RVE.  RVEs whose solve failed are skipped.
-template <int MODEL, typename MT>
+template <int MODEL, typename MT, int G>
 __global__ __launch_bounds__(kRveThreads) void k_rve_tangent(RveParams p) {
     __shared__ double lds[kRveLds];
     __shared__ double red[kRveWaves * 45];
@@ -822,7 +823,7 @@ __global__ __launch_bounds__(kRveThreads) void k_rve_tangent(RveParams p) {
         // condensed part b_k . v_m, v_m = K_red^-1 (-b_m)
         double* Aout = p.A + 81LL * rid;
         for (int m = 0; m < 9; ++m) {
-            pcg_total += rve_pcg(c, p, c.B + (long long)m * 3 * nn, -1.0, p.tangent_rtol, lds, red);
+            pcg_total += rve_pcg<MT, G>(c, p, c.B + (long long)m * 3 * nn, -1.0, p.tangent_rtol, lds, red);
             double dots[9];
 #pragma unroll
             for (int k = 0; k < 9; ++k) dots[k] = 0.0;

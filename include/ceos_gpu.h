@@ -246,6 +246,15 @@ int ceos_rve_set_precision(ceos_rve_batch *b, int mode);
 /* CG tolerance of the nine condensation solves of the homogenized tangent (default: pcg_rtol). */
 int ceos_rve_set_tangent_rtol(ceos_rve_batch *b, double rtol);
 
+/* Tile of the CG vector sweeps: every thread loads the operands of g nodes (2, 4 or 8) before
+ * computing them.  ceos_rve_autotune times the condensation of a sample of the batch's RVEs (up to
+ * max_rves, largest first, at their current states, CG to 1e-4: the time per iteration is what is
+ * compared; pure linear solves, the states are not changed, the sample's A and info entries are
+ * overwritten) for each g, keeps the fastest and returns it (chosen) with the three times in
+ * seconds (may be NULL).  Results agree to the solver tolerances across g. */
+int ceos_rve_autotune(ceos_rve_batch *b, int max_rves, int *chosen, double *seconds);
+int ceos_rve_set_cg_group(ceos_rve_batch *b, int g);
+
 /* Every RVE back to its undeformed state (F = I, zero fluctuation). */
 int ceos_rve_reset(ceos_rve_batch *b);
 

@@ -147,3 +147,25 @@ def test_fp32_stored_tangent_matches_fp64():
         print(k, rel(P32[k], P64[k]), rel(A32[k], A64[k]), flush=True)
         assert rel(P32[k], P64[k]) < 1e-9
         assert rel(A32[k], A64[k]) < 1e-5
+
+
+def test_autotune_picks_a_tile_and_every_tile_agrees():
+    """The CG sweep tile changes only the order of independent loads: solves and condensed
+    tangents with every tile agree to the solver tolerances; the autotuner times all three."""
+    sides = [4, 6, 8, 6]
+    b, _ = _batch(sides, [31, 32, 33, 34])
+    g, t = b.autotune(max_rves=4)
+    print("autotune", g, t, flush=True)
+    assert g in (2, 4, 8) and np.all(t > 0)
+    Fs = np.stack([_F(70 + k) for k in range(len(sides))])
+    out = {}
+    for tile in (2, 4, 8):
+        b.reset()
+        b.set_cg_group(tile)
+        P, A, info = b.solve(Fs, tangent=True)
+        assert np.all(info[:, 0] == 0)
+        out[tile] = (P, A)
+    for tile in (2, 8):
+        for k in range(len(sides)):
+            assert rel(out[tile][0][k], out[4][0][k]) < 1e-9
+            assert rel(out[tile][1][k], out[4][1][k]) < 1e-7
