@@ -65,6 +65,8 @@ EXPORTS = [
     "ceos_surrogate_eval_device", "ceos_rve_set_precision", "ceos_rve_set_tangent_rtol",
     "ceos_mlp_gradients_weighted", "ceos_mlp_train_step_weighted", "ceos_mlp_set_training_precision",
     "ceos_set_pipeline", "ceos_rve_autotune", "ceos_rve_set_cg_group",
+    "ceos_bc_set", "ceos_residual_bc", "ceos_residual_bc_device", "ceos_set_solver_precision",
+    "ceos_linear_solve", "ceos_linear_solve_device", "ceos_newton_update_device",
 ]
 
 
@@ -156,6 +158,13 @@ def load_library(path=LIB_PATH):
     L.ceos_surrogate_destroy.restype = None
     L.ceos_surrogate_features_device.argtypes = [vp, i, vp, vp, vp, vp, vp]
     L.ceos_surrogate_eval_device.argtypes = [vp, i, vp, vp, vp, vp, vp, vp]
+    L.ceos_bc_set.argtypes = [vp, i, vp, i, vp]
+    L.ceos_residual_bc.argtypes = [vp, i, vp, vp, vp, i, C.POINTER(d)]
+    L.ceos_residual_bc_device.argtypes = [vp, i, vp, vp, vp, vp, vp, vp, i, vp, vp]
+    L.ceos_set_solver_precision.argtypes = [vp, i]
+    L.ceos_linear_solve.argtypes = [vp, _dp, _dp, d, i, C.POINTER(i), C.POINTER(d), C.POINTER(i)]
+    L.ceos_linear_solve_device.argtypes = [vp, vp, vp, vp, d, i, C.POINTER(i), C.POINTER(d), C.POINTER(i), vp]
+    L.ceos_newton_update_device.argtypes = [vp, vp, vp, d, vp]
     _lib = L
     return L
 
@@ -339,6 +348,51 @@ class CeosMesh:
         st = CeosStatus()
         code = load_library().ceos_device_status(self._h, C.byref(st))
         return code, st
+
+    # Newton system: boundary conditions, residual norm, linear solve
+    def set_bc(self, presc_dof, fixed_dof):
+        """Constrained DOFs, 0-based (converted to the ABI's 1-based ids)."""
+        p = np.ascontiguousarray(np.asarray(presc_dof, dtype=np.int64) + 1, dtype=np.int32)
+        f = np.ascontiguousarray(np.asarray(fixed_dof, dtype=np.int64) + 1, dtype=np.int32)
+        _check(load_library().ceos_bc_set(self._h, len(p), p.ctypes.data if len(p) else None,
+                                          len(f), f.ctypes.data if len(f) else None), "ceos_bc_set")
+
+    def residual_bc(self, it, fext=None, ubar=None, norm_type=2):
+        """(R, norm) of Newton iteration `it` after solve_constitutive; K in the handle gets the BC."""
+        R = np.zeros(self.ndof)
+        nrm = C.c_double()
+        fe = None if fext is None else np.ascontiguousarray(fext, dtype=np.float64)
+        ub = None if ubar is None else np.ascontiguousarray(ubar, dtype=np.float64)
+        _check(load_library().ceos_residual_bc(self._h, it, None if fe is None else fe.ctypes.data,
+                                               None if ub is None else ub.ctypes.data, R.ctypes.data,
+                                               norm_type, C.byref(nrm)), "ceos_residual_bc")
+        return R, nrm.value
+
+    def residual_bc_device(self, it, dU, dFint, dK, dFext, dUbar, dR, dnorm, norm_type=2, stream=None):
+        _check(load_library().ceos_residual_bc_device(self._h, it, dU, dFint, dK, dFext, dUbar, dR, norm_type,
+                                                      dnorm, stream), "ceos_residual_bc_device")
+
+    def set_solver_precision(self, mode):
+        _check(load_library().ceos_set_solver_precision(self._h, mode), "ceos_set_solver_precision")
+
+    def linear_solve(self, b, rtol=1e-10, maxit=20000):
+        """x = K^-1 b with the handle's (BC-applied) tangent. Returns (x, iterations, relres, flag)."""
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        x = np.zeros(self.ndof)
+        it, rr, fl = C.c_int(), C.c_double(), C.c_int()
+        _check(load_library().ceos_linear_solve(self._h, b, x, rtol, maxit, C.byref(it), C.byref(rr), C.byref(fl)),
+               "ceos_linear_solve")
+        return x, it.value, rr.value, fl.value
+
+    def linear_solve_device(self, dK, db, dx, rtol=1e-10, maxit=20000, stream=None):
+        it, rr, fl = C.c_int(), C.c_double(), C.c_int()
+        _check(load_library().ceos_linear_solve_device(self._h, dK, db, dx, rtol, maxit, C.byref(it), C.byref(rr),
+                                                       C.byref(fl), stream), "ceos_linear_solve_device")
+        return it.value, rr.value, fl.value
+
+    def newton_update_device(self, dU, dDX, scale=1.0, stream=None):
+        _check(load_library().ceos_newton_update_device(self._h, dU, dDX, scale, stream),
+               "ceos_newton_update_device")
 
 
 class RveBatch:

@@ -22,6 +22,7 @@
 #include "element_kernel.hpp"
 #include "homogenize_kernel.hpp"
 #include "assembly_kernel.hpp"
+#include "system_kernel.hpp"
 
 using namespace ceos;
 
@@ -145,7 +146,26 @@ struct ceos_handle {
     int hpart_n = 0;
 
     bool f_valid = false, k_valid = false, have_u = false;
+    bool k_bc = false;  // the handle's K carries the boundary conditions (ceos_residual_bc)
     double time = 0.0;
+
+    // Newton system: boundary conditions, residual norm, PCG (system_kernel.hpp)
+    bool sys_ready = false;
+    int sys_width = 0;
+    int *dup_ptr = nullptr, *dup_nodes = nullptr, *dslot_nbr = nullptr;
+    uint32_t* dslot_src = nullptr;
+    uint8_t *ddof_mask = nullptr, *dnode_mask = nullptr;
+    int n_presc = 0, n_fixed = 0;
+    double *dR = nullptr, *dpart = nullptr, *dpart2 = nullptr, *dnorm = nullptr;
+    double *dFext = nullptr, *dUbar = nullptr;  // staging of the host residual call
+    int solver_precision = -1;                  // ELL values: -1 = the handle's precision
+    void* dell = nullptr;
+    bool ell_f32 = false;
+    double *dMinv = nullptr, *dcg_r = nullptr, *dcg_z = nullptr, *dcg_q = nullptr;
+    double *dcg_p[2] = {nullptr, nullptr};
+    double *dcg_b = nullptr, *dcg_x = nullptr;  // staging of the host solve call
+    CgScalars* dcg = nullptr;
+    int cg_check = 32;                          // iterations between host checks of the PCG state
 
     // phase timing: 3 events per evaluation (start, element kernels done, assembly done)
     std::vector<hipEvent_t> tev;
@@ -236,6 +256,13 @@ static void free_device(ceos_handle* h) {
     dfree(h->dconn); dfree(h->dmat); dfree(h->drowstart); dfree(h->dnup); dfree(h->dperm);
     dfree(h->dpairs); dfree(h->ddiag_first); dfree(h->delem_err); dfree(h->derr); dfree(h->dhpart);
     dfree(h->dscratch); dfree(h->dblknode); dfree(h->dblkstart); dfree(h->dcptr); dfree(h->dcword);
+    dfree(h->dup_ptr); dfree(h->dup_nodes); dfree(h->dslot_nbr); dfree(h->dslot_src);
+    dfree(h->ddof_mask); dfree(h->dnode_mask); dfree(h->dR); dfree(h->dpart); dfree(h->dpart2);
+    dfree(h->dnorm); dfree(h->dFext); dfree(h->dUbar);
+    if (h->dell) (void)hipFree(h->dell);
+    h->dell = nullptr;
+    dfree(h->dMinv); dfree(h->dcg_r); dfree(h->dcg_z); dfree(h->dcg_q); dfree(h->dcg_p[0]);
+    dfree(h->dcg_p[1]); dfree(h->dcg_b); dfree(h->dcg_x); dfree(h->dcg);
 }
 
 static int upload_props(ceos_handle* h) {
@@ -715,6 +742,195 @@ static void* kbuf(ceos_handle* h) {
 }
 
 // ------------------------------------------------------------------------------------------
+// Newton system around the element path (system_kernel.hpp)
+// ------------------------------------------------------------------------------------------
+
+// neighbour slots of every node (lower neighbours ascending, the node, upper neighbours
+// ascending), the upper lists and the per-DOF mask, on the device; once per mesh
+static int ensure_system(ceos_handle* h) {
+    if (h->sys_ready) return CEOS_OK;
+    const int nn = h->nn;
+    if (nn >= (1 << 25)) return CEOS_ERR_ARG;  // owner field of the slot words
+    std::vector<int> lptr(nn + 1, 0);
+    for (int B = 0; B < nn; ++B)
+        for (int q = h->up_ptr[B]; q < h->up_ptr[B + 1]; ++q) lptr[h->up_nodes[q] + 1]++;
+    for (int A = 0; A < nn; ++A) lptr[A + 1] += lptr[A];
+    std::vector<uint32_t> lsrc(lptr[nn]);
+    {
+        std::vector<int> fill(lptr.begin(), lptr.end() - 1);
+        for (int B = 0; B < nn; ++B)  // ascending B: every lower list comes out sorted
+            for (int q = h->up_ptr[B]; q < h->up_ptr[B + 1]; ++q)
+                lsrc[fill[h->up_nodes[q]]++] = ((uint32_t)B << 7) | (uint32_t)(q - h->up_ptr[B] + 1);
+    }
+    int width = 0;
+    for (int A = 0; A < nn; ++A) width = std::max(width, lptr[A + 1] - lptr[A] + 1 + h->nup[A]);
+    std::vector<int> nbr((size_t)width * nn);
+    std::vector<uint32_t> src((size_t)width * nn, kSlotPad);
+    for (int A = 0; A < nn; ++A) {
+        int j = 0;
+        auto put = [&](int B, uint32_t w) {
+            nbr[(size_t)j * nn + A] = B;
+            src[(size_t)j * nn + A] = w;
+            ++j;
+        };
+        for (int k = lptr[A]; k < lptr[A + 1]; ++k) put((int)(lsrc[k] >> 7), lsrc[k]);
+        put(A, (uint32_t)A << 7);
+        for (int q = h->up_ptr[A]; q < h->up_ptr[A + 1]; ++q)
+            put(h->up_nodes[q], ((uint32_t)A << 7) | (uint32_t)(q - h->up_ptr[A] + 1));
+        for (; j < width; ++j) nbr[(size_t)j * nn + A] = A;
+    }
+    const int nblk = (nn + kSysBlock - 1) / kSysBlock;
+    HIP_OK(dalloc(h->dup_ptr, nn + 1));
+    HIP_OK(dalloc(h->dup_nodes, h->up_nodes.size()));
+    HIP_OK(dalloc(h->dslot_nbr, nbr.size()));
+    HIP_OK(dalloc(h->dslot_src, src.size()));
+    HIP_OK(dalloc(h->ddof_mask, 3L * nn));
+    HIP_OK(dalloc(h->dnode_mask, nn));
+    HIP_OK(dalloc(h->dR, 3L * nn));
+    HIP_OK(dalloc(h->dpart, nblk));
+    HIP_OK(dalloc(h->dpart2, nblk));
+    HIP_OK(dalloc(h->dnorm, 3));
+    HIP_OK(hipMemcpy(h->dup_ptr, h->up_ptr.data(), sizeof(int) * (nn + 1), hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(h->dup_nodes, h->up_nodes.data(), sizeof(int) * h->up_nodes.size(), hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(h->dslot_nbr, nbr.data(), sizeof(int) * nbr.size(), hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(h->dslot_src, src.data(), sizeof(uint32_t) * src.size(), hipMemcpyHostToDevice));
+    HIP_OK(hipMemset(h->ddof_mask, 0, 3L * nn));
+    HIP_OK(hipMemset(h->dnode_mask, 0, nn));
+    h->sys_width = width;
+    h->sys_ready = true;
+    return CEOS_OK;
+}
+
+static SysGeom sys_geom(const ceos_handle* h) {
+    SysGeom g;
+    g.nn = h->nn;
+    g.width = h->sys_width;
+    g.rowstart = h->drowstart;
+    g.nup = h->dnup;
+    g.up_ptr = h->dup_ptr;
+    g.up_nodes = h->dup_nodes;
+    g.slot_nbr = h->dslot_nbr;
+    g.slot_src = h->dslot_src;
+    g.dof_mask = h->ddof_mask;
+    g.node_mask = h->dnode_mask;
+    return g;
+}
+
+// residual with the boundary conditions, the BC on K and the norm (all on stream s)
+static int residual_bc(ceos_handle* h, int it, const double* dU, const double* dFint, void* dK,
+                       const double* dFext, const double* dUbar, double* dR, int norm_type, double* dnorm,
+                       hipStream_t s) {
+    if (!h->sys_ready) return CEOS_ERR_ARG;
+    if (norm_type != 1 && norm_type != 2) return CEOS_ERR_ARG;
+    const bool rmod = it == 0 && h->n_presc > 0;
+    if (rmod && (!dUbar || !dU || !dK)) return CEOS_ERR_ARG;
+    const SysGeom g = sys_geom(h);
+    const int nblk = (h->nn + kSysBlock - 1) / kSysBlock;
+    const bool f32 = h->precision == CEOS_K_FP32_STORED;
+    if (f32)
+        hipLaunchKernelGGL(k_bc_residual<float>, dim3(nblk), dim3(kSysBlock), 0, s, g, (const float*)dK, (int)rmod,
+                           dFint, dFext, dUbar, dU, dR, h->dpart, h->dpart2);
+    else
+        hipLaunchKernelGGL(k_bc_residual<double>, dim3(nblk), dim3(kSysBlock), 0, s, g, (const double*)dK,
+                           (int)rmod, dFint, dFext, dUbar, dU, dR, h->dpart, h->dpart2);
+    hipLaunchKernelGGL(k_norm_final, dim3(1), dim3(kSysBlock), 0, s, (const double*)h->dpart,
+                       (const double*)h->dpart2, nblk, norm_type, dnorm);
+    if (dK && h->n_presc + h->n_fixed > 0) {
+        if (f32) hipLaunchKernelGGL(k_bc_matrix<float>, dim3(nblk), dim3(kSysBlock), 0, s, g, (float*)dK);
+        else hipLaunchKernelGGL(k_bc_matrix<double>, dim3(nblk), dim3(kSysBlock), 0, s, g, (double*)dK);
+    }
+    HIP_OK(hipGetLastError());
+    return CEOS_OK;
+}
+
+static int ensure_solver(ceos_handle* h) {
+    const int want_prec = h->solver_precision < 0 ? h->precision : h->solver_precision;
+    const bool f32 = want_prec == CEOS_K_FP32_STORED;
+    const size_t nval = (size_t)h->sys_width * 9 * h->nn;
+    if (!h->dell || h->ell_f32 != f32) {
+        if (h->dell) (void)hipFree(h->dell);
+        h->dell = nullptr;
+        HIP_OK(hipMalloc(&h->dell, nval * (f32 ? sizeof(float) : sizeof(double))));
+        h->ell_f32 = f32;
+    }
+    const long n = 3L * h->nn;
+    if (!h->dMinv) {
+        HIP_OK(dalloc(h->dMinv, 6L * h->nn));
+        HIP_OK(dalloc(h->dcg_r, n));
+        HIP_OK(dalloc(h->dcg_z, n));
+        HIP_OK(dalloc(h->dcg_q, n));
+        HIP_OK(dalloc(h->dcg_p[0], n));
+        HIP_OK(dalloc(h->dcg_p[1], n));
+        HIP_OK(dalloc(h->dcg, 1));
+    }
+    return CEOS_OK;
+}
+
+// x = K^-1 b by block-Jacobi PCG from x = 0; K (upper CSR, the handle's precision) after the
+// boundary conditions.  Synchronises s every cg_check iterations to read the state.
+static int linear_solve(ceos_handle* h, const void* dK, const double* db, double* dx, double rtol, int maxit,
+                        hipStream_t s, int* iters, double* relres, int* flag) {
+    if (!h->sys_ready || !dK || !db || !dx || !(rtol > 0.0) || maxit < 0) return CEOS_ERR_ARG;
+    int rc = ensure_solver(h);
+    if (rc) return rc;
+    const SysGeom g = sys_geom(h);
+    const int nn = h->nn;
+    const int nblk = (nn + kSysBlock - 1) / kSysBlock;
+    const long nt = (long)h->sys_width * nn;
+    const unsigned nbe = (unsigned)((nt + kSysBlock - 1) / kSysBlock);
+    const bool kf32 = h->precision == CEOS_K_FP32_STORED;
+    // ELL copy and preconditioner
+    if (kf32) {
+        if (h->ell_f32) hipLaunchKernelGGL((k_build_ell<float, float>), dim3(nbe), dim3(kSysBlock), 0, s, g, (const float*)dK, (float*)h->dell);
+        else hipLaunchKernelGGL((k_build_ell<float, double>), dim3(nbe), dim3(kSysBlock), 0, s, g, (const float*)dK, (double*)h->dell);
+        hipLaunchKernelGGL(k_block_jacobi<float>, dim3(nblk), dim3(kSysBlock), 0, s, g, (const float*)dK, h->dMinv);
+    } else {
+        if (h->ell_f32) hipLaunchKernelGGL((k_build_ell<double, float>), dim3(nbe), dim3(kSysBlock), 0, s, g, (const double*)dK, (float*)h->dell);
+        else hipLaunchKernelGGL((k_build_ell<double, double>), dim3(nbe), dim3(kSysBlock), 0, s, g, (const double*)dK, (double*)h->dell);
+        hipLaunchKernelGGL(k_block_jacobi<double>, dim3(nblk), dim3(kSysBlock), 0, s, g, (const double*)dK, h->dMinv);
+    }
+    CgScalars c0;
+    memset(&c0, 0, sizeof(c0));
+    c0.tol2 = rtol * rtol;
+    c0.maxit = maxit;
+    HIP_OK(hipMemcpyAsync(h->dcg, &c0, sizeof(c0), hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_cg_init, dim3(nblk), dim3(kSysBlock), 0, s, nn, (const double*)h->dMinv, db, dx, h->dcg_r,
+                       h->dcg_z, h->dcg_p[1], h->dpart, h->dpart2);
+    hipLaunchKernelGGL(k_cg_init_final, dim3(1), dim3(kSysBlock), 0, s, (const double*)h->dpart,
+                       (const double*)h->dpart2, nblk, h->dcg);
+    CgScalars c;
+    int k = 0;
+    for (;;) {
+        HIP_OK(hipMemcpyAsync(&c, h->dcg, sizeof(c), hipMemcpyDeviceToHost, s));
+        HIP_OK(hipStreamSynchronize(s));
+        if (c.done) break;
+        for (int i = 0; i < h->cg_check; ++i, ++k) {
+            double* pp = h->dcg_p[(k + 1) & 1];  // previous direction
+            double* pc = h->dcg_p[k & 1];        // this iteration's direction
+            if (h->ell_f32)
+                hipLaunchKernelGGL(k_cg_spmv<float>, dim3(nblk), dim3(kSysBlock), 0, s, g, (const float*)h->dell,
+                                   (const CgScalars*)h->dcg, (const double*)h->dcg_z, (const double*)pp, pc, h->dcg_q,
+                                   h->dpart);
+            else
+                hipLaunchKernelGGL(k_cg_spmv<double>, dim3(nblk), dim3(kSysBlock), 0, s, g, (const double*)h->dell,
+                                   (const CgScalars*)h->dcg, (const double*)h->dcg_z, (const double*)pp, pc, h->dcg_q,
+                                   h->dpart);
+            hipLaunchKernelGGL(k_cg_alpha, dim3(1), dim3(kSysBlock), 0, s, (const double*)h->dpart, nblk, h->dcg);
+            hipLaunchKernelGGL(k_cg_update, dim3(nblk), dim3(kSysBlock), 0, s, nn, (const double*)h->dMinv,
+                               (const CgScalars*)h->dcg, (const double*)pc, (const double*)h->dcg_q, dx, h->dcg_r,
+                               h->dcg_z, h->dpart, h->dpart2);
+            hipLaunchKernelGGL(k_cg_beta, dim3(1), dim3(kSysBlock), 0, s, (const double*)h->dpart,
+                               (const double*)h->dpart2, nblk, h->dcg);
+        }
+        HIP_OK(hipGetLastError());
+    }
+    if (iters) *iters = c.iters;
+    if (relres) *relres = c.bb > 0.0 ? sqrt(c.rr / c.bb) : 0.0;
+    if (flag) *flag = c.done;
+    return CEOS_OK;
+}
+
+// ------------------------------------------------------------------------------------------
 // C ABI
 // ------------------------------------------------------------------------------------------
 extern "C" {
@@ -937,6 +1153,7 @@ int ceos_solve_constitutive(ceos_handle* h, double time, const double* U, ceos_s
     if (rc) { set_status(st, rc, 0, "launch failed"); return rc; }
     h->f_valid = true;
     h->k_valid = K != nullptr;
+    h->k_bc = false;
     return read_status(h, st);
 }
 
@@ -959,6 +1176,7 @@ static int tangent_common(ceos_handle* h) {
         rc = evaluate(h, h->dU, h->dFint, kbuf(h), h->stream);
         if (rc) return rc;
         h->k_valid = h->f_valid = true;
+        h->k_bc = false;
     }
     return CEOS_OK;
 }
@@ -1230,6 +1448,107 @@ int ceos_gp_deformation_gradient(ceos_handle* h, const double* U, double* F) {
     (void)hipFree(dUt);
     (void)hipFree(dF);
     return rc;
+}
+
+int ceos_bc_set(ceos_handle* h, int n_presc, const int* presc_dof, int n_fixed, const int* fixed_dof) {
+    if (!h || n_presc < 0 || n_fixed < 0 || (n_presc && !presc_dof) || (n_fixed && !fixed_dof)) return CEOS_ERR_ARG;
+    int rc = ensure_system(h);
+    if (rc) return rc;
+    const long ndof = 3L * h->nn;
+    std::vector<uint8_t> dm(ndof, 0), nm(h->nn, 0);
+    for (int i = 0; i < n_presc; ++i) {
+        const long d = (long)presc_dof[i] - 1;
+        if (d < 0 || d >= ndof) return CEOS_ERR_ARG;
+        dm[d] |= kBcPresc;
+    }
+    for (int i = 0; i < n_fixed; ++i) {
+        const long d = (long)fixed_dof[i] - 1;
+        if (d < 0 || d >= ndof) return CEOS_ERR_ARG;
+        dm[d] |= kBcFixed;
+    }
+    for (long d = 0; d < ndof; ++d) nm[d / 3] |= dm[d];
+    HIP_OK(hipStreamSynchronize(h->stream));
+    HIP_OK(hipMemcpy(h->ddof_mask, dm.data(), ndof, hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(h->dnode_mask, nm.data(), h->nn, hipMemcpyHostToDevice));
+    h->n_presc = n_presc;
+    h->n_fixed = n_fixed;
+    return CEOS_OK;
+}
+
+int ceos_residual_bc_device(ceos_handle* h, int it, const double* dU, const double* dFint, void* dK,
+                            const double* dFext, const double* dUbar, double* dR, int norm_type, double* dnorm,
+                            void* stream) {
+    if (!h || !dFint || !dR || !dnorm || it < 0) return CEOS_ERR_ARG;
+    hipStream_t s = stream ? (hipStream_t)stream : h->stream;
+    return residual_bc(h, it, dU, dFint, dK, dFext, dUbar, dR, norm_type, dnorm, s);
+}
+
+int ceos_residual_bc(ceos_handle* h, int it, const double* Fext, const double* Ubar, double* R, int norm_type,
+                     double* normR) {
+    if (!h || it < 0 || !h->sys_ready) return CEOS_ERR_ARG;
+    if (it == 0 && h->n_presc > 0 && !Ubar) return CEOS_ERR_ARG;
+    if (h->k_bc) h->k_valid = false;  // EvaluateKt assembles a fresh tangent every call
+    int rc = tangent_common(h);
+    if (rc) return rc;
+    const long n = 3L * h->nn;
+    if (Fext) {
+        if (!h->dFext) HIP_OK(dalloc(h->dFext, n));
+        HIP_OK(hipMemcpyAsync(h->dFext, Fext, sizeof(double) * n, hipMemcpyHostToDevice, h->stream));
+    }
+    if (Ubar) {
+        if (!h->dUbar) HIP_OK(dalloc(h->dUbar, n));
+        HIP_OK(hipMemcpyAsync(h->dUbar, Ubar, sizeof(double) * n, hipMemcpyHostToDevice, h->stream));
+    }
+    rc = residual_bc(h, it, h->dU, h->dFint, kbuf(h), Fext ? h->dFext : nullptr, Ubar ? h->dUbar : nullptr, h->dR,
+                     norm_type, h->dnorm, h->stream);
+    if (rc) return rc;
+    h->k_bc = true;
+    double nrm[3];
+    HIP_OK(hipMemcpyAsync(nrm, h->dnorm, sizeof(nrm), hipMemcpyDeviceToHost, h->stream));
+    if (R) HIP_OK(hipMemcpyAsync(R, h->dR, sizeof(double) * n, hipMemcpyDeviceToHost, h->stream));
+    HIP_OK(hipStreamSynchronize(h->stream));
+    if (normR) *normR = nrm[2];
+    return CEOS_OK;
+}
+
+int ceos_set_solver_precision(ceos_handle* h, int mode) {
+    if (!h || (mode != -1 && mode != CEOS_K_FP64 && mode != CEOS_K_FP32_STORED)) return CEOS_ERR_ARG;
+    h->solver_precision = mode;
+    return CEOS_OK;
+}
+
+int ceos_linear_solve_device(ceos_handle* h, const void* dK, const double* db, double* dx, double rtol, int maxit,
+                             int* iters, double* relres, int* flag, void* stream) {
+    if (!h) return CEOS_ERR_ARG;
+    hipStream_t s = stream ? (hipStream_t)stream : h->stream;
+    return linear_solve(h, dK, db, dx, rtol, maxit, s, iters, relres, flag);
+}
+
+int ceos_linear_solve(ceos_handle* h, const double* b, double* x, double rtol, int maxit, int* iters,
+                      double* relres, int* flag) {
+    if (!h || !b || !x || !h->k_valid) return CEOS_ERR_ARG;
+    const long n = 3L * h->nn;
+    if (!h->dcg_b) {
+        HIP_OK(dalloc(h->dcg_b, n));
+        HIP_OK(dalloc(h->dcg_x, n));
+    }
+    HIP_OK(hipMemcpyAsync(h->dcg_b, b, sizeof(double) * n, hipMemcpyHostToDevice, h->stream));
+    int rc = linear_solve(h, kbuf(h), h->dcg_b, h->dcg_x, rtol, maxit, h->stream, iters, relres, flag);
+    if (rc) return rc;
+    HIP_OK(hipMemcpyAsync(x, h->dcg_x, sizeof(double) * n, hipMemcpyDeviceToHost, h->stream));
+    HIP_OK(hipStreamSynchronize(h->stream));
+    return CEOS_OK;
+}
+
+int ceos_newton_update_device(ceos_handle* h, double* dU, const double* dDX, double scale, void* stream) {
+    if (!h || !dU || !dDX) return CEOS_ERR_ARG;
+    hipStream_t s = stream ? (hipStream_t)stream : h->stream;
+    const long n = 3L * h->nn;
+    hipLaunchKernelGGL(k_newton_update, dim3((unsigned)((n + kSysBlock - 1) / kSysBlock)), dim3(kSysBlock), 0, s, n,
+                       scale, dDX, dU, (const double*)h->dX, h->nl ? h->dx : nullptr);
+    HIP_OK(hipGetLastError());
+    h->f_valid = h->k_valid = false;
+    return CEOS_OK;
 }
 
 }  // extern "C"

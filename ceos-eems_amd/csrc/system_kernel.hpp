@@ -1,0 +1,458 @@
+// The macro Newton step around the element path: Dirichlet boundary conditions on R and K, the
+// residual norm, and a block-Jacobi preconditioned conjugate-gradient solve of K DX = -R.
+//
+//   residual + BC   EvaluateR / EvaluateKt (FEM/Modules/ModFEMSystemOfEquations.f90:40-93) with
+//                   ApplyBoundaryConditionsNEW (FEM/BoundaryConditions/ModBoundaryConditions.f90:246-340):
+//                   R = Fint - Fext; at it = 0, R += K (Ubar - U) over the prescribed DOFs (the
+//                   product uses K before its rows are cleared); R(prescribed) = -(Ubar - U) at
+//                   it = 0 and 0 afterwards; R(fixed) = 0; rows and columns of constrained DOFs
+//                   of K cleared, their diagonal set to 1.
+//   constrained-entry map: AllocatePrescDispSparseMapping / AllocateFixedSupportSparseMapping
+//                   (ModBoundaryConditions.f90:344-536) search all nnz entries for every
+//                   constrained DOF; here a per-DOF mask is tested while each row is streamed.
+//   norm            NewtonRaphsonFull_Solve (Tools/MathTools/NonlinearSolver/ModNewtonRaphsonFull.f90:109-117):
+//                   1 = L2, 2 = max |R|.
+//   linear solve    the PARDISO call of the Newton step (Tools/MathTools/LinearSolver/ModPardisoSolver.f90:343-380)
+//                   is replaced by PCG on the device.
+//
+// Symmetric matrix access.  K is stored as its upper CSR triangle in node blocks (rows 3A..3A+2:
+// the diagonal block's upper part, then one 3x3 block per upper neighbour B > A, in rank order).
+// For every node the neighbour slots (all B adjacent to A, ascending, A itself included) are
+// described by `src` words: (owner << 7) | rank, owner = the node whose rows store the block
+// (A for B >= A, B for B < A, stored transposed), rank 0 = the diagonal block, rank q >= 1 the
+// q-th upper block of the owner; kSlotPad marks an unused slot.
+//
+// ELL copy for the PCG: per node `width` slots, slot-major: value (j*9 + 3a + b)*nn + A holds
+// entry (3A + a, 3 nbr + b) of slot j, so lanes of a wavefront (consecutive nodes) read
+// consecutive addresses for every (slot, entry).  Built once per Newton iteration from the
+// BC-applied CSR, FP64 or FP32 (vectors, products and dot products stay FP64).
+#pragma once
+#include <stdint.h>
+
+namespace ceos {
+
+constexpr uint8_t kBcPresc = 1, kBcFixed = 2;
+constexpr uint32_t kSlotPad = 0xffffffffu;
+constexpr int kSysBlock = 256;
+
+struct SysGeom {
+    int nn;
+    int width;                 // neighbour slots per node
+    const int* rowstart;       // [nn] CSR offset of row 3A
+    const int* nup;            // [nn] upper neighbours of A
+    const int* up_ptr;         // [nn + 1]
+    const int* up_nodes;       // upper neighbours, ascending per node
+    const int* slot_nbr;       // [width][nn] neighbour node of every slot (A for padding)
+    const uint32_t* slot_src;  // [width][nn]
+    const uint8_t* dof_mask;   // [3 nn] kBcPresc | kBcFixed
+    const uint8_t* node_mask;  // [nn] OR of the node's three DOF masks
+};
+
+// block (rows: DOFs of A, columns: DOFs of the slot's neighbour) of the symmetric matrix
+template <typename KT>
+__device__ inline void csr_block(const KT* __restrict__ K, const SysGeom& g, int A, uint32_t src, double b[9]) {
+    const int owner = (int)(src >> 7), q = (int)(src & 127u);
+    const long r0 = g.rowstart[owner];
+    const long nu = g.nup[owner];
+    const long r1 = r0 + 3 + 3 * nu, r2 = r1 + 2 + 3 * nu;
+    if (q == 0) {
+        b[0] = (double)K[r0]; b[1] = (double)K[r0 + 1]; b[2] = (double)K[r0 + 2];
+        b[4] = (double)K[r1]; b[5] = (double)K[r1 + 1];
+        b[8] = (double)K[r2];
+        b[3] = b[1]; b[6] = b[2]; b[7] = b[5];
+        return;
+    }
+    const long o = 3L * (q - 1);
+    double t[9];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        t[k] = (double)K[r0 + 3 + o + k];
+        t[3 + k] = (double)K[r1 + 2 + o + k];
+        t[6 + k] = (double)K[r2 + 1 + o + k];
+    }
+    if (owner == A) {
+#pragma unroll
+        for (int k = 0; k < 9; ++k) b[k] = t[k];
+    } else {
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int k = 0; k < 3; ++k) b[3 * i + k] = t[3 * k + i];
+    }
+}
+
+// block-level reduction of (sum, max) pairs in a fixed order; thread 0 writes part[blockIdx]
+__device__ inline void block_sum_max(double s, double m, double* part_s, double* part_m) {
+    __shared__ double ls[kSysBlock], lm[kSysBlock];
+    ls[threadIdx.x] = s;
+    lm[threadIdx.x] = m;
+    __syncthreads();
+    for (int w = kSysBlock / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            ls[threadIdx.x] += ls[threadIdx.x + w];
+            lm[threadIdx.x] = fmax(lm[threadIdx.x], lm[threadIdx.x + w]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        if (part_s) part_s[blockIdx.x] = ls[0];
+        if (part_m) part_m[blockIdx.x] = lm[0];
+    }
+}
+
+// block-level reduction of two sums in a fixed order; thread 0 writes pa/pb[blockIdx]
+__device__ inline void block_sum2(double a, double b, double* pa, double* pb) {
+    __shared__ double la[kSysBlock], lb[kSysBlock];
+    la[threadIdx.x] = a;
+    lb[threadIdx.x] = b;
+    __syncthreads();
+    for (int w = kSysBlock / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            la[threadIdx.x] += la[threadIdx.x + w];
+            lb[threadIdx.x] += lb[threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        pa[blockIdx.x] = la[0];
+        pb[blockIdx.x] = lb[0];
+    }
+}
+
+// sums (and maxima) of n partials by one workgroup, fixed order: thread t takes t, t+256, ...
+__device__ inline void reduce_partials(const double* part_s, const double* part_m, int n, double* out_s,
+                                       double* out_m) {
+    __shared__ double ls[kSysBlock], lm[kSysBlock];
+    double s = 0.0, m = 0.0;
+    for (int i = threadIdx.x; i < n; i += kSysBlock) {
+        if (part_s) s += part_s[i];
+        if (part_m) m = fmax(m, part_m[i]);
+    }
+    ls[threadIdx.x] = s;
+    lm[threadIdx.x] = m;
+    __syncthreads();
+    for (int w = kSysBlock / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            ls[threadIdx.x] += ls[threadIdx.x + w];
+            lm[threadIdx.x] = fmax(lm[threadIdx.x], lm[threadIdx.x + w]);
+        }
+        __syncthreads();
+    }
+    if (out_s) *out_s = ls[0];
+    if (out_m) *out_m = lm[0];
+}
+
+// R of the Newton step with the boundary conditions, one thread per node; partial sums of R^2
+// and maxima of |R| per workgroup.  rmod != 0: add K (Ubar - U) over the prescribed DOFs.
+template <typename KT>
+__global__ __launch_bounds__(kSysBlock) void k_bc_residual(SysGeom g, const KT* __restrict__ K, int rmod,
+                                                           const double* __restrict__ Fint,
+                                                           const double* __restrict__ Fext,
+                                                           const double* __restrict__ Ubar,
+                                                           const double* __restrict__ U, double* __restrict__ R,
+                                                           double* part_s, double* part_m) {
+    const int A = blockIdx.x * kSysBlock + threadIdx.x;
+    double ss = 0.0, mm = 0.0;
+    if (A < g.nn) {
+        double r[3];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) r[i] = Fint[3L * A + i] - (Fext ? Fext[3L * A + i] : 0.0);
+        if (rmod) {
+            for (int j = 0; j < g.width; ++j) {
+                const uint32_t src = g.slot_src[(long)j * g.nn + A];
+                if (src == kSlotPad) continue;
+                const int B = g.slot_nbr[(long)j * g.nn + A];
+                if (!(g.node_mask[B] & kBcPresc)) continue;
+                double ud[3];
+#pragma unroll
+                for (int k = 0; k < 3; ++k)
+                    ud[k] = (g.dof_mask[3L * B + k] & kBcPresc) ? Ubar[3L * B + k] - U[3L * B + k] : 0.0;
+                double b[9];
+                csr_block(K, g, A, src, b);
+#pragma unroll
+                for (int i = 0; i < 3; ++i) r[i] += b[3 * i] * ud[0] + b[3 * i + 1] * ud[1] + b[3 * i + 2] * ud[2];
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            const uint8_t m = g.dof_mask[3L * A + i];
+            if (m & kBcFixed) r[i] = 0.0;
+            else if (m & kBcPresc) r[i] = rmod ? -(Ubar[3L * A + i] - U[3L * A + i]) : 0.0;
+            R[3L * A + i] = r[i];
+            ss += r[i] * r[i];
+            mm = fmax(mm, fabs(r[i]));
+        }
+    }
+    block_sum_max(ss, mm, part_s, part_m);
+}
+
+// norm of R: out[0] = L2, out[1] = max |R|, out[2] = out[norm_type - 1]
+__global__ __launch_bounds__(kSysBlock) void k_norm_final(const double* part_s, const double* part_m, int n,
+                                                          int norm_type, double* out) {
+    double s, m;
+    reduce_partials(part_s, part_m, n, &s, &m);
+    if (threadIdx.x == 0) {
+        out[0] = sqrt(s);
+        out[1] = m;
+        out[2] = norm_type == 1 ? out[0] : out[1];
+    }
+}
+
+// rows and columns of constrained DOFs of the upper CSR: 0, diagonal 1; one thread per node
+template <typename KT>
+__global__ __launch_bounds__(kSysBlock) void k_bc_matrix(SysGeom g, KT* __restrict__ K) {
+    const int A = blockIdx.x * kSysBlock + threadIdx.x;
+    if (A >= g.nn) return;
+    const long r0 = g.rowstart[A];
+    const long nu = g.nup[A];
+    const long rs[3] = {r0, r0 + 3 + 3 * nu, r0 + 3 + 3 * nu + 2 + 3 * nu};
+    const uint8_t ma[3] = {g.dof_mask[3L * A], g.dof_mask[3L * A + 1], g.dof_mask[3L * A + 2]};
+    // diagonal block: row i, columns i..2
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+        for (int k = i; k < 3; ++k)
+            if (ma[i] | ma[k]) K[rs[i] + (k - i)] = (KT)((k == i) ? 1.0 : 0.0);
+    for (int q = 0; q < nu; ++q) {
+        const int B = g.up_nodes[g.up_ptr[A] + q];
+        const uint8_t nb = g.node_mask[B];
+        if (!(nb | ma[0] | ma[1] | ma[2])) continue;
+        uint8_t mb[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) mb[k] = g.dof_mask[3L * B + k];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            const long base = rs[i] + (3 - i) + 3L * q;
+#pragma unroll
+            for (int k = 0; k < 3; ++k)
+                if (ma[i] | mb[k]) K[base + k] = (KT)0.0;
+        }
+    }
+}
+
+// ELL copy of the symmetric matrix, one thread per (slot, node)
+template <typename KT, typename VT>
+__global__ __launch_bounds__(kSysBlock) void k_build_ell(SysGeom g, const KT* __restrict__ K, VT* __restrict__ val) {
+    const long t = (long)blockIdx.x * kSysBlock + threadIdx.x;
+    if (t >= (long)g.width * g.nn) return;
+    const int j = (int)(t / g.nn), A = (int)(t - (long)j * g.nn);
+    const uint32_t src = g.slot_src[t];
+    double b[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    if (src != kSlotPad) csr_block(K, g, A, src, b);
+    const long nn = g.nn;
+#pragma unroll
+    for (int m = 0; m < 9; ++m) val[((long)j * 9 + m) * nn + A] = (VT)b[m];
+}
+
+// inverse of every node's diagonal block (symmetric, stored 00 01 02 11 12 22 as [6][nn]);
+// a singular block is replaced by the identity
+template <typename KT>
+__global__ __launch_bounds__(kSysBlock) void k_block_jacobi(SysGeom g, const KT* __restrict__ K, double* __restrict__ Minv) {
+    const int A = blockIdx.x * kSysBlock + threadIdx.x;
+    if (A >= g.nn) return;
+    double b[9];
+    csr_block(K, g, A, (uint32_t)A << 7, b);
+    const double c00 = b[4] * b[8] - b[5] * b[7];
+    const double c01 = b[2] * b[7] - b[1] * b[8];
+    const double c02 = b[1] * b[5] - b[2] * b[4];
+    const double det = b[0] * c00 + b[3] * c01 + b[6] * c02;
+    const long nn = g.nn;
+    if (!(fabs(det) > 0.0) || !isfinite(det)) {
+        Minv[A] = 1.0; Minv[nn + A] = 0.0; Minv[2 * nn + A] = 0.0;
+        Minv[3 * nn + A] = 1.0; Minv[4 * nn + A] = 0.0; Minv[5 * nn + A] = 1.0;
+        return;
+    }
+    const double id = 1.0 / det;
+    Minv[A] = c00 * id;
+    Minv[nn + A] = c01 * id;
+    Minv[2 * nn + A] = c02 * id;
+    Minv[3 * nn + A] = (b[0] * b[8] - b[2] * b[6]) * id;
+    Minv[4 * nn + A] = (b[2] * b[3] - b[0] * b[5]) * id;
+    Minv[5 * nn + A] = (b[0] * b[4] - b[1] * b[3]) * id;
+}
+
+// PCG scalars on the device
+struct CgScalars {
+    double rz;       // r . z of the current iterate
+    double alpha, beta;
+    double rr, bb;   // r . r, b . b
+    double tol2;     // rtol^2
+    int done;        // 1 = converged, 2 = breakdown (p.Kp <= 0), 3 = iteration limit
+    int iters;
+    int maxit;
+    int pad;
+};
+
+__device__ inline void precond(const double* __restrict__ Minv, long nn, int A, const double r[3], double z[3]) {
+    const double m00 = Minv[A], m01 = Minv[nn + A], m02 = Minv[2 * nn + A];
+    const double m11 = Minv[3 * nn + A], m12 = Minv[4 * nn + A], m22 = Minv[5 * nn + A];
+    z[0] = m00 * r[0] + m01 * r[1] + m02 * r[2];
+    z[1] = m01 * r[0] + m11 * r[1] + m12 * r[2];
+    z[2] = m02 * r[0] + m12 * r[1] + m22 * r[2];
+}
+
+// x = 0, r = b, z = M^-1 b, p_prev = 0; partials of r.z and r.r
+__global__ __launch_bounds__(kSysBlock) void k_cg_init(int nn, const double* __restrict__ Minv, const double* __restrict__ b,
+                                                       double* x, double* r, double* z, double* p_prev,
+                                                       double* part_s, double* part_m) {
+    const int A = blockIdx.x * kSysBlock + threadIdx.x;
+    double rz = 0.0, rr = 0.0;
+    if (A < nn) {
+        double rv[3], zv[3];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) rv[i] = b[3L * A + i];
+        precond(Minv, nn, A, rv, zv);
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            x[3L * A + i] = 0.0;
+            r[3L * A + i] = rv[i];
+            z[3L * A + i] = zv[i];
+            p_prev[3L * A + i] = 0.0;
+            rz += rv[i] * zv[i];
+            rr += rv[i] * rv[i];
+        }
+    }
+    block_sum2(rz, rr, part_s, part_m);
+}
+
+__global__ __launch_bounds__(kSysBlock) void k_cg_init_final(const double* part_s, const double* part_m, int n,
+                                                             CgScalars* c) {
+    __shared__ double ls[kSysBlock], lr[kSysBlock];
+    double s = 0.0, q = 0.0;
+    for (int i = threadIdx.x; i < n; i += kSysBlock) {
+        s += part_s[i];
+        q += part_m[i];
+    }
+    ls[threadIdx.x] = s;
+    lr[threadIdx.x] = q;
+    __syncthreads();
+    for (int w = kSysBlock / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            ls[threadIdx.x] += ls[threadIdx.x + w];
+            lr[threadIdx.x] += lr[threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        c->rz = ls[0];
+        c->rr = c->bb = lr[0];
+        c->beta = 0.0;
+        c->alpha = 0.0;
+        c->iters = 0;
+        c->done = (lr[0] == 0.0) ? 1 : 0;
+        if (!c->done && c->maxit <= 0) c->done = 3;
+    }
+}
+
+// p = z + beta p_prev (written for the node's own rows, recomputed on the fly for neighbours),
+// q = K p; partial sums of p.q
+template <typename VT>
+__global__ __launch_bounds__(kSysBlock) void k_cg_spmv(SysGeom g, const VT* __restrict__ val, const CgScalars* c,
+                                                       const double* __restrict__ z, const double* __restrict__ p_prev,
+                                                       double* __restrict__ p, double* __restrict__ q, double* part_s) {
+    if (c->done) return;
+    const int A = blockIdx.x * kSysBlock + threadIdx.x;
+    const double beta = c->beta;
+    double pq = 0.0;
+    if (A < g.nn) {
+        const long nn = g.nn;
+        double acc[3] = {0.0, 0.0, 0.0};
+        for (int j = 0; j < g.width; ++j) {
+            const int B = g.slot_nbr[(long)j * nn + A];
+            double pb[3];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) pb[k] = z[3L * B + k] + beta * p_prev[3L * B + k];
+            const VT* v = val + (long)j * 9 * nn + A;
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+                acc[i] += (double)v[(3 * i) * nn] * pb[0] + (double)v[(3 * i + 1) * nn] * pb[1] +
+                          (double)v[(3 * i + 2) * nn] * pb[2];
+        }
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            const double pa = z[3L * A + i] + beta * p_prev[3L * A + i];
+            p[3L * A + i] = pa;
+            q[3L * A + i] = acc[i];
+            pq += pa * acc[i];
+        }
+    }
+    block_sum_max(pq, 0.0, part_s, nullptr);
+}
+
+__global__ __launch_bounds__(kSysBlock) void k_cg_alpha(const double* part_s, int n, CgScalars* c) {
+    if (c->done) return;
+    double pq;
+    reduce_partials(part_s, nullptr, n, &pq, nullptr);
+    if (threadIdx.x == 0) {
+        if (!(pq > 0.0)) c->done = 2;
+        else c->alpha = c->rz / pq;
+    }
+}
+
+// x += alpha p, r -= alpha q, z = M^-1 r; partials of r.z and r.r
+__global__ __launch_bounds__(kSysBlock) void k_cg_update(int nn, const double* __restrict__ Minv, const CgScalars* c,
+                                                         const double* __restrict__ p, const double* __restrict__ q,
+                                                         double* x, double* r, double* z, double* part_s,
+                                                         double* part_m) {
+    if (c->done) return;
+    const int A = blockIdx.x * kSysBlock + threadIdx.x;
+    const double alpha = c->alpha;
+    double rz = 0.0, rr = 0.0;
+    if (A < nn) {
+        double rv[3], zv[3];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            x[3L * A + i] += alpha * p[3L * A + i];
+            rv[i] = r[3L * A + i] - alpha * q[3L * A + i];
+            r[3L * A + i] = rv[i];
+        }
+        precond(Minv, nn, A, rv, zv);
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            z[3L * A + i] = zv[i];
+            rz += rv[i] * zv[i];
+            rr += rv[i] * rv[i];
+        }
+    }
+    block_sum2(rz, rr, part_s, part_m);
+}
+
+__global__ __launch_bounds__(kSysBlock) void k_cg_beta(const double* part_s, const double* part_m, int n, CgScalars* c) {
+    if (c->done) return;
+    __shared__ double ls[kSysBlock], lr[kSysBlock];
+    double s = 0.0, q = 0.0;
+    for (int i = threadIdx.x; i < n; i += kSysBlock) {
+        s += part_s[i];
+        q += part_m[i];
+    }
+    ls[threadIdx.x] = s;
+    lr[threadIdx.x] = q;
+    __syncthreads();
+    for (int w = kSysBlock / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            ls[threadIdx.x] += ls[threadIdx.x + w];
+            lr[threadIdx.x] += lr[threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const double rz_new = ls[0];
+        c->beta = rz_new / c->rz;
+        c->rz = rz_new;
+        c->rr = lr[0];
+        c->iters += 1;
+        if (c->rr <= c->tol2 * c->bb) c->done = 1;
+        else if (c->iters >= c->maxit) c->done = 3;
+    }
+}
+
+// u += s d; if x != null, x = X + u (UpdateMeshCoordinates)
+__global__ __launch_bounds__(kSysBlock) void k_newton_update(long n, double s, const double* __restrict__ d, double* u,
+                                                             const double* __restrict__ X, double* x) {
+    const long t = (long)blockIdx.x * kSysBlock + threadIdx.x;
+    if (t >= n) return;
+    const double v = u[t] + s * d[t];
+    u[t] = v;
+    if (x) x[t] = X[t] + v;
+}
+
+}  // namespace ceos

@@ -2,8 +2,9 @@
  * ceos_gpu.h -- C ABI of the MI355X element hot path of CEOS (libceos_gpu.so).
  *
  * Drop-in for the reference's element layer (paths relative to SourceCode/ of the reference):
- * a Fortran caller binds these entry points with ISO_C_BINDING and keeps its Newton loop,
- * Dirichlet handling and PARDISO solve.  All host arrays are caller-owned, contiguous and use
+ * a Fortran caller binds these entry points with ISO_C_BINDING and keeps its Newton loop; its
+ * Dirichlet handling and PARDISO solve either stay on the host or move to the device too
+ * (ceos_bc_set, ceos_residual_bc, ceos_linear_solve).  All host arrays are caller-owned, contiguous and use
  * the reference's layouts: nodal vectors DOF = 3*(node-1)+k (k = 1,2,3), element connectivity
  * node-major with 1-based node ids, CSR 1-based with sorted columns and RowMap(n+1) = nnz+1.
  *
@@ -200,6 +201,58 @@ int ceos_set_external_gp(ceos_handle *h, const double *dext);
  * (NULL = the handle's stream), or host arrays (synchronous). */
 int ceos_gp_deformation_gradient_device(ceos_handle *h, const double *dU, double *dF, void *stream);
 int ceos_gp_deformation_gradient(ceos_handle *h, const double *U, double *F);
+
+/* ------------------------------------------------------------------------------------------
+ * The macro Newton step around the element path: Dirichlet boundary conditions, residual norm
+ * and the linear solve, for callers that move their Newton loop's data to the device.
+ *
+ * ceos_bc_set: constrained DOFs of the load case (1-based, dof = 3 (node - 1) + k), prescribed
+ * displacements (ClassBoundaryConditions' Presc_Disp_DOF) and fixed supports (FixedSupport%dof).
+ * Replaces AllocatePrescDispSparseMapping / AllocateFixedSupportSparseMapping
+ * (FEM/BoundaryConditions/ModBoundaryConditions.f90:344-536), which search all nnz entries for
+ * every constrained DOF: here a per-DOF mask, O(ndof) to set, is tested as rows are streamed. */
+int ceos_bc_set(ceos_handle *h, int n_presc, const int *presc_dof, int n_fixed, const int *fixed_dof);
+
+/* EvaluateR + EvaluateKt of Newton iteration `it` (FEM/Modules/ModFEMSystemOfEquations.f90:40-93)
+ * with ApplyBoundaryConditionsNEW (ModBoundaryConditions.f90:246-340) and the residual norm of
+ * NewtonRaphsonFull_Solve (Tools/MathTools/NonlinearSolver/ModNewtonRaphsonFull.f90:109-117),
+ * after ceos_solve_constitutive at U:
+ *   R = Fint - Fext; at it = 0, R += K (Ubar - U) over the prescribed DOFs;
+ *   R(prescribed) = -(Ubar - U) at it = 0 and 0 afterwards; R(fixed) = 0;
+ *   K (kept in the handle; ceos_tangent returns it afterwards, as Kg after EvaluateKt): rows and
+ *   columns of constrained DOFs cleared, diagonal 1.  Like EvaluateKt, which assembles a fresh
+ *   tangent every call, a second call at the same state re-evaluates K before the product.
+ *   The device call works on the caller's dK as it is: it must come fresh from
+ *   ceos_evaluate_device.
+ * norm_type 1 = L2, 2 = max |R| (the reference's default).  Fext (NULL = 0), Ubar and R (NULL
+ * = not returned) are 3 nn host arrays.
+ * _device: the same on device arrays of a ceos_evaluate_device call (dK NULL = residual only);
+ * dnorm receives {L2, max |R|, the selected one}.  Asynchronous on `stream`. */
+int ceos_residual_bc(ceos_handle *h, int it, const double *Fext, const double *Ubar, double *R, int norm_type,
+                     double *normR);
+int ceos_residual_bc_device(ceos_handle *h, int it, const double *dU, const double *dFint, void *dK,
+                            const double *dFext, const double *dUbar, double *dR, int norm_type, double *dnorm,
+                            void *stream);
+
+/* Linear solve of the Newton step, K x = b, replacing the PARDISO call
+ * (Tools/MathTools/LinearSolver/ModPardisoSolver.f90:343-380, analysis + factorization + solve
+ * every iteration): block-Jacobi (3x3 node blocks) preconditioned CG from x = 0 on the device,
+ * stopped at ||r|| <= rtol ||b|| or after maxit iterations.  K is the handle's tangent after
+ * ceos_residual_bc (host call) or the caller's device K (device call), copied once per solve
+ * into a node-neighbour (ELL) layout of the full symmetric matrix.  Vectors, products and dot
+ * products are FP64; the ELL values follow ceos_set_solver_precision (-1: the handle's
+ * precision, or CEOS_K_FP64 / CEOS_K_FP32_STORED).  Outputs: iterations, recursive relative
+ * residual, flag (1 converged, 2 breakdown p.Kp <= 0, 3 iteration limit).  Both calls
+ * synchronise the stream every 32 iterations to read the state. */
+int ceos_set_solver_precision(ceos_handle *h, int mode);
+int ceos_linear_solve(ceos_handle *h, const double *b, double *x, double rtol, int maxit, int *iters,
+                      double *relres, int *flag);
+int ceos_linear_solve_device(ceos_handle *h, const void *dK, const double *db, double *dx, double rtol, int maxit,
+                             int *iters, double *relres, int *flag, void *stream);
+
+/* X = X + scale DX and PostUpdate (FEMUpdateMesh: x = X0 + U when the analysis is nonlinear,
+ * FEM/Modules/ModFEMSystemOfEquations.f90:99-106) on device arrays. */
+int ceos_newton_update_device(ceos_handle *h, double *dU, const double *dDX, double scale, void *stream);
 
 /* ------------------------------------------------------------------------------------------
  * Batched periodic RVEs (FE^2 micro problems).

@@ -13,6 +13,9 @@ expanded from the upper CSR.
 
 A backend provides: ndof, csr() -> (rowmap, col) 1-based; solve_constitutive(time, U) -> status;
 internal_force() -> (Fint, status); tangent() -> Kval; update_mesh(U); switch_converged_state().
+A backend may also provide the Newton system itself: newton_system(it, fext, bc, Ubar, X) ->
+(R with the boundary conditions, norm) and newton_solve(b) -> K^-1 b with K after the boundary
+conditions (the library's device BC, norm and PCG); the loop then calls those instead.
 """
 import math
 
@@ -110,17 +113,21 @@ def newton(backend, time, fext, bc, Ubar, Xguess, nl, tol, maxiter, norms_out=No
     X = Xguess.copy()
     it = 0
     normR0 = None
+    system = getattr(backend, "newton_system", None)
     while True:
         st = backend.solve_constitutive(time, X)
         if st:
             raise NewtonFailed("evaluate system status %d" % st)
-        Fint, st = backend.internal_force()
-        if st:
-            raise NewtonFailed("internal force status %d" % st)
-        R = Fint - fext
-        K = backend.tangent()
-        Kb, R = apply_dirichlet(rowmap, col, K, R, bc, Ubar, X, it)
-        normR = float(np.max(np.abs(R)))
+        if system is not None:
+            R, normR = system(it, fext, bc, Ubar, X)
+        else:
+            Fint, st = backend.internal_force()
+            if st:
+                raise NewtonFailed("internal force status %d" % st)
+            R = Fint - fext
+            K = backend.tangent()
+            Kb, R = apply_dirichlet(rowmap, col, K, R, bc, Ubar, X, it)
+            normR = float(np.max(np.abs(R)))
         if norms_out is not None:
             norms_out.append(normR)
         if it == 0:
@@ -130,8 +137,11 @@ def newton(backend, time, fext, bc, Ubar, Xguess, nl, tol, maxiter, norms_out=No
         if it >= maxiter:
             raise NewtonFailed("maximum number of iterations")
         it += 1
-        A = sym_from_upper(rowmap, col, Kb, n)
-        DX = spla.spsolve(A, -R)
+        if system is not None:
+            DX = backend.newton_solve(-R)
+        else:
+            A = sym_from_upper(rowmap, col, Kb, n)
+            DX = spla.spsolve(A, -R)
         X = X + DX
         if nl:
             backend.update_mesh(X)

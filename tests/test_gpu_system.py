@@ -1,0 +1,257 @@
+"""The macro Newton step on the device (ceos_bc_set, ceos_residual_bc(_device),
+ceos_linear_solve(_device), ceos_newton_update_device) against the test driver's restatement of
+the reference's boundary-condition and Newton code (tests/fe_driver.py: apply_dirichlet and the
+max-abs norm, pinned by the reference's own residual histories in test_oracle_golden.py) and
+against sparse direct solves.
+
+Bar: K after the boundary conditions bit-exact (cleared rows and columns, unit diagonal, the
+rest untouched); R bit-exact where no product enters (it > 0) and within 1e-12 norm-wise at
+it = 0 (the product K (Ubar - U) sums in another order); PCG solutions with ||K x - b|| <= rtol
+||b|| measured on the host in FP64; whole Newton histories through the device system with the
+iteration counts of the oracle path and converged displacements within 1e-9 relative.
+"""
+import numpy as np
+import pytest
+
+import fe_driver as fd
+import oracle_lib as ol
+from ceos_amd import gpu, hiprt, meshgen
+
+pytestmark = pytest.mark.gpu
+
+LAM = 100.0 - 2.0 / 3.0
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+class DeviceSystem:
+    """A CeosMesh whose Newton system (BC, norm, linear solve) runs in the library."""
+
+    def __init__(self, mesh, rtol=1e-12, norm_type=2):
+        self.m = mesh
+        self.ndof = mesh.ndof
+        self.rtol = rtol
+        self.norm_type = norm_type
+        self._bc = None
+        self.pcg = []
+
+    def __getattr__(self, k):
+        return getattr(self.m, k)
+
+    def newton_system(self, it, fext, bc, Ubar, X):
+        if self._bc is not bc:
+            self.m.set_bc(bc.presc_dof, bc.fixed)
+            self._bc = bc
+        return self.m.residual_bc(it, fext, Ubar, self.norm_type)
+
+    def newton_solve(self, b):
+        x, its, rr, flag = self.m.linear_solve(b, self.rtol, 50000)
+        assert flag == 1, (its, rr, flag)
+        self.pcg.append(its)
+        return x
+
+
+def _scrambled(n, seed):
+    X, conn = meshgen.cube(n, perturb=0.12, seed=seed)
+    rng = np.random.default_rng(seed)
+    perm = rng.permutation(len(X))
+    inv = np.empty_like(perm)
+    inv[perm] = np.arange(len(X))
+    return X[perm], (inv[conn - 1] + 1)[rng.permutation(len(conn))].astype(np.int32)
+
+
+def _bc_case(X, rng):
+    """x=0 face fixed, x=1 face ux prescribed, plus a few DOFs that are both, and loads."""
+    fixed, presc = meshgen.stretch_bc(X)
+    both = rng.choice(presc, size=2, replace=False)
+    fixed = np.concatenate([fixed, both])
+    ndof = 3 * len(X)
+    loads = [(int(d), (lambda t, v=v: v * t)) for d, v in zip(rng.choice(ndof, 5, replace=False), rng.normal(size=5))]
+    return fd.BoundaryConditions(ndof, fixed, [(int(d), (lambda t: 0.1 * t)) for d in presc], loads)
+
+
+@pytest.mark.parametrize("precision", [gpu.K_FP64, gpu.K_FP32_STORED])
+@pytest.mark.parametrize("it", [0, 1])
+def test_residual_and_matrix_bc_match_driver(precision, it):
+    rng = np.random.default_rng(3 + it)
+    X, conn = _scrambled(5, 7)
+    g = gpu.CeosMesh(X, conn, precision=precision)
+    g.set_material(1, ol.MODEL_CNH, [1.0, LAM])
+    bc = _bc_case(X, rng)
+    U = rng.normal(scale=0.01, size=g.ndof)
+    Ubar = U.copy()
+    Ubar[bc.presc_dof] += 0.05
+    fext = bc.fext(0.7)
+    g.update_mesh(U)
+    assert g.solve_constitutive(0.0, U) == 0
+    Fint, _ = g.internal_force()
+    K = g.tangent()
+    rm, col = g.csr()
+    Kb, Rb = fd.apply_dirichlet(rm, col, K, Fint - fext, bc, Ubar, U, it)
+    g.set_bc(bc.presc_dof, bc.fixed)
+    for norm_type in (2, 1):
+        R, nrm = g.residual_bc(it, fext, Ubar, norm_type)
+        if it == 0:
+            assert rel(R, Rb) < 1e-12
+        else:
+            assert np.array_equal(R, Rb)
+        ref = np.max(np.abs(Rb)) if norm_type == 2 else np.linalg.norm(Rb)
+        assert abs(nrm - ref) <= 1e-12 * ref
+    # the matrix carries the BC once (a second application changes nothing)
+    assert np.array_equal(g.tangent(), Kb)
+    assert np.any(Kb != K)
+
+
+def test_node_mask_skips_nothing_on_a_fully_constrained_node():
+    """Every DOF of a node constrained, one DOF of another: rows and columns of exactly those."""
+    X, conn = meshgen.cube(3, perturb=0.1, seed=2)
+    g = gpu.CeosMesh(X, conn)
+    g.set_material(1, ol.MODEL_NH, [0.5, 100.0])
+    U = np.random.default_rng(2).normal(scale=0.01, size=g.ndof)
+    g.update_mesh(U)
+    assert g.solve_constitutive(0.0, U) == 0
+    K = g.tangent()
+    rm, col = g.csr()
+    bc = fd.BoundaryConditions(g.ndof, [30, 31, 32, 7], [])
+    Kb, _ = fd.apply_dirichlet(rm, col, K, np.zeros(g.ndof), bc, U, U, 1)
+    g.set_bc([], [30, 31, 32, 7])
+    g.residual_bc(1)
+    assert np.array_equal(g.tangent(), Kb)
+
+
+@pytest.mark.parametrize("solver_precision", [gpu.K_FP64, gpu.K_FP32_STORED])
+def test_pcg_solves_the_bc_system(solver_precision):
+    X, conn = _scrambled(6, 11)
+    g = gpu.CeosMesh(X, conn)
+    g.set_material(1, ol.MODEL_CNH, [1.0, LAM])
+    rng = np.random.default_rng(5)
+    # smooth tension plus small noise: K stays positive definite (random strains of a few per
+    # cent would give pressures far above mu and an indefinite geometric stiffness)
+    U = np.stack([0.04 * X[:, 0], -0.01 * X[:, 1], -0.01 * X[:, 2]], axis=1).reshape(-1)
+    U = U + rng.normal(scale=1e-4, size=g.ndof)
+    g.update_mesh(U)
+    assert g.solve_constitutive(0.0, U) == 0
+    fixed, presc = meshgen.stretch_bc(X)
+    g.set_bc(presc, fixed)
+    g.residual_bc(1)
+    Kb = g.tangent()
+    rm, col = g.csr()
+    b = rng.normal(size=g.ndof)
+    g.set_solver_precision(solver_precision)
+    rtol = 1e-10 if solver_precision == gpu.K_FP64 else 1e-5
+    x, its, rr, flag = g.linear_solve(b, rtol, 20000)
+    res = fd.sym_upper_matvec(rm, col, Kb, x) - b
+    print("PCG %d iterations, recursive %.2e, true %.2e" % (its, rr, np.linalg.norm(res) / np.linalg.norm(b)))
+    assert flag == 1 and rr <= rtol
+    assert np.linalg.norm(res) <= 2 * rtol * np.linalg.norm(b)
+    xd = fd.spla.spsolve(fd.sym_from_upper(rm, col, Kb, g.ndof), b)
+    assert rel(x, xd) < (1e-7 if solver_precision == gpu.K_FP64 else 1e-2)
+    # iteration limit and zero right-hand side
+    _, its, _, flag = g.linear_solve(b, 1e-14, 3)
+    assert flag == 3 and its == 3
+    x0, its, _, flag = g.linear_solve(np.zeros(g.ndof), 1e-10, 100)
+    assert flag == 1 and its == 0 and not x0.any()
+
+
+def test_pcg_reports_an_indefinite_tangent():
+    """Random nodal strains of ~6 % under lambda/mu = 99: compressive pressures far above mu
+    make the geometric stiffness indefinite; CG stops with the breakdown flag, no NaN."""
+    X, conn = _scrambled(6, 11)
+    g = gpu.CeosMesh(X, conn)
+    g.set_material(1, ol.MODEL_CNH, [1.0, LAM])
+    rng = np.random.default_rng(5)
+    U = rng.normal(scale=0.01, size=g.ndof)
+    g.update_mesh(U)
+    assert g.solve_constitutive(0.0, U) == 0
+    fixed, presc = meshgen.stretch_bc(X)
+    g.set_bc(presc, fixed)
+    g.residual_bc(1)
+    rm, col = g.csr()
+    ev = np.linalg.eigvalsh(fd.sym_from_upper(rm, col, g.tangent(), g.ndof).toarray())
+    assert ev[0] < 0.0 < ev[-1]
+    x, its, rr, flag = g.linear_solve(rng.normal(size=g.ndof), 1e-10, 20000)
+    assert flag == 2 and np.all(np.isfinite(x))
+
+
+def test_golden_svk_case_through_device_system():
+    """The reference's StVenant Checker case (residual history of its log, Marc displacements)
+    with the boundary conditions, norms and linear solves of the library."""
+    from test_oracle_golden import _case, check_svk, run_svk_case
+    case = _case("svk_hex8_case.json")
+    g = gpu.CeosMesh(np.array(case["coords"]), np.array(case["conn"], dtype=np.int32))
+    g.set_material(1, ol.MODEL_SVK, [case["material"]["E"], case["material"]["nu"]])
+    ds = DeviceSystem(g, rtol=1e-13)
+    case, norms, probe = run_svk_case(ds)
+    check_svk(case, norms, probe)
+    print("PCG iterations per solve: max %d mean %.0f" % (max(ds.pcg), np.mean(ds.pcg)))
+
+
+def test_c1_newton_history_through_device_system_matches_oracle():
+    """10^3 cube, compressible Neo-Hookean, stretch 1.2 in 10 steps: the oracle with the driver's
+    BC and direct solves against the library with its own BC, norm and PCG."""
+    X, conn = meshgen.cube(10)
+    fixed, presc = meshgen.stretch_bc(X)
+    bc = fd.BoundaryConditions(3 * len(X), fixed, [(int(d), (lambda t: 0.2 * t)) for d in presc])
+    o = ol.OracleMesh(X, conn)
+    o.set_material(1, ol.MODEL_CNH, [1.0, LAM])
+    g = gpu.CeosMesh(X, conn)
+    g.set_material(1, ol.MODEL_CNH, [1.0, LAM])
+    ds = DeviceSystem(g, rtol=1e-12)
+    ho = fd.quasi_static(o, bc, [(0.0, 1.0, 10)], nl=True, tol=1e-8, maxiter=15)
+    hg = fd.quasi_static(ds, bc, [(0.0, 1.0, 10)], nl=True, tol=1e-8, maxiter=15)
+    assert [h[4] for h in hg] == [h[4] for h in ho]
+    for a, b in zip(hg, ho):
+        assert rel(a[3], b[3]) < 1e-9
+    print("PCG iterations per solve: max %d mean %.0f" % (max(ds.pcg), np.mean(ds.pcg)))
+
+
+def test_device_resident_newton_matches_host_calls():
+    """One load step by device arrays only (evaluate, BC + norm, PCG, update) against the same
+    step by the host calls: identical iterates."""
+    X, conn = meshgen.cube(6, perturb=0.1, seed=4)
+    fixed, presc = meshgen.stretch_bc(X)
+    ndof = 3 * len(X)
+    Ubar = np.zeros(ndof)
+    Ubar[presc] = 0.05
+
+    def mk():
+        m = gpu.CeosMesh(X, conn)
+        m.set_material(1, ol.MODEL_CNH, [1.0, LAM])
+        m.set_bc(presc, fixed)
+        return m
+
+    # host calls
+    h = mk()
+    U = np.zeros(ndof)
+    host = []
+    for it in range(4):
+        assert h.solve_constitutive(1.0, U) == 0
+        R, nrm = h.residual_bc(it, None, Ubar, 2)
+        x, its, _, flag = h.linear_solve(-R, 1e-12, 20000)
+        U = U + x
+        h.update_mesh(U)
+        host.append((nrm, U.copy()))
+    # device arrays
+    d = mk()
+    _, nnz = d.info()
+    D = hiprt.DeviceArray
+    dU, dF, dR, dx = D(ndof), D(ndof), D(ndof), D(ndof)
+    dK = D(nnz)
+    dUbar = D.from_numpy(Ubar)
+    dn = D(3)
+    dU.upload(np.zeros(ndof))
+    for it in range(4):
+        d.evaluate_device(1.0, dU.ptr, dF.ptr, dK.ptr)
+        d.residual_bc_device(it, dU.ptr, dF.ptr, dK.ptr, None, dUbar.ptr, dR.ptr, dn.ptr)
+        code, _ = d.device_status()
+        assert code == 0
+        d.linear_solve_device(dK.ptr, dR.ptr, dx.ptr, 1e-12, 20000)
+        d.newton_update_device(dU.ptr, dx.ptr, -1.0)
+        hiprt.synchronize()
+        nrm = dn.numpy()[2]
+        assert nrm == host[it][0], (it, nrm, host[it][0])
+        assert np.array_equal(dU.numpy(), host[it][1]), it
+    norms = [n for n, _ in host]
+    assert all(b < a for a, b in zip(norms, norms[1:])) and norms[-1] < 1e-5 * norms[0], norms
