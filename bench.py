@@ -219,6 +219,163 @@ def fe2_bench(no_cpu):
     return out
 
 
+def newton_step_device(mesh, X, ux, tol, rtol, maxit=15, stream=None):
+    """One load step of the stretch test run entirely on the device from U = 0: Newton with
+    the library's Dirichlet BC, max-abs residual norm and PCG (x=0 face fixed, u_x = ux on
+    x=1).  Returns per-iteration records (norm, PCG iterations, times in ms) and the total."""
+    from ceos_amd import gpu, hiprt, meshgen
+    fixed, presc = meshgen.stretch_bc(X)
+    mesh.set_bc(presc, fixed)
+    _, nnz = mesh.info()
+    ndof = 3 * len(X)
+    D = hiprt.DeviceArray
+    Ubar = np.zeros(ndof)
+    Ubar[presc] = ux
+    kdt = np.float64 if mesh.precision == gpu.K_FP64 else np.float32
+    dU, dF, dR, dx, dn = D(ndof), D(ndof), D(ndof), D(ndof), D(3)
+    dK = D(nnz, kdt)
+    dUbar = D.from_numpy(Ubar)
+    dU.upload(np.zeros(ndof))
+    s = stream or hiprt.Stream()
+    sp = s.ptr
+    mesh.update_mesh_device(dU.ptr, sp)
+    recs = []
+    norm0 = None
+    hiprt.synchronize()
+    t0 = time.perf_counter()
+    for it in range(maxit + 1):
+        ev = [hiprt.Event() for _ in range(4)]
+        ev[0].record(s)
+        mesh.evaluate_device(0.2, dU.ptr, dF.ptr, dK.ptr, sp)
+        ev[1].record(s)
+        mesh.residual_bc_device(it, dU.ptr, dF.ptr, dK.ptr, None, dUbar.ptr, dR.ptr, dn.ptr, 2, sp)
+        ev[2].record(s)
+        ev[2].synchronize()
+        code, _ = mesh.device_status()
+        if code != 0:
+            raise RuntimeError("evaluation status %d" % code)
+        nrm = float(dn.numpy()[2])
+        rec = {"it": it, "norm": nrm, "evaluate_ms": ev[0].elapsed_ms(ev[1]), "bc_norm_ms": ev[1].elapsed_ms(ev[2])}
+        norm0 = nrm if it == 0 else norm0
+        if it > 0 and nrm < tol * norm0:
+            recs.append(rec)
+            break
+        its, rr, flag = mesh.linear_solve_device(dK.ptr, dR.ptr, dx.ptr, rtol, 20000, sp)
+        ev[3].record(s)
+        mesh.newton_update_device(dU.ptr, dx.ptr, -1.0, sp)
+        ev[3].synchronize()
+        rec.update({"pcg_iterations": its, "pcg_relres": rr, "pcg_flag": flag, "solve_ms": ev[2].elapsed_ms(ev[3])})
+        recs.append(rec)
+        if flag != 1:
+            raise RuntimeError("PCG flag %d at Newton iteration %d" % (flag, it))
+    hiprt.synchronize()
+    total = time.perf_counter() - t0
+    return recs, total, dU.numpy()
+
+
+def newton_cpu_baseline(n_cells, ux, tol, threads):
+    """The reference's CPU path for the same load step: the CPU oracle (OpenMP) for the
+    evaluations, the driver's restatement of the reference's BC code and MKL PARDISO with the
+    reference's settings for every linear solve (tests/pardiso_lib.py)."""
+    sys.path.insert(0, os.path.join(REPO, "tests"))
+    import fe_driver as fd
+    import oracle_lib as ol
+    import pardiso_lib as pl
+    from ceos_amd import meshgen
+    if not pl.available():
+        return None
+    pl.load()
+    pl.set_threads(threads)
+    X, conn = meshgen.cube(n_cells, perturb=0.1)
+    o = ol.OracleMesh(X, conn, threads=threads)
+    o.set_material(1, ol.MODEL_CNH, [MU, LAM])
+    fixed, presc = meshgen.stretch_bc(X)
+    bc = fd.BoundaryConditions(3 * len(X), fixed, [(int(d), (lambda t: ux * t)) for d in presc])
+    rm, col = o.csr()
+    Ubar = np.zeros(3 * len(X))
+    Ubar[presc] = ux
+    U = np.zeros(3 * len(X))
+    o.update_mesh(U)
+    # a warm PARDISO call on a tiny system loads MKL and its threads outside the timed region
+    pl.solve(np.array([1, 2, 3], dtype=np.int32), np.array([1, 2, 2], dtype=np.int32), np.array([2.0, 1.0, 2.0]),
+             np.ones(2))
+    t0 = time.perf_counter()
+    t_solve, norms = 0.0, []
+    for it in range(16):
+        o.solve_constitutive(0.2, U)
+        F, _ = o.internal_force()
+        Kb, R = fd.apply_dirichlet(rm, col, o.tangent(), F, bc, Ubar, U, it)
+        norms.append(float(np.max(np.abs(R))))
+        if it > 0 and norms[-1] < tol * norms[0]:
+            break
+        ts = time.perf_counter()
+        dx, err, _ = pl.solve(rm, col, Kb, -R)
+        t_solve += time.perf_counter() - ts
+        U = U + dx
+        o.update_mesh(U)
+    total = time.perf_counter() - t0
+    return {"value": total, "unit": "s", "cores": threads, "kind": "reference",
+            "newton_iterations": len(norms) - 1, "linear_solve_s": t_solve,
+            "sample": "the same load step on the %d^3 cube (%d DOFs): CPU oracle evaluations (OpenMP), the "
+                      "reference's BC code restated in numpy, MKL PARDISO with the reference's settings "
+                      "(mtype -2, phase 13 + release per solve) on %d threads" % (n_cells, 3 * len(X), threads)}
+
+
+def newton_bench(no_cpu, n_cells=N_CELLS, n_small=40, ux=0.04, tol=1e-8, pcg_rtol=1e-6):
+    """Step 1 of the stretch test (u_x = 0.04 on x = 1, the first of 5 steps to stretch 1.2) solved on
+    the device from U = 0: evaluations, Dirichlet BC + max-abs norm, block-Jacobi PCG with the
+    symmetric tangent copied in FP32 (FP64 vectors and dots), until the residual drops below
+    tol times its first value (the reference's relative test)."""
+    from ceos_amd import gpu, meshgen
+    out = {"metric": "Newton load step s (80^3 hex8, device BC + norm + PCG)", "unit": "s",
+           "higher_is_better": False, "pcg_rtol": pcg_rtol, "newton_tol": tol}
+    res = {}
+    for n in (n_small, n_cells):
+        X, conn = meshgen.cube(n, perturb=0.1)
+        m = gpu.CeosMesh(X, conn)
+        m.set_material(1, gpu.MODEL_CNH, [MU, LAM])
+        m.set_solver_precision(gpu.K_FP32_STORED)
+        newton_step_device(m, X, ux, tol, pcg_rtol)  # first run: tables, ELL and PCG buffers
+        recs, total, _ = newton_step_device(m, X, ux, tol, pcg_rtol)
+        nn = len(X)
+        width = 27
+        res[n] = (recs, total, nn, width)
+        del m
+    recs, total, nn, width = res[n_cells]
+    pcg_its = sum(r.get("pcg_iterations", 0) for r in recs)
+    solve_ms = sum(r.get("solve_ms", 0.0) for r in recs)
+    # algorithmic bytes of one PCG iteration: the ELL values (FP32) and neighbour ids, the two
+    # gathered vectors read and p, q written by the product; p, q, x, r read, x, r, z written and
+    # the 3x3 block inverses read by the update
+    bytes_it = width * 9 * nn * 4 + width * nn * 4 + 4 * 24 * nn + 7 * 24 * nn + 48 * nn
+    ms_it = solve_ms / max(pcg_its, 1)
+    gbs = bytes_it / (ms_it * 1e-3) / 1e9
+    out.update({
+        "value": total,
+        "config": {"workload": "80^3 perturbed hex8 cube, compressible Neo-Hookean mu=1 K=100, x=0 fixed, "
+                               "u_x = 0.04 on x=1 (step 1 of 5 to stretch 1.2), Newton from U = 0",
+                   "dofs": 3 * nn},
+        "newton_iterations": len(recs) - 1,
+        "pcg_iterations": [r.get("pcg_iterations") for r in recs[:-1]],
+        "norms": [r["norm"] for r in recs],
+        "evaluate_ms": sum(r["evaluate_ms"] for r in recs),
+        "bc_norm_ms": sum(r["bc_norm_ms"] for r in recs),
+        "solve_ms": solve_ms,
+        "roofline_pcg": {"bound": "hbm", "kernel": "PCG iteration (k_cg_spmv + k_cg_update + 2 reductions)",
+                         "achieved": round(gbs, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                         "frac": round(gbs / PEAK_HBM_GBS, 4), "ms_per_iteration": round(ms_it, 4),
+                         "bytes_per_iteration": bytes_it},
+        "small": {"dofs": 3 * res[n_small][2], "value": res[n_small][1], "newton_iterations": len(res[n_small][0]) - 1,
+                  "pcg_iterations": [r.get("pcg_iterations") for r in res[n_small][0][:-1]]},
+    })
+    if not no_cpu:
+        cb = newton_cpu_baseline(n_small, ux, tol, min(os.cpu_count() or 1, 16))
+        if cb is not None:
+            out["cpu_baseline"] = cb
+            out["speedup_vs_cpu_baseline_same_size"] = cb["value"] / res[n_small][1]
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -226,6 +383,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fe2", action="store_true", help="skip the FE^2 macro-step metric")
+    ap.add_argument("--no-newton", action="store_true", help="skip the Newton load-step measurement")
+    ap.add_argument("--newton-only", action="store_true", help="only the Newton load-step measurement")
     ap.add_argument("--fe2-only", action="store_true",
                     help="only the FE^2 macro-step metric (prints its object alone; for profiling it)")
     ap.add_argument("--precision", choices=["fp64", "fp32k"], default="fp64")
@@ -246,6 +405,9 @@ def main():
 
     if args.fe2_only:
         print(json.dumps(fe2_bench(args.no_cpu)), flush=True)
+        return
+    if args.newton_only:
+        print(json.dumps(newton_bench(args.no_cpu)), flush=True)
         return
     t_setup = time.perf_counter()
     X, conn = meshgen.cube(N_CELLS, perturb=0.1)
@@ -354,8 +516,10 @@ def main():
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(X, conn, U)
         out["speedup_vs_cpu_baseline"] = value / out["cpu_baseline"]["value"]
+    del mesh, dU, dF, dK
+    if not args.no_newton:
+        out["newton"] = newton_bench(args.no_cpu)
     if not args.no_fe2:
-        del mesh, dU, dF, dK
         out["fe2"] = fe2_bench(args.no_cpu)
     print(json.dumps(out), flush=True)
 

@@ -152,17 +152,17 @@ struct ceos_handle {
     // Newton system: boundary conditions, residual norm, PCG (system_kernel.hpp)
     bool sys_ready = false;
     int sys_width = 0;
-    int *dup_ptr = nullptr, *dup_nodes = nullptr, *dslot_nbr = nullptr;
+    int *dup_ptr = nullptr, *dup_nodes = nullptr, *dslot_nbr = nullptr, *dsell_nbr = nullptr;
     uint32_t* dslot_src = nullptr;
     uint8_t *ddof_mask = nullptr, *dnode_mask = nullptr;
     int n_presc = 0, n_fixed = 0;
-    double *dR = nullptr, *dpart = nullptr, *dpart2 = nullptr, *dnorm = nullptr;
+    double *dR = nullptr, *dpart = nullptr, *dpart2 = nullptr, *dpart3 = nullptr, *dnorm = nullptr;
     double *dFext = nullptr, *dUbar = nullptr;  // staging of the host residual call
     int solver_precision = -1;                  // ELL values: -1 = the handle's precision
     void* dell = nullptr;
     bool ell_f32 = false;
     double *dMinv = nullptr, *dcg_r = nullptr, *dcg_z = nullptr, *dcg_q = nullptr;
-    double *dcg_p[2] = {nullptr, nullptr};
+    double* dcg_p = nullptr;
     double *dcg_b = nullptr, *dcg_x = nullptr;  // staging of the host solve call
     CgScalars* dcg = nullptr;
     int cg_check = 32;                          // iterations between host checks of the PCG state
@@ -256,13 +256,13 @@ static void free_device(ceos_handle* h) {
     dfree(h->dconn); dfree(h->dmat); dfree(h->drowstart); dfree(h->dnup); dfree(h->dperm);
     dfree(h->dpairs); dfree(h->ddiag_first); dfree(h->delem_err); dfree(h->derr); dfree(h->dhpart);
     dfree(h->dscratch); dfree(h->dblknode); dfree(h->dblkstart); dfree(h->dcptr); dfree(h->dcword);
-    dfree(h->dup_ptr); dfree(h->dup_nodes); dfree(h->dslot_nbr); dfree(h->dslot_src);
+    dfree(h->dup_ptr); dfree(h->dup_nodes); dfree(h->dslot_nbr); dfree(h->dslot_src); dfree(h->dsell_nbr);
     dfree(h->ddof_mask); dfree(h->dnode_mask); dfree(h->dR); dfree(h->dpart); dfree(h->dpart2);
     dfree(h->dnorm); dfree(h->dFext); dfree(h->dUbar);
     if (h->dell) (void)hipFree(h->dell);
     h->dell = nullptr;
-    dfree(h->dMinv); dfree(h->dcg_r); dfree(h->dcg_z); dfree(h->dcg_q); dfree(h->dcg_p[0]);
-    dfree(h->dcg_p[1]); dfree(h->dcg_b); dfree(h->dcg_x); dfree(h->dcg);
+    dfree(h->dMinv); dfree(h->dcg_r); dfree(h->dcg_z); dfree(h->dcg_q); dfree(h->dcg_p);
+    dfree(h->dcg_b); dfree(h->dcg_x); dfree(h->dcg); dfree(h->dpart3);
 }
 
 static int upload_props(ceos_handle* h) {
@@ -779,7 +779,15 @@ static int ensure_system(ceos_handle* h) {
             put(h->up_nodes[q], ((uint32_t)A << 7) | (uint32_t)(q - h->up_ptr[A] + 1));
         for (; j < width; ++j) nbr[(size_t)j * nn + A] = A;
     }
+    // neighbour ids in the SELL order of the PCG product (padding lanes of the last chunk: node 0)
+    const long nchunk = (nn + kSellChunk - 1) / kSellChunk;
+    std::vector<int> sell((size_t)nchunk * width * kSellChunk, 0);
+    for (int A = 0; A < nn; ++A)
+        for (int j = 0; j < width; ++j)
+            sell[((size_t)(A / kSellChunk) * width + j) * kSellChunk + (A % kSellChunk)] = nbr[(size_t)j * nn + A];
     const int nblk = (nn + kSysBlock - 1) / kSysBlock;
+    HIP_OK(dalloc(h->dsell_nbr, sell.size()));
+    HIP_OK(hipMemcpy(h->dsell_nbr, sell.data(), sizeof(int) * sell.size(), hipMemcpyHostToDevice));
     HIP_OK(dalloc(h->dup_ptr, nn + 1));
     HIP_OK(dalloc(h->dup_nodes, h->up_nodes.size()));
     HIP_OK(dalloc(h->dslot_nbr, nbr.size()));
@@ -789,6 +797,7 @@ static int ensure_system(ceos_handle* h) {
     HIP_OK(dalloc(h->dR, 3L * nn));
     HIP_OK(dalloc(h->dpart, nblk));
     HIP_OK(dalloc(h->dpart2, nblk));
+    HIP_OK(dalloc(h->dpart3, nblk));
     HIP_OK(dalloc(h->dnorm, 3));
     HIP_OK(hipMemcpy(h->dup_ptr, h->up_ptr.data(), sizeof(int) * (nn + 1), hipMemcpyHostToDevice));
     HIP_OK(hipMemcpy(h->dup_nodes, h->up_nodes.data(), sizeof(int) * h->up_nodes.size(), hipMemcpyHostToDevice));
@@ -813,6 +822,7 @@ static SysGeom sys_geom(const ceos_handle* h) {
     g.slot_src = h->dslot_src;
     g.dof_mask = h->ddof_mask;
     g.node_mask = h->dnode_mask;
+    g.sell_nbr = h->dsell_nbr;
     return g;
 }
 
@@ -846,7 +856,7 @@ static int residual_bc(ceos_handle* h, int it, const double* dU, const double* d
 static int ensure_solver(ceos_handle* h) {
     const int want_prec = h->solver_precision < 0 ? h->precision : h->solver_precision;
     const bool f32 = want_prec == CEOS_K_FP32_STORED;
-    const size_t nval = (size_t)h->sys_width * 9 * h->nn;
+    const size_t nval = (size_t)h->sys_width * 9 * kSellChunk * ((h->nn + kSellChunk - 1) / kSellChunk);
     if (!h->dell || h->ell_f32 != f32) {
         if (h->dell) (void)hipFree(h->dell);
         h->dell = nullptr;
@@ -859,8 +869,7 @@ static int ensure_solver(ceos_handle* h) {
         HIP_OK(dalloc(h->dcg_r, n));
         HIP_OK(dalloc(h->dcg_z, n));
         HIP_OK(dalloc(h->dcg_q, n));
-        HIP_OK(dalloc(h->dcg_p[0], n));
-        HIP_OK(dalloc(h->dcg_p[1], n));
+        HIP_OK(dalloc(h->dcg_p, n));
         HIP_OK(dalloc(h->dcg, 1));
     }
     return CEOS_OK;
@@ -895,9 +904,11 @@ static int linear_solve(ceos_handle* h, const void* dK, const double* db, double
     c0.maxit = maxit;
     HIP_OK(hipMemcpyAsync(h->dcg, &c0, sizeof(c0), hipMemcpyHostToDevice, s));
     hipLaunchKernelGGL(k_cg_init, dim3(nblk), dim3(kSysBlock), 0, s, nn, (const double*)h->dMinv, db, dx, h->dcg_r,
-                       h->dcg_z, h->dcg_p[1], h->dpart, h->dpart2);
+                       h->dcg_z, h->dcg_p, h->dpart, h->dpart2);
     hipLaunchKernelGGL(k_cg_init_final, dim3(1), dim3(kSysBlock), 0, s, (const double*)h->dpart,
                        (const double*)h->dpart2, nblk, h->dcg);
+    // per iteration: q = K p (partials of p.q in dpart3); alpha, x, r, z (partials of r.z and r.r
+    // in dpart, dpart2); beta, test, p.  r.z of iteration k sits in rz[k & 1].
     CgScalars c;
     int k = 0;
     for (;;) {
@@ -905,22 +916,20 @@ static int linear_solve(ceos_handle* h, const void* dK, const double* db, double
         HIP_OK(hipStreamSynchronize(s));
         if (c.done) break;
         for (int i = 0; i < h->cg_check; ++i, ++k) {
-            double* pp = h->dcg_p[(k + 1) & 1];  // previous direction
-            double* pc = h->dcg_p[k & 1];        // this iteration's direction
+            const int par = k & 1;
             if (h->ell_f32)
-                hipLaunchKernelGGL(k_cg_spmv<float>, dim3(nblk), dim3(kSysBlock), 0, s, g, (const float*)h->dell,
-                                   (const CgScalars*)h->dcg, (const double*)h->dcg_z, (const double*)pp, pc, h->dcg_q,
-                                   h->dpart);
+                hipLaunchKernelGGL((k_cg_spmv<float, kCgSlots>), dim3(nblk), dim3(kSysBlock), 0, s, g,
+                                   (const float*)h->dell, (const CgScalars*)h->dcg, (const double*)h->dcg_p, h->dcg_q,
+                                   h->dpart3);
             else
-                hipLaunchKernelGGL(k_cg_spmv<double>, dim3(nblk), dim3(kSysBlock), 0, s, g, (const double*)h->dell,
-                                   (const CgScalars*)h->dcg, (const double*)h->dcg_z, (const double*)pp, pc, h->dcg_q,
-                                   h->dpart);
-            hipLaunchKernelGGL(k_cg_alpha, dim3(1), dim3(kSysBlock), 0, s, (const double*)h->dpart, nblk, h->dcg);
-            hipLaunchKernelGGL(k_cg_update, dim3(nblk), dim3(kSysBlock), 0, s, nn, (const double*)h->dMinv,
-                               (const CgScalars*)h->dcg, (const double*)pc, (const double*)h->dcg_q, dx, h->dcg_r,
-                               h->dcg_z, h->dpart, h->dpart2);
-            hipLaunchKernelGGL(k_cg_beta, dim3(1), dim3(kSysBlock), 0, s, (const double*)h->dpart,
-                               (const double*)h->dpart2, nblk, h->dcg);
+                hipLaunchKernelGGL((k_cg_spmv<double, kCgSlots64>), dim3(nblk), dim3(kSysBlock), 0, s, g,
+                                   (const double*)h->dell, (const CgScalars*)h->dcg, (const double*)h->dcg_p, h->dcg_q,
+                                   h->dpart3);
+            hipLaunchKernelGGL(k_cg_update, dim3(nblk), dim3(kSysBlock), 0, s, nn, par, (const double*)h->dMinv, h->dcg,
+                               (const double*)h->dpart3, nblk, (const double*)h->dcg_p, (const double*)h->dcg_q, dx,
+                               h->dcg_r, h->dcg_z, h->dpart, h->dpart2);
+            hipLaunchKernelGGL(k_cg_pupdate, dim3(nblk), dim3(kSysBlock), 0, s, nn, par, h->dcg, (const double*)h->dpart,
+                               (const double*)h->dpart2, nblk, (const double*)h->dcg_z, h->dcg_p);
         }
         HIP_OK(hipGetLastError());
     }

@@ -22,9 +22,11 @@
 // (A for B >= A, B for B < A, stored transposed), rank 0 = the diagonal block, rank q >= 1 the
 // q-th upper block of the owner; kSlotPad marks an unused slot.
 //
-// ELL copy for the PCG: per node `width` slots, slot-major: value (j*9 + 3a + b)*nn + A holds
-// entry (3A + a, 3 nbr + b) of slot j, so lanes of a wavefront (consecutive nodes) read
-// consecutive addresses for every (slot, entry).  Built once per Newton iteration from the
+// SELL copy for the PCG: nodes in chunks of 64 (one wavefront); chunk c stores its `width`
+// slots, 9 entries each, 64 lanes innermost: value ((c*width + j)*9 + 3a + b)*64 + lane holds
+// entry (3A + a, 3 nbr + b) of slot j of node A = 64 c + lane.  A wavefront reads 64
+// consecutive values per (slot, entry) and a workgroup streams 4 contiguous chunk records;
+// neighbour ids likewise, (c*width + j)*64 + lane.  Built once per Newton iteration from the
 // BC-applied CSR, FP64 or FP32 (vectors, products and dot products stay FP64).
 #pragma once
 #include <stdint.h>
@@ -34,6 +36,8 @@ namespace ceos {
 constexpr uint8_t kBcPresc = 1, kBcFixed = 2;
 constexpr uint32_t kSlotPad = 0xffffffffu;
 constexpr int kSysBlock = 256;
+constexpr int kCgSlots = 9;    // neighbour slots per load batch of the FP32 product
+constexpr int kCgSlots64 = 3;  // and of the FP64 one
 
 struct SysGeom {
     int nn;
@@ -46,7 +50,14 @@ struct SysGeom {
     const uint32_t* slot_src;  // [width][nn]
     const uint8_t* dof_mask;   // [3 nn] kBcPresc | kBcFixed
     const uint8_t* node_mask;  // [nn] OR of the node's three DOF masks
+    const int* sell_nbr;       // [chunks][width][64] neighbour ids in the SELL order
 };
+
+constexpr int kSellChunk = 64;
+
+__device__ inline long sell_val(int width, int A, int j, int m) {
+    return (((long)(A >> 6) * width + j) * 9 + m) * kSellChunk + (A & 63);
+}
 
 // block (rows: DOFs of A, columns: DOFs of the slot's neighbour) of the symmetric matrix
 template <typename KT>
@@ -229,7 +240,7 @@ __global__ __launch_bounds__(kSysBlock) void k_bc_matrix(SysGeom g, KT* __restri
     }
 }
 
-// ELL copy of the symmetric matrix, one thread per (slot, node)
+// SELL copy of the symmetric matrix, one thread per (slot, node)
 template <typename KT, typename VT>
 __global__ __launch_bounds__(kSysBlock) void k_build_ell(SysGeom g, const KT* __restrict__ K, VT* __restrict__ val) {
     const long t = (long)blockIdx.x * kSysBlock + threadIdx.x;
@@ -238,9 +249,8 @@ __global__ __launch_bounds__(kSysBlock) void k_build_ell(SysGeom g, const KT* __
     const uint32_t src = g.slot_src[t];
     double b[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     if (src != kSlotPad) csr_block(K, g, A, src, b);
-    const long nn = g.nn;
 #pragma unroll
-    for (int m = 0; m < 9; ++m) val[((long)j * 9 + m) * nn + A] = (VT)b[m];
+    for (int m = 0; m < 9; ++m) val[sell_val(g.width, A, j, m)] = (VT)b[m];
 }
 
 // inverse of every node's diagonal block (symmetric, stored 00 01 02 11 12 22 as [6][nn]);
@@ -270,9 +280,10 @@ __global__ __launch_bounds__(kSysBlock) void k_block_jacobi(SysGeom g, const KT*
     Minv[5 * nn + A] = (b[0] * b[4] - b[1] * b[3]) * id;
 }
 
-// PCG scalars on the device
+// PCG scalars on the device.  r.z of iteration k lives in rz[k & 1]: the kernel that forms
+// r.z of iteration k + 1 reads rz[k & 1] in every workgroup while workgroup 0 writes rz[(k+1) & 1].
 struct CgScalars {
-    double rz;       // r . z of the current iterate
+    double rz[2];
     double alpha, beta;
     double rr, bb;   // r . r, b . b
     double tol2;     // rtol^2
@@ -290,9 +301,42 @@ __device__ inline void precond(const double* __restrict__ Minv, long nn, int A, 
     z[2] = m02 * r[0] + m12 * r[1] + m22 * r[2];
 }
 
-// x = 0, r = b, z = M^-1 b, p_prev = 0; partials of r.z and r.r
+// the PCG state as one value for the whole workgroup.  Workgroup 0 of k_cg_update and
+// k_cg_pupdate may set `done` while other workgroups of the same launch start: a workgroup then
+// sees either value, never a mix, and both are consistent (the skipped work is unused).
+__device__ inline bool cg_stopped(const CgScalars* c) {
+    __shared__ int s_done;
+    if (threadIdx.x == 0) s_done = __atomic_load_n(&c->done, __ATOMIC_RELAXED);
+    __syncthreads();
+    return s_done != 0;
+}
+
+// two sums of n partials by one workgroup in a fixed order (every workgroup that calls it gets
+// bitwise the same values)
+__device__ inline void reduce2(const double* pa, const double* pb, int n, double& a, double& b) {
+    __shared__ double la[kSysBlock], lb[kSysBlock];
+    double s = 0.0, q = 0.0;
+    for (int i = threadIdx.x; i < n; i += kSysBlock) {
+        s += pa[i];
+        if (pb) q += pb[i];
+    }
+    la[threadIdx.x] = s;
+    lb[threadIdx.x] = q;
+    __syncthreads();
+    for (int w = kSysBlock / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            la[threadIdx.x] += la[threadIdx.x + w];
+            lb[threadIdx.x] += lb[threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    a = la[0];
+    b = lb[0];
+}
+
+// x = 0, r = b, z = p = M^-1 b; partials of r.z and r.r
 __global__ __launch_bounds__(kSysBlock) void k_cg_init(int nn, const double* __restrict__ Minv, const double* __restrict__ b,
-                                                       double* x, double* r, double* z, double* p_prev,
+                                                       double* x, double* r, double* z, double* p,
                                                        double* part_s, double* part_m) {
     const int A = blockIdx.x * kSysBlock + threadIdx.x;
     double rz = 0.0, rr = 0.0;
@@ -306,7 +350,7 @@ __global__ __launch_bounds__(kSysBlock) void k_cg_init(int nn, const double* __r
             x[3L * A + i] = 0.0;
             r[3L * A + i] = rv[i];
             z[3L * A + i] = zv[i];
-            p_prev[3L * A + i] = 0.0;
+            p[3L * A + i] = zv[i];
             rz += rv[i] * zv[i];
             rr += rv[i] * rv[i];
         }
@@ -316,86 +360,89 @@ __global__ __launch_bounds__(kSysBlock) void k_cg_init(int nn, const double* __r
 
 __global__ __launch_bounds__(kSysBlock) void k_cg_init_final(const double* part_s, const double* part_m, int n,
                                                              CgScalars* c) {
-    __shared__ double ls[kSysBlock], lr[kSysBlock];
-    double s = 0.0, q = 0.0;
-    for (int i = threadIdx.x; i < n; i += kSysBlock) {
-        s += part_s[i];
-        q += part_m[i];
-    }
-    ls[threadIdx.x] = s;
-    lr[threadIdx.x] = q;
-    __syncthreads();
-    for (int w = kSysBlock / 2; w > 0; w >>= 1) {
-        if ((int)threadIdx.x < w) {
-            ls[threadIdx.x] += ls[threadIdx.x + w];
-            lr[threadIdx.x] += lr[threadIdx.x + w];
-        }
-        __syncthreads();
-    }
+    double rz, rr;
+    reduce2(part_s, part_m, n, rz, rr);
     if (threadIdx.x == 0) {
-        c->rz = ls[0];
-        c->rr = c->bb = lr[0];
+        c->rz[0] = rz;
+        c->rr = c->bb = rr;
         c->beta = 0.0;
         c->alpha = 0.0;
         c->iters = 0;
-        c->done = (lr[0] == 0.0) ? 1 : 0;
+        c->done = (rr == 0.0) ? 1 : 0;
         if (!c->done && c->maxit <= 0) c->done = 3;
     }
 }
 
-// p = z + beta p_prev (written for the node's own rows, recomputed on the fly for neighbours),
-// q = K p; partial sums of p.q
-template <typename VT>
+// q = K p, partial sums of p.q.  The neighbour slots are taken G at a time: their ids, the
+// G x 9 matrix values and the gathered p are all requested before any is used, so a thread
+// keeps G x 10 + G x 3 loads in flight instead of one dependent chain per slot.
+template <typename VT, int G>
 __global__ __launch_bounds__(kSysBlock) void k_cg_spmv(SysGeom g, const VT* __restrict__ val, const CgScalars* c,
-                                                       const double* __restrict__ z, const double* __restrict__ p_prev,
-                                                       double* __restrict__ p, double* __restrict__ q, double* part_s) {
-    if (c->done) return;
+                                                       const double* __restrict__ p, double* __restrict__ q,
+                                                       double* part_s) {
+    if (cg_stopped(c)) return;
     const int A = blockIdx.x * kSysBlock + threadIdx.x;
-    const double beta = c->beta;
     double pq = 0.0;
     if (A < g.nn) {
-        const long nn = g.nn;
         double acc[3] = {0.0, 0.0, 0.0};
-        for (int j = 0; j < g.width; ++j) {
-            const int B = g.slot_nbr[(long)j * nn + A];
-            double pb[3];
+        const int* nb = g.sell_nbr + (long)(A >> 6) * g.width * kSellChunk + (A & 63);
+        const VT* vb = val + sell_val(g.width, A, 0, 0);
+        int j0 = 0;
+        for (; j0 + G <= g.width; j0 += G) {
+            int B[G];
+            VT v[G][9];
 #pragma unroll
-            for (int k = 0; k < 3; ++k) pb[k] = z[3L * B + k] + beta * p_prev[3L * B + k];
-            const VT* v = val + (long)j * 9 * nn + A;
+            for (int u = 0; u < G; ++u) B[u] = nb[(j0 + u) * kSellChunk];
+#pragma unroll
+            for (int u = 0; u < G; ++u)
+#pragma unroll
+                for (int m = 0; m < 9; ++m) v[u][m] = __builtin_nontemporal_load(vb + ((j0 + u) * 9 + m) * kSellChunk);
+            double pb[G][3];
+#pragma unroll
+            for (int u = 0; u < G; ++u)
+#pragma unroll
+                for (int k = 0; k < 3; ++k) pb[u][k] = p[3L * B[u] + k];
+#pragma unroll
+            for (int u = 0; u < G; ++u)
+#pragma unroll
+                for (int i = 0; i < 3; ++i)
+                    acc[i] += (double)v[u][3 * i] * pb[u][0] + (double)v[u][3 * i + 1] * pb[u][1] +
+                              (double)v[u][3 * i + 2] * pb[u][2];
+        }
+        for (int j = j0; j < g.width; ++j) {
+            const int B = nb[j * kSellChunk];
+            const VT* v = vb + j * 9 * kSellChunk;
 #pragma unroll
             for (int i = 0; i < 3; ++i)
-                acc[i] += (double)v[(3 * i) * nn] * pb[0] + (double)v[(3 * i + 1) * nn] * pb[1] +
-                          (double)v[(3 * i + 2) * nn] * pb[2];
+                acc[i] += (double)v[(3 * i) * kSellChunk] * p[3L * B] + (double)v[(3 * i + 1) * kSellChunk] * p[3L * B + 1] +
+                          (double)v[(3 * i + 2) * kSellChunk] * p[3L * B + 2];
         }
 #pragma unroll
         for (int i = 0; i < 3; ++i) {
-            const double pa = z[3L * A + i] + beta * p_prev[3L * A + i];
-            p[3L * A + i] = pa;
             q[3L * A + i] = acc[i];
-            pq += pa * acc[i];
+            pq += p[3L * A + i] * acc[i];
         }
     }
     block_sum_max(pq, 0.0, part_s, nullptr);
 }
 
-__global__ __launch_bounds__(kSysBlock) void k_cg_alpha(const double* part_s, int n, CgScalars* c) {
-    if (c->done) return;
-    double pq;
-    reduce_partials(part_s, nullptr, n, &pq, nullptr);
-    if (threadIdx.x == 0) {
-        if (!(pq > 0.0)) c->done = 2;
-        else c->alpha = c->rz / pq;
-    }
-}
-
-// x += alpha p, r -= alpha q, z = M^-1 r; partials of r.z and r.r
-__global__ __launch_bounds__(kSysBlock) void k_cg_update(int nn, const double* __restrict__ Minv, const CgScalars* c,
+// alpha = r.z / p.q (p.q reduced by every workgroup alike); x += alpha p, r -= alpha q,
+// z = M^-1 r; partials of r.z and r.r
+__global__ __launch_bounds__(kSysBlock) void k_cg_update(int nn, int par, const double* __restrict__ Minv,
+                                                         CgScalars* c, const double* pq_part, int nblk,
                                                          const double* __restrict__ p, const double* __restrict__ q,
                                                          double* x, double* r, double* z, double* part_s,
                                                          double* part_m) {
-    if (c->done) return;
+    if (cg_stopped(c)) return;
+    double pq, unused;
+    reduce2(pq_part, nullptr, nblk, pq, unused);
+    if (!(pq > 0.0)) {  // the same verdict in every workgroup
+        if (blockIdx.x == 0 && threadIdx.x == 0) __atomic_store_n(&c->done, 2, __ATOMIC_RELAXED);
+        return;
+    }
+    const double alpha = c->rz[par] / pq;
+    if (blockIdx.x == 0 && threadIdx.x == 0) c->alpha = alpha;
     const int A = blockIdx.x * kSysBlock + threadIdx.x;
-    const double alpha = c->alpha;
     double rz = 0.0, rr = 0.0;
     if (A < nn) {
         double rv[3], zv[3];
@@ -416,32 +463,28 @@ __global__ __launch_bounds__(kSysBlock) void k_cg_update(int nn, const double* _
     block_sum2(rz, rr, part_s, part_m);
 }
 
-__global__ __launch_bounds__(kSysBlock) void k_cg_beta(const double* part_s, const double* part_m, int n, CgScalars* c) {
-    if (c->done) return;
-    __shared__ double ls[kSysBlock], lr[kSysBlock];
-    double s = 0.0, q = 0.0;
-    for (int i = threadIdx.x; i < n; i += kSysBlock) {
-        s += part_s[i];
-        q += part_m[i];
-    }
-    ls[threadIdx.x] = s;
-    lr[threadIdx.x] = q;
-    __syncthreads();
-    for (int w = kSysBlock / 2; w > 0; w >>= 1) {
-        if ((int)threadIdx.x < w) {
-            ls[threadIdx.x] += ls[threadIdx.x + w];
-            lr[threadIdx.x] += lr[threadIdx.x + w];
-        }
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) {
-        const double rz_new = ls[0];
-        c->beta = rz_new / c->rz;
-        c->rz = rz_new;
-        c->rr = lr[0];
+// beta = r.z (new) / r.z (old), the convergence test, p = z + beta p; workgroup 0 records the
+// scalars.  A workgroup that starts after workgroup 0 has set `done` skips the p update, which
+// is then no longer used.
+__global__ __launch_bounds__(kSysBlock) void k_cg_pupdate(int nn, int par, CgScalars* c, const double* rz_part,
+                                                          const double* rr_part, int nblk,
+                                                          const double* __restrict__ z, double* __restrict__ p) {
+    if (cg_stopped(c)) return;
+    double rz_new, rr;
+    reduce2(rz_part, rr_part, nblk, rz_new, rr);
+    const double beta = rz_new / c->rz[par];
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        c->rz[par ^ 1] = rz_new;
+        c->beta = beta;
+        c->rr = rr;
         c->iters += 1;
-        if (c->rr <= c->tol2 * c->bb) c->done = 1;
-        else if (c->iters >= c->maxit) c->done = 3;
+        if (rr <= c->tol2 * c->bb) __atomic_store_n(&c->done, 1, __ATOMIC_RELAXED);
+        else if (c->iters >= c->maxit) __atomic_store_n(&c->done, 3, __ATOMIC_RELAXED);
+    }
+    const int A = blockIdx.x * kSysBlock + threadIdx.x;
+    if (A < nn) {
+#pragma unroll
+        for (int i = 0; i < 3; ++i) p[3L * A + i] = z[3L * A + i] + beta * p[3L * A + i];
     }
 }
 
