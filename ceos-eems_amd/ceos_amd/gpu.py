@@ -24,6 +24,8 @@ LIB_PATH = os.path.join(PKG, "lib", "libceos_gpu.so")
 OK, ERR_DETJ, ERR_LOCAL_NR, ERR_SLOT, ERR_ARG, ERR_HIP = 0, -1, -2, -3, -4, -5
 MODEL_GHL, MODEL_J2, MODEL_NH, MODEL_SVK, MODEL_CNH, MODEL_VISCO = 1, 2, 3, 5, 7, 12
 MODEL_EXT = 99  # FE^2 macro point: S and dS/dE per Gauss point from the caller
+MODEL_NH_Q1P0 = 4  # mean-dilatation Neo-Hookean (with ELEM_MEAN_DILATATION)
+ELEM_FULL_INTEGRATION, ELEM_MEAN_DILATATION = 1, 2
 EXT_PER_GP = 27
 STATE_STRIDE = 20
 K_FP64, K_FP32_STORED = 0, 1
@@ -178,7 +180,7 @@ class CeosMesh:
     """One hex8 mesh on the GPU. X (nn,3), conn (ne,8) 1-based node ids, mat_id (ne,)."""
 
     def __init__(self, X, conn1, mat_id=None, nl_analysis=True, precision=K_FP64, fused=True,
-                 assembly=ASSEMBLY_GATHER):
+                 assembly=ASSEMBLY_GATHER, elem_tech=ELEM_FULL_INTEGRATION):
         L = load_library()
         X = np.ascontiguousarray(X, dtype=np.float64).reshape(-1, 3)
         conn1 = np.ascontiguousarray(conn1, dtype=np.int32).reshape(-1, 8)
@@ -191,7 +193,7 @@ class CeosMesh:
         _check(L.ceos_mesh_create(self.nn, X.reshape(-1), self.ne, 8, conn1.reshape(-1), mat_id,
                                   C.byref(h)), "ceos_mesh_create")
         self._h = h
-        _check(L.ceos_set_analysis(self._h, 1 if nl_analysis else 0, 1), "ceos_set_analysis")
+        _check(L.ceos_set_analysis(self._h, 1 if nl_analysis else 0, elem_tech), "ceos_set_analysis")
         _check(L.ceos_set_precision(self._h, precision), "ceos_set_precision")
         _check(L.ceos_set_fused(self._h, 1 if fused else 0), "ceos_set_fused")
         _check(L.ceos_set_assembly(self._h, assembly), "ceos_set_assembly")

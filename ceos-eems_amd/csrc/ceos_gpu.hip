@@ -72,7 +72,7 @@ void set_status(ceos_status* st, int code, int nfail, const char* msg) {
 }
 
 bool model_supported(int m) {
-    return m == kGHL || m == kJ2 || m == kNH || m == kSVK || m == kCNH || m == kVISCO || m == kEXT;
+    return m == kGHL || m == kJ2 || m == kNH || m == kNHQ1 || m == kSVK || m == kCNH || m == kVISCO || m == kEXT;
 }
 
 int model_nprops(int m) {
@@ -392,8 +392,12 @@ static int plan_pipeline(ceos_handle* h) {
 // slot order, launch ranges, rank/first-touch tables, per-element device arrays
 static int finalize(ceos_handle* h) {
     if (h->finalized) return CEOS_OK;
-    for (size_t i = 0; i < h->mat_ids.size(); ++i)
+    for (size_t i = 0; i < h->mat_ids.size(); ++i) {
         if (!h->mats[i].set) return CEOS_ERR_ARG;
+        // the reference pairs the mean-dilatation technology with its Q1P0 models only
+        // (ModConstitutiveModelLibrary.f90:492-556): model 4 <=> CEOS_ELEM_MEAN_DILATATION
+        if ((h->mats[i].model == kNHQ1) != (h->elem_tech == CEOS_ELEM_MEAN_DILATATION)) return CEOS_ERR_ARG;
+    }
     const int ne = h->ne;
     // J2 state of a previous slot order, kept in element order across the rebuild
     std::vector<double> keep_old, keep_new;
@@ -590,6 +594,7 @@ static void launch_range(const ElemParams& p, const LaunchRange& r, bool nl, boo
         case kGHL: launch_m<kGHL, KT, G>(p, r.begin, r.end, nl, wk, numt, s); break;
         case kJ2: launch_m<kJ2, KT, G>(p, r.begin, r.end, nl, wk, numt, s); break;
         case kNH: launch_m<kNH, KT, G>(p, r.begin, r.end, nl, wk, numt, s); break;
+        case kNHQ1: launch_m<kNHQ1, KT, G>(p, r.begin, r.end, nl, wk, numt, s); break;
         case kSVK: launch_m<kSVK, KT, G>(p, r.begin, r.end, nl, wk, numt, s); break;
         case kCNH: launch_m<kCNH, KT, G>(p, r.begin, r.end, nl, wk, numt, s); break;
         case kVISCO: launch_m<kVISCO, KT, G>(p, r.begin, r.end, nl, wk, numt, s); break;
@@ -1041,8 +1046,9 @@ int ceos_material_set(ceos_handle* h, int mat_id, int model, const double* props
 }
 
 int ceos_set_analysis(ceos_handle* h, int nl_analysis, int elem_tech) {
-    if (!h || elem_tech != CEOS_ELEM_FULL_INTEGRATION) return CEOS_ERR_ARG;
+    if (!h || (elem_tech != CEOS_ELEM_FULL_INTEGRATION && elem_tech != CEOS_ELEM_MEAN_DILATATION)) return CEOS_ERR_ARG;
     h->nl = nl_analysis ? 1 : 0;
+    if (h->elem_tech != elem_tech) h->finalized = false;  // the material check runs again
     h->elem_tech = elem_tech;
     h->f_valid = h->k_valid = false;
     return CEOS_OK;

@@ -90,7 +90,7 @@ __device__ __forceinline__ double hex8_grad(const double dN[8][3], const double 
 //   1 GHL: E, nu   2 J2: E, nu, sigma_y, H   3 NH: C10, K   5 SVK: E, nu   7 CNH: mu, lambda
 //   99 EXT: none (stress and tangent come per Gauss point from the caller, see mat_ext)
 // ------------------------------------------------------------------------------------------
-enum : int { kGHL = 1, kJ2 = 2, kNH = 3, kSVK = 5, kCNH = 7, kVISCO = 12, kEXT = 99 };
+enum : int { kGHL = 1, kJ2 = 2, kNH = 3, kNHQ1 = 4, kSVK = 5, kCNH = 7, kVISCO = 12, kEXT = 99 };
 constexpr int kExtPerGp = 27;  // external material point: S (6) + dS/dE upper (21)
 
 __device__ __forceinline__ void elastic_up(double E, double nu, double D[21]) {
@@ -251,9 +251,11 @@ __device__ __forceinline__ void mat_ext(const double* q, const double F[3][3], d
 }
 
 // model 3, ModNeoHookean.f90:409-615
-__device__ __forceinline__ void mat_nh(const double* p, const double F[3][3], double s[6], double D[21]) {
-    const double C10 = p[0], K = p[1];
-    const double J = det3(F);
+// Neo-Hookean stress and tangent from the isochoric part at F and a given volumetric part:
+// sigma = 2 C10 J^-5/3 (b - tr b/3 I) + pr I, Cbar = Ciso + cv1 Cinv (x) Cinv - cv2 Cinv (.) Cinv,
+// D = push forward of Cbar.
+__device__ __forceinline__ void nh_core(double C10, const double F[3][3], double J, double Jm23, double Jm53,
+                                        double pr, double cv1, double cv2, double s[6], double D[21]) {
     const double b00 = F[0][0] * F[0][0] + F[0][1] * F[0][1] + F[0][2] * F[0][2];
     const double b11 = F[1][0] * F[1][0] + F[1][1] * F[1][1] + F[1][2] * F[1][2];
     const double b22 = F[2][0] * F[2][0] + F[2][1] * F[2][1] + F[2][2] * F[2][2];
@@ -261,10 +263,6 @@ __device__ __forceinline__ void mat_nh(const double* p, const double F[3][3], do
     const double b12 = F[1][0] * F[2][0] + F[1][1] * F[2][1] + F[1][2] * F[2][2];
     const double b02 = F[0][0] * F[2][0] + F[0][1] * F[2][1] + F[0][2] * F[2][2];
     const double trb = b00 + b11 + b22;
-    const double J13 = pow(J, 1.0 / 3.0);
-    const double Jm23 = pow(J, -2.0 / 3.0);
-    const double Jm53 = pow(J, -5.0 / 3.0);
-    const double pr = 3.0 * K * Jm23 * (J13 - 1.0);
     const double g = 2.0 * C10 * Jm53;
     const double t3 = trb / 3.0;
     s[0] = g * (b00 - t3) + pr;
@@ -290,9 +288,7 @@ __device__ __forceinline__ void mat_nh(const double* p, const double F[3][3], do
     for (int i = 0; i < 6; ++i) Siso[i] = Jm23 * ((i < 3 ? sf : 0.0) - (1.0 / 3.0) * ipSC * CiV[i]);
     double sq[21];
     square_self_up(CiV, sq);
-    const double d2 = -K * Jm53 * (J13 - 2.0);
     const double c1 = (2.0 / 3.0) * Jm23 * ipSC;
-    const double cv1 = J * (pr + J * d2), cv2 = 2.0 * J * pr;
     double Cbar[21];
 #pragma unroll
     for (int i = 0; i < 6; ++i)
@@ -306,6 +302,38 @@ __device__ __forceinline__ void mat_nh(const double* p, const double F[3][3], do
             Cbar[k] = ciso + cvol;
         }
     push_forward_up(Cbar, F, J, D);
+}
+
+// model 3 (ModNeoHookean.f90:409-615): p = 3 K J^-2/3 (J^1/3 - 1), Cvol = J (p + J dp/dJ)
+// Cinv (x) Cinv - 2 J p Cinv (.) Cinv
+__device__ __forceinline__ void mat_nh(const double* p, const double F[3][3], double s[6], double D[21]) {
+    const double C10 = p[0], K = p[1];
+    const double J = det3(F);
+    const double J13 = pow(J, 1.0 / 3.0);
+    const double Jm23 = pow(J, -2.0 / 3.0);
+    const double Jm53 = pow(J, -5.0 / 3.0);
+    const double pr = 3.0 * K * Jm23 * (J13 - 1.0);
+    const double d2 = -K * Jm53 * (J13 - 2.0);
+    nh_core(C10, F, J, Jm23, Jm53, pr, J * (pr + J * d2), 2.0 * J * pr, s, D);
+}
+
+// model 4, the mean-dilatation (Q1P0) Neo-Hookean (ModNeoHookeanQ1P0.f90:429-484 stress,
+// :540-613 tangent): the isochoric part at the Gauss point's J, the pressure of the element's
+// Jbar = V/V0, pbar = 3 K (Jbar^1/3 - 1) / Jbar^2/3, Cp = J pbar (Cinv (x) Cinv - 2 Cinv (.) Cinv);
+// the pressure's derivative enters the element stiffness through Bdiv (nhq1p0_d2psi)
+__device__ __forceinline__ void mat_nhq1p0(const double* p, const double F[3][3], double jbar, double s[6],
+                                           double D[21]) {
+    const double C10 = p[0], K = p[1];
+    const double J = det3(F);
+    const double Jm23 = pow(J, -2.0 / 3.0);
+    const double Jm53 = pow(J, -5.0 / 3.0);
+    const double pbar = (3.0 * K * (pow(jbar, 1.0 / 3.0) - 1.0)) / pow(jbar, 2.0 / 3.0);
+    nh_core(C10, F, J, Jm23, Jm53, pbar, J * pbar, 2.0 * J * pbar, s, D);
+}
+
+// SecondDerivativesOfPSI_Jbar_NeoHookeanQ1P0 (ModNeoHookeanQ1P0.f90:625-640)
+__device__ __forceinline__ double nhq1p0_d2psi(const double* p, double jbar) {
+    return -(p[1] * (pow(jbar, 1.0 / 3.0) - 2.0)) / pow(jbar, 5.0 / 3.0);
 }
 
 // Model 3 in the reference configuration: second Piola stress and material tangent dS/dE (upper

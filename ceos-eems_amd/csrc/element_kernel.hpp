@@ -78,11 +78,12 @@ constexpr int model_nstate() {
 }
 
 // returns the J2 PlasticState flag, or for the viscoelastic model whether its local Newton
-// converged; the other models return false
+// converged; the other models return false.  jbar: the element's V/V0 (model 4 only).
 template <int MODEL>
 __device__ __forceinline__ bool material(const double* p, const double F[3][3], double time, const double* old,
-                                         double s[6], double D[21], double* nw) {
+                                         double s[6], double D[21], double* nw, double jbar = 1.0) {
     if constexpr (MODEL == kGHL) { mat_ghl(p, F, s, D); return false; }
+    else if constexpr (MODEL == kNHQ1) { mat_nhq1p0(p, F, jbar, s, D); return false; }
     else if constexpr (MODEL == kCNH) { mat_cnh(p, F, s, D); return false; }
     else if constexpr (MODEL == kNH) { mat_nh(p, F, s, D); return false; }
     else if constexpr (MODEL == kSVK) { mat_svk(p, F, s, D); return false; }
@@ -110,7 +111,8 @@ __device__ __forceinline__ void first_piola(const double s[6], const double F[3]
 
 template <int MODEL>
 __device__ __noinline__ void tangent_numerical(const double* pr, const double F0[3][3], double time,
-                                               const double* old, const double s0[6], double D[21]) {
+                                               const double* old, const double s0[6], double D[21],
+                                               double jbar) {
     constexpr int NS = model_nstate<MODEL>();
     const double h = 1.0e-8;
     double A[9][9];
@@ -127,7 +129,7 @@ __device__ __noinline__ void tangent_numerical(const double* pr, const double F0
         for (int sgn = 0; sgn < 2; ++sgn) {
             F[i][j] = F0[i][j] + (sgn == 0 ? h : -h);
             double s[6], Dt[21], nw[NS];
-            material<MODEL>(pr, F, time, old, s, Dt, nw);
+            material<MODEL>(pr, F, time, old, s, Dt, nw, jbar);
             first_piola(s, F, P[sgn]);
         }
         F[i][j] = F0[i][j];
@@ -272,6 +274,9 @@ __global__ __launch_bounds__(128) void k_element(ElemParams p, int e_begin, int 
     double wdet;
     bool bad;
     bool nrfail = false;
+    double jbar = 1.0, vol0 = 1.0;           // model 4: element volume ratio, reference volume
+    double kdiv = 0.0;                       // model 4: d2Psi/dJbar2 / V0
+    double bdiv[MODEL == kNHQ1 ? 8 : 1][3];  // model 4: integral of grad_x N over the element
     {
         double dN[8][3];
         hex8_dshape_gp(g, dN);
@@ -299,6 +304,29 @@ __global__ __launch_bounds__(128) void k_element(ElemParams p, int e_begin, int 
         const double detx = hex8_grad(dN, c, gx);
         bad = active && !(detx > kDetTol && detX > kDetTol);
         wdet = detx;  // Gauss weight 1 (ModElementHexa8.f90:326)
+        if constexpr (MODEL == kNHQ1) {
+            // ElementVolume (ModElement.f90:1313-1427): V and V0 over the element's 8 lanes
+            // (butterfly sums: every lane gets the same bits), Jbar = V/V0 at every Gauss point
+            // (SubSolveConstitutiveModel.f90:87-100); Bdiv = sum_g grad_x N w detJ, the integral
+            // of div N over the current element (ModElement.f90:459)
+            double v = detx, vx = detX;
+#pragma unroll
+            for (int o = 1; o < 8; o <<= 1) {
+                v += __shfl_xor(v, o);
+                vx += __shfl_xor(vx, o);
+            }
+            jbar = v / vx;
+            vol0 = vx;
+#pragma unroll
+            for (int a = 0; a < 8; ++a)
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    double t = gx[a][k] * detx;
+#pragma unroll
+                    for (int o = 1; o < 8; o <<= 1) t += __shfl_xor(t, o);
+                    bdiv[a][k] = t;
+                }
+        }
 
         constexpr int NS = model_nstate<MODEL>();
         double old[NS], nw[NS];
@@ -320,8 +348,9 @@ __global__ __launch_bounds__(128) void k_element(ElemParams p, int e_begin, int 
                 for (int v = 0; v < NS; ++v) old[v] = q[v];
             }
         }
-        const bool flag = material<MODEL>(pr, F, p.time, old, s, D, nw);
-        if constexpr (WANT_K && NUMT && MODEL != kEXT) tangent_numerical<MODEL>(pr, F, p.time, old, s, D);
+        const bool flag = material<MODEL>(pr, F, p.time, old, s, D, nw, jbar);
+        if constexpr (WANT_K && NUMT && MODEL != kEXT) tangent_numerical<MODEL>(pr, F, p.time, old, s, D, jbar);
+        if constexpr (MODEL == kNHQ1) kdiv = nhq1p0_d2psi(pr, jbar) / vol0;
         if (active) {
 #pragma unroll
             for (int k = 0; k < 6; ++k) p.sig[((long)k * 8 + g) * p.ne + e] = s[k];
@@ -365,6 +394,41 @@ __global__ __launch_bounds__(128) void k_element(ElemParams p, int e_begin, int 
     double kd[6];
     double ko[4][9];
     node_blocks<NL, WANT_K>(sm, b, fe0, fe1, fe2, kd, ko);
+    if constexpr (MODEL == kNHQ1 && NL && WANT_K) {
+        // Ke += d2Psi/dJbar2 / V0 Bdiv^T Bdiv (ModElement.f90:464-467): block (c,b) gains
+        // kdiv Bdiv_c (x) Bdiv_b.  The lane-dependent rows are selected with unrolled compares
+        // (a runtime index into bdiv would move it to scratch memory).
+        double bb[3] = {0.0, 0.0, 0.0}, bc[4][3];
+#pragma unroll
+        for (int a = 0; a < 8; ++a)
+#pragma unroll
+            for (int k = 0; k < 3; ++k) bb[k] = (a == b) ? bdiv[a][k] : bb[k];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int c = (b + j + 1) & 7;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) bc[j][k] = 0.0;
+#pragma unroll
+            for (int a = 0; a < 8; ++a)
+#pragma unroll
+                for (int k = 0; k < 3; ++k) bc[j][k] = (a == c) ? bdiv[a][k] : bc[j][k];
+        }
+        kd[0] += kdiv * bb[0] * bb[0];
+        kd[1] += kdiv * bb[0] * bb[1];
+        kd[2] += kdiv * bb[0] * bb[2];
+        kd[3] += kdiv * bb[1] * bb[1];
+        kd[4] += kdiv * bb[1] * bb[2];
+        kd[5] += kdiv * bb[2] * bb[2];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            if (j < npart) {
+#pragma unroll
+                for (int r = 0; r < 3; ++r)
+#pragma unroll
+                    for (int t = 0; t < 3; ++t) ko[j][3 * r + t] += kdiv * bc[j][r] * bb[t];
+            }
+        }
+    }
     if constexpr (GATHER) {
         // stage the element record in LDS (the GP data are no longer read), then write the
         // records of the block's consecutive elements with unit-stride stores

@@ -31,6 +31,7 @@ extern "C" {
 #define CEOS_MODEL_GENERALIZED_HOOKES_LAW 1 /* props: E, nu */
 #define CEOS_MODEL_J2_PLASTICITY 2          /* props: E, nu, yield stress, hardening modulus */
 #define CEOS_MODEL_NEO_HOOKEAN 3            /* props: C10, bulk modulus */
+#define CEOS_MODEL_NEO_HOOKEAN_Q1P0 4       /* props: C10, bulk modulus; CEOS_ELEM_MEAN_DILATATION only */
 #define CEOS_MODEL_ST_VENANT_KIRCHHOFF 5    /* props: E, nu */
 #define CEOS_MODEL_COMPRESSIBLE_NEO_HOOKEAN 7 /* props: mu, lambda */
 #define CEOS_MODEL_VISCOELASTIC_MATRIX 12   /* props: K_inf, Mu_inf, Lambda_inf, K_e, Mu_e, Ni_v */
@@ -44,8 +45,13 @@ extern "C" {
  *   visco: [0..8] Fv (column-major), [9..17] dv (column-major), [18] gamma, [19] time */
 #define CEOS_STATE_STRIDE 20
 
-/* element technology, FEM/Modules/ModAnalysis.f90 (only full integration is provided) */
+/* element technology, FEM/Modules/ModAnalysis.f90:47-50 (reduced integration is not provided).
+ * Mean dilatation (Q1P0): every Gauss point of an element sees the element's volume ratio
+ * Jbar = V/V0 (ElementVolume, ModElement.f90:1313-1427) and the tangent gains
+ * d2Psi/dJbar2 / V0 Bdiv^T Bdiv (ModElement.f90:429-467); it goes with model 4 only, as the
+ * reference's library pairs them (ModConstitutiveModelLibrary.f90:504). */
 #define CEOS_ELEM_FULL_INTEGRATION 1
+#define CEOS_ELEM_MEAN_DILATATION 2
 
 /* tangent storage (ceos_set_precision) */
 #define CEOS_K_FP64 0

@@ -377,6 +377,53 @@ static void mat_nh(const double *p, const double F[3][3], double s[6], double D[
     push_forward_v(Cbar, F, D);
 }
 
+/* model 4, the Q1P0 (mean dilatation) Neo-Hookean, 3D: the isochoric part of model 3 at the
+ * Gauss point's J and the pressure of the element's Jbar = V/V0
+ * (ModNeoHookeanQ1P0.f90:429-484 stress, :540-613 tangent):
+ *   pbar = 3 K (Jbar^1/3 - 1) / Jbar^2/3, sigma = 2 C10 J^-5/3 (b - tr b/3 I) + pbar I,
+ *   Cbar = Ciso + J pbar (Cinv (x) Cinv - 2 Cinv (.) Cinv), D = push forward of Cbar.
+ * The pressure's derivative enters the element stiffness through Bdiv (elem_ke). */
+static void mat_nhq1p0(const double *p, const double F[3][3], double Jbar, double s[6], double D[6][6]) {
+    double C10 = p[0], K = p[1], Ft[3][3], b[3][3];
+    transpose3(F, Ft);
+    matmul3(F, Ft, b);
+    double trb = b[0][0] + b[1][1] + b[2][2];
+    double J = det3(F);
+    double pbar = (3.0 * K * (pow(Jbar, 1.0 / 3.0) - 1.0)) / pow(Jbar, 2.0 / 3.0);
+    double S[3][3];
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++)
+            S[i][j] = 2.0 * C10 * pow(J, -5.0 / 3.0) * (b[i][j] - (trb / 3.0) * (i == j ? 1.0 : 0.0)) +
+                      pbar * (i == j ? 1.0 : 0.0);
+    to_voigt(S, s);
+    double C[3][3], Ci[3][3], CV[6], CiV[6];
+    matmul3(Ft, F, C);
+    inv3(C, Ci);
+    to_voigt(C, CV);
+    to_voigt(Ci, CiV);
+    double Sf[6] = {2.0 * C10, 2.0 * C10, 2.0 * C10, 0, 0, 0};
+    double ipSC = inner_v(Sf, CV);
+    double Siso[6], Jm23 = pow(J, -2.0 / 3.0);
+    for (int i = 0; i < 6; i++) Siso[i] = Jm23 * (Sf[i] - (1.0 / 3.0) * ipSC * CiV[i]);
+    double sq[6][6], bl[6][6], b1[6][6], b2[6][6], Cbar[6][6];
+    square_v(CiV, CiV, sq);
+    ball_v(CiV, CiV, bl);
+    ball_v(Siso, CiV, b1);
+    ball_v(CiV, Siso, b2);
+    for (int i = 0; i < 6; i++)
+        for (int j = 0; j < 6; j++) {
+            double ciso = (2.0 / 3.0) * Jm23 * ipSC * (sq[i][j] - (1.0 / 3.0) * bl[i][j]) -
+                          (2.0 / 3.0) * (b1[i][j] + b2[i][j]);
+            Cbar[i][j] = ciso + J * pbar * (bl[i][j] - 2.0 * sq[i][j]);
+        }
+    push_forward_v(Cbar, F, D);
+}
+
+/* SecondDerivativesOfPSI_Jbar_NeoHookeanQ1P0 (ModNeoHookeanQ1P0.f90:625-640) */
+static double nhq1p0_d2psi(const double *p, double Jbar) {
+    return -(p[1] * (pow(Jbar, 1.0 / 3.0) - 2.0)) / pow(Jbar, 5.0 / 3.0);
+}
+
 /* model 5: ModStVenantKirchhoff.f90:643-773 (tangent: upper triangle only) */
 static void mat_svk(const double *p, const double F[3][3], double s[6], double D[6][6]) {
     double E = p[0], nu = p[1];
@@ -881,10 +928,10 @@ static void first_piola(const double sig[6], const double F[3][3], double P[3][3
         for (int j = 0; j < 3; j++) P[i][j] *= J;
 }
 
-static int mat_update(int model, const double *p, const double F[3][3], double time,
+static int mat_update(int model, const double *p, const double F[3][3], double time, double jbar,
                       const double *old, double s[6], double *nw);
 
-static void mat_tangent_numerical(int model, const double *p, const double F0[3][3], double time,
+static void mat_tangent_numerical(int model, const double *p, const double F0[3][3], double time, double jbar,
                                   const double *old, const double sig0[6], double D[6][6]) {
     const double h = 1.0e-8;
     double A[9][9], F[3][3], s[6], nw[MAXSTATE], Pf[3][3], Pb[3][3];
@@ -892,10 +939,10 @@ static void mat_tangent_numerical(int model, const double *p, const double F0[3]
     for (int i = 0; i < 3; i++)
         for (int j = 0; j < 3; j++) {
             F[i][j] = F0[i][j] + h;
-            mat_update(model, p, F, time, old, s, nw);
+            mat_update(model, p, F, time, jbar, old, s, nw);
             first_piola(s, F, Pf);
             F[i][j] = F0[i][j] - h;
-            mat_update(model, p, F, time, old, s, nw);
+            mat_update(model, p, F, time, jbar, old, s, nw);
             first_piola(s, F, Pb);
             for (int l = 0; l < 3; l++)
                 for (int k = 0; k < 3; k++) A[i + 3 * j][k + 3 * l] = (Pf[k][l] - Pb[k][l]) / (2.0 * h);
@@ -933,6 +980,7 @@ static int mat_nprops(int model) {
     case OR_MODEL_GHL: return 2;
     case OR_MODEL_J2: return 4;
     case OR_MODEL_NEOHOOKEAN: return 2;
+    case OR_MODEL_NH_Q1P0: return 2;
     case OR_MODEL_SVK: return 2;
     case OR_MODEL_COMP_NEOHOOKEAN: return 2;
     case OR_MODEL_VISCO_MATRIX: return 6;
@@ -940,12 +988,13 @@ static int mat_nprops(int model) {
     }
 }
 
-/* stress update (UpdateStressAndStateVariables) */
-static int mat_update(int model, const double *p, const double F[3][3], double time,
+/* stress update (UpdateStressAndStateVariables); jbar: the element's V/V0 (model 4) */
+static int mat_update(int model, const double *p, const double F[3][3], double time, double jbar,
                       const double *old, double s[6], double *nw) {
     double Dtmp[6][6];
     switch (model) {
     case OR_MODEL_GHL: mat_ghl(p, F, s, Dtmp); return OR_OK;
+    case OR_MODEL_NH_Q1P0: mat_nhq1p0(p, F, jbar, s, Dtmp); return OR_OK;
     case OR_MODEL_COMP_NEOHOOKEAN: mat_cnh(p, F, s, Dtmp); return OR_OK;
     case OR_MODEL_NEOHOOKEAN: mat_nh(p, F, s, Dtmp); return OR_OK;
     case OR_MODEL_SVK: mat_svk(p, F, s, Dtmp); return OR_OK;
@@ -961,10 +1010,11 @@ static int mat_update(int model, const double *p, const double F[3][3], double t
 }
 
 /* tangent (GetTangentModulus), uses F and the state left by the last update */
-static void mat_tangent(int model, const double *p, const double F[3][3], const double *old,
+static void mat_tangent(int model, const double *p, const double F[3][3], double jbar, const double *old,
                         const double *nw, double D[6][6]) {
     double s[6];
     switch (model) {
+    case OR_MODEL_NH_Q1P0: mat_nhq1p0(p, F, jbar, s, D); return;
     case OR_MODEL_VISCO_MATRIX: {
         double F9[9];
         for (int j = 0; j < 3; j++)
@@ -989,9 +1039,10 @@ int or_material_point(int model, const double *props, const double Fv[9], double
     memcpy(F, Fv, sizeof(F));
     if (model == OR_MODEL_VISCO_MATRIX) zero[0] = zero[4] = zero[8] = 1.0; /* Fv = I */
     if (!state_old) state_old = zero;
-    int st = mat_update(model, props, F, time, state_old, sigma6, nw);
+    const double jbar = det3(F);  /* a single point: the element volume ratio is J */
+    int st = mat_update(model, props, F, time, jbar, state_old, sigma6, nw);
     if (st) return st;
-    if (D36) mat_tangent(model, props, F, state_old, nw, (double(*)[6])D36);
+    if (D36) mat_tangent(model, props, F, jbar, state_old, nw, (double(*)[6])D36);
     if (state_new) memcpy(state_new, nw, sizeof(double) * or_model_nstate(model));
     return OR_OK;
 }
@@ -1155,7 +1206,7 @@ static int solve_elem(or_mesh *m, int e, double time, const double *U) {
         int sf = or_deformation_gradient(X, nat[g], Ue, F);
         if (sf) st = sf;
         memcpy(&m->F[9 * gi], F, sizeof(F));
-        int su = mat_update(md->model, md->p, F, time, &m->st_old[MAXSTATE * gi], &m->sig[6 * gi],
+        int su = mat_update(md->model, md->p, F, time, V / VX, &m->st_old[MAXSTATE * gi], &m->sig[6 * gi],
                             &m->st_new[MAXSTATE * gi]);
         if (su) st = su;
     }
@@ -1265,14 +1316,17 @@ static int elem_ke(or_mesh *m, int e, double Ke[24][24]) {
     elem_coords(m, e, m->x, x);
     memset(Ke, 0, sizeof(double) * 576);
     const or_matdef *md = &m->mats[m->mat[e]];
+    const double jbar = m->vol[e] / m->volx[e];
+    const int q1p0 = md->model == OR_MODEL_NH_Q1P0 && m->nl;
+    double Bdiv[24] = {0};
     for (int g = 0; g < 8; g++) {
         long gi = (long)e * 8 + g;
         double D[6][6], F[3][3];
         memcpy(F, &m->F[9 * gi], sizeof(F));
         if (m->numerical_tangent)
-            mat_tangent_numerical(md->model, md->p, F, m->time, &m->st_old[MAXSTATE * gi], &m->sig[6 * gi], D);
+            mat_tangent_numerical(md->model, md->p, F, m->time, jbar, &m->st_old[MAXSTATE * gi], &m->sig[6 * gi], D);
         else
-            mat_tangent(md->model, md->p, F, &m->st_old[MAXSTATE * gi], &m->st_new[MAXSTATE * gi], D);
+            mat_tangent(md->model, md->p, F, jbar, &m->st_old[MAXSTATE * gi], &m->st_new[MAXSTATE * gi], D);
         if (or_matrix_bg(x, nat[g], B, G, &detJ)) return OR_ERR_DETJ;
         double a = w[g] * detJ;
         if (!m->nl) {
@@ -1330,6 +1384,16 @@ static int elem_ke(or_mesh *m, int e, double Ke[24][24]) {
                 for (int k = 0; k < 9; k++) s += G[k][i] * SG[k][j];
                 Ke[i][j] += a * s;
             }
+        /* mean dilatation: Bdiv = integral of div N over the current element, the rows
+         * (1,1), (2,2), (3,3) of G (ModElement.f90:429-461) */
+        if (q1p0)
+            for (int j = 0; j < 24; j++) Bdiv[j] += (G[0][j] + G[4][j] + G[8][j]) * a;
+    }
+    /* Ke += d2Psi/dJbar2 / V0 Bdiv^T Bdiv (ModElement.f90:464-467) */
+    if (q1p0) {
+        const double c = nhq1p0_d2psi(md->p, jbar) / m->volx[e];
+        for (int i = 0; i < 24; i++)
+            for (int j = 0; j < 24; j++) Ke[i][j] += c * Bdiv[i] * Bdiv[j];
     }
     return OR_OK;
 }

@@ -51,6 +51,7 @@ extern "C" {
 #define OR_MODEL_GHL 1
 #define OR_MODEL_J2 2
 #define OR_MODEL_NEOHOOKEAN 3
+#define OR_MODEL_NH_Q1P0 4   /* mean-dilatation Neo-Hookean, needs nl analysis */
 #define OR_MODEL_SVK 5
 #define OR_MODEL_COMP_NEOHOOKEAN 7
 #define OR_MODEL_VISCO_MATRIX 12

@@ -17,6 +17,10 @@ Cases (paths relative to the reference root):
       HyperMesh .cdb mesh (250 hex8), generalized Hooke's law E=100e9 nu=0.3, linear,
       1 step; CEOS displacements of node 342 and Gauss-point stresses of element 169, GP 5
       (ref_disp_*.dat, ref_stress_*.dat).
+  Checker/Cases/NonLinear_FEA_3D_Hexa8_NeoHookean_Incompressible -> nhq1p0_hex8_case.json
+      the StVenant case's mesh with its own supports and loads, Neo-Hookean C10 = 5e10,
+      K = 5e13 under the mean-dilatation (Q1P0) element technology, 50 steps; Marc
+      displacements of node 11 (checked by the reference at 1e-3) and the residual log.
 """
 import json
 import os
@@ -60,8 +64,9 @@ def _newton_log(path):
     return steps
 
 
-def svk_case(root):
-    d = os.path.join(root, "Checker/Cases/NonLinear_FEA_3D_Hexa8_StVenant")
+def _gid_case(d):
+    """Mesh, material ids, supports, nodal loads / displacements and their tables of a GiD 12
+    Mesh.dat (the reader's format, FEM/Input/ModReadInputFile.f90)."""
     lines = open(os.path.join(d, "Mesh.dat")).read().splitlines()
     i = 0
     coords, conn, mat = [], [], []
@@ -121,15 +126,7 @@ def svk_case(root):
                     out.append([node, k + 1, tab])
         return out
 
-    marc = {}
-    for c in "xyz":
-        marc[c] = _table(os.path.join(d, "marc_disp_%s.dat" % c))
     return {
-        "source": "Checker/Cases/NonLinear_FEA_3D_Hexa8_StVenant",
-        "material": {"model": 5, "E": 100.0e9, "nu": 0.3},
-        "nl_analysis": True,
-        "newton": {"tol": 1e-3, "maxiter": 10},
-        "time": _time_discretization(os.path.join(d, "Time_Discretization.dat")),
         "coords": coords,
         "conn": conn,
         "mat": mat,
@@ -137,11 +134,57 @@ def svk_case(root):
         "nodal_load": bc(loads),
         "nodal_disp": bc(disps),
         "tables": tables,
+    }
+
+
+def _marc(d):
+    return {c: _table(os.path.join(d, "marc_disp_%s.dat" % c)) for c in "xyz"}
+
+
+def svk_case(root):
+    d = os.path.join(root, "Checker/Cases/NonLinear_FEA_3D_Hexa8_StVenant")
+    g = _gid_case(d)
+    return {
+        "source": "Checker/Cases/NonLinear_FEA_3D_Hexa8_StVenant",
+        "material": {"model": 5, "E": 100.0e9, "nu": 0.3},
+        "nl_analysis": True,
+        "newton": {"tol": 1e-3, "maxiter": 10},
+        "time": _time_discretization(os.path.join(d, "Time_Discretization.dat")),
+        "coords": g["coords"],
+        "conn": g["conn"],
+        "mat": g["mat"],
+        "fixed": g["fixed"],
+        "nodal_load": g["nodal_load"],
+        "nodal_disp": g["nodal_disp"],
+        "tables": g["tables"],
         "probe_node": 11,
-        "marc_disp": marc,
+        "marc_disp": _marc(d),
         "marc_tol": 1.0e-5,
         "log_norms": _newton_log(os.path.join(d, "Log_CEOS.txt")),
     }
+
+
+def nhq1p0_case(root):
+    """Settings.dat: Neo_Hookean C10 = 5e10, bulk modulus 5e13, Element Technology = Mean
+    Dilatation (the reference selects model 4, NeoHookeanQ1P0, for that pair,
+    ModConstitutiveModelLibrary.f90:504), nonlinear, Newton tol 1e-3, 10 iterations;
+    Run_Checker.bat compares the node-11 probes with Marc at 1e-3."""
+    d = os.path.join(root, "Checker/Cases/NonLinear_FEA_3D_Hexa8_NeoHookean_Incompressible")
+    g = _gid_case(d)
+    out = {
+        "source": "Checker/Cases/NonLinear_FEA_3D_Hexa8_NeoHookean_Incompressible",
+        "material": {"model": 4, "C10": 5.0e10, "K": 5.0e13},
+        "element_technology": "mean dilatation",
+        "nl_analysis": True,
+        "newton": {"tol": 1e-3, "maxiter": 10},
+        "time": _time_discretization(os.path.join(d, "Time_Discretization.dat")),
+        "probe_node": 11,
+        "marc_disp": _marc(d),
+        "marc_tol": 1.0e-3,
+        "log_norms": _newton_log(os.path.join(d, "Log_CEOS.txt")),
+    }
+    out.update(g)
+    return out
 
 
 def ghl_case(root):
@@ -214,7 +257,8 @@ def ghl_case(root):
 def main():
     root = sys.argv[1] if len(sys.argv) > 1 else "/root/reference"
     here = os.path.dirname(os.path.abspath(__file__))
-    for name, fn in [("svk_hex8_case.json", svk_case), ("ghl_hex8_case.json", ghl_case)]:
+    for name, fn in [("svk_hex8_case.json", svk_case), ("ghl_hex8_case.json", ghl_case),
+                     ("nhq1p0_hex8_case.json", nhq1p0_case)]:
         data = fn(root)
         with open(os.path.join(here, name), "w") as f:
             json.dump(data, f, separators=(",", ":"))

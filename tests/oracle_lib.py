@@ -17,6 +17,7 @@ _dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 _ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
 
 MODEL_GHL, MODEL_J2, MODEL_NH, MODEL_SVK, MODEL_CNH, MODEL_VISCO = 1, 2, 3, 5, 7, 12
+MODEL_NH_Q1P0 = 4  # mean dilatation (Q1P0) Neo-Hookean
 STATE_STRIDE = 20  # per Gauss point in or_get_state / or_set_state
 
 _lib = None

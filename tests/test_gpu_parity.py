@@ -26,11 +26,13 @@ MATS = {
     "svk": (ol.MODEL_SVK, [100.0, 0.3], True),
     "ghl": (ol.MODEL_GHL, [210000.0, 0.3], False),
     "j2": (ol.MODEL_J2, [210000.0, 0.3, 250.0, 1000.0], False),
+    "nhq1p0": (ol.MODEL_NH_Q1P0, [0.5, 100.0], True),  # mean dilatation
 }
 
 
 def pair(X, conn, model, props, nl, mat_id=None, assembly=gpu.ASSEMBLY_GATHER):
-    g = gpu.CeosMesh(X, conn, mat_id=mat_id, nl_analysis=nl, assembly=assembly)
+    tech = gpu.ELEM_MEAN_DILATATION if model == ol.MODEL_NH_Q1P0 else gpu.ELEM_FULL_INTEGRATION
+    g = gpu.CeosMesh(X, conn, mat_id=mat_id, nl_analysis=nl, assembly=assembly, elem_tech=tech)
     o = ol.OracleMesh(X, conn, mat_id=mat_id, nl_analysis=nl)
     ids = [1] if mat_id is None else sorted(set(int(m) for m in mat_id))
     for i in ids:
@@ -182,6 +184,30 @@ def test_svk_checker_case_on_gpu():
     g.set_material(1, ol.MODEL_SVK, [case["material"]["E"], case["material"]["nu"]])
     case, norms, probe = run_svk_case(g)
     check_svk(case, norms, probe)
+
+
+def test_nhq1p0_checker_case_on_gpu():
+    """NonLinear_FEA_3D_Hexa8_NeoHookean_Incompressible (mean dilatation): the reference's
+    residual log and the Marc displacements, through the GPU element path."""
+    from test_oracle_golden import _case, check_nhq1p0, run_nhq1p0_case
+    case = _case("nhq1p0_hex8_case.json")
+    g = gpu.CeosMesh(np.array(case["coords"]), np.array(case["conn"], dtype=np.int32),
+                     elem_tech=gpu.ELEM_MEAN_DILATATION)
+    g.set_material(1, ol.MODEL_NH_Q1P0, [case["material"]["C10"], case["material"]["K"]])
+    case, norms, probe = run_nhq1p0_case(g)
+    check_nhq1p0(case, norms, probe)
+
+
+def test_mean_dilatation_pairs_with_model_4_only():
+    """The reference's library gives the mean-dilatation technology its Q1P0 models only."""
+    X, conn = meshgen.cube(2)
+    U = np.zeros(3 * len(X))
+    g = gpu.CeosMesh(X, conn)
+    g.set_material(1, ol.MODEL_NH_Q1P0, [0.5, 100.0])
+    assert g.solve_constitutive(0.0, U) == gpu.ERR_ARG
+    h = gpu.CeosMesh(X, conn, elem_tech=gpu.ELEM_MEAN_DILATATION)
+    h.set_material(1, ol.MODEL_NH, [0.5, 100.0])
+    assert h.solve_constitutive(0.0, U) == gpu.ERR_ARG
 
 
 @pytest.mark.parametrize("model", [ol.MODEL_GHL, ol.MODEL_J2])

@@ -27,15 +27,23 @@ def rel(a, b):
 
 
 class DeviceSystem:
-    """A CeosMesh whose Newton system (BC, norm, linear solve) runs in the library."""
+    """A CeosMesh whose Newton system (BC, norm, linear solve) runs in the library.
 
-    def __init__(self, mesh, rtol=1e-12, norm_type=2):
+    indefinite_fallback: when PCG reports a breakdown (p.Kp <= 0), solve that system on the
+    host with the sparse direct solver, as a caller keeping a symmetric-indefinite solver would
+    (the reference's PARDISO runs mtype -2, symmetric indefinite); the fallback first checks
+    that the matrix has a negative eigenvalue, so a breakdown on a positive definite matrix
+    still fails."""
+
+    def __init__(self, mesh, rtol=1e-12, norm_type=2, indefinite_fallback=False):
         self.m = mesh
         self.ndof = mesh.ndof
         self.rtol = rtol
         self.norm_type = norm_type
+        self.fallback = indefinite_fallback
         self._bc = None
         self.pcg = []
+        self.fallbacks = 0
 
     def __getattr__(self, k):
         return getattr(self.m, k)
@@ -48,6 +56,12 @@ class DeviceSystem:
 
     def newton_solve(self, b):
         x, its, rr, flag = self.m.linear_solve(b, self.rtol, 50000)
+        if flag == 2 and self.fallback:
+            rm, col = self.m.csr()
+            A = fd.sym_from_upper(rm, col, self.m.tangent(), self.ndof)
+            assert np.linalg.eigvalsh(A.toarray())[0] < 0.0, "PCG breakdown on a positive definite matrix"
+            self.fallbacks += 1
+            return fd.spla.spsolve(A, b)
         assert flag == 1, (its, rr, flag)
         self.pcg.append(its)
         return x
@@ -186,6 +200,24 @@ def test_golden_svk_case_through_device_system():
     case, norms, probe = run_svk_case(ds)
     check_svk(case, norms, probe)
     print("PCG iterations per solve: max %d mean %.0f" % (max(ds.pcg), np.mean(ds.pcg)))
+
+
+def test_golden_q1p0_case_through_device_system():
+    """The reference's mean-dilatation Checker case (K / mu = 1000) with the library's BC,
+    norms and PCG: the logged residual history and the Marc displacements.  The first Newton
+    iterate of most steps of this case has an indefinite tangent (step 2, iteration 1: one
+    negative eigenvalue; 49 of 206 systems); PCG reports the breakdown there and the caller
+    solves that system directly."""
+    from test_oracle_golden import _case, check_nhq1p0, run_nhq1p0_case
+    case = _case("nhq1p0_hex8_case.json")
+    g = gpu.CeosMesh(np.array(case["coords"]), np.array(case["conn"], dtype=np.int32),
+                     elem_tech=gpu.ELEM_MEAN_DILATATION)
+    g.set_material(1, ol.MODEL_NH_Q1P0, [case["material"]["C10"], case["material"]["K"]])
+    ds = DeviceSystem(g, rtol=1e-14, indefinite_fallback=True)
+    case, norms, probe = run_nhq1p0_case(ds)
+    check_nhq1p0(case, norms, probe)
+    print("PCG iterations per solve: max %d mean %.0f; %d indefinite systems solved directly"
+          % (max(ds.pcg), np.mean(ds.pcg), ds.fallbacks))
 
 
 def test_c1_newton_history_through_device_system_matches_oracle():

@@ -69,6 +69,53 @@ def test_svk_hex8_residual_history_and_marc_displacements():
     check_svk(case, norms, probe)
 
 
+def run_nhq1p0_case(backend, nsteps=50):
+    case = _case("nhq1p0_hex8_case.json")
+    bc = fd.bc_from_case(case)
+    norms = []
+    probe = {}
+    n = case["probe_node"]
+
+    def on_step(lc, st, t, U, its):
+        probe[st] = U[3 * (n - 1):3 * n].copy()
+
+    fd.quasi_static(backend, bc, [(0.0, nsteps / 50.0, nsteps)], nl=True, tol=1e-3, maxiter=10,
+                    on_step=on_step, norms=norms, absolute=True)
+    return case, norms, probe
+
+
+def check_nhq1p0(case, norms, probe, nsteps=50):
+    """The printed residual histories of the first three steps: the same iteration counts, every
+    norm within 2e-8 of the 9 printed digits plus a round-off floor of 1e-12 times the step's
+    first norm (K / mu = 1000: the last iterations sit at the level where two direct solvers'
+    rounding differs; PARDISO wrote the log, SuperLU solves here); the Marc displacements of
+    node 11 at the reference's own tolerance, 1e-3, for every step run."""
+    for step in range(3):
+        log = case["log_norms"][step]
+        mine = norms[step]
+        assert len(mine) == len(log), (step, mine, log)
+        for k in range(len(log)):
+            tol = 2e-8 * abs(log[k]) + 1e-12 * abs(log[0])
+            assert abs(mine[k] - log[k]) <= tol, (step, k, mine[k], log[k])
+    for st, u in probe.items():
+        ref = [case["marc_disp"][c][st][1] for c in "xyz"]
+        assert np.max(np.abs(u - ref)) < case["marc_tol"], (st, u, ref)
+    assert len(probe) == nsteps
+
+
+def nhq1p0_oracle():
+    case = _case("nhq1p0_hex8_case.json")
+    m = ol.OracleMesh(np.array(case["coords"]), np.array(case["conn"], dtype=np.int32))
+    m.set_material(1, ol.MODEL_NH_Q1P0, [case["material"]["C10"], case["material"]["K"]])
+    return m
+
+
+def test_nhq1p0_mean_dilatation_residual_history_and_marc_displacements():
+    """NonLinear_FEA_3D_Hexa8_NeoHookean_Incompressible: Q1P0 element technology."""
+    case, norms, probe = run_nhq1p0_case(nhq1p0_oracle())
+    check_nhq1p0(case, norms, probe)
+
+
 def ghl_backend(make, model):
     case = _case("ghl_hex8_case.json")
     X, conn = np.array(case["coords"]), np.array(case["conn"], dtype=np.int32)
