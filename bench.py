@@ -344,10 +344,10 @@ def newton_bench(no_cpu, n_cells=N_CELLS, n_small=40, ux=0.04, tol=1e-8, pcg_rto
     recs, total, nn, width = res[n_cells]
     pcg_its = sum(r.get("pcg_iterations", 0) for r in recs)
     solve_ms = sum(r.get("solve_ms", 0.0) for r in recs)
-    # algorithmic bytes of one PCG iteration: the ELL values (FP32) and neighbour ids, the two
-    # gathered vectors read and p, q written by the product; p, q, x, r read, x, r, z written and
-    # the 3x3 block inverses read by the update
-    bytes_it = width * 9 * nn * 4 + width * nn * 4 + 4 * 24 * nn + 7 * 24 * nn + 48 * nn
+    # algorithmic bytes of one PCG iteration: k_cg_spmv reads the SELL values (FP32) and neighbour
+    # ids, p (gathered and its own rows) and writes q; k_cg_update reads p, q, x, r and the 3x3
+    # block inverses and writes x, r, z; k_cg_pupdate reads z, p and writes p
+    bytes_it = width * 9 * nn * 4 + width * nn * 4 + 3 * 24 * nn + 7 * 24 * nn + 48 * nn + 3 * 24 * nn
     ms_it = solve_ms / max(pcg_its, 1)
     gbs = bytes_it / (ms_it * 1e-3) / 1e9
     out.update({
@@ -361,7 +361,7 @@ def newton_bench(no_cpu, n_cells=N_CELLS, n_small=40, ux=0.04, tol=1e-8, pcg_rto
         "evaluate_ms": sum(r["evaluate_ms"] for r in recs),
         "bc_norm_ms": sum(r["bc_norm_ms"] for r in recs),
         "solve_ms": solve_ms,
-        "roofline_pcg": {"bound": "hbm", "kernel": "PCG iteration (k_cg_spmv + k_cg_update + 2 reductions)",
+        "roofline_pcg": {"bound": "hbm", "kernel": "PCG iteration (k_cg_spmv + k_cg_update + k_cg_pupdate)",
                          "achieved": round(gbs, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                          "frac": round(gbs / PEAK_HBM_GBS, 4), "ms_per_iteration": round(ms_it, 4),
                          "bytes_per_iteration": bytes_it},

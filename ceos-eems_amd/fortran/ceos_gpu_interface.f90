@@ -4,7 +4,8 @@
 !
 ! The element loops of the reference (SolveConstitutiveModel, InternalForce,
 ! TangentStiffnessMatrix, UpdateMeshCoordinates -- FEM/Modules/ModuleInterfaces.f90:132-270)
-! are replaced by these calls; Newton, Dirichlet BCs and PARDISO stay as they are.
+! are replaced by these calls; the Newton loop stays as it is.  Dirichlet BCs and the PARDISO
+! solve either stay too, or move to the device (ceos_bc_set, ceos_residual_bc, ceos_linear_solve).
 ! Arrays are passed by reference with the reference's own layouts:
 !   U, Fint(nDOF)           DOF = 3*(node-1)+k
 !   Kg%Val(nnz), Kg%RowMap, Kg%Col   1-based upper CSR (ModGlobalSparseMatrix.f90:18-25)
@@ -19,6 +20,8 @@ module ceos_gpu_interface
     integer(c_int), parameter :: CEOS_K_FP64 = 0, CEOS_K_FP32_STORED = 1
     integer(c_int), parameter :: CEOS_TANGENT_ANALYTIC = 0, CEOS_TANGENT_NUMERICAL = 1
     integer(c_int), parameter :: CEOS_STATE_STRIDE = 20
+    integer(c_int), parameter :: CEOS_ELEM_FULL_INTEGRATION = 1, CEOS_ELEM_MEAN_DILATATION = 2
+    integer(c_int), parameter :: CEOS_NORM_L2 = 1, CEOS_NORM_MAX_ABS = 2
 
     ! mirrors ClassStatus (Tools/ModStatus.f90:3-16)
     type, bind(C) :: ceos_status
@@ -157,6 +160,36 @@ module ceos_gpu_interface
             import :: c_int, c_char, c_ptr
             type(c_ptr), value :: h
             character(kind=c_char), intent(in) :: path(*)
+        end function
+
+        ! Newton system on the device: constrained DOFs (1-based), ApplyBoundaryConditionsNEW +
+        ! the residual norm, and the linear solve that replaces the PARDISO call
+        integer(c_int) function ceos_bc_set(h, n_presc, presc_dof, n_fixed, fixed_dof) bind(C, name="ceos_bc_set")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: h
+            integer(c_int), value :: n_presc, n_fixed
+            integer(c_int), intent(in) :: presc_dof(*), fixed_dof(*)
+        end function
+
+        integer(c_int) function ceos_residual_bc(h, it, Fext, Ubar, R, norm_type, normR) bind(C, name="ceos_residual_bc")
+            import :: c_int, c_double, c_ptr
+            type(c_ptr), value :: h
+            integer(c_int), value :: it, norm_type
+            real(c_double), intent(in) :: Fext(*), Ubar(*)
+            real(c_double), intent(out) :: R(*)
+            real(c_double), intent(out) :: normR
+        end function
+
+        integer(c_int) function ceos_linear_solve(h, b, x, rtol, maxit, iters, relres, flag) &
+                bind(C, name="ceos_linear_solve")
+            import :: c_int, c_double, c_ptr
+            type(c_ptr), value :: h
+            real(c_double), intent(in) :: b(*)
+            real(c_double), intent(out) :: x(*)
+            real(c_double), value :: rtol
+            integer(c_int), value :: maxit
+            integer(c_int), intent(out) :: iters, flag
+            real(c_double), intent(out) :: relres
         end function
     end interface
 
