@@ -26,6 +26,7 @@ MODEL_GHL, MODEL_J2, MODEL_NH, MODEL_SVK, MODEL_CNH, MODEL_VISCO = 1, 2, 3, 5, 7
 MODEL_EXT = 99  # FE^2 macro point: S and dS/dE per Gauss point from the caller
 MODEL_NH_Q1P0 = 4  # mean-dilatation Neo-Hookean (with ELEM_MEAN_DILATATION)
 ELEM_FULL_INTEGRATION, ELEM_MEAN_DILATATION = 1, 2
+SOLVER_AUTO, SOLVER_PCG, SOLVER_MINRES = 0, 1, 2
 EXT_PER_GP = 27
 STATE_STRIDE = 20
 K_FP64, K_FP32_STORED = 0, 1
@@ -68,7 +69,7 @@ EXPORTS = [
     "ceos_mlp_gradients_weighted", "ceos_mlp_train_step_weighted", "ceos_mlp_set_training_precision",
     "ceos_set_pipeline", "ceos_rve_autotune", "ceos_rve_set_cg_group",
     "ceos_bc_set", "ceos_residual_bc", "ceos_residual_bc_device", "ceos_set_solver_precision",
-    "ceos_linear_solve", "ceos_linear_solve_device", "ceos_newton_update_device",
+    "ceos_linear_solve", "ceos_linear_solve_device", "ceos_newton_update_device", "ceos_set_linear_solver",
 ]
 
 
@@ -164,6 +165,7 @@ def load_library(path=LIB_PATH):
     L.ceos_residual_bc.argtypes = [vp, i, vp, vp, vp, i, C.POINTER(d)]
     L.ceos_residual_bc_device.argtypes = [vp, i, vp, vp, vp, vp, vp, vp, i, vp, vp]
     L.ceos_set_solver_precision.argtypes = [vp, i]
+    L.ceos_set_linear_solver.argtypes = [vp, i]
     L.ceos_linear_solve.argtypes = [vp, _dp, _dp, d, i, C.POINTER(i), C.POINTER(d), C.POINTER(i)]
     L.ceos_linear_solve_device.argtypes = [vp, vp, vp, vp, d, i, C.POINTER(i), C.POINTER(d), C.POINTER(i), vp]
     L.ceos_newton_update_device.argtypes = [vp, vp, vp, d, vp]
@@ -373,6 +375,9 @@ class CeosMesh:
     def residual_bc_device(self, it, dU, dFint, dK, dFext, dUbar, dR, dnorm, norm_type=2, stream=None):
         _check(load_library().ceos_residual_bc_device(self._h, it, dU, dFint, dK, dFext, dUbar, dR, norm_type,
                                                       dnorm, stream), "ceos_residual_bc_device")
+
+    def set_linear_solver(self, method):
+        _check(load_library().ceos_set_linear_solver(self._h, method), "ceos_set_linear_solver")
 
     def set_solver_precision(self, mode):
         _check(load_library().ceos_set_solver_precision(self._h, mode), "ceos_set_solver_precision")

@@ -165,7 +165,10 @@ struct ceos_handle {
     double* dcg_p = nullptr;
     double *dcg_b = nullptr, *dcg_x = nullptr;  // staging of the host solve call
     CgScalars* dcg = nullptr;
-    int cg_check = 32;                          // iterations between host checks of the PCG state
+    double *dmr[3] = {nullptr, nullptr, nullptr};  // MINRES: the third r buffer, v, and the second w
+    MrScalars* dmr_c = nullptr;
+    int solver_method = CEOS_SOLVER_AUTO;
+    int cg_check = 32;                          // iterations between host checks of the Krylov state
 
     // phase timing: 3 events per evaluation (start, element kernels done, assembly done)
     std::vector<hipEvent_t> tev;
@@ -263,6 +266,7 @@ static void free_device(ceos_handle* h) {
     h->dell = nullptr;
     dfree(h->dMinv); dfree(h->dcg_r); dfree(h->dcg_z); dfree(h->dcg_q); dfree(h->dcg_p);
     dfree(h->dcg_b); dfree(h->dcg_x); dfree(h->dcg); dfree(h->dpart3);
+    dfree(h->dmr[0]); dfree(h->dmr[1]); dfree(h->dmr[2]); dfree(h->dmr_c);
 }
 
 static int upload_props(ceos_handle* h) {
@@ -880,21 +884,23 @@ static int ensure_solver(ceos_handle* h) {
     return CEOS_OK;
 }
 
+static int ensure_minres(ceos_handle* h) {
+    if (h->dmr_c) return CEOS_OK;
+    const long n = 3L * h->nn;
+    for (int i = 0; i < 3; ++i) HIP_OK(dalloc(h->dmr[i], n));
+    HIP_OK(dalloc(h->dmr_c, 1));
+    return CEOS_OK;
+}
+
 // x = K^-1 b by block-Jacobi PCG from x = 0; K (upper CSR, the handle's precision) after the
 // boundary conditions.  Synchronises s every cg_check iterations to read the state.
-static int linear_solve(ceos_handle* h, const void* dK, const double* db, double* dx, double rtol, int maxit,
-                        hipStream_t s, int* iters, double* relres, int* flag) {
-    if (!h->sys_ready || !dK || !db || !dx || !(rtol > 0.0) || maxit < 0) return CEOS_ERR_ARG;
-    int rc = ensure_solver(h);
-    if (rc) return rc;
+// the SELL copy of the BC-applied K and the block-Jacobi inverses
+static void build_operator(ceos_handle* h, const void* dK, hipStream_t s) {
     const SysGeom g = sys_geom(h);
-    const int nn = h->nn;
-    const int nblk = (nn + kSysBlock - 1) / kSysBlock;
-    const long nt = (long)h->sys_width * nn;
+    const int nblk = (h->nn + kSysBlock - 1) / kSysBlock;
+    const long nt = (long)h->sys_width * h->nn;
     const unsigned nbe = (unsigned)((nt + kSysBlock - 1) / kSysBlock);
-    const bool kf32 = h->precision == CEOS_K_FP32_STORED;
-    // ELL copy and preconditioner
-    if (kf32) {
+    if (h->precision == CEOS_K_FP32_STORED) {
         if (h->ell_f32) hipLaunchKernelGGL((k_build_ell<float, float>), dim3(nbe), dim3(kSysBlock), 0, s, g, (const float*)dK, (float*)h->dell);
         else hipLaunchKernelGGL((k_build_ell<float, double>), dim3(nbe), dim3(kSysBlock), 0, s, g, (const float*)dK, (double*)h->dell);
         hipLaunchKernelGGL(k_block_jacobi<float>, dim3(nblk), dim3(kSysBlock), 0, s, g, (const float*)dK, h->dMinv);
@@ -903,6 +909,14 @@ static int linear_solve(ceos_handle* h, const void* dK, const double* db, double
         else hipLaunchKernelGGL((k_build_ell<double, double>), dim3(nbe), dim3(kSysBlock), 0, s, g, (const double*)dK, (double*)h->dell);
         hipLaunchKernelGGL(k_block_jacobi<double>, dim3(nblk), dim3(kSysBlock), 0, s, g, (const double*)dK, h->dMinv);
     }
+}
+
+// block-Jacobi PCG from x = 0.  Synchronises s every cg_check iterations to read the state.
+static int pcg_run(ceos_handle* h, const double* db, double* dx, double rtol, int maxit, hipStream_t s, int* iters,
+                   double* relres, int* flag) {
+    const SysGeom g = sys_geom(h);
+    const int nn = h->nn;
+    const int nblk = (nn + kSysBlock - 1) / kSysBlock;
     CgScalars c0;
     memset(&c0, 0, sizeof(c0));
     c0.tol2 = rtol * rtol;
@@ -941,6 +955,89 @@ static int linear_solve(ceos_handle* h, const void* dK, const double* db, double
     if (iters) *iters = c.iters;
     if (relres) *relres = c.bb > 0.0 ? sqrt(c.rr / c.bb) : 0.0;
     if (flag) *flag = c.done;
+    return CEOS_OK;
+}
+
+// block-Jacobi preconditioned MINRES from x = 0 (symmetric indefinite K).  r1, r2, r_new rotate
+// through dcg_r, dcg_q, dmr[0]; z = dcg_z; v = dmr[1]; w through dcg_p and dmr[2].
+static int minres_run(ceos_handle* h, const double* db, double* dx, double rtol, int maxit, hipStream_t s,
+                      int* iters, double* relres, int* flag) {
+    int rc = ensure_minres(h);
+    if (rc) return rc;
+    const SysGeom g = sys_geom(h);
+    const int nn = h->nn;
+    const int nblk = (nn + kSysBlock - 1) / kSysBlock;
+    double* R[3] = {h->dcg_r, h->dcg_q, h->dmr[0]};
+    double* W[2] = {h->dcg_p, h->dmr[2]};
+    double* z = h->dcg_z;
+    double* v = h->dmr[1];
+    MrScalars c0;
+    memset(&c0, 0, sizeof(c0));
+    c0.rtol = rtol;
+    c0.maxit = maxit;
+    HIP_OK(hipMemcpyAsync(h->dmr_c, &c0, sizeof(c0), hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_mr_init, dim3(nblk), dim3(kSysBlock), 0, s, nn, (const double*)h->dMinv, db, dx, R[0], R[1], z,
+                       W[0], W[1], h->dpart);
+    hipLaunchKernelGGL(k_mr_init_final, dim3(1), dim3(kSysBlock), 0, s, (const double*)h->dpart, nblk, h->dmr_c);
+    MrScalars c;
+    int k = 0;
+    for (;;) {
+        HIP_OK(hipMemcpyAsync(&c, h->dmr_c, sizeof(c), hipMemcpyDeviceToHost, s));
+        HIP_OK(hipStreamSynchronize(s));
+        if (c.done || c.verdict) break;
+        for (int i = 0; i < h->cg_check; ++i, ++k) {
+            const int par = k & 1;
+            double* r1 = R[k % 3];
+            double* r2 = R[(k + 1) % 3];
+            double* rn = R[(k + 2) % 3];
+            double* wkm2 = W[k & 1];  // w_{k-2}; w_k replaces it
+            double* wkm1 = W[(k + 1) & 1];
+            if (h->ell_f32)
+                hipLaunchKernelGGL((k_mr_spmv<float, kCgSlots>), dim3(nblk), dim3(kSysBlock), 0, s, g,
+                                   (const float*)h->dell, h->dmr_c, par, (const double*)z,
+                                   (const double*)r1, v, rn, h->dpart3);
+            else
+                hipLaunchKernelGGL((k_mr_spmv<double, kCgSlots64>), dim3(nblk), dim3(kSysBlock), 0, s, g,
+                                   (const double*)h->dell, h->dmr_c, par, (const double*)z,
+                                   (const double*)r1, v, rn, h->dpart3);
+            hipLaunchKernelGGL(k_mr_lanczos, dim3(nblk), dim3(kSysBlock), 0, s, nn, par, (const double*)h->dMinv,
+                               h->dmr_c, (const double*)h->dpart3, nblk, (const double*)r2, rn, z, h->dpart);
+            hipLaunchKernelGGL(k_mr_update, dim3(nblk), dim3(kSysBlock), 0, s, nn, par, h->dmr_c,
+                               (const double*)h->dpart, nblk, (const double*)v, (const double*)wkm1, wkm2, dx);
+        }
+        HIP_OK(hipGetLastError());
+    }
+    const int verdict = c.done ? c.done : c.verdict;
+    if (iters) *iters = c.iters;
+    if (relres) *relres = c.beta1 > 0.0 ? c.st[c.iters & 1].phibar / c.beta1 : 0.0;
+    if (flag) *flag = verdict == 1 ? 4 : verdict;
+    return CEOS_OK;
+}
+
+// x = K^-1 b from x = 0; K (upper CSR, the handle's precision) after the boundary conditions.
+// CEOS_SOLVER_PCG: PCG only (flag 2 on an indefinite K); CEOS_SOLVER_MINRES: MINRES only;
+// CEOS_SOLVER_AUTO: PCG, and MINRES from x = 0 when PCG breaks down.
+static int linear_solve(ceos_handle* h, const void* dK, const double* db, double* dx, double rtol, int maxit,
+                        hipStream_t s, int* iters, double* relres, int* flag) {
+    if (!h->sys_ready || !dK || !db || !dx || !(rtol > 0.0) || maxit < 0) return CEOS_ERR_ARG;
+    int rc = ensure_solver(h);
+    if (rc) return rc;
+    build_operator(h, dK, s);
+    int it = 0, fl = 0;
+    double rr = 0.0;
+    if (h->solver_method != CEOS_SOLVER_MINRES) {
+        rc = pcg_run(h, db, dx, rtol, maxit, s, &it, &rr, &fl);
+        if (rc) return rc;
+    }
+    if (h->solver_method == CEOS_SOLVER_MINRES || (h->solver_method == CEOS_SOLVER_AUTO && fl == 2)) {
+        int it2 = 0;
+        rc = minres_run(h, db, dx, rtol, maxit, s, &it2, &rr, &fl);
+        if (rc) return rc;
+        it += it2;
+    }
+    if (iters) *iters = it;
+    if (relres) *relres = rr;
+    if (flag) *flag = fl;
     return CEOS_OK;
 }
 
@@ -1523,6 +1620,13 @@ int ceos_residual_bc(ceos_handle* h, int it, const double* Fext, const double* U
     if (R) HIP_OK(hipMemcpyAsync(R, h->dR, sizeof(double) * n, hipMemcpyDeviceToHost, h->stream));
     HIP_OK(hipStreamSynchronize(h->stream));
     if (normR) *normR = nrm[2];
+    return CEOS_OK;
+}
+
+int ceos_set_linear_solver(ceos_handle* h, int method) {
+    if (!h || (method != CEOS_SOLVER_AUTO && method != CEOS_SOLVER_PCG && method != CEOS_SOLVER_MINRES))
+        return CEOS_ERR_ARG;
+    h->solver_method = method;
     return CEOS_OK;
 }
 

@@ -253,8 +253,10 @@ __global__ __launch_bounds__(kSysBlock) void k_build_ell(SysGeom g, const KT* __
     for (int m = 0; m < 9; ++m) val[sell_val(g.width, A, j, m)] = (VT)b[m];
 }
 
-// inverse of every node's diagonal block (symmetric, stored 00 01 02 11 12 22 as [6][nn]);
-// a singular block is replaced by the identity
+// inverse of every node's diagonal block (symmetric, stored 00 01 02 11 12 22 as [6][nn]).
+// The preconditioner must be positive definite (MINRES needs it; PCG breaks down without it):
+// a block that is not (leading minors not all positive) is replaced by the inverse of its
+// absolute diagonal, a zero diagonal entry by 1.
 template <typename KT>
 __global__ __launch_bounds__(kSysBlock) void k_block_jacobi(SysGeom g, const KT* __restrict__ K, double* __restrict__ Minv) {
     const int A = blockIdx.x * kSysBlock + threadIdx.x;
@@ -266,9 +268,12 @@ __global__ __launch_bounds__(kSysBlock) void k_block_jacobi(SysGeom g, const KT*
     const double c02 = b[1] * b[5] - b[2] * b[4];
     const double det = b[0] * c00 + b[3] * c01 + b[6] * c02;
     const long nn = g.nn;
-    if (!(fabs(det) > 0.0) || !isfinite(det)) {
-        Minv[A] = 1.0; Minv[nn + A] = 0.0; Minv[2 * nn + A] = 0.0;
-        Minv[3 * nn + A] = 1.0; Minv[4 * nn + A] = 0.0; Minv[5 * nn + A] = 1.0;
+    const double m2 = b[0] * b[4] - b[1] * b[3];
+    if (!(b[0] > 0.0 && m2 > 0.0 && det > 0.0) || !isfinite(det)) {
+        const double d0 = fabs(b[0]), d1 = fabs(b[4]), d2 = fabs(b[8]);
+        Minv[A] = d0 > 0.0 ? 1.0 / d0 : 1.0; Minv[nn + A] = 0.0; Minv[2 * nn + A] = 0.0;
+        Minv[3 * nn + A] = d1 > 0.0 ? 1.0 / d1 : 1.0; Minv[4 * nn + A] = 0.0;
+        Minv[5 * nn + A] = d2 > 0.0 ? 1.0 / d2 : 1.0;
         return;
     }
     const double id = 1.0 / det;
@@ -485,6 +490,225 @@ __global__ __launch_bounds__(kSysBlock) void k_cg_pupdate(int nn, int par, CgSca
     if (A < nn) {
 #pragma unroll
         for (int i = 0; i < 3; ++i) p[3L * A + i] = z[3L * A + i] + beta * p[3L * A + i];
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Preconditioned MINRES (Paige & Saunders 1975, in the form of the recurrences below) for the
+// symmetric indefinite tangents on which PCG breaks down.  M is the block-Jacobi operator above
+// (positive definite).  Per iteration k, with z = M^-1 r2 and beta = sqrt(r2 . z) from before:
+//   k_mr_spmv:    v = z / beta, y = K v - (beta / beta_old) r1, partials of v . y (= alpha)
+//   k_mr_lanczos: r_new = y - (alpha / beta) r2, z = M^-1 r_new, partials of r_new . z
+//   k_mr_update:  beta_new = sqrt(r_new . z), the next Givens rotation, w = (v - eps w_{k-2} -
+//                 delta w_{k-1}) / gamma, x += phi w; stop when phibar <= rtol beta_1 (the
+//                 residual in the M^-1 norm).
+// r1, r2, r_new rotate through 3 buffers; w_k overwrites w_{k-2}, so w takes 2.  The rotation
+// state of iteration k is st[k & 1]: every workgroup of k_mr_update reads st[k & 1] and
+// workgroup 0 writes st[(k + 1) & 1].  k_mr_update updates x, so it must not stop itself
+// half-way: it leaves its verdict in `verdict`, and the next k_mr_spmv (which writes no
+// solution) turns it into `done`.
+// ---------------------------------------------------------------------------------------------
+struct MrState {
+    double beta, oldb;            // beta_k, beta_{k-1}
+    double cs, sn, dbar, epsln;   // rotation carried into iteration k
+    double phibar;
+    double pad;
+};
+
+struct MrScalars {
+    MrState st[2];
+    double alpha;      // alpha of the current iteration (k_mr_lanczos, read by k_mr_update)
+    double beta1;      // sqrt(b . M^-1 b)
+    double rtol;
+    int done;          // 1 converged, 2 preconditioner not positive (r . z < 0), 3 iteration limit
+    int verdict;       // the same codes, set by k_mr_update, effective from the next k_mr_spmv
+    int iters;
+    int maxit;
+};
+
+__device__ inline bool mr_stopped(const MrScalars* c) {
+    __shared__ int s_done;
+    if (threadIdx.x == 0) s_done = __atomic_load_n(&c->done, __ATOMIC_RELAXED);
+    __syncthreads();
+    return s_done != 0;
+}
+
+// x = 0, r2 = b, z = M^-1 b, w's = 0; partials of b . z
+__global__ __launch_bounds__(kSysBlock) void k_mr_init(int nn, const double* __restrict__ Minv, const double* __restrict__ b,
+                                                       double* x, double* r1, double* r2, double* z, double* w1,
+                                                       double* w2, double* part) {
+    const int A = blockIdx.x * kSysBlock + threadIdx.x;
+    double bz = 0.0;
+    if (A < nn) {
+        double rv[3], zv[3];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) rv[i] = b[3L * A + i];
+        precond(Minv, nn, A, rv, zv);
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            x[3L * A + i] = 0.0;
+            r1[3L * A + i] = rv[i];  // unused at k = 0 (beta / beta_old = 0)
+            r2[3L * A + i] = rv[i];
+            z[3L * A + i] = zv[i];
+            w1[3L * A + i] = 0.0;
+            w2[3L * A + i] = 0.0;
+            bz += rv[i] * zv[i];
+        }
+    }
+    block_sum_max(bz, 0.0, part, nullptr);
+}
+
+__global__ __launch_bounds__(kSysBlock) void k_mr_init_final(const double* part, int n, MrScalars* c) {
+    double bz, unused;
+    reduce2(part, nullptr, n, bz, unused);
+    if (threadIdx.x == 0) {
+        MrState& s0 = c->st[0];
+        c->iters = 0;
+        c->verdict = 0;
+        if (!(bz >= 0.0)) {
+            c->done = 2;
+            return;
+        }
+        const double b1 = sqrt(bz);
+        c->beta1 = b1;
+        s0.beta = b1;
+        s0.oldb = 0.0;
+        s0.cs = -1.0;
+        s0.sn = 0.0;
+        s0.dbar = 0.0;
+        s0.epsln = 0.0;
+        s0.phibar = b1;
+        c->done = (b1 == 0.0) ? 1 : 0;
+        if (!c->done && c->maxit <= 0) c->done = 3;
+    }
+}
+
+template <typename VT, int G>
+__global__ __launch_bounds__(kSysBlock) void k_mr_spmv(SysGeom g, const VT* __restrict__ val, MrScalars* c, int par,
+                                                       const double* __restrict__ z, const double* __restrict__ r1,
+                                                       double* __restrict__ v, double* __restrict__ y, double* part) {
+    if (mr_stopped(c)) return;
+    if (blockIdx.x == 0 && threadIdx.x == 0 && c->verdict) __atomic_store_n(&c->done, c->verdict, __ATOMIC_RELAXED);
+    const int A = blockIdx.x * kSysBlock + threadIdx.x;
+    const MrState& s = c->st[par];
+    const double ib = 1.0 / s.beta;
+    const double f = s.oldb > 0.0 ? s.beta / s.oldb : 0.0;
+    double vy = 0.0;
+    if (A < g.nn) {
+        double acc[3] = {0.0, 0.0, 0.0};
+        const int* nb = g.sell_nbr + (long)(A >> 6) * g.width * kSellChunk + (A & 63);
+        const VT* vb = val + sell_val(g.width, A, 0, 0);
+        int j0 = 0;
+        for (; j0 + G <= g.width; j0 += G) {
+            int B[G];
+            VT m[G][9];
+#pragma unroll
+            for (int u = 0; u < G; ++u) B[u] = nb[(j0 + u) * kSellChunk];
+#pragma unroll
+            for (int u = 0; u < G; ++u)
+#pragma unroll
+                for (int q = 0; q < 9; ++q) m[u][q] = __builtin_nontemporal_load(vb + ((j0 + u) * 9 + q) * kSellChunk);
+            double pb[G][3];
+#pragma unroll
+            for (int u = 0; u < G; ++u)
+#pragma unroll
+                for (int k = 0; k < 3; ++k) pb[u][k] = z[3L * B[u] + k];
+#pragma unroll
+            for (int u = 0; u < G; ++u)
+#pragma unroll
+                for (int i = 0; i < 3; ++i)
+                    acc[i] += (double)m[u][3 * i] * pb[u][0] + (double)m[u][3 * i + 1] * pb[u][1] +
+                              (double)m[u][3 * i + 2] * pb[u][2];
+        }
+        for (int j = j0; j < g.width; ++j) {
+            const int B = nb[j * kSellChunk];
+            const VT* m = vb + j * 9 * kSellChunk;
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+                acc[i] += (double)m[(3 * i) * kSellChunk] * z[3L * B] + (double)m[(3 * i + 1) * kSellChunk] * z[3L * B + 1] +
+                          (double)m[(3 * i + 2) * kSellChunk] * z[3L * B + 2];
+        }
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            const double vi = z[3L * A + i] * ib;
+            const double yi = acc[i] * ib - f * r1[3L * A + i];
+            v[3L * A + i] = vi;
+            y[3L * A + i] = yi;
+            vy += vi * yi;
+        }
+    }
+    block_sum_max(vy, 0.0, part, nullptr);
+}
+
+__global__ __launch_bounds__(kSysBlock) void k_mr_lanczos(int nn, int par, const double* __restrict__ Minv, MrScalars* c,
+                                                          const double* vy_part, int nblk, const double* __restrict__ r2,
+                                                          double* __restrict__ y, double* __restrict__ z, double* part) {
+    if (mr_stopped(c)) return;
+    double alpha, unused;
+    reduce2(vy_part, nullptr, nblk, alpha, unused);
+    if (blockIdx.x == 0 && threadIdx.x == 0) c->alpha = alpha;
+    const double f = alpha / c->st[par].beta;
+    const int A = blockIdx.x * kSysBlock + threadIdx.x;
+    double rz = 0.0;
+    if (A < nn) {
+        double rv[3], zv[3];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            rv[i] = y[3L * A + i] - f * r2[3L * A + i];
+            y[3L * A + i] = rv[i];
+        }
+        precond(Minv, nn, A, rv, zv);
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            z[3L * A + i] = zv[i];
+            rz += rv[i] * zv[i];
+        }
+    }
+    block_sum_max(rz, 0.0, part, nullptr);
+}
+
+// the rotation and the solution update; w_new overwrites the buffer of w_{k-2}
+__global__ __launch_bounds__(kSysBlock) void k_mr_update(int nn, int par, MrScalars* c, const double* rz_part, int nblk,
+                                                         const double* __restrict__ v, const double* __restrict__ w_km1,
+                                                         double* __restrict__ w_km2, double* __restrict__ x) {
+    if (mr_stopped(c)) return;
+    double rz, unused;
+    reduce2(rz_part, nullptr, nblk, rz, unused);
+    const MrState s = c->st[par];
+    const double alpha = c->alpha;
+    const double beta_new = rz >= 0.0 ? sqrt(rz) : 0.0;
+    const double oldeps = s.epsln;
+    const double delta = s.cs * s.dbar + s.sn * alpha;
+    const double gbar = s.sn * s.dbar - s.cs * alpha;
+    const double epsln = s.sn * beta_new;
+    const double dbar = -s.cs * beta_new;
+    double gamma = sqrt(gbar * gbar + beta_new * beta_new);
+    gamma = fmax(gamma, 1e-300);
+    const double cs = gbar / gamma, sn = beta_new / gamma;
+    const double phi = cs * s.phibar, phibar = sn * s.phibar;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        MrState& t = c->st[par ^ 1];
+        t.beta = beta_new;
+        t.oldb = s.beta;
+        t.cs = cs;
+        t.sn = sn;
+        t.dbar = dbar;
+        t.epsln = epsln;
+        t.phibar = phibar;
+        c->iters += 1;
+        if (!(rz >= 0.0)) c->verdict = 2;
+        else if (phibar <= c->rtol * c->beta1 || beta_new == 0.0) c->verdict = 1;
+        else if (c->iters >= c->maxit) c->verdict = 3;
+    }
+    const int A = blockIdx.x * kSysBlock + threadIdx.x;
+    if (A < nn) {
+        const double ig = 1.0 / gamma;
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            const double w = (v[3L * A + i] - oldeps * w_km2[3L * A + i] - delta * w_km1[3L * A + i]) * ig;
+            w_km2[3L * A + i] = w;
+            x[3L * A + i] += phi * w;
+        }
     }
 }
 

@@ -241,15 +241,26 @@ int ceos_residual_bc_device(ceos_handle *h, int it, const double *dU, const doub
                             void *stream);
 
 /* Linear solve of the Newton step, K x = b, replacing the PARDISO call
- * (Tools/MathTools/LinearSolver/ModPardisoSolver.f90:343-380, analysis + factorization + solve
- * every iteration): block-Jacobi (3x3 node blocks) preconditioned CG from x = 0 on the device,
- * stopped at ||r|| <= rtol ||b|| or after maxit iterations.  K is the handle's tangent after
- * ceos_residual_bc (host call) or the caller's device K (device call), copied once per solve
- * into a node-neighbour (ELL) layout of the full symmetric matrix.  Vectors, products and dot
- * products are FP64; the ELL values follow ceos_set_solver_precision (-1: the handle's
- * precision, or CEOS_K_FP64 / CEOS_K_FP32_STORED).  Outputs: iterations, recursive relative
- * residual, flag (1 converged, 2 breakdown p.Kp <= 0, 3 iteration limit).  Both calls
- * synchronise the stream every 32 iterations to read the state. */
+ * (Tools/MathTools/LinearSolver/ModPardisoSolver.f90:343-380: analysis + factorization + solve
+ * every iteration, mtype -2, symmetric indefinite).  Krylov methods from x = 0 on the device with
+ * a block-Jacobi (3x3 node blocks) preconditioner (a block that is not positive definite is
+ * replaced by its absolute diagonal):
+ *   CEOS_SOLVER_PCG     conjugate gradients, stopped at ||r|| <= rtol ||b||; an indefinite K
+ *                       stops it with flag 2 (p.Kp <= 0);
+ *   CEOS_SOLVER_MINRES  MINRES, for symmetric indefinite K, stopped when the residual in the
+ *                       M^-1 norm is <= rtol times that of b;
+ *   CEOS_SOLVER_AUTO    (default) PCG, and MINRES from x = 0 when PCG breaks down.
+ * K is the handle's tangent after ceos_residual_bc (host call) or the caller's device K (device
+ * call), copied once per solve into a neighbour-slot (SELL) layout of the full symmetric matrix.
+ * Vectors, products and dot products are FP64; the matrix values follow
+ * ceos_set_solver_precision (-1: the handle's precision, or CEOS_K_FP64 / CEOS_K_FP32_STORED).
+ * Outputs: iterations (PCG + MINRES), relative residual of the last method, flag (1 converged by
+ * PCG, 4 converged by MINRES, 2 breakdown, 3 iteration limit).  Both calls synchronise the
+ * stream every 32 iterations to read the state. */
+#define CEOS_SOLVER_AUTO 0
+#define CEOS_SOLVER_PCG 1
+#define CEOS_SOLVER_MINRES 2
+int ceos_set_linear_solver(ceos_handle *h, int method);
 int ceos_set_solver_precision(ceos_handle *h, int mode);
 int ceos_linear_solve(ceos_handle *h, const double *b, double *x, double rtol, int maxit, int *iters,
                       double *relres, int *flag);
