@@ -22,6 +22,7 @@ module ceos_gpu_interface
     integer(c_int), parameter :: CEOS_STATE_STRIDE = 20
     integer(c_int), parameter :: CEOS_ELEM_FULL_INTEGRATION = 1, CEOS_ELEM_MEAN_DILATATION = 2
     integer(c_int), parameter :: CEOS_NORM_L2 = 1, CEOS_NORM_MAX_ABS = 2
+    integer(c_int), parameter :: CEOS_SOLVER_AUTO = 0, CEOS_SOLVER_PCG = 1, CEOS_SOLVER_MINRES = 2
 
     ! mirrors ClassStatus (Tools/ModStatus.f90:3-16)
     type, bind(C) :: ceos_status
@@ -178,6 +179,12 @@ module ceos_gpu_interface
             real(c_double), intent(in) :: Fext(*), Ubar(*)
             real(c_double), intent(out) :: R(*)
             real(c_double), intent(out) :: normR
+        end function
+
+        integer(c_int) function ceos_set_linear_solver(h, method) bind(C, name="ceos_set_linear_solver")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: h
+            integer(c_int), value :: method
         end function
 
         integer(c_int) function ceos_linear_solve(h, b, x, rtol, maxit, iters, relres, flag) &
