@@ -1,0 +1,172 @@
+"""Quasi-static analysis of a case on the GPU: the reference's driver loop around the library.
+
+  load cases, steps, golden-ratio cut-back    FEM/ModFEMAnalysis.f90:1136-1374
+  external forces and prescribed displacement FEM/BoundaryConditions/ModBoundaryConditions.f90:540-640
+  Newton, relative test on the max-abs norm   Tools/MathTools/NonlinearSolver/ModNewtonRaphsonFull.f90:44-168
+  FEMAnalysis.result records                  FEM/ModFEMAnalysis.f90:1094-1129, 1211-1212, 1314-1336
+
+Every evaluation, the boundary conditions, the residual norm and the linear solves run in
+libceos_gpu.so (ceos_solve_constitutive, ceos_residual_bc, ceos_linear_solve: PCG, and MINRES
+for indefinite tangents); the host keeps only the loop, U and the tables.
+
+    python -m ceos_amd.analysis <case directory> [output directory]
+"""
+import math
+import os
+import sys
+import time as _time
+
+import numpy as np
+
+from . import caseio, gpu
+
+GOLDEN_RATIO = (1.0 + math.sqrt(5.0)) / 2.0
+
+
+class NewtonFailed(Exception):
+    pass
+
+
+def make_mesh(case):
+    s = case.settings
+    m = case.mesh
+    mesh = gpu.CeosMesh(m["coords"], m["conn"], mat_id=m["mat"], nl_analysis=s["nl_analysis"],
+                        elem_tech=s["element_technology"])
+    for mid, (model, props) in s["materials"].items():
+        mesh.set_material(mid, model, props)
+    return mesh
+
+
+def linear_solve(mesh, b, rtol, minres_rtol):
+    """PCG at rtol; after a breakdown (indefinite tangent) MINRES at minres_rtol, whose residual
+    is measured in the M^-1 norm and levels off earlier than PCG's 2-norm on stiff systems."""
+    x, its, rr, flag = mesh.linear_solve(b, rtol, 20000)
+    if flag == 2:
+        mesh.set_linear_solver(gpu.SOLVER_MINRES)
+        try:
+            x, its2, rr, flag = mesh.linear_solve(b, minres_rtol, 20000)
+        finally:
+            mesh.set_linear_solver(gpu.SOLVER_PCG)
+    if flag not in (1, 4):
+        raise NewtonFailed("linear solve flag %d" % flag)
+    return x
+
+
+def newton(mesh, time, fext, Ubar, Xguess, nl, tol, maxiter, norms, absolute, linear_rtol, minres_rtol):
+    """NewtonRaphsonFull_Solve on the device system.  Returns (X, iterations)."""
+    X = Xguess.copy()
+    it = 0
+    norm0 = None
+    while True:
+        st = mesh.solve_constitutive(time, X)
+        if st:
+            raise NewtonFailed("evaluation status %d" % st)
+        R, normR = mesh.residual_bc(it, fext, Ubar, 2)
+        norms.append(normR)
+        if it == 0:
+            norm0 = normR
+        if it > 0 and normR < (tol if absolute else norm0 * tol):
+            return X, it
+        if it >= maxiter:
+            raise NewtonFailed("maximum number of iterations")
+        it += 1
+        X = X + linear_solve(mesh, -R, linear_rtol, minres_rtol)
+        if nl:
+            mesh.update_mesh(X)
+
+
+def run(case, out_dir=None, linear_rtol=1e-12, minres_rtol=1e-12, absolute=False, verbose=False):
+    """The quasi-static analysis of `case`.  Writes FEMAnalysis.result, the probe files and
+    NumberOfIterationsToConverge.dat into out_dir when given.  Returns (history, norms): history
+    rows (load case, step, time, U, iterations) per converged step; norms, the residual norms of
+    every Newton solve (one list per substep attempt).  absolute=True stops Newton at
+    norm < tol instead of norm < tol * first norm: the test of the binary that wrote the
+    reference's Checker logs."""
+    s = case.settings
+    mesh = make_mesh(case)
+    mesh.set_bc(case.presc_dof, case.fixed_dof)
+    mesh.set_linear_solver(gpu.SOLVER_PCG)
+    nl = s["nl_analysis"]
+    n = case.ndof
+    U = np.zeros(n)
+    history, norms = [], []
+    res = None
+    if out_dir:
+        os.makedirs(out_dir, exist_ok=True)
+        res = open(os.path.join(out_dir, "FEMAnalysis.result"), "w")
+        caseio.write_results_header(res, n)
+        caseio.write_results_record(res, U, 0.0, 1, 1, 0, 0, 1, 0)
+    try:
+        for lc, c in enumerate(case.load_cases, start=1):
+            dt = (c["t1"] - c["t0"]) / c["steps"]
+            for stp in range(1, c["steps"] + 1):
+                ta, tb = c["t0"] + (stp - 1) * dt, c["t0"] + stp * dt
+                fext0, dfext = case.fext(ta), case.fext(tb) - case.fext(ta)
+                dpresc = np.zeros(n)
+                if len(case.presc_dof):
+                    v0, v1 = case.presc_values(ta), case.presc_values(tb)
+                    U[case.presc_dof] = v0
+                    dpresc[case.presc_dof] = v1 - v0
+                Ubar0, Uconv = U.copy(), U.copy()
+                amax, amin = 1.0, 0.0
+                alpha = amax
+                cutback = substep = 0
+                while True:
+                    t = ta + alpha * dt
+                    nrm = []
+                    try:
+                        U, its = newton(mesh, t, fext0 + alpha * dfext, Ubar0 + alpha * dpresc, Uconv, nl,
+                                        s["newton_tol"], s["newton_maxiter"], nrm, absolute, linear_rtol,
+                                        minres_rtol)
+                        ok = True
+                    except NewtonFailed:
+                        ok = False
+                    norms.append(nrm)
+                    if verbose:
+                        print("LC %d step %d alpha %.4f: %s" % (lc, stp, alpha, " ".join("%.3e" % v for v in nrm)),
+                              flush=True)
+                    if not ok:
+                        alpha = amin + (1.0 - 1.0 / GOLDEN_RATIO) * (alpha - amin)
+                        U = Uconv.copy()
+                        if nl:
+                            mesh.update_mesh(U)
+                        cutback += 1
+                        if cutback > s["max_cut_backs"]:
+                            raise NewtonFailed("load case %d step %d did not converge with %d cut backs"
+                                               % (lc, stp, s["max_cut_backs"]))
+                    elif alpha == 1.0:
+                        history.append((lc, stp, t, U.copy(), its))
+                        if res:
+                            caseio.write_results_record(res, U, t, lc, stp, cutback, substep, 1, its)
+                        break
+                    else:
+                        if res:
+                            caseio.write_results_record(res, U, t, lc, stp, cutback, substep, 0, its)
+                        substep += 1
+                        aux, amin = amin, alpha
+                        alpha = min(alpha + GOLDEN_RATIO * (alpha - aux), 1.0)
+                        Uconv = U.copy()
+                mesh.switch_converged_state()
+    finally:
+        if res:
+            res.close()
+    if out_dir:
+        caseio.post_process(case, os.path.join(out_dir, "FEMAnalysis.result"), out_dir)
+    mesh.close()
+    return history, norms
+
+
+def main(argv):
+    if len(argv) < 2:
+        print(__doc__)
+        return 2
+    case = caseio.Case(argv[1])
+    out = argv[2] if len(argv) > 2 else argv[1]
+    t0 = _time.perf_counter()
+    hist, _ = run(case, out, verbose=True)
+    print("%d steps in %.2f s; results in %s" % (len(hist), _time.perf_counter() - t0, out))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main(sys.argv))

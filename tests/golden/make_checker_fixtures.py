@@ -21,11 +21,22 @@ Cases (paths relative to the reference root):
       the StVenant case's mesh with its own supports and loads, Neo-Hookean C10 = 5e10,
       K = 5e13 under the mean-dilatation (Q1P0) element technology, 50 steps; Marc
       displacements of node 11 (checked by the reference at 1e-3) and the residual log.
+The raw data files of these two GiD cases (inputs, Marc results, logs) are copied into
+checker/<case>/ for the tests of the package's case reader and runner.
 """
 import json
 import os
 import re
+import shutil
 import sys
+
+# the input and result files of the two GiD hex8 cases, copied as they are into checker/<case>/ so
+# that the package's case reader and runner (ceos_amd.caseio, ceos_amd.analysis) are tested on
+# the reference's own files
+RAW_CASES = ["NonLinear_FEA_3D_Hexa8_StVenant", "NonLinear_FEA_3D_Hexa8_NeoHookean_Incompressible"]
+RAW_FILES = ["Settings.dat", "Mesh.dat", "Time_Discretization.dat", "Table_Disp_Z.tab", "Table_Load_Y.tab",
+             "Post_Processing.post", "marc_disp_x.dat", "marc_disp_y.dat", "marc_disp_z.dat", "Log_CEOS.txt",
+             "NumOfIterRef.dat"]
 
 
 def _table(path):
@@ -263,6 +274,13 @@ def main():
         with open(os.path.join(here, name), "w") as f:
             json.dump(data, f, separators=(",", ":"))
         print("wrote", name, "nodes", len(data["coords"]), "elems", len(data["conn"]))
+    for case in RAW_CASES:
+        src = os.path.join(root, "Checker/Cases", case)
+        dst = os.path.join(here, "checker", case)
+        os.makedirs(dst, exist_ok=True)
+        for fn in RAW_FILES:
+            shutil.copyfile(os.path.join(src, fn), os.path.join(dst, fn))
+        print("copied", case)
 
 
 if __name__ == "__main__":
