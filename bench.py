@@ -40,9 +40,9 @@ LAM = K_BULK - 2.0 * MU / 3.0
 # algorithmic work per element and evaluation (structure-exploiting count, FP64 tangent):
 FLOP_PER_ELEM = 37.5e3
 BYTES_PER_ELEM = 1103.0
-# gather assembly: per element one 360-double record and 36 contribution words; per node block
+# gather assembly: per element one 324-double record and 36 contribution words; per node block
 # its cptr and blknode words; per CSR entry the stored value
-RECORD_BYTES = 360 * 8
+RECORD_BYTES = 324 * 8
 CWORD_BYTES = 36 * 4
 PEAK_FP64_TFLOPS = 78.6
 PEAK_HBM_GBS = 8000.0

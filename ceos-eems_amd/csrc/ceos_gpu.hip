@@ -310,7 +310,7 @@ static int build_gather_map(ceos_handle* h) {
             const long long base = (long long)h->iperm[e] * kRecord;
             for (int b = 0; b < 8; ++b) {
                 const int B = cn[b];
-                const long long lb = base + (long long)b * kLaneRec;
+                const long long lb = base + rec_off(b);
                 const int np = b < 4 ? 4 : 3;
                 int blks[5];
                 uint32_t words[5];

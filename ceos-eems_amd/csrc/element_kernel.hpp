@@ -37,11 +37,12 @@ constexpr int kOffSig = 192;
 constexpr int kOffD = 240;
 constexpr int kJ2State = 8;           // [0..5] plastic strain, [6] acc. plastic strain, [7] flag
 constexpr int kStateStride = 20;      // state rows per Gauss point in the device arrays (max of models)
-// Element record of the gather assembly: lane b owns doubles [45 b, 45 b + 45):
+// Element record of the gather assembly: lane b owns rec_off(b) .. + 45 (b < 4, four partner
+// blocks) or + 36 (b >= 4, three):
 //   [0..2] Fe_b, [3..8] diagonal block (b,b) upper (00 01 02 11 12 22),
 //   [9 + 9 j .. 17 + 9 j] block (c,b), c = (b + j + 1) mod 8, rows = dofs of c, row-major.
-constexpr int kLaneRec = 45;
-constexpr int kRecord = 8 * kLaneRec;
+constexpr int kRecord = 4 * 45 + 4 * 36;  // 324 doubles per element
+__host__ __device__ constexpr int rec_off(int b) { return b < 4 ? 45 * b : 180 + 36 * (b - 4); }
 
 struct ElemParams {
     int ne;                        // elements (device slot order)
@@ -433,7 +434,7 @@ __global__ __launch_bounds__(128) void k_element(ElemParams p, int e_begin, int 
         // stage the element record in LDS (the GP data are no longer read), then write the
         // records of the block's consecutive elements with unit-stride stores
         __syncthreads();
-        double* rec = sm + b * kLaneRec;
+        double* rec = sm + rec_off(b);
         rec[0] = fe0;
         rec[1] = fe1;
         rec[2] = fe2;
@@ -441,8 +442,10 @@ __global__ __launch_bounds__(128) void k_element(ElemParams p, int e_begin, int 
         for (int q = 0; q < 6; ++q) rec[3 + q] = kd[q];
 #pragma unroll
         for (int j = 0; j < 4; ++j)
+            if (j < npart) {
 #pragma unroll
-            for (int q = 0; q < 9; ++q) rec[9 + 9 * j + q] = (j < npart) ? ko[j][q] : 0.0;
+                for (int q = 0; q < 9; ++q) rec[9 + 9 * j + q] = ko[j][q];
+            }
         __syncthreads();
         const int e0 = e_begin + blockIdx.x * kEPB;
         const int nel = min(kEPB, e_end - e0);
