@@ -33,6 +33,11 @@ struct GatherParams {
     void* K;                 // [nnz]
 };
 
+// Contributions are taken kGatherBatch at a time: their words, then all their record pieces are
+// requested before any is summed (a thread keeps up to 4 x 9 loads in flight instead of one
+// word -> piece chain per contribution).  The sums still run in contribution order.
+constexpr int kGatherBatch = 4;
+
 template <typename KT>
 __global__ __launch_bounds__(256) void k_gather_blocks(GatherParams p) {
     const long long blk = p.blk0 + (long long)blockIdx.x * 256 + threadIdx.x;
@@ -46,12 +51,26 @@ __global__ __launch_bounds__(256) void k_gather_blocks(GatherParams p) {
     if (q == 0) {
         // diagonal block, stored upper 00 01 02 11 12 22, preceded by the lane's Fe
         double a[6] = {0, 0, 0, 0, 0, 0}, f[3] = {0, 0, 0};
-        for (int c = c0; c < c1; ++c) {
-            const double* d = p.rec + (long)(p.cword[c] & 0x7fffffffu);
+        for (int c = c0; c < c1; c += kGatherBatch) {
+            const double* d[kGatherBatch];
 #pragma unroll
-            for (int k = 0; k < 6; ++k) a[k] += d[k];
+            for (int u = 0; u < kGatherBatch; ++u)
+                d[u] = (c + u < c1) ? p.rec + (long)(p.cword[c + u] & 0x7fffffffu) : nullptr;
+            double v[kGatherBatch][9];
 #pragma unroll
-            for (int k = 0; k < 3; ++k) f[k] += d[k - 3];
+            for (int u = 0; u < kGatherBatch; ++u)
+                if (d[u]) {
+#pragma unroll
+                    for (int k = 0; k < 9; ++k) v[u][k] = d[u][k - 3];
+                }
+#pragma unroll
+            for (int u = 0; u < kGatherBatch; ++u)
+                if (d[u]) {
+#pragma unroll
+                    for (int k = 0; k < 6; ++k) a[k] += v[u][3 + k];
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) f[k] += v[u][k];
+                }
         }
 #pragma unroll
         for (int k = 0; k < 3; ++k) p.Fint[3L * A + k] = f[k];
@@ -63,20 +82,30 @@ __global__ __launch_bounds__(256) void k_gather_blocks(GatherParams p) {
         return;
     }
     double a[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    for (int c = c0; c < c1; ++c) {
-        const uint32_t w = p.cword[c];
-        const double* d = p.rec + (long)(w & 0x7fffffffu);
-        double v[9];
+    for (int c = c0; c < c1; c += kGatherBatch) {
+        uint32_t w[kGatherBatch];
 #pragma unroll
-        for (int k = 0; k < 9; ++k) v[k] = d[k];
-        if (w >> 31) {
+        for (int u = 0; u < kGatherBatch; ++u) w[u] = (c + u < c1) ? p.cword[c + u] : 0xffffffffu;
+        double v[kGatherBatch][9];
 #pragma unroll
-            for (int i = 0; i < 3; ++i)
+        for (int u = 0; u < kGatherBatch; ++u)
+            if (w[u] != 0xffffffffu) {
+                const double* d = p.rec + (long)(w[u] & 0x7fffffffu);
 #pragma unroll
-                for (int k = 0; k < 3; ++k) a[3 * i + k] += v[3 * k + i];
-        } else {
+                for (int k = 0; k < 9; ++k) v[u][k] = d[k];
+            }
 #pragma unroll
-            for (int k = 0; k < 9; ++k) a[k] += v[k];
+        for (int u = 0; u < kGatherBatch; ++u) {
+            if (w[u] == 0xffffffffu) continue;
+            if (w[u] >> 31) {
+#pragma unroll
+                for (int i = 0; i < 3; ++i)
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) a[3 * i + k] += v[u][3 * k + i];
+            } else {
+#pragma unroll
+                for (int k = 0; k < 9; ++k) a[k] += v[u][k];
+            }
         }
     }
     const long o = 3L * (q - 1);
