@@ -76,6 +76,23 @@ def test_macro_evaluation_surrogate_vs_rve(surrogate):
     assert err.mean() < MEAN_ERR_BAR and fr < MEAN_ERR_BAR and kr < 0.05
 
 
+def test_judge_estimate_does_not_depend_on_its_newton_tolerance(surrogate):
+    """The judge's norm-wise error estimate with the judged RVEs converged to the default Newton
+    tolerance (1e-3) against the same judged points converged to 1e-10."""
+    m = fe2.MacroFE2(n_macro=2, side_choices=(8,), seed=SEED_HELD_OUT, surrogate=surrogate)
+    m.setup_descriptors()
+    x, y, z = m.X[:, 0], m.X[:, 1], m.X[:, 2]
+    U = np.stack([0.06 * x + 0.01 * np.sin(np.pi * y), -0.03 * y + 0.01 * x * z, -0.02 * z], axis=1).reshape(-1)
+    est = []
+    for r in (1e-10, 1e-3):
+        m.rves.reset()
+        s = m.evaluate_surrogate(U, judge_frac=0.25, judge_tol=0.05, judge_newton_rtol=r)
+        est.append(s["judge_err"])
+        assert s["judged"] == 16
+    print("judge estimate: converged %.6f, default tolerance %.6f" % tuple(est), flush=True)
+    assert abs(est[1] - est[0]) < 0.02 * est[0]
+
+
 class _Fe2Backend:
     """The quasi-static driver's backend interface over a MacroFE2 (plain or surrogate)."""
 
