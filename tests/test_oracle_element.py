@@ -127,9 +127,13 @@ def test_patch_test_uniform_stress_zero_interior_residual():
     (ol.MODEL_CNH, [1.0, 99.3333333333]),
     (ol.MODEL_NH, [0.5, 100.0]),
     (ol.MODEL_SVK, [10.0, 0.3]),
+    (ol.MODEL_NH_Q1P0, [0.5, 100.0]),
 ])
 def test_ke_is_derivative_of_fe(model, props):
-    """Ke = dFe/du (updated Lagrangian: x = X + u, F from X and u) by central differences."""
+    """Ke = dFe/du (updated Lagrangian: x = X + u, F from X and u) by central differences.
+    For the mean-dilatation model this checks the Bdiv term: Fe carries the pressure of
+    Jbar = V/V0, dV/du = Bdiv, so dFe/du gains d2Psi/dJbar2 / V0 Bdiv^T Bdiv, and a dropped or
+    mis-scaled term fails."""
     X, conn = meshgen.cube(1, perturb=0.0)
     rng = np.random.default_rng(11)
     X = X + rng.uniform(-0.1, 0.1, size=X.shape)
