@@ -61,9 +61,26 @@ __device__ __forceinline__ void hex8_dshape_gp(int g, double dN[8][3]) {
     }
 }
 
-// Jacobian from nodal coordinates c[a][j]; returns det; fills grad[a][i] = (J^-1 dN_a)_i
+// dN_a/dxi of one node a at Gauss point g: the entries hex8_dshape_gp forms, by the same expressions
+__device__ __forceinline__ void hex8_dshape_node(int a, int g, double d[3]) {
+    const double x = 0.57735026918962576451;  // 1/sqrt(3)
+    const double xi = hex_sign(g, 0) * x, eta = hex_sign(g, 1) * x, zeta = hex_sign(g, 2) * x;
+    const double s0 = hex_sign(a, 0), s1 = hex_sign(a, 1), s2 = hex_sign(a, 2);
+    d[0] = s0 * (1.0 + s1 * eta) * (1.0 + s2 * zeta) * 0.125;
+    d[1] = s1 * (1.0 + s0 * xi) * (1.0 + s2 * zeta) * 0.125;
+    d[2] = s2 * (1.0 + s0 * xi) * (1.0 + s1 * eta) * 0.125;
+}
+
+// gradient of one shape function from J^-1 and its natural derivatives: (J^-1 dN_a)_i
+__device__ __forceinline__ void grad_from_jinv(const double Ji[9], const double d[3], double g[3]) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i) g[i] = Ji[3 * i] * d[0] + Ji[3 * i + 1] * d[1] + Ji[3 * i + 2] * d[2];
+}
+
+// Jacobian from nodal coordinates c[a][j]; returns det; fills grad[a][i] = (J^-1 dN_a)_i and,
+// when Jout is given, J^-1 row-major
 __device__ __forceinline__ double hex8_grad(const double dN[8][3], const double c[8][3],
-                                            double grad[8][3]) {
+                                            double grad[8][3], double* Jout = nullptr) {
     double J[3][3];
 #pragma unroll
     for (int i = 0; i < 3; ++i)
@@ -77,11 +94,17 @@ __device__ __forceinline__ double hex8_grad(const double dN[8][3], const double 
     const double d = det3(J);
     double Ji[3][3];
     inv3(J, d, Ji);
+    double Jf[9];
 #pragma unroll
-    for (int a = 0; a < 8; ++a)
+    for (int i = 0; i < 3; ++i)
 #pragma unroll
-        for (int i = 0; i < 3; ++i)
-            grad[a][i] = Ji[i][0] * dN[a][0] + Ji[i][1] * dN[a][1] + Ji[i][2] * dN[a][2];
+        for (int j = 0; j < 3; ++j) Jf[3 * i + j] = Ji[i][j];
+#pragma unroll
+    for (int a = 0; a < 8; ++a) grad_from_jinv(Jf, dN[a], grad[a]);
+    if (Jout) {
+#pragma unroll
+        for (int k = 0; k < 9; ++k) Jout[k] = Jf[k];
+    }
     return d;
 }
 

@@ -32,9 +32,13 @@
 namespace ceos {
 
 constexpr int kEPB = 16;              // elements per workgroup (128 threads = 2 waves)
-constexpr int kLdsPerElem = 408;      // doubles: gradN 8x8x3 + sigma 8x6 + D 8x21
-constexpr int kOffSig = 192;
-constexpr int kOffD = 240;
+// doubles per element in LDS between the phases: J^-1 of the current configuration 8x9, w detJ
+// sigma 8x6, w detJ D (upper) 8x21.  Phase 2 recomputes grad_x N from J^-1 and the natural
+// derivatives (the 192 doubles of grad_x N itself would take 52.7 KB per workgroup, 3 workgroups
+// per CU; 288 doubles take 36.9 KB, 4 workgroups, the 8 waves the register count allows)
+constexpr int kLdsPerElem = 288;
+constexpr int kOffSig = 72;
+constexpr int kOffD = 120;
 constexpr int kJ2State = 8;           // [0..5] plastic strain, [6] acc. plastic strain, [7] flag
 constexpr int kStateStride = 20;      // state rows per Gauss point in the device arrays (max of models)
 // Element record of the gather assembly: lane b owns rec_off(b) .. + 45 (b < 4, four partner
@@ -163,8 +167,9 @@ __device__ __noinline__ void tangent_numerical(const double* pr, const double F0
     push_forward_up(C, F0, det3(F0), D);
 }
 
-// Phase 2 of the element evaluation, lane = node b, from the Gauss-point data in LDS (sm: gradN
-// [8][8][3], w detJ sigma [8][6] at kOffSig, w detJ D (upper, 21) [8] at kOffD):
+// Phase 2 of the element evaluation, lane = node b, from the Gauss-point data in LDS (sm: J^-1
+// [8][9], w detJ sigma [8][6] at kOffSig, w detJ D (upper, 21) [8] at kOffD; grad_x N of a node
+// at a Gauss point is recomputed as J^-1 dN/dxi, the expression hex8_grad uses):
 //   Fe_b = sum_g B_b^T sigma, the upper diagonal block K_bb (00 01 02 11 12 22) and the blocks
 //   K_cb (rows = dofs of c, cols = dofs of b) for c = (b + j + 1) mod 8, j < 4 (b < 4) or j < 3,
 //   K_cb = sum_g B_c^T D B_b + (gradN_c . sigma gradN_b) I (the geometric part for NL).
@@ -184,9 +189,17 @@ __device__ __forceinline__ void node_blocks(const double* sm, int b, double& fe0
 
 #pragma unroll 1
     for (int gp = 0; gp < 8; ++gp) {
-        const double* Gp = sm + gp * 24;
+        double Ji[9];
+#pragma unroll
+        for (int k = 0; k < 9; ++k) Ji[k] = sm[gp * 9 + k];
         const double* Sp = sm + kOffSig + gp * 6;
-        const double b0 = Gp[b * 3 + 0], b1 = Gp[b * 3 + 1], b2 = Gp[b * 3 + 2];
+        double gb[3];
+        {
+            double d[3];
+            hex8_dshape_node(b, gp, d);
+            grad_from_jinv(Ji, d, gb);
+        }
+        const double b0 = gb[0], b1 = gb[1], b2 = gb[2];
         const double S0 = Sp[0], S1 = Sp[1], S2 = Sp[2], S3 = Sp[3], S4 = Sp[4], S5 = Sp[5];
         const double sg0 = S0 * b0 + S3 * b1 + S5 * b2;
         const double sg1 = S3 * b0 + S1 * b1 + S4 * b2;
@@ -222,7 +235,10 @@ __device__ __forceinline__ void node_blocks(const double* sm, int b, double& fe0
             for (int j = 0; j < 4; ++j) {
                 if (j < npart) {
                     const int c = (b + j + 1) & 7;
-                    const double c0 = Gp[c * 3 + 0], c1 = Gp[c * 3 + 1], c2 = Gp[c * 3 + 2];
+                    double gc[3], d[3];
+                    hex8_dshape_node(c, gp, d);
+                    grad_from_jinv(Ji, d, gc);
+                    const double c0 = gc[0], c1 = gc[1], c2 = gc[2];
                     const double geo = NL ? (c0 * sg0 + c1 * sg1 + c2 * sg2) : 0.0;
 #pragma unroll
                     for (int k = 0; k < 3; ++k) {
@@ -244,7 +260,7 @@ __device__ __forceinline__ void node_blocks(const double* sm, int b, double& fe0
 template <int MODEL, bool NL, bool WANT_K, typename KT, bool GATHER, bool NUMT = false>
 __global__ __launch_bounds__(128) void k_element(ElemParams p, int e_begin, int e_end) {
     __shared__ double smem[kEPB * kLdsPerElem];
-    __shared__ int snode[kEPB][8];
+    __shared__ int snode[GATHER ? 1 : kEPB][8];  // partner node ids (scatter assembly only)
     const int lane = threadIdx.x & 7;
     const int le = threadIdx.x >> 3;
     const int e = e_begin + blockIdx.x * kEPB + le;
@@ -254,7 +270,7 @@ __global__ __launch_bounds__(128) void k_element(ElemParams p, int e_begin, int 
     // ---------------- phase 0: gather node lane -----------------
     int node = 0;
     if (active) node = p.conn[(long)e * 8 + lane];
-    snode[le][lane] = node;
+    if constexpr (!GATHER) snode[le][lane] = node;
     {
         double v[9];
 #pragma unroll
@@ -271,6 +287,7 @@ __global__ __launch_bounds__(128) void k_element(ElemParams p, int e_begin, int 
     // ---------------- phase 1: Gauss point g = lane -----------------
     const int g = lane;
     double gx[8][3];
+    double jix[9];  // J^-1 of the current configuration at this Gauss point
     double s[6], D[21];
     double wdet;
     bool bad;
@@ -302,7 +319,7 @@ __global__ __launch_bounds__(128) void k_element(ElemParams p, int e_begin, int 
         for (int a = 0; a < 8; ++a)
 #pragma unroll
             for (int k = 0; k < 3; ++k) c[a][k] = sm[a * 9 + 3 + k];
-        const double detx = hex8_grad(dN, c, gx);
+        const double detx = hex8_grad(dN, c, gx, jix);
         bad = active && !(detx > kDetTol && detX > kDetTol);
         wdet = detx;  // Gauss weight 1 (ModElementHexa8.f90:326)
         if constexpr (MODEL == kNHQ1) {
@@ -377,9 +394,7 @@ __global__ __launch_bounds__(128) void k_element(ElemParams p, int e_begin, int 
     }
     __syncthreads();  // node data no longer needed: reuse the LDS slice for GP data
 #pragma unroll
-    for (int a = 0; a < 8; ++a)
-#pragma unroll
-        for (int k = 0; k < 3; ++k) sm[g * 24 + a * 3 + k] = gx[a][k];
+    for (int k = 0; k < 9; ++k) sm[g * 9 + k] = jix[k];
 #pragma unroll
     for (int k = 0; k < 6; ++k) sm[kOffSig + g * 6 + k] = s[k] * wdet;
     if constexpr (WANT_K) {
@@ -431,29 +446,37 @@ __global__ __launch_bounds__(128) void k_element(ElemParams p, int e_begin, int 
         }
     }
     if constexpr (GATHER) {
-        // stage the element record in LDS (the GP data are no longer read), then write the
-        // records of the block's consecutive elements with unit-stride stores
-        __syncthreads();
-        double* rec = sm + rec_off(b);
-        rec[0] = fe0;
-        rec[1] = fe1;
-        rec[2] = fe2;
-#pragma unroll
-        for (int q = 0; q < 6; ++q) rec[3 + q] = kd[q];
-#pragma unroll
-        for (int j = 0; j < 4; ++j)
-            if (j < npart) {
-#pragma unroll
-                for (int q = 0; q < 9; ++q) rec[9 + 9 * j + q] = ko[j][q];
-            }
-        __syncthreads();
+        // stage the element records in LDS (the GP data are no longer read) and write the
+        // records of the block's consecutive elements with unit-stride stores, in two halves:
+        // lanes 0-3 (record doubles 0..179), then lanes 4-7 (180..323); a slice holds 288
         const int e0 = e_begin + blockIdx.x * kEPB;
         const int nel = min(kEPB, e_end - e0);
-        const int total = nel * kRecord;
         double* dst = p.scratch + (long)e0 * kRecord;
-        for (int g2 = threadIdx.x; g2 < total; g2 += kEPB * 8) {
-            const int el = g2 / kRecord, r = g2 - el * kRecord;
-            dst[g2] = smem[el * kLdsPerElem + r];
+        constexpr int kHalf0 = 180, kHalf1 = kRecord - 180;
+#pragma unroll
+        for (int half = 0; half < 2; ++half) {
+            __syncthreads();
+            if ((b >= 4) == (half == 1)) {
+                double* rec = sm + rec_off(b) - (half ? kHalf0 : 0);
+                rec[0] = fe0;
+                rec[1] = fe1;
+                rec[2] = fe2;
+#pragma unroll
+                for (int q = 0; q < 6; ++q) rec[3 + q] = kd[q];
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    if (j < npart) {
+#pragma unroll
+                        for (int q = 0; q < 9; ++q) rec[9 + 9 * j + q] = ko[j][q];
+                    }
+            }
+            __syncthreads();
+            const int len = half ? kHalf1 : kHalf0;
+            const int off = half ? kHalf0 : 0;
+            for (int g2 = threadIdx.x; g2 < nel * len; g2 += kEPB * 8) {
+                const int el = g2 / len, r = g2 - el * len;
+                dst[(long)el * kRecord + off + r] = smem[el * kLdsPerElem + r];
+            }
         }
         return;
     }

@@ -37,7 +37,10 @@ constexpr int kRveWaves = kRveThreads / 64;
 constexpr int kRveElemPerRound = 32;          // assembly: threads 0..255, 8 lanes per element
 constexpr int kRveAsmThreads = 8 * kRveElemPerRound;
 constexpr int kRveMaxSide = 16;
-constexpr int kRveLds = kRveElemPerRound * kLdsPerElem;  // 13056 doubles >= 3 * 16^3
+// the assembly's element slices and, in the CG, p of the largest RVE share the LDS
+constexpr int kRveLds = kRveElemPerRound * kLdsPerElem > 3 * kRveMaxSide * kRveMaxSide * kRveMaxSide
+                            ? kRveElemPerRound * kLdsPerElem
+                            : 3 * kRveMaxSide * kRveMaxSide * kRveMaxSide;  // 12288 doubles
 constexpr int kRveVecPerNode = 51;  // w 3, R 3, x 3, r 3, q 3, z 3, Minv 6, B 27
 static_assert(kRveLds >= 3 * kRveMaxSide * kRveMaxSide * kRveMaxSide, "p must fit in LDS");
 
@@ -172,6 +175,7 @@ __device__ bool rve_assemble(const RveCtx<MT>& c, const RveParams& p, const doub
             // phase 1: Gauss point g = lane
             const int g = lane;
             double gxs[8][3];
+            double jix[9];  // J^-1 of the current configuration (phase 2 recomputes grad_x N)
             double s[6], D[21];
             double wdet = 0.0;
             if (worker) {
@@ -198,7 +202,7 @@ __device__ bool rve_assemble(const RveCtx<MT>& c, const RveParams& p, const doub
                 for (int a = 0; a < 8; ++a)
 #pragma unroll
                     for (int k = 0; k < 3; ++k) cc[a][k] = sm[a * 9 + 3 + k];
-                const double detx = hex8_grad(dN, cc, gxs);
+                const double detx = hex8_grad(dN, cc, gxs, jix);
                 bad |= active && !(detx > kDetTol && detX > kDetTol);
                 wdet = detx;
                 const int ph = active ? c.phase[ex + n * (ey + n * ez)] : 0;
@@ -215,9 +219,7 @@ __device__ bool rve_assemble(const RveCtx<MT>& c, const RveParams& p, const doub
             __syncthreads();  // node data no longer needed
             if (worker) {
 #pragma unroll
-                for (int a = 0; a < 8; ++a)
-#pragma unroll
-                    for (int k = 0; k < 3; ++k) sm[g * 24 + a * 3 + k] = gxs[a][k];
+                for (int k = 0; k < 9; ++k) sm[g * 9 + k] = jix[k];
 #pragma unroll
                 for (int k = 0; k < 6; ++k) sm[kOffSig + g * 6 + k] = s[k] * wdet;
                 if constexpr (WANT_K) {
