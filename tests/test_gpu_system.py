@@ -328,3 +328,34 @@ def test_device_resident_newton_matches_host_calls():
         assert np.array_equal(dU.numpy(), host[it][1]), it
     norms = [n for n, _ in host]
     assert all(b < a for a, b in zip(norms, norms[1:])) and norms[-1] < 1e-5 * norms[0], norms
+
+
+def test_entry_points_refuse_bad_arguments():
+    """Out-of-range DOFs, a residual before ceos_bc_set, a non-positive tolerance and an unknown
+    solver or precision are refused with CEOS_ERR_ARG, and the handle stays usable."""
+    X, conn = meshgen.cube(2)
+    g = gpu.CeosMesh(X, conn)
+    g.set_material(1, ol.MODEL_CNH, [1.0, LAM])
+    U = np.zeros(g.ndof)
+    assert g.solve_constitutive(0.0, U) == 0
+    with pytest.raises(gpu.CeosError) as e:
+        g.residual_bc(1)  # no ceos_bc_set yet
+    assert e.value.code == gpu.ERR_ARG
+    for bad in ([g.ndof], [-1]):
+        with pytest.raises(gpu.CeosError) as e:
+            g.set_bc(bad, [])
+        assert e.value.code == gpu.ERR_ARG
+    fixed, presc = meshgen.stretch_bc(X)
+    g.set_bc(presc, fixed)
+    with pytest.raises(gpu.CeosError) as e:
+        g.residual_bc(0)  # prescribed DOFs at it = 0 need Ubar
+    assert e.value.code == gpu.ERR_ARG
+    R, nrm = g.residual_bc(0, None, np.zeros(g.ndof))
+    assert nrm == 0.0 and not R.any()
+    for call in (lambda: g.linear_solve(np.ones(g.ndof), 0.0), lambda: g.set_linear_solver(7),
+                 lambda: g.set_solver_precision(5)):
+        with pytest.raises(gpu.CeosError) as e:
+            call()
+        assert e.value.code == gpu.ERR_ARG
+    x, its, _, flag = g.linear_solve(np.ones(g.ndof), 1e-10)
+    assert flag == 1 and np.all(np.isfinite(x))
