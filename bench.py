@@ -273,6 +273,165 @@ def newton_step_device(mesh, X, ux, tol, rtol, maxit=15, stream=None):
     return recs, total, dU.numpy()
 
 
+J2_PROPS = [210000.0, 0.3, 250.0, 1000.0]  # E, nu, yield stress, hardening modulus (MPa)
+J2_TABLE = [(0.0, 0.0), (1.0, 0.02), (3.0, -0.02), (4.0, 0.02)]  # u_x on x = 1 against time
+# J2 under linear analysis, per Gauss point: B and detJ 328, F 472, residual 144, B^T D B 2,808,
+# radial return ~300 flops
+J2_FLOP_PER_ELEM = 8 * (328 + 472 + 144 + 2808 + 300)
+
+
+def j2_cpu_baseline(X, conn, U, t, state_old, F_gpu, K_gpu, flags_gpu, budget_s=12.0):
+    """The CPU oracle (OpenMP) on the same J2 evaluation: the full mesh, the same U and committed
+    state; timed over a bounded number of F+K evaluations.  Its F, K and plastic flags are compared
+    with the device's at full size."""
+    sys.path.insert(0, os.path.join(REPO, "tests"))
+    import oracle_lib as ol
+    threads = min(os.cpu_count() or 1, 16)
+    o = ol.OracleMesh(X, conn, nl_analysis=False, threads=threads)
+    o.set_material(1, ol.MODEL_J2, J2_PROPS)
+    o.set_state(0, state_old)
+    o.csr()
+    n, t0 = 0, time.perf_counter()
+    while True:
+        o.solve_constitutive(t, U)
+        F = o.internal_force()[0]
+        K = o.tangent()
+        n += 1
+        if time.perf_counter() - t0 > budget_s:
+            break
+    dt = time.perf_counter() - t0
+    flags = o.state(1)[..., 15]  # the oracle's J2 rows: [15] PlasticState (the device's row 7)
+    rel = lambda a, b: float(np.linalg.norm(a - b) / np.linalg.norm(b))
+    parity = {"fint_rel": rel(F_gpu, F), "k_rel": rel(K_gpu, K),
+              "flag_mismatches": int(np.count_nonzero(flags != flags_gpu)), "gauss_points": int(flags.size)}
+    return ({"value": n * len(conn) / dt, "unit": "elem/s", "cores": threads, "kind": "port",
+             "sample": "%d F+K evaluations (FP64) of the full %d-element J2 mesh at the same state, CPU oracle "
+                       "with %d OpenMP threads" % (n, len(conn), threads)}, parity)
+
+
+def j2_bench(steps=100, warmup=5, n_cells=64, n_hist=10, tol=1e-8, pcg_rtol=1e-6, newton_maxit=15,
+             max_cut_backs=10, solver_precision=None, no_cpu=False):
+    """The J2 cyclic workload: 64^3 hex8 cube, J2 with linear isotropic hardening, linear analysis,
+    x=0 fixed, u_x on x=1 cycling 0 -> +2 % -> -2 % -> +2 % over 40 steps.  Steps 1..n_hist run on
+    the device (evaluation, BC + norm, PCG, state commit per step) with the reference driver's
+    golden-ratio cut-back when a Newton solve does not converge (FEM/ModFEMAnalysis.f90:1136-1374:
+    full Newton on J2 stalls at this size when the plastic zone spreads over a whole step); the
+    F+K evaluation is then timed inside step n_hist, at its converged U against the committed
+    step n_hist - 1 state (mixed elastic / plastic Gauss points)."""
+    from ceos_amd import gpu, hiprt, meshgen
+    phi = (1.0 + 5.0 ** 0.5) / 2.0
+    X, conn = meshgen.cube(n_cells)
+    m = gpu.CeosMesh(X, conn, nl_analysis=False)
+    m.set_material(1, gpu.MODEL_J2, J2_PROPS)
+    m.set_solver_precision(gpu.K_FP32_STORED if solver_precision is None else solver_precision)
+    fixed, presc = meshgen.stretch_bc(X)
+    m.set_bc(presc, fixed)
+    _, nnz = m.info()
+    ndof = 3 * len(X)
+    D = hiprt.DeviceArray
+    dU, dF, dR, dx, dn, dK = D(ndof), D(ndof), D(ndof), D(ndof), D(3), D(nnz)
+    dUbar = D(ndof)
+    s = hiprt.Stream()
+    sp = s.ptr
+
+    def newton(t, Uguess, Ubar):
+        """Full Newton from Uguess toward Ubar on the prescribed DOFs; (converged, norms, PCG its)."""
+        dU.upload(Uguess)
+        dUbar.upload(Ubar)
+        its, pcg, norm0, norms = 0, [], None, []
+        while True:
+            m.evaluate_device(t, dU.ptr, dF.ptr, dK.ptr, sp)
+            m.residual_bc_device(its, dU.ptr, dF.ptr, dK.ptr, None, dUbar.ptr, dR.ptr, dn.ptr, 2, sp)
+            s.synchronize()
+            code, _ = m.device_status()
+            if code != 0:
+                raise RuntimeError("J2 evaluation status %d" % code)
+            nrm = float(dn.numpy()[2])
+            norms.append(nrm)
+            norm0 = nrm if its == 0 else norm0
+            if its > 0 and nrm < tol * norm0:
+                return True, norms, pcg
+            if its >= newton_maxit:
+                return False, norms, pcg
+            k, rr, flag = m.linear_solve_device(dK.ptr, dR.ptr, dx.ptr, pcg_rtol, 50000, sp)
+            if flag not in (1, 4):
+                raise RuntimeError("J2 linear solve flag %d (%d iterations, relres %g)" % (flag, k, rr))
+            m.newton_update_device(dU.ptr, dx.ptr, -1.0, sp)
+            pcg.append(k)
+            its += 1
+
+    U = np.zeros(ndof)
+    t_hist = time.perf_counter()
+    hist = []
+    dt = 4.0 / 40
+    for st in range(1, n_hist + 1):
+        ta = (st - 1) * dt
+        va = float(np.interp(ta, *zip(*J2_TABLE)))
+        vb = float(np.interp(ta + dt, *zip(*J2_TABLE)))
+        U[presc] = va
+        Uconv = U.copy()
+        amin, alpha, cut = 0.0, 1.0, 0
+        solves = []
+        while True:
+            Ubar = U.copy()
+            Ubar[presc] = va + alpha * (vb - va)
+            ok, norms, pcg = newton(ta + alpha * dt, Uconv, Ubar)
+            solves.append({"alpha": round(alpha, 6), "converged": ok, "newton_iterations": len(pcg),
+                           "pcg_iterations": sum(pcg), "relative_norm": norms[-1] / norms[0]})
+            if not ok:
+                alpha = amin + (1.0 - 1.0 / phi) * (alpha - amin)
+                cut += 1
+                if cut > max_cut_backs:
+                    raise RuntimeError("J2 step %d did not converge with %d cut backs: %s" % (st, max_cut_backs, solves))
+                continue
+            if alpha == 1.0:
+                break
+            aux, amin = amin, alpha
+            alpha = min(alpha + phi * (alpha - aux), 1.0)
+            Uconv = dU.numpy()
+        U = dU.numpy()
+        if st < n_hist:
+            m.switch_converged_state()
+        hist.append({"step": st, "u_x": vb, "cut_backs": cut, "solves": solves})
+    t_hist = time.perf_counter() - t_hist
+    # the F+K evaluation at step n_hist's converged U against the committed step n_hist - 1 state:
+    # what the last Newton evaluation of that step computes (Gauss points still loading return to
+    # the yield surface, the others stay elastic)
+    t = n_hist * dt
+    for _ in range(warmup):
+        m.evaluate_device(t, dU.ptr, dF.ptr, dK.ptr, sp)
+    s.synchronize()
+    m.timing_start(steps)
+    e0, e1 = hiprt.Event(), hiprt.Event()
+    e0.record(s)
+    for _ in range(steps):
+        m.evaluate_device(t, dU.ptr, dF.ptr, dK.ptr, sp)
+    e1.record(s)
+    e1.synchronize()
+    elem_ms, asm_ms, n_timed, n_launch = m.timing_stop()
+    ms = e0.elapsed_ms(e1) / steps
+    ne = len(conn)
+    flags = m.state(1)[..., 7]
+    launch_s = (elem_ms * 1e-3) / n_launch
+    tf = J2_FLOP_PER_ELEM * (ne * n_timed / n_launch) / launch_s / 1e12
+    out = {"metric": "elem/s assembled (F+K, hex8), J2 cyclic workload", "value": ne / (ms * 1e-3),
+           "unit": "elem/s", "ms_per_step": ms, "steps": steps,
+           "config": {"workload": "%d^3 hex8 cube, J2 plasticity E=210000 nu=0.3 sigma_y=250 H=1000 (MPa), linear "
+                                  "analysis, x=0 fixed, u_x on x=1 cycling 0 -> +2%% -> -2%% -> +2%% over 40 steps; "
+                                  "F+K timed at the state of step %d" % (n_cells, n_hist), "elements": ne, "dofs": ndof},
+           "plastic_fraction": float(flags.mean()),
+           "history": hist, "history_s": t_hist,
+           "roofline": {"bound": "mfma", "kernel": "k_element<2> (FP64 vector + LDS)", "achieved": round(tf, 3),
+                        "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s", "frac": round(tf / PEAK_FP64_TFLOPS, 4),
+                        "avg_launch_ms": round(launch_s * 1e3, 4), "flop_per_elem": J2_FLOP_PER_ELEM},
+           "assembly_ms": asm_ms / max(n_timed, 1)}
+    if not no_cpu:
+        out["cpu_baseline"], out["parity_full_size"] = j2_cpu_baseline(
+            X, conn, U, t, m.state(0), dF.numpy(), dK.numpy(), flags)
+        out["speedup_vs_cpu_baseline"] = out["value"] / out["cpu_baseline"]["value"]
+    return out
+
+
 def newton_cpu_baseline(n_cells, ux, tol, threads):
     """The reference's CPU path for the same load step: the CPU oracle (OpenMP) for the
     evaluations, the driver's restatement of the reference's BC code and MKL PARDISO with the
@@ -385,6 +544,8 @@ def main():
     ap.add_argument("--no-fe2", action="store_true", help="skip the FE^2 macro-step metric")
     ap.add_argument("--no-newton", action="store_true", help="skip the Newton load-step measurement")
     ap.add_argument("--newton-only", action="store_true", help="only the Newton load-step measurement")
+    ap.add_argument("--no-j2", action="store_true", help="skip the J2 cyclic workload")
+    ap.add_argument("--j2-only", action="store_true", help="only the J2 cyclic workload")
     ap.add_argument("--fe2-only", action="store_true",
                     help="only the FE^2 macro-step metric (prints its object alone; for profiling it)")
     ap.add_argument("--precision", choices=["fp64", "fp32k"], default="fp64")
@@ -408,6 +569,9 @@ def main():
         return
     if args.newton_only:
         print(json.dumps(newton_bench(args.no_cpu)), flush=True)
+        return
+    if args.j2_only:
+        print(json.dumps(j2_bench(no_cpu=args.no_cpu)), flush=True)
         return
     t_setup = time.perf_counter()
     X, conn = meshgen.cube(N_CELLS, perturb=0.1)
@@ -517,6 +681,8 @@ def main():
         out["cpu_baseline"] = cpu_baseline(X, conn, U)
         out["speedup_vs_cpu_baseline"] = value / out["cpu_baseline"]["value"]
     del mesh, dU, dF, dK
+    if not args.no_j2:
+        out["j2"] = j2_bench(no_cpu=args.no_cpu)
     if not args.no_newton:
         out["newton"] = newton_bench(args.no_cpu)
     if not args.no_fe2:
