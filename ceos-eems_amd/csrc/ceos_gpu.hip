@@ -310,15 +310,14 @@ static int build_gather_map(ceos_handle* h) {
             const long long base = (long long)h->iperm[e] * kRecord;
             for (int b = 0; b < 8; ++b) {
                 const int B = cn[b];
-                const long long lb = base + rec_off(b);
                 const int np = b < 4 ? 4 : 3;
                 int blks[5];
                 uint32_t words[5];
                 blks[0] = blkstart[B];
-                words[0] = (uint32_t)(lb + 3);
+                words[0] = (uint32_t)(base + 9 * piece_pos(b, 0) + 3);
                 for (int j = 0; j < np; ++j) {
                     const int C = cn[(b + j + 1) & 7];
-                    const uint32_t off = (uint32_t)(lb + 9 + 9 * j);
+                    const uint32_t off = (uint32_t)(base + 9 * piece_pos(b, 1 + j));
                     if (C < B) {
                         const int r = rank_of(h, C, B);
                         if (r < 0) return CEOS_ERR_SLOT;

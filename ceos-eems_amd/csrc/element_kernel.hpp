@@ -41,12 +41,26 @@ constexpr int kOffSig = 72;
 constexpr int kOffD = 120;
 constexpr int kJ2State = 8;           // [0..5] plastic strain, [6] acc. plastic strain, [7] flag
 constexpr int kStateStride = 20;      // state rows per Gauss point in the device arrays (max of models)
-// Element record of the gather assembly: lane b owns rec_off(b) .. + 45 (b < 4, four partner
-// blocks) or + 36 (b >= 4, three):
-//   [0..2] Fe_b, [3..8] diagonal block (b,b) upper (00 01 02 11 12 22),
-//   [9 + 9 j .. 17 + 9 j] block (c,b), c = (b + j + 1) mod 8, rows = dofs of c, row-major.
-constexpr int kRecord = 4 * 45 + 4 * 36;  // 324 doubles per element
-__host__ __device__ constexpr int rec_off(int b) { return b < 4 ? 45 * b : 180 + 36 * (b - 4); }
+// Element record of the gather assembly: 36 pieces of 9 doubles.  Lane b computes piece k = 0,
+// [Fe_b (3), diagonal block (b,b) upper 00 01 02 11 12 22], and pieces k = 1 + j, the block (c,b)
+// of its partner c = (b + j + 1) mod 8 (j < 4 for b < 4, j < 3 otherwise), rows = dofs of c,
+// row-major.  Pieces sit in the record grouped by the node whose CSR row receives them (the
+// diagonal of n, then every pair whose other node follows n), the groups ordered as the nodes of
+// a hex numbered lexicographically (x fastest: local 0 1 3 2 4 5 7 6).  On such meshes each
+// group is read by one node's gather threads, so the gather reads whole runs of 1 to 8 pieces
+// instead of lone 72-byte pieces; on any other mesh the placement is merely a fixed permutation
+// (the contribution map records where every piece is).
+constexpr int kRecord = 36 * 9;  // 324 doubles per element
+__host__ __device__ constexpr int lex_rank(int n) { return (n == 2 || n == 6) ? n + 1 : (n == 3 || n == 7) ? n - 1 : n; }
+__host__ __device__ constexpr int group_start(int t) { return 8 * t - t * (t - 1) / 2; }  // 8 - t pieces in group t
+// record position (in pieces) of piece k of lane b
+__host__ __device__ constexpr int piece_pos(int b, int k) {
+    const int rb = lex_rank(b);
+    if (k == 0) return group_start(rb);
+    const int rc = lex_rank((b + k) & 7);
+    const int r = rb < rc ? rb : rc, o = rb < rc ? rc : rb;
+    return group_start(r) + o - r;
+}
 
 struct ElemParams {
     int ne;                        // elements (device slot order)
@@ -448,7 +462,7 @@ __global__ __launch_bounds__(128) void k_element(ElemParams p, int e_begin, int 
     if constexpr (GATHER) {
         // stage the element records in LDS (the GP data are no longer read) and write the
         // records of the block's consecutive elements with unit-stride stores, in two halves:
-        // lanes 0-3 (record doubles 0..179), then lanes 4-7 (180..323); a slice holds 288
+        // pieces 0..19 (record doubles 0..179), then 20..35 (180..323); a slice holds 288
         const int e0 = e_begin + blockIdx.x * kEPB;
         const int nel = min(kEPB, e_end - e0);
         double* dst = p.scratch + (long)e0 * kRecord;
@@ -456,19 +470,24 @@ __global__ __launch_bounds__(128) void k_element(ElemParams p, int e_begin, int 
 #pragma unroll
         for (int half = 0; half < 2; ++half) {
             __syncthreads();
-            if ((b >= 4) == (half == 1)) {
-                double* rec = sm + rec_off(b) - (half ? kHalf0 : 0);
+            const int skip = half ? kHalf0 : 0;
+            const int p0 = piece_pos(b, 0);
+            if ((p0 >= 20) == (half == 1)) {
+                double* rec = sm + 9 * p0 - skip;
                 rec[0] = fe0;
                 rec[1] = fe1;
                 rec[2] = fe2;
 #pragma unroll
                 for (int q = 0; q < 6; ++q) rec[3 + q] = kd[q];
+            }
 #pragma unroll
-                for (int j = 0; j < 4; ++j)
-                    if (j < npart) {
+            for (int j = 0; j < 4; ++j) {
+                const int pj = piece_pos(b, 1 + j);
+                if (j < npart && (pj >= 20) == (half == 1)) {
+                    double* rec = sm + 9 * pj - skip;
 #pragma unroll
-                        for (int q = 0; q < 9; ++q) rec[9 + 9 * j + q] = ko[j][q];
-                    }
+                    for (int q = 0; q < 9; ++q) rec[q] = ko[j][q];
+                }
             }
             __syncthreads();
             const int len = half ? kHalf1 : kHalf0;
