@@ -166,6 +166,45 @@ def test_visco_relaxes_towards_equilibrium():
     assert abs(prev - s_inf[0]) < 0.05 * abs(s_inst[0] - s_inf[0])
 
 
+@pytest.mark.parametrize("dt", [0.1, 0.5, 2.0])
+def test_visco_small_strain_relaxation_rate(dt):
+    """Small-strain limit of the local problem (ModViscoelasticMatrix.f90:452-623): with
+    Fv = (I - dt dv)^-1 Fv_old, Me = Ce Se = Mu_e (Ce - I) and the residual
+    -dt sym(FvFo^T Me) + dt Ni_v dv + gamma dt Jv FvFo = 0, Jv = 1, a small held strain gives
+    the backward-Euler Maxwell branch Ni_v dv = 2 Mu_e dev(eps - eps_v): under a held shear the
+    overstress (stress minus the long-term branch) shrinks by exactly
+    r = 1 / (1 + 2 Mu_e dt / Ni_v) per step.  The deviation is second order in the strain
+    (1e-5 at 1e-2, 1e-7 at 1e-3).  A wrong role of Ni_v, dt or the factor 2 moves r by 10-60 %."""
+    g = 1e-3
+    F = np.eye(3)
+    F[0, 1] = F[1, 0] = g
+    s_inf = ol.material_point(ol.MODEL_CNH, [VISCO[1], VISCO[2]], F)[0][3]
+    tau0 = ol.material_point(ol.MODEL_CNH, [VISCO[1] + VISCO[4], VISCO[2]], F)[0][3] - s_inf
+    r = 1.0 / (1.0 + 2.0 * VISCO[4] * dt / VISCO[5])
+    st = _virgin_visco_state()
+    for n in range(1, 6):
+        s, _, st = ol.material_point(ol.MODEL_VISCO, VISCO, F, time=n * dt, state_old=st)
+        st = st.copy()
+        assert abs((s[3] - s_inf) / (tau0 * r ** n) - 1.0) < 1e-5
+
+
+@pytest.mark.parametrize("lam", [1.05, 0.9])
+def test_visco_pure_dilation_has_no_viscous_flow(lam):
+    """F = lam I: Ce = lam^2 I, so the residual vanishes at dv = 0 with the multiplier
+    gamma = Mu_e (lam^2 - 1) (the pressure that keeps the viscous flow isochoric): Fv stays I
+    and the stress stays the instantaneous one (Neo-Hookean with Mu_inf + Mu_e) at any time step,
+    at finite strain."""
+    F = lam * np.eye(3)
+    s0 = ol.material_point(ol.MODEL_CNH, [VISCO[1] + VISCO[4], VISCO[2]], F)[0]
+    st = _virgin_visco_state()
+    for n in range(1, 4):
+        s, _, st = ol.material_point(ol.MODEL_VISCO, VISCO, F, time=0.5 * n, state_old=st)
+        st = st.copy()
+        assert np.allclose(s, s0, rtol=1e-12, atol=1e-12)
+        assert np.allclose(st[:9], [1, 0, 0, 0, 1, 0, 0, 0, 1], rtol=0, atol=1e-14)
+        assert st[18] == pytest.approx(VISCO[4] * (lam * lam - 1.0), rel=1e-12)
+
+
 def test_visco_element_tangent_is_consistent():
     """Ke = dFe/du (finite differences) after a committed viscous step; the algorithmic tangent
     assumes an exactly converged local Newton (tolerance 1e-5), hence the 1e-6 bar."""
