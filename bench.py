@@ -100,6 +100,64 @@ def cpu_baseline(X, conn, U, n_sub=32, budget_s=12.0):
                       % (n, n_sub, len(sub_conn), threads)}
 
 
+# per element: NH(C10, K) constitutive update + tangent ~1,600 flop per GP instead of ~250 for CNH;
+# residual only: geometry 328 + F 472 + residual 144 per GP, plus the CNH stress
+FLOP_PER_ELEM_NH = FLOP_PER_ELEM - 250.0 + 8 * 1600.0
+FLOP_PER_ELEM_F = 8 * (328 + 472 + 144) + 250.0
+BYTES_PER_ELEM_FP32K = 605.0
+
+
+def variants_bench(X, conn, U, steps, warmup):
+    """The same C2 mesh and state in the other modes: Neo-Hookean (C10 = 0.5, K = 100) instead of
+    the compressible one; the FP32-stored tangent; the residual-only evaluation (Fint, no K)."""
+    from ceos_amd import gpu, hiprt
+    ne, nn = len(conn), len(X)
+    out = {}
+    for name, model, props, prec, want_k, flop, nbytes in (
+            ("neo_hookean_c10_k_fp64", gpu.MODEL_NH, [0.5, K_BULK], gpu.K_FP64, True, FLOP_PER_ELEM_NH,
+             BYTES_PER_ELEM),
+            ("cnh_fp32_stored_k", gpu.MODEL_CNH, [MU, LAM], gpu.K_FP32_STORED, True, FLOP_PER_ELEM,
+             BYTES_PER_ELEM_FP32K),
+            ("cnh_residual_only", gpu.MODEL_CNH, [MU, LAM], gpu.K_FP64, False, FLOP_PER_ELEM_F,
+             (72.0 * nn + 32.0 * ne) / ne)):
+        m = gpu.CeosMesh(X, conn, precision=prec)
+        m.set_material(1, model, props)
+        _, nnz = m.info()
+        dU = hiprt.DeviceArray.from_numpy(U)
+        dF = hiprt.DeviceArray(len(U))
+        dK = hiprt.DeviceArray(nnz, np.float64 if prec == gpu.K_FP64 else np.float32) if want_k else None
+        s = hiprt.Stream()
+        m.update_mesh_device(dU.ptr, s.ptr)
+        kp = dK.ptr if want_k else None
+        for _ in range(warmup):
+            m.evaluate_device(0.2, dU.ptr, dF.ptr, kp, s.ptr)
+        s.synchronize()
+        code, _ = m.device_status()
+        if code != 0:
+            raise RuntimeError("%s: evaluation status %d" % (name, code))
+        m.timing_start(steps)
+        e0, e1 = hiprt.Event(), hiprt.Event()
+        e0.record(s)
+        for _ in range(steps):
+            m.evaluate_device(0.2, dU.ptr, dF.ptr, kp, s.ptr)
+        e1.record(s)
+        e1.synchronize()
+        elem_ms, asm_ms, n_timed, n_launch = m.timing_stop()
+        ms = e0.elapsed_ms(e1) / steps
+        launch_s = elem_ms * 1e-3 / n_launch
+        tf = flop * (ne * n_timed / n_launch) / launch_s / 1e12
+        gbs = nbytes * ne / (ms * 1e-3) / 1e9
+        out[name] = {"value": ne / (ms * 1e-3), "unit": "elem/s", "ms_per_step": ms,
+                     "element_ms": round(launch_s * 1e3, 4), "assembly_ms": round(asm_ms / max(n_timed, 1), 4),
+                     "fp64_frac": round(tf / PEAK_FP64_TFLOPS, 4), "flop_per_elem": flop,
+                     "path_hbm_frac": round(gbs / PEAK_HBM_GBS, 4), "algorithmic_bytes_per_elem": round(nbytes, 1)}
+        for a in (dU, dF, dK):
+            if a is not None:
+                a.free()
+        m.close()
+    return out
+
+
 def macro_displacement(X):
     """Macro field of the FE^2 workload: a homogeneous stretch-shear plus a smooth fluctuation
     (principal stretches at the Gauss points within about [0.93, 1.09])."""
@@ -545,6 +603,8 @@ def main():
     ap.add_argument("--no-newton", action="store_true", help="skip the Newton load-step measurement")
     ap.add_argument("--newton-only", action="store_true", help="only the Newton load-step measurement")
     ap.add_argument("--no-j2", action="store_true", help="skip the J2 cyclic workload")
+    ap.add_argument("--no-variants", action="store_true",
+                    help="skip the NH (C10, K), FP32-stored K and residual-only timings of the same mesh")
     ap.add_argument("--j2-only", action="store_true", help="only the J2 cyclic workload")
     ap.add_argument("--fe2-only", action="store_true",
                     help="only the FE^2 macro-step metric (prints its object alone; for profiling it)")
@@ -681,6 +741,8 @@ def main():
         out["cpu_baseline"] = cpu_baseline(X, conn, U)
         out["speedup_vs_cpu_baseline"] = value / out["cpu_baseline"]["value"]
     del mesh, dU, dF, dK
+    if not args.no_variants:
+        out["variants"] = variants_bench(X, conn, U, args.steps, args.warmup)
     if not args.no_j2:
         out["j2"] = j2_bench(no_cpu=args.no_cpu)
     if not args.no_newton:

@@ -16,6 +16,8 @@
 #pragma once
 #include <stdint.h>
 
+#include "element_kernel.hpp"
+
 namespace ceos {
 
 struct GatherParams {
@@ -114,15 +116,40 @@ __global__ __launch_bounds__(256) void k_gather_blocks(GatherParams p) {
     K[r2 + 1 + o] = (KT)a[6]; K[r2 + 2 + o] = (KT)a[7]; K[r2 + 3 + o] = (KT)a[8];
 }
 
-// residual-only gather: Fint of every node from the Fe parts of its diagonal contributions
+// residual-only gather: Fint of node A from the Fe parts of its diagonal contributions, in
+// contribution order (the order k_gather_blocks sums them in).  In this pass the element kernel
+// writes only Fe, [slot][8][3]; the diagonal contribution word (slot * kRecord +
+// 9 piece_pos(b, 0) + 3) names the slot and the lane b.  One thread per node, its contribution
+// words and then their Fe parts requested kGatherBatch at a time.
 __global__ __launch_bounds__(256) void k_gather_fint(GatherParams p) {
-    const int t = blockIdx.x * 256 + threadIdx.x;
-    if (t >= 3 * p.nn) return;
-    const int A = t / 3, i = t - 3 * A;
+    const int A = blockIdx.x * 256 + threadIdx.x;
+    if (A >= p.nn) return;
     const int blk = p.blkstart[A];
-    double f = 0.0;
-    for (int c = p.cptr[blk]; c < p.cptr[blk + 1]; ++c) f += p.rec[(long)(p.cword[c] & 0x7fffffffu) - 3 + i];
-    p.Fint[t] = f;
+    const int c0 = p.cptr[blk], c1 = p.cptr[blk + 1];
+    double f[3] = {0.0, 0.0, 0.0};
+    for (int c = c0; c < c1; c += kGatherBatch) {
+        uint32_t w[kGatherBatch];
+#pragma unroll
+        for (int u = 0; u < kGatherBatch; ++u) w[u] = (c + u < c1) ? (p.cword[c + u] & 0x7fffffffu) : 0u;
+        double v[kGatherBatch][3];
+#pragma unroll
+        for (int u = 0; u < kGatherBatch; ++u)
+            if (c + u < c1) {
+                const uint32_t slot = w[u] / kRecord;
+                const int b = diag_lane((int)(w[u] - slot * kRecord - 3) / 9);
+                const double* d = p.rec + 24L * slot + 3 * b;
+#pragma unroll
+                for (int k = 0; k < 3; ++k) v[u][k] = d[k];
+            }
+#pragma unroll
+        for (int u = 0; u < kGatherBatch; ++u)
+            if (c + u < c1) {
+#pragma unroll
+                for (int k = 0; k < 3; ++k) f[k] += v[u][k];
+            }
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) p.Fint[3L * A + k] = f[k];
 }
 
 }  // namespace ceos

@@ -703,7 +703,7 @@ static int evaluate(ceos_handle* h, const double* dU, double* dFint, void* dK, h
             if (f32) hipLaunchKernelGGL(k_gather_blocks<float>, dim3(nbk), dim3(256), 0, s, g);
             else hipLaunchKernelGGL(k_gather_blocks<double>, dim3(nbk), dim3(256), 0, s, g);
         } else {
-            hipLaunchKernelGGL(k_gather_fint, dim3((3 * h->nn + 255) / 256), dim3(256), 0, s, g);
+            hipLaunchKernelGGL(k_gather_fint, dim3((h->nn + 255) / 256), dim3(256), 0, s, g);
         }
     }
     if (ev) {

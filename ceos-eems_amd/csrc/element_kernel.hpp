@@ -53,6 +53,12 @@ constexpr int kStateStride = 20;      // state rows per Gauss point in the devic
 constexpr int kRecord = 36 * 9;  // 324 doubles per element
 __host__ __device__ constexpr int lex_rank(int n) { return (n == 2 || n == 6) ? n + 1 : (n == 3 || n == 7) ? n - 1 : n; }
 __host__ __device__ constexpr int group_start(int t) { return 8 * t - t * (t - 1) / 2; }  // 8 - t pieces in group t
+// lane whose diagonal piece sits at record position pos (the inverse of piece_pos(b, 0))
+__host__ __device__ constexpr int diag_lane(int pos) {
+    int t = 0;
+    while (t < 7 && group_start(t + 1) <= pos) ++t;
+    return lex_rank(t);  // lex_rank is its own inverse
+}
 // record position (in pieces) of piece k of lane b
 __host__ __device__ constexpr int piece_pos(int b, int k) {
     const int rb = lex_rank(b);
@@ -77,7 +83,8 @@ struct ElemParams {
     double* Fint;                  // [3 nn]
     void* K;                       // [nnz] double or float
     double* sig;                   // [6][8][ne]
-    double* scratch;               // [ne][kRecord] element records (gather assembly)
+    double* scratch;               // [ne][kRecord] element records (gather assembly); [ne][8][3] Fe
+                                   // parts in the residual-only pass
     const double* st_old;          // [nstate][8][ne] committed state
     double* st_new;                // [nstate][8][ne] state of the last update
     uint8_t* elem_err;             // [ne]: bit 0 detJ <= 1e-13, bit 1 local Newton failed
@@ -458,6 +465,16 @@ __global__ __launch_bounds__(128) void k_element(ElemParams p, int e_begin, int 
                     for (int t = 0; t < 3; ++t) ko[j][3 * r + t] += kdiv * bc[j][r] * bb[t];
             }
         }
+    }
+    if constexpr (GATHER && !WANT_K) {
+        // residual only: the element's 8 Fe parts, 24 consecutive doubles at 24 slot
+        if (active) {
+            double* f = p.scratch + 24L * e + 3 * b;
+            f[0] = fe0;
+            f[1] = fe1;
+            f[2] = fe2;
+        }
+        return;
     }
     if constexpr (GATHER) {
         // stage the element records in LDS (the GP data are no longer read) and write the

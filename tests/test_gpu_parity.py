@@ -534,3 +534,37 @@ def test_pipeline_gives_bitwise_identical_results():
     for f, K, fd, kd in out[1:]:
         assert np.array_equal(f, out[0][0]) and np.array_equal(K, out[0][1])
         assert np.array_equal(fd, out[0][0]) and np.array_equal(kd, out[0][1])
+
+
+@pytest.mark.parametrize("mesh", ["lexicographic", "scrambled"])
+def test_residual_only_pass_matches_fint_of_full_evaluation(mesh):
+    """The residual-only evaluation (no K: the element kernel writes only the 24 Fe doubles of
+    each element, the Fint gather finds them from the diagonal contribution words) gives the
+    Fint of the F+K evaluation bitwise (same Fe, same contribution order), with two materials,
+    on a lexicographically numbered and on a scrambled mesh; and the oracle's Fint."""
+    from ceos_amd import hiprt
+    if mesh == "lexicographic":
+        X, conn = meshgen.cube(6, perturb=0.1, seed=9)
+    else:
+        X, conn = scrambled(5, seed=9)
+    ne = len(conn)
+    mat = np.where(np.arange(ne) % 4 == 1, 2, 1).astype(np.int32)
+    U = np.random.default_rng(9).normal(scale=0.01, size=3 * len(X))
+    g = gpu.CeosMesh(X, conn, mat_id=mat)
+    o = ol.OracleMesh(X, conn, mat_id=mat)
+    for m in (g, o):
+        m.set_material(1, ol.MODEL_CNH, [1.0, 99.0])
+        m.set_material(2, ol.MODEL_NH, [0.5, 50.0])
+    _, nnz = g.info()
+    dU = hiprt.DeviceArray.from_numpy(U)
+    dF, dF2 = hiprt.DeviceArray(len(U)), hiprt.DeviceArray(len(U))
+    dK = hiprt.DeviceArray(nnz)
+    g.update_mesh_device(dU.ptr)
+    g.evaluate_device(0.0, dU.ptr, dF.ptr, dK.ptr)
+    g.evaluate_device(0.0, dU.ptr, dF2.ptr, None)
+    hiprt.synchronize()
+    assert g.device_status()[0] == 0
+    assert np.array_equal(dF2.numpy(), dF.numpy())
+    o.update_mesh(U)
+    assert o.solve_constitutive(0.0, U) == 0
+    assert rel(dF2.numpy(), o.internal_force()[0]) < RTOL
