@@ -33,10 +33,14 @@ namespace ceos {
 
 constexpr int kEPB = 16;              // elements per workgroup (128 threads = 2 waves)
 // doubles per element in LDS between the phases: J^-1 of the current configuration 8x9, w detJ
-// sigma 8x6, w detJ D (upper) 8x21.  Phase 2 recomputes grad_x N from J^-1 and the natural
+// sigma 8x6, w detJ D (upper) 8x21 = 288.  Phase 2 recomputes grad_x N from J^-1 and the natural
 // derivatives (the 192 doubles of grad_x N itself would take 52.7 KB per workgroup, 3 workgroups
-// per CU; 288 doubles take 36.9 KB, 4 workgroups, the 8 waves the register count allows)
-constexpr int kLdsPerElem = 288;
+// per CU).  The slice stride is padded to 296: the 8 elements of a wavefront read the same offset
+// of their slices at once, and a stride of 288 doubles (576 dwords, a multiple of the bank count)
+// puts all 8 on the same banks; 296 (592 dwords) moves each slice by 16 banks and keeps the
+// slices 64-byte aligned for 128-bit LDS reads.  37.9 KB per workgroup: still 4 workgroups per
+// CU, the 8 waves the register count allows.
+constexpr int kLdsPerElem = 296;
 constexpr int kOffSig = 72;
 constexpr int kOffD = 120;
 constexpr int kJ2State = 8;           // [0..5] plastic strain, [6] acc. plastic strain, [7] flag
@@ -479,7 +483,7 @@ __global__ __launch_bounds__(128) void k_element(ElemParams p, int e_begin, int 
     if constexpr (GATHER) {
         // stage the element records in LDS (the GP data are no longer read) and write the
         // records of the block's consecutive elements with unit-stride stores, in two halves:
-        // pieces 0..19 (record doubles 0..179), then 20..35 (180..323); a slice holds 288
+        // pieces 0..19 (record doubles 0..179), then 20..35 (180..323); a slice holds 296
         const int e0 = e_begin + blockIdx.x * kEPB;
         const int nel = min(kEPB, e_end - e0);
         double* dst = p.scratch + (long)e0 * kRecord;
