@@ -38,8 +38,9 @@ constexpr int kEPB = 16;              // elements per workgroup (128 threads = 2
 // per CU).  The slice stride is padded to 296: the 8 elements of a wavefront read the same offset
 // of their slices at once, and a stride of 288 doubles (576 dwords, a multiple of the bank count)
 // puts all 8 on the same banks; 296 (592 dwords) moves each slice by 16 banks and keeps the
-// slices 64-byte aligned for 128-bit LDS reads.  37.9 KB per workgroup: still 4 workgroups per
-// CU, the 8 waves the register count allows.
+// slices 64-byte aligned for 128-bit LDS reads.  37.9 KB per workgroup, plus the 2 KB table of
+// dN/dxi phase 2 reads (39.9 KB): still 4 workgroups per CU, the 8 waves the register count
+// allows.
 constexpr int kLdsPerElem = 296;
 constexpr int kOffSig = 72;
 constexpr int kOffD = 120;
@@ -198,9 +199,11 @@ __device__ __noinline__ void tangent_numerical(const double* pr, const double F0
 //   Fe_b = sum_g B_b^T sigma, the upper diagonal block K_bb (00 01 02 11 12 22) and the blocks
 //   K_cb (rows = dofs of c, cols = dofs of b) for c = (b + j + 1) mod 8, j < 4 (b < 4) or j < 3,
 //   K_cb = sum_g B_c^T D B_b + (gradN_c . sigma gradN_b) I (the geometric part for NL).
-template <bool NL, bool WANT_K>
+// TABLE: dN/dxi of node a at Gauss point g read from dnt[(8 g + a) * 4 + k] (an LDS table)
+// instead of evaluated (9 FP64 instructions per node and Gauss point)
+template <bool NL, bool WANT_K, bool TABLE = false>
 __device__ __forceinline__ void node_blocks(const double* sm, int b, double& fe0, double& fe1, double& fe2,
-                                            double kd[6], double ko[4][9]) {
+                                            double kd[6], double ko[4][9], const double* dnt = nullptr) {
     const int npart = (b < 4) ? 4 : 3;
     fe0 = 0.0;
     fe1 = 0.0;
@@ -221,7 +224,12 @@ __device__ __forceinline__ void node_blocks(const double* sm, int b, double& fe0
         double gb[3];
         {
             double d[3];
-            hex8_dshape_node(b, gp, d);
+            if constexpr (TABLE) {
+#pragma unroll
+                for (int k = 0; k < 3; ++k) d[k] = dnt[(gp * 8 + b) * 4 + k];
+            } else {
+                hex8_dshape_node(b, gp, d);
+            }
             grad_from_jinv(Ji, d, gb);
         }
         const double b0 = gb[0], b1 = gb[1], b2 = gb[2];
@@ -261,7 +269,12 @@ __device__ __forceinline__ void node_blocks(const double* sm, int b, double& fe0
                 if (j < npart) {
                     const int c = (b + j + 1) & 7;
                     double gc[3], d[3];
-                    hex8_dshape_node(c, gp, d);
+                    if constexpr (TABLE) {
+#pragma unroll
+                        for (int k = 0; k < 3; ++k) d[k] = dnt[(gp * 8 + c) * 4 + k];
+                    } else {
+                        hex8_dshape_node(c, gp, d);
+                    }
                     grad_from_jinv(Ji, d, gc);
                     const double c0 = gc[0], c1 = gc[1], c2 = gc[2];
                     const double geo = NL ? (c0 * sg0 + c1 * sg1 + c2 * sg2) : 0.0;
@@ -286,6 +299,15 @@ template <int MODEL, bool NL, bool WANT_K, typename KT, bool GATHER, bool NUMT =
 __global__ __launch_bounds__(128) void k_element(ElemParams p, int e_begin, int e_end) {
     __shared__ double smem[kEPB * kLdsPerElem];
     __shared__ int snode[GATHER ? 1 : kEPB][8];  // partner node ids (scatter assembly only)
+    __shared__ double sdn[8 * 8 * 4];             // dN/dxi of node a at Gauss point g: [g][a][3 + pad]
+    for (int i = threadIdx.x; i < 64; i += kEPB * 8) {
+        double d[3];
+        hex8_dshape_node(i & 7, i >> 3, d);
+        sdn[4 * i] = d[0];
+        sdn[4 * i + 1] = d[1];
+        sdn[4 * i + 2] = d[2];
+        sdn[4 * i + 3] = 0.0;
+    }
     const int lane = threadIdx.x & 7;
     const int le = threadIdx.x >> 3;
     const int e = e_begin + blockIdx.x * kEPB + le;
@@ -434,7 +456,7 @@ __global__ __launch_bounds__(128) void k_element(ElemParams p, int e_begin, int 
     double fe0, fe1, fe2;
     double kd[6];
     double ko[4][9];
-    node_blocks<NL, WANT_K>(sm, b, fe0, fe1, fe2, kd, ko);
+    node_blocks<NL, WANT_K, true>(sm, b, fe0, fe1, fe2, kd, ko, sdn);
     if constexpr (MODEL == kNHQ1 && NL && WANT_K) {
         // Ke += d2Psi/dJbar2 / V0 Bdiv^T Bdiv (ModElement.f90:464-467): block (c,b) gains
         // kdiv Bdiv_c (x) Bdiv_b.  The lane-dependent rows are selected with unrolled compares
