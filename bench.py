@@ -67,7 +67,7 @@ def read_traffic():
     return float(m.group(1)) if m else None
 
 
-def cpu_baseline(X, conn, U, n_sub=32, budget_s=12.0):
+def cpu_baseline(X, conn, U, n_sub=32, budget_s=12.0, threads=None):
     sys.path.insert(0, os.path.join(REPO, "tests"))
     import oracle_lib as ol
     m1 = N_CELLS + 1
@@ -80,7 +80,7 @@ def cpu_baseline(X, conn, U, n_sub=32, budget_s=12.0):
     remap[nodes] = np.arange(1, len(nodes) + 1)
     Xs = X[nodes - 1]
     Us = U.reshape(-1, 3)[nodes - 1].reshape(-1)
-    threads = min(os.cpu_count() or 1, 16)
+    threads = threads or min(os.cpu_count() or 1, 16)
     o = ol.OracleMesh(Xs, remap[sub_conn], threads=threads)
     o.set_material(1, ol.MODEL_CNH, [MU, LAM])
     o.update_mesh(Us)
@@ -96,8 +96,8 @@ def cpu_baseline(X, conn, U, n_sub=32, budget_s=12.0):
     dt = time.perf_counter() - t0
     return {"value": n * len(sub_conn) / dt, "unit": "elem/s", "cores": threads, "kind": "port",
             "sample": "%d evaluations (F+K, FP64) of the %d^3 corner sub-cube (%d hex8) of the same "
-                      "perturbed mesh and displacement field, CPU oracle with %d OpenMP threads"
-                      % (n, n_sub, len(sub_conn), threads)}
+                      "perturbed mesh and displacement field, CPU oracle with %d OpenMP thread%s"
+                      % (n, n_sub, len(sub_conn), threads, "" if threads == 1 else "s")}
 
 
 # per element: NH(C10, K) constitutive update + tangent ~1,600 flop per GP instead of ~250 for CNH;
@@ -740,6 +740,8 @@ def main():
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(X, conn, U)
         out["speedup_vs_cpu_baseline"] = value / out["cpu_baseline"]["value"]
+        # the serial figure beside it: the same oracle on one thread, on the 16^3 corner sub-cube
+        out["cpu_baseline_serial"] = cpu_baseline(X, conn, U, n_sub=16, budget_s=6.0, threads=1)
     del mesh, dU, dF, dK
     if not args.no_variants:
         out["variants"] = variants_bench(X, conn, U, args.steps, args.warmup)
