@@ -439,6 +439,20 @@ __global__ __launch_bounds__(kSysBlock) void k_cg_update(int nn, int par, const 
                                                          const double* __restrict__ p, const double* __restrict__ q,
                                                          double* x, double* r, double* z, double* part_s,
                                                          double* part_m) {
+    // the node's operands are requested first: their latency overlaps the stop-flag read and
+    // the reduction of p.q (a thread past nn loads node 0 and discards it)
+    const int A = blockIdx.x * kSysBlock + threadIdx.x;
+    const int Al = A < nn ? A : 0;
+    double pv[3], qv[3], xv[3], rv0[3], mi[6];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        pv[i] = p[3L * Al + i];
+        qv[i] = q[3L * Al + i];
+        xv[i] = x[3L * Al + i];
+        rv0[i] = r[3L * Al + i];
+    }
+#pragma unroll
+    for (int k = 0; k < 6; ++k) mi[k] = Minv[(long)k * nn + Al];
     if (cg_stopped(c)) return;
     double pq, unused;
     reduce2(pq_part, nullptr, nblk, pq, unused);
@@ -448,17 +462,18 @@ __global__ __launch_bounds__(kSysBlock) void k_cg_update(int nn, int par, const 
     }
     const double alpha = c->rz[par] / pq;
     if (blockIdx.x == 0 && threadIdx.x == 0) c->alpha = alpha;
-    const int A = blockIdx.x * kSysBlock + threadIdx.x;
     double rz = 0.0, rr = 0.0;
     if (A < nn) {
         double rv[3], zv[3];
 #pragma unroll
         for (int i = 0; i < 3; ++i) {
-            x[3L * A + i] += alpha * p[3L * A + i];
-            rv[i] = r[3L * A + i] - alpha * q[3L * A + i];
+            x[3L * A + i] = xv[i] + alpha * pv[i];
+            rv[i] = rv0[i] - alpha * qv[i];
             r[3L * A + i] = rv[i];
         }
-        precond(Minv, nn, A, rv, zv);
+        zv[0] = mi[0] * rv[0] + mi[1] * rv[1] + mi[2] * rv[2];  // precond() on the loaded inverse
+        zv[1] = mi[1] * rv[0] + mi[3] * rv[1] + mi[4] * rv[2];
+        zv[2] = mi[2] * rv[0] + mi[4] * rv[1] + mi[5] * rv[2];
 #pragma unroll
         for (int i = 0; i < 3; ++i) {
             z[3L * A + i] = zv[i];
@@ -475,6 +490,15 @@ __global__ __launch_bounds__(kSysBlock) void k_cg_update(int nn, int par, const 
 __global__ __launch_bounds__(kSysBlock) void k_cg_pupdate(int nn, int par, CgScalars* c, const double* rz_part,
                                                           const double* rr_part, int nblk,
                                                           const double* __restrict__ z, double* __restrict__ p) {
+    // z and p of the node are requested before the stop-flag read and the reductions
+    const int A = blockIdx.x * kSysBlock + threadIdx.x;
+    const int Al = A < nn ? A : 0;
+    double zv[3], pv[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        zv[i] = z[3L * Al + i];
+        pv[i] = p[3L * Al + i];
+    }
     if (cg_stopped(c)) return;
     double rz_new, rr;
     reduce2(rz_part, rr_part, nblk, rz_new, rr);
@@ -487,10 +511,9 @@ __global__ __launch_bounds__(kSysBlock) void k_cg_pupdate(int nn, int par, CgSca
         if (rr <= c->tol2 * c->bb) __atomic_store_n(&c->done, 1, __ATOMIC_RELAXED);
         else if (c->iters >= c->maxit) __atomic_store_n(&c->done, 3, __ATOMIC_RELAXED);
     }
-    const int A = blockIdx.x * kSysBlock + threadIdx.x;
     if (A < nn) {
 #pragma unroll
-        for (int i = 0; i < 3; ++i) p[3L * A + i] = z[3L * A + i] + beta * p[3L * A + i];
+        for (int i = 0; i < 3; ++i) p[3L * A + i] = zv[i] + beta * pv[i];
     }
 }
 
