@@ -925,8 +925,10 @@ static int pcg_run(ceos_handle* h, const double* db, double* dx, double rtol, in
                        h->dcg_z, h->dcg_p, h->dpart, h->dpart2);
     hipLaunchKernelGGL(k_cg_init_final, dim3(1), dim3(kSysBlock), 0, s, (const double*)h->dpart,
                        (const double*)h->dpart2, nblk, h->dcg);
-    // per iteration: q = K p (partials of p.q in dpart3); alpha, x, r, z (partials of r.z and r.r
-    // in dpart, dpart2); beta, test, p.  r.z of iteration k sits in rz[k & 1].
+    // per iteration: q = K p (partials of p.q in dpart3, one per node block); alpha, x, r, z
+    // (partials of r.z and r.r in dpart, dpart2, one per update workgroup); beta, test, p.  r.z of
+    // iteration k sits in rz[k & 1].
+    const int nupd = std::min(nblk, kCgUpdBlocks);
     CgScalars c;
     int k = 0;
     for (;;) {
@@ -943,11 +945,11 @@ static int pcg_run(ceos_handle* h, const double* db, double* dx, double rtol, in
                 hipLaunchKernelGGL((k_cg_spmv<double, kCgSlots64>), dim3(nblk), dim3(kSysBlock), 0, s, g,
                                    (const double*)h->dell, (const CgScalars*)h->dcg, (const double*)h->dcg_p, h->dcg_q,
                                    h->dpart3);
-            hipLaunchKernelGGL(k_cg_update, dim3(nblk), dim3(kSysBlock), 0, s, nn, par, (const double*)h->dMinv, h->dcg,
+            hipLaunchKernelGGL(k_cg_update, dim3(nupd), dim3(kSysBlock), 0, s, nn, par, (const double*)h->dMinv, h->dcg,
                                (const double*)h->dpart3, nblk, (const double*)h->dcg_p, (const double*)h->dcg_q, dx,
                                h->dcg_r, h->dcg_z, h->dpart, h->dpart2);
-            hipLaunchKernelGGL(k_cg_pupdate, dim3(nblk), dim3(kSysBlock), 0, s, nn, par, h->dcg, (const double*)h->dpart,
-                               (const double*)h->dpart2, nblk, (const double*)h->dcg_z, h->dcg_p);
+            hipLaunchKernelGGL(k_cg_pupdate, dim3(nupd), dim3(kSysBlock), 0, s, nn, par, h->dcg, (const double*)h->dpart,
+                               (const double*)h->dpart2, nupd, (const double*)h->dcg_z, h->dcg_p);
         }
         HIP_OK(hipGetLastError());
     }

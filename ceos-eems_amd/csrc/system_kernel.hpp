@@ -39,6 +39,9 @@ constexpr int kSysBlock = 256;
 constexpr int kCgSlots = 3;    // neighbour slots per load batch of the FP32 product (1, 2, 3, 9, 27
                                // measured 0.163, 0.156, 0.154, 0.180, 0.262 ms per 80^3 PCG iteration)
 constexpr int kCgSlots64 = 3;  // and of the FP64 one
+// workgroups of k_cg_update and k_cg_pupdate (grid-stride over the nodes): every workgroup of the
+// next kernel sums all their partials, so fewer partials cut that re-read traffic
+constexpr int kCgUpdBlocks = 1024;
 
 struct SysGeom {
     int nn;
@@ -439,20 +442,23 @@ __global__ __launch_bounds__(kSysBlock) void k_cg_update(int nn, int par, const 
                                                          const double* __restrict__ p, const double* __restrict__ q,
                                                          double* x, double* r, double* z, double* part_s,
                                                          double* part_m) {
-    // the node's operands are requested first: their latency overlaps the stop-flag read and
-    // the reduction of p.q (a thread past nn loads node 0 and discards it)
-    const int A = blockIdx.x * kSysBlock + threadIdx.x;
-    const int Al = A < nn ? A : 0;
+    // the first node's operands are requested first: their latency overlaps the stop-flag read
+    // and the reduction of p.q (a thread past nn loads node 0 and discards it)
+    const int stride = gridDim.x * kSysBlock;
+    int A = blockIdx.x * kSysBlock + threadIdx.x;
     double pv[3], qv[3], xv[3], rv0[3], mi[6];
+    auto load = [&](int n) {
 #pragma unroll
-    for (int i = 0; i < 3; ++i) {
-        pv[i] = p[3L * Al + i];
-        qv[i] = q[3L * Al + i];
-        xv[i] = x[3L * Al + i];
-        rv0[i] = r[3L * Al + i];
-    }
+        for (int i = 0; i < 3; ++i) {
+            pv[i] = p[3L * n + i];
+            qv[i] = q[3L * n + i];
+            xv[i] = x[3L * n + i];
+            rv0[i] = r[3L * n + i];
+        }
 #pragma unroll
-    for (int k = 0; k < 6; ++k) mi[k] = Minv[(long)k * nn + Al];
+        for (int k = 0; k < 6; ++k) mi[k] = Minv[(long)k * nn + n];
+    };
+    load(A < nn ? A : 0);
     if (cg_stopped(c)) return;
     double pq, unused;
     reduce2(pq_part, nullptr, nblk, pq, unused);
@@ -463,7 +469,8 @@ __global__ __launch_bounds__(kSysBlock) void k_cg_update(int nn, int par, const 
     const double alpha = c->rz[par] / pq;
     if (blockIdx.x == 0 && threadIdx.x == 0) c->alpha = alpha;
     double rz = 0.0, rr = 0.0;
-    if (A < nn) {
+    for (bool first = true; A < nn; A += stride, first = false) {
+        if (!first) load(A);
         double rv[3], zv[3];
 #pragma unroll
         for (int i = 0; i < 3; ++i) {
@@ -490,15 +497,18 @@ __global__ __launch_bounds__(kSysBlock) void k_cg_update(int nn, int par, const 
 __global__ __launch_bounds__(kSysBlock) void k_cg_pupdate(int nn, int par, CgScalars* c, const double* rz_part,
                                                           const double* rr_part, int nblk,
                                                           const double* __restrict__ z, double* __restrict__ p) {
-    // z and p of the node are requested before the stop-flag read and the reductions
-    const int A = blockIdx.x * kSysBlock + threadIdx.x;
-    const int Al = A < nn ? A : 0;
+    // z and p of the first node are requested before the stop-flag read and the reductions
+    const int stride = gridDim.x * kSysBlock;
+    int A = blockIdx.x * kSysBlock + threadIdx.x;
     double zv[3], pv[3];
+    auto load = [&](int n) {
 #pragma unroll
-    for (int i = 0; i < 3; ++i) {
-        zv[i] = z[3L * Al + i];
-        pv[i] = p[3L * Al + i];
-    }
+        for (int i = 0; i < 3; ++i) {
+            zv[i] = z[3L * n + i];
+            pv[i] = p[3L * n + i];
+        }
+    };
+    load(A < nn ? A : 0);
     if (cg_stopped(c)) return;
     double rz_new, rr;
     reduce2(rz_part, rr_part, nblk, rz_new, rr);
@@ -511,7 +521,8 @@ __global__ __launch_bounds__(kSysBlock) void k_cg_pupdate(int nn, int par, CgSca
         if (rr <= c->tol2 * c->bb) __atomic_store_n(&c->done, 1, __ATOMIC_RELAXED);
         else if (c->iters >= c->maxit) __atomic_store_n(&c->done, 3, __ATOMIC_RELAXED);
     }
-    if (A < nn) {
+    for (bool first = true; A < nn; A += stride, first = false) {
+        if (!first) load(A);
 #pragma unroll
         for (int i = 0; i < 3; ++i) p[3L * A + i] = zv[i] + beta * pv[i];
     }
