@@ -327,6 +327,8 @@ class CeosMesh:
 
     # device-resident path (pointers are integers, e.g. torch.Tensor.data_ptr())
     def evaluate_device(self, time, dU, dFint, dK, stream=None):
+        """ceos_evaluate_device on device pointers (ints); dK None = residual only (Fint without
+        forming the tangent).  Asynchronous on `stream` (None = the handle's stream)."""
         _check(load_library().ceos_evaluate_device(self._h, time, dU, dFint, dK, stream),
                "ceos_evaluate_device")
 
