@@ -537,7 +537,9 @@ __global__ __launch_bounds__(128) void k_element(ElemParams p, int e_begin, int 
             const int off = half ? kHalf0 : 0;
             for (int g2 = threadIdx.x; g2 < nel * len; g2 += kEPB * 8) {
                 const int el = g2 / len, r = g2 - el * len;
-                dst[(long)el * kRecord + off + r] = smem[el * kLdsPerElem + r];
+                // non-temporal: the records are read back once, by the gather, long after this
+                // workgroup ends (1.33 GB per evaluation at 80^3), so they need not hold L2 lines
+                __builtin_nontemporal_store(smem[el * kLdsPerElem + r], dst + (long)el * kRecord + off + r);
             }
         }
         return;
