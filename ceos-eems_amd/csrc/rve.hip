@@ -99,13 +99,19 @@ __global__ void k_rve_gather(int n_sel, const int* sel, const double* P, const d
 }
 
 // solve = false: the condensation only (the autotuner's probe)
-template <int MODEL, typename MT, int G>
-void launch(const RveParams& p, int grid, bool solve, bool tangent, hipStream_t s) {
-    if (solve) hipLaunchKernelGGL((k_rve_solve<MODEL, MT, G>), dim3(grid), dim3(kRveThreads), 0, s, p);
+template <int MODEL, typename MT, int G, bool NT>
+void launch_nt(const RveParams& p, int grid, bool solve, bool tangent, hipStream_t s) {
+    if (solve) hipLaunchKernelGGL((k_rve_solve<MODEL, MT, G, NT>), dim3(grid), dim3(kRveThreads), 0, s, p);
     if (tangent) {
         hipLaunchKernelGGL(k_zero_queue, dim3(1), dim3(1), 0, s, p.queue);
-        hipLaunchKernelGGL((k_rve_tangent<MODEL, MT, G>), dim3(grid), dim3(kRveThreads), 0, s, p);
+        hipLaunchKernelGGL((k_rve_tangent<MODEL, MT, G, NT>), dim3(grid), dim3(kRveThreads), 0, s, p);
     }
+}
+
+template <int MODEL, typename MT, int G>
+void launch(const RveParams& p, int grid, bool solve, bool tangent, hipStream_t s) {
+    if (p.nt_tangent) launch_nt<MODEL, MT, G, true>(p, grid, solve, tangent, s);
+    else launch_nt<MODEL, MT, G, false>(p, grid, solve, tangent, s);
 }
 
 template <int MODEL, typename MT>
@@ -129,6 +135,8 @@ int reset(ceos_rve_batch* b) {
     RVE_HIP_OK(hipStreamSynchronize(b->stream));
     return CEOS_OK;
 }
+
+constexpr int kRveNtQueue = 128;  // queue length from which the CG reads tangents non-temporally
 
 // one persistent launch over the queue `order` (n_queue RVE ids, largest first); solve = false
 // runs the condensation alone at the RVEs' current states
@@ -160,6 +168,11 @@ int run(ceos_rve_batch* b, bool want_A, hipStream_t s, const int* order, int n_q
     p.pcg_maxit = b->pcg_maxit;
     p.max_cutbacks = b->max_cutbacks;
     p.inexact = b->inexact;
+    // A full batch streams far more reduced tangents than the caches hold (256 slots x up to 4 MB
+    // in FP32 at 16^3): its CG reads them non-temporally (the plain FE^2 evaluation: 17.5 -> 15.6 s).
+    // A short queue (judges, fallbacks: tens of RVEs) keeps ordinary loads, whose tangents the
+    // 256 MB last-level cache can hold between CG iterations.
+    p.nt_tangent = n_queue >= kRveNtQueue ? 1 : 0;
     p.prof = b->dprof;
     hipLaunchKernelGGL(k_zero_queue, dim3(1), dim3(1), 0, s, b->dqueue);
     const int grid = std::min(n_queue, b->slots);

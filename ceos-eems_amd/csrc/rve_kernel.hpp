@@ -66,6 +66,8 @@ struct RveParams {
     double tangent_rtol;          // CG tolerance of the condensation solves
     int newton_maxit, pcg_maxit, max_cutbacks;
     int inexact;                  // 1: CG tolerance follows the Newton residual (forcing term)
+    int nt_tangent;               // 1: launch the kernels whose CG reads the reduced tangent with
+                                  // non-temporal loads (NT = true)
     long long* prof;              // [n_rve][2] clock ticks in assembly, in CG (or nullptr)
 };
 
@@ -288,7 +290,10 @@ __device__ bool rve_assemble(const RveCtx<MT>& c, const RveParams& p, const doub
 // its column never multiplies a nonzero since v_0 = 0).  The 81 tangent entries of one z-plane of
 // neighbours are loaded before any of them is used (the scheduling barrier keeps the loads
 // together), so a row costs three HBM round trips instead of one per block.
-template <typename MT>
+// NT: the tangent rows are read with non-temporal loads (a full batch streams far more tangents
+// than the caches hold; a compile-time choice, as a runtime branch here costs the condensation
+// kernel registers)
+template <typename MT, bool NT>
 __device__ __forceinline__ void rve_row(const RveCtx<MT>& c, int i, const double* v, double y[3]) {
     const int n = c.n, nn = c.nn;
     const int ix = i % n, iy = (i / n) % n, iz = i / (n * n);
@@ -302,7 +307,10 @@ __device__ __forceinline__ void rve_row(const RveCtx<MT>& c, int i, const double
         const MT* mp = c.M + (long long)(81 * (dz + 1)) * nn + i;
         double m[81];
 #pragma unroll
-        for (int q = 0; q < 81; ++q) m[q] = (double)mp[(long long)q * nn];
+        for (int q = 0; q < 81; ++q) {
+            if constexpr (NT) m[q] = (double)__builtin_nontemporal_load(mp + (long long)q * nn);
+            else m[q] = (double)mp[(long long)q * nn];
+        }
         __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
         for (int by = 0; by < 3; ++by) {
@@ -359,7 +367,7 @@ __device__ __forceinline__ void apply_minv(const double* mi, const double r[3], 
 constexpr int kCgGroups[3] = {2, 4, 8};
 constexpr int kCgGroupDefault = 4;
 
-template <typename MT, int G>
+template <typename MT, int G, bool NT>
 __device__ int rve_pcg(const RveCtx<MT>& c, const RveParams& p, const double* rhs, double sign, double rtol,
                        double* pl, double* red) {
     const int tid = threadIdx.x;
@@ -410,7 +418,7 @@ __device__ int rve_pcg(const RveCtx<MT>& c, const RveParams& p, const double* rh
             if (i == 0) {
                 y[0] = y[1] = y[2] = 0.0;
             } else {
-                rve_row(c, i, pl, y);
+                rve_row<MT, NT>(c, i, pl, y);
             }
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
@@ -672,7 +680,7 @@ __device__ __forceinline__ RveCtx<MT> rve_ctx(const RveParams& p, int rid, long 
     return c;
 }
 
-template <int MODEL, typename MT, int G>
+template <int MODEL, typename MT, int G, bool NT>
 __global__ __launch_bounds__(kRveThreads) void k_rve_solve(RveParams p) {
     __shared__ double lds[kRveLds];
     __shared__ double red[kRveWaves * 45];
@@ -735,7 +743,7 @@ __global__ __launch_bounds__(kRveThreads) void k_rve_solve(RveParams p) {
                 // reduction still to come (forcing 0.1 ||R|| / ||R_0||, within [pcg_rtol, 1e-2])
                 const double eta = p.inexact ? fmin(1e-2, fmax(p.pcg_rtol, 0.1 * nr / nr0)) : p.pcg_rtol;
                 const long long t1 = wall_clock64();
-                pcg_total += rve_pcg<MT, G>(c, p, c.R, -1.0, eta, lds, red);
+                pcg_total += rve_pcg<MT, G, NT>(c, p, c.R, -1.0, eta, lds, red);
                 if (p.prof && tid == 0) p.prof[2 * rid + 1] += wall_clock64() - t1;
                 for (int k = tid; k < 3 * c.nn; k += kRveThreads) c.wc[k] += c.x[k];
                 __syncthreads();
@@ -781,7 +789,7 @@ __global__ __launch_bounds__(kRveThreads) void k_rve_solve(RveParams p) {
 // launch, after k_rve_solve): the reduced tangent at the converged state is assembled again,
 //   A = (Phi^T K Phi + sum_m b_m^T v_m) / V,  K_red v_m = -b_m,  b_m = T^T K Phi_m,
 // nine CG solves per RVE.  RVEs whose solve failed are skipped.
-template <int MODEL, typename MT, int G>
+template <int MODEL, typename MT, int G, bool NT>
 __global__ __launch_bounds__(kRveThreads) void k_rve_tangent(RveParams p) {
     __shared__ double lds[kRveLds];
     __shared__ double red[kRveWaves * 45];
@@ -825,7 +833,7 @@ __global__ __launch_bounds__(kRveThreads) void k_rve_tangent(RveParams p) {
         // condensed part b_k . v_m, v_m = K_red^-1 (-b_m)
         double* Aout = p.A + 81LL * rid;
         for (int m = 0; m < 9; ++m) {
-            pcg_total += rve_pcg<MT, G>(c, p, c.B + (long long)m * 3 * nn, -1.0, p.tangent_rtol, lds, red);
+            pcg_total += rve_pcg<MT, G, NT>(c, p, c.B + (long long)m * 3 * nn, -1.0, p.tangent_rtol, lds, red);
             double dots[9];
 #pragma unroll
             for (int k = 0; k < 9; ++k) dots[k] = 0.0;
