@@ -389,23 +389,30 @@ template <typename VT, int G>
 __global__ __launch_bounds__(kSysBlock) void k_cg_spmv(SysGeom g, const VT* __restrict__ val, const CgScalars* c,
                                                        const double* __restrict__ p, double* __restrict__ q,
                                                        double* part_s) {
-    if (cg_stopped(c)) return;
     const int A = blockIdx.x * kSysBlock + threadIdx.x;
+    // the first slot batch is requested before the stop-flag read (a thread past nn reads node
+    // 0's slots and discards them)
+    const int Al = A < g.nn ? A : 0;
+    const int* nb = g.sell_nbr + (long)(Al >> 6) * g.width * kSellChunk + (Al & 63);
+    const VT* vb = val + sell_val(g.width, Al, 0, 0);
+    int B[G];
+    VT v[G][9];
+    auto load = [&](int j0) {
+#pragma unroll
+        for (int u = 0; u < G; ++u) B[u] = nb[(j0 + u) * kSellChunk];
+#pragma unroll
+        for (int u = 0; u < G; ++u)
+#pragma unroll
+            for (int m = 0; m < 9; ++m) v[u][m] = __builtin_nontemporal_load(vb + ((j0 + u) * 9 + m) * kSellChunk);
+    };
+    if (G <= g.width) load(0);
+    if (cg_stopped(c)) return;
     double pq = 0.0;
     if (A < g.nn) {
         double acc[3] = {0.0, 0.0, 0.0};
-        const int* nb = g.sell_nbr + (long)(A >> 6) * g.width * kSellChunk + (A & 63);
-        const VT* vb = val + sell_val(g.width, A, 0, 0);
         int j0 = 0;
         for (; j0 + G <= g.width; j0 += G) {
-            int B[G];
-            VT v[G][9];
-#pragma unroll
-            for (int u = 0; u < G; ++u) B[u] = nb[(j0 + u) * kSellChunk];
-#pragma unroll
-            for (int u = 0; u < G; ++u)
-#pragma unroll
-                for (int m = 0; m < 9; ++m) v[u][m] = __builtin_nontemporal_load(vb + ((j0 + u) * 9 + m) * kSellChunk);
+            if (j0 > 0) load(j0);
             double pb[G][3];
 #pragma unroll
             for (int u = 0; u < G; ++u)
@@ -419,12 +426,12 @@ __global__ __launch_bounds__(kSysBlock) void k_cg_spmv(SysGeom g, const VT* __re
                               (double)v[u][3 * i + 2] * pb[u][2];
         }
         for (int j = j0; j < g.width; ++j) {
-            const int B = nb[j * kSellChunk];
-            const VT* v = vb + j * 9 * kSellChunk;
+            const int Bj = nb[j * kSellChunk];
+            const VT* vj = vb + j * 9 * kSellChunk;
 #pragma unroll
             for (int i = 0; i < 3; ++i)
-                acc[i] += (double)v[(3 * i) * kSellChunk] * p[3L * B] + (double)v[(3 * i + 1) * kSellChunk] * p[3L * B + 1] +
-                          (double)v[(3 * i + 2) * kSellChunk] * p[3L * B + 2];
+                acc[i] += (double)vj[(3 * i) * kSellChunk] * p[3L * Bj] + (double)vj[(3 * i + 1) * kSellChunk] * p[3L * Bj + 1] +
+                          (double)vj[(3 * i + 2) * kSellChunk] * p[3L * Bj + 2];
         }
 #pragma unroll
         for (int i = 0; i < 3; ++i) {
