@@ -4,6 +4,7 @@
   external forces and prescribed displacement FEM/BoundaryConditions/ModBoundaryConditions.f90:540-640
   Newton, relative test on the max-abs norm   Tools/MathTools/NonlinearSolver/ModNewtonRaphsonFull.f90:44-168
   FEMAnalysis.result records                  FEM/ModFEMAnalysis.f90:1094-1129, 1211-1212, 1314-1336
+  restart from FEMAnalysis.result             FEM/ModFEMAnalysis.f90:2827-2956 (replay), 1476-1516 (resume)
 
 Every evaluation, the boundary conditions, the residual norm and the linear solves run in
 libceos_gpu.so (ceos_solve_constitutive, ceos_residual_bc, ceos_linear_solve: PCG, and MINRES
@@ -75,13 +76,71 @@ def newton(mesh, time, fext, Ubar, Xguess, nl, tol, maxiter, norms, absolute, li
             mesh.update_mesh(X)
 
 
-def run(case, out_dir=None, linear_rtol=1e-12, minres_rtol=1e-12, absolute=False, verbose=False):
+def replay_results(mesh, path, nl):
+    """RestartAnalysis (FEM/ModFEMAnalysis.f90:2827-2956): every end-of-step record of
+    FEMAnalysis.result, in file order, sets U, moves the mesh (finite strain), updates the
+    constitutive model at the record's time and commits the state.  Returns (U, last load case,
+    last step).  The time-zero record is written with load case 1, step 1 and the end-of-step
+    flag, so a file holding no finished step gives (1, 1), as in the reference."""
+    ndof, recs = caseio.read_results(path)
+    if ndof != 3 * mesh.nn:
+        raise caseio.CaseError("FEMAnalysis.result holds %d DOFs, the mesh %d" % (ndof, 3 * mesh.nn))
+    U, last = np.zeros(ndof), None
+    for r in recs:
+        if r["flag_end_step"] != 1:
+            continue
+        U = r["U"].copy()
+        last = (r["load_case"], r["step"])
+        if nl:
+            mesh.update_mesh(U)
+        st = mesh.solve_constitutive(r["time"], U)
+        if st:
+            raise NewtonFailed("restart: evaluation status %d at load case %d step %d" % (st, *last))
+        mesh.switch_converged_state()
+    if last is None:
+        raise caseio.CaseError("FEMAnalysis.result holds no end-of-step record")
+    return U, last[0], last[1]
+
+
+def restart_start(steps, last_lc, last_st):
+    """First load case of a restarted analysis and the step to resume within it
+    (FEM/ModFEMAnalysis.f90:1476-1514); steps[lc - 1] = number of steps of load case lc.
+    Returns (init_lc, init_st), or None when the last step of the last load case is done."""
+    nlc = len(steps)
+    if nlc > last_lc:
+        if steps[last_lc - 1] > last_st:
+            init_lc = last_lc
+        elif steps[last_lc - 1] == last_st:
+            init_lc, last_st = last_lc + 1, 0
+        else:
+            raise caseio.CaseError("Error restarting load case.")
+    elif nlc == last_lc:
+        init_lc = last_lc
+    else:
+        raise caseio.CaseError("Error restarting load case.")
+    if steps[init_lc - 1] > last_st:
+        return init_lc, last_st + 1
+    if steps[init_lc - 1] == last_st:
+        return None  # "Last load step already calculated!" (the only load case left)
+    raise caseio.CaseError("Error restarting load step.")
+
+
+def run(case, out_dir=None, linear_rtol=1e-12, minres_rtol=1e-12, absolute=False, verbose=False,
+        restart=None, stop_after=None, attempts=None):
     """The quasi-static analysis of `case`.  Writes FEMAnalysis.result, the probe files and
     NumberOfIterationsToConverge.dat into out_dir when given.  Returns (history, norms): history
     rows (load case, step, time, U, iterations) per converged step; norms, the residual norms of
     every Newton solve (one list per substep attempt).  absolute=True stops Newton at
     norm < tol instead of norm < tol * first norm: the test of the binary that wrote the
-    reference's Checker logs."""
+    reference's Checker logs.
+
+    restart (default: the case's Restart option) resumes from out_dir/FEMAnalysis.result: the
+    end-of-step records are replayed through the constitutive update and the state commit, and
+    the loop continues after the last finished step, appending to the file.  The reference
+    accepts the option for its periodic multiscale analysis only (ModReadInputFile.f90:287-301);
+    the replay and the resume rules are the same for this quasi-static loop.  stop_after =
+    (load case, step) ends the run after that step, as an interrupted analysis would.
+    attempts, a list, receives (load case, step, alpha, converged) for every Newton solve."""
     s = case.settings
     mesh = make_mesh(case)
     mesh.set_bc(case.presc_dof, case.fixed_dof)
@@ -91,15 +150,27 @@ def run(case, out_dir=None, linear_rtol=1e-12, minres_rtol=1e-12, absolute=False
     U = np.zeros(n)
     history, norms = [], []
     res = None
-    if out_dir:
+    restart = s.get("restart", False) if restart is None else restart
+    steps = [c["steps"] for c in case.load_cases]
+    start = (1, 1)
+    if restart:
+        path = os.path.join(out_dir or "", "FEMAnalysis.result")
+        if not out_dir or not os.path.exists(path):
+            raise caseio.CaseError("no FEMAnalysis.result found")
+        U, last_lc, last_st = replay_results(mesh, path, nl)
+        start = restart_start(steps, last_lc, last_st)
+        res = open(path, "a")
+    elif out_dir:
         os.makedirs(out_dir, exist_ok=True)
         res = open(os.path.join(out_dir, "FEMAnalysis.result"), "w")
         caseio.write_results_header(res, n)
         caseio.write_results_record(res, U, 0.0, 1, 1, 0, 0, 1, 0)
     try:
         for lc, c in enumerate(case.load_cases, start=1):
+            if start is None or lc < start[0]:
+                continue
             dt = (c["t1"] - c["t0"]) / c["steps"]
-            for stp in range(1, c["steps"] + 1):
+            for stp in range(start[1] if lc == start[0] else 1, c["steps"] + 1):
                 ta, tb = c["t0"] + (stp - 1) * dt, c["t0"] + stp * dt
                 fext0, dfext = case.fext(ta), case.fext(tb) - case.fext(ta)
                 dpresc = np.zeros(n)
@@ -122,6 +193,8 @@ def run(case, out_dir=None, linear_rtol=1e-12, minres_rtol=1e-12, absolute=False
                     except NewtonFailed:
                         ok = False
                     norms.append(nrm)
+                    if attempts is not None:
+                        attempts.append((lc, stp, alpha, ok))
                     if verbose:
                         print("LC %d step %d alpha %.4f: %s" % (lc, stp, alpha, " ".join("%.3e" % v for v in nrm)),
                               flush=True)
@@ -131,22 +204,29 @@ def run(case, out_dir=None, linear_rtol=1e-12, minres_rtol=1e-12, absolute=False
                         if nl:
                             mesh.update_mesh(U)
                         cutback += 1
+                        substep = 1
                         if cutback > s["max_cut_backs"]:
                             raise NewtonFailed("load case %d step %d did not converge with %d cut backs"
                                                % (lc, stp, s["max_cut_backs"]))
                     elif alpha == 1.0:
+                        substep += 1
                         history.append((lc, stp, t, U.copy(), its))
                         if res:
                             caseio.write_results_record(res, U, t, lc, stp, cutback, substep, 1, its)
                         break
                     else:
+                        substep += 1
                         if res:
                             caseio.write_results_record(res, U, t, lc, stp, cutback, substep, 0, its)
-                        substep += 1
                         aux, amin = amin, alpha
                         alpha = min(alpha + GOLDEN_RATIO * (alpha - aux), 1.0)
                         Uconv = U.copy()
                 mesh.switch_converged_state()
+                if stop_after is not None and (lc, stp) == tuple(stop_after):
+                    start = None
+                    break
+            if start is None:
+                break
     finally:
         if res:
             res.close()

@@ -98,6 +98,10 @@ def read_settings(path):
     if tech is None:
         raise CaseError("element technology %r is not supported" % an.get("elementtechnology"))
     nl = _key(an.get("nonlinearanalysis", "true")) == "true"
+    # Restart (ModReadInputFile.f90:287-301): True resumes from the FEMAnalysis.result of the case
+    restart = _key(an.get("restart", "false"))
+    if restart not in ("true", "false"):
+        raise CaseError("Restart option not identified or not implemented")
     ns = _options(sec.get("nonlinearsolver", []))
     mats = {}
     lines = sec.get("material", [])
@@ -131,6 +135,7 @@ def read_settings(path):
         "nl_analysis": nl,
         "element_technology": tech,
         "max_cut_backs": int(_num(an.get("maximumcutbacks", "10"))),
+        "restart": restart == "true",
         "newton_tol": _num(ns.get("tol", "1e-3")),
         "newton_maxiter": int(_num(ns.get("maxiter", "10"))),
         "materials": mats,

@@ -230,9 +230,8 @@ class MacroFE2:
         self.judge_ids = None
         self.graphs = {}
         # set up the mesh's device tables now: graph capture must find nothing left to allocate
+        self.mesh.prepare()
         self.dU.upload(np.zeros(self.ndof))
-        self.mesh.gp_deformation_gradient_device(self.dU.ptr, self.dF.ptr, self.stream.ptr)
-        self.stream.synchronize()
 
     def setup_descriptors(self, pcg_rtol=1e-8):
         t = time.perf_counter()

@@ -70,6 +70,7 @@ EXPORTS = [
     "ceos_set_pipeline", "ceos_rve_autotune", "ceos_rve_set_cg_group",
     "ceos_bc_set", "ceos_residual_bc", "ceos_residual_bc_device", "ceos_set_solver_precision",
     "ceos_linear_solve", "ceos_linear_solve_device", "ceos_newton_update_device", "ceos_set_linear_solver",
+    "ceos_mesh_prepare",
 ]
 
 
@@ -112,6 +113,7 @@ def load_library(path=LIB_PATH):
     L.ceos_state_save.argtypes = [vp, C.c_char_p]
     L.ceos_state_load.argtypes = [vp, C.c_char_p]
     L.ceos_evaluate_device.argtypes = [vp, d, vp, vp, vp, vp]
+    L.ceos_mesh_prepare.argtypes = [vp, i]
     L.ceos_update_mesh_device.argtypes = [vp, vp, vp]
     L.ceos_device_status.argtypes = [vp, C.POINTER(CeosStatus)]
     L.ceos_mesh_info.argtypes = [vp, C.POINTER(i), C.POINTER(C.c_longlong)]
@@ -331,6 +333,11 @@ class CeosMesh:
         forming the tangent).  Asynchronous on `stream` (None = the handle's stream)."""
         _check(load_library().ceos_evaluate_device(self._h, time, dU, dFint, dK, stream),
                "ceos_evaluate_device")
+
+    def prepare(self, records=True):
+        """ceos_mesh_prepare: build the evaluation tables (and the element-record buffer), so the
+        next evaluate_device allocates nothing and may be captured in a HIP graph."""
+        _check(load_library().ceos_mesh_prepare(self._h, 1 if records else 0), "ceos_mesh_prepare")
 
     def set_external(self, dext):
         """Device pointer (int) of the [ne][8][27] external material points (MODEL_EXT)."""

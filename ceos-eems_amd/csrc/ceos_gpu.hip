@@ -63,6 +63,14 @@ hipError_t dalloc(T*& p, size_t n) {
     return hipMalloc((void**)&p, sizeof(T) * (n ? n : 1));
 }
 
+// buffers whose size is fixed for the life of a handle (it depends on nn, ne or the pattern only):
+// allocated once and kept across table rebuilds, so their addresses never change
+template <typename T>
+hipError_t dalloc_once(T*& p, size_t n) {
+    if (p) return hipSuccess;
+    return hipMalloc((void**)&p, sizeof(T) * (n ? n : 1));
+}
+
 void set_status(ceos_status* st, int code, int nfail, const char* msg) {
     if (!st) return;
     st->error = code != 0;
@@ -140,7 +148,10 @@ struct ceos_handle {
     uint8_t *ddiag_first = nullptr, *delem_err = nullptr;
     int* derr = nullptr;
     double* dhpart = nullptr;  // homogenisation partial sums
-    double* dscratch = nullptr;  // element records (gather assembly)
+    double* dscratch = nullptr;  // element records (gather assembly), allocated by ensure_records
+    // ordering of the handle's own stream after device entries that ran on a caller's stream
+    hipEvent_t ev_caller = nullptr;
+    bool caller_pending = false;
     int *dblknode = nullptr, *dblkstart = nullptr, *dcptr = nullptr;
     uint32_t* dcword = nullptr;
     int hpart_n = 0;
@@ -280,7 +291,7 @@ static int upload_props(ceos_handle* h) {
 }
 
 // contribution map of the gather assembly (see assembly_kernel.hpp), element-id order inside
-// every block, and the element record buffer
+// every block
 static int build_gather_map(ceos_handle* h) {
     const int nn = h->nn, ne = h->ne;
     if ((long long)ne * kRecord >= (1LL << 31)) return CEOS_ERR_ARG;
@@ -341,11 +352,10 @@ static int build_gather_map(ceos_handle* h) {
             }
         }
     }
-    HIP_OK(dalloc(h->dblkstart, nn));
-    HIP_OK(dalloc(h->dblknode, nb));
-    HIP_OK(dalloc(h->dcptr, nb + 1));
-    HIP_OK(dalloc(h->dcword, (size_t)ne * 36));
-    HIP_OK(dalloc(h->dscratch, (size_t)ne * kRecord));
+    HIP_OK(dalloc_once(h->dblkstart, nn));
+    HIP_OK(dalloc_once(h->dblknode, nb));
+    HIP_OK(dalloc_once(h->dcptr, nb + 1));
+    HIP_OK(dalloc_once(h->dcword, (size_t)ne * 36));
     HIP_OK(hipMemcpy(h->dblkstart, blkstart.data(), sizeof(int) * nn, hipMemcpyHostToDevice));
     HIP_OK(hipMemcpy(h->dblknode, blknode.data(), sizeof(int) * nb, hipMemcpyHostToDevice));
     HIP_OK(hipMemcpy(h->dcptr, cptr.data(), sizeof(int) * (nb + 1), hipMemcpyHostToDevice));
@@ -511,19 +521,19 @@ static int finalize(ceos_handle* h) {
         }
     }
 
-    HIP_OK(dalloc(h->dconn, (size_t)ne * 8));
-    HIP_OK(dalloc(h->dmat, ne));
-    HIP_OK(dalloc(h->dperm, ne));
+    HIP_OK(dalloc_once(h->dconn, (size_t)ne * 8));
+    HIP_OK(dalloc_once(h->dmat, ne));
+    HIP_OK(dalloc_once(h->dperm, ne));
     HIP_OK(hipMemcpyAsync(h->dperm, h->perm.data(), sizeof(int) * ne, hipMemcpyHostToDevice, h->stream));
     if (colored) {
-        HIP_OK(dalloc(h->dpairs, (size_t)ne * 8));
-        HIP_OK(dalloc(h->ddiag_first, ne));
+        HIP_OK(dalloc_once(h->dpairs, (size_t)ne * 8));
+        HIP_OK(dalloc_once(h->ddiag_first, ne));
     } else {
         dfree(h->dpairs);
         dfree(h->ddiag_first);
     }
-    HIP_OK(dalloc(h->delem_err, ne));
-    HIP_OK(dalloc(h->dsig, (size_t)ne * 48));
+    HIP_OK(dalloc_once(h->delem_err, ne));
+    HIP_OK(dalloc_once(h->dsig, (size_t)ne * 48));
     HIP_OK(hipMemcpyAsync(h->dconn, conn_s.data(), sizeof(int) * conn_s.size(), hipMemcpyHostToDevice, h->stream));
     HIP_OK(hipMemcpyAsync(h->dmat, mat_s.data(), sizeof(int) * ne, hipMemcpyHostToDevice, h->stream));
     if (colored) {
@@ -535,8 +545,8 @@ static int finalize(ceos_handle* h) {
     HIP_OK(hipMemsetAsync(h->dsig, 0, sizeof(double) * (size_t)ne * 48, h->stream));
     if (h->has_state) {
         const size_t n = (size_t)ne * 8 * kStateStride;
-        HIP_OK(dalloc(h->dst_old, n));
-        HIP_OK(dalloc(h->dst_new, n));
+        HIP_OK(dalloc_once(h->dst_old, n));
+        HIP_OK(dalloc_once(h->dst_new, n));
         if (!keep_old.empty()) {
             std::vector<double> a(n), b(n);
             for (int sl = 0; sl < ne; ++sl) {
@@ -566,6 +576,47 @@ static int finalize(ceos_handle* h) {
     HIP_OK(hipStreamSynchronize(h->stream));
     h->finalized = true;
     return CEOS_OK;
+}
+
+// element record buffer of the gather assembly (records of every element, ne * kRecord doubles)
+static int ensure_records(ceos_handle* h) {
+    if (h->assembly != CEOS_ASSEMBLY_GATHER) return CEOS_OK;
+    HIP_OK(dalloc_once(h->dscratch, (size_t)h->ne * kRecord));
+    return CEOS_OK;
+}
+
+static bool stream_capturing(hipStream_t s) {
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(s, &cs) != hipSuccess) return false;
+    return cs != hipStreamCaptureStatusNone;
+}
+
+// A device entry that queued work on a caller's stream: the entries that run on the handle's
+// own stream (state commit, status, state and stress reads, host-array calls) wait for it.
+// Work queued while the caller's stream is being captured is ordered by whoever launches the
+// graph, not by the library.
+static int note_caller_stream(ceos_handle* h, hipStream_t s) {
+    if (s == h->stream || stream_capturing(s)) return CEOS_OK;
+    HIP_OK(hipEventRecord(h->ev_caller, s));
+    h->caller_pending = true;
+    return CEOS_OK;
+}
+
+static int after_caller_stream(ceos_handle* h) {
+    if (!h->caller_pending) return CEOS_OK;
+    HIP_OK(hipStreamWaitEvent(h->stream, h->ev_caller, 0));
+    h->caller_pending = false;
+    return CEOS_OK;
+}
+
+// tables of the evaluation (finalize) and the record buffer; refuses to build them on a stream
+// under capture (building synchronises and allocates)
+static int prepare_for(ceos_handle* h, hipStream_t s) {
+    if (h->finalized && (h->assembly != CEOS_ASSEMBLY_GATHER || h->dscratch)) return CEOS_OK;
+    if (stream_capturing(s)) return CEOS_ERR_ARG;
+    int rc = finalize(h);
+    if (rc) return rc;
+    return ensure_records(h);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -608,8 +659,12 @@ static void launch_range(const ElemParams& p, const LaunchRange& r, bool nl, boo
 
 // one evaluation: sigma/state for every GP, Fint, and (if K != null) the tangent
 static int evaluate(ceos_handle* h, const double* dU, double* dFint, void* dK, hipStream_t s) {
-    int st = finalize(h);
+    int st = prepare_for(h, s);
     if (st) return st;
+    if (s == h->stream) {
+        st = after_caller_stream(h);
+        if (st) return st;
+    }
     if (h->has_ext && !h->dext) return CEOS_ERR_ARG;  // ceos_set_external_gp was not called
     ElemParams p;
     p.ne = h->ne;
@@ -717,6 +772,8 @@ static int evaluate(ceos_handle* h, const double* dU, double* dFint, void* dK, h
 
 static int read_status(ceos_handle* h, ceos_status* st) {
     int nerr[2] = {0, 0};
+    int rc = after_caller_stream(h);
+    if (rc) return rc;
     HIP_OK(hipMemcpyAsync(nerr, h->derr, sizeof(nerr), hipMemcpyDeviceToHost, h->stream));
     HIP_OK(hipStreamSynchronize(h->stream));
     if (nerr[0] || nerr[1]) {
@@ -1082,7 +1139,18 @@ int ceos_mesh_create(int nn, const double* X, int ne, int npe, const int* conn, 
         delete h;
         return CEOS_ERR_HIP;
     }
-    auto fail = [&](int code) { free_device(h); (void)hipStreamDestroy(h->stream); delete h; return code; };
+    if (hipEventCreateWithFlags(&h->ev_caller, hipEventDisableTiming) != hipSuccess) {
+        (void)hipStreamDestroy(h->stream);
+        delete h;
+        return CEOS_ERR_HIP;
+    }
+    auto fail = [&](int code) {
+        free_device(h);
+        (void)hipEventDestroy(h->ev_caller);
+        (void)hipStreamDestroy(h->stream);
+        delete h;
+        return code;
+    };
     if (dalloc(h->dX, 3L * nn) || dalloc(h->dx, 3L * nn) || dalloc(h->dU, 3L * nn) ||
         dalloc(h->dFint, 3L * nn) || dalloc(h->drowstart, nn) || dalloc(h->dnup, nn) ||
         dalloc(h->dprops, (size_t)kMaxMat * kMaxProps) || dalloc(h->derr, 2))
@@ -1114,6 +1182,7 @@ void ceos_mesh_destroy(ceos_handle* h) {
     timing_free(h);
     free_device(h);
     for (hipEvent_t e : h->pipe_ev) (void)hipEventDestroy(e);
+    (void)hipEventDestroy(h->ev_caller);
     if (h->stream2) (void)hipStreamDestroy(h->stream2);
     (void)hipStreamDestroy(h->stream);
     delete h;
@@ -1251,8 +1320,10 @@ int ceos_csr_pattern(ceos_handle* h, int upper, int* n, long long* nnz, int* row
 
 int ceos_solve_constitutive(ceos_handle* h, double time, const double* U, ceos_status* st) {
     if (!h || !U) return CEOS_ERR_ARG;
-    int rc = finalize(h);
+    int rc = prepare_for(h, h->stream);
     if (rc) { set_status(st, rc, 0, "material not set or mesh setup failed"); return rc; }
+    rc = after_caller_stream(h);
+    if (rc) return rc;
     HIP_OK(hipMemcpyAsync(h->dU, U, sizeof(double) * 3L * h->nn, hipMemcpyHostToDevice, h->stream));
     h->have_u = true;
     h->time = time;
@@ -1318,6 +1389,8 @@ int ceos_update_mesh(ceos_handle* h, const double* U) {
     if (!h || !U) return CEOS_ERR_ARG;
     std::vector<double> x(3L * h->nn);
     for (long i = 0; i < 3L * h->nn; ++i) x[i] = h->X[i] + U[i];
+    int rc = after_caller_stream(h);
+    if (rc) return rc;
     HIP_OK(hipMemcpyAsync(h->dx, x.data(), sizeof(double) * x.size(), hipMemcpyHostToDevice, h->stream));
     HIP_OK(hipStreamSynchronize(h->stream));
     h->f_valid = h->k_valid = false;
@@ -1326,6 +1399,8 @@ int ceos_update_mesh(ceos_handle* h, const double* U) {
 
 int ceos_switch_converged_state(ceos_handle* h) {
     if (!h) return CEOS_ERR_ARG;
+    int rc = after_caller_stream(h);
+    if (rc) return rc;
     if (h->finalized && h->has_state) {
         HIP_OK(hipMemcpyAsync(h->dst_old, h->dst_new, sizeof(double) * (size_t)h->ne * 8 * kStateStride,
                               hipMemcpyDeviceToDevice, h->stream));
@@ -1337,6 +1412,7 @@ int ceos_switch_converged_state(ceos_handle* h) {
 
 int ceos_failed_elements(ceos_handle* h, int* ids, int cap) {
     if (!h) return CEOS_ERR_ARG;
+    if (after_caller_stream(h)) return CEOS_ERR_HIP;
     HIP_OK(hipStreamSynchronize(h->stream));
     if (!h->finalized) return 0;
     std::vector<uint8_t> flags(h->ne);
@@ -1351,6 +1427,7 @@ int ceos_failed_elements(ceos_handle* h, int* ids, int cap) {
 
 int ceos_get_gp_stress(ceos_handle* h, double* sig) {
     if (!h || !sig || !h->finalized) return CEOS_ERR_ARG;
+    if (after_caller_stream(h)) return CEOS_ERR_HIP;
     HIP_OK(hipStreamSynchronize(h->stream));
     const int ne = h->ne;
     std::vector<double> d((size_t)ne * 48);
@@ -1365,6 +1442,7 @@ int ceos_get_gp_stress(ceos_handle* h, double* sig) {
 
 int ceos_get_state(ceos_handle* h, int which, double* state) {
     if (!h || !state) return CEOS_ERR_ARG;
+    if (after_caller_stream(h)) return CEOS_ERR_HIP;
     HIP_OK(hipStreamSynchronize(h->stream));
     int rc = finalize(h);
     if (rc) return rc;
@@ -1386,6 +1464,7 @@ int ceos_get_state(ceos_handle* h, int which, double* state) {
 
 int ceos_set_state(ceos_handle* h, int which, const double* state) {
     if (!h || !state) return CEOS_ERR_ARG;
+    if (after_caller_stream(h)) return CEOS_ERR_HIP;
     HIP_OK(hipStreamSynchronize(h->stream));
     int rc = finalize(h);
     if (rc) return rc;
@@ -1405,6 +1484,7 @@ int ceos_set_state(ceos_handle* h, int which, const double* state) {
 
 int ceos_homogenize(ceos_handle* h, double P[9], double F[9]) {
     if (!h || !P || !F || !h->have_u || !h->finalized) return CEOS_ERR_ARG;
+    if (after_caller_stream(h)) return CEOS_ERR_HIP;
     const int nblk = (h->ne * 8 + kHomBlock - 1) / kHomBlock;
     if (h->hpart_n < nblk) {
         HIP_OK(dalloc(h->dhpart, (size_t)nblk * kHomVals));
@@ -1431,6 +1511,9 @@ int ceos_state_save(ceos_handle* h, const char* path) {
     if (!h || !path) return CEOS_ERR_ARG;
     int rc = finalize(h);
     if (rc) return rc;
+    rc = after_caller_stream(h);
+    if (rc) return rc;
+    HIP_OK(hipStreamSynchronize(h->stream));
     FILE* f = fopen(path, "wb");
     if (!f) return CEOS_ERR_ARG;
     const int hdr[4] = {0x43454f53, h->nn, h->ne, h->has_state ? 1 : 0};
@@ -1452,6 +1535,9 @@ int ceos_state_load(ceos_handle* h, const char* path) {
     if (!h || !path) return CEOS_ERR_ARG;
     int rc = finalize(h);
     if (rc) return rc;
+    rc = after_caller_stream(h);
+    if (rc) return rc;
+    HIP_OK(hipStreamSynchronize(h->stream));
     FILE* f = fopen(path, "rb");
     if (!f) return CEOS_ERR_ARG;
     int hdr[4];
@@ -1474,17 +1560,27 @@ int ceos_evaluate_device(ceos_handle* h, double time, const double* dU, double* 
     if (!h || !dU || !dFint) return CEOS_ERR_ARG;
     h->time = time;
     hipStream_t s = stream ? (hipStream_t)stream : h->stream;
-    return evaluate(h, dU, dFint, dK, s);
+    int rc = evaluate(h, dU, dFint, dK, s);
+    if (rc) return rc;
+    return note_caller_stream(h, s);
+}
+
+int ceos_mesh_prepare(ceos_handle* h, int records) {
+    if (!h) return CEOS_ERR_ARG;
+    int rc = finalize(h);
+    if (rc) return rc;
+    return records ? ensure_records(h) : CEOS_OK;
 }
 
 int ceos_update_mesh_device(ceos_handle* h, const double* dU, void* stream) {
     if (!h || !dU) return CEOS_ERR_ARG;
     hipStream_t s = stream ? (hipStream_t)stream : h->stream;
+    if (s == h->stream && after_caller_stream(h)) return CEOS_ERR_HIP;
     const long n = 3L * h->nn;
     hipLaunchKernelGGL(k_add, dim3((n + 255) / 256), dim3(256), 0, s, n, h->dX, dU, h->dx);
     HIP_OK(hipGetLastError());
     h->f_valid = h->k_valid = false;
-    return CEOS_OK;
+    return note_caller_stream(h, s);
 }
 
 int ceos_timing_start(ceos_handle* h, int max_evaluations) {
@@ -1539,7 +1635,7 @@ int ceos_gp_deformation_gradient_device(ceos_handle* h, const double* dU, double
     hipLaunchKernelGGL(k_gp_deformation_gradient, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, h->ne,
                        h->dconn, h->dperm, h->dX, dU, dF);
     HIP_OK(hipGetLastError());
-    return CEOS_OK;
+    return note_caller_stream(h, s);
 }
 
 int ceos_gp_deformation_gradient(ceos_handle* h, const double* U, double* F) {
@@ -1593,7 +1689,9 @@ int ceos_residual_bc_device(ceos_handle* h, int it, const double* dU, const doub
                             void* stream) {
     if (!h || !dFint || !dR || !dnorm || it < 0) return CEOS_ERR_ARG;
     hipStream_t s = stream ? (hipStream_t)stream : h->stream;
-    return residual_bc(h, it, dU, dFint, dK, dFext, dUbar, dR, norm_type, dnorm, s);
+    int rc = residual_bc(h, it, dU, dFint, dK, dFext, dUbar, dR, norm_type, dnorm, s);
+    if (rc) return rc;
+    return note_caller_stream(h, s);
 }
 
 int ceos_residual_bc(ceos_handle* h, int it, const double* Fext, const double* Ubar, double* R, int norm_type,
@@ -1641,7 +1739,9 @@ int ceos_linear_solve_device(ceos_handle* h, const void* dK, const double* db, d
                              int* iters, double* relres, int* flag, void* stream) {
     if (!h) return CEOS_ERR_ARG;
     hipStream_t s = stream ? (hipStream_t)stream : h->stream;
-    return linear_solve(h, dK, db, dx, rtol, maxit, s, iters, relres, flag);
+    int rc = linear_solve(h, dK, db, dx, rtol, maxit, s, iters, relres, flag);
+    if (rc) return rc;
+    return note_caller_stream(h, s);
 }
 
 int ceos_linear_solve(ceos_handle* h, const double* b, double* x, double rtol, int maxit, int* iters,
@@ -1668,7 +1768,7 @@ int ceos_newton_update_device(ceos_handle* h, double* dU, const double* dDX, dou
                        scale, dDX, dU, (const double*)h->dX, h->nl ? h->dx : nullptr);
     HIP_OK(hipGetLastError());
     h->f_valid = h->k_valid = false;
-    return CEOS_OK;
+    return note_caller_stream(h, s);
 }
 
 }  // extern "C"

@@ -139,10 +139,29 @@ int ceos_set_fused(ceos_handle *h, int fused);
 /* Device-resident evaluation for callers that keep U, Fint and K in HBM (no host copies):
  * dU (3*nn doubles), dFint (3*nn doubles) and dK (nnz doubles, or floats in CEOS_K_FP32_STORED
  * mode; NULL = residual only) are device pointers; stream is a hipStream_t (NULL = the
- * handle's stream).  The call is asynchronous: no host synchronisation, and the detJ check is
- * left in a device counter read by ceos_device_status. */
+ * handle's stream).  The detJ check is left in a device counter read by ceos_device_status.
+ *
+ * Once the handle is prepared (ceos_mesh_prepare with records = 1, or one earlier call of this
+ * function) the call is asynchronous: kernel launches on `stream` only, no host
+ * synchronisation, no allocation, no memset, so a caller may capture it in a HIP graph.  An
+ * unprepared handle is prepared by the call itself, which synchronises and allocates; on a
+ * stream under capture the call returns CEOS_ERR_ARG instead.  ceos_set_schedule,
+ * ceos_set_pipeline, ceos_set_assembly, a change of material model and a change of element
+ * technology make the handle unprepared and invalidate graphs captured before them: prepare and
+ * capture again.
+ *
+ * Ordering: the entries that run on the handle's own stream (ceos_device_status,
+ * ceos_switch_converged_state, ceos_get_state, ceos_set_state, ceos_get_gp_stress,
+ * ceos_failed_elements, ceos_homogenize, the host-array entries) wait, through an event, for the
+ * last eager call of a device entry on a caller's stream.  Work replayed from a captured graph is
+ * not seen by the library: synchronise the stream the graph was launched on first. */
 int ceos_evaluate_device(ceos_handle *h, double time, const double *dU, double *dFint, void *dK,
                          void *stream);
+/* Builds the per-element tables of the evaluation (launch order, slot maps) and, with
+ * records = 1, the element-record buffer of the gather assembly (2,592 B per element), so that
+ * the next ceos_evaluate_device neither allocates nor synchronises.  No reference counterpart:
+ * the reference builds its element data once at input (FEM/Input/ModReadInputFile.f90). */
+int ceos_mesh_prepare(ceos_handle *h, int records);
 int ceos_update_mesh_device(ceos_handle *h, const double *dU, void *stream);
 int ceos_device_status(ceos_handle *h, ceos_status *st);
 

@@ -83,3 +83,33 @@ def test_unsupported_inputs_are_refused(tmp_path):
                                     encoding="latin-1")
     with pytest.raises(caseio.CaseError):
         caseio.read_settings(tmp_path / "b.dat")
+
+
+def test_restart_start_follows_the_reference_rules():
+    """Resume point of a restarted analysis (FEM/ModFEMAnalysis.f90:1476-1514)."""
+    from ceos_amd import analysis
+    rs = analysis.restart_start
+    assert rs([50], 1, 25) == (1, 26)
+    assert rs([50], 1, 50) is None  # last step of the last load case done
+    assert rs([10, 20], 1, 4) == (1, 5)
+    assert rs([10, 20], 1, 10) == (2, 1)  # load case finished: the next one from step 1
+    assert rs([10, 20], 2, 7) == (2, 8)
+    assert rs([50], 1, 1) == (1, 2)  # the time-zero record reads as load case 1, step 1
+    for bad in (([10], 2, 1), ([10, 20], 1, 11), ([10], 1, 11)):
+        with pytest.raises(caseio.CaseError):
+            rs(*bad)
+
+
+def test_restart_option_is_read(tmp_path):
+    src = os.path.join(CHECKER, "NonLinear_FEA_3D_Hexa8_StVenant", "Settings.dat")
+    text = open(src, encoding="latin-1").read()
+    for value, expect in (("True", True), ("False", False)):
+        p = tmp_path / ("Settings_%s.dat" % value)
+        p.write_text(text.replace("Maximum Cut Backs = 10", "Maximum Cut Backs = 10\n\tRestart = %s" % value),
+                     encoding="latin-1")
+        assert caseio.read_settings(str(p))["restart"] is expect
+    assert caseio.read_settings(src)["restart"] is False
+    p = tmp_path / "Settings_bad.dat"
+    p.write_text(text.replace("Maximum Cut Backs = 10", "Maximum Cut Backs = 10\n\tRestart = Maybe"), encoding="latin-1")
+    with pytest.raises(caseio.CaseError):
+        caseio.read_settings(str(p))

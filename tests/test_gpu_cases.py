@@ -45,3 +45,95 @@ def test_checker_case_from_its_own_files(name, fixture, tol, tmp_path):
     ndof, recs = caseio.read_results(tmp_path / "FEMAnalysis.result")
     assert ndof == case.ndof and len(recs) == 51
     assert np.array_equal(recs[-1]["U"], hist[-1][3])
+
+
+def _records_from_attempts(attempts):
+    """Header fields (load case, step, cut back, substep, end-of-step flag) the reference's
+    quasi-static loop writes for a sequence of Newton solves (FEM/ModFEMAnalysis.f90:1284-1336):
+    a cut back sets SubStep = 1, every converged solve increments it before its record."""
+    out, key, cut, sub = [], None, 0, 0
+    for lc, st, alpha, ok in attempts:
+        if (lc, st) != key:
+            key, cut, sub = (lc, st), 0, 0
+        if not ok:
+            cut, sub = cut + 1, 1
+        else:
+            sub += 1
+            out.append((lc, st, cut, sub, 1 if alpha == 1.0 else 0))
+    return out
+
+
+def test_result_records_follow_the_reference_substep_counter(tmp_path):
+    """StVenant in 5 steps with at most 2 Newton iterations: cut backs occur, and every record's
+    CUT BACK / SUBSTEP / FLAG END STEP follow the reference's counters.  A step without cut back
+    writes one record with SUBSTEP 1; after a cut back the next record carries SUBSTEP 2."""
+    case = caseio.Case(os.path.join(CHECKER, "NonLinear_FEA_3D_Hexa8_StVenant"))
+    case.load_cases = [dict(c, steps=5) for c in case.load_cases]
+    case.settings["newton_maxiter"] = 2
+    attempts = []
+    hist, _ = analysis.run(case, out_dir=str(tmp_path), attempts=attempts)
+    assert len(hist) == 5
+    assert any(not ok for *_, ok in attempts), "no cut back occurred"
+    _, recs = caseio.read_results(tmp_path / "FEMAnalysis.result")
+    got = [(r["load_case"], r["step"], r["cut_back"], r["substep"], r["flag_end_step"]) for r in recs[1:]]
+    assert got == _records_from_attempts(attempts)
+    assert (recs[0]["cut_back"], recs[0]["substep"], recs[0]["flag_end_step"]) == (0, 0, 1)
+    for lc, st, alpha, ok in attempts:
+        mine = [g for g in got if g[:2] == (lc, st)]
+        first = [a for a in attempts if a[:2] == (lc, st)][0]
+        if first[3] and first[2] == 1.0:
+            assert mine == [(lc, st, 0, 1, 1)]
+        elif not first[3]:
+            assert mine[0][2] >= 1 and mine[0][3] == 2  # cut back(s) first: SUBSTEP 1, then + 1
+
+
+def test_restart_from_result_file_continues_bitwise(tmp_path):
+    """The StVenant case interrupted after step 25 and restarted from its FEMAnalysis.result
+    (end-of-step records replayed through the constitutive update and the state commit, then
+    the loop resumes at step 26) writes the same records, probes and displacements, bitwise,
+    as the uninterrupted run."""
+    d = os.path.join(CHECKER, "NonLinear_FEA_3D_Hexa8_StVenant")
+    full, part = tmp_path / "full", tmp_path / "part"
+    hist_full, _ = analysis.run(caseio.Case(d), out_dir=str(full))
+    hist_a, _ = analysis.run(caseio.Case(d), out_dir=str(part), stop_after=(1, 25))
+    assert len(hist_a) == 25
+    hist_b, _ = analysis.run(caseio.Case(d), out_dir=str(part), restart=True)
+    assert [h[1] for h in hist_b] == list(range(26, 51))
+    for a, b in zip(hist_full, hist_a + hist_b):
+        assert a[:3] == b[:3] and a[4] == b[4] and np.array_equal(a[3], b[3])
+    _, rf = caseio.read_results(full / "FEMAnalysis.result")
+    _, rp = caseio.read_results(part / "FEMAnalysis.result")
+    assert len(rf) == len(rp) == 51
+    for a, b in zip(rf, rp):
+        assert {k: v for k, v in a.items() if k != "U"} == {k: v for k, v in b.items() if k != "U"}
+        assert np.array_equal(a["U"], b["U"])
+    for c in "XYZ":
+        name = "CEOS_Disp_%s.probe" % c
+        assert (full / name).read_text() == (part / name).read_text()
+    # a finished analysis restarts into nothing: "Last load step already calculated!"
+    hist_c, _ = analysis.run(caseio.Case(d), out_dir=str(part), restart=True)
+    assert hist_c == []
+    _, rp2 = caseio.read_results(part / "FEMAnalysis.result")
+    assert len(rp2) == 51
+
+
+def test_restart_replays_plastic_state(tmp_path):
+    """J2 (linear analysis) on the StVenant mesh, loaded past yield: the committed plastic state
+    rebuilt by the replay equals the state the uninterrupted run committed, so the restarted run
+    continues bitwise."""
+    d = os.path.join(CHECKER, "NonLinear_FEA_3D_Hexa8_StVenant")
+
+    def j2_case():
+        c = caseio.Case(d)
+        c.settings["nl_analysis"] = False
+        c.settings["materials"] = {1: (2, [100.0e9, 0.3, 250.0e6, 21.0e9])}
+        c.load_cases = [dict(c.load_cases[0], steps=12)]
+        return c
+
+    full, part = tmp_path / "full", tmp_path / "part"
+    hist_full, _ = analysis.run(j2_case(), out_dir=str(full))
+    analysis.run(j2_case(), out_dir=str(part), stop_after=(1, 7))
+    hist_b, _ = analysis.run(j2_case(), out_dir=str(part), restart=True)
+    assert [h[1] for h in hist_b] == list(range(8, 13))
+    for a, b in zip(hist_full[7:], hist_b):
+        assert a[4] == b[4] and np.array_equal(a[3], b[3])
