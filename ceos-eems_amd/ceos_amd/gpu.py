@@ -70,7 +70,7 @@ EXPORTS = [
     "ceos_set_pipeline", "ceos_rve_autotune", "ceos_rve_set_cg_group",
     "ceos_bc_set", "ceos_residual_bc", "ceos_residual_bc_device", "ceos_set_solver_precision",
     "ceos_linear_solve", "ceos_linear_solve_device", "ceos_newton_update_device", "ceos_set_linear_solver",
-    "ceos_mesh_prepare",
+    "ceos_mesh_prepare", "ceos_evaluate_device_static",
 ]
 
 
@@ -114,6 +114,7 @@ def load_library(path=LIB_PATH):
     L.ceos_state_load.argtypes = [vp, C.c_char_p]
     L.ceos_evaluate_device.argtypes = [vp, d, vp, vp, vp, vp]
     L.ceos_mesh_prepare.argtypes = [vp, i]
+    L.ceos_evaluate_device_static.argtypes = [vp, d, vp, vp, vp, vp]
     L.ceos_update_mesh_device.argtypes = [vp, vp, vp]
     L.ceos_device_status.argtypes = [vp, C.POINTER(CeosStatus)]
     L.ceos_mesh_info.argtypes = [vp, C.POINTER(i), C.POINTER(C.c_longlong)]
@@ -338,6 +339,12 @@ class CeosMesh:
         """ceos_mesh_prepare: build the evaluation tables (and the element-record buffer), so the
         next evaluate_device allocates nothing and may be captured in a HIP graph."""
         _check(load_library().ceos_mesh_prepare(self._h, 1 if records else 0), "ceos_mesh_prepare")
+
+    def evaluate_device_static(self, time, dU, dFint, dK, stream=None):
+        """ceos_evaluate_device_static: evaluate_device with the element records in the library's
+        static device buffer (no allocation in the call); the handle must be prepared."""
+        _check(load_library().ceos_evaluate_device_static(self._h, time, dU, dFint, dK, stream),
+               "ceos_evaluate_device_static")
 
     def set_external(self, dext):
         """Device pointer (int) of the [ne][8][27] external material points (MODEL_EXT)."""

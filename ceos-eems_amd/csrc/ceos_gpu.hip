@@ -16,6 +16,7 @@
 #include <algorithm>
 #include <map>
 #include <string>
+#include <mutex>
 #include <vector>
 
 #include "../../include/ceos_gpu.h"
@@ -36,7 +37,39 @@ using namespace ceos;
         }                                                                                      \
     } while (0)
 
+// Element records of ceos_evaluate_device_static: a fixed buffer in the code object, one copy per
+// device, held until the process ends.  Host code reaches it only through the per-device address
+// that static_scratch() obtains with hipGetSymbolAddress (the symbol itself is a host-side
+// stand-in, not a device address).  Nothing in it has to start at zero: an evaluation writes
+// every record it later reads.
+constexpr size_t kStaticScratchBytes = size_t(128) << 20;
+constexpr size_t kStaticScratchDoubles = kStaticScratchBytes / sizeof(double);
+static __device__ __align__(256) double scratch[kStaticScratchDoubles];
+
 namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+double* g_scratch_ptr[kMaxDevices] = {nullptr};
+bool g_scratch_tried[kMaxDevices] = {false};
+
+// device address of `scratch` on `device` (which must be current), looked up once per device
+double* static_scratch(int device) {
+    if (device < 0 || device >= kMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_tried[device]) {
+        g_scratch_tried[device] = true;
+        void* ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) {
+            fprintf(stderr, "ceos_gpu: hipGetSymbolAddress(scratch) failed on device %d: %s\n", device,
+                    hipGetErrorString(e));
+            ptr = nullptr;
+        }
+        g_scratch_ptr[device] = (double*)ptr;
+    }
+    return g_scratch_ptr[device];
+}
 
 constexpr int kMaxMat = 64;
 constexpr int kMaxProps = 8;
@@ -657,10 +690,18 @@ static void launch_range(const ElemParams& p, const LaunchRange& r, bool nl, boo
     }
 }
 
-// one evaluation: sigma/state for every GP, Fint, and (if K != null) the tangent
-static int evaluate(ceos_handle* h, const double* dU, double* dFint, void* dK, hipStream_t s) {
-    int st = prepare_for(h, s);
-    if (st) return st;
+// one evaluation: sigma/state for every GP, Fint, and (if K != null) the tangent.  rec = the
+// element-record buffer; null = the handle's own (prepared, and allocated if missing)
+static int evaluate(ceos_handle* h, const double* dU, double* dFint, void* dK, hipStream_t s,
+                    double* rec = nullptr) {
+    int st = CEOS_OK;
+    if (!rec) {
+        st = prepare_for(h, s);
+        if (st) return st;
+        rec = h->dscratch;
+    } else if (!h->finalized) {
+        return CEOS_ERR_ARG;
+    }
     if (s == h->stream) {
         st = after_caller_stream(h);
         if (st) return st;
@@ -689,7 +730,7 @@ static int evaluate(ceos_handle* h, const double* dU, double* dFint, void* dK, h
     p.err_count = h->derr;
     p.time = h->time;
     hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(1), 0, s, h->derr);
-    p.scratch = h->dscratch;
+    p.scratch = rec;
     hipEvent_t* ev = (h->timing_on && h->timing_n < h->timing_cap) ? &h->tev[3 * h->timing_n] : nullptr;
     if (ev) HIP_OK(hipEventRecord(ev[0], s));
     const bool wk = dK != nullptr;
@@ -707,7 +748,7 @@ static int evaluate(ceos_handle* h, const double* dU, double* dFint, void* dK, h
     g.blkstart = h->dblkstart;
     g.cptr = h->dcptr;
     g.cword = h->dcword;
-    g.rec = h->dscratch;
+    g.rec = rec;
     g.Fint = dFint;
     g.K = dK;
     if (gather && wk && h->pipe_chunks > 1) {
@@ -1139,6 +1180,7 @@ int ceos_mesh_create(int nn, const double* X, int ne, int npe, const int* conn, 
         delete h;
         return CEOS_ERR_HIP;
     }
+    (void)static_scratch(h->device);  // once per device: the address ceos_evaluate_device_static uses
     if (hipEventCreateWithFlags(&h->ev_caller, hipEventDisableTiming) != hipSuccess) {
         (void)hipStreamDestroy(h->stream);
         delete h;
@@ -1561,6 +1603,30 @@ int ceos_evaluate_device(ceos_handle* h, double time, const double* dU, double* 
     h->time = time;
     hipStream_t s = stream ? (hipStream_t)stream : h->stream;
     int rc = evaluate(h, dU, dFint, dK, s);
+    if (rc) return rc;
+    return note_caller_stream(h, s);
+}
+
+int ceos_evaluate_device_static(ceos_handle* h, double time, const double* dU, double* dFint, void* dK,
+                                void* stream) {
+    if (!h || !dU || !dFint) return CEOS_ERR_ARG;
+    if (!h->finalized) {
+        fprintf(stderr, "ceos_evaluate_device_static: the handle is not prepared (call ceos_mesh_prepare)\n");
+        return CEOS_ERR_ARG;
+    }
+    const size_t need = (size_t)h->ne * (dK ? kRecord : 24);
+    if (h->assembly == CEOS_ASSEMBLY_GATHER && need > kStaticScratchDoubles) {
+        fprintf(stderr,
+                "ceos_evaluate_device_static: %d elements need %.1f MiB of element records (%s), the static "
+                "buffer holds %zu MiB\n",
+                h->ne, need * sizeof(double) / 1048576.0, dK ? "F+K" : "residual only", kStaticScratchBytes >> 20);
+        return CEOS_ERR_ARG;
+    }
+    double* buf = static_scratch(h->device);
+    if (!buf) return CEOS_ERR_HIP;
+    h->time = time;
+    hipStream_t s = stream ? (hipStream_t)stream : h->stream;
+    int rc = evaluate(h, dU, dFint, dK, s, buf);
     if (rc) return rc;
     return note_caller_stream(h, s);
 }

@@ -162,6 +162,20 @@ int ceos_evaluate_device(ceos_handle *h, double time, const double *dU, double *
  * the next ceos_evaluate_device neither allocates nor synchronises.  No reference counterpart:
  * the reference builds its element data once at input (FEM/Input/ModReadInputFile.f90). */
 int ceos_mesh_prepare(ceos_handle *h, int records);
+
+/* ceos_evaluate_device for callers that cannot allocate (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  No counterpart in the reference.  Same
+ * arguments and results as ceos_evaluate_device, but the element records of the gather assembly
+ * go to a static device buffer of CEOS_STATIC_SCRATCH_MIB MiB that the library holds on every
+ * device it runs on (one per device, kept until the process ends), so the call allocates nothing,
+ * frees nothing and synchronises nothing.  The handle must be prepared first
+ * (ceos_mesh_prepare, records = 0 suffices); otherwise CEOS_ERR_ARG.  A mesh whose records do not
+ * fit is refused with CEOS_ERR_ARG: F+K needs 2,592 B per element (at most 51,781 elements),
+ * residual only (dK NULL) 192 B per element (at most 699,050).  Every call on one device shares
+ * that buffer: calls that overlap in time on one device (two streams, two threads) race. */
+#define CEOS_STATIC_SCRATCH_MIB 128
+int ceos_evaluate_device_static(ceos_handle *h, double time, const double *dU, double *dFint, void *dK,
+                                void *stream);
 int ceos_update_mesh_device(ceos_handle *h, const double *dU, void *stream);
 int ceos_device_status(ceos_handle *h, ceos_status *st);
 
