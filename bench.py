@@ -10,8 +10,9 @@ C-ABI entry ceos_evaluate_device.
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-fe2 | --fe2-only]
 
 Prints one JSON line.  cpu_baseline: the CPU oracle (a plain-C restatement of the reference
-loops, OpenMP over element colours) timed on the host cores on a bounded sample of the same
-workload: the 32^3 corner sub-cube of the same perturbed mesh and displacement field.
+loops, OpenMP over element colours) timed on the host cores this process may use, on the same
+512,000-element mesh and displacement field (a bounded number of evaluations); its Fint and K are
+compared with the device's at full size (parity_full_size).
 
 The line also carries the second metric, "fe2": FE^2 macro-step seconds, one macro Newton
 evaluation (Fint + K of the macro mesh) at all 4,096 macro Gauss points of an 8^3 hex8 macro cube
@@ -59,12 +60,71 @@ def displacement(X):
     return U.reshape(-1)
 
 
+def progress(msg):
+    """One progress line on stderr (the JSON result is the only stdout line)."""
+    print("bench: %s" % msg, file=sys.stderr, flush=True)
+
+
+def host_cpu():
+    """The host cores the CPU baselines use: the process's CPU affinity, capped by
+    OMP_NUM_THREADS when that is set (the box's CPU share for its one GPU), with the machine's
+    core count and CPU model beside it."""
+    aff = len(os.sched_getaffinity(0))
+    cap = os.environ.get("OMP_NUM_THREADS")
+    threads = min(aff, int(cap)) if cap and cap.isdigit() and int(cap) > 0 else aff
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"threads": threads, "affinity_cores": aff, "cpu_count": os.cpu_count(),
+            "omp_num_threads": cap, "model": model}
+
+
 def read_traffic():
     """HBM bytes per launch of the element kernel from the committed rocprofv3 --pmc summary."""
     if not os.path.exists(PROFILE_PMC):
         return None
     m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", open(PROFILE_PMC).read())
     return float(m.group(1)) if m else None
+
+
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def cpu_baseline_full(X, conn, U, F_gpu, K_gpu, budget_s=12.0):
+    """The CPU oracle on the whole 80^3 mesh at the same state, OpenMP on the host cores this
+    process may use, timed over a bounded number of F+K evaluations; its Fint and K against the
+    device's (norm-wise relative difference, the whole arrays)."""
+    sys.path.insert(0, os.path.join(REPO, "tests"))
+    import oracle_lib as ol
+    cpu = host_cpu()
+    o = ol.OracleMesh(X, conn, threads=cpu["threads"])
+    o.set_material(1, ol.MODEL_CNH, [MU, LAM])
+    o.update_mesh(U)
+    o.csr()
+    n, t0 = 0, time.perf_counter()
+    while True:
+        o.solve_constitutive(0.2, U)
+        F = o.internal_force()[0]
+        K = o.tangent()
+        n += 1
+        if time.perf_counter() - t0 > budget_s:
+            break
+    dt = time.perf_counter() - t0
+    parity = {"fint_rel": rel(F_gpu, F), "k_rel": rel(K_gpu, K),
+              "fint_max_abs": float(np.max(np.abs(F_gpu - F))), "k_max_abs": float(np.max(np.abs(K_gpu - K))),
+              "entries": {"fint": int(F.size), "k": int(K.size)}, "bar": 1e-12}
+    o.close()
+    base = {"value": n * len(conn) / dt, "unit": "elem/s", "cores": cpu["threads"], "kind": "port", "host": cpu,
+            "sample": "%d F+K evaluations (FP64) of the full %d-element mesh at the same state, CPU oracle with "
+                      "%d OpenMP threads" % (n, len(conn), cpu["threads"])}
+    return base, parity
 
 
 def cpu_baseline(X, conn, U, n_sub=32, budget_s=12.0, threads=None):
@@ -80,7 +140,7 @@ def cpu_baseline(X, conn, U, n_sub=32, budget_s=12.0, threads=None):
     remap[nodes] = np.arange(1, len(nodes) + 1)
     Xs = X[nodes - 1]
     Us = U.reshape(-1, 3)[nodes - 1].reshape(-1)
-    threads = threads or min(os.cpu_count() or 1, 16)
+    threads = threads or host_cpu()["threads"]
     o = ol.OracleMesh(Xs, remap[sub_conn], threads=threads)
     o.set_material(1, ol.MODEL_CNH, [MU, LAM])
     o.update_mesh(Us)
@@ -107,9 +167,10 @@ FLOP_PER_ELEM_F = 8 * (328 + 472 + 144) + 250.0
 BYTES_PER_ELEM_FP32K = 605.0
 
 
-def variants_bench(X, conn, U, steps, warmup):
+def variants_bench(X, conn, U, steps, warmup, F64=None, K64=None):
     """The same C2 mesh and state in the other modes: Neo-Hookean (C10 = 0.5, K = 100) instead of
-    the compressible one; the FP32-stored tangent; the residual-only evaluation (Fint, no K)."""
+    the compressible one; the FP32-stored tangent (its K and Fint against the FP64 evaluation's,
+    F64 / K64); the residual-only evaluation (Fint, no K)."""
     from ceos_amd import gpu, hiprt
     ne, nn = len(conn), len(X)
     out = {}
@@ -151,6 +212,13 @@ def variants_bench(X, conn, U, steps, warmup):
                      "element_ms": round(launch_s * 1e3, 4), "assembly_ms": round(asm_ms / max(n_timed, 1), 4),
                      "fp64_frac": round(tf / PEAK_FP64_TFLOPS, 4), "flop_per_elem": flop,
                      "path_hbm_frac": round(gbs / PEAK_HBM_GBS, 4), "algorithmic_bytes_per_elem": round(nbytes, 1)}
+        if K64 is not None and prec == gpu.K_FP32_STORED:
+            k32 = dK.numpy().astype(np.float64)
+            out[name]["k_rel_vs_fp64"] = rel(k32, K64)
+            out[name]["k_max_rel_entry_vs_fp64"] = float(np.max(np.abs(k32 - K64) / np.maximum(np.abs(K64), 1e-300)))
+            out[name]["fint_rel_vs_fp64"] = rel(dF.numpy(), F64)
+        if F64 is not None and not want_k and model == gpu.MODEL_CNH:
+            out[name]["fint_rel_vs_fp64_fk"] = rel(dF.numpy(), F64)
         for a in (dU, dF, dK):
             if a is not None:
                 a.free()
@@ -178,20 +246,51 @@ def newton_correction(X):
 RVE_CG_BYTES_PER_NODE = 243 * 4 + 240  # FP32-stored tangent row + the FP64 CG vector traffic
 
 
-def fe2_cpu_baseline(F, budget_note):
-    """One RVE per size class solved from the undeformed state and condensed by the CPU restatement
-    (sparse direct solves), extrapolated to the workload's 4,096 RVEs over the host cores."""
+def _fe2_cpu_job(side, rve_id, F):
+    """One RVE of the FE^2 workload (its own id and macro F) solved from the undeformed state and
+    condensed by the CPU restatement on one core (runs in a worker process)."""
     sys.path.insert(0, os.path.join(REPO, "tests"))
+    from threadpoolctl import threadpool_limits
     import rve_oracle as ro
     from ceos_amd import rve
-    per = {}
-    for n in rve.SIDES:
-        o = ro.OracleRVE(n, rve.phases(n, 0))
+    with threadpool_limits(1):
         t0 = time.perf_counter()
-        o.solve(F)
+        o = ro.OracleRVE(int(side), rve.phases(int(side), int(rve_id)))
+        o.solve(np.asarray(F))
         o.tangent()
-        per[n] = time.perf_counter() - t0
-    return per
+        return time.perf_counter() - t0
+
+
+def _noop():
+    return os.getpid()
+
+
+def fe2_cpu_baseline(sides, Fgp):
+    """The CPU restatement on the host cores, all busy: per size class, as many of the workload's
+    own RVEs as there are cores (each with its own macro F) solved and condensed at once, one per
+    worker process; the measured wall time per class gives the all-cores throughput, and the
+    total for the workload's 4,096 RVEs is the sum over classes of count x wall / batch."""
+    import multiprocessing as mp
+    cpu = host_cpu()
+    nw = cpu["threads"]
+    ctx = mp.get_context("spawn")  # fresh interpreters: the workers never touch the GPU
+    per = {}
+    with ctx.Pool(nw) as pool:
+        pool.starmap(_noop, [()] * nw)  # start the workers outside the timed region
+        for n in sorted(set(int(x) for x in sides)):
+            ids = np.nonzero(sides == n)[0][:nw]
+            t0 = time.perf_counter()
+            solo = pool.starmap(_fe2_cpu_job, [(n, int(i), Fgp[i]) for i in ids])
+            wall = time.perf_counter() - t0
+            per[n] = {"rves": int(len(ids)), "wall_s": wall, "mean_rve_s": float(np.mean(solo)),
+                      "count_in_workload": int(np.sum(sides == n))}
+    total = sum(v["count_in_workload"] * v["wall_s"] / v["rves"] for v in per.values())
+    return {"value": total, "unit": "s", "cores": nw, "kind": "port", "host": cpu,
+            "per_side": {str(k): v for k, v in per.items()},
+            "sample": "per side (8/12/16): %d of the workload's own RVEs (their ids and macro F) solved from the "
+                      "undeformed state and condensed by the CPU restatement (sparse direct solves), one per "
+                      "worker process, all %d at once; total = sum over sides of count x measured wall / %d"
+                      % (nw, nw, nw)}
 
 
 def fe2_bench(no_cpu):
@@ -206,6 +305,7 @@ def fe2_bench(no_cpu):
     tile, tile_s = m.rves.autotune(max_rves=256)
     t_tune = time.perf_counter() - t0
     t_desc = m.setup_descriptors()
+    progress("FE2 setup done (descriptors %.1f s)" % t_desc)
     U1 = macro_displacement(m.X)
     U2 = U1 + newton_correction(m.X)
     # surrogate path first, on undeformed RVEs (its judges and fallbacks start there)
@@ -213,12 +313,14 @@ def fe2_bench(no_cpu):
     s1 = m.evaluate_surrogate(U1, use_graph=True)
     s2 = m.evaluate_surrogate(U2, use_graph=True)
     ext_s = m.ext()
+    progress("FE2 surrogate path: %.3f s, %.3f s" % (s1["seconds"], s2["seconds"]))
     f_s, k_s = m.fint(), m.tangent()
     Fgp = m.gp_F()
     m.rves.reset()
     p1 = m.evaluate_rve(U1, use_graph=True)
     p2 = m.evaluate_rve(U2, use_graph=True)
     ext_p = m.ext()
+    progress("FE2 plain path: %.2f s, %.2f s" % (p1["seconds"], p2["seconds"]))
     f_p, k_p = m.fint(), m.tangent()
     info = m.dinfo.numpy().reshape(-1, 4)
     nodes = m.sides.astype(np.float64) ** 3
@@ -258,22 +360,17 @@ def fe2_bench(no_cpu):
                        "note": "algorithmic CG bytes (FP32 tangent rows + FP64 vectors) over the whole plain "
                                "evaluation's wall time, a lower bound for the CG kernels alone"},
         "setup_s": {"rve_batch_and_surrogate": t_build, "cg_tile_autotune": t_tune, "descriptors": t_desc},
+        "descriptor_amortisation": "the descriptors (one condensed RVE tangent at F = I per microstructure) are "
+                                   "computed once per set of microstructures and reused by every surrogate "
+                                   "evaluation of every load step; the pass costs %.2f plain Newton evaluations"
+                                   % (t_desc / p2["seconds"]),
         "surrogate_test_err": {"mean": float(sur[2]["test_err_mean"]), "p99": float(sur[2]["test_err_p99"]),
                                "max": float(sur[2]["test_err_max"]),
                                "base_only_mean": float(sur[2]["test_err_base_mean"])},
     }
     if not no_cpu:
-        F = np.array([[1.06, 0.02, 0.0], [0.0, 0.97, 0.01], [0.0, 0.0, 0.98]])
-        per = fe2_cpu_baseline(F, "")
-        counts = {n: int(np.sum(m.sides == n)) for n in per}
-        cores = min(os.cpu_count() or 1, 16)
-        total = sum(counts[n] * per[n] for n in per) / cores
-        out["cpu_baseline"] = {"value": total, "unit": "s", "cores": cores, "kind": "port",
-                               "sample": "one RVE per side (8/12/16) solved from the undeformed state to the "
-                                         "macro stretch-shear F and condensed by the CPU restatement (sparse "
-                                         "direct solves, one core each: %s s); the total for the workload's "
-                                         "4,096 RVEs is extrapolated as %d busy cores"
-                                         % (", ".join("%.2f" % per[n] for n in sorted(per)), cores)}
+        out["cpu_baseline"] = fe2_cpu_baseline(np.asarray(m.sides), Fgp)
+        out["speedup_plain_first_vs_cpu_baseline"] = out["cpu_baseline"]["value"] / p1["seconds"]
     return out
 
 
@@ -331,6 +428,83 @@ def newton_step_device(mesh, X, ux, tol, rtol, maxit=15, stream=None):
     return recs, total, dU.numpy()
 
 
+def c2_history(n_cells=N_CELLS, n_steps=5, ux_total=0.2, tol=1e-8, pcg_rtol=1e-6, maxit=15):
+    """C2's load history on the device: the 80^3 perturbed cube, compressible Neo-Hookean, x=0 fixed,
+    u_x on x=1 ramped to 0.2 (stretch 1.2) in 5 steps, each solved by full Newton (evaluation,
+    Dirichlet BC + max-abs norm, PCG, update) to tol times the step's first norm.  Run twice: FP64
+    tangent and FP64 PCG values (the reference's arithmetic), and the FP32-stored tangent with FP32
+    PCG values, FP64 residual (inexact Newton).  Per step: Newton and PCG iterations, norms, time;
+    the converged displacements of the two runs are compared step by step."""
+    from ceos_amd import gpu, hiprt, meshgen
+    X, conn = meshgen.cube(n_cells, perturb=0.1)
+    fixed, presc = meshgen.stretch_bc(X)
+    ndof = 3 * len(X)
+    runs, Us = {}, {}
+    for name, prec in (("fp64", gpu.K_FP64), ("fp32_stored_k", gpu.K_FP32_STORED)):
+        m = gpu.CeosMesh(X, conn, precision=prec)
+        m.set_material(1, gpu.MODEL_CNH, [MU, LAM])
+        m.set_solver_precision(prec)
+        m.set_bc(presc, fixed)
+        _, nnz = m.info()
+        D = hiprt.DeviceArray
+        dU, dF, dR, dx, dn = D(ndof), D(ndof), D(ndof), D(ndof), D(3)
+        dK = D(nnz, np.float64 if prec == gpu.K_FP64 else np.float32)
+        dUbar = D(ndof)
+        s = hiprt.Stream()
+        sp = s.ptr
+        U = np.zeros(ndof)
+        dU.upload(U)
+        m.update_mesh_device(dU.ptr, sp)
+        steps, Ulist = [], []
+        t_all = time.perf_counter()
+        for st in range(1, n_steps + 1):
+            Ubar = U.copy()
+            Ubar[presc] = ux_total * st / n_steps
+            dUbar.upload(Ubar)
+            t0 = time.perf_counter()
+            norms, pcg = [], []
+            it = 0
+            while True:
+                m.evaluate_device(st / n_steps, dU.ptr, dF.ptr, dK.ptr, sp)
+                m.residual_bc_device(it, dU.ptr, dF.ptr, dK.ptr, None, dUbar.ptr, dR.ptr, dn.ptr, 2, sp)
+                s.synchronize()
+                code, _ = m.device_status()
+                if code != 0:
+                    raise RuntimeError("C2 history %s step %d: evaluation status %d" % (name, st, code))
+                norms.append(float(dn.numpy()[2]))
+                if it > 0 and norms[-1] < tol * norms[0]:
+                    break
+                if it >= maxit:
+                    raise RuntimeError("C2 history %s step %d: Newton did not converge: %s" % (name, st, norms))
+                k, rr, flag = m.linear_solve_device(dK.ptr, dR.ptr, dx.ptr, pcg_rtol, 50000, sp)
+                if flag not in (1, 4):
+                    raise RuntimeError("C2 history %s step %d: PCG flag %d" % (name, st, flag))
+                m.newton_update_device(dU.ptr, dx.ptr, -1.0, sp)
+                pcg.append(k)
+                it += 1
+            U = dU.numpy()
+            Ulist.append(U.copy())
+            m.switch_converged_state()
+            steps.append({"step": st, "u_x": ux_total * st / n_steps, "newton_iterations": it, "pcg_iterations": pcg,
+                          "norms": norms, "seconds": time.perf_counter() - t0})
+            progress("C2 history %s step %d: %d Newton iterations, PCG %s" % (name, st, it, pcg))
+        runs[name] = {"steps": steps, "seconds": time.perf_counter() - t_all}
+        Us[name] = Ulist
+        for a in (dU, dF, dR, dx, dn, dK, dUbar):
+            a.free()
+        m.close()
+    cmp = []
+    for st in range(n_steps):
+        a, b = Us["fp64"][st], Us["fp32_stored_k"][st]
+        cmp.append({"step": st + 1, "du_rel": rel(b, a),
+                    "same_newton_iterations": runs["fp64"]["steps"][st]["newton_iterations"] ==
+                    runs["fp32_stored_k"]["steps"][st]["newton_iterations"]})
+    return {"config": {"workload": "80^3 perturbed hex8 cube, compressible Neo-Hookean mu=1 K=100, x=0 fixed, u_x on "
+                                   "x=1 ramped to 0.2 in 5 steps, full Newton to 1e-8 relative max-abs residual",
+                       "dofs": ndof, "pcg_rtol": pcg_rtol},
+            "fp64": runs["fp64"], "fp32_stored_k": runs["fp32_stored_k"], "fp32_vs_fp64": cmp}
+
+
 J2_PROPS = [210000.0, 0.3, 250.0, 1000.0]  # E, nu, yield stress, hardening modulus (MPa)
 J2_TABLE = [(0.0, 0.0), (1.0, 0.02), (3.0, -0.02), (4.0, 0.02)]  # u_x on x = 1 against time
 # J2 under linear analysis, per Gauss point: B and detJ 328, F 472, residual 144, B^T D B 2,808,
@@ -338,44 +512,56 @@ J2_TABLE = [(0.0, 0.0), (1.0, 0.02), (3.0, -0.02), (4.0, 0.02)]  # u_x on x = 1 
 J2_FLOP_PER_ELEM = 8 * (328 + 472 + 144 + 2808 + 300)
 
 
-def j2_cpu_baseline(X, conn, U, t, state_old, F_gpu, K_gpu, flags_gpu, budget_s=12.0):
-    """The CPU oracle (OpenMP) on the same J2 evaluation: the full mesh, the same U and committed
-    state; timed over a bounded number of F+K evaluations.  Its F, K and plastic flags are compared
-    with the device's at full size."""
-    sys.path.insert(0, os.path.join(REPO, "tests"))
-    import oracle_lib as ol
-    threads = min(os.cpu_count() or 1, 16)
-    o = ol.OracleMesh(X, conn, nl_analysis=False, threads=threads)
-    o.set_material(1, ol.MODEL_J2, J2_PROPS)
-    o.set_state(0, state_old)
-    o.csr()
-    n, t0 = 0, time.perf_counter()
-    while True:
-        o.solve_constitutive(t, U)
-        F = o.internal_force()[0]
-        K = o.tangent()
-        n += 1
-        if time.perf_counter() - t0 > budget_s:
-            break
-    dt = time.perf_counter() - t0
-    flags = o.state(1)[..., 15]  # the oracle's J2 rows: [15] PlasticState (the device's row 7)
-    rel = lambda a, b: float(np.linalg.norm(a - b) / np.linalg.norm(b))
-    parity = {"fint_rel": rel(F_gpu, F), "k_rel": rel(K_gpu, K),
-              "flag_mismatches": int(np.count_nonzero(flags != flags_gpu)), "gauss_points": int(flags.size)}
-    return ({"value": n * len(conn) / dt, "unit": "elem/s", "cores": threads, "kind": "port",
-             "sample": "%d F+K evaluations (FP64) of the full %d-element J2 mesh at the same state, CPU oracle "
-                       "with %d OpenMP threads" % (n, len(conn), threads)}, parity)
+class J2Oracle:
+    """The CPU oracle (OpenMP on the host cores) on the J2 mesh: F+K at a given U against a given
+    committed state, compared with the device's (Fint, K, plastic flags at every Gauss point)."""
+
+    def __init__(self, X, conn):
+        sys.path.insert(0, os.path.join(REPO, "tests"))
+        import oracle_lib as ol
+        self.cpu = host_cpu()
+        self.o = ol.OracleMesh(X, conn, nl_analysis=False, threads=self.cpu["threads"])
+        self.o.set_material(1, ol.MODEL_J2, J2_PROPS)
+        self.o.csr()
+        self.ne = len(conn)
+
+    def evaluate(self, t, U, state_old):
+        self.o.set_state(0, state_old)
+        self.o.solve_constitutive(t, U)
+        return self.o.internal_force()[0], self.o.tangent()
+
+    def parity(self, t, U, state_old, F_gpu, K_gpu, flags_gpu, step):
+        F, K = self.evaluate(t, U, state_old)
+        flags = self.o.state(1)[..., 15]  # the oracle's J2 rows: [15] PlasticState (the device's row 7)
+        return {"step": step, "fint_rel": rel(F_gpu, F), "k_rel": rel(K_gpu, K),
+                "flag_mismatches": int(np.count_nonzero(flags != flags_gpu)), "gauss_points": int(flags.size),
+                "plastic_fraction": float(np.mean(flags_gpu))}
+
+    def baseline(self, t, U, state_old, budget_s=12.0):
+        n, t0 = 0, time.perf_counter()
+        while True:
+            self.evaluate(t, U, state_old)
+            n += 1
+            if time.perf_counter() - t0 > budget_s:
+                break
+        dt = time.perf_counter() - t0
+        th = self.cpu["threads"]
+        return {"value": n * self.ne / dt, "unit": "elem/s", "cores": th, "kind": "port", "host": self.cpu,
+                "sample": "%d F+K evaluations (FP64) of the full %d-element J2 mesh at the same state, CPU oracle "
+                          "with %d OpenMP threads" % (n, self.ne, th)}
 
 
-def j2_bench(steps=100, warmup=5, n_cells=64, n_hist=10, tol=1e-8, pcg_rtol=1e-6, newton_maxit=15,
-             max_cut_backs=10, solver_precision=None, no_cpu=False):
+def j2_bench(steps=100, warmup=5, n_cells=64, n_hist=40, n_time=10, parity_steps=(20, 40), tol=1e-8,
+             pcg_rtol=1e-6, newton_maxit=15, max_cut_backs=10, solver_precision=None, no_cpu=False):
     """The J2 cyclic workload: 64^3 hex8 cube, J2 with linear isotropic hardening, linear analysis,
-    x=0 fixed, u_x on x=1 cycling 0 -> +2 % -> -2 % -> +2 % over 40 steps.  Steps 1..n_hist run on
-    the device (evaluation, BC + norm, PCG, state commit per step) with the reference driver's
-    golden-ratio cut-back when a Newton solve does not converge (FEM/ModFEMAnalysis.f90:1136-1374:
-    full Newton on J2 stalls at this size when the plastic zone spreads over a whole step); the
-    F+K evaluation is then timed inside step n_hist, at its converged U against the committed
-    step n_hist - 1 state (mixed elastic / plastic Gauss points)."""
+    x=0 fixed, u_x on x=1 cycling 0 -> +2 % -> -2 % -> +2 % over 40 steps, all run on the device
+    (evaluation, BC + norm, PCG, state commit per step) with the reference driver's golden-ratio
+    cut-back when a Newton solve does not converge (FEM/ModFEMAnalysis.f90:1136-1374: full Newton on
+    J2 stalls at this size when the plastic zone spreads over a whole step).  The F+K evaluation is
+    timed inside step n_time (10, the peak of the first loading), at its converged U against the
+    committed step 9 state (mixed elastic / plastic Gauss points); the oracle's F, K and plastic
+    flags are compared with the device's at full size there and after the load reversal (steps 20
+    and 40, each at its converged U against the previous committed state)."""
     from ceos_amd import gpu, hiprt, meshgen
     phi = (1.0 + 5.0 ** 0.5) / 2.0
     X, conn = meshgen.cube(n_cells)
@@ -386,11 +572,13 @@ def j2_bench(steps=100, warmup=5, n_cells=64, n_hist=10, tol=1e-8, pcg_rtol=1e-6
     m.set_bc(presc, fixed)
     _, nnz = m.info()
     ndof = 3 * len(X)
+    ne = len(conn)
     D = hiprt.DeviceArray
     dU, dF, dR, dx, dn, dK = D(ndof), D(ndof), D(ndof), D(ndof), D(3), D(nnz)
     dUbar = D(ndof)
     s = hiprt.Stream()
     sp = s.ptr
+    oracle = None if no_cpu else J2Oracle(X, conn)
 
     def newton(t, Uguess, Ubar):
         """Full Newton from Uguess toward Ubar on the prescribed DOFs; (converged, norms, PCG its)."""
@@ -420,7 +608,7 @@ def j2_bench(steps=100, warmup=5, n_cells=64, n_hist=10, tol=1e-8, pcg_rtol=1e-6
 
     U = np.zeros(ndof)
     t_hist = time.perf_counter()
-    hist = []
+    hist, parity, timed = [], [], None
     dt = 4.0 / 40
     for st in range(1, n_hist + 1):
         ta = (st - 1) * dt
@@ -430,6 +618,7 @@ def j2_bench(steps=100, warmup=5, n_cells=64, n_hist=10, tol=1e-8, pcg_rtol=1e-6
         Uconv = U.copy()
         amin, alpha, cut = 0.0, 1.0, 0
         solves = []
+        t_step = time.perf_counter()
         while True:
             Ubar = U.copy()
             Ubar[presc] = va + alpha * (vb - va)
@@ -448,45 +637,62 @@ def j2_bench(steps=100, warmup=5, n_cells=64, n_hist=10, tol=1e-8, pcg_rtol=1e-6
             alpha = min(alpha + phi * (alpha - aux), 1.0)
             Uconv = dU.numpy()
         U = dU.numpy()
-        if st < n_hist:
-            m.switch_converged_state()
-        hist.append({"step": st, "u_x": vb, "cut_backs": cut, "solves": solves})
+        hist.append({"step": st, "u_x": vb, "cut_backs": cut, "solves": solves,
+                     "seconds": time.perf_counter() - t_step})
+        progress("J2 step %d: %d cut backs, %.2f s" % (st, cut, time.perf_counter() - t_step))
+        t = st * dt
+        if st == n_time:
+            # the F+K evaluation at this step's converged U against the committed previous state:
+            # what the last Newton evaluation of the step computes
+            t_pause = time.perf_counter()
+            for _ in range(warmup):
+                m.evaluate_device(t, dU.ptr, dF.ptr, dK.ptr, sp)
+            s.synchronize()
+            m.timing_start(steps)
+            e0, e1 = hiprt.Event(), hiprt.Event()
+            e0.record(s)
+            for _ in range(steps):
+                m.evaluate_device(t, dU.ptr, dF.ptr, dK.ptr, sp)
+            e1.record(s)
+            e1.synchronize()
+            elem_ms, asm_ms, n_timed, n_launch = m.timing_stop()
+            timed = (e0.elapsed_ms(e1) / steps, elem_ms, asm_ms, n_timed, n_launch, float(m.state(1)[..., 7].mean()))
+            if oracle is not None:
+                state_old = m.state(0)
+                flags = m.state(1)[..., 7]
+                parity.append(oracle.parity(t, U, state_old, dF.numpy(), dK.numpy(), flags, st))
+                baseline = oracle.baseline(t, U, state_old)
+            t_hist += time.perf_counter() - t_pause  # the history time excludes the timing and the oracle
+        elif oracle is not None and st in parity_steps:
+            t_pause = time.perf_counter()
+            m.evaluate_device(t, dU.ptr, dF.ptr, dK.ptr, sp)
+            s.synchronize()
+            parity.append(oracle.parity(t, U, m.state(0), dF.numpy(), dK.numpy(), m.state(1)[..., 7], st))
+            t_hist += time.perf_counter() - t_pause
+        m.switch_converged_state()
     t_hist = time.perf_counter() - t_hist
-    # the F+K evaluation at step n_hist's converged U against the committed step n_hist - 1 state:
-    # what the last Newton evaluation of that step computes (Gauss points still loading return to
-    # the yield surface, the others stay elastic)
-    t = n_hist * dt
-    for _ in range(warmup):
-        m.evaluate_device(t, dU.ptr, dF.ptr, dK.ptr, sp)
-    s.synchronize()
-    m.timing_start(steps)
-    e0, e1 = hiprt.Event(), hiprt.Event()
-    e0.record(s)
-    for _ in range(steps):
-        m.evaluate_device(t, dU.ptr, dF.ptr, dK.ptr, sp)
-    e1.record(s)
-    e1.synchronize()
-    elem_ms, asm_ms, n_timed, n_launch = m.timing_stop()
-    ms = e0.elapsed_ms(e1) / steps
-    ne = len(conn)
-    flags = m.state(1)[..., 7]
+    ms, elem_ms, asm_ms, n_timed, n_launch, plastic = timed
     launch_s = (elem_ms * 1e-3) / n_launch
     tf = J2_FLOP_PER_ELEM * (ne * n_timed / n_launch) / launch_s / 1e12
     out = {"metric": "elem/s assembled (F+K, hex8), J2 cyclic workload", "value": ne / (ms * 1e-3),
            "unit": "elem/s", "ms_per_step": ms, "steps": steps,
            "config": {"workload": "%d^3 hex8 cube, J2 plasticity E=210000 nu=0.3 sigma_y=250 H=1000 (MPa), linear "
                                   "analysis, x=0 fixed, u_x on x=1 cycling 0 -> +2%% -> -2%% -> +2%% over 40 steps; "
-                                  "F+K timed at the state of step %d" % (n_cells, n_hist), "elements": ne, "dofs": ndof},
-           "plastic_fraction": float(flags.mean()),
-           "history": hist, "history_s": t_hist,
-           "roofline": {"bound": "mfma", "kernel": "k_element<2> (FP64 vector + LDS)", "achieved": round(tf, 3),
+                                  "F+K timed at the state of step %d" % (n_cells, n_time), "elements": ne, "dofs": ndof},
+           "plastic_fraction": plastic,
+           "history": hist, "history_s": t_hist, "history_steps": n_hist,
+           "roofline": {"bound": "fp64-valu", "kernel": "k_element<2> (FP64 vector + LDS)", "achieved": round(tf, 3),
                         "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s", "frac": round(tf / PEAK_FP64_TFLOPS, 4),
                         "avg_launch_ms": round(launch_s * 1e3, 4), "flop_per_elem": J2_FLOP_PER_ELEM},
            "assembly_ms": asm_ms / max(n_timed, 1)}
-    if not no_cpu:
-        out["cpu_baseline"], out["parity_full_size"] = j2_cpu_baseline(
-            X, conn, U, t, m.state(0), dF.numpy(), dK.numpy(), flags)
-        out["speedup_vs_cpu_baseline"] = out["value"] / out["cpu_baseline"]["value"]
+    if oracle is not None:
+        out["cpu_baseline"] = baseline
+        out["parity_full_size"] = parity[0]
+        out["parity_after_reversal"] = parity[1:]
+        out["speedup_vs_cpu_baseline"] = out["value"] / baseline["value"]
+    for a in (dU, dF, dR, dx, dn, dK, dUbar):
+        a.free()
+    m.close()
     return out
 
 
@@ -531,7 +737,7 @@ def newton_cpu_baseline(n_cells, ux, tol, threads):
         U = U + dx
         o.update_mesh(U)
     total = time.perf_counter() - t0
-    return {"value": total, "unit": "s", "cores": threads, "kind": "reference",
+    return {"value": total, "unit": "s", "cores": threads, "kind": "reference", "host": host_cpu(),
             "newton_iterations": len(norms) - 1, "linear_solve_s": t_solve,
             "sample": "the same load step on the %d^3 cube (%d DOFs): CPU oracle evaluations (OpenMP), the "
                       "reference's BC code restated in numpy, MKL PARDISO with the reference's settings "
@@ -586,7 +792,7 @@ def newton_bench(no_cpu, n_cells=N_CELLS, n_small=40, ux=0.04, tol=1e-8, pcg_rto
                   "pcg_iterations": [r.get("pcg_iterations") for r in res[n_small][0][:-1]]},
     })
     if not no_cpu:
-        cb = newton_cpu_baseline(n_small, ux, tol, min(os.cpu_count() or 1, 16))
+        cb = newton_cpu_baseline(n_small, ux, tol, host_cpu()["threads"])
         if cb is not None:
             out["cpu_baseline"] = cb
             out["speedup_vs_cpu_baseline_same_size"] = cb["value"] / res[n_small][1]
@@ -603,6 +809,8 @@ def main():
     ap.add_argument("--no-newton", action="store_true", help="skip the Newton load-step measurement")
     ap.add_argument("--newton-only", action="store_true", help="only the Newton load-step measurement")
     ap.add_argument("--no-j2", action="store_true", help="skip the J2 cyclic workload")
+    ap.add_argument("--no-history", action="store_true",
+                    help="skip the C2 5-step load history (FP64 and FP32-stored tangent)")
     ap.add_argument("--no-variants", action="store_true",
                     help="skip the NH (C10, K), FP32-stored K and residual-only timings of the same mesh")
     ap.add_argument("--j2-only", action="store_true", help="only the J2 cyclic workload")
@@ -696,7 +904,7 @@ def main():
     flops_launch = FLOP_PER_ELEM * elems_per_launch
     achieved_tf = flops_launch / launch_s / 1e12
     traffic = read_traffic()
-    roofline = {"bound": "mfma", "achieved": round(achieved_tf, 3), "peak": PEAK_FP64_TFLOPS,
+    roofline = {"bound": "fp64-valu", "achieved": round(achieved_tf, 3), "peak": PEAK_FP64_TFLOPS,
                 "unit": "TFLOP/s", "frac": round(achieved_tf / PEAK_FP64_TFLOPS, 4),
                 "traffic": traffic, "kernel": "k_element (FP64 vector + LDS)",
                 "avg_launch_ms": round(launch_s * 1e3, 4),
@@ -737,19 +945,29 @@ def main():
         "wall_s_timed": wall,
         "setup_s": setup_s,
     }
+    progress("C2 F+K timed: %.4f ms per step" % ms_per_step)
+    F64, K64 = dF.numpy(), dK.numpy().astype(np.float64)
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(X, conn, U)
+        out["cpu_baseline"], out["parity_full_size"] = cpu_baseline_full(X, conn, U, F64, K64)
         out["speedup_vs_cpu_baseline"] = value / out["cpu_baseline"]["value"]
         # the serial figure beside it: the same oracle on one thread, on the 16^3 corner sub-cube
         out["cpu_baseline_serial"] = cpu_baseline(X, conn, U, n_sub=16, budget_s=6.0, threads=1)
     del mesh, dU, dF, dK
+    progress("C2 CPU baseline and full-size parity done")
     if not args.no_variants:
-        out["variants"] = variants_bench(X, conn, U, args.steps, args.warmup)
+        out["variants"] = variants_bench(X, conn, U, args.steps, args.warmup,
+                                         F64, K64 if prec == gpu.K_FP64 else None)
+    del F64, K64
+    if not args.no_history:
+        out["c2_history"] = c2_history()
     if not args.no_j2:
+        progress("J2 cyclic workload")
         out["j2"] = j2_bench(no_cpu=args.no_cpu)
     if not args.no_newton:
+        progress("Newton load step")
         out["newton"] = newton_bench(args.no_cpu)
     if not args.no_fe2:
+        progress("FE2 macro step")
         out["fe2"] = fe2_bench(args.no_cpu)
     print(json.dumps(out), flush=True)
 

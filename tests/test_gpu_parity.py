@@ -400,6 +400,47 @@ def test_full_size_c2_properties():
         assert np.max(np.abs(y)) < 1e-10 * np.max(np.abs(K))
 
 
+def _c2_state(X):
+    """The step-1 state of the C2 stretch test that the benchmark evaluates (u_x = 0.04 X plus a
+    smooth transverse field, F non-trivial at every Gauss point)."""
+    U = np.zeros_like(X)
+    U[:, 0] = 0.04 * X[:, 0]
+    U[:, 1] = -0.006 * X[:, 1] + 0.002 * np.sin(3.1 * X[:, 0]) * X[:, 2]
+    U[:, 2] = -0.006 * X[:, 2] + 0.002 * np.cos(2.3 * X[:, 1]) * X[:, 0]
+    return U.reshape(-1)
+
+
+def test_full_size_c2_matches_oracle():
+    """The whole 80^3 perturbed cube (512,000 hex8, 63,786,006 tangent entries): the device's Fint,
+    K and Gauss-point stresses at the benchmark's state against the oracle's on every entry
+    (norm-wise 1e-12, as on the small meshes); the FP32-stored tangent of the same state against the
+    FP64 one (2e-7: FP32 rounding of each stored entry) with Fint unchanged."""
+    X, conn = meshgen.cube(80, perturb=0.1)
+    U = _c2_state(X)
+    props = [1.0, 100.0 - 2.0 / 3.0]
+    g = gpu.CeosMesh(X, conn)
+    g.set_material(1, ol.MODEL_CNH, props)
+    g.update_mesh(U)
+    assert g.solve_constitutive(0.2, U) == 0
+    fg, kg, sg = g.internal_force()[0], g.tangent(), g.gp_stress()
+    g.close()
+    o = ol.OracleMesh(X, conn, threads=min(len(os.sched_getaffinity(0)), 16))
+    o.set_material(1, ol.MODEL_CNH, props)
+    o.update_mesh(U)
+    assert o.solve_constitutive(0.2, U) == 0
+    fo, ko, so = o.internal_force()[0], o.tangent(), o.gp_stress()
+    o.close()
+    assert rel(fg, fo) < RTOL and rel(kg, ko) < RTOL and rel(sg, so) < RTOL, (rel(fg, fo), rel(kg, ko))
+    g32 = gpu.CeosMesh(X, conn, precision=gpu.K_FP32_STORED)
+    g32.set_material(1, ol.MODEL_CNH, props)
+    g32.update_mesh(U)
+    assert g32.solve_constitutive(0.2, U) == 0
+    f32, k32 = g32.internal_force()[0], g32.tangent()
+    g32.close()
+    assert np.array_equal(f32, fg)
+    assert rel(k32.astype(np.float64), kg) < 2e-7
+
+
 def test_schedule_does_not_change_results():
     """Any launch schedule (slab count) gives the oracle's Fint and K to rounding, and a J2
     state committed under one schedule survives a switch to another."""
