@@ -11,8 +11,9 @@
 // element blocks and writes its 3 pieces of the rows 3A..3A+2; consecutive threads write
 // consecutive pieces of the same rows.
 // Contribution word: bits 0..30 = offset of the 3x3 block (or of the 6-value diagonal block)
-// in the record array, bit 31 = stored transposed (the element computed block (c,b) with
-// global node C > B, whose row-major values are the columns of CSR block (B, C)).
+// in the record array, bit 31 = stored transposed (row-major values that are the columns of the
+// CSR block).  The element kernel writes patch records (element_kernel.hpp) already in the CSR
+// block's orientation, so the maps built for it never set the bit.
 #pragma once
 #include <stdint.h>
 
@@ -29,16 +30,20 @@ struct GatherParams {
     const int* nup;          // [nn]
     const int* blkstart;     // [nn] global block index of the diagonal block of A
     const int* cptr;         // [nblocks+1] contribution ranges per block
-    const uint32_t* cword;   // [36 ne]
-    const double* rec;       // element records
+    const uint32_t* cword;   // [contributions]
+    const int* fptr;         // [nn+1] residual-only pass: Fe records of node A at fword[fptr[A]..]
+    const uint32_t* fword;   // offsets (doubles) of the Fe records
+    const double* rec;       // patch records (F+K) or Fe parts (residual only)
     double* Fint;            // [3 nn]
     void* K;                 // [nnz]
 };
 
 // Contributions are taken kGatherBatch at a time: their words, then all their record pieces are
-// requested before any is summed (a thread keeps up to 4 x 9 loads in flight instead of one
-// word -> piece chain per contribution).  The sums still run in contribution order.
-constexpr int kGatherBatch = 4;
+// requested before any is summed (a thread keeps up to 2 x 9 loads in flight instead of one
+// word -> piece chain per contribution).  The sums still run in contribution order.  With patch
+// records a node block has 1.5 contributions on average (80^3 cube): batches of 2 measured
+// 0.397 ms per gather, 4 0.402 ms, 8 0.477 ms.
+constexpr int kGatherBatch = 2;
 
 template <typename KT>
 __global__ __launch_bounds__(256) void k_gather_blocks(GatherParams p) {
@@ -116,28 +121,24 @@ __global__ __launch_bounds__(256) void k_gather_blocks(GatherParams p) {
     K[r2 + 1 + o] = (KT)a[6]; K[r2 + 2 + o] = (KT)a[7]; K[r2 + 3 + o] = (KT)a[8];
 }
 
-// residual-only gather: Fint of node A from the Fe parts of its diagonal contributions, in
-// contribution order (the order k_gather_blocks sums them in).  In this pass the element kernel
-// writes only Fe, [slot][8][3]; the diagonal contribution word (slot * kRecord +
-// 9 piece_pos(b, 0) + 3) names the slot and the lane b.  One thread per node, its contribution
-// words and then their Fe parts requested kGatherBatch at a time.
+// residual-only gather: Fint of node A from the Fe records of its patches, in patch order (the
+// element kernel writes one 3-double Fe record per diagonal patch record in this pass; fword
+// holds their offsets).  One thread per node, the offsets and then their Fe records requested
+// kGatherBatch at a time.
 __global__ __launch_bounds__(256) void k_gather_fint(GatherParams p) {
     const int A = blockIdx.x * 256 + threadIdx.x;
     if (A >= p.nn) return;
-    const int blk = p.blkstart[A];
-    const int c0 = p.cptr[blk], c1 = p.cptr[blk + 1];
+    const int c0 = p.fptr[A], c1 = p.fptr[A + 1];
     double f[3] = {0.0, 0.0, 0.0};
     for (int c = c0; c < c1; c += kGatherBatch) {
         uint32_t w[kGatherBatch];
 #pragma unroll
-        for (int u = 0; u < kGatherBatch; ++u) w[u] = (c + u < c1) ? (p.cword[c + u] & 0x7fffffffu) : 0u;
+        for (int u = 0; u < kGatherBatch; ++u) w[u] = (c + u < c1) ? p.fword[c + u] : 0u;
         double v[kGatherBatch][3];
 #pragma unroll
         for (int u = 0; u < kGatherBatch; ++u)
             if (c + u < c1) {
-                const uint32_t slot = w[u] / kRecord;
-                const int b = diag_lane((int)(w[u] - slot * kRecord - 3) / 9);
-                const double* d = p.rec + 24L * slot + 3 * b;
+                const double* d = p.rec + w[u];
 #pragma unroll
                 for (int k = 0; k < 3; ++k) v[u][k] = d[k];
             }

@@ -14,6 +14,8 @@
 #include <string.h>
 
 #include <algorithm>
+#include <array>
+#include <cmath>
 #include <map>
 #include <string>
 #include <mutex>
@@ -82,6 +84,7 @@ struct MatDef {
 
 struct LaunchRange {
     int begin, end, model;
+    int patch0 = 0;  // gather assembly: patch of the range's first slot (patches = kEPB-slot chunks)
 };
 
 template <typename T>
@@ -102,6 +105,17 @@ template <typename T>
 hipError_t dalloc_once(T*& p, size_t n) {
     if (p) return hipSuccess;
     return hipMalloc((void**)&p, sizeof(T) * (n ? n : 1));
+}
+
+// buffers whose size depends on the slot order: reallocated only when they must grow
+template <typename T>
+hipError_t dalloc_fit(T*& p, size_t& cap, size_t n) {
+    if (p && cap >= n) return hipSuccess;
+    dfree(p);
+    cap = 0;
+    const hipError_t e = hipMalloc((void**)&p, sizeof(T) * (n ? n : 1));
+    if (e == hipSuccess) cap = n;
+    return e;
 }
 
 void set_status(ceos_status* st, int code, int nfail, const char* msg) {
@@ -181,7 +195,17 @@ struct ceos_handle {
     uint8_t *ddiag_first = nullptr, *delem_err = nullptr;
     int* derr = nullptr;
     double* dhpart = nullptr;  // homogenisation partial sums
-    double* dscratch = nullptr;  // element records (gather assembly), allocated by ensure_records
+    double* dscratch = nullptr;  // patch records (gather assembly), allocated by ensure_records
+    size_t scratch_cap = 0;
+    long long rec_doubles = 0;   // doubles of the patch records of one F+K evaluation
+    uint16_t* dptab = nullptr;   // [ne][36] staging entries of the pieces
+    int4* dpinfo = nullptr;      // [patches]
+    int2* dpdiag = nullptr;      // [patches] residual-only Fe records: offset (doubles), count
+    long long fe_doubles = 0;    // doubles of the Fe records of one residual-only evaluation
+    uint16_t* dpslot = nullptr;  // [patch records]
+    int* dfptr = nullptr;        // [nn+1] residual-only Fe map
+    uint32_t* dfword = nullptr;  // [8 ne]
+    size_t pinfo_cap = 0, pslot_cap = 0, cword_cap = 0, pdiag_cap = 0, fword_cap = 0;
     // ordering of the handle's own stream after device entries that ran on a caller's stream
     hipEvent_t ev_caller = nullptr;
     bool caller_pending = false;
@@ -303,6 +327,7 @@ static void free_device(ceos_handle* h) {
     dfree(h->dconn); dfree(h->dmat); dfree(h->drowstart); dfree(h->dnup); dfree(h->dperm);
     dfree(h->dpairs); dfree(h->ddiag_first); dfree(h->delem_err); dfree(h->derr); dfree(h->dhpart);
     dfree(h->dscratch); dfree(h->dblknode); dfree(h->dblkstart); dfree(h->dcptr); dfree(h->dcword);
+    dfree(h->dptab); dfree(h->dpinfo); dfree(h->dpslot); dfree(h->dfptr); dfree(h->dfword); dfree(h->dpdiag);
     dfree(h->dup_ptr); dfree(h->dup_nodes); dfree(h->dslot_nbr); dfree(h->dslot_src); dfree(h->dsell_nbr);
     dfree(h->ddof_mask); dfree(h->dnode_mask); dfree(h->dR); dfree(h->dpart); dfree(h->dpart2);
     dfree(h->dnorm); dfree(h->dFext); dfree(h->dUbar);
@@ -323,11 +348,55 @@ static int upload_props(ceos_handle* h) {
     return CEOS_OK;
 }
 
-// contribution map of the gather assembly (see assembly_kernel.hpp), element-id order inside
-// every block
+// Spatially compact order of one launch group's elements, whose consecutive runs of kEPB slots
+// are the patches of the gather assembly: elements sorted by the tile of their reference centroid
+// (tiles of 4h x 2h x 2h, h the mean element size from the body diagonals), tiles in z, y, x
+// order, element ids inside a tile.  On a structured mesh every patch is a 4x2x2 brick; on any
+// other mesh the order only decides how many node blocks a patch shares (never correctness).
+static void patch_order(const ceos_handle* h, std::vector<int>& elems) {
+    const size_t n = elems.size();
+    if (n < 2) return;
+    double lo[3] = {1e300, 1e300, 1e300}, hsum = 0.0;
+    std::vector<double> cen(3 * n);
+    for (size_t i = 0; i < n; ++i) {
+        const int* cn = &h->conn0[(size_t)8 * elems[i]];
+        double c[3] = {0, 0, 0};
+        for (int a = 0; a < 8; ++a)
+            for (int k = 0; k < 3; ++k) {
+                const double x = h->X[3L * cn[a] + k];
+                c[k] += 0.125 * x;
+                lo[k] = std::min(lo[k], x);
+            }
+        for (int k = 0; k < 3; ++k) cen[3 * i + k] = c[k];
+        double d2 = 0.0;
+        for (int k = 0; k < 3; ++k) {
+            const double d = h->X[3L * cn[6] + k] - h->X[3L * cn[0] + k];
+            d2 += d * d;
+        }
+        hsum += std::sqrt(d2 / 3.0);
+    }
+    const double hh = hsum / (double)n;
+    if (!(hh > 0.0)) return;
+    const double tile[3] = {4.0 * hh, 2.0 * hh, 2.0 * hh};
+    std::vector<std::pair<std::array<long long, 3>, int>> key(n);
+    for (size_t i = 0; i < n; ++i) {
+        std::array<long long, 3> t;
+        for (int k = 0; k < 3; ++k) t[2 - k] = (long long)std::floor((cen[3 * i + k] - lo[k]) / tile[k]);
+        key[i] = {t, elems[i]};
+    }
+    std::sort(key.begin(), key.end());
+    for (size_t i = 0; i < n; ++i) elems[i] = key[i].second;
+}
+
+// Tables of the patch assembly (element_kernel.hpp, assembly_kernel.hpp): for every patch (kEPB
+// consecutive slots of a launch range) the distinct node blocks its pieces contribute to (the
+// diagonal blocks first, each group in block order), their contributions in slot order and the
+// staging place of every piece; the contribution map of the gather over patch records (patch
+// order inside every block); the map of the residual-only pass over the patches' Fe records (the
+// same sums, so both passes give the same Fint bit for bit).
 static int build_gather_map(ceos_handle* h) {
     const int nn = h->nn, ne = h->ne;
-    if ((long long)ne * kRecord >= (1LL << 31)) return CEOS_ERR_ARG;
+    if ((long long)ne * 9 * kPieces >= (1LL << 31)) return CEOS_ERR_ARG;
     std::vector<int> blkstart(nn);
     long long nb = 0;
     for (int A = 0; A < nn; ++A) {
@@ -339,60 +408,136 @@ static int build_gather_map(ceos_handle* h) {
     for (int A = 0; A < nn; ++A)
         for (int q = 0; q <= h->nup[A]; ++q) blknode[blkstart[A] + q] = A;
     h->nblocks = nb;
-    std::vector<int> cptr(nb + 1, 0);
-    std::vector<uint32_t> cword((size_t)ne * 36);
-    // pass 0 counts, pass 1 fills
-    std::vector<int> fill;
-    h->blk_last_slot.assign(nb, -1);
-    for (int pass = 0; pass < 2; ++pass) {
-        if (pass == 1) {
-            for (long long b = 0; b < nb; ++b) cptr[b + 1] += cptr[b];
-            fill.assign(cptr.begin(), cptr.end() - 1);
-        }
-        for (int e = 0; e < ne; ++e) {
-            const int* cn = &h->conn0[(size_t)8 * e];
-            const long long base = (long long)h->iperm[e] * kRecord;
+    // patches: kEPB-slot chunks of every launch range (the ranges tile [0, ne) in order)
+    std::vector<int> pstart;
+    for (auto& r : h->ranges) {
+        r.patch0 = (int)pstart.size();
+        for (int sl = r.begin; sl < r.end; sl += kEPB) pstart.push_back(sl);
+    }
+    const int np = (int)pstart.size();
+    pstart.push_back(ne);
+    std::vector<uint16_t> ptab((size_t)ne * kPieces);
+    std::vector<int4> pinfo(np);
+    std::vector<int2> pdiag(np);
+    std::vector<int> fe_node;  // node of every Fe record (diagonal records, patch order)
+    fe_node.reserve((size_t)ne * 4);
+    std::vector<uint16_t> pslot;
+    std::vector<int> rec_blk;  // node block of every patch record
+    std::vector<int> rec_patch;
+    pslot.reserve((size_t)ne * 24);
+    rec_blk.reserve((size_t)ne * 24);
+    rec_patch.reserve((size_t)ne * 24);
+    struct Piece {
+        int blk, slot, q;
+        bool tr;
+    };
+    std::vector<Piece> pc;
+    long long rec_total = 0;
+    for (int p = 0; p < np; ++p) {
+        pc.clear();
+        for (int sl = pstart[p]; sl < pstart[p + 1]; ++sl) {
+            const int* cn = &h->conn0[(size_t)8 * h->perm[sl]];
             for (int b = 0; b < 8; ++b) {
                 const int B = cn[b];
-                const int np = b < 4 ? 4 : 3;
-                int blks[5];
-                uint32_t words[5];
-                blks[0] = blkstart[B];
-                words[0] = (uint32_t)(base + 9 * piece_pos(b, 0) + 3);
-                for (int j = 0; j < np; ++j) {
+                const int npt = b < 4 ? 4 : 3;
+                pc.push_back({blkstart[B], sl, piece_base(b), false});
+                for (int j = 0; j < npt; ++j) {
                     const int C = cn[(b + j + 1) & 7];
-                    const uint32_t off = (uint32_t)(base + 9 * piece_pos(b, 1 + j));
-                    if (C < B) {
-                        const int r = rank_of(h, C, B);
-                        if (r < 0) return CEOS_ERR_SLOT;
-                        blks[1 + j] = blkstart[C] + 1 + r;
-                        words[1 + j] = off;
-                    } else {
-                        const int r = rank_of(h, B, C);
-                        if (r < 0) return CEOS_ERR_SLOT;
-                        blks[1 + j] = blkstart[B] + 1 + r;
-                        words[1 + j] = off | 0x80000000u;
-                    }
-                }
-                for (int k = 0; k <= np; ++k) {
-                    if (pass == 0) {
-                        cptr[blks[k] + 1]++;
-                    } else {
-                        cword[fill[blks[k]]++] = words[k];
-                        h->blk_last_slot[blks[k]] = std::max(h->blk_last_slot[blks[k]], h->iperm[e]);
-                    }
+                    const int lo = std::min(B, C), hi = std::max(B, C);
+                    const int r = rank_of(h, lo, hi);
+                    if (r < 0) return CEOS_ERR_SLOT;
+                    pc.push_back({blkstart[lo] + 1 + r, sl, piece_base(b) + 1 + j, C > B});
                 }
             }
         }
+        std::stable_sort(pc.begin(), pc.end(), [](const Piece& a, const Piece& b) {
+            return a.blk != b.blk ? a.blk < b.blk : a.slot < b.slot;
+        });
+        // records = runs of equal blocks, the diagonal ones first (stable: block order inside
+        // each group, slot order inside each run); the halves split the pieces about evenly
+        std::stable_partition(pc.begin(), pc.end(), [&](const Piece& a) { return a.blk == blkstart[blknode[a.blk]]; });
+        std::vector<int> run;  // first piece of every record
+        int ndiag = 0;
+        for (size_t i = 0; i < pc.size(); ++i)
+            if (i == 0 || pc[i].blk != pc[i - 1].blk) {
+                run.push_back((int)i);
+                if (pc[i].blk == blkstart[blknode[pc[i].blk]]) {
+                    ++ndiag;
+                    fe_node.push_back(blknode[pc[i].blk]);
+                }
+            }
+        const int nrec = (int)run.size();
+        run.push_back((int)pc.size());
+        int split = nrec;
+        for (int r = 0; r < nrec; ++r)
+            if (2 * run[r] >= (int)pc.size()) { split = r; break; }
+        if (run[split] > kStageMax || (int)pc.size() - run[split] > kStageMax) return CEOS_ERR_ARG;
+        if (rec_total * 9 >= (1LL << 31)) return CEOS_ERR_ARG;
+        pinfo[p] = make_int4((int)(rec_total * 9), (int)pslot.size(), nrec, split);
+        pdiag[p] = make_int2((int)(3 * (fe_node.size() - ndiag)), ndiag);
+        // staging of a half: the first contribution of record r at r - r0, the further ones of
+        // every record after all the firsts, in record order
+        int extra = 0;
+        for (int r = 0; r < nrec; ++r) {
+            const int half = r >= split ? 1 : 0;
+            const int r0 = half ? split : 0;
+            if (r == r0) extra = (half ? nrec : split) - r0;
+            const int cnt = run[r + 1] - run[r];
+            pslot.push_back((uint16_t)(extra | (cnt << 10)));
+            for (int i = run[r]; i < run[r + 1]; ++i) {
+                const Piece& q = pc[i];
+                const int pos = i == run[r] ? r - r0 : extra++;
+                ptab[(size_t)q.slot * kPieces + q.q] = (uint16_t)(pos | (half << 10) | ((q.tr ? 1 : 0) << 11));
+            }
+            rec_blk.push_back(pc[run[r]].blk);
+            rec_patch.push_back(p);
+        }
+        rec_total += nrec;
+    }
+    h->rec_doubles = rec_total * 9;
+    // contribution map: block -> patch records (patch order)
+    std::vector<int> cptr(nb + 1, 0);
+    for (int b : rec_blk) cptr[b + 1]++;
+    for (long long b = 0; b < nb; ++b) cptr[b + 1] += cptr[b];
+    std::vector<uint32_t> cword(rec_total);
+    std::vector<int> fill(cptr.begin(), cptr.end() - 1);
+    h->blk_last_slot.assign(nb, -1);
+    for (long long i = 0; i < rec_total; ++i) {
+        const int b = rec_blk[i], p = rec_patch[i];
+        const bool diag = b == blkstart[blknode[b]];
+        cword[fill[b]++] = (uint32_t)(9 * i + (diag ? 3 : 0));
+        h->blk_last_slot[b] = std::max(h->blk_last_slot[b], pstart[p + 1] - 1);
+    }
+    // residual-only pass: node -> Fe records (3 doubles each), patch order
+    h->fe_doubles = 3LL * (long long)fe_node.size();
+    std::vector<int> fptr(nn + 1, 0);
+    for (int A : fe_node) fptr[A + 1]++;
+    for (int A = 0; A < nn; ++A) fptr[A + 1] += fptr[A];
+    std::vector<uint32_t> fword(fe_node.size());
+    {
+        std::vector<int> ff(fptr.begin(), fptr.end() - 1);
+        for (size_t i = 0; i < fe_node.size(); ++i) fword[ff[fe_node[i]]++] = (uint32_t)(3 * i);
     }
     HIP_OK(dalloc_once(h->dblkstart, nn));
     HIP_OK(dalloc_once(h->dblknode, nb));
     HIP_OK(dalloc_once(h->dcptr, nb + 1));
-    HIP_OK(dalloc_once(h->dcword, (size_t)ne * 36));
+    HIP_OK(dalloc_fit(h->dcword, h->cword_cap, cword.size()));
+    HIP_OK(dalloc_once(h->dptab, ptab.size()));
+    HIP_OK(dalloc_fit(h->dpinfo, h->pinfo_cap, pinfo.size()));
+    HIP_OK(dalloc_fit(h->dpslot, h->pslot_cap, pslot.size()));
+    HIP_OK(dalloc_once(h->dfptr, nn + 1));
+    HIP_OK(dalloc_fit(h->dfword, h->fword_cap, fword.size()));
+    HIP_OK(dalloc_fit(h->dpdiag, h->pdiag_cap, pdiag.size()));
+    HIP_OK(hipMemcpy(h->dpdiag, pdiag.data(), sizeof(int2) * pdiag.size(), hipMemcpyHostToDevice));
     HIP_OK(hipMemcpy(h->dblkstart, blkstart.data(), sizeof(int) * nn, hipMemcpyHostToDevice));
     HIP_OK(hipMemcpy(h->dblknode, blknode.data(), sizeof(int) * nb, hipMemcpyHostToDevice));
     HIP_OK(hipMemcpy(h->dcptr, cptr.data(), sizeof(int) * (nb + 1), hipMemcpyHostToDevice));
     HIP_OK(hipMemcpy(h->dcword, cword.data(), sizeof(uint32_t) * cword.size(), hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(h->dptab, ptab.data(), sizeof(uint16_t) * ptab.size(), hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(h->dpinfo, pinfo.data(), sizeof(int4) * pinfo.size(), hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(h->dpslot, pslot.data(), sizeof(uint16_t) * pslot.size(), hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(h->dfptr, fptr.data(), sizeof(int) * (nn + 1), hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(h->dfword, fword.data(), sizeof(uint32_t) * fword.size(), hipMemcpyHostToDevice));
     return CEOS_OK;
 }
 
@@ -407,7 +552,16 @@ static int plan_pipeline(ceos_handle* h) {
     S = std::min(S, std::max(1, h->ne / 1024));
     h->pipe_chunks = S;
     h->pipe_slot.assign(S + 1, 0);
-    for (int k = 0; k <= S; ++k) h->pipe_slot[k] = (long long)h->ne * k / S;
+    // chunk boundaries at patch starts (a patch's records come from one workgroup)
+    std::vector<int> starts;
+    for (const auto& r : h->ranges)
+        for (int sl = r.begin; sl < r.end; sl += kEPB) starts.push_back(sl);
+    for (int k = 0; k <= S; ++k) {
+        const long long want = (long long)h->ne * k / S;
+        const auto it = std::lower_bound(starts.begin(), starts.end(), (int)std::min<long long>(want, h->ne));
+        h->pipe_slot[k] = (k == S || it == starts.end()) ? h->ne : *it;
+    }
+    h->pipe_slot[0] = 0;
     h->pipe_blk.assign(S + 1, h->nblocks);
     h->pipe_blk[0] = 0;
     int chunk_prefix = 0;
@@ -481,13 +635,14 @@ static int finalize(ceos_handle* h) {
             bucket[(size_t)h->color[e] * models.size() + mi].push_back(e);
         }
         if (h->assembly == CEOS_ASSEMBLY_GATHER) {
-            // no write conflicts: one launch per model, elements in id order
+            // no write conflicts: one launch per model, elements in patch order
             for (auto& v : bucket) v.clear();
             for (int e = 0; e < ne; ++e) {
                 const int m = h->mats[h->mat_of[e]].model;
                 const int mi = (int)(std::lower_bound(models.begin(), models.end(), m) - models.begin());
                 bucket[mi].push_back(e);
             }
+            for (size_t mi = 0; mi < models.size(); ++mi) patch_order(h, bucket[mi]);
         }
         const int ncol = h->assembly == CEOS_ASSEMBLY_GATHER ? 1 : h->ncolor;
         // execution order: for each chunk of the id-ordered colour lists, every colour (and
@@ -614,7 +769,7 @@ static int finalize(ceos_handle* h) {
 // element record buffer of the gather assembly (records of every element, ne * kRecord doubles)
 static int ensure_records(ceos_handle* h) {
     if (h->assembly != CEOS_ASSEMBLY_GATHER) return CEOS_OK;
-    HIP_OK(dalloc_once(h->dscratch, (size_t)h->ne * kRecord));
+    HIP_OK(dalloc_fit(h->dscratch, h->scratch_cap, (size_t)h->rec_doubles));
     return CEOS_OK;
 }
 
@@ -645,7 +800,8 @@ static int after_caller_stream(ceos_handle* h) {
 // tables of the evaluation (finalize) and the record buffer; refuses to build them on a stream
 // under capture (building synchronises and allocates)
 static int prepare_for(ceos_handle* h, hipStream_t s) {
-    if (h->finalized && (h->assembly != CEOS_ASSEMBLY_GATHER || h->dscratch)) return CEOS_OK;
+    if (h->finalized && (h->assembly != CEOS_ASSEMBLY_GATHER || (h->dscratch && h->scratch_cap >= (size_t)h->rec_doubles)))
+        return CEOS_OK;
     if (stream_capturing(s)) return CEOS_ERR_ARG;
     int rc = finalize(h);
     if (rc) return rc;
@@ -656,36 +812,37 @@ static int prepare_for(ceos_handle* h, hipStream_t s) {
 // kernel dispatch
 // ------------------------------------------------------------------------------------------
 template <int MODEL, bool NL, bool WANT_K, typename KT, bool G, bool NUMT = false>
-static void launch_t(const ElemParams& p, int b, int e, hipStream_t s) {
+static void launch_t(const ElemParams& p, int b, int e, int patch0, hipStream_t s) {
     const int n = e - b;
     const int grid = (n + kEPB - 1) / kEPB;
-    hipLaunchKernelGGL((k_element<MODEL, NL, WANT_K, KT, G, NUMT>), dim3(grid), dim3(kEPB * 8), 0, s, p, b, e);
+    hipLaunchKernelGGL((k_element<MODEL, NL, WANT_K, KT, G, NUMT>), dim3(grid), dim3(kEPB * 8), 0, s, p, b, e,
+                       patch0);
 }
 
 template <int MODEL, typename KT, bool G>
-static void launch_m(const ElemParams& p, int b, int e, bool nl, bool wk, bool numt, hipStream_t s) {
+static void launch_m(const ElemParams& p, int b, int e, int pc, bool nl, bool wk, bool numt, hipStream_t s) {
     if (nl) {
-        if (wk && numt) launch_t<MODEL, true, true, KT, G, true>(p, b, e, s);
-        else if (wk) launch_t<MODEL, true, true, KT, G>(p, b, e, s);
-        else launch_t<MODEL, true, false, KT, G>(p, b, e, s);
+        if (wk && numt) launch_t<MODEL, true, true, KT, G, true>(p, b, e, pc, s);
+        else if (wk) launch_t<MODEL, true, true, KT, G>(p, b, e, pc, s);
+        else launch_t<MODEL, true, false, KT, G>(p, b, e, pc, s);
     } else {
-        if (wk && numt) launch_t<MODEL, false, true, KT, G, true>(p, b, e, s);
-        else if (wk) launch_t<MODEL, false, true, KT, G>(p, b, e, s);
-        else launch_t<MODEL, false, false, KT, G>(p, b, e, s);
+        if (wk && numt) launch_t<MODEL, false, true, KT, G, true>(p, b, e, pc, s);
+        else if (wk) launch_t<MODEL, false, true, KT, G>(p, b, e, pc, s);
+        else launch_t<MODEL, false, false, KT, G>(p, b, e, pc, s);
     }
 }
 
 template <typename KT, bool G>
 static void launch_range(const ElemParams& p, const LaunchRange& r, bool nl, bool wk, bool numt, hipStream_t s) {
     switch (r.model) {
-        case kGHL: launch_m<kGHL, KT, G>(p, r.begin, r.end, nl, wk, numt, s); break;
-        case kJ2: launch_m<kJ2, KT, G>(p, r.begin, r.end, nl, wk, numt, s); break;
-        case kNH: launch_m<kNH, KT, G>(p, r.begin, r.end, nl, wk, numt, s); break;
-        case kNHQ1: launch_m<kNHQ1, KT, G>(p, r.begin, r.end, nl, wk, numt, s); break;
-        case kSVK: launch_m<kSVK, KT, G>(p, r.begin, r.end, nl, wk, numt, s); break;
-        case kCNH: launch_m<kCNH, KT, G>(p, r.begin, r.end, nl, wk, numt, s); break;
-        case kVISCO: launch_m<kVISCO, KT, G>(p, r.begin, r.end, nl, wk, numt, s); break;
-        case kEXT: launch_m<kEXT, KT, G>(p, r.begin, r.end, nl, wk, false, s); break;  // tangent is given
+        case kGHL: launch_m<kGHL, KT, G>(p, r.begin, r.end, r.patch0, nl, wk, numt, s); break;
+        case kJ2: launch_m<kJ2, KT, G>(p, r.begin, r.end, r.patch0, nl, wk, numt, s); break;
+        case kNH: launch_m<kNH, KT, G>(p, r.begin, r.end, r.patch0, nl, wk, numt, s); break;
+        case kNHQ1: launch_m<kNHQ1, KT, G>(p, r.begin, r.end, r.patch0, nl, wk, numt, s); break;
+        case kSVK: launch_m<kSVK, KT, G>(p, r.begin, r.end, r.patch0, nl, wk, numt, s); break;
+        case kCNH: launch_m<kCNH, KT, G>(p, r.begin, r.end, r.patch0, nl, wk, numt, s); break;
+        case kVISCO: launch_m<kVISCO, KT, G>(p, r.begin, r.end, r.patch0, nl, wk, numt, s); break;
+        case kEXT: launch_m<kEXT, KT, G>(p, r.begin, r.end, r.patch0, nl, wk, false, s); break;  // tangent is given
         default: break;
     }
 }
@@ -731,6 +888,10 @@ static int evaluate(ceos_handle* h, const double* dU, double* dFint, void* dK, h
     p.time = h->time;
     hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(1), 0, s, h->derr);
     p.scratch = rec;
+    p.ptab = h->dptab;
+    p.pinfo = h->dpinfo;
+    p.pslot = h->dpslot;
+    p.pdiag = h->dpdiag;
     hipEvent_t* ev = (h->timing_on && h->timing_n < h->timing_cap) ? &h->tev[3 * h->timing_n] : nullptr;
     if (ev) HIP_OK(hipEventRecord(ev[0], s));
     const bool wk = dK != nullptr;
@@ -748,6 +909,8 @@ static int evaluate(ceos_handle* h, const double* dU, double* dFint, void* dK, h
     g.blkstart = h->dblkstart;
     g.cptr = h->dcptr;
     g.cword = h->dcword;
+    g.fptr = h->dfptr;
+    g.fword = h->dfword;
     g.rec = rec;
     g.Fint = dFint;
     g.K = dK;
@@ -761,6 +924,7 @@ static int evaluate(ceos_handle* h, const double* dU, double* dFint, void* dK, h
                 LaunchRange q = r;
                 q.begin = (int)std::max<long long>(r.begin, h->pipe_slot[k]);
                 q.end = (int)std::min<long long>(r.end, h->pipe_slot[k + 1]);
+                q.patch0 = r.patch0 + (q.begin - r.begin) / kEPB;  // chunks start at patch starts
                 if (q.end > q.begin) launch_range<double, true>(p, q, h->nl != 0, wk, numt, s);
             }
             HIP_OK(hipEventRecord(h->pipe_ev[k], s));
@@ -1614,7 +1778,7 @@ int ceos_evaluate_device_static(ceos_handle* h, double time, const double* dU, d
         fprintf(stderr, "ceos_evaluate_device_static: the handle is not prepared (call ceos_mesh_prepare)\n");
         return CEOS_ERR_ARG;
     }
-    const size_t need = (size_t)h->ne * (dK ? kRecord : 24);
+    const size_t need = dK ? (size_t)h->rec_doubles : (size_t)h->fe_doubles;
     if (h->assembly == CEOS_ASSEMBLY_GATHER && need > kStaticScratchDoubles) {
         fprintf(stderr,
                 "ceos_evaluate_device_static: %d elements need %.1f MiB of element records (%s), the static "

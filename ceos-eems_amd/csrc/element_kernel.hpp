@@ -46,32 +46,31 @@ constexpr int kOffSig = 72;
 constexpr int kOffD = 120;
 constexpr int kJ2State = 8;           // [0..5] plastic strain, [6] acc. plastic strain, [7] flag
 constexpr int kStateStride = 20;      // state rows per Gauss point in the device arrays (max of models)
-// Element record of the gather assembly: 36 pieces of 9 doubles.  Lane b computes piece k = 0,
-// [Fe_b (3), diagonal block (b,b) upper 00 01 02 11 12 22], and pieces k = 1 + j, the block (c,b)
-// of its partner c = (b + j + 1) mod 8 (j < 4 for b < 4, j < 3 otherwise), rows = dofs of c,
-// row-major.  Pieces sit in the record grouped by the node whose CSR row receives them (the
-// diagonal of n, then every pair whose other node follows n), the groups ordered as the nodes of
-// a hex numbered lexicographically (x fastest: local 0 1 3 2 4 5 7 6).  On such meshes each
-// group is read by one node's gather threads, so the gather reads whole runs of 1 to 8 pieces
-// instead of lone 72-byte pieces; on any other mesh the placement is merely a fixed permutation
-// (the contribution map records where every piece is).
-constexpr int kRecord = 36 * 9;  // 324 doubles per element
-__host__ __device__ constexpr int lex_rank(int n) { return (n == 2 || n == 6) ? n + 1 : (n == 3 || n == 7) ? n - 1 : n; }
-__host__ __device__ constexpr int group_start(int t) { return 8 * t - t * (t - 1) / 2; }  // 8 - t pieces in group t
-// lane whose diagonal piece sits at record position pos (the inverse of piece_pos(b, 0))
-__host__ __device__ constexpr int diag_lane(int pos) {
-    int t = 0;
-    while (t < 7 && group_start(t + 1) <= pos) ++t;
-    return lex_rank(t);  // lex_rank is its own inverse
-}
-// record position (in pieces) of piece k of lane b
-__host__ __device__ constexpr int piece_pos(int b, int k) {
-    const int rb = lex_rank(b);
-    if (k == 0) return group_start(rb);
-    const int rc = lex_rank((b + k) & 7);
-    const int r = rb < rc ? rb : rc, o = rb < rc ? rc : rb;
-    return group_start(r) + o - r;
-}
+// Pieces of the gather assembly: lane b computes piece k = 0, [Fe_b (3), diagonal block (b,b)
+// upper 00 01 02 11 12 22], and pieces k = 1 + j, the block (c,b) of its partner
+// c = (b + j + 1) mod 8 (j < 4 for b < 4, j < 3 otherwise), rows = dofs of c, row-major; 36 pieces
+// of 9 doubles per element, numbered q = kPieceBase[b] + k.
+//
+// Patches: the 16 elements of one workgroup (consecutive slots, chosen spatially compact on the
+// host) share node blocks.  Their pieces are summed in LDS, block by block in slot order, and the
+// workgroup writes one 9-double patch record per distinct node block of the patch (on a 4x2x2
+// brick of a structured mesh 341 instead of 576 pieces).  Every off-diagonal patch record holds its
+// CSR block (A,B), A < B, with rows = dofs of A: a piece whose partner node has the larger global
+// id is staged transposed.  Tables built once per mesh (ceos_gpu.hip):
+//   ptab[slot][36]  staging entry of each piece: bits 0..9 position in its half's staging area,
+//                   bit 10 half, bit 11 transposed (a half's staging area holds the first
+//                   contribution of its record r at r - r0, the further ones after all firsts);
+//   pinfo[patch]    {offset of the patch's records (doubles), offset of its slot table, number of
+//                   records, first record of the second half};
+//   pslot[]         per record: bits 0..9 staging position of its second contribution, bits
+//                   10..13 contributions;
+//   pdiag[patch]    {offset of its Fe records (residual-only pass), number of diagonal records}.
+// A patch's diagonal records come first; the residual-only pass stages only the Fe parts and
+// writes one 3-double Fe record per diagonal record, summed in the same order as the F+K pass.
+// The staging area of one half holds at most kStageMax pieces.
+constexpr int kPieces = 36;
+constexpr int kStageMax = (kEPB * kLdsPerElem) / 9;  // 526 pieces of 9 doubles in the LDS slices
+__host__ __device__ constexpr int piece_base(int b) { return b < 4 ? 5 * b : 20 + 4 * (b - 4); }
 
 struct ElemParams {
     int ne;                        // elements (device slot order)
@@ -88,8 +87,12 @@ struct ElemParams {
     double* Fint;                  // [3 nn]
     void* K;                       // [nnz] double or float
     double* sig;                   // [6][8][ne]
-    double* scratch;               // [ne][kRecord] element records (gather assembly); [ne][8][3] Fe
-                                   // parts in the residual-only pass
+    double* scratch;               // patch records (gather assembly); [ne][8][3] Fe parts in the
+                                   // residual-only pass
+    const uint16_t* ptab;          // [ne][36] staging entries of the pieces (gather assembly)
+    const int4* pinfo;             // [patches] record offset, slot-table offset, records, split
+    const uint16_t* pslot;         // per patch record: position of its 2nd contribution | contributions << 10
+    const int2* pdiag;             // [patches] residual-only pass: offset of its Fe records, count
     const double* st_old;          // [nstate][8][ne] committed state
     double* st_new;                // [nstate][8][ne] state of the last update
     uint8_t* elem_err;             // [ne]: bit 0 detJ <= 1e-13, bit 1 local Newton failed
@@ -254,15 +257,23 @@ __device__ __forceinline__ void node_blocks(const double* sm, int b, double& fe0
                 DB[I][2] = DD(I, 2) * b2 + DD(I, 4) * b1 + DD(I, 5) * b0;
             }
 #undef DD
+            // The block entries accumulate through FMA chains straight into the accumulators
+            // (acc = fma(c, DB, acc) three times): a sum of three products added to acc would
+            // cost a MUL, two FMAs and an ADD per entry.
             // diagonal block, upper part (k >= i)
             {
-                const double geo = NL ? (b0 * sg0 + b1 * sg1 + b2 * sg2) : 0.0;
-                kd[0] += b0 * DB[0][0] + b1 * DB[3][0] + b2 * DB[5][0] + geo;
-                kd[1] += b0 * DB[0][1] + b1 * DB[3][1] + b2 * DB[5][1];
-                kd[2] += b0 * DB[0][2] + b1 * DB[3][2] + b2 * DB[5][2];
-                kd[3] += b1 * DB[1][1] + b0 * DB[3][1] + b2 * DB[4][1] + geo;
-                kd[4] += b1 * DB[1][2] + b0 * DB[3][2] + b2 * DB[4][2];
-                kd[5] += b2 * DB[2][2] + b1 * DB[4][2] + b0 * DB[5][2] + geo;
+                kd[0] = fma(b0, DB[0][0], kd[0]); kd[0] = fma(b1, DB[3][0], kd[0]); kd[0] = fma(b2, DB[5][0], kd[0]);
+                kd[1] = fma(b0, DB[0][1], kd[1]); kd[1] = fma(b1, DB[3][1], kd[1]); kd[1] = fma(b2, DB[5][1], kd[1]);
+                kd[2] = fma(b0, DB[0][2], kd[2]); kd[2] = fma(b1, DB[3][2], kd[2]); kd[2] = fma(b2, DB[5][2], kd[2]);
+                kd[3] = fma(b1, DB[1][1], kd[3]); kd[3] = fma(b0, DB[3][1], kd[3]); kd[3] = fma(b2, DB[4][1], kd[3]);
+                kd[4] = fma(b1, DB[1][2], kd[4]); kd[4] = fma(b0, DB[3][2], kd[4]); kd[4] = fma(b2, DB[4][2], kd[4]);
+                kd[5] = fma(b2, DB[2][2], kd[5]); kd[5] = fma(b1, DB[4][2], kd[5]); kd[5] = fma(b0, DB[5][2], kd[5]);
+                if constexpr (NL) {
+                    const double geo = b0 * sg0 + b1 * sg1 + b2 * sg2;
+                    kd[0] += geo;
+                    kd[3] += geo;
+                    kd[5] += geo;
+                }
             }
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
@@ -277,16 +288,22 @@ __device__ __forceinline__ void node_blocks(const double* sm, int b, double& fe0
                     }
                     grad_from_jinv(Ji, d, gc);
                     const double c0 = gc[0], c1 = gc[1], c2 = gc[2];
-                    const double geo = NL ? (c0 * sg0 + c1 * sg1 + c2 * sg2) : 0.0;
 #pragma unroll
                     for (int k = 0; k < 3; ++k) {
-                        ko[j][0 * 3 + k] += c0 * DB[0][k] + c1 * DB[3][k] + c2 * DB[5][k];
-                        ko[j][1 * 3 + k] += c1 * DB[1][k] + c0 * DB[3][k] + c2 * DB[4][k];
-                        ko[j][2 * 3 + k] += c2 * DB[2][k] + c1 * DB[4][k] + c0 * DB[5][k];
+                        double& r0 = ko[j][0 * 3 + k];
+                        double& r1 = ko[j][1 * 3 + k];
+                        double& r2 = ko[j][2 * 3 + k];
+                        r0 = fma(c0, DB[0][k], r0); r0 = fma(c1, DB[3][k], r0); r0 = fma(c2, DB[5][k], r0);
+                        r1 = fma(c1, DB[1][k], r1); r1 = fma(c0, DB[3][k], r1); r1 = fma(c2, DB[4][k], r1);
+                        r2 = fma(c2, DB[2][k], r2); r2 = fma(c1, DB[4][k], r2); r2 = fma(c0, DB[5][k], r2);
                     }
-                    ko[j][0] += geo;
-                    ko[j][4] += geo;
-                    ko[j][8] += geo;
+                    if constexpr (NL) {
+                        // geometric part (gradN_c . sigma gradN_b) on the block's diagonal
+                        const double geo = c0 * sg0 + c1 * sg1 + c2 * sg2;
+                        ko[j][0] += geo;
+                        ko[j][4] += geo;
+                        ko[j][8] += geo;
+                    }
                 }
             }
         }
@@ -296,7 +313,7 @@ __device__ __forceinline__ void node_blocks(const double* sm, int b, double& fe0
 // NUMT: tangent modulus by central differences (a separate instantiation, so that the default
 // kernel carries no call and keeps its register allocation)
 template <int MODEL, bool NL, bool WANT_K, typename KT, bool GATHER, bool NUMT = false>
-__global__ __launch_bounds__(128) void k_element(ElemParams p, int e_begin, int e_end) {
+__global__ __launch_bounds__(128) void k_element(ElemParams p, int e_begin, int e_end, int patch_begin) {
     __shared__ double smem[kEPB * kLdsPerElem];
     __shared__ int snode[GATHER ? 1 : kEPB][8];  // partner node ids (scatter assembly only)
     __shared__ double sdn[8 * 8 * 4];             // dN/dxi of node a at Gauss point g: [g][a][3 + pad]
@@ -318,6 +335,10 @@ __global__ __launch_bounds__(128) void k_element(ElemParams p, int e_begin, int 
     int node = 0;
     if (active) node = p.conn[(long)e * 8 + lane];
     if constexpr (!GATHER) snode[le][lane] = node;
+    // patch header and this lane's staging entries, requested with the node data
+    int4 pi = make_int4(0, 0, 0, 0);
+    int2 pd = make_int2(0, 0);
+    unsigned ptv[5] = {0u, 0u, 0u, 0u, 0u};
     {
         double v[9];
 #pragma unroll
@@ -325,6 +346,13 @@ __global__ __launch_bounds__(128) void k_element(ElemParams p, int e_begin, int 
             v[k] = active ? p.X[3L * node + k] : 0.0;
             v[3 + k] = active ? p.x[3L * node + k] : 0.0;
             v[6 + k] = active ? p.U[3L * node + k] : 0.0;
+        }
+        if constexpr (GATHER) {
+            pi = p.pinfo[patch_begin + blockIdx.x];
+            if constexpr (!WANT_K) pd = p.pdiag[patch_begin + blockIdx.x];
+            const uint16_t* tab = p.ptab + (long)(active ? e : e_begin) * kPieces + piece_base(lane);
+#pragma unroll
+            for (int k = 0; k < 5; ++k) ptv[k] = (WANT_K || k == 0) && (k < 4 || lane < 4) ? tab[k] : 0u;
         }
 #pragma unroll
         for (int k = 0; k < 9; ++k) sm[lane * 9 + k] = v[k];
@@ -451,6 +479,17 @@ __global__ __launch_bounds__(128) void k_element(ElemParams p, int e_begin, int 
     __syncthreads();
 
     // ---------------- phase 2: node b = lane -----------------
+    // words of the patch records this thread sums (records tid + 128 i; a patch has at most
+    // 36 kEPB = 576), requested before the block computation that hides their latency
+    constexpr int kRecPerThread = (kPieces * kEPB + kEPB * 8 - 1) / (kEPB * 8);
+    unsigned psw[kRecPerThread];
+    if constexpr (GATHER) {
+#pragma unroll
+        for (int i = 0; i < kRecPerThread; ++i) {
+            const int r = (int)threadIdx.x + kEPB * 8 * i;
+            psw[i] = r < pi.z ? p.pslot[pi.y + r] : 0u;
+        }
+    }
     const int b = lane;
     const int npart = (b < 4) ? 4 : 3;
     double fe0, fe1, fe2;
@@ -493,54 +532,112 @@ __global__ __launch_bounds__(128) void k_element(ElemParams p, int e_begin, int 
         }
     }
     if constexpr (GATHER && !WANT_K) {
-        // residual only: the element's 8 Fe parts, 24 consecutive doubles at 24 slot
-        if (active) {
-            double* f = p.scratch + 24L * e + 3 * b;
-            f[0] = fe0;
-            f[1] = fe1;
-            f[2] = fe2;
+        // residual only: the Fe parts through the patch's diagonal records (the F+K pass's sums)
+        const unsigned t = ptv[0];
+#pragma unroll
+        for (int half = 0; half < 2; ++half) {
+            __syncthreads();
+            if (active && ((t >> 10) & 1u) == (unsigned)half) {
+                double* dst = smem + 9 * (t & 1023u);
+                dst[0] = fe0;
+                dst[1] = fe1;
+                dst[2] = fe2;
+            }
+            __syncthreads();
+            const int r0 = half ? pi.w : 0, r1 = min(half ? pi.z : pi.w, pd.y);
+#pragma unroll
+            for (int i = 0; i < kRecPerThread; ++i) {
+                const int r = (int)threadIdx.x + kEPB * 8 * i;
+                if (r < r0 || r >= r1) continue;
+                const unsigned w = psw[i];
+                double* dst = smem + 9 * (r - r0);
+                const double* ext = smem + 9 * (w & 1023u);
+                const int cnt = (int)(w >> 10);
+                double a0 = dst[0], a1 = dst[1], a2 = dst[2];
+#pragma unroll
+                for (int c = 1; c < 8; ++c) {
+                    if (c < cnt) {
+                        a0 += ext[9 * (c - 1)];
+                        a1 += ext[9 * (c - 1) + 1];
+                        a2 += ext[9 * (c - 1) + 2];
+                    }
+                }
+                dst[0] = a0;
+                dst[1] = a1;
+                dst[2] = a2;
+            }
+            __syncthreads();
+            const int n3 = 3 * max(r1 - r0, 0);
+            for (int o = (int)threadIdx.x; o < n3; o += kEPB * 8)
+                p.scratch[(long)pd.x + 3L * r0 + o] = smem[9 * (o / 3) + o % 3];
         }
         return;
     }
     if constexpr (GATHER) {
-        // stage the element records in LDS (the GP data are no longer read) and write the
-        // records of the block's consecutive elements with unit-stride stores, in two halves:
-        // pieces 0..19 (record doubles 0..179), then 20..35 (180..323); a slice holds 296
-        const int e0 = e_begin + blockIdx.x * kEPB;
-        const int nel = min(kEPB, e_end - e0);
-        double* dst = p.scratch + (long)e0 * kRecord;
-        constexpr int kHalf0 = 180, kHalf1 = kRecord - 180;
+        // patch assembly: in two halves, stage the pieces whose patch record falls in the half
+        // (each at its position, transposed where the record's orientation asks), then sum every
+        // record of the half over its contributions in slot order and write it (unit-stride
+        // across the workgroup's threads)
 #pragma unroll
         for (int half = 0; half < 2; ++half) {
-            __syncthreads();
-            const int skip = half ? kHalf0 : 0;
-            const int p0 = piece_pos(b, 0);
-            if ((p0 >= 20) == (half == 1)) {
-                double* rec = sm + 9 * p0 - skip;
-                rec[0] = fe0;
-                rec[1] = fe1;
-                rec[2] = fe2;
+            __syncthreads();  // the Gauss-point data (half 0) or the staged pieces (half 1) are read
+            if (active) {
 #pragma unroll
-                for (int q = 0; q < 6; ++q) rec[3 + q] = kd[q];
-            }
+                for (int k = 0; k < 5; ++k) {
+                    if (k <= npart) {
+                        const unsigned t = ptv[k];
+                        if (((t >> 10) & 1u) == (unsigned)half) {
+                            double* dst = smem + 9 * (t & 1023u);
+                            if (k == 0) {
+                                dst[0] = fe0;
+                                dst[1] = fe1;
+                                dst[2] = fe2;
 #pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                const int pj = piece_pos(b, 1 + j);
-                if (j < npart && (pj >= 20) == (half == 1)) {
-                    double* rec = sm + 9 * pj - skip;
+                                for (int q = 0; q < 6; ++q) dst[3 + q] = kd[q];
+                            } else if ((t >> 11) & 1u) {
 #pragma unroll
-                    for (int q = 0; q < 9; ++q) rec[q] = ko[j][q];
+                                for (int r = 0; r < 3; ++r)
+#pragma unroll
+                                    for (int c = 0; c < 3; ++c) dst[3 * r + c] = ko[k - 1][3 * c + r];
+                            } else {
+#pragma unroll
+                                for (int q = 0; q < 9; ++q) dst[q] = ko[k - 1][q];
+                            }
+                        }
+                    }
                 }
             }
             __syncthreads();
-            const int len = half ? kHalf1 : kHalf0;
-            const int off = half ? kHalf0 : 0;
-            for (int g2 = threadIdx.x; g2 < nel * len; g2 += kEPB * 8) {
-                const int el = g2 / len, r = g2 - el * len;
-                // non-temporal: the records are read back once, by the gather, long after this
-                // workgroup ends (1.33 GB per evaluation at 80^3), so they need not hold L2 lines
-                __builtin_nontemporal_store(smem[el * kLdsPerElem + r], dst + (long)el * kRecord + off + r);
+            const int r0 = half ? pi.w : 0, r1 = half ? pi.z : pi.w;
+            // sum every record in place: its first contribution sits at r - r0, the others at
+            // the record's extra positions; the half's records are then one contiguous block
+#pragma unroll
+            for (int i = 0; i < kRecPerThread; ++i) {
+                const int r = (int)threadIdx.x + kEPB * 8 * i;
+                if (r < r0 || r >= r1) continue;
+                const unsigned w = psw[i];
+                double* dst = smem + 9 * (r - r0);
+                const double* ext = smem + 9 * (w & 1023u);
+                const int cnt = (int)(w >> 10);
+                double acc[9];
+#pragma unroll
+                for (int q = 0; q < 9; ++q) acc[q] = dst[q];
+#pragma unroll
+                for (int c = 1; c < 8; ++c) {
+                    if (c < cnt) {
+#pragma unroll
+                        for (int q = 0; q < 9; ++q) acc[q] += ext[9 * (c - 1) + q];
+                    }
+                }
+#pragma unroll
+                for (int q = 0; q < 9; ++q) dst[q] = acc[q];
             }
+            __syncthreads();
+            // non-temporal, unit stride across the workgroup: the records are read back once, by
+            // the gather, long after this workgroup ends, so they need not hold L2 lines
+            double* out = p.scratch + (long)pi.x + 9L * r0;
+            const int n9 = 9 * (r1 - r0);
+            for (int o = (int)threadIdx.x; o < n9; o += kEPB * 8) __builtin_nontemporal_store(smem[o], out + o);
         }
         return;
     }

@@ -170,9 +170,10 @@ int ceos_mesh_prepare(ceos_handle *h, int records);
  * device it runs on (one per device, kept until the process ends), so the call allocates nothing,
  * frees nothing and synchronises nothing.  The handle must be prepared first
  * (ceos_mesh_prepare, records = 0 suffices); otherwise CEOS_ERR_ARG.  A mesh whose records do not
- * fit is refused with CEOS_ERR_ARG: F+K needs 2,592 B per element (at most 51,781 elements),
- * residual only (dK NULL) 192 B per element (at most 699,050).  Every call on one device shares
- * that buffer: calls that overlap in time on one device (two streams, two threads) race. */
+ * fit is refused with CEOS_ERR_ARG: F+K needs 72 B per node block of every 16-element patch (at
+ * most 2,592 B per element; about 1,535 B on a structured hex mesh, about 87,000 elements), residual
+ * only (dK NULL) 192 B per element (at most 699,050).  Every call on one device shares that
+ * buffer: calls that overlap in time on one device (two streams, two threads) race. */
 #define CEOS_STATIC_SCRATCH_MIB 128
 int ceos_evaluate_device_static(ceos_handle *h, double time, const double *dU, double *dFint, void *dK,
                                 void *stream);

@@ -59,8 +59,8 @@ def test_static_entry_matches_allocating_entry_twice_then_smaller():
 
 def test_static_entry_refuses_oversized_and_unprepared():
     s = hiprt.Stream()
-    # 38^3 = 54,872 elements: F+K records need 135.6 MiB > 128 MiB; residual only 10 MiB
-    g, U = _mesh(38, 23)
+    # 48^3 = 110,592 elements: F+K patch records need about 162 MiB > 128 MiB; residual only 20 MiB
+    g, U = _mesh(48, 23)
     _, nnz = g.info()
     dU = hiprt.DeviceArray.from_numpy(U)
     dF, dK = hiprt.DeviceArray(len(U)), hiprt.DeviceArray(nnz)
