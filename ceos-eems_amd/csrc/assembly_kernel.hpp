@@ -5,6 +5,9 @@
 // by its transpose: every CSR row segment owns its output and sums the element blocks that
 // contribute to it, in element order, through a contribution map built once per mesh.
 //
+// Blocks whose contributions all come from one patch are written by the element kernel itself
+// (they have no contribution here and are skipped), as are the Fint parts of their nodes.
+//
 // Work unit = one node block of the upper pattern (the diagonal block of node A, or the block
 // (A, B) of its q-th upper neighbour B), enumerated in CSR order: block blk belongs to node
 // blknode[blk] and is its q = blk - blkstart[A]-th block.  The thread sums the contributing
@@ -50,6 +53,7 @@ __global__ __launch_bounds__(256) void k_gather_blocks(GatherParams p) {
     const long long blk = p.blk0 + (long long)blockIdx.x * 256 + threadIdx.x;
     if (blk >= p.blk1) return;
     const int c0 = p.cptr[blk], c1 = p.cptr[blk + 1];
+    if (c0 == c1) return;  // a complete block: its patch's workgroup wrote it (and Fint)
     const int A = p.blknode[blk];
     const int q = (int)(blk - p.blkstart[A]);
     const int nu = p.nup[A];
@@ -122,13 +126,14 @@ __global__ __launch_bounds__(256) void k_gather_blocks(GatherParams p) {
 }
 
 // residual-only gather: Fint of node A from the Fe records of its patches, in patch order (the
-// element kernel writes one 3-double Fe record per diagonal patch record in this pass; fword
-// holds their offsets).  One thread per node, the offsets and then their Fe records requested
+// element kernel writes one 3-double Fe record per boundary diagonal patch record in this pass;
+// fword holds their offsets).  One thread per node, the offsets and then their Fe records requested
 // kGatherBatch at a time.
 __global__ __launch_bounds__(256) void k_gather_fint(GatherParams p) {
     const int A = blockIdx.x * 256 + threadIdx.x;
     if (A >= p.nn) return;
     const int c0 = p.fptr[A], c1 = p.fptr[A + 1];
+    if (c0 == c1) return;  // the node's diagonal block is complete: its patch wrote Fint
     double f[3] = {0.0, 0.0, 0.0};
     for (int c = c0; c < c1; c += kGatherBatch) {
         uint32_t w[kGatherBatch];
@@ -151,6 +156,23 @@ __global__ __launch_bounds__(256) void k_gather_fint(GatherParams p) {
     }
 #pragma unroll
     for (int k = 0; k < 3; ++k) p.Fint[3L * A + k] = f[k];
+}
+
+// nodes that belong to no element: Fint and the diagonal K block are zero (the reference zeroes
+// Kg%Val and Fint before assembling)
+template <typename KT>
+__global__ __launch_bounds__(256) void k_zero_free_nodes(int n, const int* nodes, const int* rowstart, const int* nup,
+                                                         double* Fint, void* Kv) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const int A = nodes[i];
+    for (int k = 0; k < 3; ++k) Fint[3L * A + k] = 0.0;
+    if (!Kv) return;
+    KT* K = (KT*)Kv;
+    const long r0 = rowstart[A], r1 = r0 + 3 + 3 * nup[A], r2 = r1 + 2 + 3 * nup[A];
+    K[r0] = (KT)0; K[r0 + 1] = (KT)0; K[r0 + 2] = (KT)0;
+    K[r1] = (KT)0; K[r1 + 1] = (KT)0;
+    K[r2] = (KT)0;
 }
 
 }  // namespace ceos

@@ -200,12 +200,17 @@ struct ceos_handle {
     long long rec_doubles = 0;   // doubles of the patch records of one F+K evaluation
     uint16_t* dptab = nullptr;   // [ne][36] staging entries of the pieces
     int4* dpinfo = nullptr;      // [patches]
-    int2* dpdiag = nullptr;      // [patches] residual-only Fe records: offset (doubles), count
+    int4* dpinfo2 = nullptr;     // [patches] boundary records, boundary diagonal records, Fe offset,
+                                 // complete diagonal records
+    int2* dpkslot = nullptr;     // [patch records] complete records: K slot of row 0, node << 7 | nup
+    long long n_complete_records = 0;
+    int n_free_nodes = 0;        // nodes in no element (their Fint and diagonal K are zeroed)
+    int* dfree_nodes = nullptr;
     long long fe_doubles = 0;    // doubles of the Fe records of one residual-only evaluation
     uint16_t* dpslot = nullptr;  // [patch records]
     int* dfptr = nullptr;        // [nn+1] residual-only Fe map
     uint32_t* dfword = nullptr;  // [8 ne]
-    size_t pinfo_cap = 0, pslot_cap = 0, cword_cap = 0, pdiag_cap = 0, fword_cap = 0;
+    size_t pinfo_cap = 0, pslot_cap = 0, cword_cap = 0, pinfo2_cap = 0, pkslot_cap = 0, fword_cap = 0;
     // ordering of the handle's own stream after device entries that ran on a caller's stream
     hipEvent_t ev_caller = nullptr;
     bool caller_pending = false;
@@ -327,7 +332,8 @@ static void free_device(ceos_handle* h) {
     dfree(h->dconn); dfree(h->dmat); dfree(h->drowstart); dfree(h->dnup); dfree(h->dperm);
     dfree(h->dpairs); dfree(h->ddiag_first); dfree(h->delem_err); dfree(h->derr); dfree(h->dhpart);
     dfree(h->dscratch); dfree(h->dblknode); dfree(h->dblkstart); dfree(h->dcptr); dfree(h->dcword);
-    dfree(h->dptab); dfree(h->dpinfo); dfree(h->dpslot); dfree(h->dfptr); dfree(h->dfword); dfree(h->dpdiag);
+    dfree(h->dptab); dfree(h->dpinfo); dfree(h->dpslot); dfree(h->dfptr); dfree(h->dfword); dfree(h->dpinfo2);
+    dfree(h->dpkslot); dfree(h->dfree_nodes);
     dfree(h->dup_ptr); dfree(h->dup_nodes); dfree(h->dslot_nbr); dfree(h->dslot_src); dfree(h->dsell_nbr);
     dfree(h->ddof_mask); dfree(h->dnode_mask); dfree(h->dR); dfree(h->dpart); dfree(h->dpart2);
     dfree(h->dnorm); dfree(h->dFext); dfree(h->dUbar);
@@ -388,15 +394,19 @@ static void patch_order(const ceos_handle* h, std::vector<int>& elems) {
     for (size_t i = 0; i < n; ++i) elems[i] = key[i].second;
 }
 
-// Tables of the patch assembly (element_kernel.hpp, assembly_kernel.hpp): for every patch (kEPB
-// consecutive slots of a launch range) the distinct node blocks its pieces contribute to (the
-// diagonal blocks first, each group in block order), their contributions in slot order and the
-// staging place of every piece; the contribution map of the gather over patch records (patch
-// order inside every block); the map of the residual-only pass over the patches' Fe records (the
-// same sums, so both passes give the same Fint bit for bit).
+// Tables of the patch assembly (element_kernel.hpp, assembly_kernel.hpp).  For every patch (kEPB
+// consecutive slots of a launch range): the distinct node blocks its pieces contribute to, each a
+// record summed over its contributions in slot order.  A record is complete when no other patch
+// contributes to its block: the element kernel writes it straight into K (and a diagonal one into
+// Fint).  The other, boundary records go to the record buffer and through the gather.  Record
+// order inside a patch: boundary diagonal, boundary off-diagonal, complete diagonal, complete
+// off-diagonal, block order inside each group.  Also: the staging place of every piece, the
+// contribution map of the gather over boundary records (patch order inside every block), and the
+// map of the residual-only pass over the Fe parts of boundary diagonal records (the same sums, so
+// both passes give the same Fint bit for bit).
 static int build_gather_map(ceos_handle* h) {
     const int nn = h->nn, ne = h->ne;
-    if ((long long)ne * 9 * kPieces >= (1LL << 31)) return CEOS_ERR_ARG;
+    if ((long long)ne * 9 * kPieces >= (1LL << 31) || nn >= (1 << 24)) return CEOS_ERR_ARG;
     std::vector<int> blkstart(nn);
     long long nb = 0;
     for (int A = 0; A < nn; ++A) {
@@ -416,24 +426,12 @@ static int build_gather_map(ceos_handle* h) {
     }
     const int np = (int)pstart.size();
     pstart.push_back(ne);
-    std::vector<uint16_t> ptab((size_t)ne * kPieces);
-    std::vector<int4> pinfo(np);
-    std::vector<int2> pdiag(np);
-    std::vector<int> fe_node;  // node of every Fe record (diagonal records, patch order)
-    fe_node.reserve((size_t)ne * 4);
-    std::vector<uint16_t> pslot;
-    std::vector<int> rec_blk;  // node block of every patch record
-    std::vector<int> rec_patch;
-    pslot.reserve((size_t)ne * 24);
-    rec_blk.reserve((size_t)ne * 24);
-    rec_patch.reserve((size_t)ne * 24);
     struct Piece {
         int blk, slot, q;
         bool tr;
     };
     std::vector<Piece> pc;
-    long long rec_total = 0;
-    for (int p = 0; p < np; ++p) {
+    auto pieces_of = [&](int p) -> int {
         pc.clear();
         for (int sl = pstart[p]; sl < pstart[p + 1]; ++sl) {
             const int* cn = &h->conn0[(size_t)8 * h->perm[sl]];
@@ -450,31 +448,57 @@ static int build_gather_map(ceos_handle* h) {
                 }
             }
         }
-        std::stable_sort(pc.begin(), pc.end(), [](const Piece& a, const Piece& b) {
+        std::sort(pc.begin(), pc.end(), [](const Piece& a, const Piece& b) {
             return a.blk != b.blk ? a.blk < b.blk : a.slot < b.slot;
         });
-        // records = runs of equal blocks, the diagonal ones first (stable: block order inside
-        // each group, slot order inside each run); the halves split the pieces about evenly
-        std::stable_partition(pc.begin(), pc.end(), [&](const Piece& a) { return a.blk == blkstart[blknode[a.blk]]; });
-        std::vector<int> run;  // first piece of every record
-        int ndiag = 0;
+        return CEOS_OK;
+    };
+    // pass 1: how many patches contribute to every block
+    std::vector<uint8_t> npatch(nb, 0);
+    for (int p = 0; p < np; ++p) {
+        if (pieces_of(p)) return CEOS_ERR_SLOT;
         for (size_t i = 0; i < pc.size(); ++i)
-            if (i == 0 || pc[i].blk != pc[i - 1].blk) {
-                run.push_back((int)i);
-                if (pc[i].blk == blkstart[blknode[pc[i].blk]]) {
-                    ++ndiag;
-                    fe_node.push_back(blknode[pc[i].blk]);
-                }
-            }
-        const int nrec = (int)run.size();
-        run.push_back((int)pc.size());
+            if (i == 0 || pc[i].blk != pc[i - 1].blk) npatch[pc[i].blk] = (uint8_t)std::min(255, npatch[pc[i].blk] + 1);
+    }
+    // pass 2: records, staging, tables
+    std::vector<uint16_t> ptab((size_t)ne * kPieces);
+    std::vector<int4> pinfo(np), pinfo2(np);
+    std::vector<uint16_t> pslot;
+    std::vector<int2> pkslot;  // per record: complete records: K slot of row 0, node A << 7 | nup[A]
+    std::vector<int> rec_blk;  // block of every boundary record
+    std::vector<int> rec_patch;
+    std::vector<int> fe_node;  // node of every boundary diagonal record (residual-only Fe records)
+    pslot.reserve((size_t)ne * 24);
+    pkslot.reserve((size_t)ne * 24);
+    long long rec_total = 0;
+    long long ncomplete = 0;
+    for (int p = 0; p < np; ++p) {
+        if (pieces_of(p)) return CEOS_ERR_SLOT;
+        // runs of equal blocks, grouped: boundary diagonal, boundary off-diagonal, complete
+        // diagonal, complete off-diagonal (block order inside each group)
+        std::vector<std::pair<int, int>> runs[4];  // (first piece, count)
+        for (size_t i = 0; i < pc.size();) {
+            size_t j = i;
+            while (j < pc.size() && pc[j].blk == pc[i].blk) ++j;
+            const int blk = pc[i].blk;
+            const bool diag = blk == blkstart[blknode[blk]];
+            const bool complete = npatch[blk] == 1;
+            runs[(complete ? 2 : 0) + (diag ? 0 : 1)].push_back({(int)i, (int)(j - i)});
+            i = j;
+        }
+        std::vector<std::pair<int, int>> rec;
+        for (auto& g : runs) rec.insert(rec.end(), g.begin(), g.end());
+        const int nrec = (int)rec.size();
+        const int nbd = (int)runs[0].size(), nbnd = nbd + (int)runs[1].size(), ncd = (int)runs[2].size();
+        std::vector<int> cum(nrec + 1, 0);
+        for (int r = 0; r < nrec; ++r) cum[r + 1] = cum[r] + rec[r].second;
         int split = nrec;
         for (int r = 0; r < nrec; ++r)
-            if (2 * run[r] >= (int)pc.size()) { split = r; break; }
-        if (run[split] > kStageMax || (int)pc.size() - run[split] > kStageMax) return CEOS_ERR_ARG;
-        if (rec_total * 9 >= (1LL << 31)) return CEOS_ERR_ARG;
+            if (2 * cum[r] >= cum[nrec]) { split = r; break; }
+        if (cum[split] > kStageMax || cum[nrec] - cum[split] > kStageMax) return CEOS_ERR_ARG;
+        if ((rec_total + nbnd) * 9 >= (1LL << 31)) return CEOS_ERR_ARG;
         pinfo[p] = make_int4((int)(rec_total * 9), (int)pslot.size(), nrec, split);
-        pdiag[p] = make_int2((int)(3 * (fe_node.size() - ndiag)), ndiag);
+        pinfo2[p] = make_int4(nbnd, nbd, (int)(3 * fe_node.size()), ncd);
         // staging of a half: the first contribution of record r at r - r0, the further ones of
         // every record after all the firsts, in record order
         int extra = 0;
@@ -482,20 +506,31 @@ static int build_gather_map(ceos_handle* h) {
             const int half = r >= split ? 1 : 0;
             const int r0 = half ? split : 0;
             if (r == r0) extra = (half ? nrec : split) - r0;
-            const int cnt = run[r + 1] - run[r];
+            const int first = rec[r].first, cnt = rec[r].second;
             pslot.push_back((uint16_t)(extra | (cnt << 10)));
-            for (int i = run[r]; i < run[r + 1]; ++i) {
+            for (int i = first; i < first + cnt; ++i) {
                 const Piece& q = pc[i];
-                const int pos = i == run[r] ? r - r0 : extra++;
+                const int pos = i == first ? r - r0 : extra++;
                 ptab[(size_t)q.slot * kPieces + q.q] = (uint16_t)(pos | (half << 10) | ((q.tr ? 1 : 0) << 11));
             }
-            rec_blk.push_back(pc[run[r]].blk);
-            rec_patch.push_back(p);
+            const int blk = pc[first].blk;
+            const int A = blknode[blk];
+            if (r < nbnd) {
+                pkslot.push_back(make_int2(0, 0));
+                rec_blk.push_back(blk);
+                rec_patch.push_back(p);
+                if (r < nbd) fe_node.push_back(A);
+            } else {
+                const int q = blk - blkstart[A];  // 0 = diagonal, else 1 + rank of the neighbour
+                const long long row0 = (long long)h->rowstart[A] + (q ? 3 + 3LL * (q - 1) : 0);
+                pkslot.push_back(make_int2((int)row0, (A << 7) | h->nup[A]));
+                ++ncomplete;
+            }
         }
-        rec_total += nrec;
+        rec_total += nbnd;
     }
     h->rec_doubles = rec_total * 9;
-    // contribution map: block -> patch records (patch order)
+    // contribution map: block -> boundary records (patch order); complete blocks have none
     std::vector<int> cptr(nb + 1, 0);
     for (int b : rec_blk) cptr[b + 1]++;
     for (long long b = 0; b < nb; ++b) cptr[b + 1] += cptr[b];
@@ -508,7 +543,8 @@ static int build_gather_map(ceos_handle* h) {
         cword[fill[b]++] = (uint32_t)(9 * i + (diag ? 3 : 0));
         h->blk_last_slot[b] = std::max(h->blk_last_slot[b], pstart[p + 1] - 1);
     }
-    // residual-only pass: node -> Fe records (3 doubles each), patch order
+    // residual-only pass: node -> Fe records of boundary diagonal records (3 doubles each),
+    // patch order; a node whose diagonal block is complete gets its Fint from the element kernel
     h->fe_doubles = 3LL * (long long)fe_node.size();
     std::vector<int> fptr(nn + 1, 0);
     for (int A : fe_node) fptr[A + 1]++;
@@ -524,20 +560,31 @@ static int build_gather_map(ceos_handle* h) {
     HIP_OK(dalloc_fit(h->dcword, h->cword_cap, cword.size()));
     HIP_OK(dalloc_once(h->dptab, ptab.size()));
     HIP_OK(dalloc_fit(h->dpinfo, h->pinfo_cap, pinfo.size()));
+    HIP_OK(dalloc_fit(h->dpinfo2, h->pinfo2_cap, pinfo2.size()));
     HIP_OK(dalloc_fit(h->dpslot, h->pslot_cap, pslot.size()));
+    HIP_OK(dalloc_fit(h->dpkslot, h->pkslot_cap, pkslot.size()));
     HIP_OK(dalloc_once(h->dfptr, nn + 1));
     HIP_OK(dalloc_fit(h->dfword, h->fword_cap, fword.size()));
-    HIP_OK(dalloc_fit(h->dpdiag, h->pdiag_cap, pdiag.size()));
-    HIP_OK(hipMemcpy(h->dpdiag, pdiag.data(), sizeof(int2) * pdiag.size(), hipMemcpyHostToDevice));
     HIP_OK(hipMemcpy(h->dblkstart, blkstart.data(), sizeof(int) * nn, hipMemcpyHostToDevice));
     HIP_OK(hipMemcpy(h->dblknode, blknode.data(), sizeof(int) * nb, hipMemcpyHostToDevice));
     HIP_OK(hipMemcpy(h->dcptr, cptr.data(), sizeof(int) * (nb + 1), hipMemcpyHostToDevice));
     HIP_OK(hipMemcpy(h->dcword, cword.data(), sizeof(uint32_t) * cword.size(), hipMemcpyHostToDevice));
     HIP_OK(hipMemcpy(h->dptab, ptab.data(), sizeof(uint16_t) * ptab.size(), hipMemcpyHostToDevice));
     HIP_OK(hipMemcpy(h->dpinfo, pinfo.data(), sizeof(int4) * pinfo.size(), hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(h->dpinfo2, pinfo2.data(), sizeof(int4) * pinfo2.size(), hipMemcpyHostToDevice));
     HIP_OK(hipMemcpy(h->dpslot, pslot.data(), sizeof(uint16_t) * pslot.size(), hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(h->dpkslot, pkslot.data(), sizeof(int2) * pkslot.size(), hipMemcpyHostToDevice));
     HIP_OK(hipMemcpy(h->dfptr, fptr.data(), sizeof(int) * (nn + 1), hipMemcpyHostToDevice));
     HIP_OK(hipMemcpy(h->dfword, fword.data(), sizeof(uint32_t) * fword.size(), hipMemcpyHostToDevice));
+    h->n_complete_records = ncomplete;
+    std::vector<int> freen;
+    for (int A = 0; A < nn; ++A)
+        if (npatch[blkstart[A]] == 0) freen.push_back(A);
+    h->n_free_nodes = (int)freen.size();
+    if (!freen.empty()) {
+        HIP_OK(dalloc(h->dfree_nodes, freen.size()));
+        HIP_OK(hipMemcpy(h->dfree_nodes, freen.data(), sizeof(int) * freen.size(), hipMemcpyHostToDevice));
+    }
     return CEOS_OK;
 }
 
@@ -891,7 +938,8 @@ static int evaluate(ceos_handle* h, const double* dU, double* dFint, void* dK, h
     p.ptab = h->dptab;
     p.pinfo = h->dpinfo;
     p.pslot = h->dpslot;
-    p.pdiag = h->dpdiag;
+    p.pinfo2 = h->dpinfo2;
+    p.pkslot = h->dpkslot;
     hipEvent_t* ev = (h->timing_on && h->timing_n < h->timing_cap) ? &h->tev[3 * h->timing_n] : nullptr;
     if (ev) HIP_OK(hipEventRecord(ev[0], s));
     const bool wk = dK != nullptr;
@@ -925,7 +973,10 @@ static int evaluate(ceos_handle* h, const double* dU, double* dFint, void* dK, h
                 q.begin = (int)std::max<long long>(r.begin, h->pipe_slot[k]);
                 q.end = (int)std::min<long long>(r.end, h->pipe_slot[k + 1]);
                 q.patch0 = r.patch0 + (q.begin - r.begin) / kEPB;  // chunks start at patch starts
-                if (q.end > q.begin) launch_range<double, true>(p, q, h->nl != 0, wk, numt, s);
+                if (q.end > q.begin) {
+                    if (f32) launch_range<float, true>(p, q, h->nl != 0, wk, numt, s);
+                    else launch_range<double, true>(p, q, h->nl != 0, wk, numt, s);
+                }
             }
             HIP_OK(hipEventRecord(h->pipe_ev[k], s));
             HIP_OK(hipStreamWaitEvent(s2, h->pipe_ev[k], 0));
@@ -940,6 +991,13 @@ static int evaluate(ceos_handle* h, const double* dU, double* dFint, void* dK, h
         if (ev) HIP_OK(hipEventRecord(ev[1], s));
         HIP_OK(hipEventRecord(h->pipe_ev[S], s2));
         HIP_OK(hipStreamWaitEvent(s, h->pipe_ev[S], 0));
+        if (h->n_free_nodes) {
+            const unsigned nbk = (unsigned)((h->n_free_nodes + 255) / 256);
+            if (f32) hipLaunchKernelGGL(k_zero_free_nodes<float>, dim3(nbk), dim3(256), 0, s, h->n_free_nodes,
+                                        h->dfree_nodes, h->drowstart, h->dnup, dFint, dK);
+            else hipLaunchKernelGGL(k_zero_free_nodes<double>, dim3(nbk), dim3(256), 0, s, h->n_free_nodes,
+                                    h->dfree_nodes, h->drowstart, h->dnup, dFint, dK);
+        }
         if (ev) {
             HIP_OK(hipEventRecord(ev[2], s));
             h->timing_n++;
@@ -950,7 +1008,9 @@ static int evaluate(ceos_handle* h, const double* dU, double* dFint, void* dK, h
     }
     for (const auto& r : h->ranges) {
         if (gather) {
-            launch_range<double, true>(p, r, h->nl != 0, wk, numt, s);  // records are FP64 either way
+            // records are FP64 either way; KT is the type of the complete blocks written to K
+            if (f32) launch_range<float, true>(p, r, h->nl != 0, wk, numt, s);
+            else launch_range<double, true>(p, r, h->nl != 0, wk, numt, s);
         } else {
             if (f32) launch_range<float, false>(p, r, h->nl != 0, wk, numt, s);
             else launch_range<double, false>(p, r, h->nl != 0, wk, numt, s);
@@ -964,6 +1024,13 @@ static int evaluate(ceos_handle* h, const double* dU, double* dFint, void* dK, h
             else hipLaunchKernelGGL(k_gather_blocks<double>, dim3(nbk), dim3(256), 0, s, g);
         } else {
             hipLaunchKernelGGL(k_gather_fint, dim3((h->nn + 255) / 256), dim3(256), 0, s, g);
+        }
+        if (h->n_free_nodes) {
+            const unsigned nbk = (unsigned)((h->n_free_nodes + 255) / 256);
+            if (f32) hipLaunchKernelGGL(k_zero_free_nodes<float>, dim3(nbk), dim3(256), 0, s, h->n_free_nodes,
+                                        h->dfree_nodes, h->drowstart, h->dnup, dFint, dK);
+            else hipLaunchKernelGGL(k_zero_free_nodes<double>, dim3(nbk), dim3(256), 0, s, h->n_free_nodes,
+                                    h->dfree_nodes, h->drowstart, h->dnup, dFint, dK);
         }
     }
     if (ev) {

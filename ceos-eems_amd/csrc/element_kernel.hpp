@@ -64,9 +64,15 @@ constexpr int kStateStride = 20;      // state rows per Gauss point in the devic
 //                   records, first record of the second half};
 //   pslot[]         per record: bits 0..9 staging position of its second contribution, bits
 //                   10..13 contributions;
-//   pdiag[patch]    {offset of its Fe records (residual-only pass), number of diagonal records}.
-// A patch's diagonal records come first; the residual-only pass stages only the Fe parts and
-// writes one 3-double Fe record per diagonal record, summed in the same order as the F+K pass.
+//   pinfo2[patch]   {boundary records, boundary diagonal records, offset of the patch's Fe
+//                   records (residual-only pass), complete diagonal records};
+//   pkslot[]        per complete record: {K slot of its row 0, node A << 7 | nup[A]}.
+// A record is complete when no other patch contributes to its node block: the workgroup writes
+// it straight into K (a diagonal one also into Fint).  The boundary records, which the gather
+// sums over patches, come first in a patch (diagonal ones first), then the complete ones
+// (diagonal ones first).  The residual-only pass stages only the Fe parts: complete diagonal
+// records go to Fint, boundary diagonal ones to 3-double Fe records, summed in the same order as
+// in the F+K pass.
 // The staging area of one half holds at most kStageMax pieces.
 constexpr int kPieces = 36;
 constexpr int kStageMax = (kEPB * kLdsPerElem) / 9;  // 526 pieces of 9 doubles in the LDS slices
@@ -92,7 +98,8 @@ struct ElemParams {
     const uint16_t* ptab;          // [ne][36] staging entries of the pieces (gather assembly)
     const int4* pinfo;             // [patches] record offset, slot-table offset, records, split
     const uint16_t* pslot;         // per patch record: position of its 2nd contribution | contributions << 10
-    const int2* pdiag;             // [patches] residual-only pass: offset of its Fe records, count
+    const int4* pinfo2;            // [patches] boundary / boundary diagonal records, Fe offset, complete diagonal
+    const int2* pkslot;            // [patch records] complete records: K slot of row 0, node << 7 | nup
     const double* st_old;          // [nstate][8][ne] committed state
     double* st_new;                // [nstate][8][ne] state of the last update
     uint8_t* elem_err;             // [ne]: bit 0 detJ <= 1e-13, bit 1 local Newton failed
@@ -337,7 +344,7 @@ __global__ __launch_bounds__(128) void k_element(ElemParams p, int e_begin, int 
     if constexpr (!GATHER) snode[le][lane] = node;
     // patch header and this lane's staging entries, requested with the node data
     int4 pi = make_int4(0, 0, 0, 0);
-    int2 pd = make_int2(0, 0);
+    int4 pi2 = make_int4(0, 0, 0, 0);
     unsigned ptv[5] = {0u, 0u, 0u, 0u, 0u};
     {
         double v[9];
@@ -349,7 +356,7 @@ __global__ __launch_bounds__(128) void k_element(ElemParams p, int e_begin, int 
         }
         if constexpr (GATHER) {
             pi = p.pinfo[patch_begin + blockIdx.x];
-            if constexpr (!WANT_K) pd = p.pdiag[patch_begin + blockIdx.x];
+            pi2 = p.pinfo2[patch_begin + blockIdx.x];
             const uint16_t* tab = p.ptab + (long)(active ? e : e_begin) * kPieces + piece_base(lane);
 #pragma unroll
             for (int k = 0; k < 5; ++k) ptv[k] = (WANT_K || k == 0) && (k < 4 || lane < 4) ? tab[k] : 0u;
@@ -483,11 +490,13 @@ __global__ __launch_bounds__(128) void k_element(ElemParams p, int e_begin, int 
     // 36 kEPB = 576), requested before the block computation that hides their latency
     constexpr int kRecPerThread = (kPieces * kEPB + kEPB * 8 - 1) / (kEPB * 8);
     unsigned psw[kRecPerThread];
+    int2 pks[kRecPerThread];
     if constexpr (GATHER) {
 #pragma unroll
         for (int i = 0; i < kRecPerThread; ++i) {
             const int r = (int)threadIdx.x + kEPB * 8 * i;
             psw[i] = r < pi.z ? p.pslot[pi.y + r] : 0u;
+            pks[i] = (r < pi.z && r >= pi2.x) ? p.pkslot[pi.y + r] : make_int2(0, 0);
         }
     }
     const int b = lane;
@@ -544,11 +553,12 @@ __global__ __launch_bounds__(128) void k_element(ElemParams p, int e_begin, int 
                 dst[2] = fe2;
             }
             __syncthreads();
-            const int r0 = half ? pi.w : 0, r1 = min(half ? pi.z : pi.w, pd.y);
+            const int r0 = half ? pi.w : 0, r1 = half ? pi.z : pi.w;
+            const int nbd = pi2.y, cd0 = pi2.x, cd1 = pi2.x + pi2.w;  // diagonal records
 #pragma unroll
             for (int i = 0; i < kRecPerThread; ++i) {
                 const int r = (int)threadIdx.x + kEPB * 8 * i;
-                if (r < r0 || r >= r1) continue;
+                if (r < r0 || r >= r1 || !(r < nbd || (r >= cd0 && r < cd1))) continue;
                 const unsigned w = psw[i];
                 double* dst = smem + 9 * (r - r0);
                 const double* ext = smem + 9 * (w & 1023u);
@@ -562,14 +572,22 @@ __global__ __launch_bounds__(128) void k_element(ElemParams p, int e_begin, int 
                         a2 += ext[9 * (c - 1) + 2];
                     }
                 }
-                dst[0] = a0;
-                dst[1] = a1;
-                dst[2] = a2;
+                if (r >= cd0) {
+                    // complete: the node's whole Fint
+                    const int A = pks[i].y >> 7;
+                    p.Fint[3L * A] = a0;
+                    p.Fint[3L * A + 1] = a1;
+                    p.Fint[3L * A + 2] = a2;
+                } else {
+                    dst[0] = a0;
+                    dst[1] = a1;
+                    dst[2] = a2;
+                }
             }
             __syncthreads();
-            const int n3 = 3 * max(r1 - r0, 0);
+            const int n3 = 3 * max(min(r1, nbd) - r0, 0);
             for (int o = (int)threadIdx.x; o < n3; o += kEPB * 8)
-                p.scratch[(long)pd.x + 3L * r0 + o] = smem[9 * (o / 3) + o % 3];
+                p.scratch[(long)pi2.z + 3L * r0 + o] = smem[9 * (o / 3) + o % 3];
         }
         return;
     }
@@ -633,11 +651,37 @@ __global__ __launch_bounds__(128) void k_element(ElemParams p, int e_begin, int 
                 for (int q = 0; q < 9; ++q) dst[q] = acc[q];
             }
             __syncthreads();
-            // non-temporal, unit stride across the workgroup: the records are read back once, by
-            // the gather, long after this workgroup ends, so they need not hold L2 lines
+            // boundary records: non-temporal, unit stride across the workgroup (they are read back
+            // once, by the gather, long after this workgroup ends, so they need not hold L2 lines)
+            const int nbnd = pi2.x;
             double* out = p.scratch + (long)pi.x + 9L * r0;
-            const int n9 = 9 * (r1 - r0);
+            const int n9 = 9 * max(min(r1, nbnd) - r0, 0);
             for (int o = (int)threadIdx.x; o < n9; o += kEPB * 8) __builtin_nontemporal_store(smem[o], out + o);
+            // complete records: straight into K (rows 3A..3A+2 of the CSR block) and Fint,
+            // non-temporal like the records (the gather that follows then keeps L2 for its reads)
+            KT* Kg = (KT*)p.K;
+#pragma unroll
+            for (int i = 0; i < kRecPerThread; ++i) {
+                const int r = (int)threadIdx.x + kEPB * 8 * i;
+                if (r < r0 || r >= r1 || r < nbnd) continue;
+                const double* v = smem + 9 * (r - r0);
+                const long row0 = pks[i].x;
+                const int A = pks[i].y >> 7, nu = pks[i].y & 127;
+                if (r < nbnd + pi2.w) {
+                    __builtin_nontemporal_store(v[0], p.Fint + (3L * A));
+                    __builtin_nontemporal_store(v[1], p.Fint + (3L * A + 1));
+                    __builtin_nontemporal_store(v[2], p.Fint + (3L * A + 2));
+                    const long row1 = row0 + 3 + 3 * nu, row2 = row1 + 2 + 3 * nu;
+                    __builtin_nontemporal_store((KT)v[3], Kg + (row0)); __builtin_nontemporal_store((KT)v[4], Kg + (row0 + 1)); __builtin_nontemporal_store((KT)v[5], Kg + (row0 + 2));
+                    __builtin_nontemporal_store((KT)v[6], Kg + (row1)); __builtin_nontemporal_store((KT)v[7], Kg + (row1 + 1));
+                    __builtin_nontemporal_store((KT)v[8], Kg + (row2));
+                } else {
+                    const long row1 = row0 + 2 + 3 * nu, row2 = row1 + 1 + 3 * nu;
+                    __builtin_nontemporal_store((KT)v[0], Kg + (row0)); __builtin_nontemporal_store((KT)v[1], Kg + (row0 + 1)); __builtin_nontemporal_store((KT)v[2], Kg + (row0 + 2));
+                    __builtin_nontemporal_store((KT)v[3], Kg + (row1)); __builtin_nontemporal_store((KT)v[4], Kg + (row1 + 1)); __builtin_nontemporal_store((KT)v[5], Kg + (row1 + 2));
+                    __builtin_nontemporal_store((KT)v[6], Kg + (row2)); __builtin_nontemporal_store((KT)v[7], Kg + (row2 + 1)); __builtin_nontemporal_store((KT)v[8], Kg + (row2 + 2));
+                }
+            }
         }
         return;
     }
