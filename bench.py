@@ -41,10 +41,11 @@ LAM = K_BULK - 2.0 * MU / 3.0
 # algorithmic work per element and evaluation (structure-exploiting count, FP64 tangent):
 FLOP_PER_ELEM = 37.5e3
 BYTES_PER_ELEM = 1103.0
-# gather assembly: per element one 324-double record and 36 contribution words; per node block
-# its cptr and blknode words; per CSR entry the stored value
-RECORD_BYTES = 324 * 8
-CWORD_BYTES = 36 * 4
+# gather assembly (patch records): per boundary record 9 doubles and one contribution word; per
+# node block its cptr word; per block the gather writes its blknode word and its CSR values
+# (9 off the diagonal, 6 plus 3 Fint doubles on it)
+RECORD_BYTES = 9 * 8
+CWORD_BYTES = 4
 PEAK_FP64_TFLOPS = 78.6
 PEAK_HBM_GBS = 8000.0
 PROFILE_PMC = os.path.join(REPO, "profiles", "element_kernel_c2_pmc.txt")
@@ -911,8 +912,11 @@ def main():
                 "flop_per_elem": FLOP_PER_ELEM, "elems_per_launch": elems_per_launch}
     # assembly kernel (gather of the element records into the CSR rows): HBM-bound
     nn = len(X)
-    nblocks = nn + (nnz - 6 * nn) // 9
-    asm_bytes = (RECORD_BYTES + CWORD_BYTES) * ne + 8 * nblocks + KBYTES * nnz + 24 * nn
+    ai = mesh.assembly_info()
+    g_off = ai["gather_blocks"] - ai["gather_diag_blocks"]
+    asm_bytes = ((RECORD_BYTES + CWORD_BYTES) * ai["boundary_records"] + 4 * (ai["node_blocks"] + 1)
+                 + 4 * ai["gather_blocks"] + KBYTES * (9 * g_off + 6 * ai["gather_diag_blocks"])
+                 + 24 * ai["gather_diag_blocks"])
     asm_launch_s = (asm_ms * 1e-3) / max(n_timed, 1)
     asm_gbs = asm_bytes / asm_launch_s / 1e9 if asm_ms > 0 else None
     # the whole F+K evaluation against the path's algorithmic bytes
@@ -939,7 +943,7 @@ def main():
         "roofline_assembly": None if asm_gbs is None else {
             "bound": "hbm", "kernel": "k_gather_blocks", "achieved": round(asm_gbs, 1), "peak": PEAK_HBM_GBS,
             "unit": "GB/s", "frac": round(asm_gbs / PEAK_HBM_GBS, 4), "avg_launch_ms": round(asm_launch_s * 1e3, 4),
-            "bytes_per_launch": asm_bytes},
+            "bytes_per_launch": asm_bytes, "patch_records": ai},
         "path_hbm": {"achieved": round(path_gbs, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": round(path_gbs / PEAK_HBM_GBS, 4), "algorithmic_bytes_per_elem": BYTES_PER_ELEM},
         "wall_s_timed": wall,

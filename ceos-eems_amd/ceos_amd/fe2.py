@@ -33,6 +33,10 @@ LAM_LO, LAM_HI = 0.8, 1.25     # principal stretches of the training distributio
 U_SCALE = 0.25                 # U features: (U - I) / U_SCALE
 E_REF = 0.1                    # output scaling: S = S_base + (|E|^2 / E_REF^2) (t_mean + t_std y)
 N_IN, WIDTH, N_OUT = 27, 256, 6
+# probe stretches of the probe correction: isochoric extensions along x, y, z (stretch 1.06) and
+# isochoric pure shears in xy, yz, xz (U = exp of 0.04 (e_i e_j + e_j e_i)); PROBE_EP is the strain
+# scale e_p of the correction's monomials
+PROBE_LAM, PROBE_GAMMA, PROBE_EP = 1.06, 0.04, 0.06
 VOIGT = ((0, 0), (1, 1), (2, 2), (0, 1), (1, 2), (0, 2))
 PROPS = np.array([rve.MATRIX_PROPS, rve.INCLUSION_PROPS])
 
@@ -87,6 +91,34 @@ def walk_stretch(rng, U, sig_lam=0.05, sig_rot=0.25, lo=LAM_LO, hi=LAM_HI, sig_l
     R = np.eye(3)[None] + s * K + (1.0 - c) * (K @ K)  # Rodrigues
     Q = R @ Q
     return np.einsum("nij,nj,nkj->nik", Q, lam, Q)
+
+
+def probe_stretches():
+    """The 6 probe stretch tensors (6, 3, 3) of the probe correction."""
+    lam = PROBE_LAM
+    out = [np.diag(np.roll([lam, lam ** -0.5, lam ** -0.5], k)) for k in range(3)]
+    for i, j in ((0, 1), (1, 2), (0, 2)):
+        # exp of the symmetric traceless G = g (e_i e_j + e_j e_i): eigenvalues +-g, 0
+        g = PROBE_GAMMA
+        U = np.eye(3)
+        U[i, i] = U[j, j] = np.cosh(g)
+        U[i, j] = U[j, i] = np.sinh(g)
+        out.append(U)
+    return np.array(out)
+
+
+def correction_monomials(F, e_p=PROBE_EP):
+    """m_q(E) = Ee_a Ee_b / e_p^2 (a <= b, engineering strains), (..., 21)."""
+    E = to_voigt(green_lagrange(F)) * np.array([1, 1, 1, 2, 2, 2.0])
+    return np.stack([E[..., a] * E[..., b] for a in range(6) for b in range(a, 6)], axis=-1) / e_p ** 2
+
+
+def correction(W, R, F, e_p=PROBE_EP):
+    """The probe correction of S (..., 6): sum_{j,q} W[o, j, q] r_j m_q(E), r = [R, 1] (numpy
+    restatement of ceos_surrogate_correct_device's stress part)."""
+    R = np.asarray(R)
+    r = np.concatenate([R, np.ones(R.shape[:-1] + (1,))], axis=-1)
+    return np.einsum("ojq,...j,...q->...o", W, r, correction_monomials(F, e_p))
 
 
 def green_lagrange(F):
@@ -176,6 +208,42 @@ def descriptors(batch, pcg_rtol=1e-8):
     return dext.numpy().reshape(n, gpu.EXT_PER_GP)[:, 6:].copy()
 
 
+def surrogate_stress(so, F, Clin, max_points):
+    """S (Voigt) of the surrogate `so` (gpu.Surrogate, without the probe correction) at F (n, 3, 3)
+    with descriptors Clin (n, 21), evaluated in chunks of max_points on the device."""
+    n = len(F)
+    S = np.zeros((n, 6))
+    m0 = min(n, max_points)
+    dF, dC, dext = hiprt.DeviceArray(9 * m0), hiprt.DeviceArray(21 * m0), hiprt.DeviceArray(gpu.EXT_PER_GP * m0)
+    for a in range(0, n, m0):
+        m = min(m0, n - a)
+        dF.view(0, 9 * m).upload(np.ascontiguousarray(np.asarray(F[a:a + m]).transpose(0, 2, 1).reshape(-1)))
+        dC.view(0, 21 * m).upload(np.ascontiguousarray(Clin[a:a + m]).reshape(-1))
+        so.eval_device(m, dF.ptr, dC.ptr, dext.ptr)
+        S[a:a + m] = dext.view(0, gpu.EXT_PER_GP * m).numpy().reshape(m, gpu.EXT_PER_GP)[:, :6]
+    return S
+
+
+def probe_residuals(batch, so, Clin, max_points, newton_rtol=1e-10, pcg_rtol=1e-10):
+    """The second descriptor of every RVE of `batch`: r = S_RVE(U_p) - S_surrogate(U_p) at the 6
+    probe stretches (n_rve, 36), each probe solved from the undeformed state.  Leaves the batch
+    reset and with the default solver settings."""
+    n = batch.n
+    R = np.zeros((n, 6, 6))
+    batch.set_solver(newton_rtol=newton_rtol, pcg_rtol=pcg_rtol)
+    try:
+        for k, Up in enumerate(probe_stretches()):
+            batch.reset()
+            Ub = np.repeat(Up[None], n, 0)
+            P, _, _ = batch.solve(Ub)
+            S = np.linalg.solve(Ub, P)
+            R[:, k] = to_voigt(0.5 * (S + S.transpose(0, 2, 1))) - surrogate_stress(so, Ub, Clin, max_points)
+    finally:
+        batch.set_solver()
+        batch.reset()
+    return R.reshape(n, 36)
+
+
 def surrogate_cfg(meta):
     return {k: meta[k] for k in ("u_scale", "lam_lo", "lam_hi", "e_ref", "d_mean", "d_std", "d_lo", "d_hi",
                                  "t_mean", "t_std")}
@@ -189,6 +257,8 @@ def load_surrogate(path=SURROGATE_PATH, max_points=4096):
     mlp = gpu.Mlp(N_IN, width, N_OUT, max_batch=max_points, seed=1)
     mlp.set_params(meta["params"].astype(np.float32))
     sur = gpu.Surrogate(mlp, surrogate_cfg(meta), max_points)
+    if "corr_W" in meta:
+        sur.set_correction(meta["corr_W"], float(meta["corr_ep"]))
     return mlp, sur, meta
 
 
@@ -221,6 +291,8 @@ class MacroFE2:
         self.dFint = D(self.ndof)
         self.dK = D(self.nnz)
         self.dClin = D(21 * self.ngp)
+        self.dR = None      # probe residuals [ngp][36] of the probe correction
+        self.probe_s = 0.0
         self.dflags = D(self.ngp, np.int32)
         self.stream = hiprt.Stream()
         self.mesh.set_external(self.dext.ptr)
@@ -234,11 +306,19 @@ class MacroFE2:
         self.dU.upload(np.zeros(self.ndof))
 
     def setup_descriptors(self, pcg_rtol=1e-8):
+        """C_lin of every RVE, and the probe residuals when the surrogate carries a probe
+        correction (their time in self.probe_s); returns the seconds spent on C_lin."""
         t = time.perf_counter()
         self.Clin = descriptors(self.rves, pcg_rtol)
         self.dClin.upload(self.Clin.reshape(-1))
         self.stress_floor = 1e-9 * float(np.abs(self.Clin).max())  # the stress of a 1e-9 strain
-        return time.perf_counter() - t
+        t_clin = time.perf_counter() - t
+        if self.sur is not None and getattr(self.sur[1], "n_probe", 0):
+            t = time.perf_counter()
+            self.R = probe_residuals(self.rves, self.sur[1], self.Clin, self.sur[1].max_points)
+            self.dR = hiprt.DeviceArray.from_numpy(self.R.reshape(-1))
+            self.probe_s = time.perf_counter() - t
+        return t_clin
 
     def _kinematics(self, U):
         self.dU.upload(np.ascontiguousarray(U, dtype=np.float64))
@@ -260,6 +340,8 @@ class MacroFE2:
         self._chain_kinematics()
         self.sur[1].eval_device(self.ngp, self.dF.ptr, self.dClin.ptr, self.dext.ptr, self.dflags.ptr, None,
                                 self.stream.ptr)
+        if self.dR is not None:
+            self.sur[1].correct_device(self.ngp, self.dF.ptr, self.dR.ptr, self.dext.ptr, None, self.stream.ptr)
 
     def _chain_assemble(self):
         self.mesh.evaluate_device(0.0, self.dU.ptr, self.dFint.ptr, self.dK.ptr, self.stream.ptr)
@@ -322,15 +404,18 @@ class MacroFE2:
         self.judge_ids = None
 
     def evaluate_surrogate(self, U, judge_frac=0.01, judge_tol=0.01, fallback_tangent="surrogate",
-                           judge_newton_rtol=1e-3, use_graph=False):
-        """Surrogate macro evaluation: the MLP at points inside the training hull; outside it the
+                           judge_newton_rtol=1e-3, use_graph=False, judge_gate="point"):
+        """Surrogate macro evaluation: the MLP (plus the probe correction when the surrogate carries
+        one) at points inside the training hull; outside it the
         RVE solve gives the stress (with fallback_tangent="surrogate" the point keeps the
         surrogate's tangent, an inexact macro Newton tangent; "rve" condenses the RVE tangent too);
-        ceil(judge_frac ngp) sampled inside points are re-solved as judges.  If the judged
-        stresses differ from the surrogate's by more than judge_tol norm-wise (the RMS error of the
-        judged points over the RMS stress of all points, tensor norms: a sample estimate of
-        |S_nn - S_rve| / |S_rve| over all points) the whole evaluation falls back to the plain
-        path.  Judged RVEs are converged to judge_newton_rtol (relative Newton test): 1e-3 moves
+        ceil(judge_frac ngp) sampled inside points are re-solved as judges.  The judges gate the
+        evaluation: with judge_gate="point" (the per-point bar, max over points of
+        |S_nn - S_rve| / |S_rve| <= judge_tol) any judged point above judge_tol, with "aggregate"
+        the norm-wise estimate above judge_tol (the RMS error of the judged points over the RMS
+        stress of all points, tensor norms), makes the whole evaluation fall back to the plain
+        path.  judge_err reports the norm-wise estimate, judge_point_max the largest judged error.
+        Judged RVEs are converged to judge_newton_rtol (relative Newton test): 1e-3 moves
         the estimate by about 1e-5 against fully converged judges (the Newton iterates converge
         quadratically, so the last one is far more accurate than the test), three orders below the
         1 % judge tolerance."""
@@ -383,7 +468,11 @@ class MacroFE2:
         stats = {"mode": "surrogate", "outside_hull": int(len(out)), "judged": int(len(err)),
                  "judge_err": agg, "judge_point_max": float(err.max()) if len(err) else 0.0,
                  "judge_point_mean": float(err.mean()) if len(err) else 0.0, "fallback": False}
-        if len(err) and agg > judge_tol:
+        if judge_gate not in ("point", "aggregate"):
+            raise ValueError("judge_gate must be 'point' or 'aggregate'")
+        gate = (err.max() if len(err) else 0.0) if judge_gate == "point" else agg
+        stats["judge_pass"] = bool(gate <= judge_tol)
+        if len(err) and gate > judge_tol:
             r = self.evaluate_rve(U, use_graph)
             stats["fallback"] = True
             stats["rve_solves"] = r["rve_solves"] + int(len(out)) + len(err)

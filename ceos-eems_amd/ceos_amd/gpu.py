@@ -70,7 +70,9 @@ EXPORTS = [
     "ceos_set_pipeline", "ceos_rve_autotune", "ceos_rve_set_cg_group",
     "ceos_bc_set", "ceos_residual_bc", "ceos_residual_bc_device", "ceos_set_solver_precision",
     "ceos_linear_solve", "ceos_linear_solve_device", "ceos_newton_update_device", "ceos_set_linear_solver",
-    "ceos_mesh_prepare", "ceos_evaluate_device_static",
+    "ceos_mesh_prepare", "ceos_evaluate_device_static", "ceos_assembly_info",
+    "ceos_rve_state_save", "ceos_rve_state_load", "ceos_surrogate_set_correction",
+    "ceos_surrogate_correct_device",
 ]
 
 
@@ -118,6 +120,11 @@ def load_library(path=LIB_PATH):
     L.ceos_update_mesh_device.argtypes = [vp, vp, vp]
     L.ceos_device_status.argtypes = [vp, C.POINTER(CeosStatus)]
     L.ceos_mesh_info.argtypes = [vp, C.POINTER(i), C.POINTER(C.c_longlong)]
+    L.ceos_assembly_info.argtypes = [vp, C.POINTER(C.c_longlong)]
+    L.ceos_rve_state_save.argtypes = [vp, C.c_char_p]
+    L.ceos_rve_state_load.argtypes = [vp, C.c_char_p]
+    L.ceos_surrogate_set_correction.argtypes = [vp, i, _dp, d]
+    L.ceos_surrogate_correct_device.argtypes = [vp, i, vp, vp, vp, vp, vp]
     L.ceos_set_schedule.argtypes = [vp, i]
     L.ceos_set_pipeline.argtypes = [vp, i]
     L.ceos_set_assembly.argtypes = [vp, i]
@@ -252,6 +259,14 @@ class CeosMesh:
         nc, nnz = C.c_int(), C.c_longlong()
         _check(load_library().ceos_mesh_info(self._h, C.byref(nc), C.byref(nnz)), "ceos_mesh_info")
         return nc.value, nnz.value
+
+    def assembly_info(self):
+        """ceos_assembly_info of a prepared handle: patch and record counts of the gather assembly."""
+        v = (C.c_longlong * 8)()
+        _check(load_library().ceos_assembly_info(self._h, v), "ceos_assembly_info")
+        keys = ("patches", "boundary_records", "boundary_diag_records", "complete_records",
+                "complete_diag_records", "node_blocks", "gather_blocks", "gather_diag_blocks")
+        return dict(zip(keys, (int(x) for x in v)))
 
     def csr(self, upper=True):
         if upper and self._csr is not None:
@@ -520,6 +535,14 @@ class RveBatch:
         _check(load_library().ceos_rve_profile(self._h, t.ctypes.data), "ceos_rve_profile")
         return t / 1.0e8
 
+    def state_save(self, path):
+        """ceos_rve_state_save: F_conv, fluctuations and the last P, dP/dF of every RVE to a file."""
+        _check(load_library().ceos_rve_state_save(self._h, os.fsencode(path)), "ceos_rve_state_save")
+
+    def state_load(self, path):
+        """ceos_rve_state_load: the state of ceos_rve_state_save, into a batch of the same RVEs."""
+        _check(load_library().ceos_rve_state_load(self._h, os.fsencode(path)), "ceos_rve_state_load")
+
     def fluctuation(self, r):
         n = int(self.sides[r])
         w = np.zeros(3 * n ** 3)
@@ -630,3 +653,23 @@ class Surrogate:
     def eval_device(self, n, dF, dClin, dext, dflags=None, dP=None, stream=None):
         _check(load_library().ceos_surrogate_eval_device(self._h, n, dF, dClin, dext, dflags, dP, stream),
                "ceos_surrogate_eval_device")
+
+    def set_correction(self, W, e_p):
+        """ceos_surrogate_set_correction: probe-correction weights W (6, 6 n_probe + 1, 21) and the
+        probe strain scale e_p; W None removes the correction."""
+        if W is None:
+            _check(load_library().ceos_surrogate_set_correction(self._h, 0, np.zeros(1), 1.0),
+                   "ceos_surrogate_set_correction")
+            self.n_probe = 0
+            return
+        W = np.ascontiguousarray(W, dtype=np.float64)
+        n_probe = (W.shape[1] - 1) // 6
+        assert W.shape == (6, 6 * n_probe + 1, 21), W.shape
+        _check(load_library().ceos_surrogate_set_correction(self._h, n_probe, W.reshape(-1), float(e_p)),
+               "ceos_surrogate_set_correction")
+        self.n_probe = n_probe
+
+    def correct_device(self, n, dF, dR, dext, dP=None, stream=None):
+        """ceos_surrogate_correct_device: add the probe correction to n records of eval_device."""
+        _check(load_library().ceos_surrogate_correct_device(self._h, n, dF, dR, dext, dP, stream),
+               "ceos_surrogate_correct_device")

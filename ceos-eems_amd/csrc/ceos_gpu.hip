@@ -204,6 +204,10 @@ struct ceos_handle {
                                  // complete diagonal records
     int2* dpkslot = nullptr;     // [patch records] complete records: K slot of row 0, node << 7 | nup
     long long n_complete_records = 0;
+    long long n_complete_diag = 0;     // complete diagonal records (node's whole Fint in the element kernel)
+    long long n_boundary_diag = 0;     // boundary diagonal records
+    long long n_gather_blocks = 0;     // node blocks with at least one boundary record
+    long long n_gather_diag = 0;       // of which diagonal
     int n_free_nodes = 0;        // nodes in no element (their Fint and diagonal K are zeroed)
     int* dfree_nodes = nullptr;
     long long fe_doubles = 0;    // doubles of the Fe records of one residual-only evaluation
@@ -472,6 +476,7 @@ static int build_gather_map(ceos_handle* h) {
     pkslot.reserve((size_t)ne * 24);
     long long rec_total = 0;
     long long ncomplete = 0;
+    h->n_complete_diag = 0;
     for (int p = 0; p < np; ++p) {
         if (pieces_of(p)) return CEOS_ERR_SLOT;
         // runs of equal blocks, grouped: boundary diagonal, boundary off-diagonal, complete
@@ -525,6 +530,7 @@ static int build_gather_map(ceos_handle* h) {
                 const long long row0 = (long long)h->rowstart[A] + (q ? 3 + 3LL * (q - 1) : 0);
                 pkslot.push_back(make_int2((int)row0, (A << 7) | h->nup[A]));
                 ++ncomplete;
+                if (q == 0) ++h->n_complete_diag;
             }
         }
         rec_total += nbnd;
@@ -533,6 +539,12 @@ static int build_gather_map(ceos_handle* h) {
     // contribution map: block -> boundary records (patch order); complete blocks have none
     std::vector<int> cptr(nb + 1, 0);
     for (int b : rec_blk) cptr[b + 1]++;
+    h->n_gather_blocks = h->n_gather_diag = 0;
+    for (long long b = 0; b < nb; ++b)
+        if (cptr[b + 1]) {
+            ++h->n_gather_blocks;
+            if (b == blkstart[blknode[b]]) ++h->n_gather_diag;
+        }
     for (long long b = 0; b < nb; ++b) cptr[b + 1] += cptr[b];
     std::vector<uint32_t> cword(rec_total);
     std::vector<int> fill(cptr.begin(), cptr.end() - 1);
@@ -546,6 +558,7 @@ static int build_gather_map(ceos_handle* h) {
     // residual-only pass: node -> Fe records of boundary diagonal records (3 doubles each),
     // patch order; a node whose diagonal block is complete gets its Fint from the element kernel
     h->fe_doubles = 3LL * (long long)fe_node.size();
+    h->n_boundary_diag = (long long)fe_node.size();
     std::vector<int> fptr(nn + 1, 0);
     for (int A : fe_node) fptr[A + 1]++;
     for (int A = 0; A < nn; ++A) fptr[A + 1] += fptr[A];
@@ -1536,6 +1549,22 @@ int ceos_mesh_info(ceos_handle* h, int* n_colors, long long* nnz_upper) {
     if (!h) return CEOS_ERR_ARG;
     if (n_colors) *n_colors = h->ncolor;
     if (nnz_upper) *nnz_upper = h->nnz;
+    return CEOS_OK;
+}
+
+int ceos_assembly_info(ceos_handle* h, long long info[8]) {
+    if (!h || !info) return CEOS_ERR_ARG;
+    if (!h->finalized) return CEOS_ERR_ARG;
+    long long patches = 0;
+    for (auto& r : h->ranges) patches += (r.end - r.begin + kEPB - 1) / kEPB;
+    info[0] = patches;
+    info[1] = h->rec_doubles / 9;       // boundary records (one contribution word each)
+    info[2] = h->n_boundary_diag;
+    info[3] = h->n_complete_records;
+    info[4] = h->n_complete_diag;
+    info[5] = h->nblocks;
+    info[6] = h->n_gather_blocks;
+    info[7] = h->n_gather_diag;
     return CEOS_OK;
 }
 

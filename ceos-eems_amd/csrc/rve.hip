@@ -443,6 +443,53 @@ int ceos_rve_profile(ceos_rve_batch* b, long long* ticks) {
     return CEOS_OK;
 }
 
+// Converged state of every RVE of the batch: F_conv (the macro F of the last converged solve),
+// the fluctuation w and the last homogenized P and dP/dF.  The file starts with a header (magic,
+// RVE count, total torus nodes) and the sides, so a state only loads into a batch of the same
+// RVEs.  Both calls synchronise the device first: a solve enqueued on any stream has finished.
+static const int kRveMagic = 0x43525645;
+
+int ceos_rve_state_save(ceos_rve_batch* b, const char* path) {
+    if (!b || !path) return CEOS_ERR_ARG;
+    RVE_HIP_OK(hipDeviceSynchronize());
+    const long long n = b->n_rve, tot = b->total_nodes;
+    std::vector<double> Fc(9 * n), w(3 * tot), P(9 * n), A(81 * n);
+    RVE_HIP_OK(hipMemcpy(Fc.data(), b->dFconv, sizeof(double) * Fc.size(), hipMemcpyDeviceToHost));
+    RVE_HIP_OK(hipMemcpy(w.data(), b->dw, sizeof(double) * w.size(), hipMemcpyDeviceToHost));
+    RVE_HIP_OK(hipMemcpy(P.data(), b->dP, sizeof(double) * P.size(), hipMemcpyDeviceToHost));
+    RVE_HIP_OK(hipMemcpy(A.data(), b->dA, sizeof(double) * A.size(), hipMemcpyDeviceToHost));
+    FILE* f = fopen(path, "wb");
+    if (!f) return CEOS_ERR_ARG;
+    const long long hdr[3] = {kRveMagic, n, tot};
+    bool ok = fwrite(hdr, sizeof(hdr), 1, f) == 1 && fwrite(b->side.data(), sizeof(int), n, f) == (size_t)n;
+    for (const auto* v : {&Fc, &w, &P, &A})
+        ok = ok && fwrite(v->data(), sizeof(double), v->size(), f) == v->size();
+    ok = (fclose(f) == 0) && ok;
+    return ok ? CEOS_OK : CEOS_ERR_ARG;
+}
+
+int ceos_rve_state_load(ceos_rve_batch* b, const char* path) {
+    if (!b || !path) return CEOS_ERR_ARG;
+    RVE_HIP_OK(hipDeviceSynchronize());
+    FILE* f = fopen(path, "rb");
+    if (!f) return CEOS_ERR_ARG;
+    const long long n = b->n_rve, tot = b->total_nodes;
+    long long hdr[3];
+    std::vector<int> side(n);
+    bool ok = fread(hdr, sizeof(hdr), 1, f) == 1 && hdr[0] == kRveMagic && hdr[1] == n && hdr[2] == tot &&
+              fread(side.data(), sizeof(int), n, f) == (size_t)n && side == b->side;
+    std::vector<double> Fc(9 * n), w(3 * tot), P(9 * n), A(81 * n);
+    for (auto* v : {&Fc, &w, &P, &A})
+        ok = ok && fread(v->data(), sizeof(double), v->size(), f) == v->size();
+    fclose(f);
+    if (!ok) return CEOS_ERR_ARG;
+    RVE_HIP_OK(hipMemcpy(b->dFconv, Fc.data(), sizeof(double) * Fc.size(), hipMemcpyHostToDevice));
+    RVE_HIP_OK(hipMemcpy(b->dw, w.data(), sizeof(double) * w.size(), hipMemcpyHostToDevice));
+    RVE_HIP_OK(hipMemcpy(b->dP, P.data(), sizeof(double) * P.size(), hipMemcpyHostToDevice));
+    RVE_HIP_OK(hipMemcpy(b->dA, A.data(), sizeof(double) * A.size(), hipMemcpyHostToDevice));
+    return CEOS_OK;
+}
+
 int ceos_rve_get_fluctuation(ceos_rve_batch* b, int rve, double* w) {
     if (!b || !w || rve < 0 || rve >= b->n_rve) return CEOS_ERR_ARG;
     long long off = 0;

@@ -223,6 +223,14 @@ int ceos_set_pipeline(ceos_handle *h, int chunks);
 /* Mesh facts after setup: number of element colours and the nnz of the upper pattern. */
 int ceos_mesh_info(ceos_handle *h, int *n_colors, long long *nnz_upper);
 
+/* Facts of the gather assembly of a prepared handle (ceos_mesh_prepare), for traffic accounting.
+ * No reference counterpart.  info: [0] patches (16-slot element groups), [1] boundary records
+ * (9 doubles each, one contribution word each, summed by the gather), [2] of which diagonal,
+ * [3] complete records (written into K by the element kernel), [4] of which diagonal, [5] node
+ * blocks of the upper pattern, [6] node blocks the gather writes, [7] of which diagonal.
+ * CEOS_ERR_ARG on an unprepared handle. */
+int ceos_assembly_info(ceos_handle *h, long long info[8]);
+
 /* FE^2 macro meshes (elements of CEOS_MODEL_EXTERNAL).  The reference has no concurrent FE^2
  * loop; these stand where a macro analysis would call the RVE analysis of
  * MultiScale/ModMultiscaleAnalysis.f90 at every Gauss point.
@@ -382,6 +390,14 @@ int ceos_rve_profile(ceos_rve_batch *b, long long *ticks);
 /* Converged fluctuation of RVE `rve` (3 side^3 doubles, torus node x + s (y + s z)). */
 int ceos_rve_get_fluctuation(ceos_rve_batch *b, int rve, double *w);
 
+/* Converged state of the whole batch (F_conv, fluctuation, last homogenized P and dP/dF) to and
+ * from a file, for restarting a multiscale analysis (FEM/ModFEMAnalysis.f90:2827-2956 rebuilds
+ * the same state by replaying the end-of-step macro F through the RVE analysis).  The file carries
+ * the RVE count and sides; loading into a batch of other RVEs returns CEOS_ERR_ARG.  Both calls
+ * synchronise the device. */
+int ceos_rve_state_save(ceos_rve_batch *b, const char *path);
+int ceos_rve_state_load(ceos_rve_batch *b, const char *path);
+
 /* The same solve for the RVEs listed in sel only (distinct ids), each from its last converged
  * state; Fsel / Psel / Asel (may be NULL) / info_sel hold one entry per listed RVE, in the order
  * of sel.  Host arrays; the call synchronises.  The other RVEs keep their state. */
@@ -468,6 +484,19 @@ int ceos_surrogate_features_device(ceos_surrogate *s, int n, const double *dF, c
  * Device arrays; stream NULL = synchronous. */
 int ceos_surrogate_eval_device(ceos_surrogate *s, int n, const double *dF, const double *dClin, double *dext,
                                int *dflags, double *dP, void *stream);
+
+/* Probe correction of the surrogate (no reference counterpart).  Every microstructure is also
+ * solved at n_probe fixed probe stretches; r = S_RVE - S_surrogate at the probes (6 per probe,
+ * Voigt) is its second descriptor, extended by r_{6 n_probe} = 1.  ceos_surrogate_set_correction
+ * stores the weights W[6][6 n_probe + 1][21] and the probe strain scale e_p (n_probe = 0 removes the
+ * correction; at most 8 probes; synchronises the device).  ceos_surrogate_correct_device adds
+ *   dS_o = sum_{j,q} W[o][j][q] r_j m_q(E),  m_q = Ee_a Ee_b / e_p^2 (a <= b, engineering strains)
+ * and its derivative (symmetrized) to the records dext of n points written by
+ * ceos_surrogate_eval_device; dR [n][6 n_probe] holds each point's r; dP (may be NULL) is
+ * rewritten from the corrected S. */
+int ceos_surrogate_set_correction(ceos_surrogate *s, int n_probe, const double *W, double e_p);
+int ceos_surrogate_correct_device(ceos_surrogate *s, int n, const double *dF, const double *dR, double *dext,
+                                  double *dP, void *stream);
 
 #ifdef __cplusplus
 }

@@ -11,6 +11,7 @@
 import numpy as np
 import pytest
 
+import fe2_oracle as fo
 from ceos_amd import fe2, gpu, hiprt, rve
 
 pytestmark = pytest.mark.gpu
@@ -150,3 +151,37 @@ def test_macro_newton_solve_surrogate_vs_rve(surrogate):
     assert d < 1e-2
     assert all(e["fallback"] is False for e in out["surrogate"][2]), out["surrogate"][2]
     assert max(out["surrogate"][1]) <= max(out["rve"][1]) + 1
+
+
+def test_probe_correction_kernel_matches_oracle():
+    """ceos_surrogate_correct_device adds the oracle's probe correction (S and the symmetrized
+    tangent) to the records of ceos_surrogate_eval_device, and rewrites P from the corrected S."""
+    mlp, _, meta = fe2.load_surrogate(max_points=64)
+    so = gpu.Surrogate(mlp, fe2.surrogate_cfg(meta), 64)
+    rng = np.random.default_rng(3)
+    n = 50
+    W = rng.normal(scale=1e-2, size=(6, 37, 21))
+    so.set_correction(W, fe2.PROBE_EP)
+    F = np.eye(3)[None] + rng.uniform(-0.08, 0.08, size=(n, 3, 3))
+    Clin = np.repeat(np.asarray(meta["d_mean"])[None], n, 0)
+    R = rng.normal(scale=1e-2, size=(n, 36))
+    D = hiprt.DeviceArray.from_numpy
+    dF = D(np.ascontiguousarray(F.transpose(0, 2, 1).reshape(-1)))
+    dC, dR = D(Clin.reshape(-1)), D(R.reshape(-1))
+    dext, dP = hiprt.DeviceArray(27 * n), hiprt.DeviceArray(9 * n)
+    so.eval_device(n, dF.ptr, dC.ptr, dext.ptr)
+    e0 = dext.numpy().reshape(n, 27).copy()
+    so.correct_device(n, dF.ptr, dR.ptr, dext.ptr, dP.ptr)
+    e1 = dext.numpy().reshape(n, 27)
+    P = dP.numpy().reshape(n, 3, 3).transpose(0, 2, 1)
+    worst = 0.0
+    for k in range(n):
+        dS, Dc = fo.probe_correction(W, R[k], F[k], fe2.PROBE_EP)
+        ds = e1[k, :6] - e0[k, :6]
+        dd = e1[k, 6:] - e0[k, 6:]
+        worst = max(worst, np.abs(ds - dS).max() / np.abs(dS).max(), np.abs(dd - fo.up21(Dc)).max() / np.abs(Dc).max())
+        S = fo.from_voigt(e1[k, :6])
+        assert np.allclose(P[k], F[k] @ S, rtol=0, atol=1e-13 * np.abs(S).max())
+    print("correction kernel vs oracle: worst relative difference %.2e" % worst, flush=True)
+    assert worst < 1e-10
+    so.close()

@@ -169,3 +169,27 @@ def test_autotune_picks_a_tile_and_every_tile_agrees():
         for k in range(len(sides)):
             assert rel(out[tile][0][k], out[4][0][k]) < 1e-9
             assert rel(out[tile][1][k], out[4][1][k]) < 1e-7
+
+
+def test_state_save_load_continues_bitwise(tmp_path):
+    """A batch restarted from ceos_rve_state_save continues exactly as the uninterrupted batch
+    (the restart of a multiscale analysis, FEM/ModFEMAnalysis.f90:2827-2956, rebuilds the same
+    converged RVE state); a state file does not load into a batch of other RVEs."""
+    sides, ids = [6, 4, 8], [11, 12, 13]
+    F1 = np.stack([_F(40 + k, 0.04) for k in range(3)])
+    F2 = np.stack([_F(50 + k, 0.05) for k in range(3)])
+    a, _ = _batch(sides, ids)
+    a.solve(F1)
+    path = str(tmp_path / "rve.state")
+    a.state_save(path)
+    P_a, A_a, info_a = a.solve(F2, tangent=True)
+    b, _ = _batch(sides, ids)
+    b.state_load(path)
+    P_b, A_b, info_b = b.solve(F2, tangent=True)
+    print("info", info_a.tolist(), info_b.tolist(), flush=True)
+    assert np.array_equal(P_a, P_b) and np.array_equal(A_a, A_b) and np.array_equal(info_a, info_b)
+    for r in range(3):
+        assert np.array_equal(a.fluctuation(r), b.fluctuation(r))
+    other, _ = _batch([6, 4, 6], ids)
+    with pytest.raises(gpu.CeosError):
+        other.state_load(path)
