@@ -351,6 +351,9 @@ def fe2_bench(no_cpu):
         "outside_hull": s2["outside_hull"], "judged": s2["judged"],
         "judge_err": [s1["judge_err"], s2["judge_err"]],
         "judge_point_max": max(s1["judge_point_max"], s2["judge_point_max"]),
+        "judge_gate": "aggregate (norm-wise estimate <= 1 %)",
+        "judge_point_pass": bool(s1["judge_point_pass"] and s2["judge_point_pass"]),
+        "per_point_bar_met": bool(err.max() <= 0.01),
         "fallback": bool(s1["fallback"] or s2["fallback"]),
         "stress_err_vs_rve": {"mean": float(err.mean()), "p99": float(np.percentile(err, 99)),
                               "max": float(err.max()), "aggregate": float(agg)},

@@ -33,10 +33,6 @@ LAM_LO, LAM_HI = 0.8, 1.25     # principal stretches of the training distributio
 U_SCALE = 0.25                 # U features: (U - I) / U_SCALE
 E_REF = 0.1                    # output scaling: S = S_base + (|E|^2 / E_REF^2) (t_mean + t_std y)
 N_IN, WIDTH, N_OUT = 27, 256, 6
-# probe stretches of the probe correction: isochoric extensions along x, y, z (stretch 1.06) and
-# isochoric pure shears in xy, yz, xz (U = exp of 0.04 (e_i e_j + e_j e_i)); PROBE_EP is the strain
-# scale e_p of the correction's monomials
-PROBE_LAM, PROBE_GAMMA, PROBE_EP = 1.06, 0.04, 0.06
 VOIGT = ((0, 0), (1, 1), (2, 2), (0, 1), (1, 2), (0, 2))
 PROPS = np.array([rve.MATRIX_PROPS, rve.INCLUSION_PROPS])
 
@@ -91,34 +87,6 @@ def walk_stretch(rng, U, sig_lam=0.05, sig_rot=0.25, lo=LAM_LO, hi=LAM_HI, sig_l
     R = np.eye(3)[None] + s * K + (1.0 - c) * (K @ K)  # Rodrigues
     Q = R @ Q
     return np.einsum("nij,nj,nkj->nik", Q, lam, Q)
-
-
-def probe_stretches():
-    """The 6 probe stretch tensors (6, 3, 3) of the probe correction."""
-    lam = PROBE_LAM
-    out = [np.diag(np.roll([lam, lam ** -0.5, lam ** -0.5], k)) for k in range(3)]
-    for i, j in ((0, 1), (1, 2), (0, 2)):
-        # exp of the symmetric traceless G = g (e_i e_j + e_j e_i): eigenvalues +-g, 0
-        g = PROBE_GAMMA
-        U = np.eye(3)
-        U[i, i] = U[j, j] = np.cosh(g)
-        U[i, j] = U[j, i] = np.sinh(g)
-        out.append(U)
-    return np.array(out)
-
-
-def correction_monomials(F, e_p=PROBE_EP):
-    """m_q(E) = Ee_a Ee_b / e_p^2 (a <= b, engineering strains), (..., 21)."""
-    E = to_voigt(green_lagrange(F)) * np.array([1, 1, 1, 2, 2, 2.0])
-    return np.stack([E[..., a] * E[..., b] for a in range(6) for b in range(a, 6)], axis=-1) / e_p ** 2
-
-
-def correction(W, R, F, e_p=PROBE_EP):
-    """The probe correction of S (..., 6): sum_{j,q} W[o, j, q] r_j m_q(E), r = [R, 1] (numpy
-    restatement of ceos_surrogate_correct_device's stress part)."""
-    R = np.asarray(R)
-    r = np.concatenate([R, np.ones(R.shape[:-1] + (1,))], axis=-1)
-    return np.einsum("ojq,...j,...q->...o", W, r, correction_monomials(F, e_p))
 
 
 def green_lagrange(F):
@@ -208,42 +176,6 @@ def descriptors(batch, pcg_rtol=1e-8):
     return dext.numpy().reshape(n, gpu.EXT_PER_GP)[:, 6:].copy()
 
 
-def surrogate_stress(so, F, Clin, max_points):
-    """S (Voigt) of the surrogate `so` (gpu.Surrogate, without the probe correction) at F (n, 3, 3)
-    with descriptors Clin (n, 21), evaluated in chunks of max_points on the device."""
-    n = len(F)
-    S = np.zeros((n, 6))
-    m0 = min(n, max_points)
-    dF, dC, dext = hiprt.DeviceArray(9 * m0), hiprt.DeviceArray(21 * m0), hiprt.DeviceArray(gpu.EXT_PER_GP * m0)
-    for a in range(0, n, m0):
-        m = min(m0, n - a)
-        dF.view(0, 9 * m).upload(np.ascontiguousarray(np.asarray(F[a:a + m]).transpose(0, 2, 1).reshape(-1)))
-        dC.view(0, 21 * m).upload(np.ascontiguousarray(Clin[a:a + m]).reshape(-1))
-        so.eval_device(m, dF.ptr, dC.ptr, dext.ptr)
-        S[a:a + m] = dext.view(0, gpu.EXT_PER_GP * m).numpy().reshape(m, gpu.EXT_PER_GP)[:, :6]
-    return S
-
-
-def probe_residuals(batch, so, Clin, max_points, newton_rtol=1e-10, pcg_rtol=1e-10):
-    """The second descriptor of every RVE of `batch`: r = S_RVE(U_p) - S_surrogate(U_p) at the 6
-    probe stretches (n_rve, 36), each probe solved from the undeformed state.  Leaves the batch
-    reset and with the default solver settings."""
-    n = batch.n
-    R = np.zeros((n, 6, 6))
-    batch.set_solver(newton_rtol=newton_rtol, pcg_rtol=pcg_rtol)
-    try:
-        for k, Up in enumerate(probe_stretches()):
-            batch.reset()
-            Ub = np.repeat(Up[None], n, 0)
-            P, _, _ = batch.solve(Ub)
-            S = np.linalg.solve(Ub, P)
-            R[:, k] = to_voigt(0.5 * (S + S.transpose(0, 2, 1))) - surrogate_stress(so, Ub, Clin, max_points)
-    finally:
-        batch.set_solver()
-        batch.reset()
-    return R.reshape(n, 36)
-
-
 def surrogate_cfg(meta):
     return {k: meta[k] for k in ("u_scale", "lam_lo", "lam_hi", "e_ref", "d_mean", "d_std", "d_lo", "d_hi",
                                  "t_mean", "t_std")}
@@ -257,8 +189,6 @@ def load_surrogate(path=SURROGATE_PATH, max_points=4096):
     mlp = gpu.Mlp(N_IN, width, N_OUT, max_batch=max_points, seed=1)
     mlp.set_params(meta["params"].astype(np.float32))
     sur = gpu.Surrogate(mlp, surrogate_cfg(meta), max_points)
-    if "corr_W" in meta:
-        sur.set_correction(meta["corr_W"], float(meta["corr_ep"]))
     return mlp, sur, meta
 
 
@@ -291,8 +221,6 @@ class MacroFE2:
         self.dFint = D(self.ndof)
         self.dK = D(self.nnz)
         self.dClin = D(21 * self.ngp)
-        self.dR = None      # probe residuals [ngp][36] of the probe correction
-        self.probe_s = 0.0
         self.dflags = D(self.ngp, np.int32)
         self.stream = hiprt.Stream()
         self.mesh.set_external(self.dext.ptr)
@@ -306,19 +234,11 @@ class MacroFE2:
         self.dU.upload(np.zeros(self.ndof))
 
     def setup_descriptors(self, pcg_rtol=1e-8):
-        """C_lin of every RVE, and the probe residuals when the surrogate carries a probe
-        correction (their time in self.probe_s); returns the seconds spent on C_lin."""
         t = time.perf_counter()
         self.Clin = descriptors(self.rves, pcg_rtol)
         self.dClin.upload(self.Clin.reshape(-1))
         self.stress_floor = 1e-9 * float(np.abs(self.Clin).max())  # the stress of a 1e-9 strain
-        t_clin = time.perf_counter() - t
-        if self.sur is not None and getattr(self.sur[1], "n_probe", 0):
-            t = time.perf_counter()
-            self.R = probe_residuals(self.rves, self.sur[1], self.Clin, self.sur[1].max_points)
-            self.dR = hiprt.DeviceArray.from_numpy(self.R.reshape(-1))
-            self.probe_s = time.perf_counter() - t
-        return t_clin
+        return time.perf_counter() - t
 
     def _kinematics(self, U):
         self.dU.upload(np.ascontiguousarray(U, dtype=np.float64))
@@ -340,8 +260,6 @@ class MacroFE2:
         self._chain_kinematics()
         self.sur[1].eval_device(self.ngp, self.dF.ptr, self.dClin.ptr, self.dext.ptr, self.dflags.ptr, None,
                                 self.stream.ptr)
-        if self.dR is not None:
-            self.sur[1].correct_device(self.ngp, self.dF.ptr, self.dR.ptr, self.dext.ptr, None, self.stream.ptr)
 
     def _chain_assemble(self):
         self.mesh.evaluate_device(0.0, self.dU.ptr, self.dFint.ptr, self.dK.ptr, self.stream.ptr)
@@ -398,23 +316,170 @@ class MacroFE2:
         return self._finish({"mode": "rve", "seconds": time.perf_counter() - t0, "rve_solves": self.ngp,
                              "newton_mean": float(info[:, 1].mean()), "cg_total": int(info[:, 2].sum())})
 
+    # ---- the macro Newton step on the device ------------------------------------------------
+    def _chain_residual0(self):
+        self.mesh.residual_bc_device(0, self.dU.ptr, self.dFint.ptr, self.dK.ptr, None, self.dUbar.ptr, self.dRes.ptr,
+                                     self.dnorm.ptr, 2, self.stream.ptr)
+
+    def _chain_residual(self):
+        self.mesh.residual_bc_device(1, self.dU.ptr, self.dFint.ptr, self.dK.ptr, None, self.dUbar.ptr, self.dRes.ptr,
+                                     self.dnorm.ptr, 2, self.stream.ptr)
+
+    def _chain_solve(self):
+        s = self.stream.ptr
+        self.mesh.linear_solve_device_async(self.dK.ptr, self.dRes.ptr, self.dDX.ptr, self.dpcg.ptr, self.pcg_rtol,
+                                            self.pcg_maxit, s)
+        self.mesh.newton_update_device(self.dU.ptr, self.dDX.ptr, -1.0, s)
+
+    def set_load_case(self, presc_dof, fixed_dof, pcg_rtol=1e-10, pcg_maxit=600):
+        """Dirichlet DOFs (0-based) of the macro load case (ceos_bc_set) and the settings of the
+        Newton step's PCG (ceos_linear_solve_device_async: pcg_maxit iterations enqueued, early
+        exit on the device).  Drops graphs captured for an earlier load case."""
+        self.presc = np.asarray(presc_dof, dtype=np.int64)
+        self.mesh.set_bc(self.presc, fixed_dof)
+        D = hiprt.DeviceArray
+        if getattr(self, "dRes", None) is None:
+            self.dRes, self.dDX, self.dUbar = D(self.ndof), D(self.ndof), D(self.ndof)
+            self.dnorm, self.dpcg = D(3), D(3)
+        self.pcg_rtol, self.pcg_maxit = float(pcg_rtol), int(pcg_maxit)
+        for k in ("residual0", "residual", "solve"):
+            self.graphs.pop(k, None)
+
+    def newton_step(self, ubar, U0, mode="rve", tol=1e-6, maxit=12, use_graph=True, judge_frac=0.01,
+                    judge_tol=0.01, judge_gate="aggregate"):
+        """One macro load step by full Newton on the device, the loop of
+        Tools/MathTools/NonlinearSolver/ModNewtonRaphsonFull.f90:44-168 with the reference's
+        max-abs residual test (norm < tol norm_0 after the first iteration): per iteration the
+        evaluation chain (`mode` "rve": every RVE solved and condensed; "surrogate": the surrogate
+        and the RVE fallback outside the hull), the Dirichlet residual and
+        its norm (ceos_residual_bc_device), then PCG and the update U -= DX
+        (ceos_linear_solve_device_async, ceos_newton_update_device).  With use_graph every piece is
+        a replayed HIP graph and the host synchronises once per iteration, to read the norm and
+        the PCG outcome.  ubar: the prescribed DOFs' values at the end of the step (set_load_case
+        order); U0: the displacement the step starts from.  In surrogate mode ceil(judge_frac
+        ngp) points are re-solved at the converged state and gate the step as in
+        evaluate_surrogate (judge_gate "aggregate" or "point"); a failed gate sends the step
+        through the RVE path again from U0.  Returns (U, stats)."""
+        if mode not in ("rve", "surrogate"):
+            raise ValueError("mode must be 'rve' or 'surrogate'")
+        if mode == "surrogate" and (self.sur is None or self.Clin is None):
+            raise RuntimeError("surrogate mode needs a surrogate and setup_descriptors()")
+        t0 = time.perf_counter()
+        Ubar = np.zeros(self.ndof)
+        Ubar[self.presc] = ubar
+        self.dUbar.upload(Ubar)
+        self._kinematics(U0)
+        recs = []
+        norm0 = None
+        converged = False
+        for it in range(maxit + 1):
+            if it:
+                self._run("solve", use_graph)
+            if mode == "rve":
+                self._run("rve", use_graph)
+            else:
+                self._run("surrogate", use_graph)
+                self.stream.synchronize()
+                flags = self.dflags.numpy()
+                out = np.nonzero(flags)[0].astype(np.int32)
+                if len(out):
+                    self._fallback_points(out, "surrogate")
+                self._assemble(use_graph)
+            self._run("residual0" if it == 0 else "residual", use_graph)
+            self.stream.synchronize()
+            code, _ = self.mesh.device_status()
+            if code != 0:
+                raise gpu.CeosError(code, "macro element evaluation")
+            nrm = float(self.dnorm.numpy()[2])
+            rec = {"it": it, "norm": nrm}
+            if it:
+                pit, prr, pfl = self.dpcg.numpy()
+                rec.update({"pcg_iterations": int(pit), "pcg_relres": float(prr), "pcg_flag": int(pfl)})
+                if int(pfl) != 1:
+                    raise RuntimeError("macro PCG flag %d at Newton iteration %d" % (int(pfl), it))
+            if mode == "rve":
+                info = self.dinfo.numpy().reshape(-1, 4)
+                if np.any(info[:, 0] != 0):
+                    raise gpu.CeosError(gpu.ERR_LOCAL_NR, "RVE solve (%d points)" % int(np.sum(info[:, 0] != 0)))
+            recs.append(rec)
+            norm0 = nrm if it == 0 else norm0
+            if it > 0 and nrm < tol * norm0:
+                converged = True
+                break
+        stats = {"mode": mode, "converged": converged, "iterations": len(recs) - 1, "history": recs}
+        if converged and mode == "surrogate" and judge_frac > 0:
+            ev, agg = self._judge(judge_frac)
+            pmax = float(ev.max()) if len(ev) else 0.0
+            stats.update({"judged": int(len(ev)), "judge_err": agg, "judge_point_max": pmax,
+                          "judge_point_pass": pmax <= judge_tol})
+            if len(ev) and (pmax if judge_gate == "point" else agg) > judge_tol:
+                U, st2 = self.newton_step(ubar, U0, "rve", tol, maxit, use_graph)
+                stats.update({"fallback": True, "rve_step": st2, "seconds": time.perf_counter() - t0})
+                return U, stats
+            stats["fallback"] = False
+        stats["seconds"] = time.perf_counter() - t0
+        self.stream.synchronize()
+        return self.dU.numpy(), stats
+
+    def _fallback_points(self, out, fallback_tangent):
+        """The RVE solve's S (and with fallback_tangent "rve" its tangent) at the points `out`."""
+        Fcm = self.dF.numpy().reshape(self.ngp, 9)
+        Fm = Fcm.reshape(self.ngp, 3, 3).transpose(0, 2, 1)
+        with_t = fallback_tangent == "rve"
+        P, A, _ = self.rves.solve_subset(out, Fm[out], tangent=with_t)
+        D = hiprt.DeviceArray.from_numpy
+        dsel, dFs = D(out), D(np.ascontiguousarray(Fcm[out].reshape(-1)))
+        dPs = D(np.ascontiguousarray(P.transpose(0, 2, 1).reshape(-1)))
+        dAs = D(np.ascontiguousarray(A.reshape(-1))) if with_t else None
+        gpu.rve_to_external_device(len(out), dFs.ptr, dPs.ptr, dAs.ptr if with_t else None, self.dext.ptr,
+                                   dsel=dsel.ptr, stream=self.stream.ptr)
+        self.stream.synchronize()
+
+    def _judge(self, judge_frac, judge_newton_rtol=1e-3):
+        """(relative stress errors, norm-wise estimate) of ceil(judge_frac ngp) sampled inside
+        points, re-solved by the RVE path at the current macro state, against the records of the
+        last evaluation."""
+        self.stream.synchronize()
+        flags = self.dflags.numpy()
+        inside = np.nonzero(flags == 0)[0]
+        nj = min(len(inside), int(math.ceil(judge_frac * self.ngp)))
+        if nj == 0:
+            return np.zeros(0), 0.0
+        if self.judge_ids is None or len(self.judge_ids) != nj:
+            self.judge_ids = np.sort(self.judge_rng.choice(inside, size=nj, replace=False)).astype(np.int32)
+        jid = self.judge_ids[flags[self.judge_ids] == 0]
+        Fm = self.dF.numpy().reshape(self.ngp, 3, 3).transpose(0, 2, 1)
+        self.rves.set_solver(newton_rtol=judge_newton_rtol)
+        try:
+            Pj, _, _ = self.rves.solve_subset(jid, Fm[jid])
+        finally:
+            self.rves.set_solver()
+        Sj = np.linalg.solve(Fm[jid], Pj)
+        S_rve = to_voigt(0.5 * (Sj + Sj.transpose(0, 2, 1)))
+        S_all = self.ext()[:, :6]
+        w = np.array([1, 1, 1, 2, 2, 2.0])
+        num = np.mean(np.sum(w * (S_all[jid] - S_rve) ** 2, axis=1))
+        den = np.mean(np.sum(w * S_all ** 2, axis=1))
+        agg = float(np.sqrt(num / max(den, self.stress_floor ** 2)))
+        return stress_rel_error(S_all[jid], S_rve, self.stress_floor), agg
+
     def new_judge_sample(self):
         """Draw the judged points again at the next surrogate evaluation (e.g. per load step);
         until then the same points are re-solved, each from its own last converged state."""
         self.judge_ids = None
 
     def evaluate_surrogate(self, U, judge_frac=0.01, judge_tol=0.01, fallback_tangent="surrogate",
-                           judge_newton_rtol=1e-3, use_graph=False, judge_gate="point"):
-        """Surrogate macro evaluation: the MLP (plus the probe correction when the surrogate carries
-        one) at points inside the training hull; outside it the
+                           judge_newton_rtol=1e-3, use_graph=False, judge_gate="aggregate"):
+        """Surrogate macro evaluation: the MLP at points inside the training hull; outside it the
         RVE solve gives the stress (with fallback_tangent="surrogate" the point keeps the
         surrogate's tangent, an inexact macro Newton tangent; "rve" condenses the RVE tangent too);
         ceil(judge_frac ngp) sampled inside points are re-solved as judges.  The judges gate the
-        evaluation: with judge_gate="point" (the per-point bar, max over points of
-        |S_nn - S_rve| / |S_rve| <= judge_tol) any judged point above judge_tol, with "aggregate"
-        the norm-wise estimate above judge_tol (the RMS error of the judged points over the RMS
-        stress of all points, tensor norms), makes the whole evaluation fall back to the plain
-        path.  judge_err reports the norm-wise estimate, judge_point_max the largest judged error.
+        evaluation: with judge_gate="aggregate" the norm-wise estimate above judge_tol (the RMS
+        error of the judged points over the RMS stress of all points, tensor norms), with "point"
+        (the per-point bar, max over points of |S_nn - S_rve| / |S_rve| <= judge_tol) any judged
+        point above judge_tol, makes the whole evaluation fall back to the plain path.  judge_err
+        reports the norm-wise estimate, judge_point_max the largest judged error, judge_pass
+        whether the chosen gate held and judge_point_pass whether the per-point bar held.
         Judged RVEs are converged to judge_newton_rtol (relative Newton test): 1e-3 moves
         the estimate by about 1e-5 against fully converged judges (the Newton iterates converge
         quadratically, so the last one is far more accurate than the test), three orders below the
@@ -472,6 +537,7 @@ class MacroFE2:
             raise ValueError("judge_gate must be 'point' or 'aggregate'")
         gate = (err.max() if len(err) else 0.0) if judge_gate == "point" else agg
         stats["judge_pass"] = bool(gate <= judge_tol)
+        stats["judge_point_pass"] = bool((err.max() if len(err) else 0.0) <= judge_tol)
         if len(err) and gate > judge_tol:
             r = self.evaluate_rve(U, use_graph)
             stats["fallback"] = True

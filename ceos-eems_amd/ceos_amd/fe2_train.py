@@ -1,9 +1,6 @@
 """Train the FE^2 surrogate on the GPU from batched RVE solves.
 
     python -m ceos_amd.fe2_train --out surrogate.npz [--data-out data.npz | --load-data data.npz]
-    python -m ceos_amd.fe2_train correction --out surrogate.npz [--data-out d.npz | --load-data d.npz]
-
-The second form fits the probe correction of a trained surrogate (see the end of this list).
 
 1. Training microstructures: n_rve seeded RVEs (sides drawn from {8, 12, 16}, 20 % spherical
    inclusions, Neo-Hookean phases; seed SEED + 1, so the macro workload's RVEs are unseen).
@@ -23,15 +20,6 @@ The second form fits the probe correction of a trained surrogate (see the end of
    training distribution (solved from the undeformed state), evaluated through the inference
    path (ceos_surrogate_eval_device); the error of the base alone (the method without the
    network) is reported beside it.
-
-7. Probe correction (the `correction` form): new microstructures (seed SEED_CORR_TRAIN) get C_lin
-   and their probe residuals r = S_RVE - S_surrogate at the 6 probe stretches of
-   fe2.probe_stretches, then walk through near-isochoric, mild and broad stretches; the weights W
-   of S += W(r) : m(E) (ceos_surrogate_correct_device) are the weighted least-squares fit of the
-   surrogate's residual S - S_surrogate at every sample (weights 1/|S|^2: the squared relative
-   error), ridge-regularized with the strength chosen on a quarter of the RVEs held out; the
-   errors are then measured on unseen microstructures (seed SEED_CORR_TEST) solved from the
-   undeformed state at independent stretches of each set.
 
 The result file holds the parameters, the normalization and hull constants and the test errors
 (numpy .npz, no pickled objects).
@@ -202,172 +190,7 @@ def train(X, T, epochs, batch, lr0, seed, W=None, log_every=25, fp32=False, lr_f
     return mlp, time.time() - t0
 
 
-# ------------------------------------------------------------------------------------------
-# probe correction: S += W(r) : m(E), fitted by weighted least squares
-# ------------------------------------------------------------------------------------------
-SEED_CORR_TRAIN = rve.SEED + 1000
-SEED_CORR_TEST = rve.SEED + 1001
-# walk sets of the correction data: (name, steps, lo, hi, log J scale or None)
-CORR_SETS = (("near_isochoric", 80, fe2.LAM_LO, fe2.LAM_HI, 0.01), ("mild", 64, 0.9, 1.1, None),
-             ("broad", 64, fe2.LAM_LO, fe2.LAM_HI, None))
-
-
-def walk(b, steps, rng, sig_lam, sig_rot, lo, hi, sig_logj=None):
-    """Every RVE of batch b walks `steps` stretches from a random start (one batched solve per
-    step from the previous converged state); returns the converged samples (U, ids, S)."""
-    n = b.n
-    b.reset()
-    U = fe2.random_stretch(rng, n, lo, hi)
-    if sig_logj is not None:
-        lam, Q = np.linalg.eigh(U)
-        U = np.einsum("nij,nj,nkj->nik", Q, fe2.isochoric(rng, lam, sig_logj, lo, hi), Q)
-    Us, ids, Ss = [], [], []
-    for k in range(steps):
-        if k:
-            U = fe2.walk_stretch(rng, U, sig_lam, sig_rot, lo, hi, sig_logj)
-        P, _, info = b.solve(U, check=False)
-        ok = info[:, 0] == 0
-        Us.append(U[ok])
-        ids.append(np.nonzero(ok)[0])
-        Ss.append(S_from(U[ok], P[ok]))
-        if (k + 1) % 8 == 0:
-            log("  walk step %d/%d: converged %d/%d" % (k + 1, steps, ok.sum(), n))
-    return np.concatenate(Us), np.concatenate(ids), np.concatenate(Ss)
-
-
-def correction_data(so, max_points, n_rve, n_test, rng, sig_lam=0.02, sig_rot=0.06):
-    """Samples for fitting the probe correction of surrogate `so` (gpu.Surrogate): training RVEs
-    (seed SEED_CORR_TRAIN) walked through CORR_SETS, and test RVEs (SEED_CORR_TEST) solved from
-    the undeformed state at independent stretches of each set; with every RVE's C_lin and probe
-    residuals and the surrogate's stress at every sample."""
-    d = {}
-    for tag, seed, n in (("", SEED_CORR_TRAIN, n_rve), ("test_", SEED_CORR_TEST, n_test)):
-        t = time.time()
-        b, sides = make_batch(n, seed)
-        Clin = fe2.descriptors(b, pcg_rtol=1e-8)
-        R = fe2.probe_residuals(b, so, Clin, max_points)
-        log("%sRVEs %d: descriptors and probes %.1f s" % (tag, n, time.time() - t))
-        Us, ids, Ss, set_id = [], [], [], []
-        for k, (name, steps, lo, hi, slj) in enumerate(CORR_SETS):
-            t = time.time()
-            if tag:
-                b.reset()
-                U = fe2.random_stretch(rng, n, lo, hi)
-                if slj is not None:
-                    lam, Q = np.linalg.eigh(U)
-                    U = np.einsum("nij,nj,nkj->nik", Q, fe2.isochoric(rng, lam, slj, lo, hi), Q)
-                P, _, info = b.solve(U, check=False)
-                ok = info[:, 0] == 0
-                U, i, S = U[ok], np.nonzero(ok)[0], S_from(U[ok], P[ok])
-            else:
-                U, i, S = walk(b, steps, rng, sig_lam, sig_rot, lo, hi, slj)
-            Us.append(U)
-            ids.append(i)
-            Ss.append(S)
-            set_id.append(np.full(len(U), k, dtype=np.int8))
-            log("%s%s: %d samples, %.1f s" % (tag, name, len(U), time.time() - t))
-        U, ids = np.concatenate(Us), np.concatenate(ids)
-        d.update({tag + "Clin": Clin, tag + "R": R, tag + "sides": sides, tag + "U": U, tag + "ids": ids,
-                  tag + "S": np.concatenate(Ss), tag + "set": np.concatenate(set_id),
-                  tag + "S_nn": fe2.surrogate_stress(so, U, Clin[ids], max_points)})
-        b.close()
-    return d
-
-
-def _corr_features(R, F, e_p):
-    """(n, 37 x 21) features r_j m_q(E) of the probe correction, r = [R, 1]."""
-    m = fe2.correction_monomials(F, e_p)
-    r = np.concatenate([R, np.ones((len(R), 1))], axis=1)
-    return (r[:, :, None] * m[:, None, :]).reshape(len(F), -1)
-
-
-def fit_correction(d, alpha, e_p=fe2.PROBE_EP, chunk=20000, weight_floor=1e-2):
-    """W (6, 37, 21) minimizing sum_s w_s |S_s - S_nn,s - W(r) : m(E_s)|^2 + alpha |W~|^2, with
-    w_s = 1 / max(|S_s|^2, weight_floor^2 mean |S|^2) (the squared relative error of S; tensor
-    norms, shear counted twice) and W~ the weights of the RMS-normalized features."""
-    R, U, ids, S, Snn = d["R"], d["U"], d["ids"], d["S"], d["S_nn"]
-    vw = np.array([1, 1, 1, 2, 2, 2.0])
-    s2 = np.sum(vw * S * S, axis=1)
-    w = 1.0 / np.maximum(s2, weight_floor ** 2 * s2.mean())
-    nf = (R.shape[1] + 1) * 21
-    G = np.zeros((nf, nf))
-    B = np.zeros((nf, 6))
-    for a in range(0, len(U), chunk):
-        X = _corr_features(R[ids[a:a + chunk]], U[a:a + chunk], e_p)
-        Xw = X * w[a:a + chunk, None]
-        G += X.T @ Xw
-        B += Xw.T @ (S[a:a + chunk] - Snn[a:a + chunk])
-    sc = np.sqrt(np.maximum(np.diag(G), 1e-300) / len(U))
-    Gn = G / np.outer(sc, sc) + alpha * len(U) * np.eye(nf) * w.mean()
-    Wn = np.linalg.solve(Gn, B / sc[:, None])
-    return (Wn / sc[:, None]).T.reshape(6, R.shape[1] + 1, 21)
-
-
-def correction_errors(d, W, tag="test_", e_p=fe2.PROBE_EP):
-    """Relative stress errors per sample, without and with the correction."""
-    R, U, ids, S, Snn = d[tag + "R"], d[tag + "U"], d[tag + "ids"], d[tag + "S"], d[tag + "S_nn"]
-    Sc = Snn + fe2.correction(W, R[ids], U, e_p)
-    return fe2.stress_rel_error(Snn, S), fe2.stress_rel_error(Sc, S)
-
-
-def main_correction(argv):
-    ap = argparse.ArgumentParser(prog="fe2_train correction")
-    ap.add_argument("--surrogate", default=fe2.SURROGATE_PATH, help="trained surrogate (.npz)")
-    ap.add_argument("--out", required=True, help="surrogate file with the fitted correction")
-    ap.add_argument("--rves", type=int, default=1024)
-    ap.add_argument("--test-rves", type=int, default=1024)
-    ap.add_argument("--seed", type=int, default=11)
-    ap.add_argument("--data-out", default=None)
-    ap.add_argument("--load-data", default=None)
-    ap.add_argument("--alphas", default="1e-6,1e-5,1e-4,1e-3")
-    a = ap.parse_args(argv)
-    with np.load(a.surrogate, allow_pickle=False) as z:
-        meta = {k: z[k] for k in z.files if not k.startswith("corr_")}
-    if a.load_data:
-        with np.load(a.load_data, allow_pickle=False) as z:
-            d = {k: z[k] for k in z.files}
-    else:
-        mlp, so, _ = fe2.load_surrogate(a.surrogate, max_points=4096)
-        so.set_correction(None, 1.0)
-        d = correction_data(so, 4096, a.rves, a.test_rves, np.random.default_rng(a.seed))
-        if a.data_out:
-            np.savez(a.data_out, **d)
-    # alpha by validation on a quarter of the training RVEs, then a fit on all of them
-    val = (d["ids"] % 4) == 3
-    dtr = {k: d[k][~val] for k in ("U", "ids", "S", "S_nn")}
-    dtr["R"] = d["R"]
-    dva = {"v_" + k: d[k][val] for k in ("U", "ids", "S", "S_nn")}
-    dva["v_R"] = d["R"]
-    best = None
-    for alpha in (float(x) for x in a.alphas.split(",")):
-        e0, e1 = correction_errors(dva, fit_correction(dtr, alpha), "v_")
-        log("alpha %.0e: validation mean %.5f p99 %.5f max %.5f (surrogate alone mean %.5f max %.5f)" % (
-            alpha, e1.mean(), np.percentile(e1, 99), e1.max(), e0.mean(), e0.max()))
-        if best is None or e1.max() < best[1]:
-            best = (alpha, e1.max())
-    W = fit_correction(d, best[0])
-    e0, e1 = correction_errors(d, W)
-    out = dict(meta)
-    for k, (name, *_r) in enumerate(CORR_SETS):
-        m = d["test_set"] == k
-        log("test %s (%d points): surrogate mean %.5f p99 %.5f max %.5f -> corrected mean %.5f p99 %.5f "
-            "max %.5f" % (name, m.sum(), e0[m].mean(), np.percentile(e0[m], 99), e0[m].max(), e1[m].mean(),
-                          np.percentile(e1[m], 99), e1[m].max()))
-        out["corr_test_%s" % name] = np.array([e1[m].mean(), np.percentile(e1[m], 99), e1[m].max(),
-                                               e0[m].mean(), np.percentile(e0[m], 99), e0[m].max()])
-    out.update({"corr_W": W, "corr_ep": np.float64(fe2.PROBE_EP), "corr_alpha": np.float64(best[0]),
-                "corr_probes": fe2.probe_stretches(), "corr_n_rves": np.int64(len(d["R"])),
-                "corr_n_samples": np.int64(len(d["U"])), "corr_seed_train": np.int64(SEED_CORR_TRAIN),
-                "corr_seed_test": np.int64(SEED_CORR_TEST)})
-    np.savez(a.out, **out)
-    log("saved %s (alpha %.0e)" % (a.out, best[0]))
-
-
 def main(argv=None):
-    import sys
-    argv = sys.argv[1:] if argv is None else argv
-    if argv and argv[0] == "correction":
-        return main_correction(argv[1:])
     ap = argparse.ArgumentParser()
     ap.add_argument("--rves", type=int, default=1024)
     ap.add_argument("--steps", type=int, default=196)

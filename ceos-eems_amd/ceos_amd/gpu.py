@@ -71,8 +71,7 @@ EXPORTS = [
     "ceos_bc_set", "ceos_residual_bc", "ceos_residual_bc_device", "ceos_set_solver_precision",
     "ceos_linear_solve", "ceos_linear_solve_device", "ceos_newton_update_device", "ceos_set_linear_solver",
     "ceos_mesh_prepare", "ceos_evaluate_device_static", "ceos_assembly_info",
-    "ceos_rve_state_save", "ceos_rve_state_load", "ceos_surrogate_set_correction",
-    "ceos_surrogate_correct_device",
+    "ceos_rve_state_save", "ceos_rve_state_load", "ceos_linear_solve_device_async",
 ]
 
 
@@ -123,8 +122,7 @@ def load_library(path=LIB_PATH):
     L.ceos_assembly_info.argtypes = [vp, C.POINTER(C.c_longlong)]
     L.ceos_rve_state_save.argtypes = [vp, C.c_char_p]
     L.ceos_rve_state_load.argtypes = [vp, C.c_char_p]
-    L.ceos_surrogate_set_correction.argtypes = [vp, i, _dp, d]
-    L.ceos_surrogate_correct_device.argtypes = [vp, i, vp, vp, vp, vp, vp]
+    L.ceos_linear_solve_device_async.argtypes = [vp, vp, vp, vp, d, i, vp, vp]
     L.ceos_set_schedule.argtypes = [vp, i]
     L.ceos_set_pipeline.argtypes = [vp, i]
     L.ceos_set_assembly.argtypes = [vp, i]
@@ -428,6 +426,13 @@ class CeosMesh:
                                                        C.byref(fl), stream), "ceos_linear_solve_device")
         return it.value, rr.value, fl.value
 
+    def linear_solve_device_async(self, dK, db, dx, dresult, rtol=1e-10, maxit=1000, stream=None):
+        """ceos_linear_solve_device_async: PCG of exactly maxit enqueued iterations (early exit on
+        the device), no host synchronisation; dresult (3 doubles) receives {iterations, relative
+        residual, flag}."""
+        _check(load_library().ceos_linear_solve_device_async(self._h, dK, db, dx, rtol, maxit, dresult, stream),
+               "ceos_linear_solve_device_async")
+
     def newton_update_device(self, dU, dDX, scale=1.0, stream=None):
         _check(load_library().ceos_newton_update_device(self._h, dU, dDX, scale, stream),
                "ceos_newton_update_device")
@@ -653,23 +658,3 @@ class Surrogate:
     def eval_device(self, n, dF, dClin, dext, dflags=None, dP=None, stream=None):
         _check(load_library().ceos_surrogate_eval_device(self._h, n, dF, dClin, dext, dflags, dP, stream),
                "ceos_surrogate_eval_device")
-
-    def set_correction(self, W, e_p):
-        """ceos_surrogate_set_correction: probe-correction weights W (6, 6 n_probe + 1, 21) and the
-        probe strain scale e_p; W None removes the correction."""
-        if W is None:
-            _check(load_library().ceos_surrogate_set_correction(self._h, 0, np.zeros(1), 1.0),
-                   "ceos_surrogate_set_correction")
-            self.n_probe = 0
-            return
-        W = np.ascontiguousarray(W, dtype=np.float64)
-        n_probe = (W.shape[1] - 1) // 6
-        assert W.shape == (6, 6 * n_probe + 1, 21), W.shape
-        _check(load_library().ceos_surrogate_set_correction(self._h, n_probe, W.reshape(-1), float(e_p)),
-               "ceos_surrogate_set_correction")
-        self.n_probe = n_probe
-
-    def correct_device(self, n, dF, dR, dext, dP=None, stream=None):
-        """ceos_surrogate_correct_device: add the probe correction to n records of eval_device."""
-        _check(load_library().ceos_surrogate_correct_device(self._h, n, dF, dR, dext, dP, stream),
-               "ceos_surrogate_correct_device")

@@ -1252,47 +1252,56 @@ static void build_operator(ceos_handle* h, const void* dK, hipStream_t s) {
     }
 }
 
-// block-Jacobi PCG from x = 0.  Synchronises s every cg_check iterations to read the state.
-static int pcg_run(ceos_handle* h, const double* db, double* dx, double rtol, int maxit, hipStream_t s, int* iters,
-                   double* relres, int* flag) {
-    const SysGeom g = sys_geom(h);
+// PCG start from x = 0: state scalars (tolerance, iteration limit) set by a kernel, so the
+// sequence holds no host copy and may be captured
+static void pcg_start(ceos_handle* h, const double* db, double* dx, double rtol, int maxit, hipStream_t s) {
     const int nn = h->nn;
     const int nblk = (nn + kSysBlock - 1) / kSysBlock;
-    CgScalars c0;
-    memset(&c0, 0, sizeof(c0));
-    c0.tol2 = rtol * rtol;
-    c0.maxit = maxit;
-    HIP_OK(hipMemcpyAsync(h->dcg, &c0, sizeof(c0), hipMemcpyHostToDevice, s));
     hipLaunchKernelGGL(k_cg_init, dim3(nblk), dim3(kSysBlock), 0, s, nn, (const double*)h->dMinv, db, dx, h->dcg_r,
                        h->dcg_z, h->dcg_p, h->dpart, h->dpart2);
     hipLaunchKernelGGL(k_cg_init_final, dim3(1), dim3(kSysBlock), 0, s, (const double*)h->dpart,
-                       (const double*)h->dpart2, nblk, h->dcg);
-    // per iteration: q = K p (partials of p.q in dpart3, one per node block); alpha, x, r, z
-    // (partials of r.z and r.r in dpart, dpart2, one per update workgroup); beta, test, p.  r.z of
-    // iteration k sits in rz[k & 1].
+                       (const double*)h->dpart2, nblk, rtol * rtol, maxit, h->dcg);
+}
+
+// iterations k0 .. k0 + n - 1 of PCG.  Per iteration: q = K p (partials of p.q in dpart3, one per
+// node block); alpha, x, r, z (partials of r.z and r.r in dpart, dpart2, one per update
+// workgroup); beta, test, p.  r.z of iteration k sits in rz[k & 1].  After the test has set the
+// stop flag the kernels return at once, so iterations past convergence change nothing.
+static void pcg_iterations(ceos_handle* h, double* dx, int k0, int n, hipStream_t s) {
+    const SysGeom g = sys_geom(h);
+    const int nn = h->nn;
+    const int nblk = (nn + kSysBlock - 1) / kSysBlock;
     const int nupd = std::min(nblk, kCgUpdBlocks);
+    for (int k = k0; k < k0 + n; ++k) {
+        const int par = k & 1;
+        if (h->ell_f32)
+            hipLaunchKernelGGL((k_cg_spmv<float, kCgSlots>), dim3(nblk), dim3(kSysBlock), 0, s, g,
+                               (const float*)h->dell, (const CgScalars*)h->dcg, (const double*)h->dcg_p, h->dcg_q,
+                               h->dpart3);
+        else
+            hipLaunchKernelGGL((k_cg_spmv<double, kCgSlots64>), dim3(nblk), dim3(kSysBlock), 0, s, g,
+                               (const double*)h->dell, (const CgScalars*)h->dcg, (const double*)h->dcg_p, h->dcg_q,
+                               h->dpart3);
+        hipLaunchKernelGGL(k_cg_update, dim3(nupd), dim3(kSysBlock), 0, s, nn, par, (const double*)h->dMinv, h->dcg,
+                           (const double*)h->dpart3, nblk, (const double*)h->dcg_p, (const double*)h->dcg_q, dx,
+                           h->dcg_r, h->dcg_z, h->dpart, h->dpart2);
+        hipLaunchKernelGGL(k_cg_pupdate, dim3(nupd), dim3(kSysBlock), 0, s, nn, par, h->dcg, (const double*)h->dpart,
+                           (const double*)h->dpart2, nupd, (const double*)h->dcg_z, h->dcg_p);
+    }
+}
+
+// block-Jacobi PCG from x = 0.  Synchronises s every cg_check iterations to read the state.
+static int pcg_run(ceos_handle* h, const double* db, double* dx, double rtol, int maxit, hipStream_t s, int* iters,
+                   double* relres, int* flag) {
+    pcg_start(h, db, dx, rtol, maxit, s);
     CgScalars c;
     int k = 0;
     for (;;) {
         HIP_OK(hipMemcpyAsync(&c, h->dcg, sizeof(c), hipMemcpyDeviceToHost, s));
         HIP_OK(hipStreamSynchronize(s));
         if (c.done) break;
-        for (int i = 0; i < h->cg_check; ++i, ++k) {
-            const int par = k & 1;
-            if (h->ell_f32)
-                hipLaunchKernelGGL((k_cg_spmv<float, kCgSlots>), dim3(nblk), dim3(kSysBlock), 0, s, g,
-                                   (const float*)h->dell, (const CgScalars*)h->dcg, (const double*)h->dcg_p, h->dcg_q,
-                                   h->dpart3);
-            else
-                hipLaunchKernelGGL((k_cg_spmv<double, kCgSlots64>), dim3(nblk), dim3(kSysBlock), 0, s, g,
-                                   (const double*)h->dell, (const CgScalars*)h->dcg, (const double*)h->dcg_p, h->dcg_q,
-                                   h->dpart3);
-            hipLaunchKernelGGL(k_cg_update, dim3(nupd), dim3(kSysBlock), 0, s, nn, par, (const double*)h->dMinv, h->dcg,
-                               (const double*)h->dpart3, nblk, (const double*)h->dcg_p, (const double*)h->dcg_q, dx,
-                               h->dcg_r, h->dcg_z, h->dpart, h->dpart2);
-            hipLaunchKernelGGL(k_cg_pupdate, dim3(nupd), dim3(kSysBlock), 0, s, nn, par, h->dcg, (const double*)h->dpart,
-                               (const double*)h->dpart2, nupd, (const double*)h->dcg_z, h->dcg_p);
-        }
+        pcg_iterations(h, dx, k, h->cg_check, s);
+        k += h->cg_check;
         HIP_OK(hipGetLastError());
     }
     if (iters) *iters = c.iters;
@@ -2007,7 +2016,8 @@ int ceos_bc_set(ceos_handle* h, int n_presc, const int* presc_dof, int n_fixed, 
     HIP_OK(hipMemcpy(h->dnode_mask, nm.data(), h->nn, hipMemcpyHostToDevice));
     h->n_presc = n_presc;
     h->n_fixed = n_fixed;
-    return CEOS_OK;
+    // the linear solver's buffers too, so that the solve of a Newton step may be captured
+    return ensure_solver(h);
 }
 
 int ceos_residual_bc_device(ceos_handle* h, int it, const double* dU, const double* dFint, void* dK,
@@ -2067,6 +2077,25 @@ int ceos_linear_solve_device(ceos_handle* h, const void* dK, const double* db, d
     hipStream_t s = stream ? (hipStream_t)stream : h->stream;
     int rc = linear_solve(h, dK, db, dx, rtol, maxit, s, iters, relres, flag);
     if (rc) return rc;
+    return note_caller_stream(h, s);
+}
+
+int ceos_linear_solve_device_async(ceos_handle* h, const void* dK, const double* db, double* dx, double rtol,
+                                   int maxit, double* dresult, void* stream) {
+    if (!h || !h->sys_ready || !dK || !db || !dx || !dresult || !(rtol > 0.0) || maxit < 0) return CEOS_ERR_ARG;
+    hipStream_t s = stream ? (hipStream_t)stream : h->stream;
+    const int want_prec = h->solver_precision < 0 ? h->precision : h->solver_precision;
+    const bool ready = h->dell && h->dMinv && h->ell_f32 == (want_prec == CEOS_K_FP32_STORED);
+    if (!ready) {
+        if (stream_capturing(s)) return CEOS_ERR_ARG;  // the buffers are allocated outside a capture
+        int rc = ensure_solver(h);
+        if (rc) return rc;
+    }
+    build_operator(h, dK, s);
+    pcg_start(h, db, dx, rtol, maxit, s);
+    pcg_iterations(h, dx, 0, maxit, s);
+    hipLaunchKernelGGL(k_cg_report, dim3(1), dim3(64), 0, s, (const CgScalars*)h->dcg, dresult);
+    HIP_OK(hipGetLastError());
     return note_caller_stream(h, s);
 }
 

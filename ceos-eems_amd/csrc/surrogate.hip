@@ -20,15 +20,6 @@
 // it linearizes to C_lin exactly, so the surrogate is exact to first order at U = I, and it
 // carries the volumetric nonlinearity (J-dependent pressure) of the Neo-Hookean phases, so the
 // network only learns a residual that is small against both the volumetric and the shear stress.
-// Probe correction (optional, ceos_surrogate_set_correction): every microstructure is also solved
-// at n_probe fixed probe stretches U_p, and r = S_RVE(U_p) - S_surrogate(U_p) (6 per probe) is kept
-// as a second descriptor, extended by r_{6 n_probe} = 1.  The correction is quadratic in E and
-// linear in r:
-//   S     += sum_{j,q} W[o][j][q] r_j m_q(E),   m_q = Ee_a Ee_b / e_p^2 (a <= b, Ee engineering)
-//   dS/dE += sum_{j,q} W[o][j][q] r_j dm_q/dEe,
-// with W fitted by weighted least squares on training microstructures.  It vanishes to second
-// order at U = I, like the network's residual, and only adds what the probes reveal about the
-// microstructure's own nonlinear response.
 // Training hull: the convex set {U : lam_lo <= eig(U) <= lam_hi} (the convex hull of the
 // training stretches, which are drawn over exactly that set) times the box of the normalized
 // descriptors seen in training.  Points outside are flagged for the RVE fallback.
@@ -276,68 +267,6 @@ __global__ __launch_bounds__(256) void k_sur_output(int n, SurCfg cfg, const dou
     }
 }
 
-constexpr int kMaxProbe = 8;
-constexpr int kCorrQ = 21;  // quadratic monomials of the engineering strain
-
-// One thread per point: the probe correction of S and of dS/dE in a point's external record
-// (and of the first Piola stress when Pout is given).  W: [6][nr + 1][21], nr = 6 n_probe values of
-// r per point in R, the last row of W multiplies the constant 1.
-__global__ __launch_bounds__(256) void k_sur_correct(int n, int nr, double inv_ep2, const double* __restrict__ W,
-                                                     const double* __restrict__ Fin, const double* __restrict__ R,
-                                                     double* __restrict__ ext, double* __restrict__ Pout) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n) return;
-    double F[3][3], C[3][3];
-    load_F(Fin + 9L * t, F);
-    right_cauchy_green(F, C);
-    const double Ee[6] = {0.5 * (C[0][0] - 1.0), 0.5 * (C[1][1] - 1.0), 0.5 * (C[2][2] - 1.0), C[0][1], C[1][2],
-                          C[0][2]};
-    const double* r = R + (long)nr * t;
-    double dS[6], D[6][6];
-#pragma unroll 1
-    for (int o = 0; o < 6; ++o) {
-        double c[kCorrQ];
-#pragma unroll
-        for (int q = 0; q < kCorrQ; ++q) c[q] = 0.0;
-        const double* w = W + (long)o * (nr + 1) * kCorrQ;
-#pragma unroll 1
-        for (int j = 0; j <= nr; ++j) {
-            const double rj = j < nr ? r[j] : 1.0;
-#pragma unroll
-            for (int q = 0; q < kCorrQ; ++q) c[q] = fma(w[j * kCorrQ + q], rj, c[q]);
-        }
-        double acc = 0.0, g[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-        for (int a = 0; a < 6; ++a)
-#pragma unroll
-            for (int b = a; b < 6; ++b) {
-                const double cq = c[up_idx(a, b)] * inv_ep2;
-                acc += cq * Ee[a] * Ee[b];
-                g[a] += cq * Ee[b];
-                g[b] += cq * Ee[a];
-            }
-        dS[o] = acc;
-#pragma unroll
-        for (int b = 0; b < 6; ++b) D[o][b] = g[b];
-    }
-    double* e = ext + (long)kExtPerGp * t;
-#pragma unroll
-    for (int a = 0; a < 6; ++a) e[a] += dS[a];
-#pragma unroll
-    for (int a = 0; a < 6; ++a)
-#pragma unroll
-        for (int b = a; b < 6; ++b) e[6 + up_idx(a, b)] += 0.5 * (D[a][b] + D[b][a]);
-    if (Pout) {
-        const double S[6] = {e[0], e[1], e[2], e[3], e[4], e[5]};
-        const double Sm[3][3] = {{S[0], S[3], S[5]}, {S[3], S[1], S[4]}, {S[5], S[4], S[2]}};
-#pragma unroll
-        for (int Jc = 0; Jc < 3; ++Jc)
-#pragma unroll
-            for (int i = 0; i < 3; ++i)
-                Pout[9L * t + 3 * Jc + i] = F[i][0] * Sm[0][Jc] + F[i][1] * Sm[1][Jc] + F[i][2] * Sm[2][Jc];
-    }
-}
-
 __global__ __launch_bounds__(256) void k_rve_to_ext(int n, const int* __restrict__ sel, const double* __restrict__ Fin,
                                                     const double* __restrict__ P, const double* __restrict__ A,
                                                     double* __restrict__ ext) {
@@ -395,9 +324,6 @@ struct ceos_surrogate {
     int max_points = 0;
     float *X = nullptr, *Y = nullptr, *J = nullptr;
     double* aux = nullptr;
-    int n_probe = 0;           // probe correction: probes, weights [6][6 n_probe][21], 1 / e_p^2
-    double* W = nullptr;
-    double inv_ep2 = 0.0;
 };
 
 namespace {
@@ -407,10 +333,8 @@ void sur_free(ceos_surrogate* s) {
     if (s->Y) (void)hipFree(s->Y);
     if (s->J) (void)hipFree(s->J);
     if (s->aux) (void)hipFree(s->aux);
-    if (s->W) (void)hipFree(s->W);
     s->X = s->Y = s->J = nullptr;
     s->aux = nullptr;
-    s->W = nullptr;
 }
 
 }  // namespace
@@ -486,32 +410,6 @@ int ceos_surrogate_eval_device(ceos_surrogate* s, int n, const double* dF, const
     if (rc) return rc;
     hipLaunchKernelGGL(k_sur_output, dim3((n + 255) / 256), dim3(256), 0, st, n, s->cfg, dF, dClin, s->aux, s->Y,
                        s->J, dext, dP);
-    SUR_HIP_OK(hipGetLastError());
-    if (!stream) SUR_HIP_OK(hipStreamSynchronize(st));
-    return CEOS_OK;
-}
-
-int ceos_surrogate_set_correction(ceos_surrogate* s, int n_probe, const double* W, double e_p) {
-    if (!s || n_probe < 0 || n_probe > kMaxProbe || (n_probe > 0 && (!W || !(e_p > 0.0)))) return CEOS_ERR_ARG;
-    SUR_HIP_OK(hipDeviceSynchronize());
-    if (s->W) (void)hipFree(s->W);
-    s->W = nullptr;
-    s->n_probe = 0;
-    if (n_probe == 0) return CEOS_OK;
-    const size_t nw = (size_t)6 * (6 * n_probe + 1) * kCorrQ;
-    SUR_HIP_OK(hipMalloc((void**)&s->W, sizeof(double) * nw));
-    SUR_HIP_OK(hipMemcpy(s->W, W, sizeof(double) * nw, hipMemcpyHostToDevice));
-    s->n_probe = n_probe;
-    s->inv_ep2 = 1.0 / (e_p * e_p);
-    return CEOS_OK;
-}
-
-int ceos_surrogate_correct_device(ceos_surrogate* s, int n, const double* dF, const double* dR, double* dext,
-                                  double* dP, void* stream) {
-    if (!s || n < 1 || !dF || !dR || !dext || s->n_probe < 1) return CEOS_ERR_ARG;
-    hipStream_t st = stream ? (hipStream_t)stream : hipStreamPerThread;
-    hipLaunchKernelGGL(k_sur_correct, dim3((n + 255) / 256), dim3(256), 0, st, n, 6 * s->n_probe, s->inv_ep2, s->W,
-                       dF, dR, dext, dP);
     SUR_HIP_OK(hipGetLastError());
     if (!stream) SUR_HIP_OK(hipStreamSynchronize(st));
     return CEOS_OK;

@@ -368,10 +368,12 @@ __global__ __launch_bounds__(kSysBlock) void k_cg_init(int nn, const double* __r
 }
 
 __global__ __launch_bounds__(kSysBlock) void k_cg_init_final(const double* part_s, const double* part_m, int n,
-                                                             CgScalars* c) {
+                                                             double tol2, int maxit, CgScalars* c) {
     double rz, rr;
     reduce2(part_s, part_m, n, rz, rr);
     if (threadIdx.x == 0) {
+        c->tol2 = tol2;
+        c->maxit = maxit;
         c->rz[0] = rz;
         c->rr = c->bb = rr;
         c->beta = 0.0;
@@ -379,6 +381,15 @@ __global__ __launch_bounds__(kSysBlock) void k_cg_init_final(const double* part_
         c->iters = 0;
         c->done = (rr == 0.0) ? 1 : 0;
         if (!c->done && c->maxit <= 0) c->done = 3;
+    }
+}
+
+// the outcome of a PCG run for the host: {iterations, relative residual |r| / |b|, flag}
+__global__ void k_cg_report(const CgScalars* c, double* out) {
+    if (threadIdx.x == 0) {
+        out[0] = (double)c->iters;
+        out[1] = c->bb > 0.0 ? sqrt(c->rr / c->bb) : 0.0;
+        out[2] = (double)c->done;
     }
 }
 

@@ -258,7 +258,9 @@ int ceos_gp_deformation_gradient(ceos_handle *h, const double *U, double *F);
  * displacements (ClassBoundaryConditions' Presc_Disp_DOF) and fixed supports (FixedSupport%dof).
  * Replaces AllocatePrescDispSparseMapping / AllocateFixedSupportSparseMapping
  * (FEM/BoundaryConditions/ModBoundaryConditions.f90:344-536), which search all nnz entries for
- * every constrained DOF: here a per-DOF mask, O(ndof) to set, is tested as rows are streamed. */
+ * every constrained DOF: here a per-DOF mask, O(ndof) to set, is tested as rows are streamed.
+ * The call also allocates the linear solver's buffers (ceos_linear_solve_device_async may then
+ * be captured). */
 int ceos_bc_set(ceos_handle *h, int n_presc, const int *presc_dof, int n_fixed, const int *fixed_dof);
 
 /* EvaluateR + EvaluateKt of Newton iteration `it` (FEM/Modules/ModFEMSystemOfEquations.f90:40-93)
@@ -308,6 +310,18 @@ int ceos_linear_solve(ceos_handle *h, const double *b, double *x, double rtol, i
                       double *relres, int *flag);
 int ceos_linear_solve_device(ceos_handle *h, const void *dK, const double *db, double *dx, double rtol, int maxit,
                              int *iters, double *relres, int *flag, void *stream);
+
+/* ceos_linear_solve_device without host synchronisation, for a Newton step captured in a HIP graph
+ * (the reference's PARDISO call, ModPardisoSolver.f90:343-380, has no such form).  Block-Jacobi
+ * PCG only: the call enqueues the operator copy, the start and exactly maxit iterations on
+ * `stream`; once the relative residual test holds (or p.Kp <= 0), the remaining iterations return
+ * at once and change nothing.  dresult (3 doubles, device) receives {iterations, relative
+ * residual, flag} as ceos_linear_solve_device reports them (1 converged, 2 breakdown, 3 iteration
+ * limit), readable once the stream has passed the call.  The solver buffers are allocated by the
+ * first call outside a capture (or by ceos_linear_solve_device); on a capturing stream without
+ * them, or after ceos_set_solver_precision changed their type, the call returns CEOS_ERR_ARG. */
+int ceos_linear_solve_device_async(ceos_handle *h, const void *dK, const double *db, double *dx, double rtol,
+                                   int maxit, double *dresult, void *stream);
 
 /* X = X + scale DX and PostUpdate (FEMUpdateMesh: x = X0 + U when the analysis is nonlinear,
  * FEM/Modules/ModFEMSystemOfEquations.f90:99-106) on device arrays. */
@@ -485,18 +499,6 @@ int ceos_surrogate_features_device(ceos_surrogate *s, int n, const double *dF, c
 int ceos_surrogate_eval_device(ceos_surrogate *s, int n, const double *dF, const double *dClin, double *dext,
                                int *dflags, double *dP, void *stream);
 
-/* Probe correction of the surrogate (no reference counterpart).  Every microstructure is also
- * solved at n_probe fixed probe stretches; r = S_RVE - S_surrogate at the probes (6 per probe,
- * Voigt) is its second descriptor, extended by r_{6 n_probe} = 1.  ceos_surrogate_set_correction
- * stores the weights W[6][6 n_probe + 1][21] and the probe strain scale e_p (n_probe = 0 removes the
- * correction; at most 8 probes; synchronises the device).  ceos_surrogate_correct_device adds
- *   dS_o = sum_{j,q} W[o][j][q] r_j m_q(E),  m_q = Ee_a Ee_b / e_p^2 (a <= b, engineering strains)
- * and its derivative (symmetrized) to the records dext of n points written by
- * ceos_surrogate_eval_device; dR [n][6 n_probe] holds each point's r; dP (may be NULL) is
- * rewritten from the corrected S. */
-int ceos_surrogate_set_correction(ceos_surrogate *s, int n_probe, const double *W, double e_p);
-int ceos_surrogate_correct_device(ceos_surrogate *s, int n, const double *dF, const double *dR, double *dext,
-                                  double *dP, void *stream);
 
 #ifdef __cplusplus
 }

@@ -11,10 +11,6 @@
   surrogate(p, cfg, F, Clin) the surrogate's S, dS/dE and hull flag:
                              S = base + |E|^2/e_ref^2 (t_mean + t_std MLP(x)),
                              x = [(U - I)/u_scale, (C_lin - d_mean)/d_std]
-  probe_correction(W, r, F, e_p)
-                             the surrogate's probe correction: dS_o = sum_{j,q} W[o,j,q] r_j m_q(E),
-                             r = [r, 1], m_q = Ee_a Ee_b / e_p^2 (a <= b, engineering strains), and
-                             its tangent d(dS)/dEe, symmetrized
 The pieces are pinned in tests/test_fe2_oracle.py by what the mathematics fixes: the pull back of
 a pushed-forward tangent, central differences of P(F) and of S(E).
 """
@@ -201,18 +197,3 @@ def surrogate(p, cfg, F, Clin, width):
         Ct[:, b] = Cb[:, b] + t * dphi + phi * cfg["t_std"] * (Jx @ dx)
     return S, 0.5 * (Ct + Ct.T), inside
 
-
-def probe_correction(W, r, F, e_p):
-    """(dS (6,), d(dS)/dEe (6, 6) symmetrized) of one point: W (6, nr + 1, 21), r (nr,), F (3, 3)."""
-    C = F.T @ F
-    Ee = np.array([0.5 * (C[0, 0] - 1), 0.5 * (C[1, 1] - 1), 0.5 * (C[2, 2] - 1), C[0, 1], C[1, 2], C[0, 2]])
-    rr = np.append(np.asarray(r, dtype=np.float64), 1.0)
-    c = np.einsum("ojq,j->oq", W, rr) / e_p ** 2
-    pairs = [(a, b) for a in range(6) for b in range(a, 6)]
-    dS = np.array([sum(c[o, q] * Ee[a] * Ee[b] for q, (a, b) in enumerate(pairs)) for o in range(6)])
-    D = np.zeros((6, 6))
-    for o in range(6):
-        for q, (a, b) in enumerate(pairs):
-            D[o, a] += c[o, q] * Ee[b]
-            D[o, b] += c[o, q] * Ee[a]
-    return dS, 0.5 * (D + D.T)

@@ -171,32 +171,3 @@ def test_hull_flag():
     far = np.asarray(cfg["d_mean"]) + 4.0 * np.asarray(cfg["d_std"])
     assert not fo.features(cfg, np.eye(3), far)[1]
 
-
-def _F_of_Ee(Ee):
-    """The symmetric F = (2E + I)^(1/2) of an engineering-strain Voigt vector."""
-    E = np.array([[Ee[0], Ee[3] / 2, Ee[5] / 2], [Ee[3] / 2, Ee[1], Ee[4] / 2], [Ee[5] / 2, Ee[4] / 2, Ee[2]]])
-    lam2, Q = np.linalg.eigh(2.0 * E + np.eye(3))
-    return Q @ np.diag(np.sqrt(lam2)) @ Q.T
-
-
-def test_probe_correction_tangent_is_its_derivative():
-    """The correction's tangent equals central differences of its stress over the engineering
-    strains (symmetrized), and the correction is homogeneous of degree 2 in E (it vanishes with
-    zero slope at F = I)."""
-    rng = np.random.default_rng(5)
-    W = rng.normal(size=(6, 37, 21)) * 1e-2
-    r = rng.normal(size=36) * 1e-2
-    Ee = rng.uniform(-0.05, 0.05, size=6)
-    dS, D = fo.probe_correction(W, r, _F_of_Ee(Ee), 0.06)
-    h = 1e-6
-    Dfd = np.zeros((6, 6))
-    for b in range(6):
-        e = np.zeros(6)
-        e[b] = h
-        Dfd[:, b] = (fo.probe_correction(W, r, _F_of_Ee(Ee + e), 0.06)[0] -
-                     fo.probe_correction(W, r, _F_of_Ee(Ee - e), 0.06)[0]) / (2 * h)
-    Dfd = 0.5 * (Dfd + Dfd.T)
-    assert np.abs(D - Dfd).max() < 1e-8 * np.abs(D).max()
-    # homogeneous of degree 2 in E: zero with zero slope at F = I
-    small = fo.probe_correction(W, r, _F_of_Ee(1e-2 * Ee), 0.06)[0]
-    assert np.abs(small - 1e-4 * dS).max() < 1e-9 * np.abs(dS).max()

@@ -11,7 +11,6 @@
 import numpy as np
 import pytest
 
-import fe2_oracle as fo
 from ceos_amd import fe2, gpu, hiprt, rve
 
 pytestmark = pytest.mark.gpu
@@ -55,6 +54,35 @@ def test_surrogate_on_held_out_microstructures(surrogate):
     assert inside.sum() >= 0.8 * n
     assert err.mean() < MEAN_ERR_BAR
     assert err_lin.mean() > GAIN_BAR * err.mean()
+
+
+@pytest.mark.xfail(reason="the per-point 1 % bar is not met: the committed surrogate's largest "
+                          "held-out error is 7 % (6.5 % on near-isochoric states, p99 2-4 %); "
+                          "C_lin does not carry a microstructure's nonlinear shear response",
+                   strict=False)
+def test_surrogate_meets_per_point_bar(surrogate):
+    """The bar the FE^2 metric is quoted at: max over points |S_nn - S_rve| / |S_rve| <= 1 %, on
+    held-out microstructures at stretches of the training distribution."""
+    mlp, sur, meta = surrogate
+    n = 96
+    sides = np.where(np.arange(n) % 2 == 0, 8, 12).astype(np.int32)
+    ph = np.concatenate([rve.phases(int(s), i, SEED_HELD_OUT) for i, s in enumerate(sides)])
+    b = gpu.RveBatch(sides, ph, gpu.MODEL_NH, fe2.PROPS)
+    b.set_precision(gpu.K_FP32_STORED)
+    Clin = fe2.descriptors(b)
+    U = fe2.random_stretch(np.random.default_rng(11), n)
+    P, _, info = b.solve(U, check=False)
+    S_rve = np.linalg.solve(U, P)
+    S_rve = fe2.to_voigt(0.5 * (S_rve + S_rve.transpose(0, 2, 1)))
+    D = hiprt.DeviceArray.from_numpy
+    dF = D(np.ascontiguousarray(U.transpose(0, 2, 1).reshape(-1)))
+    dC = D(np.ascontiguousarray(Clin.reshape(-1)))
+    dext, dfl = hiprt.DeviceArray(27 * n), hiprt.DeviceArray(n, np.int32)
+    sur.eval_device(n, dF.ptr, dC.ptr, dext.ptr, dfl.ptr)
+    inside = (dfl.numpy() == 0) & (info[:, 0] == 0)
+    err = fe2.stress_rel_error(dext.numpy().reshape(n, 27)[:, :6], S_rve)[inside]
+    print("per-point bar: max %.4f p99 %.4f" % (err.max(), np.percentile(err, 99)), flush=True)
+    assert err.max() <= 0.01
 
 
 def test_macro_evaluation_surrogate_vs_rve(surrogate):
@@ -153,35 +181,61 @@ def test_macro_newton_solve_surrogate_vs_rve(surrogate):
     assert max(out["surrogate"][1]) <= max(out["rve"][1]) + 1
 
 
-def test_probe_correction_kernel_matches_oracle():
-    """ceos_surrogate_correct_device adds the oracle's probe correction (S and the symmetrized
-    tangent) to the records of ceos_surrogate_eval_device, and rewrites P from the corrected S."""
-    mlp, _, meta = fe2.load_surrogate(max_points=64)
-    so = gpu.Surrogate(mlp, fe2.surrogate_cfg(meta), 64)
-    rng = np.random.default_rng(3)
-    n = 50
-    W = rng.normal(scale=1e-2, size=(6, 37, 21))
-    so.set_correction(W, fe2.PROBE_EP)
-    F = np.eye(3)[None] + rng.uniform(-0.08, 0.08, size=(n, 3, 3))
-    Clin = np.repeat(np.asarray(meta["d_mean"])[None], n, 0)
-    R = rng.normal(scale=1e-2, size=(n, 36))
-    D = hiprt.DeviceArray.from_numpy
-    dF = D(np.ascontiguousarray(F.transpose(0, 2, 1).reshape(-1)))
-    dC, dR = D(Clin.reshape(-1)), D(R.reshape(-1))
-    dext, dP = hiprt.DeviceArray(27 * n), hiprt.DeviceArray(9 * n)
-    so.eval_device(n, dF.ptr, dC.ptr, dext.ptr)
-    e0 = dext.numpy().reshape(n, 27).copy()
-    so.correct_device(n, dF.ptr, dR.ptr, dext.ptr, dP.ptr)
-    e1 = dext.numpy().reshape(n, 27)
-    P = dP.numpy().reshape(n, 3, 3).transpose(0, 2, 1)
-    worst = 0.0
-    for k in range(n):
-        dS, Dc = fo.probe_correction(W, R[k], F[k], fe2.PROBE_EP)
-        ds = e1[k, :6] - e0[k, :6]
-        dd = e1[k, 6:] - e0[k, 6:]
-        worst = max(worst, np.abs(ds - dS).max() / np.abs(dS).max(), np.abs(dd - fo.up21(Dc)).max() / np.abs(Dc).max())
-        S = fo.from_voigt(e1[k, :6])
-        assert np.allclose(P[k], F[k] @ S, rtol=0, atol=1e-13 * np.abs(S).max())
-    print("correction kernel vs oracle: worst relative difference %.2e" % worst, flush=True)
-    assert worst < 1e-10
-    so.close()
+def _stretch_case(m):
+    X = m.X
+    fixed = [3 * i + k for i in np.nonzero(X[:, 0] < 1e-12)[0] for k in range(3)]
+    presc = [3 * i for i in np.nonzero(X[:, 0] > 1 - 1e-12)[0]]
+    return fixed, presc
+
+
+def test_device_newton_step_matches_host_driver(surrogate):
+    """The macro Newton step on the device (ceos_bc_set, ceos_residual_bc_device,
+    ceos_linear_solve_device_async, ceos_newton_update_device; every piece a replayed HIP graph)
+    against the quasi-static host driver (the reference's Newton loop and BC code restated, sparse
+    direct solve) on the same plain FE^2 load history: the same Newton iteration counts and the
+    same converged macro displacements (to the PCG tolerance); the surrogate mode converges to
+    the RVE mode's displacements within the surrogate's accuracy, with no judge fallback."""
+    import fe_driver as fd
+    m = fe2.MacroFE2(n_macro=2, side_choices=(8,), seed=SEED_HELD_OUT, surrogate=surrogate)
+    fixed, presc = _stretch_case(m)
+    bc = fd.BoundaryConditions(m.ndof, fixed, [(d, (lambda t: 0.06 * t)) for d in presc])
+    hist = fd.quasi_static(_Fe2Backend(m, "rve"), bc, [(0.0, 1.0, 2)], nl=True, tol=1e-6, maxiter=12)
+    U_host, its_host = hist[-1][3], [h[4] for h in hist]
+    out = {}
+    for mode in ("rve", "surrogate"):
+        m = fe2.MacroFE2(n_macro=2, side_choices=(8,), seed=SEED_HELD_OUT, surrogate=surrogate)
+        if mode == "surrogate":
+            m.setup_descriptors()
+            m.rves.reset()
+        m.set_load_case(presc, fixed)
+        U, its, fb = np.zeros(m.ndof), [], []
+        for t in (0.5, 1.0):
+            U, st = m.newton_step(0.06 * t, U, mode=mode, tol=1e-6)
+            assert st["converged"]
+            its.append(st["iterations"])
+            fb.append(st.get("fallback", False))
+            print(mode, "t", t, "iterations", st["iterations"], "norms",
+                  ["%.2e" % r["norm"] for r in st["history"]], "pcg",
+                  [r.get("pcg_iterations") for r in st["history"]], flush=True)
+        out[mode] = (U, its, fb)
+    U_dev, its_dev, _ = out["rve"]
+    d_host = np.linalg.norm(U_dev - U_host) / np.linalg.norm(U_host)
+    d_sur = np.linalg.norm(out["surrogate"][0] - U_dev) / np.linalg.norm(U_dev)
+    print("device vs host driver: iterations %s / %s, |dU|/|U| %.2e; surrogate vs RVE %.2e" % (
+        its_dev, its_host, d_host, d_sur), flush=True)
+    assert its_dev == its_host
+    assert d_host < 1e-8
+    assert d_sur < 1e-2 and not any(out["surrogate"][2])
+
+
+def test_device_newton_graph_replays_match_eager(surrogate):
+    """The same device Newton step with and without HIP graphs gives bitwise the same iterates."""
+    res = []
+    for use_graph in (False, True):
+        m = fe2.MacroFE2(n_macro=2, side_choices=(8,), seed=SEED_HELD_OUT, surrogate=surrogate)
+        fixed, presc = _stretch_case(m)
+        m.set_load_case(presc, fixed)
+        U, st = m.newton_step(0.03, np.zeros(m.ndof), mode="rve", tol=1e-6, use_graph=use_graph)
+        U, st2 = m.newton_step(0.06, U, mode="rve", tol=1e-6, use_graph=use_graph)
+        res.append((U, [r["norm"] for r in st["history"] + st2["history"]]))
+    assert np.array_equal(res[0][0], res[1][0]) and res[0][1] == res[1][1]
