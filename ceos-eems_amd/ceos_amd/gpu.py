@@ -22,6 +22,7 @@ PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB_PATH = os.path.join(PKG, "lib", "libceos_gpu.so")
 
 OK, ERR_DETJ, ERR_LOCAL_NR, ERR_SLOT, ERR_ARG, ERR_HIP = 0, -1, -2, -3, -4, -5
+PREC_BLOCK_JACOBI, PREC_TWO_LEVEL = 0, 1
 MODEL_GHL, MODEL_J2, MODEL_NH, MODEL_SVK, MODEL_CNH, MODEL_VISCO = 1, 2, 3, 5, 7, 12
 MODEL_EXT = 99  # FE^2 macro point: S and dS/dE per Gauss point from the caller
 MODEL_NH_Q1P0 = 4  # mean-dilatation Neo-Hookean (with ELEM_MEAN_DILATATION)
@@ -72,6 +73,7 @@ EXPORTS = [
     "ceos_linear_solve", "ceos_linear_solve_device", "ceos_newton_update_device", "ceos_set_linear_solver",
     "ceos_mesh_prepare", "ceos_evaluate_device_static", "ceos_assembly_info",
     "ceos_rve_state_save", "ceos_rve_state_load", "ceos_linear_solve_device_async",
+    "ceos_set_preconditioner", "ceos_preconditioner_info", "ceos_coarse_space",
 ]
 
 
@@ -123,6 +125,9 @@ def load_library(path=LIB_PATH):
     L.ceos_rve_state_save.argtypes = [vp, C.c_char_p]
     L.ceos_rve_state_load.argtypes = [vp, C.c_char_p]
     L.ceos_linear_solve_device_async.argtypes = [vp, vp, vp, vp, d, i, vp, vp]
+    L.ceos_set_preconditioner.argtypes = [vp, i, d, i]
+    L.ceos_preconditioner_info.argtypes = [vp, C.POINTER(i), C.POINTER(i)]
+    L.ceos_coarse_space.argtypes = [vp, vp, vp, vp]
     L.ceos_set_schedule.argtypes = [vp, i]
     L.ceos_set_pipeline.argtypes = [vp, i]
     L.ceos_set_assembly.argtypes = [vp, i]
@@ -425,6 +430,28 @@ class CeosMesh:
         _check(load_library().ceos_linear_solve_device(self._h, dK, db, dx, rtol, maxit, C.byref(it), C.byref(rr),
                                                        C.byref(fl), stream), "ceos_linear_solve_device")
         return it.value, rr.value, fl.value
+
+    def set_preconditioner(self, kind, aggregate_size=0.0, coarse_reuse=0):
+        """ceos_set_preconditioner: PREC_BLOCK_JACOBI or PREC_TWO_LEVEL (aggregate edge in mean
+        element edges, <= 0: 8; a coarse inverse serves coarse_reuse further solves)."""
+        _check(load_library().ceos_set_preconditioner(self._h, kind, float(aggregate_size), int(coarse_reuse)),
+               "ceos_set_preconditioner")
+
+    def preconditioner_info(self):
+        na, nc = C.c_int(), C.c_int()
+        _check(load_library().ceos_preconditioner_info(self._h, C.byref(na), C.byref(nc)),
+               "ceos_preconditioner_info")
+        return na.value, nc.value
+
+    def coarse_space(self):
+        """(aggregate of every node, P rows (nn, 3, 6), P^T K P (nc, nc)) of the last two-level solve."""
+        na, nc = self.preconditioner_info()
+        agg = np.zeros(self.nn, dtype=np.int32)
+        B = np.zeros((self.nn, 3, 6))
+        Ac = np.zeros((nc, nc))
+        _check(load_library().ceos_coarse_space(self._h, agg.ctypes.data, B.ctypes.data, Ac.ctypes.data),
+               "ceos_coarse_space")
+        return agg, B, Ac
 
     def linear_solve_device_async(self, dK, db, dx, dresult, rtol=1e-10, maxit=1000, stream=None):
         """ceos_linear_solve_device_async: PCG of exactly maxit enqueued iterations (early exit on

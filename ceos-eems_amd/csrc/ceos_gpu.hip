@@ -9,6 +9,8 @@
 //     and the first-touch bits of its 8 nodes (see element_kernel.hpp).
 // Evaluation: one launch of the fused element kernel per (colour, model) range, in stream order.
 #include <hip/hip_runtime.h>
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
 #include <stdint.h>
 #include <stdio.h>
 #include <string.h>
@@ -26,6 +28,7 @@
 #include "homogenize_kernel.hpp"
 #include "assembly_kernel.hpp"
 #include "system_kernel.hpp"
+#include "coarse_kernel.hpp"
 
 using namespace ceos;
 
@@ -246,6 +249,25 @@ struct ceos_handle {
     MrScalars* dmr_c = nullptr;
     int solver_method = CEOS_SOLVER_AUTO;
     int cg_check = 32;                          // iterations between host checks of the Krylov state
+    std::vector<int> slot_nbr_host;             // [width][nn] neighbour of every slot (host copy)
+    std::vector<uint32_t> slot_src_host;        // [width][nn]
+    std::vector<uint8_t> dof_mask_host;         // [3 nn] constrained DOFs of the load case
+    // two-level preconditioner (coarse_kernel.hpp)
+    int prec_kind = CEOS_PREC_BLOCK_JACOBI;
+    double agg_size = 8.0;                      // aggregate box edge, in mean element edge lengths
+    bool coarse_ready = false;                  // aggregates, basis and maps of the current load case
+    bool coarse_ok = false;                     // the last coarse factorisation succeeded
+    int n_agg = 0, n_coarse = 0, n_pair = 0, n_dead = 0;
+    int *dagg = nullptr, *dagg_ptr = nullptr, *dagg_nodes = nullptr, *dpair_ptr = nullptr, *ddead = nullptr;
+    int2* dpair_ij = nullptr;
+    uint32_t* dpair_ent = nullptr;
+    float *dbasis = nullptr, *dAci32 = nullptr;
+    double *dAc = nullptr, *dAci = nullptr, *dgc = nullptr, *dyc = nullptr;
+    int coarse_reuse = 0;                       // solves a coarse inverse serves after the one that built it
+    int coarse_uses = 0;                        // solves since the last build
+    bool coarse_built = false;
+    rocblas_handle blas = nullptr;
+    rocblas_int* dpotrf_info = nullptr;
 
     // phase timing: 3 events per evaluation (start, element kernels done, assembly done)
     std::vector<hipEvent_t> tev;
@@ -346,6 +368,12 @@ static void free_device(ceos_handle* h) {
     dfree(h->dMinv); dfree(h->dcg_r); dfree(h->dcg_z); dfree(h->dcg_q); dfree(h->dcg_p);
     dfree(h->dcg_b); dfree(h->dcg_x); dfree(h->dcg); dfree(h->dpart3);
     dfree(h->dmr[0]); dfree(h->dmr[1]); dfree(h->dmr[2]); dfree(h->dmr_c);
+    dfree(h->dagg); dfree(h->dagg_ptr); dfree(h->dagg_nodes); dfree(h->dpair_ptr); dfree(h->ddead);
+    dfree(h->dpair_ij); dfree(h->dpair_ent); dfree(h->dbasis); dfree(h->dAc); dfree(h->dAci); dfree(h->dgc);
+    dfree(h->dAci32);
+    dfree(h->dyc); dfree(h->dpotrf_info);
+    if (h->blas) (void)rocblas_destroy_handle(h->blas);
+    h->blas = nullptr;
 }
 
 static int upload_props(ceos_handle* h) {
@@ -1156,6 +1184,8 @@ static int ensure_system(ceos_handle* h) {
     HIP_OK(hipMemset(h->ddof_mask, 0, 3L * nn));
     HIP_OK(hipMemset(h->dnode_mask, 0, nn));
     h->sys_width = width;
+    h->slot_nbr_host.swap(nbr);
+    h->slot_src_host.swap(src);
     h->sys_ready = true;
     return CEOS_OK;
 }
@@ -1233,6 +1263,212 @@ static int ensure_minres(ceos_handle* h) {
     return CEOS_OK;
 }
 
+// ------------------------------------------------------------------------------------------
+// two-level preconditioner: aggregates, coarse basis and coarse-matrix maps (coarse_kernel.hpp)
+// ------------------------------------------------------------------------------------------
+constexpr int kMaxCoarse = 12288;  // coarse DOFs: the dense A_c and its inverse take 2 x 1.2 GB at most
+
+// Boxes of the reference coordinates with an edge of agg_size mean element edges (grown until the
+// coarse space fits kMaxCoarse); the 6 rigid-body modes of every aggregate about its centroid,
+// zero on constrained DOFs, orthonormalized (modified Gram-Schmidt, twice); the entries of every
+// coarse block pair.  Built on the host once per load case (it depends on the BC mask).
+static int coarse_setup(ceos_handle* h) {
+    const int nn = h->nn, width = h->sys_width;
+    if (width >= 128 || nn >= (1 << 25)) return CEOS_ERR_ARG;  // entry word: node 25 bits, slot 7 bits
+    const double* X = h->X.data();
+    double hmean = 0.0;
+    for (int e = 0; e < h->ne; ++e) {
+        const int a = h->conn0[8L * e], b = h->conn0[8L * e + 1];
+        double d2 = 0.0;
+        for (int k = 0; k < 3; ++k) d2 += (X[3L * b + k] - X[3L * a + k]) * (X[3L * b + k] - X[3L * a + k]);
+        hmean += std::sqrt(d2);
+    }
+    hmean /= std::max(h->ne, 1);
+    double lo[3], ext[3];
+    for (int k = 0; k < 3; ++k) {
+        double mn = X[k], mx = X[k];
+        for (int A = 1; A < nn; ++A) mn = std::min(mn, X[3L * A + k]), mx = std::max(mx, X[3L * A + k]);
+        lo[k] = mn;
+        ext[k] = mx - mn;
+    }
+    std::vector<int> agg(nn);
+    int na = 0;
+    for (double H = h->agg_size * hmean;; H *= 1.25) {
+        int nb[3];
+        for (int k = 0; k < 3; ++k) nb[k] = std::max(1, (int)std::floor(ext[k] / std::max(H, 1e-300) + 0.5));
+        std::vector<int> bin(nn);
+        for (int A = 0; A < nn; ++A) {
+            int c[3];
+            for (int k = 0; k < 3; ++k)
+                c[k] = ext[k] > 0.0 ? std::min(nb[k] - 1, (int)((X[3L * A + k] - lo[k]) / ext[k] * nb[k])) : 0;
+            bin[A] = c[0] + nb[0] * (c[1] + nb[1] * c[2]);
+        }
+        std::vector<int> id((size_t)nb[0] * nb[1] * nb[2], -1);
+        na = 0;
+        for (int A = 0; A < nn; ++A)
+            if (id[bin[A]] < 0) id[bin[A]] = 0;
+        for (auto& v : id)
+            if (v == 0) v = na++;
+        if (6L * na <= kMaxCoarse) {
+            for (int A = 0; A < nn; ++A) agg[A] = id[bin[A]];
+            break;
+        }
+    }
+    const int nc = 6 * na;
+    std::vector<int> aptr(na + 1, 0), anodes(nn);
+    for (int A = 0; A < nn; ++A) aptr[agg[A] + 1]++;
+    for (int I = 0; I < na; ++I) aptr[I + 1] += aptr[I];
+    {
+        std::vector<int> f(aptr.begin(), aptr.end() - 1);
+        for (int A = 0; A < nn; ++A) anodes[f[agg[A]]++] = A;
+    }
+    // coarse basis
+    std::vector<double> basis(18L * nn, 0.0);
+    std::vector<int> dead;
+    const std::vector<uint8_t>& dm = h->dof_mask_host;
+    for (int I = 0; I < na; ++I) {
+        const int n0 = aptr[I], n1 = aptr[I + 1], m = n1 - n0;
+        double c[3] = {0, 0, 0};
+        for (int k = n0; k < n1; ++k)
+            for (int d = 0; d < 3; ++d) c[d] += X[3L * anodes[k] + d] / m;
+        std::vector<double> M(18L * m);  // [node][dof][mode]
+        for (int k = 0; k < m; ++k) {
+            const int A = anodes[n0 + k];
+            const double dx = X[3L * A] - c[0], dy = X[3L * A + 1] - c[1], dz = X[3L * A + 2] - c[2];
+            const double v[3][6] = {{1, 0, 0, 0, dz, -dy}, {0, 1, 0, -dz, 0, dx}, {0, 0, 1, dy, -dx, 0}};
+            for (int i = 0; i < 3; ++i) {
+                const bool free_dof = dm.empty() || dm[3L * A + i] == 0;
+                for (int q = 0; q < 6; ++q) M[18L * k + 6 * i + q] = free_dof ? v[i][q] : 0.0;
+            }
+        }
+        for (int q = 0; q < 6; ++q) {
+            double n0q = 0.0;
+            for (long r = 0; r < 3L * m; ++r) n0q += M[6 * r + q] * M[6 * r + q];
+            n0q = std::sqrt(n0q);
+            for (int pass = 0; pass < 2; ++pass)
+                for (int l = 0; l < q; ++l) {
+                    double d = 0.0;
+                    for (long r = 0; r < 3L * m; ++r) d += M[6 * r + l] * M[6 * r + q];
+                    for (long r = 0; r < 3L * m; ++r) M[6 * r + q] -= d * M[6 * r + l];
+                }
+            double nq = 0.0;
+            for (long r = 0; r < 3L * m; ++r) nq += M[6 * r + q] * M[6 * r + q];
+            nq = std::sqrt(nq);
+            const bool alive = n0q > 0.0 && nq > 1e-8 * n0q;
+            for (long r = 0; r < 3L * m; ++r) M[6 * r + q] = alive ? M[6 * r + q] / nq : 0.0;
+            if (!alive) dead.push_back(6 * I + q);
+        }
+        for (int k = 0; k < m; ++k)
+            for (int t = 0; t < 18; ++t) basis[18L * anodes[n0 + k] + t] = M[18L * k + t];
+    }
+    // pair lists: per aggregate I, the (node, slot) entries of its nodes grouped by the neighbour's
+    // aggregate J (ascending J, then node, then slot)
+    std::vector<int> pptr(1, 0);
+    std::vector<int2> pij;
+    std::vector<uint32_t> pent;
+    pent.reserve((size_t)width * nn);
+    std::vector<std::pair<int, uint32_t>> ent;
+    for (int I = 0; I < na; ++I) {
+        ent.clear();
+        for (int k = aptr[I]; k < aptr[I + 1]; ++k) {
+            const int A = anodes[k];
+            for (int j = 0; j < width; ++j) {
+                const size_t t = (size_t)j * nn + A;
+                if (h->slot_src_host[t] == kSlotPad) continue;
+                ent.push_back({agg[h->slot_nbr_host[t]], (uint32_t)A | ((uint32_t)j << 25)});
+            }
+        }
+        std::stable_sort(ent.begin(), ent.end(), [](const std::pair<int, uint32_t>& a,
+                                                    const std::pair<int, uint32_t>& b) { return a.first < b.first; });
+        for (size_t i = 0; i < ent.size(); ++i) {
+            if (i == 0 || ent[i].first != ent[i - 1].first) {
+                if (i) pptr.push_back((int)pent.size());
+                pij.push_back(make_int2(I, ent[i].first));
+            }
+            pent.push_back(ent[i].second);
+        }
+        if (!ent.empty()) pptr.push_back((int)pent.size());
+    }
+    if (pent.size() >= (size_t)INT32_MAX) return CEOS_ERR_ARG;
+    HIP_OK(dalloc(h->dagg, nn));
+    HIP_OK(dalloc(h->dagg_ptr, na + 1));
+    HIP_OK(dalloc(h->dagg_nodes, nn));
+    std::vector<float> basis32(basis.begin(), basis.end());
+    HIP_OK(dalloc(h->dbasis, basis32.size()));
+    HIP_OK(dalloc(h->dpair_ptr, pptr.size()));
+    HIP_OK(dalloc(h->dpair_ij, pij.size()));
+    HIP_OK(dalloc(h->dpair_ent, pent.size()));
+    HIP_OK(dalloc(h->ddead, dead.size()));
+    HIP_OK(dalloc(h->dAc, (size_t)nc * nc));
+    HIP_OK(dalloc(h->dAci, (size_t)nc * nc));
+    HIP_OK(dalloc(h->dAci32, (size_t)nc * nc));
+    HIP_OK(dalloc(h->dgc, nc));
+    HIP_OK(dalloc(h->dyc, nc));
+    HIP_OK(dalloc(h->dpotrf_info, 1));
+    HIP_OK(hipMemcpy(h->dagg, agg.data(), sizeof(int) * nn, hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(h->dagg_ptr, aptr.data(), sizeof(int) * (na + 1), hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(h->dagg_nodes, anodes.data(), sizeof(int) * nn, hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(h->dbasis, basis32.data(), sizeof(float) * basis32.size(), hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(h->dpair_ptr, pptr.data(), sizeof(int) * pptr.size(), hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(h->dpair_ij, pij.data(), sizeof(int2) * pij.size(), hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(h->dpair_ent, pent.data(), sizeof(uint32_t) * pent.size(), hipMemcpyHostToDevice));
+    if (!dead.empty()) HIP_OK(hipMemcpy(h->ddead, dead.data(), sizeof(int) * dead.size(), hipMemcpyHostToDevice));
+    if (!h->blas && rocblas_create_handle(&h->blas) != rocblas_status_success) return CEOS_ERR_HIP;
+    h->n_agg = na;
+    h->n_coarse = nc;
+    h->n_pair = (int)pij.size();
+    h->n_dead = (int)dead.size();
+    h->coarse_ready = true;
+    h->coarse_built = false;
+    return CEOS_OK;
+}
+
+// A_c = P^T K P of the BC-applied K and its inverse (Cholesky, rocSOLVER); on a factorisation
+// that fails (A_c not positive definite) the solve falls back to block Jacobi.  Synchronises s.
+static int coarse_operator(ceos_handle* h, const void* dK, hipStream_t s) {
+    const int nc = h->n_coarse;
+    const SysGeom g = sys_geom(h);
+    // blocks of aggregate pairs that share no nonzero of K are zero and written by no pair
+    HIP_OK(hipMemsetAsync(h->dAc, 0, sizeof(double) * nc * nc, s));
+    if (h->precision == CEOS_K_FP32_STORED)
+        hipLaunchKernelGGL(k_coarse_assemble<float>, dim3(h->n_pair), dim3(kCoarseBlock), 0, s, g, (const float*)dK,
+                           (const float*)h->dbasis, (const int*)h->dpair_ptr, (const int2*)h->dpair_ij,
+                           (const uint32_t*)h->dpair_ent, nc, h->dAc);
+    else
+        hipLaunchKernelGGL(k_coarse_assemble<double>, dim3(h->n_pair), dim3(kCoarseBlock), 0, s, g, (const double*)dK,
+                           (const float*)h->dbasis, (const int*)h->dpair_ptr, (const int2*)h->dpair_ij,
+                           (const uint32_t*)h->dpair_ent, nc, h->dAc);
+    if (h->n_dead)
+        hipLaunchKernelGGL(k_coarse_unit_diag, dim3((h->n_dead + 255) / 256), dim3(256), 0, s, h->n_dead,
+                           (const int*)h->ddead, nc, h->dAc);
+    HIP_OK(hipGetLastError());
+    HIP_OK(hipMemcpyAsync(h->dAci, h->dAc, sizeof(double) * nc * nc, hipMemcpyDeviceToDevice, s));
+    if (rocblas_set_stream(h->blas, s) != rocblas_status_success) return CEOS_ERR_HIP;
+    rocblas_int info = 0;
+    bool ok = rocsolver_dpotrf(h->blas, rocblas_fill_upper, nc, h->dAci, nc, h->dpotrf_info) == rocblas_status_success;
+    if (ok) {
+        HIP_OK(hipMemcpyAsync(&info, h->dpotrf_info, sizeof(info), hipMemcpyDeviceToHost, s));
+        HIP_OK(hipStreamSynchronize(s));
+        ok = info == 0;
+    }
+    if (ok) {
+        ok = rocsolver_dpotri(h->blas, rocblas_fill_upper, nc, h->dAci, nc, h->dpotrf_info) == rocblas_status_success;
+        if (ok) {
+            HIP_OK(hipMemcpyAsync(&info, h->dpotrf_info, sizeof(info), hipMemcpyDeviceToHost, s));
+            HIP_OK(hipStreamSynchronize(s));
+            ok = info == 0;
+        }
+    }
+    if (ok) {
+        const long nt = (long)nc * nc;
+        hipLaunchKernelGGL(k_coarse_inverse_f32, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, s, nc,
+                           (const double*)h->dAci, h->dAci32);
+        HIP_OK(hipGetLastError());
+    }
+    h->coarse_ok = ok;
+    return CEOS_OK;
+}
+
 // x = K^-1 b by block-Jacobi PCG from x = 0; K (upper CSR, the handle's precision) after the
 // boundary conditions.  Synchronises s every cg_check iterations to read the state.
 // the SELL copy of the BC-applied K and the block-Jacobi inverses
@@ -1252,13 +1488,40 @@ static void build_operator(ceos_handle* h, const void* dK, hipStream_t s) {
     }
 }
 
+// the coarse correction of the two-level preconditioner: z = D^-1 r + P A_c^-1 P^T r and the
+// partials of r.z, r.r (one per update workgroup); at the start also p = z
+static void coarse_apply(ceos_handle* h, int start, hipStream_t s) {
+    const int nupd = std::min((h->nn + kSysBlock - 1) / kSysBlock, kCgUpdBlocks);
+    hipLaunchKernelGGL(k_cg_restrict, dim3(h->n_agg), dim3(kSysBlock), 0, s, (const CgScalars*)h->dcg,
+                       (const int*)h->dagg_ptr, (const int*)h->dagg_nodes, (const float*)h->dbasis,
+                       (const double*)h->dcg_r, h->dgc);
+    hipLaunchKernelGGL(k_cg_coarse_solve, dim3((h->n_coarse + 3) / 4), dim3(256), 0, s, (const CgScalars*)h->dcg,
+                       h->n_coarse, (const float*)h->dAci32, (const double*)h->dgc, h->dyc);
+    hipLaunchKernelGGL(k_cg_prolong, dim3(nupd), dim3(kSysBlock), 0, s, h->nn, (const CgScalars*)h->dcg, start,
+                       (const double*)h->dMinv, (const int*)h->dagg, (const float*)h->dbasis,
+                       (const double*)h->dyc, (const double*)h->dcg_r, h->dcg_z, h->dcg_p, h->dpart, h->dpart2);
+}
+
+static bool two_level_on(const ceos_handle* h) {
+    return h->prec_kind == CEOS_PREC_TWO_LEVEL && h->coarse_ready && h->coarse_built && h->coarse_ok;
+}
+
 // PCG start from x = 0: state scalars (tolerance, iteration limit) set by a kernel, so the
 // sequence holds no host copy and may be captured
-static void pcg_start(ceos_handle* h, const double* db, double* dx, double rtol, int maxit, hipStream_t s) {
+static void pcg_start(ceos_handle* h, const double* db, double* dx, double rtol, int maxit, hipStream_t s,
+                      bool two_level = false) {
     const int nn = h->nn;
     const int nblk = (nn + kSysBlock - 1) / kSysBlock;
-    hipLaunchKernelGGL(k_cg_init, dim3(nblk), dim3(kSysBlock), 0, s, nn, (const double*)h->dMinv, db, dx, h->dcg_r,
-                       h->dcg_z, h->dcg_p, h->dpart, h->dpart2);
+    if (two_level) {
+        hipLaunchKernelGGL(k_cg_init<true>, dim3(nblk), dim3(kSysBlock), 0, s, nn, (const double*)h->dMinv, db, dx,
+                           h->dcg_r, h->dcg_z, h->dcg_p, h->dpart, h->dpart2, h->dcg);
+        coarse_apply(h, 1, s);
+        hipLaunchKernelGGL(k_cg_init_final, dim3(1), dim3(kSysBlock), 0, s, (const double*)h->dpart,
+                           (const double*)h->dpart2, std::min(nblk, kCgUpdBlocks), rtol * rtol, maxit, h->dcg);
+        return;
+    }
+    hipLaunchKernelGGL(k_cg_init<false>, dim3(nblk), dim3(kSysBlock), 0, s, nn, (const double*)h->dMinv, db, dx, h->dcg_r,
+                       h->dcg_z, h->dcg_p, h->dpart, h->dpart2, (CgScalars*)nullptr);
     hipLaunchKernelGGL(k_cg_init_final, dim3(1), dim3(kSysBlock), 0, s, (const double*)h->dpart,
                        (const double*)h->dpart2, nblk, rtol * rtol, maxit, h->dcg);
 }
@@ -1267,7 +1530,7 @@ static void pcg_start(ceos_handle* h, const double* db, double* dx, double rtol,
 // node block); alpha, x, r, z (partials of r.z and r.r in dpart, dpart2, one per update
 // workgroup); beta, test, p.  r.z of iteration k sits in rz[k & 1].  After the test has set the
 // stop flag the kernels return at once, so iterations past convergence change nothing.
-static void pcg_iterations(ceos_handle* h, double* dx, int k0, int n, hipStream_t s) {
+static void pcg_iterations(ceos_handle* h, double* dx, int k0, int n, hipStream_t s, bool two_level = false) {
     const SysGeom g = sys_geom(h);
     const int nn = h->nn;
     const int nblk = (nn + kSysBlock - 1) / kSysBlock;
@@ -1282,9 +1545,16 @@ static void pcg_iterations(ceos_handle* h, double* dx, int k0, int n, hipStream_
             hipLaunchKernelGGL((k_cg_spmv<double, kCgSlots64>), dim3(nblk), dim3(kSysBlock), 0, s, g,
                                (const double*)h->dell, (const CgScalars*)h->dcg, (const double*)h->dcg_p, h->dcg_q,
                                h->dpart3);
-        hipLaunchKernelGGL(k_cg_update, dim3(nupd), dim3(kSysBlock), 0, s, nn, par, (const double*)h->dMinv, h->dcg,
-                           (const double*)h->dpart3, nblk, (const double*)h->dcg_p, (const double*)h->dcg_q, dx,
-                           h->dcg_r, h->dcg_z, h->dpart, h->dpart2);
+        if (two_level) {
+            hipLaunchKernelGGL(k_cg_update<true>, dim3(nupd), dim3(kSysBlock), 0, s, nn, par, (const double*)h->dMinv,
+                               h->dcg, (const double*)h->dpart3, nblk, (const double*)h->dcg_p,
+                               (const double*)h->dcg_q, dx, h->dcg_r, h->dcg_z, h->dpart, h->dpart2);
+            coarse_apply(h, 0, s);
+        } else {
+            hipLaunchKernelGGL(k_cg_update<false>, dim3(nupd), dim3(kSysBlock), 0, s, nn, par, (const double*)h->dMinv,
+                               h->dcg, (const double*)h->dpart3, nblk, (const double*)h->dcg_p,
+                               (const double*)h->dcg_q, dx, h->dcg_r, h->dcg_z, h->dpart, h->dpart2);
+        }
         hipLaunchKernelGGL(k_cg_pupdate, dim3(nupd), dim3(kSysBlock), 0, s, nn, par, h->dcg, (const double*)h->dpart,
                            (const double*)h->dpart2, nupd, (const double*)h->dcg_z, h->dcg_p);
     }
@@ -1293,14 +1563,15 @@ static void pcg_iterations(ceos_handle* h, double* dx, int k0, int n, hipStream_
 // block-Jacobi PCG from x = 0.  Synchronises s every cg_check iterations to read the state.
 static int pcg_run(ceos_handle* h, const double* db, double* dx, double rtol, int maxit, hipStream_t s, int* iters,
                    double* relres, int* flag) {
-    pcg_start(h, db, dx, rtol, maxit, s);
+    const bool tl = two_level_on(h);
+    pcg_start(h, db, dx, rtol, maxit, s, tl);
     CgScalars c;
     int k = 0;
     for (;;) {
         HIP_OK(hipMemcpyAsync(&c, h->dcg, sizeof(c), hipMemcpyDeviceToHost, s));
         HIP_OK(hipStreamSynchronize(s));
         if (c.done) break;
-        pcg_iterations(h, dx, k, h->cg_check, s);
+        pcg_iterations(h, dx, k, h->cg_check, s, tl);
         k += h->cg_check;
         HIP_OK(hipGetLastError());
     }
@@ -1375,6 +1646,19 @@ static int linear_solve(ceos_handle* h, const void* dK, const double* db, double
     int rc = ensure_solver(h);
     if (rc) return rc;
     build_operator(h, dK, s);
+    if (h->prec_kind == CEOS_PREC_TWO_LEVEL && h->solver_method != CEOS_SOLVER_MINRES) {
+        if (!h->coarse_ready) {
+            rc = coarse_setup(h);
+            if (rc) return rc;
+        }
+        if (!h->coarse_built || h->coarse_uses > h->coarse_reuse) {
+            rc = coarse_operator(h, dK, s);
+            if (rc) return rc;
+            h->coarse_built = true;
+            h->coarse_uses = 0;
+        }
+        h->coarse_uses++;
+    }
     int it = 0, fl = 0;
     double rr = 0.0;
     if (h->solver_method != CEOS_SOLVER_MINRES) {
@@ -2016,6 +2300,8 @@ int ceos_bc_set(ceos_handle* h, int n_presc, const int* presc_dof, int n_fixed, 
     HIP_OK(hipMemcpy(h->dnode_mask, nm.data(), h->nn, hipMemcpyHostToDevice));
     h->n_presc = n_presc;
     h->n_fixed = n_fixed;
+    if (dm != h->dof_mask_host) h->coarse_ready = h->coarse_built = false;  // P vanishes on constrained DOFs
+    h->dof_mask_host.swap(dm);
     // the linear solver's buffers too, so that the solve of a Newton step may be captured
     return ensure_solver(h);
 }
@@ -2062,6 +2348,38 @@ int ceos_set_linear_solver(ceos_handle* h, int method) {
     if (!h || (method != CEOS_SOLVER_AUTO && method != CEOS_SOLVER_PCG && method != CEOS_SOLVER_MINRES))
         return CEOS_ERR_ARG;
     h->solver_method = method;
+    return CEOS_OK;
+}
+
+int ceos_set_preconditioner(ceos_handle* h, int kind, double aggregate_size, int coarse_reuse) {
+    if (!h || (kind != CEOS_PREC_BLOCK_JACOBI && kind != CEOS_PREC_TWO_LEVEL) || coarse_reuse < 0)
+        return CEOS_ERR_ARG;
+    const double a = aggregate_size > 0.0 ? aggregate_size : 8.0;
+    if (a != h->agg_size) h->coarse_ready = false;
+    h->prec_kind = kind;
+    h->agg_size = a;
+    h->coarse_reuse = coarse_reuse;
+    h->coarse_built = false;
+    return CEOS_OK;
+}
+
+int ceos_preconditioner_info(ceos_handle* h, int* n_aggregates, int* n_coarse) {
+    if (!h) return CEOS_ERR_ARG;
+    if (n_aggregates) *n_aggregates = h->coarse_ready ? h->n_agg : 0;
+    if (n_coarse) *n_coarse = h->coarse_ready ? h->n_coarse : 0;
+    return CEOS_OK;
+}
+
+int ceos_coarse_space(ceos_handle* h, int* agg, double* basis, double* Ac) {
+    if (!h || !h->coarse_ready) return CEOS_ERR_ARG;
+    HIP_OK(hipDeviceSynchronize());
+    if (agg) HIP_OK(hipMemcpy(agg, h->dagg, sizeof(int) * h->nn, hipMemcpyDeviceToHost));
+    if (basis) {
+        std::vector<float> b32(18L * h->nn);
+        HIP_OK(hipMemcpy(b32.data(), h->dbasis, sizeof(float) * b32.size(), hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < b32.size(); ++i) basis[i] = b32[i];
+    }
+    if (Ac) HIP_OK(hipMemcpy(Ac, h->dAc, sizeof(double) * (size_t)h->n_coarse * h->n_coarse, hipMemcpyDeviceToHost));
     return CEOS_OK;
 }
 

@@ -343,11 +343,24 @@ __device__ inline void reduce2(const double* pa, const double* pb, int n, double
     b = lb[0];
 }
 
-// x = 0, r = b, z = p = M^-1 b; partials of r.z and r.r
+// x = 0, r = b, z = p = M^-1 b; partials of r.z and r.r.  TWOLEVEL: x = 0, r = b only, and the
+// stop flag cleared (the coarse correction, coarse_kernel.hpp, forms z, p and the partials)
+template <bool TWOLEVEL = false>
 __global__ __launch_bounds__(kSysBlock) void k_cg_init(int nn, const double* __restrict__ Minv, const double* __restrict__ b,
                                                        double* x, double* r, double* z, double* p,
-                                                       double* part_s, double* part_m) {
+                                                       double* part_s, double* part_m, CgScalars* c = nullptr) {
     const int A = blockIdx.x * kSysBlock + threadIdx.x;
+    if constexpr (TWOLEVEL) {
+        if (A == 0) c->done = 0;
+        if (A < nn) {
+#pragma unroll
+            for (int i = 0; i < 3; ++i) {
+                x[3L * A + i] = 0.0;
+                r[3L * A + i] = b[3L * A + i];
+            }
+        }
+        return;
+    }
     double rz = 0.0, rr = 0.0;
     if (A < nn) {
         double rv[3], zv[3];
@@ -454,7 +467,9 @@ __global__ __launch_bounds__(kSysBlock) void k_cg_spmv(SysGeom g, const VT* __re
 }
 
 // alpha = r.z / p.q (p.q reduced by every workgroup alike); x += alpha p, r -= alpha q,
-// z = M^-1 r; partials of r.z and r.r
+// z = M^-1 r; partials of r.z and r.r.  TWOLEVEL: x and r only (the coarse correction forms z and
+// the partials)
+template <bool TWOLEVEL = false>
 __global__ __launch_bounds__(kSysBlock) void k_cg_update(int nn, int par, const double* __restrict__ Minv,
                                                          CgScalars* c, const double* pq_part, int nblk,
                                                          const double* __restrict__ p, const double* __restrict__ q,
@@ -496,6 +511,7 @@ __global__ __launch_bounds__(kSysBlock) void k_cg_update(int nn, int par, const 
             rv[i] = rv0[i] - alpha * qv[i];
             r[3L * A + i] = rv[i];
         }
+        if constexpr (TWOLEVEL) continue;
         zv[0] = mi[0] * rv[0] + mi[1] * rv[1] + mi[2] * rv[2];  // precond() on the loaded inverse
         zv[1] = mi[1] * rv[0] + mi[3] * rv[1] + mi[4] * rv[2];
         zv[2] = mi[2] * rv[0] + mi[4] * rv[1] + mi[5] * rv[2];
@@ -506,7 +522,7 @@ __global__ __launch_bounds__(kSysBlock) void k_cg_update(int nn, int par, const 
             rr += rv[i] * rv[i];
         }
     }
-    block_sum2(rz, rr, part_s, part_m);
+    if constexpr (!TWOLEVEL) block_sum2(rz, rr, part_s, part_m);
 }
 
 // beta = r.z (new) / r.z (old), the convergence test, p = z + beta p; workgroup 0 records the

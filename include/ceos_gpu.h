@@ -306,6 +306,31 @@ int ceos_residual_bc_device(ceos_handle *h, int it, const double *dU, const doub
 #define CEOS_SOLVER_MINRES 2
 int ceos_set_linear_solver(ceos_handle *h, int method);
 int ceos_set_solver_precision(ceos_handle *h, int mode);
+
+/* Preconditioner of the PCG (no reference counterpart: the reference factorises with PARDISO,
+ * ModPardisoSolver.f90:343-380).
+ *   CEOS_PREC_BLOCK_JACOBI (default): the inverse 3x3 diagonal block of every node.
+ *   CEOS_PREC_TWO_LEVEL: block Jacobi plus a coarse correction, M^-1 = D^-1 + P (P^T K P)^-1 P^T,
+ *   P the 6 rigid-body modes of aggregates of nodes (boxes of the reference coordinates with an
+ *   edge of aggregate_size mean element edges, <= 0: 8; grown until the coarse space has at most
+ *   12,288 DOFs), zero on the constrained DOFs and orthonormalized per aggregate (stored FP32).
+ *   The aggregates are built at the first solve after a ceos_bc_set that changes the constrained
+ *   DOFs or a change of aggregate_size.  A solve assembles P^T K P (dense, FP64) and inverts it
+ *   (Cholesky, rocSOLVER; the inverse is applied in FP32); that coarse inverse then also serves
+ *   the next coarse_reuse solves (0: every solve builds its own; a preconditioner built from an
+ *   earlier tangent of the same load step is still symmetric positive definite, PCG only needs
+ *   that).  If the factorisation fails (P^T K P not positive definite) the solves use block
+ *   Jacobi.  PCG only: MINRES, and ceos_linear_solve_device_async, always use block Jacobi.  All
+ *   reductions run in a fixed order (bitwise reproducible).
+ * ceos_preconditioner_info: aggregates and coarse DOFs of the last two-level setup (0 before).
+ * ceos_coarse_space: the aggregate of every node (nn ints), the rows of P (nn x 3 x 6 doubles,
+ * row-major per node) and the last assembled P^T K P (n_coarse^2 doubles, row-major); any of them
+ * may be NULL.  For checks; synchronises the device. */
+#define CEOS_PREC_BLOCK_JACOBI 0
+#define CEOS_PREC_TWO_LEVEL 1
+int ceos_set_preconditioner(ceos_handle *h, int kind, double aggregate_size, int coarse_reuse);
+int ceos_preconditioner_info(ceos_handle *h, int *n_aggregates, int *n_coarse);
+int ceos_coarse_space(ceos_handle *h, int *agg, double *basis, double *Ac);
 int ceos_linear_solve(ceos_handle *h, const double *b, double *x, double rtol, int maxit, int *iters,
                       double *relres, int *flag);
 int ceos_linear_solve_device(ceos_handle *h, const void *dK, const double *db, double *dx, double rtol, int maxit,
