@@ -435,7 +435,7 @@ def newton_step_device(mesh, X, ux, tol, rtol, maxit=15, stream=None):
 def c2_history(n_cells=N_CELLS, n_steps=5, ux_total=0.2, tol=1e-8, pcg_rtol=1e-6, maxit=15):
     """C2's load history on the device: the 80^3 perturbed cube, compressible Neo-Hookean, x=0 fixed,
     u_x on x=1 ramped to 0.2 (stretch 1.2) in 5 steps, each solved by full Newton (evaluation,
-    Dirichlet BC + max-abs norm, PCG, update) to tol times the step's first norm.  Run twice: FP64
+    Dirichlet BC + max-abs norm, two-level PCG, update) to tol times the step's first norm.  Run twice: FP64
     tangent and FP64 PCG values (the reference's arithmetic), and the FP32-stored tangent with FP32
     PCG values, FP64 residual (inexact Newton).  Per step: Newton and PCG iterations, norms, time;
     the converged displacements of the two runs are compared step by step."""
@@ -448,6 +448,7 @@ def c2_history(n_cells=N_CELLS, n_steps=5, ux_total=0.2, tol=1e-8, pcg_rtol=1e-6
         m = gpu.CeosMesh(X, conn, precision=prec)
         m.set_material(1, gpu.MODEL_CNH, [MU, LAM])
         m.set_solver_precision(prec)
+        m.set_preconditioner(gpu.PREC_TWO_LEVEL, 0.0, 2)
         m.set_bc(presc, fixed)
         _, nnz = m.info()
         D = hiprt.DeviceArray
@@ -572,6 +573,7 @@ def j2_bench(steps=100, warmup=5, n_cells=64, n_hist=40, n_time=10, parity_steps
     m = gpu.CeosMesh(X, conn, nl_analysis=False)
     m.set_material(1, gpu.MODEL_J2, J2_PROPS)
     m.set_solver_precision(gpu.K_FP32_STORED if solver_precision is None else solver_precision)
+    m.set_preconditioner(gpu.PREC_TWO_LEVEL, 0.0, 2)
     fixed, presc = meshgen.stretch_bc(X)
     m.set_bc(presc, fixed)
     _, nnz = m.info()
