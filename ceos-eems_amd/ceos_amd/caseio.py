@@ -8,22 +8,26 @@ own input files through the library.
                             ModReadInputFile.f90:1396-1714: COORDINATES, MATERIAL ID,
                             CONNECTIVITY (hex8), FIXED SUPPORTS, Nodal Load, Nodal Displacement
                             (per node three flag / table pairs)
+  Mesh.cdb (Preprocessor = HyperMesh)
+                            ReadMeshHyperMesh / ReadBoundaryConditionsHyperMesh,
+                            ModReadInputFile.f90:909-1395: the ANSYS deck HyperMesh writes
+                            (NUMOFF, NBLOCK, ET 185 = hex8, EBLOCK), then one load collector per
+                            table ('Fixed Supports', or the table's name for its D and F lines)
   <table>.tab               ModLoadHistoryData.f90:122-223: a header line, then 'time value' rows,
                             linearly interpolated
   Time_Discretization.dat   load cases: initial time, final time, number of steps
-  Post_Processing.post      node probes of the displacements (FEM/PostProcessing/ModProbe.f90)
+  Post_Processing.post      the probes (ceos_amd.probes, FEM/PostProcessing/ModProbe.f90)
   FEMAnalysis.result        WriteFEMResults (FEM/ModFEMAnalysis.f90:1094-1129), one record per
-                            converged (sub)step after a 'Total Number of DOF' line; read back for
-                            the probes as ModExportResultFile.f90:340-420 does
-  probe files               ModProbe.f90:170-206: header ' Time                    Value', then
-                            rows in E23.15
+                            converged (sub)step after a 'Total Number of DOF' line; replayed for
+                            the probes as ModExportResultFile.f90:254-440 does
 Material names map to the library's model ids as ModConstitutiveModelLibrary.f90:492-556 pairs
 them with the element technology; option names follow ReadMaterialParameters_* of each model.
 """
 import os
-import re
 
 import numpy as np
+
+from . import probes as _probes
 
 # (name, element technology) -> model id, option names in the order of the library's props
 MATERIALS = {
@@ -129,8 +133,9 @@ def read_settings(path):
             continue
         i += 1
     mb = _options(sec.get("meshandboundaryconditions", []))
-    if _key(mb.get("preprocessor", "gid12")) != "gid12":
-        raise CaseError("only GiD 12 meshes are read")
+    pre = _key(mb.get("preprocessor", "gid12"))
+    if pre not in ("gid12", "hypermesh"):
+        raise CaseError("only GiD 12 and HyperMesh meshes are read")
     return {
         "nl_analysis": nl,
         "element_technology": tech,
@@ -139,6 +144,7 @@ def read_settings(path):
         "newton_tol": _num(ns.get("tol", "1e-3")),
         "newton_maxiter": int(_num(ns.get("maxiter", "10"))),
         "materials": mats,
+        "preprocessor": pre,
         "mesh_file": mb.get("meshfile", "Mesh.dat"),
         "time_file": mb.get("timediscretizationfile", "Time_Discretization.dat"),
     }
@@ -199,6 +205,95 @@ def read_gid_mesh(path):
             "mat": np.array(mat, dtype=np.int32), "fixed": fixed, "nodal_load": loads, "nodal_disp": disps}
 
 
+def read_hypermesh_mesh(path):
+    """The mesh and boundary conditions of a HyperMesh ANSYS deck, in the dictionary
+    read_gid_mesh returns.  As ReadMeshHyperMesh (ModReadInputFile.f90:909-1080): the NUMOFF
+    lines give the node and element counts; NBLOCK is followed by a format line and one line per
+    node, numbered in file order, whose 4th to 6th blank-separated fields are x, y, z; ET type 185
+    is the hex8; EBLOCK is followed by a format line and one line per element up to '-1', with
+    the material id in field 1, the node count in field 9, the element id in field 11 and the
+    nodes from field 12.  From the '!! Loadstep Data' line on, as
+    ReadBoundaryConditionsHyperMesh (:1182-1395): every '!!HMNAME LOADCOL' line is followed by the
+    collector's name in double quotes, and every later line of four comma-separated fields
+    belongs to it; 'Fixed Supports' lists the fixed DOFs (D, node, UX|UY|UZ), collectors named
+    'boundary...' (RVE node sets) and 'reaction...' are skipped, and any other collector names the
+    table (<name>.tab) of its prescribed displacements (D lines) and nodal loads (F, node,
+    FX|FY|FZ)."""
+    lines = open(path, encoding="latin-1").read().splitlines()
+    nn = ne = None
+    node_lines, elems, etype = None, {}, None
+    i = 0
+    while i < len(lines):
+        line = lines[i]
+        f = line.split(",")
+        if _key(line) == "!!loadstepdata":
+            i += 1
+            break
+        if len(f) >= 3 and _key(f[0]) == "numoff":
+            if _key(f[1]) == "node":
+                nn = int(f[2])
+            elif _key(f[1]) == "elem":
+                ne = int(f[2])
+        elif len(f) >= 3 and _key(f[0]) == "nblock":
+            if nn is None:
+                raise CaseError("NBLOCK before NUMOFF,NODE in %s" % path)
+            node_lines = lines[i + 2:i + 2 + nn]
+            i += 2 + nn
+            continue
+        elif len(f) >= 3 and _key(f[0]) == "et":
+            etype = int(f[2])
+            if etype != 185:
+                raise CaseError("ANSYS element type %d is not supported: only 185 (hex8)" % etype)
+        elif len(f) >= 3 and _key(f[0]) == "eblock":
+            i += 2
+            while i < len(lines) and lines[i].split() and lines[i].split()[0] != "-1":
+                p = [int(v) for v in lines[i].split()]
+                if p[8] != 8:
+                    raise CaseError("element %d has %d nodes: only hex8 is supported" % (p[10], p[8]))
+                elems[p[10]] = (p[0], p[11:19])
+                i += 1
+        i += 1
+    if nn is None or ne is None or node_lines is None or etype is None:
+        raise CaseError("no NUMOFF, NBLOCK or ET line in %s" % path)
+    if len(node_lines) != nn or sorted(elems) != list(range(1, ne + 1)):
+        raise CaseError("%s: the node or element blocks do not match NUMOFF" % path)
+    coords = np.array([[_num(v) for v in ln.split()[3:6]] for ln in node_lines])
+    conn = np.array([elems[k][1] for k in range(1, ne + 1)], dtype=np.int32)
+    mat = np.array([elems[k][0] for k in range(1, ne + 1)], dtype=np.int32)
+    fixed, loads, disps = {}, [], []
+    name = None
+    dof_of = {"ux": 1, "uy": 2, "uz": 3, "fx": 1, "fy": 2, "fz": 3}
+    while i < len(lines):
+        line = lines[i]
+        if _key(line) == "!!hmnameloadcol" and i + 1 < len(lines):
+            q = lines[i + 1].split('"')
+            name = q[1] if len(q) > 1 else ""
+            i += 2
+            continue
+        f = [v.strip() for v in line.split(",")]
+        if len(f) == 4 and name is not None:
+            k = _key(name)
+            first = name.split()[0].lower() if name.split() else ""
+            if first in ("boundary", "reaction"):
+                pass
+            elif k == "fixedsupports":
+                d = dof_of.get(f[2].lower())
+                if d is None or d > 3 or f[2].lower()[0] != "u":
+                    raise CaseError("DOF %r not identified in %s" % (f[2], path))
+                fixed.setdefault(int(f[1]), [0, 0, 0])[d - 1] = 1
+            elif _key(f[0]) == "d":
+                if f[2].lower() not in ("ux", "uy", "uz"):
+                    raise CaseError("DOF %r not identified in %s" % (f[2], path))
+                disps.append([int(f[1]), dof_of[f[2].lower()], name])
+            elif _key(f[0]) == "f":
+                if f[2].lower() not in ("fx", "fy", "fz"):
+                    raise CaseError("DOF %r not identified in %s" % (f[2], path))
+                loads.append([int(f[1]), dof_of[f[2].lower()], name])
+        i += 1
+    return {"coords": coords, "conn": conn, "mat": mat, "fixed": [[n] + fixed[n] for n in sorted(fixed)],
+            "nodal_load": loads, "nodal_disp": disps}
+
+
 def read_table(path):
     rows = []
     with open(path, encoding="latin-1") as f:
@@ -216,28 +311,6 @@ def read_time_discretization(path):
     return [{"t0": _num(r[1]), "t1": _num(r[2]), "steps": int(r[3])} for r in t[3:3 + n]]
 
 
-def read_probes(path):
-    """Node displacement probes of a Post_Processing.post: [{file, node, component}]."""
-    out = []
-    if not os.path.exists(path):
-        return out
-    blocks = re.split(r"(?im)^\s*end\s+probe\s*$", open(path, encoding="latin-1").read())
-    for b in blocks:
-        m = re.search(r"(?im)^\s*probe\s*$", b)
-        if not m:
-            continue
-        opts = _options(_lines_from_text(b[m.end():]))
-        if _key(opts.get("location", "")) != "node" or _key(opts.get("variablename", "")) != "displacements":
-            continue
-        out.append({"file": opts["filename"], "node": int(_num(opts["node"])),
-                    "component": int(_num(opts["components"]))})
-    return out
-
-
-def _lines_from_text(text):
-    return [ln.split("!", 1)[0].strip() for ln in text.splitlines() if ln.split("!", 1)[0].strip()]
-
-
 def interp(table, t):
     ts = [p[0] for p in table]
     vs = [p[1] for p in table]
@@ -250,9 +323,13 @@ class Case:
     def __init__(self, directory, settings="Settings.dat"):
         self.dir = directory
         self.settings = read_settings(os.path.join(directory, settings))
-        self.mesh = read_gid_mesh(os.path.join(directory, self.settings["mesh_file"]))
+        mesh_path = os.path.join(directory, self.settings["mesh_file"])
+        if self.settings["preprocessor"] == "hypermesh":
+            self.mesh = read_hypermesh_mesh(mesh_path)
+        else:
+            self.mesh = read_gid_mesh(mesh_path)
         self.load_cases = read_time_discretization(os.path.join(directory, self.settings["time_file"]))
-        self.probes = read_probes(os.path.join(directory, "Post_Processing.post"))
+        self.probes = _probes.read_probes(os.path.join(directory, "Post_Processing.post"), directory)
         self.tables = {}
         for _, _, tab in self.mesh["nodal_load"] + self.mesh["nodal_disp"]:
             if tab not in self.tables:
@@ -306,26 +383,47 @@ def read_results(path):
     return ndof, recs
 
 
-def write_probe(path, rows):
-    with open(path, "w") as f:
-        f.write("  Time                    Value\n")
-        for t, v in rows:
-            f.write("%23.15E %23.15E \n" % (t, v))
-
-
-def post_process(case, results_path, out_dir):
-    """The probes and NumberOfIterationsToConverge.dat from FEMAnalysis.result: one row per
-    record whose FLAG END STEP is 1 (ModExportResultFile.f90:340-420).  Returns {file: rows}."""
-    _, recs = read_results(results_path)
-    ends = [r for r in recs if r["flag_end_step"] == 1]
-    out = {}
-    for p in case.probes:
-        dof = 3 * (p["node"] - 1) + p["component"] - 1
-        rows = [(r["time"], r["U"][dof]) for r in ends]
-        write_probe(os.path.join(out_dir, p["file"]), rows)
-        out[p["file"]] = rows
-    with open(os.path.join(out_dir, "NumberOfIterationsToConverge.dat"), "w") as f:
-        f.write("   Time                    Number Of Iterations To Converge\n")
-        for r in ends:
-            f.write(" %24.15E %11d\n" % (r["time"], r["iterations"]))
-    return out
+def post_process(case, results_path, out_dir, mesh_factory=None):
+    """PostProcessingResults (ModExportResultFile.f90:254-440): the probe files start with their
+    header; every record of FEMAnalysis.result, in file order, sets the time and U and, when a
+    probe reads the Gauss points or the internal force, is replayed through a fresh mesh (the
+    mesh moves to X + U in a finite-strain analysis, the constitutive model is updated at the
+    record's time, and the state is committed after the record); the records whose FLAG END
+    STEP is 1 write one row of every active probe and of NumberOfIterationsToConverge.dat.
+    mesh_factory(case) makes the replay mesh (default: the library's, as the analysis runs on).
+    Returns {probe file: rows as written, (time, values) or message}."""
+    plist = case.probes
+    for p in plist:
+        p.active = True
+        p.rows = []
+        p.initialize(out_dir)
+    ndof, recs = read_results(results_path)
+    if ndof != case.ndof:
+        raise CaseError("FEMAnalysis.result holds %d DOFs, the case %d" % (ndof, case.ndof))
+    mesh = None
+    if any(p.needs_mesh for p in plist):
+        if mesh_factory is None:
+            from .analysis import make_mesh as mesh_factory
+        mesh = mesh_factory(case)
+    nl = case.settings["nl_analysis"]
+    nn, ne = case.ndof // 3, len(case.mesh["conn"])
+    try:
+        with open(os.path.join(out_dir, "NumberOfIterationsToConverge.dat"), "w") as itf:
+            itf.write("   Time                    Number Of Iterations To Converge\n")
+            for r in recs:
+                if mesh is not None:
+                    if nl:
+                        mesh.update_mesh(r["U"])
+                    mesh.solve_constitutive(r["time"], r["U"])
+                if r["flag_end_step"] == 1:
+                    rec = _probes.Record(r["time"], r["U"], nn, ne, mesh, case.settings["materials"],
+                                         case.mesh["mat"])
+                    for p in plist:
+                        p.write(out_dir, rec)
+                    itf.write(" %24.15E %11d\n" % (r["time"], r["iterations"]))
+                if mesh is not None:
+                    mesh.switch_converged_state()
+    finally:
+        if mesh is not None:
+            mesh.close()
+    return {p.file: p.rows for p in plist}

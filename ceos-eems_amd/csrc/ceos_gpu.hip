@@ -1401,8 +1401,13 @@ static int coarse_setup(ceos_handle* h) {
     HIP_OK(dalloc(h->ddead, dead.size()));
     HIP_OK(dalloc(h->dAc, (size_t)nc * nc));
     HIP_OK(dalloc(h->dAci, (size_t)nc * nc));
-    HIP_OK(dalloc(h->dAci32, (size_t)nc * nc));
-    HIP_OK(dalloc(h->dgc, nc));
+    const int ld = (nc + 3) & ~3;
+    HIP_OK(dalloc(h->dAci32, (size_t)nc * ld));
+    HIP_OK(dalloc(h->dgc, ld));
+    {
+        const std::vector<double> zeros(ld, 0.0);  // the entries past nc stay zero
+        HIP_OK(hipMemcpy(h->dgc, zeros.data(), sizeof(double) * ld, hipMemcpyHostToDevice));
+    }
     HIP_OK(dalloc(h->dyc, nc));
     HIP_OK(dalloc(h->dpotrf_info, 1));
     HIP_OK(hipMemcpy(h->dagg, agg.data(), sizeof(int) * nn, hipMemcpyHostToDevice));
@@ -1460,8 +1465,9 @@ static int coarse_operator(ceos_handle* h, const void* dK, hipStream_t s) {
         }
     }
     if (ok) {
-        const long nt = (long)nc * nc;
-        hipLaunchKernelGGL(k_coarse_inverse_f32, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, s, nc,
+        const int ld = (nc + 3) & ~3;
+        const long nt = (long)nc * ld;
+        hipLaunchKernelGGL(k_coarse_inverse_f32, dim3((unsigned)((nt + 255) / 256)), dim3(256), 0, s, nc, ld,
                            (const double*)h->dAci, h->dAci32);
         HIP_OK(hipGetLastError());
     }
@@ -1496,7 +1502,7 @@ static void coarse_apply(ceos_handle* h, int start, hipStream_t s) {
                        (const int*)h->dagg_ptr, (const int*)h->dagg_nodes, (const float*)h->dbasis,
                        (const double*)h->dcg_r, h->dgc);
     hipLaunchKernelGGL(k_cg_coarse_solve, dim3((h->n_coarse + 3) / 4), dim3(256), 0, s, (const CgScalars*)h->dcg,
-                       h->n_coarse, (const float*)h->dAci32, (const double*)h->dgc, h->dyc);
+                       h->n_coarse, (h->n_coarse + 3) & ~3, (const float*)h->dAci32, (const double*)h->dgc, h->dyc);
     hipLaunchKernelGGL(k_cg_prolong, dim3(nupd), dim3(kSysBlock), 0, s, h->nn, (const CgScalars*)h->dcg, start,
                        (const double*)h->dMinv, (const int*)h->dagg, (const float*)h->dbasis,
                        (const double*)h->dyc, (const double*)h->dcg_r, h->dcg_z, h->dcg_p, h->dpart, h->dpart2);

@@ -18,8 +18,9 @@
 //                          the coarse blocks (I, J) of the aggregate pairs that share a nonzero
 //                          of K, and per pair the (node A of I, slot j of A) whose neighbour lies
 //                          in J: A_c(I, J) = sum over them of B_A^T K_(A, nbr) B_nbr
-//   Ac[nc][nc], Aci[nc][nc]  the coarse matrix (nc = 6 na, FP64) and its inverse (FP32, both
-//                          triangles; dense: nc is a few thousand)
+//   Ac[nc][nc], Aci[nc][ld]  the coarse matrix (nc = 6 na, FP64) and its inverse (FP32, both
+//                          triangles, rows padded to ld = nc rounded up to 4; dense: nc is a few
+//                          thousand)
 // FP32 for P and the inverse halves the preconditioner's traffic per iteration; the
 // preconditioner only has to be symmetric positive definite and close to K^-1, and the Krylov
 // vectors, dot products and the convergence test stay FP64.
@@ -94,12 +95,13 @@ __global__ void k_coarse_unit_diag(int n, const int* __restrict__ dead, int nc, 
 }
 
 // the inverse from the Cholesky factorisation holds the upper triangle in column-major order
-// (element (i, j), i <= j, at j nc + i): both triangles, row-major, in FP32
-__global__ void k_coarse_inverse_f32(int nc, const double* __restrict__ A, float* __restrict__ out) {
+// (element (i, j), i <= j, at j nc + i): both triangles, row-major in FP32 with a row stride ld
+// (nc rounded up to a multiple of 4, the padding zero) so that rows are read as float4
+__global__ void k_coarse_inverse_f32(int nc, int ld, const double* __restrict__ A, float* __restrict__ out) {
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (long)nc * nc) return;
-    const int i = (int)(t / nc), j = (int)(t - (long)i * nc);
-    out[t] = (float)(i <= j ? A[(long)j * nc + i] : A[(long)i * nc + j]);
+    if (t >= (long)nc * ld) return;
+    const int i = (int)(t / ld), j = (int)(t - (long)i * ld);
+    out[t] = j >= nc ? 0.0f : (float)(i <= j ? A[(long)j * nc + i] : A[(long)i * nc + j]);
 }
 
 // g = P^T r: one workgroup per aggregate, the nodes of the aggregate in ascending order per
@@ -132,16 +134,28 @@ __global__ __launch_bounds__(kSysBlock) void k_cg_restrict(const CgScalars* c, c
     if (threadIdx.x < 6) gc[6 * I + threadIdx.x] = ls[threadIdx.x][0];
 }
 
-// y = A_c^-1 g: one wavefront per row of the (symmetric, dense) inverse
-__global__ __launch_bounds__(256) void k_cg_coarse_solve(const CgScalars* c, int nc, const float* __restrict__ Aci,
+// y = A_c^-1 g: one wavefront per row of the (symmetric, dense) inverse; every lane reads 4
+// entries of the row at once (row stride ld, a multiple of 4; g holds ld entries, those past nc
+// zero) into two partial sums, then a fixed-order reduction over the wavefront
+__global__ __launch_bounds__(256) void k_cg_coarse_solve(const CgScalars* c, int nc, int ld,
+                                                         const float* __restrict__ Aci,
                                                          const double* __restrict__ gc, double* __restrict__ y) {
     if (cg_stopped(c)) return;
     const int row = blockIdx.x * 4 + (int)(threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     if (row >= nc) return;
-    const float* a = Aci + (long)row * nc;
-    double s = 0.0;
-    for (int j = lane; j < nc; j += 64) s = fma((double)a[j], gc[j], s);
+    const float4* a = reinterpret_cast<const float4*>(Aci + (long)row * ld);
+    const double2* g2 = reinterpret_cast<const double2*>(gc);
+    double s0 = 0.0, s1 = 0.0;
+    for (int q = lane; q < (ld >> 2); q += 64) {
+        const float4 v = a[q];
+        const double2 ga = g2[2 * q], gb = g2[2 * q + 1];
+        s0 = fma((double)v.x, ga.x, s0);
+        s1 = fma((double)v.y, ga.y, s1);
+        s0 = fma((double)v.z, gb.x, s0);
+        s1 = fma((double)v.w, gb.y, s1);
+    }
+    double s = s0 + s1;
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
     if (lane == 0) y[row] = s;

@@ -21,7 +21,7 @@ Cases (paths relative to the reference root):
       the StVenant case's mesh with its own supports and loads, Neo-Hookean C10 = 5e10,
       K = 5e13 under the mean-dilatation (Q1P0) element technology, 50 steps; Marc
       displacements of node 11 (checked by the reference at 1e-3) and the residual log.
-The raw data files of these two GiD cases (inputs, Marc results, logs) are copied into
+The raw data files of the three cases (inputs, Marc or CEOS results, logs) are copied into
 checker/<case>/ for the tests of the package's case reader and runner.
 """
 import json
@@ -30,13 +30,20 @@ import re
 import shutil
 import sys
 
-# the input and result files of the two GiD hex8 cases, copied as they are into checker/<case>/ so
+# the input and result files of the three hex8 cases, copied as they are into checker/<case>/ so
 # that the package's case reader and runner (ceos_amd.caseio, ceos_amd.analysis) are tested on
 # the reference's own files
-RAW_CASES = ["NonLinear_FEA_3D_Hexa8_StVenant", "NonLinear_FEA_3D_Hexa8_NeoHookean_Incompressible"]
-RAW_FILES = ["Settings.dat", "Mesh.dat", "Time_Discretization.dat", "Table_Disp_Z.tab", "Table_Load_Y.tab",
-             "Post_Processing.post", "marc_disp_x.dat", "marc_disp_y.dat", "marc_disp_z.dat", "Log_CEOS.txt",
-             "NumOfIterRef.dat"]
+_GID_FILES = ["Settings.dat", "Mesh.dat", "Time_Discretization.dat", "Table_Disp_Z.tab", "Table_Load_Y.tab",
+              "Post_Processing.post", "marc_disp_x.dat", "marc_disp_y.dat", "marc_disp_z.dat", "Log_CEOS.txt",
+              "NumOfIterRef.dat"]
+RAW_CASES = {
+    "NonLinear_FEA_3D_Hexa8_StVenant": _GID_FILES,
+    "NonLinear_FEA_3D_Hexa8_NeoHookean_Incompressible": _GID_FILES,
+    "Linear_FEA_3D_Hexa8_GeneralizedHookesLaw": [
+        "Settings.dat", "Mesh.cdb", "Time_Discretization.dat", "Table_Disp_Z.tab", "Post_Processing.post",
+        "ref_disp_x.dat", "ref_disp_y.dat", "ref_disp_z.dat", "ref_stress_11.dat", "ref_stress_33.dat",
+        "ref_stress_12.dat", "Log_CEOS.txt"],
+}
 
 
 def _table(path):
@@ -274,11 +281,11 @@ def main():
         with open(os.path.join(here, name), "w") as f:
             json.dump(data, f, separators=(",", ":"))
         print("wrote", name, "nodes", len(data["coords"]), "elems", len(data["conn"]))
-    for case in RAW_CASES:
+    for case, files in RAW_CASES.items():
         src = os.path.join(root, "Checker/Cases", case)
         dst = os.path.join(here, "checker", case)
         os.makedirs(dst, exist_ok=True)
-        for fn in RAW_FILES:
+        for fn in files:
             shutil.copyfile(os.path.join(src, fn), os.path.join(dst, fn))
         print("copied", case)
 

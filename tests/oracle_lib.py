@@ -184,6 +184,11 @@ class OracleMesh:
         lib().or_get_gp_F(self._h, out)
         return out.reshape(self.ne, 8, 3, 3)
 
+    def gp_deformation_gradient(self, U):
+        """The library's accessor: F at the Gauss points for U; here those of the last
+        solve_constitutive, which the callers made at the same U."""
+        return self.gp_F()
+
     def state(self, which=1):
         out = np.zeros(self.ne * 8 * STATE_STRIDE)
         lib().or_get_state(self._h, which, out)

@@ -38,7 +38,8 @@ def test_case_reader_matches_fixture(name):
         assert props == [fx["material"]["E"], fx["material"]["nu"]] and s["element_technology"] == 1
     else:
         assert props == [fx["material"]["C10"], fx["material"]["K"]] and s["element_technology"] == 2
-    assert [(p["node"], p["component"]) for p in case.probes] == [(11, 1), (11, 2), (11, 3)]
+    assert [(p.location, p.node, p.components) for p in case.probes] == [("node", 11, [1]), ("node", 11, [2]),
+                                                                        ("node", 11, [3])]
     # boundary conditions in DOF form
     assert len(case.fixed_dof) == 45 and len(case.presc_dof) == 15
     assert np.all(case.presc_dof % 3 == 2)
@@ -61,13 +62,13 @@ def test_results_file_round_trip(tmp_path):
     for r, U in zip(recs, Us):
         assert np.array_equal(r["U"], U)  # 17 significant digits: exact
     assert [r["flag_end_step"] for r in recs] == [1, 0, 1] and recs[2]["iterations"] == 5
-    rows = caseio.post_process(case, path, tmp_path)
+    rows = caseio.post_process(case, path, tmp_path)  # node probes only: no replay mesh
     # only the end-of-step records reach the probes
     assert [t for t, _ in rows["CEOS_Disp_Z.probe"]] == [0.0, 0.02]
-    assert rows["CEOS_Disp_Z.probe"][1][1] == Us[2][3 * 10 + 2]
-    back = caseio.read_table(tmp_path / "CEOS_Disp_Z.probe")  # E23.15: 16 significant digits
+    assert rows["CEOS_Disp_Z.probe"][1][1][0] == Us[2][3 * 10 + 2]
+    back = caseio.read_table(tmp_path / "CEOS_Disp_Z.probe")  # E23.15: 15 significant digits
     assert [t for t, _ in back] == [0.0, 0.02]
-    assert np.allclose([v for _, v in back], [Us[0][32], Us[2][32]], rtol=1e-15, atol=0)
+    assert np.allclose([v for _, v in back], [Us[0][32], Us[2][32]], rtol=5e-15, atol=0)
     its = open(tmp_path / "NumberOfIterationsToConverge.dat").read().split()
     assert its[-1] == "5"
 
