@@ -74,6 +74,8 @@ EXPORTS = [
     "ceos_mesh_prepare", "ceos_evaluate_device_static", "ceos_assembly_info",
     "ceos_rve_state_save", "ceos_rve_state_load", "ceos_linear_solve_device_async",
     "ceos_set_preconditioner", "ceos_preconditioner_info", "ceos_coarse_space",
+    "ceos_periodic_create", "ceos_periodic_destroy", "ceos_periodic_info", "ceos_periodic_reduce_device",
+    "ceos_periodic_expand_device", "ceos_periodic_homogenize_device", "ceos_spmv_device",
 ]
 
 
@@ -182,6 +184,14 @@ def load_library(path=LIB_PATH):
     L.ceos_linear_solve.argtypes = [vp, _dp, _dp, d, i, C.POINTER(i), C.POINTER(d), C.POINTER(i)]
     L.ceos_linear_solve_device.argtypes = [vp, vp, vp, vp, d, i, C.POINTER(i), C.POINTER(d), C.POINTER(i), vp]
     L.ceos_newton_update_device.argtypes = [vp, vp, vp, d, vp]
+    L.ceos_periodic_create.argtypes = [vp, vp, _ip, C.POINTER(vp)]
+    L.ceos_periodic_destroy.argtypes = [vp]
+    L.ceos_periodic_destroy.restype = None
+    L.ceos_periodic_info.argtypes = [vp, C.POINTER(i), C.POINTER(C.c_longlong), C.POINTER(d)]
+    L.ceos_periodic_reduce_device.argtypes = [vp, vp, vp, vp, vp, i, vp]
+    L.ceos_periodic_expand_device.argtypes = [vp, vp, d, vp, vp]
+    L.ceos_periodic_homogenize_device.argtypes = [vp, vp, vp, vp]
+    L.ceos_spmv_device.argtypes = [vp, vp, vp, vp, vp]
     _lib = L
     return L
 
@@ -463,6 +473,53 @@ class CeosMesh:
     def newton_update_device(self, dU, dDX, scale=1.0, stream=None):
         _check(load_library().ceos_newton_update_device(self._h, dU, dDX, scale, stream),
                "ceos_newton_update_device")
+
+    def spmv_device(self, dK, dx, dy, stream=None):
+        """ceos_spmv_device: dy = K dx for the symmetric tangent in the handle's upper CSR."""
+        _check(load_library().ceos_spmv_device(self._h, dK, dx, dy, stream), "ceos_spmv_device")
+
+
+class Periodic:
+    """ceos_periodic_*: T^T K T, T^T R and T w between the handle of an RVE mesh and the handle
+    of its periodic mesh (node_map: reduced node of every node, 0-based)."""
+
+    def __init__(self, full, red, node_map):
+        L = load_library()
+        nm = np.ascontiguousarray(node_map, dtype=np.int32)
+        if len(nm) != full.nn:
+            raise ValueError("node_map needs one entry per node")
+        h = C.c_void_p()
+        _check(L.ceos_periodic_create(full._h, red._h, nm, C.byref(h)), "ceos_periodic_create")
+        self._h = h
+        self.full, self.red = full, red  # the handles must outlive this object
+
+    def close(self):
+        if self._h:
+            load_library().ceos_periodic_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def info(self):
+        n, c, v = C.c_int(), C.c_longlong(), C.c_double()
+        _check(load_library().ceos_periodic_info(self._h, C.byref(n), C.byref(c), C.byref(v)), "ceos_periodic_info")
+        return n.value, c.value, v.value
+
+    def reduce_device(self, dK, dF, dKred, dRred, mask_constrained=False, stream=None):
+        _check(load_library().ceos_periodic_reduce_device(self._h, dK, dF, dKred, dRred, 1 if mask_constrained else 0,
+                                                          stream), "ceos_periodic_reduce_device")
+
+    def expand_device(self, dw, scale, dU, stream=None):
+        _check(load_library().ceos_periodic_expand_device(self._h, dw, float(scale), dU, stream),
+               "ceos_periodic_expand_device")
+
+    def homogenize_device(self, dF, dP, stream=None):
+        _check(load_library().ceos_periodic_homogenize_device(self._h, dF, dP, stream),
+               "ceos_periodic_homogenize_device")
 
 
 class RveBatch:

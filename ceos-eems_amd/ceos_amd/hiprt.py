@@ -24,6 +24,7 @@ def hip():
         L.hipMalloc.argtypes = [C.POINTER(vp), C.c_size_t]
         L.hipFree.argtypes = [vp]
         L.hipMemcpy.argtypes = [vp, vp, C.c_size_t, C.c_int]
+        L.hipMemcpyAsync.argtypes = [vp, vp, C.c_size_t, C.c_int, vp]
         L.hipMemset.argtypes = [vp, C.c_int, C.c_size_t]
         L.hipDeviceSynchronize.argtypes = []
         L.hipStreamCreateWithFlags.argtypes = [C.POINTER(vp), C.c_uint]

@@ -29,6 +29,7 @@
 #include "assembly_kernel.hpp"
 #include "system_kernel.hpp"
 #include "coarse_kernel.hpp"
+#include "periodic_kernel.hpp"
 
 using namespace ceos;
 
@@ -2419,6 +2420,226 @@ int ceos_linear_solve_device_async(ceos_handle* h, const void* dK, const double*
     pcg_start(h, db, dx, rtol, maxit, s);
     pcg_iterations(h, dx, 0, maxit, s);
     hipLaunchKernelGGL(k_cg_report, dim3(1), dim3(64), 0, s, (const CgScalars*)h->dcg, dresult);
+    HIP_OK(hipGetLastError());
+    return note_caller_stream(h, s);
+}
+
+// ------------------------------------------------------------------------------------------
+// Periodic reduction (general-mesh RVEs): T^T K T, T^T R and T w between a full hex8 handle and
+// the handle of its periodic (torus) mesh; see periodic_kernel.hpp.
+// ------------------------------------------------------------------------------------------
+struct ceos_periodic {
+    ceos_handle* full = nullptr;
+    ceos_handle* red = nullptr;
+    int nn = 0, nn_red = 0;
+    long long nnz_red = 0;
+    double v0 = 0.0;
+    int nblk = 0;
+    int *dmap = nullptr, *dkptr = nullptr, *drptr = nullptr, *drnodes = nullptr;
+    uint32_t* dkidx = nullptr;
+    double* dpart = nullptr;
+};
+
+// reference volume of a hex8 element by the 2x2x2 Gauss rule (exact for the trilinear map)
+static double hex8_volume_host(const double* X, const int* c) {
+    const double x = 0.57735026918962576451;
+    auto sg = [](int a, int k) {
+        const int s = k == 0 ? (((a + 1) >> 1) & 1) : (k == 1 ? ((a >> 1) & 1) : ((a >> 2) & 1));
+        return s ? 1.0 : -1.0;
+    };
+    double v = 0.0;
+    for (int g = 0; g < 8; ++g) {
+        const double xi = sg(g, 0) * x, eta = sg(g, 1) * x, zeta = sg(g, 2) * x;
+        double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+        for (int a = 0; a < 8; ++a) {
+            const double s0 = sg(a, 0), s1 = sg(a, 1), s2 = sg(a, 2);
+            const double d[3] = {s0 * (1.0 + s1 * eta) * (1.0 + s2 * zeta) * 0.125,
+                                 s1 * (1.0 + s0 * xi) * (1.0 + s2 * zeta) * 0.125,
+                                 s2 * (1.0 + s0 * xi) * (1.0 + s1 * eta) * 0.125};
+            for (int i = 0; i < 3; ++i)
+                for (int k = 0; k < 3; ++k) J[i][k] += X[3L * c[a] + i] * d[k];
+        }
+        v += J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) - J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
+             J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
+    }
+    return v;
+}
+
+// 0-based CSR slot of the entry (3A+i, 3B+k), A <= B (i <= k when A == B), of a handle's upper
+// pattern, or -1 when B is not an upper neighbour of A
+static long long upper_slot(const ceos_handle* h, const std::vector<long long>& rs, int A, int B, int i, int k) {
+    const long long nu = h->nup[A];
+    const long long row = rs[A] + (i == 0 ? 0 : (i == 1 ? 3 + 3 * nu : 5 + 6 * nu));
+    if (A == B) return row + (k - i);
+    const int* b = h->up_nodes.data() + h->up_ptr[A];
+    const int* e = h->up_nodes.data() + h->up_ptr[A + 1];
+    const int* f = std::lower_bound(b, e, B);
+    if (f == e || *f != B) return -1;
+    return row + (3 - i) + 3LL * (f - b) + k;
+}
+
+void ceos_periodic_destroy(ceos_periodic* p) {
+    if (!p) return;
+    dfree(p->dmap); dfree(p->dkptr); dfree(p->drptr); dfree(p->drnodes); dfree(p->dkidx); dfree(p->dpart);
+    delete p;
+}
+
+int ceos_periodic_create(ceos_handle* full, ceos_handle* red, const int* node_map, ceos_periodic** out) {
+    if (!full || !red || !node_map || !out || full == red) return CEOS_ERR_ARG;
+    *out = nullptr;
+    if (full->precision != CEOS_K_FP64 || red->precision != CEOS_K_FP64) return CEOS_ERR_ARG;
+    const int nn = full->nn, nr = red->nn;
+    std::vector<int> cnt(nr, 0);
+    for (int A = 0; A < nn; ++A) {
+        if (node_map[A] < 0 || node_map[A] >= nr) return CEOS_ERR_ARG;
+        cnt[node_map[A]]++;
+    }
+    for (int a = 0; a < nr; ++a)
+        if (!cnt[a]) return CEOS_ERR_ARG;  // every reduced node is the image of some node
+    auto row_starts = [](const ceos_handle* h) {
+        std::vector<long long> rs(h->nn + 1, 0);
+        for (int A = 0; A < h->nn; ++A) rs[A + 1] = rs[A] + 6 + 9LL * h->nup[A];
+        return rs;
+    };
+    const std::vector<long long> rsf = row_starts(full), rsr = row_starts(red);
+    if (rsf[nn] != full->nnz || rsr[nr] != red->nnz || full->nnz >= (1LL << 31)) return CEOS_ERR_ARG;
+    // every stored full entry, in full slot order, onto its reduced upper entry
+    std::vector<std::pair<long long, uint32_t>> con;  // (reduced slot, full slot | doubled)
+    con.reserve((size_t)full->nnz);
+    for (int A = 0; A < nn; ++A) {
+        const int a = node_map[A];
+        for (int i = 0; i < 3; ++i)
+            for (int k = i; k < 3; ++k) {
+                const long long fs = upper_slot(full, rsf, A, A, i, k);
+                con.emplace_back(upper_slot(red, rsr, a, a, i, k), (uint32_t)fs);
+            }
+        for (int q = full->up_ptr[A]; q < full->up_ptr[A + 1]; ++q) {
+            const int B = full->up_nodes[q], b = node_map[B];
+            for (int i = 0; i < 3; ++i)
+                for (int k = 0; k < 3; ++k) {
+                    const long long fs = upper_slot(full, rsf, A, B, i, k);
+                    long long rs;
+                    uint32_t w = (uint32_t)fs;
+                    if (a < b) rs = upper_slot(red, rsr, a, b, i, k);
+                    else if (a > b) rs = upper_slot(red, rsr, b, a, k, i);
+                    else {
+                        // two periodic images in one element: K(i,k) and K(k,i) of the pair both
+                        // land on the reduced diagonal block
+                        rs = upper_slot(red, rsr, a, a, std::min(i, k), std::max(i, k));
+                        if (i == k) w |= kPerDoubled;
+                    }
+                    if (rs < 0) return CEOS_ERR_SLOT;
+                    con.emplace_back(rs, w);
+                }
+        }
+    }
+    std::vector<int> kptr((size_t)red->nnz + 1, 0);
+    for (const auto& c : con) kptr[c.first + 1]++;
+    for (long long e = 0; e < red->nnz; ++e) {
+        if (!kptr[e + 1]) return CEOS_ERR_SLOT;  // a reduced entry no full entry reaches
+        kptr[e + 1] += kptr[e];
+    }
+    std::vector<uint32_t> kidx(con.size());
+    {
+        std::vector<int> fill(kptr.begin(), kptr.end() - 1);
+        for (const auto& c : con) kidx[fill[c.first]++] = c.second;  // stable: full slot order
+    }
+    std::vector<int> rptr(nr + 1, 0), rnodes(nn);
+    for (int a = 0; a < nr; ++a) rptr[a + 1] = rptr[a] + cnt[a];
+    {
+        std::vector<int> fill(rptr.begin(), rptr.end() - 1);
+        for (int A = 0; A < nn; ++A) rnodes[fill[node_map[A]]++] = A;
+    }
+    double v0 = 0.0;
+    for (int e = 0; e < full->ne; ++e) v0 += hex8_volume_host(full->X.data(), full->conn0.data() + 8L * e);
+    if (!(v0 > 0.0)) return CEOS_ERR_ARG;
+    ceos_periodic* p = new ceos_periodic();
+    p->full = full;
+    p->red = red;
+    p->nn = nn;
+    p->nn_red = nr;
+    p->nnz_red = red->nnz;
+    p->v0 = v0;
+    p->nblk = std::min((nn + kPerBlock - 1) / kPerBlock, 256);
+    hipError_t e = hipSetDevice(full->device);
+    if (e == hipSuccess) e = dalloc(p->dmap, nn);
+    if (e == hipSuccess) e = dalloc(p->dkptr, kptr.size());
+    if (e == hipSuccess) e = dalloc(p->dkidx, kidx.size());
+    if (e == hipSuccess) e = dalloc(p->drptr, rptr.size());
+    if (e == hipSuccess) e = dalloc(p->drnodes, rnodes.size());
+    if (e == hipSuccess) e = dalloc(p->dpart, 9L * p->nblk);
+    if (e == hipSuccess) e = hipMemcpy(p->dmap, node_map, sizeof(int) * nn, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(p->dkptr, kptr.data(), sizeof(int) * kptr.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(p->dkidx, kidx.data(), sizeof(uint32_t) * kidx.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(p->drptr, rptr.data(), sizeof(int) * rptr.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(p->drnodes, rnodes.data(), sizeof(int) * rnodes.size(), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        ceos_periodic_destroy(p);
+        return CEOS_ERR_HIP;
+    }
+    *out = p;
+    return CEOS_OK;
+}
+
+int ceos_periodic_info(const ceos_periodic* p, int* nn_red, long long* contributions, double* volume) {
+    if (!p) return CEOS_ERR_ARG;
+    if (nn_red) *nn_red = p->nn_red;
+    if (contributions) *contributions = 0;
+    if (contributions) {
+        int last = 0;
+        HIP_OK(hipMemcpy(&last, p->dkptr + p->nnz_red, sizeof(int), hipMemcpyDeviceToHost));
+        *contributions = last;
+    }
+    if (volume) *volume = p->v0;
+    return CEOS_OK;
+}
+
+int ceos_periodic_reduce_device(ceos_periodic* p, const double* dK, const double* dF, double* dKred, double* dRred,
+                                int mask_constrained, void* stream) {
+    if (!p || (!dK && !dF) || (dK && !dKred) || (dF && !dRred)) return CEOS_ERR_ARG;
+    if (mask_constrained && !p->red->sys_ready) return CEOS_ERR_ARG;
+    hipStream_t s = stream ? (hipStream_t)stream : p->red->stream;
+    if (dK)
+        hipLaunchKernelGGL(k_periodic_reduce_k, dim3((unsigned)((p->nnz_red + kPerBlock - 1) / kPerBlock)),
+                           dim3(kPerBlock), 0, s, p->nnz_red, (const int*)p->dkptr, (const uint32_t*)p->dkidx, dK,
+                           dKred);
+    if (dF)
+        hipLaunchKernelGGL(k_periodic_reduce_r, dim3((unsigned)((p->nn_red + kPerBlock - 1) / kPerBlock)),
+                           dim3(kPerBlock), 0, s, p->nn_red, (const int*)p->drptr, (const int*)p->drnodes, dF,
+                           mask_constrained ? (const uint8_t*)p->red->ddof_mask : nullptr, dRred);
+    HIP_OK(hipGetLastError());
+    return CEOS_OK;
+}
+
+int ceos_periodic_expand_device(ceos_periodic* p, const double* dw, double scale, double* dU, void* stream) {
+    if (!p || !dw || !dU) return CEOS_ERR_ARG;
+    hipStream_t s = stream ? (hipStream_t)stream : p->red->stream;
+    hipLaunchKernelGGL(k_periodic_expand, dim3((unsigned)((p->nn + kPerBlock - 1) / kPerBlock)), dim3(kPerBlock), 0,
+                       s, p->nn, (const int*)p->dmap, dw, scale, dU);
+    HIP_OK(hipGetLastError());
+    return CEOS_OK;
+}
+
+int ceos_periodic_homogenize_device(ceos_periodic* p, const double* dF, double* dP, void* stream) {
+    if (!p || !dF || !dP) return CEOS_ERR_ARG;
+    hipStream_t s = stream ? (hipStream_t)stream : p->red->stream;
+    hipLaunchKernelGGL(k_periodic_moment_partial, dim3(p->nblk), dim3(kPerBlock), 0, s, p->nn,
+                       (const double*)p->full->dX, dF, p->dpart);
+    hipLaunchKernelGGL(k_periodic_moment_final, dim3(1), dim3(64), 0, s, p->nblk, (const double*)p->dpart,
+                       1.0 / p->v0, dP);
+    HIP_OK(hipGetLastError());
+    return CEOS_OK;
+}
+
+int ceos_spmv_device(ceos_handle* h, const void* dK, const double* dx, double* dy, void* stream) {
+    if (!h || !dK || !dx || !dy || !h->sys_ready) return CEOS_ERR_ARG;
+    hipStream_t s = stream ? (hipStream_t)stream : h->stream;
+    const SysGeom g = sys_geom(h);
+    const unsigned nb = (unsigned)((h->nn + kSysBlock - 1) / kSysBlock);
+    if (h->precision == CEOS_K_FP32_STORED)
+        hipLaunchKernelGGL(k_sym_spmv<float>, dim3(nb), dim3(kSysBlock), 0, s, g, (const float*)dK, dx, dy);
+    else
+        hipLaunchKernelGGL(k_sym_spmv<double>, dim3(nb), dim3(kSysBlock), 0, s, g, (const double*)dK, dx, dy);
     HIP_OK(hipGetLastError());
     return note_caller_stream(h, s);
 }

@@ -353,6 +353,47 @@ int ceos_linear_solve_device_async(ceos_handle *h, const void *dK, const double 
 int ceos_newton_update_device(ceos_handle *h, double *dU, const double *dDX, double scale, void *stream);
 
 /* ------------------------------------------------------------------------------------------
+ * Periodic RVEs on general hex8 meshes: the products with the periodicity matrix T of the
+ * reference's periodic system of equations (FEM/Modules/ModMultiscalePeriodicFEMSoE.f90:77-222,
+ * EvaluateR / EvaluateKt / ExpandFluctuation: K_red = T^T K T, R_red = T^T R, DeltaU = T DeltaU_red),
+ * between a handle `full` of the RVE mesh and a handle `red` of its periodic mesh: the same
+ * elements with every node renamed to its reduced node (node_map[A], 0-based, one entry per node
+ * of `full`: a slave's master, every corner the first corner; BuildT, :226-895, finds them).
+ * `red` supplies the reduced pattern, its boundary conditions (the fixed corner) and the linear
+ * solve; its elements are never evaluated.  Both handles store K in FP64.  Sums run in a fixed
+ * order: bitwise reproducible.
+ * ------------------------------------------------------------------------------------------ */
+typedef struct ceos_periodic ceos_periodic;
+
+/* Builds the contribution lists (host, once).  CEOS_ERR_SLOT: a pair of reduced nodes that the
+ * map produces is missing from the reduced pattern (node_map and red's connectivity disagree). */
+int ceos_periodic_create(ceos_handle *full, ceos_handle *red, const int *node_map, ceos_periodic **p);
+void ceos_periodic_destroy(ceos_periodic *p);
+
+/* Reduced nodes, stored contributions to K_red, and the RVE's reference volume V0. */
+int ceos_periodic_info(const ceos_periodic *p, int *nn_red, long long *contributions, double *volume);
+
+/* dKred = T^T dK T (red's upper CSR) and dRred = T^T dF (3 per reduced node); either pair may be
+ * NULL.  mask_constrained: entries of dRred at DOFs that red's boundary conditions constrain are
+ * zeroed (a right-hand side of the constrained system).  Device arrays, asynchronous on stream
+ * (NULL = red's stream). */
+int ceos_periodic_reduce_device(ceos_periodic *p, const double *dK, const double *dF, double *dKred, double *dRred,
+                                int mask_constrained, void *stream);
+
+/* dU += scale T dw (dw: 3 per reduced node, dU: 3 per node of full). */
+int ceos_periodic_expand_device(ceos_periodic *p, const double *dw, double scale, double *dU, void *stream);
+
+/* dP = (1/V0) sum_A dF_A (x) X_A (9 values, column-major): for the internal force of a
+ * total-Lagrangian hex8 mesh this is the volume average of the first Piola stress that
+ * HomogenizeStress forms (MultiScale/ModMultiscaleAnalysis.f90:140-329), since the shape
+ * functions reproduce X; for K dU it is the matching increment. */
+int ceos_periodic_homogenize_device(ceos_periodic *p, const double *dF, double *dP, void *stream);
+
+/* dy = K dx for the handle's symmetric tangent stored as its upper CSR (the handle's precision).
+ * Needs the neighbour slot tables of ceos_bc_set (an empty set of conditions will do). */
+int ceos_spmv_device(ceos_handle *h, const void *dK, const double *dx, double *dy, void *stream);
+
+/* ------------------------------------------------------------------------------------------
  * Batched periodic RVEs (FE^2 micro problems).
  *
  * Replaces, for every RVE of a batch at once, the reference's periodic multiscale analysis of
