@@ -236,6 +236,106 @@ def run(case, out_dir=None, linear_rtol=1e-12, minres_rtol=1e-12, absolute=False
     return history, norms
 
 
+def run_multiscale(case, out_dir=None, verbose=False, restart=None, stop_after=None, attempts=None,
+                   pcg_rtol=1e-12):
+    """The periodic multiscale analysis of one RVE (FEM/ModFEMAnalysis.f90:1440-1640): for every
+    load case and step the macroscopic deformation gradient of the MACROSCOPIC DEFORMATION
+    GRADIENT tables at the step's end, the RVE (the case's mesh, periodic on its box, the first
+    boundary node set as BuildT's candidates) solved by ceos_amd.periodic.PeriodicRve with the
+    case's Newton tolerance, the golden-ratio cut-back on failure (F = F_start + alpha (F_end -
+    F_start), as GetBoundaryConditionsMultiscalePeriodic forms it), one FEMAnalysis.result record
+    per converged substep and the commit of the Gauss-point states at the end of the step; then
+    the post-processing, whose Micro Structure probes read the homogenized P, F and Cauchy stress.
+    restart (default: the case's Restart option, which the reference accepts for this model)
+    replays the end-of-step records and resumes after the last finished step.  Returns
+    (history, attempts): history rows (load case, step, time, U, iterations, P (9, column-major))."""
+    from .periodic import PeriodicRve
+    s = case.settings
+    if not s.get("multiscale"):
+        raise caseio.CaseError("not a multiscale analysis")
+    m = case.mesh
+    nl = s["nl_analysis"]
+    rv = PeriodicRve(m["coords"], m["conn"], m["mat"], s["materials"], nl=nl, boundary=case.boundary_nodes(),
+                     newton_rtol=s["newton_tol"], newton_atol=1e-300, newton_maxit=s["newton_maxiter"],
+                     pcg_rtol=pcg_rtol)
+    n = case.ndof
+    history, tries = [], []
+    restart = s.get("restart", False) if restart is None else restart
+    steps = [c["steps"] for c in case.load_cases]
+    start = (1, 1)
+    res = None
+    if restart:
+        path = os.path.join(out_dir or "", "FEMAnalysis.result")
+        if not out_dir or not os.path.exists(path):
+            raise caseio.CaseError("no FEMAnalysis.result found")
+        U, last_lc, last_st = replay_results(rv.full, path, nl)
+        c = case.load_cases[last_lc - 1]
+        rv.set_start(U, case.macro_F(c["t0"] + last_st * (c["t1"] - c["t0"]) / c["steps"]))
+        start = restart_start(steps, last_lc, last_st)
+        res = open(path, "a")
+    elif out_dir:
+        os.makedirs(out_dir, exist_ok=True)
+        res = open(os.path.join(out_dir, "FEMAnalysis.result"), "w")
+        caseio.write_results_header(res, n)
+        caseio.write_results_record(res, np.zeros(n), 0.0, 1, 1, 0, 0, 1, 0)
+    try:
+        for lc, c in enumerate(case.load_cases, start=1):
+            if start is None or lc < start[0]:
+                continue
+            dt = (c["t1"] - c["t0"]) / c["steps"]
+            for stp in range(start[1] if lc == start[0] else 1, c["steps"] + 1):
+                ta, tb = c["t0"] + (stp - 1) * dt, c["t0"] + stp * dt
+                F0, F1 = case.macro_F(ta), case.macro_F(tb)
+                amin, alpha = 0.0, 1.0
+                cutback = substep = 0
+                while True:
+                    t = ta + alpha * dt
+                    try:
+                        P, _, info = rv.solve(F0 + alpha * (F1 - F0), time=t)
+                        ok = True
+                    except (gpu.CeosError, RuntimeError):
+                        ok = False
+                    tries.append((lc, stp, alpha, ok))
+                    if verbose:
+                        print("LC %d step %d alpha %.4f: %s" % (lc, stp, alpha, "converged" if ok else "cut back"),
+                              flush=True)
+                    if not ok:
+                        alpha = amin + (1.0 - 1.0 / GOLDEN_RATIO) * (alpha - amin)
+                        cutback += 1
+                        substep = 1
+                        if cutback > s["max_cut_backs"]:
+                            raise NewtonFailed("load case %d step %d did not converge with %d cut backs"
+                                               % (lc, stp, s["max_cut_backs"]))
+                        continue
+                    substep += 1
+                    U = rv.displacement()
+                    end = abs(alpha - 1.0) < 1e-6
+                    if res:
+                        caseio.write_results_record(res, U, t, lc, stp, cutback, substep, 1 if end else 0,
+                                                    info["iterations"])
+                    if end:
+                        history.append((lc, stp, t, U, info["iterations"], P))
+                        break
+                    rv.accept()
+                    aux, amin = amin, alpha
+                    alpha = min(alpha + GOLDEN_RATIO * (alpha - aux), 1.0)
+                rv.commit()
+                if stop_after is not None and (lc, stp) == tuple(stop_after):
+                    start = None
+                    break
+            if start is None:
+                break
+    finally:
+        if res:
+            res.close()
+        rv.close()
+    if attempts is not None:
+        attempts.extend(tries)
+    if out_dir:
+        caseio.post_process(case, os.path.join(out_dir, "FEMAnalysis.result"), out_dir)
+    return history, tries
+
+
 def main(argv):
     if len(argv) < 2:
         print(__doc__)
@@ -243,7 +343,10 @@ def main(argv):
     case = caseio.Case(argv[1])
     out = argv[2] if len(argv) > 2 else argv[1]
     t0 = _time.perf_counter()
-    hist, _ = run(case, out, verbose=True)
+    if case.settings.get("multiscale"):
+        hist, _ = run_multiscale(case, out, verbose=True)
+    else:
+        hist, _ = run(case, out, verbose=True)
     print("%d steps in %.2f s; results in %s" % (len(hist), _time.perf_counter() - t0, out))
     return 0
 

@@ -94,8 +94,12 @@ def read_settings(path):
     """Analysis settings, Newton parameters, materials and file names of a Settings.dat."""
     sec = _sections(_lines(path))
     an = _options(sec.get("analysissettings", []))
-    if _key(an.get("multiscaleanalysis", "false")) != "false":
-        raise CaseError("multiscale analyses are not read")
+    multiscale = _key(an.get("multiscaleanalysis", "false"))
+    if multiscale not in ("true", "false"):
+        raise CaseError("Multiscale Analysis not identified")
+    multiscale = multiscale == "true"
+    if multiscale and _key(an.get("multiscalemodel", "")) != "periodic":
+        raise CaseError("multiscale model %r is not run: only Periodic" % an.get("multiscalemodel", ""))
     if _key(an.get("hypothesisofanalysis", "3d")) != "3d":
         raise CaseError("only 3D analyses are supported")
     tech = ELEMENT_TECHNOLOGY.get(_key(an.get("elementtechnology", "fullintegration")))
@@ -136,7 +140,22 @@ def read_settings(path):
     pre = _key(mb.get("preprocessor", "gid12"))
     if pre not in ("gid12", "hypermesh"):
         raise CaseError("only GiD 12 and HyperMesh meshes are read")
+    macro = None
+    if multiscale:
+        # MACROSCOPIC DEFORMATION GRADIENT (ReadMacroscopicDeformationGradient,
+        # ModReadInputFile.f90:437-560): F11 ... F33, each 'Zero', 'One' or a table name
+        mf = _options(sec.get("macroscopicdeformationgradient", []))
+        macro = {}
+        for i in range(3):
+            for j in range(3):
+                k = "f%d%d" % (i + 1, j + 1)
+                if k not in mf:
+                    raise CaseError("Multiscale Settings :: Option not found [F%d%d]" % (i + 1, j + 1))
+                v = mf[k].strip()
+                macro[(i, j)] = 0.0 if _key(v) == "zero" else (1.0 if _key(v) == "one" else v)
     return {
+        "multiscale": multiscale,
+        "macro_F": macro,
         "nl_analysis": nl,
         "element_technology": tech,
         "max_cut_backs": int(_num(an.get("maximumcutbacks", "10"))),
@@ -202,7 +221,8 @@ def read_gid_mesh(path):
     if not mat:
         mat = [1] * len(conn)
     return {"coords": np.array(coords), "conn": np.array(conn, dtype=np.int32),
-            "mat": np.array(mat, dtype=np.int32), "fixed": fixed, "nodal_load": loads, "nodal_disp": disps}
+            "mat": np.array(mat, dtype=np.int32), "fixed": fixed, "nodal_load": loads, "nodal_disp": disps,
+            "boundary": []}
 
 
 def read_hypermesh_mesh(path):
@@ -261,6 +281,7 @@ def read_hypermesh_mesh(path):
     conn = np.array([elems[k][1] for k in range(1, ne + 1)], dtype=np.int32)
     mat = np.array([elems[k][0] for k in range(1, ne + 1)], dtype=np.int32)
     fixed, loads, disps = {}, [], []
+    boundary = []
     name = None
     dof_of = {"ux": 1, "uy": 2, "uz": 3, "fx": 1, "fy": 2, "fz": 3}
     while i < len(lines):
@@ -274,7 +295,12 @@ def read_hypermesh_mesh(path):
         if len(f) == 4 and name is not None:
             k = _key(name)
             first = name.split()[0].lower() if name.split() else ""
-            if first in ("boundary", "reaction"):
+            if first == "boundary":
+                # node sets of an RVE's boundary (BC%BoundaryNodes, ModReadInputFile.f90:1290-1300)
+                if not boundary or boundary[-1][0] != name:
+                    boundary.append((name, []))
+                boundary[-1][1].append(int(f[1]))
+            elif first == "reaction":
                 pass
             elif k == "fixedsupports":
                 d = dof_of.get(f[2].lower())
@@ -291,7 +317,13 @@ def read_hypermesh_mesh(path):
                 loads.append([int(f[1]), dof_of[f[2].lower()], name])
         i += 1
     return {"coords": coords, "conn": conn, "mat": mat, "fixed": [[n] + fixed[n] for n in sorted(fixed)],
-            "nodal_load": loads, "nodal_disp": disps}
+            "nodal_load": loads, "nodal_disp": disps, "boundary": boundary}
+
+
+def table_path(directory, name):
+    """File of a load table (ReadValueDiscretization, ModLoadHistoryData.f90:140-148): the name
+    itself when it has an extension, else name + '.tab'."""
+    return os.path.join(directory, name if "." in name else name + ".tab")
 
 
 def read_table(path):
@@ -331,9 +363,12 @@ class Case:
         self.load_cases = read_time_discretization(os.path.join(directory, self.settings["time_file"]))
         self.probes = _probes.read_probes(os.path.join(directory, "Post_Processing.post"), directory)
         self.tables = {}
-        for _, _, tab in self.mesh["nodal_load"] + self.mesh["nodal_disp"]:
+        names = [tab for _, _, tab in self.mesh["nodal_load"] + self.mesh["nodal_disp"]]
+        if self.settings["multiscale"]:
+            names += [v for v in self.settings["macro_F"].values() if isinstance(v, str)]
+        for tab in names:
             if tab not in self.tables:
-                self.tables[tab] = read_table(os.path.join(directory, tab + ".tab"))
+                self.tables[tab] = read_table(table_path(directory, tab))
         nn = len(self.mesh["coords"])
         self.ndof = 3 * nn
         fx = []
@@ -352,6 +387,19 @@ class Case:
 
     def presc_values(self, t):
         return np.array([interp(self.tables[tab], t) for _, tab in self.presc])
+
+    def macro_F(self, t):
+        """The macroscopic deformation gradient of a multiscale case at time t (each component a
+        constant or its table, linearly interpolated)."""
+        F = np.zeros((3, 3))
+        for (i, j), v in self.settings["macro_F"].items():
+            F[i, j] = interp(self.tables[v], t) if isinstance(v, str) else v
+        return F
+
+    def boundary_nodes(self):
+        """0-based nodes of the mesh's first boundary set (BC%BoundaryNodes(1)), or None."""
+        b = self.mesh.get("boundary") or []
+        return np.asarray(b[0][1], dtype=np.int64) - 1 if b else None
 
 
 def write_results_header(f, ndof):
@@ -383,10 +431,20 @@ def read_results(path):
     return ndof, recs
 
 
+def _homogenized(mesh):
+    """HomogenizeStress / HomogenizeDeformationGradient of the replayed RVE (the volume averages of
+    P and F over its Gauss points): (P 9 column-major, F 3x3)."""
+    def get():
+        P, F = mesh.homogenize()
+        return P, np.asarray(F).reshape(3, 3).T
+    return get
+
+
 def post_process(case, results_path, out_dir, mesh_factory=None):
     """PostProcessingResults (ModExportResultFile.f90:254-440): the probe files start with their
     header; every record of FEMAnalysis.result, in file order, sets the time and U and, when a
-    probe reads the Gauss points or the internal force, is replayed through a fresh mesh (the
+    probe reads the Gauss points or the internal force (or, in a multiscale case, the homogenized
+    stress and deformation gradient of the RVE), is replayed through a fresh mesh (the
     mesh moves to X + U in a finite-strain analysis, the constitutive model is updated at the
     record's time, and the state is committed after the record); the records whose FLAG END
     STEP is 1 write one row of every active probe and of NumberOfIterationsToConverge.dat.
@@ -401,7 +459,8 @@ def post_process(case, results_path, out_dir, mesh_factory=None):
     if ndof != case.ndof:
         raise CaseError("FEMAnalysis.result holds %d DOFs, the case %d" % (ndof, case.ndof))
     mesh = None
-    if any(p.needs_mesh for p in plist):
+    multiscale = case.settings.get("multiscale", False)
+    if any(p.needs_mesh or (multiscale and p.location == "microstructure") for p in plist):
         if mesh_factory is None:
             from .analysis import make_mesh as mesh_factory
         mesh = mesh_factory(case)
@@ -417,7 +476,7 @@ def post_process(case, results_path, out_dir, mesh_factory=None):
                     mesh.solve_constitutive(r["time"], r["U"])
                 if r["flag_end_step"] == 1:
                     rec = _probes.Record(r["time"], r["U"], nn, ne, mesh, case.settings["materials"],
-                                         case.mesh["mat"])
+                                         case.mesh["mat"], homogenized=_homogenized(mesh) if multiscale else None)
                     for p in plist:
                         p.write(out_dir, rec)
                     itf.write(" %24.15E %11d\n" % (r["time"], r["iterations"]))

@@ -244,13 +244,27 @@ class PeriodicRve:
                 A[9 * iJ + kl] = dP[iJ]
         return A
 
+    def accept(self):
+        """The solved state becomes the start of the next solve (a converged substep: the
+        Gauss-point states stay committed at the start of the step)."""
+        self.stream.synchronize()
+        hiprt.check(hiprt.hip().hipMemcpy(self.dUc.ptr, self.dU.ptr, self.dU.nbytes, 3), "hipMemcpy")
+        self.Fconv = self.F.copy()
+
     def commit(self):
         """SwitchConvergedState of the RVE's Gauss points; the solved state becomes the start of
         the next solve."""
-        self.stream.synchronize()
+        self.accept()
         self.full.switch_converged_state()
-        hiprt.check(hiprt.hip().hipMemcpy(self.dUc.ptr, self.dU.ptr, self.dU.nbytes, 3), "hipMemcpy")
-        self.Fconv = self.F.copy()
+
+    def set_start(self, U, F):
+        """Start the next solve from the displacement U at the macro F (a restart; the Gauss-point
+        states are the handle's committed ones)."""
+        self.stream.synchronize()
+        self.dUc.upload(np.asarray(U, dtype=np.float64).reshape(-1))
+        hiprt.check(hiprt.hip().hipMemcpy(self.dU.ptr, self.dUc.ptr, self.dU.nbytes, 3), "hipMemcpy")
+        self.Fconv = np.asarray(F, dtype=np.float64).reshape(3, 3).copy()
+        self.F = self.Fconv.copy()
 
     def displacement(self):
         self.stream.synchronize()

@@ -103,3 +103,49 @@ def test_condensed_tangent_matches_finite_differences():
         fd = (Pp - Pm) / (2 * h)
         assert np.max(np.abs(fd - A[:, kl])) <= 1e-6 * np.max(np.abs(A)), (kl, fd, A[:, kl])
     r.close()
+
+
+def test_multiscale_case_runs_from_its_files_and_restarts(tmp_path):
+    """A periodic multiscale case written in the reference's formats runs end to end
+    (analysis.run_multiscale): its Micro Structure probes (homogenized P, F, Cauchy stress from the
+    replay) and a Gauss-point probe match the oracle's periodic solve of the same macro F history;
+    an interrupted run restarted from its FEMAnalysis.result writes the same records bitwise."""
+    import case_writer as cw
+    from ceos_amd import analysis, caseio, probes
+    X, conn, mid, ph = _two_phase(4, 5, 0.15)
+    full, part = tmp_path / "full", tmp_path / "part"
+    for d in (full, part):
+        d.mkdir()
+        cw.write_multiscale_case(str(d), X, conn, mid)
+    case = caseio.Case(str(full))
+    hist, tries = analysis.run_multiscale(case, str(full))
+    assert len(hist) == 3 and all(ok for *_, ok in tries)
+    Xc = case.mesh["coords"]  # the deck's E16.9 coordinates
+    o = po.OraclePeriodicRve(Xc, conn, mid, MAT)
+    rows = {n: probes.probe_file_rows(full / n)[1] for n in ("P.probe", "F.probe", "S.probe", "gp.probe")}
+    assert [t for t, _ in rows["P.probe"]] == pytest.approx([0.0, 1 / 3, 2 / 3, 1.0], rel=1e-14)
+    assert np.all(rows["P.probe"][0][1] == 0.0)
+    for k, t in enumerate([1 / 3, 2 / 3, 1.0], start=1):
+        F = case.macro_F(t)
+        Po, Uo, _ = o.solve(F)
+        sig_o = o.m.gp_stress()[4, 2].copy()
+        o.commit()
+        P = rows["P.probe"][k][1]
+        assert np.max(np.abs(P - Po)) <= 1e-8 * np.max(np.abs(Po)), (t, P, Po)
+        assert np.allclose(rows["F.probe"][k][1], [F[0, 0], F[0, 1], F[2, 2]], rtol=0, atol=1e-13)
+        Pm = P.reshape(3, 3).T
+        S = Pm @ F.T / np.linalg.det(F)
+        assert np.allclose(rows["S.probe"][k][1], [S[0, 0], S[1, 1], S[2, 2], S[0, 1], S[1, 2], S[0, 2]],
+                           rtol=0, atol=1e-8 * np.abs(S).max())
+        assert np.max(np.abs(rows["gp.probe"][k][1] - sig_o)) <= 1e-8 * np.abs(sig_o).max()
+    # interrupted after step 2, restarted: the same records
+    case2 = caseio.Case(str(part))
+    analysis.run_multiscale(case2, str(part), stop_after=(1, 2))
+    hist_b, _ = analysis.run_multiscale(caseio.Case(str(part)), str(part), restart=True)
+    assert [h[1] for h in hist_b] == [3]
+    _, rf = caseio.read_results(full / "FEMAnalysis.result")
+    _, rp = caseio.read_results(part / "FEMAnalysis.result")
+    assert len(rf) == len(rp) == 4
+    for a, b in zip(rf, rp):
+        assert a["time"] == b["time"] and np.array_equal(a["U"], b["U"])
+    assert (full / "P.probe").read_text() == (part / "P.probe").read_text()

@@ -62,3 +62,33 @@ def test_oracle_restatement_on_a_homogeneous_rve():
     assert np.allclose(P, Pref.T.reshape(9), rtol=0, atol=1e-12 * np.abs(Pref).max())
     w = U - (X @ (F - np.eye(3)).T).reshape(-1)
     assert np.max(np.abs(w)) < 1e-12 and its <= 1
+
+
+def test_multiscale_case_files_are_read(tmp_path):
+    """A periodic multiscale case in the reference's files: HyperMesh mesh with its boundary node
+    set, the MACROSCOPIC DEFORMATION GRADIENT block (tables, 'One', 'Zero'), the Micro Structure
+    and Gauss-point probes."""
+    import case_writer as cw
+    from ceos_amd import caseio, rve
+    X, conn = meshgen.cube(4, perturb=0.15, seed=11)
+    X = po.periodic_perturbation(X, 0.25)
+    mat = rve.phases(4, 5).astype(np.int32) + 1
+    cw.write_multiscale_case(str(tmp_path), X, conn, mat)
+    case = caseio.Case(str(tmp_path))
+    s = case.settings
+    assert s["multiscale"] and s["preprocessor"] == "hypermesh" and s["newton_tol"] == 1e-10
+    assert s["materials"] == {1: (3, [0.5, 50.0]), 2: (3, [5.0, 500.0])}
+    assert np.allclose(case.mesh["coords"], X, rtol=0, atol=5e-9)  # E16.9 in the deck
+    assert np.array_equal(case.mesh["conn"], conn) and np.array_equal(case.mesh["mat"], mat)
+    b = case.boundary_nodes()
+    lo, hi = X.min(axis=0), X.max(axis=0)
+    on = np.any((np.abs(X - lo) < 1e-9) | (np.abs(X - hi) < 1e-9), axis=1)
+    assert np.array_equal(np.sort(b), np.nonzero(on)[0])
+    F = case.macro_F(0.5)
+    assert np.allclose(F, [[1.05, 0.025, 0.0], [0.0, 1.0, 0.0], [0.0, 0.0, 0.975]], rtol=0, atol=1e-15)
+    assert [p.location for p in case.probes] == ["microstructure"] * 3 + ["gausspoint"]
+    assert case.ndof == 3 * len(X) and len(case.fixed_dof) == 0 and len(case.presc_dof) == 0
+    # the pairing from the boundary set alone equals the one from every node
+    pm_b = periodic.periodic_map(case.mesh["coords"], boundary=b)
+    pm_a = periodic.periodic_map(case.mesh["coords"])
+    assert np.array_equal(pm_b.node_map, pm_a.node_map)
