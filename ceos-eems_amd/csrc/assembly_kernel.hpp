@@ -9,10 +9,9 @@
 // (they have no contribution here and are skipped), as are the Fint parts of their nodes.
 //
 // Work unit = one node block of the upper pattern (the diagonal block of node A, or the block
-// (A, B) of its q-th upper neighbour B), enumerated in CSR order: block blk belongs to node
-// blknode[blk] and is its q = blk - blkstart[A]-th block.  The thread sums the contributing
-// element blocks and writes its 3 pieces of the rows 3A..3A+2; consecutive threads write
-// consecutive pieces of the same rows.
+// (A, B) of its q-th upper neighbour B) that has boundary records, in CSR order.  The thread
+// sums the contributing patch records and writes its 3 pieces of the rows 3A..3A+2;
+// consecutive threads write neighbouring pieces of the same rows.
 // Contribution word: bits 0..30 = offset of the 3x3 block (or of the 6-value diagonal block)
 // in the record array, bit 31 = stored transposed (row-major values that are the columns of the
 // CSR block).  The element kernel writes patch records (element_kernel.hpp) already in the CSR
@@ -27,12 +26,8 @@ namespace ceos {
 struct GatherParams {
     int nn;
     long long nblocks;
-    long long blk0, blk1;    // the launch covers blocks [blk0, blk1)
-    const int* blknode;      // [nblocks] node A of block
-    const int* rowstart;     // [nn] slot of (3A, 3A)
-    const int* nup;          // [nn]
-    const int* blkstart;     // [nn] global block index of the diagonal block of A
-    const int* cptr;         // [nblocks+1] contribution ranges per block
+    long long blk0, blk1;    // the launch covers work-list entries [blk0, blk1)
+    const int4* glist;       // work list (k_gather_list)
     const uint32_t* cword;   // [contributions]
     const int* fptr;         // [nn+1] residual-only pass: Fe records of node A at fword[fptr[A]..]
     const uint32_t* fword;   // offsets (doubles) of the Fe records
@@ -48,35 +43,40 @@ struct GatherParams {
 // 0.397 ms per gather, 4 0.402 ms, 8 0.477 ms.
 constexpr int kGatherBatch = 2;
 
+// Work list of the gather: one entry per node block that has boundary records, in CSR order
+// (blocks written by the element kernel are absent, so no thread starts for them):
+//   x = first contribution, y = contributions | (diagonal block) << 30,
+//   z = K slot of the block's first entry in row 3A (diagonal: the row start; else the column 3B),
+//   w = A << 7 | nup[A].
+// Rows 3A+1 and 3A+2 of the block start at z + 3 nup + 3 and z + 6 nup + 5 (diagonal: at its
+// (1,1) and (2,2) entries) or at z + 3 nup + 2 and z + 6 nup + 3 (off-diagonal).
 template <typename KT>
-__global__ __launch_bounds__(256) void k_gather_blocks(GatherParams p) {
-    const long long blk = p.blk0 + (long long)blockIdx.x * 256 + threadIdx.x;
-    if (blk >= p.blk1) return;
-    const int c0 = p.cptr[blk], c1 = p.cptr[blk + 1];
-    if (c0 == c1) return;  // a complete block: its patch's workgroup wrote it (and Fint)
-    const int A = p.blknode[blk];
-    const int q = (int)(blk - p.blkstart[A]);
-    const int nu = p.nup[A];
-    const long r0 = p.rowstart[A], r1 = r0 + 3 + 3 * nu, r2 = r1 + 2 + 3 * nu;
+__global__ __launch_bounds__(256) void k_gather_list(GatherParams p) {
+    const long long i = p.blk0 + (long long)blockIdx.x * 256 + threadIdx.x;
+    if (i >= p.blk1) return;
+    const int4 d = p.glist[i];
+    const int c0 = d.x, c1 = d.x + (d.y & 0x3fffffff);
+    const long s = d.z;
+    const int A = d.w >> 7, nu = d.w & 127;
     KT* K = (KT*)p.K;
-    if (q == 0) {
+    if ((d.y >> 30) & 1) {
         // diagonal block, stored upper 00 01 02 11 12 22, preceded by the lane's Fe
         double a[6] = {0, 0, 0, 0, 0, 0}, f[3] = {0, 0, 0};
         for (int c = c0; c < c1; c += kGatherBatch) {
-            const double* d[kGatherBatch];
+            const double* dd[kGatherBatch];
 #pragma unroll
             for (int u = 0; u < kGatherBatch; ++u)
-                d[u] = (c + u < c1) ? p.rec + (long)(p.cword[c + u] & 0x7fffffffu) : nullptr;
+                dd[u] = (c + u < c1) ? p.rec + (long)(p.cword[c + u] & 0x7fffffffu) : nullptr;
             double v[kGatherBatch][9];
 #pragma unroll
             for (int u = 0; u < kGatherBatch; ++u)
-                if (d[u]) {
+                if (dd[u]) {
 #pragma unroll
-                    for (int k = 0; k < 9; ++k) v[u][k] = d[u][k - 3];
+                    for (int k = 0; k < 9; ++k) v[u][k] = dd[u][k - 3];
                 }
 #pragma unroll
             for (int u = 0; u < kGatherBatch; ++u)
-                if (d[u]) {
+                if (dd[u]) {
 #pragma unroll
                     for (int k = 0; k < 6; ++k) a[k] += v[u][3 + k];
 #pragma unroll
@@ -85,11 +85,10 @@ __global__ __launch_bounds__(256) void k_gather_blocks(GatherParams p) {
         }
 #pragma unroll
         for (int k = 0; k < 3; ++k) p.Fint[3L * A + k] = f[k];
-        if (K) {
-            K[r0] = (KT)a[0]; K[r0 + 1] = (KT)a[1]; K[r0 + 2] = (KT)a[2];
-            K[r1] = (KT)a[3]; K[r1 + 1] = (KT)a[4];
-            K[r2] = (KT)a[5];
-        }
+        const long r1 = s + 3 + 3 * nu, r2 = r1 + 2 + 3 * nu;
+        K[s] = (KT)a[0]; K[s + 1] = (KT)a[1]; K[s + 2] = (KT)a[2];
+        K[r1] = (KT)a[3]; K[r1 + 1] = (KT)a[4];
+        K[r2] = (KT)a[5];
         return;
     }
     double a[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
@@ -101,28 +100,28 @@ __global__ __launch_bounds__(256) void k_gather_blocks(GatherParams p) {
 #pragma unroll
         for (int u = 0; u < kGatherBatch; ++u)
             if (w[u] != 0xffffffffu) {
-                const double* d = p.rec + (long)(w[u] & 0x7fffffffu);
+                const double* dd = p.rec + (long)(w[u] & 0x7fffffffu);
 #pragma unroll
-                for (int k = 0; k < 9; ++k) v[u][k] = d[k];
+                for (int k = 0; k < 9; ++k) v[u][k] = dd[k];
             }
 #pragma unroll
         for (int u = 0; u < kGatherBatch; ++u) {
             if (w[u] == 0xffffffffu) continue;
             if (w[u] >> 31) {
 #pragma unroll
-                for (int i = 0; i < 3; ++i)
+                for (int r = 0; r < 3; ++r)
 #pragma unroll
-                    for (int k = 0; k < 3; ++k) a[3 * i + k] += v[u][3 * k + i];
+                    for (int k = 0; k < 3; ++k) a[3 * r + k] += v[u][3 * k + r];
             } else {
 #pragma unroll
                 for (int k = 0; k < 9; ++k) a[k] += v[u][k];
             }
         }
     }
-    const long o = 3L * (q - 1);
-    K[r0 + 3 + o] = (KT)a[0]; K[r0 + 4 + o] = (KT)a[1]; K[r0 + 5 + o] = (KT)a[2];
-    K[r1 + 2 + o] = (KT)a[3]; K[r1 + 3 + o] = (KT)a[4]; K[r1 + 4 + o] = (KT)a[5];
-    K[r2 + 1 + o] = (KT)a[6]; K[r2 + 2 + o] = (KT)a[7]; K[r2 + 3 + o] = (KT)a[8];
+    const long s1 = s + 3 * nu + 2, s2 = s + 6 * nu + 3;
+    K[s] = (KT)a[0]; K[s + 1] = (KT)a[1]; K[s + 2] = (KT)a[2];
+    K[s1] = (KT)a[3]; K[s1 + 1] = (KT)a[4]; K[s1 + 2] = (KT)a[5];
+    K[s2] = (KT)a[6]; K[s2 + 1] = (KT)a[7]; K[s2 + 2] = (KT)a[8];
 }
 
 // residual-only gather: Fint of node A from the Fe records of its patches, in patch order (the

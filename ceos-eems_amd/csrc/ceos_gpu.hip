@@ -222,7 +222,9 @@ struct ceos_handle {
     // ordering of the handle's own stream after device entries that ran on a caller's stream
     hipEvent_t ev_caller = nullptr;
     bool caller_pending = false;
-    int *dblknode = nullptr, *dblkstart = nullptr, *dcptr = nullptr;
+    int4* dglist = nullptr;      // gather work list (assembly_kernel.hpp)
+    long long n_glist = 0;
+    std::vector<long long> glist_pos;  // [nblocks + 1] work-list entries before every block
     uint32_t* dcword = nullptr;
     int hpart_n = 0;
 
@@ -358,7 +360,7 @@ static void free_device(ceos_handle* h) {
     dfree(h->dsig); dfree(h->dst_old); dfree(h->dst_new); dfree(h->dprops);
     dfree(h->dconn); dfree(h->dmat); dfree(h->drowstart); dfree(h->dnup); dfree(h->dperm);
     dfree(h->dpairs); dfree(h->ddiag_first); dfree(h->delem_err); dfree(h->derr); dfree(h->dhpart);
-    dfree(h->dscratch); dfree(h->dblknode); dfree(h->dblkstart); dfree(h->dcptr); dfree(h->dcword);
+    dfree(h->dscratch); dfree(h->dglist); dfree(h->dcword);
     dfree(h->dptab); dfree(h->dpinfo); dfree(h->dpslot); dfree(h->dfptr); dfree(h->dfword); dfree(h->dpinfo2);
     dfree(h->dpkslot); dfree(h->dfree_nodes);
     dfree(h->dup_ptr); dfree(h->dup_nodes); dfree(h->dslot_nbr); dfree(h->dslot_src); dfree(h->dsell_nbr);
@@ -416,7 +418,8 @@ static void patch_order(const ceos_handle* h, std::vector<int>& elems) {
     }
     const double hh = hsum / (double)n;
     if (!(hh > 0.0)) return;
-    const double tile[3] = {4.0 * hh, 2.0 * hh, 2.0 * hh};
+    // 4 x 4 x 2 elements for 32-element patches (4 x 2 x 2 for 16, 4 x 4 x 4 for 64)
+    const double tile[3] = {4.0 * hh, (kEPB >= 32 ? 4.0 : 2.0) * hh, (kEPB >= 64 ? 4.0 : 2.0) * hh};
     std::vector<std::pair<std::array<long long, 3>, int>> key(n);
     for (size_t i = 0; i < n; ++i) {
         std::array<long long, 3> t;
@@ -541,11 +544,11 @@ static int build_gather_map(ceos_handle* h) {
             const int r0 = half ? split : 0;
             if (r == r0) extra = (half ? nrec : split) - r0;
             const int first = rec[r].first, cnt = rec[r].second;
-            pslot.push_back((uint16_t)(extra | (cnt << 10)));
+            pslot.push_back((uint16_t)(extra | (cnt << kPosBits)));
             for (int i = first; i < first + cnt; ++i) {
                 const Piece& q = pc[i];
                 const int pos = i == first ? r - r0 : extra++;
-                ptab[(size_t)q.slot * kPieces + q.q] = (uint16_t)(pos | (half << 10) | ((q.tr ? 1 : 0) << 11));
+                ptab[(size_t)q.slot * kPieces + q.q] = (uint16_t)(pos | (half << kPosBits) | ((q.tr ? 1 : 0) << (kPosBits + 1)));
             }
             const int blk = pc[first].blk;
             const int A = blknode[blk];
@@ -596,9 +599,23 @@ static int build_gather_map(ceos_handle* h) {
         std::vector<int> ff(fptr.begin(), fptr.end() - 1);
         for (size_t i = 0; i < fe_node.size(); ++i) fword[ff[fe_node[i]]++] = (uint32_t)(3 * i);
     }
-    HIP_OK(dalloc_once(h->dblkstart, nn));
-    HIP_OK(dalloc_once(h->dblknode, nb));
-    HIP_OK(dalloc_once(h->dcptr, nb + 1));
+    // the work list of the gather: the blocks with boundary records, in CSR order
+    std::vector<int4> glist;
+    glist.reserve((size_t)h->n_gather_blocks);
+    h->glist_pos.assign(nb + 1, 0);
+    for (long long b = 0; b < nb; ++b) {
+        h->glist_pos[b] = (long long)glist.size();
+        const int cnt = cptr[b + 1] - cptr[b];
+        if (!cnt) continue;
+        const int A = blknode[b];
+        const int q = (int)(b - blkstart[A]);
+        const long long slot = (long long)h->rowstart[A] + (q ? 3 + 3LL * (q - 1) : 0);
+        if (cnt >= (1 << 30) || slot >= (1LL << 31)) return CEOS_ERR_ARG;
+        glist.push_back(make_int4(cptr[b], cnt | (q == 0 ? (1 << 30) : 0), (int)slot, (A << 7) | h->nup[A]));
+    }
+    h->glist_pos[nb] = (long long)glist.size();
+    h->n_glist = (long long)glist.size();
+    HIP_OK(dalloc(h->dglist, glist.size()));
     HIP_OK(dalloc_fit(h->dcword, h->cword_cap, cword.size()));
     HIP_OK(dalloc_once(h->dptab, ptab.size()));
     HIP_OK(dalloc_fit(h->dpinfo, h->pinfo_cap, pinfo.size()));
@@ -607,9 +624,8 @@ static int build_gather_map(ceos_handle* h) {
     HIP_OK(dalloc_fit(h->dpkslot, h->pkslot_cap, pkslot.size()));
     HIP_OK(dalloc_once(h->dfptr, nn + 1));
     HIP_OK(dalloc_fit(h->dfword, h->fword_cap, fword.size()));
-    HIP_OK(hipMemcpy(h->dblkstart, blkstart.data(), sizeof(int) * nn, hipMemcpyHostToDevice));
-    HIP_OK(hipMemcpy(h->dblknode, blknode.data(), sizeof(int) * nb, hipMemcpyHostToDevice));
-    HIP_OK(hipMemcpy(h->dcptr, cptr.data(), sizeof(int) * (nb + 1), hipMemcpyHostToDevice));
+    if (!glist.empty())
+        HIP_OK(hipMemcpy(h->dglist, glist.data(), sizeof(int4) * glist.size(), hipMemcpyHostToDevice));
     HIP_OK(hipMemcpy(h->dcword, cword.data(), sizeof(uint32_t) * cword.size(), hipMemcpyHostToDevice));
     HIP_OK(hipMemcpy(h->dptab, ptab.data(), sizeof(uint16_t) * ptab.size(), hipMemcpyHostToDevice));
     HIP_OK(hipMemcpy(h->dpinfo, pinfo.data(), sizeof(int4) * pinfo.size(), hipMemcpyHostToDevice));
@@ -992,12 +1008,8 @@ static int evaluate(ceos_handle* h, const double* dU, double* dFint, void* dK, h
     g.nn = h->nn;
     g.nblocks = h->nblocks;
     g.blk0 = 0;
-    g.blk1 = h->nblocks;
-    g.blknode = h->dblknode;
-    g.rowstart = h->drowstart;
-    g.nup = h->dnup;
-    g.blkstart = h->dblkstart;
-    g.cptr = h->dcptr;
+    g.blk1 = h->n_glist;
+    g.glist = h->dglist;
     g.cword = h->dcword;
     g.fptr = h->dfptr;
     g.fword = h->dfword;
@@ -1022,12 +1034,12 @@ static int evaluate(ceos_handle* h, const double* dU, double* dFint, void* dK, h
             }
             HIP_OK(hipEventRecord(h->pipe_ev[k], s));
             HIP_OK(hipStreamWaitEvent(s2, h->pipe_ev[k], 0));
-            g.blk0 = h->pipe_blk[k];
-            g.blk1 = h->pipe_blk[k + 1];
+            g.blk0 = h->glist_pos[h->pipe_blk[k]];
+            g.blk1 = h->glist_pos[h->pipe_blk[k + 1]];
             if (g.blk1 > g.blk0) {
                 const unsigned nbk = (unsigned)((g.blk1 - g.blk0 + 255) / 256);
-                if (f32) hipLaunchKernelGGL(k_gather_blocks<float>, dim3(nbk), dim3(256), 0, s2, g);
-                else hipLaunchKernelGGL(k_gather_blocks<double>, dim3(nbk), dim3(256), 0, s2, g);
+                if (f32) hipLaunchKernelGGL(k_gather_list<float>, dim3(nbk), dim3(256), 0, s2, g);
+                else hipLaunchKernelGGL(k_gather_list<double>, dim3(nbk), dim3(256), 0, s2, g);
             }
         }
         if (ev) HIP_OK(hipEventRecord(ev[1], s));
@@ -1061,9 +1073,11 @@ static int evaluate(ceos_handle* h, const double* dU, double* dFint, void* dK, h
     if (ev) HIP_OK(hipEventRecord(ev[1], s));
     if (gather) {
         if (wk) {
-            const unsigned nbk = (unsigned)((h->nblocks + 255) / 256);
-            if (f32) hipLaunchKernelGGL(k_gather_blocks<float>, dim3(nbk), dim3(256), 0, s, g);
-            else hipLaunchKernelGGL(k_gather_blocks<double>, dim3(nbk), dim3(256), 0, s, g);
+            if (h->n_glist) {
+                const unsigned nbk = (unsigned)((h->n_glist + 255) / 256);
+                if (f32) hipLaunchKernelGGL(k_gather_list<float>, dim3(nbk), dim3(256), 0, s, g);
+                else hipLaunchKernelGGL(k_gather_list<double>, dim3(nbk), dim3(256), 0, s, g);
+            }
         } else {
             hipLaunchKernelGGL(k_gather_fint, dim3((h->nn + 255) / 256), dim3(256), 0, s, g);
         }

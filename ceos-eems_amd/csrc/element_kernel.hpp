@@ -31,7 +31,7 @@
 
 namespace ceos {
 
-constexpr int kEPB = 16;              // elements per workgroup (128 threads = 2 waves)
+constexpr int kEPB = 32;              // elements per workgroup (256 threads = 4 waves)
 // doubles per element in LDS between the phases: J^-1 of the current configuration 8x9, w detJ
 // sigma 8x6, w detJ D (upper) 8x21 = 288.  Phase 2 recomputes grad_x N from J^-1 and the natural
 // derivatives (the 192 doubles of grad_x N itself would take 52.7 KB per workgroup, 3 workgroups
@@ -75,7 +75,12 @@ constexpr int kStateStride = 20;      // state rows per Gauss point in the devic
 // in the F+K pass.
 // The staging area of one half holds at most kStageMax pieces.
 constexpr int kPieces = 36;
-constexpr int kStageMax = (kEPB * kLdsPerElem) / 9;  // 526 pieces of 9 doubles in the LDS slices
+constexpr int kStageMax = (kEPB * kLdsPerElem) / 9;  // 1052 pieces of 9 doubles in the LDS slices
+// staging positions in the tables: ptab bits 0..11 position, bit 12 half, bit 13 transposed;
+// pslot bits 0..11 position of the second contribution, bits 12..15 contributions (at most 8)
+constexpr int kPosBits = 12;
+constexpr unsigned kPosMask = (1u << kPosBits) - 1u;
+static_assert(kStageMax <= (1 << kPosBits), "staging positions need more bits");
 __host__ __device__ constexpr int piece_base(int b) { return b < 4 ? 5 * b : 20 + 4 * (b - 4); }
 
 struct ElemParams {
@@ -97,7 +102,7 @@ struct ElemParams {
                                    // residual-only pass
     const uint16_t* ptab;          // [ne][36] staging entries of the pieces (gather assembly)
     const int4* pinfo;             // [patches] record offset, slot-table offset, records, split
-    const uint16_t* pslot;         // per patch record: position of its 2nd contribution | contributions << 10
+    const uint16_t* pslot;         // per patch record: position of its 2nd contribution | contributions << 12
     const int4* pinfo2;            // [patches] boundary / boundary diagonal records, Fe offset, complete diagonal
     const int2* pkslot;            // [patch records] complete records: K slot of row 0, node << 7 | nup
     const double* st_old;          // [nstate][8][ne] committed state
@@ -320,7 +325,7 @@ __device__ __forceinline__ void node_blocks(const double* sm, int b, double& fe0
 // NUMT: tangent modulus by central differences (a separate instantiation, so that the default
 // kernel carries no call and keeps its register allocation)
 template <int MODEL, bool NL, bool WANT_K, typename KT, bool GATHER, bool NUMT = false>
-__global__ __launch_bounds__(128) void k_element(ElemParams p, int e_begin, int e_end, int patch_begin) {
+__global__ __launch_bounds__(kEPB * 8) void k_element(ElemParams p, int e_begin, int e_end, int patch_begin) {
     __shared__ double smem[kEPB * kLdsPerElem];
     __shared__ int snode[GATHER ? 1 : kEPB][8];  // partner node ids (scatter assembly only)
     __shared__ double sdn[8 * 8 * 4];             // dN/dxi of node a at Gauss point g: [g][a][3 + pad]
@@ -546,8 +551,8 @@ __global__ __launch_bounds__(128) void k_element(ElemParams p, int e_begin, int 
 #pragma unroll
         for (int half = 0; half < 2; ++half) {
             __syncthreads();
-            if (active && ((t >> 10) & 1u) == (unsigned)half) {
-                double* dst = smem + 9 * (t & 1023u);
+            if (active && ((t >> kPosBits) & 1u) == (unsigned)half) {
+                double* dst = smem + 9 * (t & kPosMask);
                 dst[0] = fe0;
                 dst[1] = fe1;
                 dst[2] = fe2;
@@ -561,8 +566,8 @@ __global__ __launch_bounds__(128) void k_element(ElemParams p, int e_begin, int 
                 if (r < r0 || r >= r1 || !(r < nbd || (r >= cd0 && r < cd1))) continue;
                 const unsigned w = psw[i];
                 double* dst = smem + 9 * (r - r0);
-                const double* ext = smem + 9 * (w & 1023u);
-                const int cnt = (int)(w >> 10);
+                const double* ext = smem + 9 * (w & kPosMask);
+                const int cnt = (int)(w >> kPosBits);
                 double a0 = dst[0], a1 = dst[1], a2 = dst[2];
 #pragma unroll
                 for (int c = 1; c < 8; ++c) {
@@ -604,15 +609,15 @@ __global__ __launch_bounds__(128) void k_element(ElemParams p, int e_begin, int 
                 for (int k = 0; k < 5; ++k) {
                     if (k <= npart) {
                         const unsigned t = ptv[k];
-                        if (((t >> 10) & 1u) == (unsigned)half) {
-                            double* dst = smem + 9 * (t & 1023u);
+                        if (((t >> kPosBits) & 1u) == (unsigned)half) {
+                            double* dst = smem + 9 * (t & kPosMask);
                             if (k == 0) {
                                 dst[0] = fe0;
                                 dst[1] = fe1;
                                 dst[2] = fe2;
 #pragma unroll
                                 for (int q = 0; q < 6; ++q) dst[3 + q] = kd[q];
-                            } else if ((t >> 11) & 1u) {
+                            } else if ((t >> (kPosBits + 1)) & 1u) {
 #pragma unroll
                                 for (int r = 0; r < 3; ++r)
 #pragma unroll
@@ -635,8 +640,8 @@ __global__ __launch_bounds__(128) void k_element(ElemParams p, int e_begin, int 
                 if (r < r0 || r >= r1) continue;
                 const unsigned w = psw[i];
                 double* dst = smem + 9 * (r - r0);
-                const double* ext = smem + 9 * (w & 1023u);
-                const int cnt = (int)(w >> 10);
+                const double* ext = smem + 9 * (w & kPosMask);
+                const int cnt = (int)(w >> kPosBits);
                 double acc[9];
 #pragma unroll
                 for (int q = 0; q < 9; ++q) acc[q] = dst[q];
