@@ -39,9 +39,10 @@ struct GatherParams {
 // Contributions are taken kGatherBatch at a time: their words, then all their record pieces are
 // requested before any is summed (a thread keeps up to 2 x 9 loads in flight instead of one
 // word -> piece chain per contribution).  The sums still run in contribution order.  With patch
-// records a node block has 1.5 contributions on average (80^3 cube): batches of 2 measured
-// 0.397 ms per gather, 4 0.402 ms, 8 0.477 ms.
-constexpr int kGatherBatch = 2;
+// records and 32-element patches a listed block has 2.3 contributions on average (80^3 cube):
+// batches of 2 measured 0.237 ms per gather, 3 0.236 ms, 4 0.233 ms; non-temporal record loads
+// 0.317 ms (the records the element kernel just wrote are still in the last-level cache).
+constexpr int kGatherBatch = 4;
 
 // Work list of the gather: one entry per node block that has boundary records, in CSR order
 // (blocks written by the element kernel are absent, so no thread starts for them):
