@@ -35,12 +35,14 @@ struct ceos_rve_batch {
     int precision = CEOS_K_FP64;  // storage of the reduced tangent
     int cg_group = kCgGroupDefault;  // nodes per thread and operand batch in the CG sweeps
     double tangent_rtol = -1.0;   // < 0: pcg_rtol
+    int coarse = 1;               // two-level CG preconditioner (aggregate translations)
     // device
     int *dorder = nullptr, *dside = nullptr, *dinfo = nullptr, *dqueue = nullptr;
     long long *dnode_off = nullptr, *dphase_off = nullptr;
     uint8_t* dphase = nullptr;
     double *dprops = nullptr, *dFbar = nullptr, *dFconv = nullptr, *dw = nullptr, *dP = nullptr, *dA = nullptr;
     double *dM = nullptr, *dV = nullptr;
+    float* dC = nullptr;  // [slots][kRveMaxCoarse^2] coarse inverses
     long long* dprof = nullptr;
     // subset solves: selected ids in queue order and in caller order, compact F / P / A / info
     int *dsel_q = nullptr, *dsel_u = nullptr, *dinfo_s = nullptr;
@@ -64,7 +66,7 @@ void free_all(ceos_rve_batch* b) {
     rfree(b->dorder); rfree(b->dside); rfree(b->dinfo); rfree(b->dqueue);
     rfree(b->dnode_off); rfree(b->dphase_off); rfree(b->dphase);
     rfree(b->dprops); rfree(b->dFbar); rfree(b->dFconv); rfree(b->dw); rfree(b->dP); rfree(b->dA);
-    rfree(b->dM); rfree(b->dV); rfree(b->dprof);
+    rfree(b->dM); rfree(b->dV); rfree(b->dC); rfree(b->dprof);
     rfree(b->dsel_q); rfree(b->dsel_u); rfree(b->dinfo_s); rfree(b->dF_s); rfree(b->dP_s); rfree(b->dA_s);
 }
 
@@ -174,6 +176,8 @@ int run(ceos_rve_batch* b, bool want_A, hipStream_t s, const int* order, int n_q
     // 256 MB last-level cache can hold between CG iterations.
     p.nt_tangent = n_queue >= kRveNtQueue ? 1 : 0;
     p.prof = b->dprof;
+    p.Cbuf = b->dC;
+    p.coarse = b->coarse;
     hipLaunchKernelGGL(k_zero_queue, dim3(1), dim3(1), 0, s, b->dqueue);
     const int grid = std::min(n_queue, b->slots);
     const bool f32 = b->precision == CEOS_K_FP32_STORED;
@@ -234,7 +238,7 @@ int ceos_rve_batch_create(int n_rve, const int* side, const unsigned char* phase
         delete b;
         return CEOS_ERR_HIP;
     }
-    b->slots = cus;  // ~107 KB of LDS per 512-thread workgroup: one per CU
+    b->slots = cus;  // ~155 KB of LDS per 256-thread workgroup: one per CU
     const long long slots = std::min<long long>(b->slots, n_rve);
     bool okm =
         rmalloc(b->dorder, n_rve) == hipSuccess && rmalloc(b->dside, n_rve) == hipSuccess &&
@@ -246,6 +250,7 @@ int ceos_rve_batch_create(int n_rve, const int* side, const unsigned char* phase
         rmalloc(b->dA, 81LL * n_rve) == hipSuccess && rmalloc(b->dprof, 2LL * n_rve) == hipSuccess &&
         rmalloc(b->dM, (size_t)(243 * b->max_nodes * slots)) == hipSuccess &&
         rmalloc(b->dV, (size_t)(kRveVecPerNode * b->max_nodes * slots)) == hipSuccess &&
+        rmalloc(b->dC, (size_t)kRveMaxCoarse * kRveMaxCoarse * slots) == hipSuccess &&
         rmalloc(b->dsel_q, n_rve) == hipSuccess && rmalloc(b->dsel_u, n_rve) == hipSuccess &&
         rmalloc(b->dinfo_s, 4LL * n_rve) == hipSuccess && rmalloc(b->dF_s, 9LL * n_rve) == hipSuccess &&
         rmalloc(b->dP_s, 9LL * n_rve) == hipSuccess && rmalloc(b->dA_s, 81LL * n_rve) == hipSuccess;
@@ -308,6 +313,12 @@ int ceos_rve_set_precision(ceos_rve_batch* b, int mode) {
 int ceos_rve_set_tangent_rtol(ceos_rve_batch* b, double rtol) {
     if (!b || !(rtol > 0.0 && rtol < 1.0)) return CEOS_ERR_ARG;
     b->tangent_rtol = rtol;
+    return CEOS_OK;
+}
+
+int ceos_rve_set_coarse(ceos_rve_batch* b, int on) {
+    if (!b || (on != 0 && on != 1)) return CEOS_ERR_ARG;
+    b->coarse = on;
     return CEOS_OK;
 }
 

@@ -37,12 +37,21 @@ constexpr int kRveWaves = kRveThreads / 64;
 constexpr int kRveElemPerRound = 32;          // assembly: threads 0..255, 8 lanes per element
 constexpr int kRveAsmThreads = 8 * kRveElemPerRound;
 constexpr int kRveMaxSide = 16;
-// the assembly's element slices and, in the CG, p of the largest RVE share the LDS
-constexpr int kRveLds = kRveElemPerRound * kLdsPerElem > 3 * kRveMaxSide * kRveMaxSide * kRveMaxSide
-                            ? kRveElemPerRound * kLdsPerElem
-                            : 3 * kRveMaxSide * kRveMaxSide * kRveMaxSide;  // 12288 doubles
+// Coarse level of the CG preconditioner: the three translations of every aggregate of
+// kRveAgg^3 torus nodes (the last aggregate along an axis is shorter when kRveAgg does not
+// divide n), node 0 left out.  At most 4^3 aggregates: 192 coarse unknowns.
+constexpr int kRveAgg = 4;
+constexpr int kRveMaxAggSide = (kRveMaxSide + kRveAgg - 1) / kRveAgg;
+constexpr int kRveMaxCoarse = 3 * kRveMaxAggSide * kRveMaxAggSide * kRveMaxAggSide;
+constexpr int kRveCoarsePacked = kRveMaxCoarse * (kRveMaxCoarse + 1) / 2;  // 18528 doubles
+constexpr int kRveMax3(int a, int b, int c) { return a > b ? (a > c ? a : c) : (b > c ? b : c); }
+// the assembly's element slices, the coarse matrix while it is inverted (packed upper triangle,
+// FP64) and, in the CG, p of the largest RVE share the LDS: 148 KB
+constexpr int kRveLds = kRveMax3(kRveElemPerRound * kLdsPerElem, 3 * kRveMaxSide * kRveMaxSide * kRveMaxSide,
+                                 kRveCoarsePacked);
 constexpr int kRveVecPerNode = 51;  // w 3, R 3, x 3, r 3, q 3, z 3, Minv 6, B 27
 static_assert(kRveLds >= 3 * kRveMaxSide * kRveMaxSide * kRveMaxSide, "p must fit in LDS");
+static_assert(kRveMaxCoarse <= kRveThreads, "one thread per coarse unknown");
 
 struct RveParams {
     int n_rve;
@@ -69,6 +78,8 @@ struct RveParams {
     int nt_tangent;               // 1: launch the kernels whose CG reads the reduced tangent with
                                   // non-temporal loads (NT = true)
     long long* prof;              // [n_rve][2] clock ticks in assembly, in CG (or nullptr)
+    float* Cbuf;                  // [slots][kRveMaxCoarse^2] inverse of the coarse matrix
+    int coarse;                   // 1: two-level preconditioner (block Jacobi + aggregate translations)
 };
 
 __device__ __forceinline__ double wave_sum(double v) {
@@ -132,6 +143,8 @@ struct RveCtx {
     double* wconv;  // converged fluctuation of this RVE (global state)
     MT* M;          // reduced tangent: FP64, or FP32 storage read into FP64 products
     double *wc, *R, *x, *r, *q, *z, *Minv, *B;
+    int na, nc;     // aggregates per axis, coarse unknowns (3 na^3)
+    float* C;       // [nc][nc] inverse of the coarse matrix (symmetric)
 };
 
 // Element pass over the whole RVE at the macro gradient Fa: zero and assemble M (if WANT_K) and
@@ -359,17 +372,187 @@ __device__ __forceinline__ void apply_minv(const double* mi, const double r[3], 
     z[2] = mi[2] * r[0] + mi[4] * r[1] + mi[5] * r[2];
 }
 
+// packed row-major upper triangle of an nc x nc matrix: entry (i, j), i <= j
+__device__ __forceinline__ int coarse_upk(int i, int j, int nc) { return i * nc - (i * (i - 1)) / 2 + (j - i); }
+
+// Coarse matrix A_c = P^T K P of the aggregate translations (P maps the coarse unknowns of an
+// aggregate to the three unknowns of each of its nodes; node 0 and its column left out), then
+// inverted.  One wave per aggregate, lane = node (lane & 3, lane >> 2 & 3, lane >> 4) of its
+// block: the lane reads the 27 neighbour blocks of its node once (the same rows as the CG product)
+// and sorts them into 8 classes by the axes along which the step leaves the aggregate; the wave
+// then sums, for every neighbour aggregate b >= a, the classes of its lanes that land on b (in
+// the fixed butterfly order of wave_sum; one aggregate is one wave's, so the adds need no
+// atomics).  The packed upper triangle (FP64, Ap in LDS) is
+// inverted by the symmetric sweep operator (pivot k: a_kk <- -1 / a_kk, a_ik <- a_ik / a_kk,
+// a_ij <- a_ij - a_ik a_kj / a_kk), which leaves -A_c^-1 (A_c is SPD: no pivoting).  The inverse
+// goes to c.C in FP32, both triangles from the same value, so the preconditioner stays exactly
+// symmetric.  rk: nc doubles of LDS.  Returns false (no coarse level for this tangent) if a
+// pivot is not positive.
+template <typename MT>
+__device__ bool rve_coarse_setup(const RveCtx<MT>& c, double* Ap, double* rk) {
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int n = c.n, na = c.na, nc = c.nc;
+    const long long nn = c.nn;
+    const int np = nc * (nc + 1) / 2;
+    for (int e = tid; e < np; e += kRveThreads) Ap[e] = 0.0;
+    __syncthreads();
+    const int lx = lane & 3, ly = (lane >> 2) & 3, lz = lane >> 4;
+    const int cnt = na >= 3 ? 3 : na, d0 = na >= 3 ? -1 : 0;  // distinct neighbour aggregates per axis
+    for (int a = wave; a < na * na * na; a += kRveWaves) {
+        const int ax = a % na, ay = (a / na) % na, az = a / (na * na);
+        const int x0 = kRveAgg * ax, y0 = kRveAgg * ay, z0 = kRveAgg * az;
+        const int wx = min(kRveAgg, n - x0), wy = min(kRveAgg, n - y0), wz = min(kRveAgg, n - z0);  // >= 2
+        const bool live = lx < wx && ly < wy && lz < wz;
+        const int x = x0 + lx, y = y0 + ly, z = z0 + lz;
+        const int i = live ? x + n * (y + n * z) : 0;
+        // two passes, by whether the step leaves the aggregate along z (4 classes of 9 sums each
+        // in registers); the second pass adds to what the first wrote
+#pragma unroll 1
+        for (int zp = 0; zp < 2; ++zp) {
+            double acc[4][9];
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+#pragma unroll
+                for (int k = 0; k < 9; ++k) acc[q][k] = 0.0;
+            if (i != 0) {
+#pragma unroll 1
+                for (int o = 0; o < 27; ++o) {
+                    const int dx = o % 3 - 1, dy = (o / 3) % 3 - 1, dz = o / 9 - 1;
+                    if ((lz + dz < 0 || lz + dz >= wz ? 1 : 0) != zp) continue;
+                    const int xx = (x + dx + n) % n, yy = (y + dy + n) % n, zz = (z + dz + n) % n;
+                    if (xx + n * (yy + n * zz) == 0) continue;  // node 0's column
+                    const int cls = (lx + dx < 0 || lx + dx >= wx ? 1 : 0) + (ly + dy < 0 || ly + dy >= wy ? 2 : 0);
+                    double m[9];
+#pragma unroll
+                    for (int k = 0; k < 9; ++k) m[k] = (double)c.M[((long long)o * 9 + k) * nn + i];
+#pragma unroll
+                    for (int q = 0; q < 4; ++q)
+                        if (q == cls) {
+#pragma unroll
+                            for (int k = 0; k < 9; ++k) acc[q][k] += m[k];
+                        }
+                }
+            }
+            // the aggregate each class lands on: the neighbour along every axis whose bit is set,
+            // on the side where the lane sits (a block is at least 2 nodes wide: never both sides)
+            int bt[4];
+            const int bz = (az + (zp ? (lz == 0 ? -1 : 1) : 0) + na) % na;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int bx = (ax + ((q & 1) ? (lx == 0 ? -1 : 1) : 0) + na) % na;
+                const int by = (ay + ((q & 2) ? (ly == 0 ? -1 : 1) : 0) + na) % na;
+                bt[q] = bx + na * (by + na * bz);
+            }
+            for (int t = 0; t < cnt * cnt * cnt; ++t) {
+                const int tx = (ax + d0 + t % cnt + na) % na, ty = (ay + d0 + (t / cnt) % cnt + na) % na,
+                          tz = (az + d0 + t / (cnt * cnt) + na) % na;
+                const int b = tx + na * (ty + na * tz);
+                if (b < a) continue;  // the lower triangle is the transpose
+                double mine = 0.0;
+#pragma unroll
+                for (int k = 0; k < 9; ++k) {
+                    double v = 0.0;
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) v += bt[q] == b ? acc[q][k] : 0.0;
+                    v = wave_sum(v);
+                    if (lane == k) mine = v;
+                }
+                if (lane < 9) {
+                    const int r = 3 * a + lane / 3, col = 3 * b + lane % 3;
+                    if (r <= col) Ap[coarse_upk(r, col, nc)] += mine;
+                }
+            }
+        }
+    }
+    // this thread's first packed entry e = tid as (row i, offset j in the row: column i + j)
+    int i0 = 0, j0 = tid;
+    while (i0 < nc && j0 >= nc - i0) {
+        j0 -= nc - i0;
+        ++i0;
+    }
+    for (int k = 0; k < nc; ++k) {
+        __syncthreads();  // the previous pivot's updates (or the assembly) complete
+        for (int t = tid; t < nc; t += kRveThreads) rk[t] = Ap[t <= k ? coarse_upk(t, k, nc) : coarse_upk(k, t, nc)];
+        __syncthreads();
+        const double piv = rk[k];
+        if (!(piv > 0.0)) return false;  // the same value for every thread
+        const double pinv = 1.0 / piv;
+        int i = i0, j = j0;
+        for (int e = tid; e < np; e += kRveThreads) {
+            const int col = i + j;
+            const double v = Ap[e];
+            Ap[e] = i == k ? (col == k ? -pinv : v * pinv) : (col == k ? v * pinv : v - rk[i] * rk[col] * pinv);
+            j += kRveThreads;
+            while (i < nc && j >= nc - i) {
+                j -= nc - i;
+                ++i;
+            }
+        }
+    }
+    __syncthreads();
+    for (int e = tid; e < nc * nc; e += kRveThreads) {
+        const int r = e / nc, col = e - r * nc;
+        c.C[e] = (float)(-Ap[r <= col ? coarse_upk(r, col, nc) : coarse_upk(col, r, nc)]);
+    }
+    __syncthreads();
+    return true;
+}
+
+// Restriction rc = P^T r into cv[0, nc) and the coarse correction yc = A_c^-1 rc into
+// cv[nc, 2 nc); returns this thread's part of rc . yc (threads t < nc: unknown t = 3 a + comp).
+// r must be complete (a barrier before the call); cv[nc, 2 nc) may be read after the caller's
+// next barrier.  r is zero at node 0, so the restriction needs no mask.
+template <typename MT>
+__device__ double rve_coarse_apply(const RveCtx<MT>& c, double* cv) {
+    const int tid = threadIdx.x, n = c.n, na = c.na, nc = c.nc;
+    if (tid < nc) {
+        const int a = tid / 3, comp = tid - 3 * a;
+        const int x0 = kRveAgg * (a % na), y0 = kRveAgg * ((a / na) % na), z0 = kRveAgg * (a / (na * na));
+        const int wx = min(kRveAgg, n - x0), wy = min(kRveAgg, n - y0), wz = min(kRveAgg, n - z0);
+        double s = 0.0;
+        for (int lz = 0; lz < wz; ++lz)
+            for (int ly = 0; ly < wy; ++ly) {
+                const double* rp = c.r + 3LL * (x0 + n * (y0 + ly + n * (z0 + lz))) + comp;
+                double v[kRveAgg];
+#pragma unroll
+                for (int lx = 0; lx < kRveAgg; ++lx) v[lx] = lx < wx ? rp[3 * lx] : 0.0;
+#pragma unroll
+                for (int lx = 0; lx < kRveAgg; ++lx) s += v[lx];
+            }
+        cv[tid] = s;
+    }
+    __syncthreads();
+    double part = 0.0;
+    if (tid < nc) {
+        const float* Cp = c.C + tid;
+        double y = 0.0;
+#pragma unroll 8
+        for (int k = 0; k < nc; ++k) y += (double)Cp[(long long)k * nc] * cv[k];
+        cv[nc + tid] = y;
+        part = cv[tid] * y;
+    }
+    return part;
+}
+
+// coarse unknown index 3 a of node i's aggregate
+__device__ __forceinline__ int rve_agg3(int i, int n, int na) {
+    const int ix = i % n, iy = (i / n) % n, iz = i / (n * n);
+    return 3 * (ix / kRveAgg + na * (iy / kRveAgg + na * (iz / kRveAgg)));
+}
+
 // preconditioned CG on the reduced system: M x = sign * rhs (node 0 fixed) to
 // ||r|| <= rtol ||b||.  p lives in LDS, the other vectors in the workgroup's scratch.  The vector
 // sweeps take G nodes per thread at a time and load all their operands before computing (one
-// round trip per group); G is chosen among kCgGroups by ceos_rve_autotune.  Returns the
+// round trip per group); G is chosen among kCgGroups by ceos_rve_autotune.  Preconditioner:
+// block Jacobi, plus (coarse) the aggregate translations, z = D^-1 r + P A_c^-1 P^T r (additive
+// two-level, SPD; r . z gains rc . yc).  cv: 2 kRveMaxCoarse doubles of LDS.  Returns the
 // iterations used.
 constexpr int kCgGroups[3] = {2, 4, 8};
 constexpr int kCgGroupDefault = 4;
 
 template <typename MT, int G, bool NT>
 __device__ int rve_pcg(const RveCtx<MT>& c, const RveParams& p, const double* rhs, double sign, double rtol,
-                       double* pl, double* red) {
+                       double* pl, double* red, double* cv, bool coarse) {
     const int tid = threadIdx.x;
     const int nn = c.nn;
     constexpr int S = kRveThreads * G;
@@ -404,10 +587,23 @@ __device__ int rve_pcg(const RveCtx<MT>& c, const RveParams& p, const double* rh
             }
         }
     }
-    wg_sum<2>(acc, red);
-    double rz = acc[0];
-    const double tol2 = rtol * rtol * acc[1];
-    if (!(acc[1] > 0.0)) return 0;
+    double sums[3] = {acc[0], acc[1], 0.0};
+    if (coarse) {
+        __syncthreads();  // r complete
+        sums[2] = rve_coarse_apply(c, cv);
+    }
+    wg_sum<3>(sums, red);
+    double rz = sums[0] + sums[2];
+    const double tol2 = rtol * rtol * sums[1];
+    if (!(sums[1] > 0.0)) return 0;
+    if (coarse) {
+        for (int i = tid; i < nn; i += kRveThreads) {
+            if (i == 0) continue;
+            const int a3 = rve_agg3(i, c.n, c.na);
+#pragma unroll
+            for (int k = 0; k < 3; ++k) pl[3 * i + k] += cv[c.nc + a3 + k];
+        }
+    }
     int it = 0;
     while (it < p.pcg_maxit) {
         ++it;
@@ -468,6 +664,11 @@ __device__ int rve_pcg(const RveCtx<MT>& c, const RveParams& p, const double* rh
         }
         wg_sum<2>(s2, red);
         if (s2[0] <= tol2) break;
+        if (coarse) {
+            double cz[1] = {rve_coarse_apply(c, cv)};  // r complete: wg_sum ended on a barrier
+            wg_sum<1>(cz, red);
+            s2[1] += cz[0];
+        }
         const double beta = s2[1] / rz;
         rz = s2[1];
         for (int i0 = tid; i0 < nn; i0 += S) {
@@ -484,8 +685,14 @@ __device__ int rve_pcg(const RveCtx<MT>& c, const RveParams& p, const double* rh
             for (int g = 0; g < G; ++g) {
                 const int i = i0 + g * kRveThreads;
                 if (i >= nn) continue;
+                double yc[3] = {0.0, 0.0, 0.0};
+                if (coarse && i != 0) {
+                    const int a3 = rve_agg3(i, c.n, c.na);
 #pragma unroll
-                for (int k = 0; k < 3; ++k) pl[3 * i + k] = zv[g][k] + beta * pl[3 * i + k];
+                    for (int k = 0; k < 3; ++k) yc[k] = cv[c.nc + a3 + k];
+                }
+#pragma unroll
+                for (int k = 0; k < 3; ++k) pl[3 * i + k] = zv[g][k] + yc[k] + beta * pl[3 * i + k];
             }
         }
     }
@@ -677,6 +884,9 @@ __device__ __forceinline__ RveCtx<MT> rve_ctx(const RveParams& p, int rid, long 
     c.z = V + 15 * mn;
     c.Minv = V + 18 * mn;
     c.B = V + 24 * mn;
+    c.na = (c.n + kRveAgg - 1) / kRveAgg;
+    c.nc = 3 * c.na * c.na * c.na;
+    c.C = p.Cbuf + slot * (long long)(kRveMaxCoarse * kRveMaxCoarse);
     return c;
 }
 
@@ -684,6 +894,7 @@ template <int MODEL, typename MT, int G, bool NT>
 __global__ __launch_bounds__(kRveThreads) void k_rve_solve(RveParams p) {
     __shared__ double lds[kRveLds];
     __shared__ double red[kRveWaves * 45];
+    __shared__ double cv[2 * kRveMaxCoarse];  // coarse residual and correction (setup: a pivot row)
     __shared__ int snode[kRveElemPerRound][8];
     __shared__ int sq;
     __shared__ double sFa[9];
@@ -738,12 +949,13 @@ __global__ __launch_bounds__(kRveThreads) void k_rve_solve(RveParams p) {
                 }
                 if (it == p.newton_maxit) break;
                 rve_precond(c);
+                const bool coarse = p.coarse && rve_coarse_setup(c, lds, cv);
                 __syncthreads();
                 // inexact Newton: the linear solve only needs to beat the nonlinear residual
                 // reduction still to come (forcing 0.1 ||R|| / ||R_0||, within [pcg_rtol, 1e-2])
                 const double eta = p.inexact ? fmin(1e-2, fmax(p.pcg_rtol, 0.1 * nr / nr0)) : p.pcg_rtol;
                 const long long t1 = wall_clock64();
-                pcg_total += rve_pcg<MT, G, NT>(c, p, c.R, -1.0, eta, lds, red);
+                pcg_total += rve_pcg<MT, G, NT>(c, p, c.R, -1.0, eta, lds, red, cv, coarse);
                 if (p.prof && tid == 0) p.prof[2 * rid + 1] += wall_clock64() - t1;
                 for (int k = tid; k < 3 * c.nn; k += kRveThreads) c.wc[k] += c.x[k];
                 __syncthreads();
@@ -793,6 +1005,7 @@ template <int MODEL, typename MT, int G, bool NT>
 __global__ __launch_bounds__(kRveThreads) void k_rve_tangent(RveParams p) {
     __shared__ double lds[kRveLds];
     __shared__ double red[kRveWaves * 45];
+    __shared__ double cv[2 * kRveMaxCoarse];  // coarse residual and correction (setup: a pivot row)
     __shared__ int snode[kRveElemPerRound][8];
     __shared__ int sq;
     __shared__ double sFa[9];
@@ -827,13 +1040,15 @@ __global__ __launch_bounds__(kRveThreads) void k_rve_tangent(RveParams p) {
         }
         rve_affine_rhs(c);
         rve_precond(c);
+        const bool coarse = p.coarse && rve_coarse_setup(c, lds, cv);
         __syncthreads();
         const long long nn = c.nn;
         int pcg_total = 0;
         // condensed part b_k . v_m, v_m = K_red^-1 (-b_m)
         double* Aout = p.A + 81LL * rid;
         for (int m = 0; m < 9; ++m) {
-            pcg_total += rve_pcg<MT, G, NT>(c, p, c.B + (long long)m * 3 * nn, -1.0, p.tangent_rtol, lds, red);
+            pcg_total += rve_pcg<MT, G, NT>(c, p, c.B + (long long)m * 3 * nn, -1.0, p.tangent_rtol, lds, red, cv,
+                                           coarse);
             double dots[9];
 #pragma unroll
             for (int k = 0; k < 9; ++k) dots[k] = 0.0;
