@@ -624,7 +624,8 @@ class RveBatch:
         _check(load_library().ceos_rve_solve_device(self._h, dF, dP, dA, dinfo, stream), "ceos_rve_solve_device")
 
     def profile(self):
-        """(n_rve, 2) seconds in the Newton assemblies and in CG since the last reset."""
+        """(n_rve, 2) seconds in the Newton assemblies and in their linear solves (preconditioner
+        set-up included) since the last reset."""
         t = np.zeros((self.n, 2), dtype=np.int64)
         _check(load_library().ceos_rve_profile(self._h, t.ctypes.data), "ceos_rve_profile")
         return t / 1.0e8

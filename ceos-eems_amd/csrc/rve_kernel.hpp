@@ -77,7 +77,8 @@ struct RveParams {
     int inexact;                  // 1: CG tolerance follows the Newton residual (forcing term)
     int nt_tangent;               // 1: launch the kernels whose CG reads the reduced tangent with
                                   // non-temporal loads (NT = true)
-    long long* prof;              // [n_rve][2] clock ticks in assembly, in CG (or nullptr)
+    long long* prof;              // [n_rve][2] clock ticks in the Newton assemblies, in the Newton linear
+                                  // solves (preconditioner set-up included), or nullptr
     float* Cbuf;                  // [slots][kRveMaxCoarse^2] inverse of the coarse matrix
     int coarse;                   // 1: two-level preconditioner (block Jacobi + aggregate translations)
 };
@@ -415,22 +416,31 @@ __device__ bool rve_coarse_setup(const RveCtx<MT>& c, double* Ap, double* rk) {
 #pragma unroll
                 for (int k = 0; k < 9; ++k) acc[q][k] = 0.0;
             if (i != 0) {
+                // one z-plane of neighbours (9 offsets, 81 entries) requested at a time; a plane
+                // belongs to this pass if the step along z stays in (zp = 0) or leaves (zp = 1)
+                // the aggregate
 #pragma unroll 1
-                for (int o = 0; o < 27; ++o) {
-                    const int dx = o % 3 - 1, dy = (o / 3) % 3 - 1, dz = o / 9 - 1;
+                for (int dz = -1; dz <= 1; ++dz) {
                     if ((lz + dz < 0 || lz + dz >= wz ? 1 : 0) != zp) continue;
-                    const int xx = (x + dx + n) % n, yy = (y + dy + n) % n, zz = (z + dz + n) % n;
-                    if (xx + n * (yy + n * zz) == 0) continue;  // node 0's column
-                    const int cls = (lx + dx < 0 || lx + dx >= wx ? 1 : 0) + (ly + dy < 0 || ly + dy >= wy ? 2 : 0);
-                    double m[9];
+                    const int zz = (z + dz + n) % n;
+                    const MT* mp = c.M + (long long)(81 * (dz + 1)) * nn + i;
+                    double m[81];
 #pragma unroll
-                    for (int k = 0; k < 9; ++k) m[k] = (double)c.M[((long long)o * 9 + k) * nn + i];
+                    for (int q = 0; q < 81; ++q) m[q] = (double)mp[(long long)q * nn];
+                    __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-                    for (int q = 0; q < 4; ++q)
-                        if (q == cls) {
+                    for (int ob = 0; ob < 9; ++ob) {
+                        const int dx = ob % 3 - 1, dy = ob / 3 - 1;
+                        const int xx = (x + dx + n) % n, yy = (y + dy + n) % n;
+                        const bool col0 = xx + n * (yy + n * zz) == 0;  // node 0's column
+                        const int cls = (lx + dx < 0 || lx + dx >= wx ? 1 : 0) + (ly + dy < 0 || ly + dy >= wy ? 2 : 0);
 #pragma unroll
-                            for (int k = 0; k < 9; ++k) acc[q][k] += m[k];
-                        }
+                        for (int q = 0; q < 4; ++q)
+                            if (q == cls && !col0) {
+#pragma unroll
+                                for (int k = 0; k < 9; ++k) acc[q][k] += m[9 * ob + k];
+                            }
+                    }
                 }
             }
             // the aggregate each class lands on: the neighbour along every axis whose bit is set,
@@ -464,12 +474,9 @@ __device__ bool rve_coarse_setup(const RveCtx<MT>& c, double* Ap, double* rk) {
             }
         }
     }
-    // this thread's first packed entry e = tid as (row i, offset j in the row: column i + j)
-    int i0 = 0, j0 = tid;
-    while (i0 < nc && j0 >= nc - i0) {
-        j0 -= nc - i0;
-        ++i0;
-    }
+    // symmetric sweep, one thread per column j of the upper triangle: entries (i, j), i <= j, at
+    // packed offsets j, j + (nc - 1), j + (nc - 1) + (nc - 2), ...; at row i the lanes of a wave
+    // touch consecutive doubles
     for (int k = 0; k < nc; ++k) {
         __syncthreads();  // the previous pivot's updates (or the assembly) complete
         for (int t = tid; t < nc; t += kRveThreads) rk[t] = Ap[t <= k ? coarse_upk(t, k, nc) : coarse_upk(k, t, nc)];
@@ -477,15 +484,14 @@ __device__ bool rve_coarse_setup(const RveCtx<MT>& c, double* Ap, double* rk) {
         const double piv = rk[k];
         if (!(piv > 0.0)) return false;  // the same value for every thread
         const double pinv = 1.0 / piv;
-        int i = i0, j = j0;
-        for (int e = tid; e < np; e += kRveThreads) {
-            const int col = i + j;
-            const double v = Ap[e];
-            Ap[e] = i == k ? (col == k ? -pinv : v * pinv) : (col == k ? v * pinv : v - rk[i] * rk[col] * pinv);
-            j += kRveThreads;
-            while (i < nc && j >= nc - i) {
-                j -= nc - i;
-                ++i;
+        if (tid < nc) {
+            const int j = tid;
+            const double rj = rk[j] * pinv;
+            int e = j;
+            for (int i = 0; i <= j; ++i) {
+                const double v = Ap[e];
+                Ap[e] = i == k ? (j == k ? -pinv : v * pinv) : (j == k ? v * pinv : v - rk[i] * rj);
+                e += nc - 1 - i;
             }
         }
     }
@@ -501,7 +507,12 @@ __device__ bool rve_coarse_setup(const RveCtx<MT>& c, double* Ap, double* rk) {
 // Restriction rc = P^T r into cv[0, nc) and the coarse correction yc = A_c^-1 rc into
 // cv[nc, 2 nc); returns this thread's part of rc . yc (threads t < nc: unknown t = 3 a + comp).
 // r must be complete (a barrier before the call); cv[nc, 2 nc) may be read after the caller's
-// next barrier.  r is zero at node 0, so the restriction needs no mask.
+// next barrier.  r is zero at node 0, so the restriction needs no mask.  Both steps are chains of
+// dependent global loads inside one workgroup, so each requests a batch of operands before
+// summing any: the restriction one z-layer of the aggregate (16 nodes) at a time, the coarse
+// product kRveCoarseBatch entries at a time (larger batches cost the CG loop registers).
+constexpr int kRveCoarseBatch = 16;
+
 template <typename MT>
 __device__ double rve_coarse_apply(const RveCtx<MT>& c, double* cv) {
     const int tid = threadIdx.x, n = c.n, na = c.na, nc = c.nc;
@@ -510,15 +521,21 @@ __device__ double rve_coarse_apply(const RveCtx<MT>& c, double* cv) {
         const int x0 = kRveAgg * (a % na), y0 = kRveAgg * ((a / na) % na), z0 = kRveAgg * (a / (na * na));
         const int wx = min(kRveAgg, n - x0), wy = min(kRveAgg, n - y0), wz = min(kRveAgg, n - z0);
         double s = 0.0;
-        for (int lz = 0; lz < wz; ++lz)
-            for (int ly = 0; ly < wy; ++ly) {
-                const double* rp = c.r + 3LL * (x0 + n * (y0 + ly + n * (z0 + lz))) + comp;
-                double v[kRveAgg];
+#pragma unroll 1
+        for (int lz = 0; lz < wz; ++lz) {
+            const double* r0 = c.r + 3LL * (x0 + n * (y0 + n * (z0 + lz))) + comp;
+            // clamped addresses, masked values: the loads stay unconditional (one batch)
+            double v[kRveAgg][kRveAgg];
 #pragma unroll
-                for (int lx = 0; lx < kRveAgg; ++lx) v[lx] = lx < wx ? rp[3 * lx] : 0.0;
+            for (int ly = 0; ly < kRveAgg; ++ly)
 #pragma unroll
-                for (int lx = 0; lx < kRveAgg; ++lx) s += v[lx];
-            }
+                for (int lx = 0; lx < kRveAgg; ++lx) v[ly][lx] = r0[3 * (min(lx, wx - 1) + n * min(ly, wy - 1))];
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int ly = 0; ly < kRveAgg; ++ly)
+#pragma unroll
+                for (int lx = 0; lx < kRveAgg; ++lx) s += (lx < wx && ly < wy) ? v[ly][lx] : 0.0;
+        }
         cv[tid] = s;
     }
     __syncthreads();
@@ -526,8 +543,16 @@ __device__ double rve_coarse_apply(const RveCtx<MT>& c, double* cv) {
     if (tid < nc) {
         const float* Cp = c.C + tid;
         double y = 0.0;
-#pragma unroll 8
-        for (int k = 0; k < nc; ++k) y += (double)Cp[(long long)k * nc] * cv[k];
+#pragma unroll 1
+        for (int k0 = 0; k0 < nc; k0 += kRveCoarseBatch) {
+            float cb[kRveCoarseBatch];
+#pragma unroll
+            for (int k = 0; k < kRveCoarseBatch; ++k) cb[k] = Cp[(long long)min(k0 + k, nc - 1) * nc];
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int k = 0; k < kRveCoarseBatch; ++k)
+                if (k0 + k < nc) y += (double)cb[k] * cv[k0 + k];
+        }
         cv[nc + tid] = y;
         part = cv[tid] * y;
     }
@@ -926,6 +951,7 @@ __global__ __launch_bounds__(kRveThreads) void k_rve_solve(RveParams p) {
             __syncthreads();
             // Newton on the reduced system
             bool ok = false;
+            bool coarse = false;  // coarse inverse of this substep's first tangent, kept for the others
             double nr0 = 0.0;
             for (int it = 0; it <= p.newton_maxit; ++it) {
                 const long long t0 = wall_clock64();
@@ -948,13 +974,15 @@ __global__ __launch_bounds__(kRveThreads) void k_rve_solve(RveParams p) {
                     break;
                 }
                 if (it == p.newton_maxit) break;
+                const long long t1 = wall_clock64();
                 rve_precond(c);
-                const bool coarse = p.coarse && rve_coarse_setup(c, lds, cv);
+                // the first step's coarse inverse serves the substep's later steps too: the
+                // preconditioner stays SPD, and the tangent moves little between Newton steps
+                if (it == 0) coarse = p.coarse && rve_coarse_setup(c, lds, cv);
                 __syncthreads();
                 // inexact Newton: the linear solve only needs to beat the nonlinear residual
                 // reduction still to come (forcing 0.1 ||R|| / ||R_0||, within [pcg_rtol, 1e-2])
                 const double eta = p.inexact ? fmin(1e-2, fmax(p.pcg_rtol, 0.1 * nr / nr0)) : p.pcg_rtol;
-                const long long t1 = wall_clock64();
                 pcg_total += rve_pcg<MT, G, NT>(c, p, c.R, -1.0, eta, lds, red, cv, coarse);
                 if (p.prof && tid == 0) p.prof[2 * rid + 1] += wall_clock64() - t1;
                 for (int k = tid; k < 3 * c.nn; k += kRveThreads) c.wc[k] += c.x[k];

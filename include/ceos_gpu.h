@@ -474,7 +474,8 @@ int ceos_rve_solve_device(ceos_rve_batch *b, const double *dFbar, double *dP, do
                           void *stream);
 
 /* Instrumentation (no reference counterpart): per RVE, the device wall-clock ticks (100 MHz)
- * spent in the Newton assemblies and in the CG solves of the solves since the last reset. */
+ * spent in the Newton assemblies and in the Newton steps' linear solves (preconditioner set-up
+ * included) of the solves since the last reset. */
 int ceos_rve_profile(ceos_rve_batch *b, long long *ticks);
 
 /* Converged fluctuation of RVE `rve` (3 side^3 doubles, torus node x + s (y + s z)). */
