@@ -245,14 +245,19 @@ def newton_correction(X):
     return 0.002 * np.stack([np.sin(2 * np.pi * x) * y, np.cos(np.pi * y) * z, x * y], axis=1).reshape(-1)
 
 
-RVE_CG_BYTES_PER_NODE = 243 * 4 + 240 + 24  # FP32-stored tangent row, FP64 CG vectors, r again (restriction)
+# Bytes per node of one CG iteration (one right-hand side): the condensation's three solves share
+# every sweep of the FP32-stored tangent row (243 x 4 B), so an iteration of theirs reads a third
+# of it; plus the FP64 vectors (x, r, q, z, the block-Jacobi inverse: ~240 B) and r again for the
+# coarse restriction.  The Newton steps' iterations read the whole row, so counting every
+# iteration at a third of a row gives a lower bound.
+RVE_CG_BYTES_PER_NODE = 243 * 4 / 3 + 240 + 24
 
 
 def rve_coarse_bytes(side):
     """Bytes per CG iteration of the coarse correction: the FP32 inverse of the coarse matrix
-    (3 translations per aggregate of 4^3 nodes)."""
+    (3 translations per aggregate of 4^3 nodes), read once per sweep of three right-hand sides."""
     na = -(-np.asarray(side, dtype=np.int64) // 4)
-    return 4.0 * (3 * na ** 3) ** 2
+    return 4.0 * (3 * na ** 3) ** 2 / 3
 
 
 def _fe2_cpu_job(side, rve_id, F):
@@ -370,8 +375,10 @@ def fe2_bench(no_cpu):
         "rve_cg_hbm": {"achieved": float(np.sum(info[:, 2] * (nodes * RVE_CG_BYTES_PER_NODE + rve_coarse_bytes(m.sides)))
                                          / p2["seconds"] / 1e9),
                        "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                       "note": "algorithmic CG bytes (FP32 tangent rows + FP64 vectors + the coarse inverse) over "
-                               "the whole plain evaluation's wall time, a lower bound for the CG kernels alone"},
+                       "note": "algorithmic CG bytes (FP32 tangent rows, FP64 vectors, the coarse inverse; every "
+                               "iteration charged a third of a tangent sweep, as the condensation's three "
+                               "right-hand sides share one) over the whole plain evaluation's wall time: a lower "
+                               "bound for the CG kernels alone"},
         "rve_cg_iterations": {"preconditioner": "block Jacobi + translations of 4^3-node aggregates (two-level)",
                               "per_rve_mean": float(info[:, 2].mean()),
                               "per_side_mean": {str(n): float(info[m.sides == n, 2].mean())

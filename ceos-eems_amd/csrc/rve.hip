@@ -141,9 +141,10 @@ int reset(ceos_rve_batch* b) {
 constexpr int kRveNtQueue = 128;  // queue length from which the CG reads tangents non-temporally
 
 // one persistent launch over the queue `order` (n_queue RVE ids, largest first); solve = false
-// runs the condensation alone at the RVEs' current states
+// runs the condensation alone at the RVEs' current states; probe = true runs it as the tile probe
+// (one single-right-hand-side solve per RVE; states, A and info unchanged)
 int run(ceos_rve_batch* b, bool want_A, hipStream_t s, const int* order, int n_queue, bool solve = true,
-        int cg_group = 0) {
+        int cg_group = 0, bool probe = false) {
     RveParams p;
     p.n_rve = n_queue;
     p.order = order;
@@ -170,6 +171,7 @@ int run(ceos_rve_batch* b, bool want_A, hipStream_t s, const int* order, int n_q
     p.pcg_maxit = b->pcg_maxit;
     p.max_cutbacks = b->max_cutbacks;
     p.inexact = b->inexact;
+    p.probe = probe ? 1 : 0;
     // A full batch streams far more reduced tangents than the caches hold (256 slots x up to 4 MB
     // in FP32 at 16^3): its CG reads them non-temporally (the plain FE^2 evaluation: 17.5 -> 15.6 s).
     // A short queue (judges, fallbacks: tens of RVEs) keeps ordinary loads, whose tangents the
@@ -399,29 +401,21 @@ int ceos_rve_autotune(ceos_rve_batch* b, int max_rves, int* chosen, double* seco
     const int n = std::min(max_rves, b->n_rve);
     hipStream_t s = b->stream;
     RVE_HIP_OK(hipMemcpyAsync(b->dsel_q, q.data(), sizeof(int) * n, hipMemcpyHostToDevice, s));
-    // the condensation needs converged states: mark the sample converged at its current state
-    // (the other RVEs keep their info)
-    std::vector<int> inf(4 * (size_t)b->n_rve);
-    RVE_HIP_OK(hipMemcpyAsync(inf.data(), b->dinfo, sizeof(int) * inf.size(), hipMemcpyDeviceToHost, s));
-    RVE_HIP_OK(hipStreamSynchronize(s));
-    for (int k = 0; k < n; ++k) inf[4 * (size_t)q[k]] = 0;
-    RVE_HIP_OK(hipMemcpyAsync(b->dinfo, inf.data(), sizeof(int) * inf.size(), hipMemcpyHostToDevice, s));
-    RVE_HIP_OK(hipStreamSynchronize(s));
     hipEvent_t e0, e1;
     RVE_HIP_OK(hipEventCreate(&e0));
     RVE_HIP_OK(hipEventCreate(&e1));
-    // the probe only needs the time per CG iteration: a loose condensation tolerance keeps it short
-    // (iteration counts do not depend on the tile)
+    // the probe only needs the time per CG iteration: a loose tolerance keeps it short (iteration
+    // counts do not depend on the tile)
     const double keep_rtol = b->tangent_rtol;
     b->tangent_rtol = 1e-4;
     int best = kCgGroupDefault;
     double tbest = 1e300;
-    int rc = run(b, true, s, b->dsel_q, n, false, kCgGroupDefault);  // warm-up
+    int rc = run(b, true, s, b->dsel_q, n, false, kCgGroupDefault, true);  // warm-up
     for (int k = 0; k < 3 && rc == CEOS_OK; ++k) {
         const int g = kCgGroups[k];
         float ms = 0.0f;
         if (hipEventRecord(e0, s) != hipSuccess) rc = CEOS_ERR_HIP;
-        if (rc == CEOS_OK) rc = run(b, true, s, b->dsel_q, n, false, g);
+        if (rc == CEOS_OK) rc = run(b, true, s, b->dsel_q, n, false, g, true);
         if (rc == CEOS_OK && (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
                               hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
             rc = CEOS_ERR_HIP;

@@ -49,7 +49,9 @@ constexpr int kRveMax3(int a, int b, int c) { return a > b ? (a > c ? a : c) : (
 // FP64) and, in the CG, p of the largest RVE share the LDS: 148 KB
 constexpr int kRveLds = kRveMax3(kRveElemPerRound * kLdsPerElem, 3 * kRveMaxSide * kRveMaxSide * kRveMaxSide,
                                  kRveCoarsePacked);
-constexpr int kRveVecPerNode = 51;  // w 3, R 3, x 3, r 3, q 3, z 3, Minv 6, B 27
+// w 3, R 3, x 3, r 3, q 3, z 3, Minv 6, B 27, then x, r, q, z of the condensation's three right-hand
+// sides (9 each, interleaved per node: [9 i + 3 s + k]; rve_vec3, right after the RVE's B)
+constexpr int kRveVecPerNode = 87;
 static_assert(kRveLds >= 3 * kRveMaxSide * kRveMaxSide * kRveMaxSide, "p must fit in LDS");
 static_assert(kRveMaxCoarse <= kRveThreads, "one thread per coarse unknown");
 
@@ -74,6 +76,8 @@ struct RveParams {
     double newton_rtol, newton_atol, pcg_rtol;
     double tangent_rtol;          // CG tolerance of the condensation solves
     int newton_maxit, pcg_maxit, max_cutbacks;
+    int probe;                    // 1: k_rve_tangent only runs one condensation solve per RVE with the
+                                  // Newton steps' CG (the tile probe); A and info are left as they are
     int inexact;                  // 1: CG tolerance follows the Newton residual (forcing term)
     int nt_tangent;               // 1: launch the kernels whose CG reads the reduced tangent with
                                   // non-temporal loads (NT = true)
@@ -725,6 +729,300 @@ __device__ int rve_pcg(const RveCtx<MT>& c, const RveParams& p, const double* rh
     return it;
 }
 
+// x, r, q, z (k = 0..3) of the condensation's three right-hand sides: 9 nn doubles each after the
+// RVE's 27 nn values of B (within the slot: 24 + 27 + 36 <= kRveVecPerNode doubles per node);
+// derived here rather than kept in RveCtx, whose every pointer costs both kernels registers
+template <typename MT>
+__device__ __forceinline__ double* rve_vec3(const RveCtx<MT>& c, int k) {
+    return c.B + (27LL + 9LL * k) * c.nn;
+}
+
+// y_s = M p_s for node i and the three right-hand sides s of the condensation (p in LDS, FP32,
+// [9 j + 3 s + k]): the tangent rows are read once for the three products
+template <typename MT, bool NT>
+__device__ __forceinline__ void rve_row3(const RveCtx<MT>& c, int i, const float* v3, double y[9]) {
+    const int n = c.n, nn = c.nn;
+    const int ix = i % n, iy = (i / n) % n, iz = i / (n * n);
+    const int xs[3] = {ix == 0 ? n - 1 : ix - 1, ix, ix == n - 1 ? 0 : ix + 1};
+    const int ys[3] = {iy == 0 ? n - 1 : iy - 1, iy, iy == n - 1 ? 0 : iy + 1};
+#pragma unroll
+    for (int t = 0; t < 9; ++t) y[t] = 0.0;
+#pragma unroll 1
+    for (int dz = -1; dz <= 1; ++dz) {
+        const int zz = iz + dz < 0 ? n - 1 : (iz + dz == n ? 0 : iz + dz);
+        const MT* mp = c.M + (long long)(81 * (dz + 1)) * nn + i;
+        double m[81];
+#pragma unroll
+        for (int q = 0; q < 81; ++q) {
+            if constexpr (NT) m[q] = (double)__builtin_nontemporal_load(mp + (long long)q * nn);
+            else m[q] = (double)mp[(long long)q * nn];
+        }
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int by = 0; by < 3; ++by) {
+#pragma unroll
+            for (int bx = 0; bx < 3; ++bx) {
+                const int j = xs[bx] + n * (ys[by] + n * zz);
+                const int b = bx + 3 * by;
+                const float* vj = v3 + 9 * j;
+                double v[9];
+#pragma unroll
+                for (int t = 0; t < 9; ++t) v[t] = (double)vj[t];
+#pragma unroll
+                for (int q = 0; q < 3; ++q)
+#pragma unroll
+                    for (int a = 0; a < 3; ++a)
+                        y[3 * q + a] += m[9 * b + 3 * a] * v[3 * q] + m[9 * b + 3 * a + 1] * v[3 * q + 1] +
+                                        m[9 * b + 3 * a + 2] * v[3 * q + 2];
+            }
+        }
+    }
+}
+
+// rve_coarse_apply for the three right-hand sides: rc_s = P^T r_s into cv[s nc + u] and
+// yc_s = A_c^-1 rc_s into cv[3 nc + s nc + u]; part[s] = this thread's part of rc_s . yc_s.  The
+// coarse inverse is read once for the three.  Restriction: two rows of an aggregate layer (8
+// nodes, 24 values) per round trip, clamped addresses and masked values.
+template <typename MT>
+__device__ void rve_coarse_apply3(const RveCtx<MT>& c, double* cv, double part[3]) {
+    const int tid = threadIdx.x, n = c.n, na = c.na, nc = c.nc;
+    if (tid < nc) {
+        const int a = tid / 3, comp = tid - 3 * a;
+        const int x0 = kRveAgg * (a % na), y0 = kRveAgg * ((a / na) % na), z0 = kRveAgg * (a / (na * na));
+        const int wx = min(kRveAgg, n - x0), wy = min(kRveAgg, n - y0), wz = min(kRveAgg, n - z0);
+        double s3[3] = {0.0, 0.0, 0.0};
+#pragma unroll 1
+        for (int lz = 0; lz < wz; ++lz) {
+            const double* r0 = rve_vec3(c, 1) + 9LL * (x0 + n * (y0 + n * (z0 + lz))) + comp;
+#pragma unroll 1
+            for (int ly0 = 0; ly0 < kRveAgg; ly0 += 2) {
+                double v[2][kRveAgg][3];
+#pragma unroll
+                for (int dy = 0; dy < 2; ++dy)
+#pragma unroll
+                    for (int lx = 0; lx < kRveAgg; ++lx)
+#pragma unroll
+                        for (int q = 0; q < 3; ++q)
+                            v[dy][lx][q] = r0[9 * (min(lx, wx - 1) + n * min(ly0 + dy, wy - 1)) + 3 * q];
+                __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                for (int dy = 0; dy < 2; ++dy)
+#pragma unroll
+                    for (int lx = 0; lx < kRveAgg; ++lx)
+#pragma unroll
+                        for (int q = 0; q < 3; ++q) s3[q] += (lx < wx && ly0 + dy < wy) ? v[dy][lx][q] : 0.0;
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < 3; ++q) cv[q * nc + tid] = s3[q];
+    }
+    __syncthreads();
+    part[0] = part[1] = part[2] = 0.0;
+    if (tid < nc) {
+        const float* Cp = c.C + tid;
+        double y3[3] = {0.0, 0.0, 0.0};
+#pragma unroll 1
+        for (int k0 = 0; k0 < nc; k0 += kRveCoarseBatch) {
+            float cb[kRveCoarseBatch];
+#pragma unroll
+            for (int k = 0; k < kRveCoarseBatch; ++k) cb[k] = Cp[(long long)min(k0 + k, nc - 1) * nc];
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int k = 0; k < kRveCoarseBatch; ++k)
+                if (k0 + k < nc) {
+                    const double cf = (double)cb[k];
+#pragma unroll
+                    for (int q = 0; q < 3; ++q) y3[q] += cf * cv[q * nc + k0 + k];
+                }
+        }
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            cv[3 * nc + q * nc + tid] = y3[q];
+            part[q] = cv[q * nc + tid] * y3[q];
+        }
+    }
+}
+
+// Three condensation solves at once: K_red v_s = -b_(m0+s), s = 0, 1, 2, each a PCG (block Jacobi
+// plus the coarse level) to ||r_s|| <= rtol ||b_s||.  Every iteration streams the tangent rows once
+// for the three solves, where three single solves stream them three times: the tangent is the
+// dominant traffic of a CG iteration (972 of about 1,250 bytes per node in FP32 storage).  The
+// three directions p_s live in LDS in FP32 (they fit where one FP64 p did); x_s and r_s are updated
+// with the same rounded p_s that q_s = K p_s is formed from, so r_s stays the residual of x_s and
+// the solves reach the FP64 tolerances (the rounded directions cost about 1 % more iterations).
+// A converged right-hand side is frozen (alpha_s = 0) while the others go on.  x_s ends in
+// rve_vec3(c, 0);
+// its[s] = iterations of solve s.
+template <typename MT, bool NT>
+__device__ void rve_pcg3(const RveCtx<MT>& c, const RveParams& p, int m0, double rtol, float* pl, double* red,
+                         double* cv, bool coarse, int its[3]) {
+    const int tid = threadIdx.x;
+    const int nn = c.nn, nc = c.nc;
+    const long long nn3 = 3LL * nn;
+    double* x3 = rve_vec3(c, 0);
+    double* r3 = rve_vec3(c, 1);
+    double* q3 = rve_vec3(c, 2);
+    double* z3 = rve_vec3(c, 3);
+    double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};  // r_s . z_s, r_s . r_s
+    for (int i = tid; i < nn; i += kRveThreads) {
+        double b[9], mi[6];
+#pragma unroll
+        for (int q = 0; q < 3; ++q)
+#pragma unroll
+            for (int k = 0; k < 3; ++k) b[3 * q + k] = c.B[(m0 + q) * nn3 + 3 * i + k];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) mi[k] = c.Minv[6 * i + k];
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            double rr[3], z[3];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) rr[k] = i == 0 ? 0.0 : -b[3 * q + k];
+            apply_minv(mi, rr, z);
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                x3[9 * i + 3 * q + k] = 0.0;
+                r3[9 * i + 3 * q + k] = rr[k];
+                pl[9 * i + 3 * q + k] = (float)z[k];
+                acc[q] += rr[k] * z[k];
+                acc[3 + q] += rr[k] * rr[k];
+            }
+        }
+    }
+    double sums[9] = {acc[0], acc[1], acc[2], acc[3], acc[4], acc[5], 0.0, 0.0, 0.0};
+    if (coarse) {
+        __syncthreads();  // r complete
+        double pc[3];
+        rve_coarse_apply3(c, cv, pc);
+        sums[6] = pc[0];
+        sums[7] = pc[1];
+        sums[8] = pc[2];
+    }
+    wg_sum<9>(sums, red);
+    double rz[3], tol2[3];
+    bool act[3];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        rz[q] = sums[q] + sums[6 + q];
+        tol2[q] = rtol * rtol * sums[3 + q];
+        act[q] = sums[3 + q] > 0.0;
+        its[q] = 0;
+    }
+    if (coarse) {
+        for (int i = tid; i < nn; i += kRveThreads) {
+            if (i == 0) continue;
+            const int a3 = rve_agg3(i, c.n, c.na);
+#pragma unroll
+            for (int q = 0; q < 3; ++q)
+#pragma unroll
+                for (int k = 0; k < 3; ++k)
+                    pl[9 * i + 3 * q + k] = (float)((double)pl[9 * i + 3 * q + k] + cv[3 * nc + q * nc + a3 + k]);
+        }
+    }
+    int it = 0;
+    while ((act[0] || act[1] || act[2]) && it < p.pcg_maxit) {
+        ++it;
+        __syncthreads();  // p complete
+        double pq[3] = {0.0, 0.0, 0.0};
+        for (int i = tid; i < nn; i += kRveThreads) {
+            double y[9];
+            if (i == 0) {
+#pragma unroll
+                for (int t = 0; t < 9; ++t) y[t] = 0.0;
+            } else {
+                rve_row3<MT, NT>(c, i, pl, y);
+            }
+#pragma unroll
+            for (int q = 0; q < 3; ++q)
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    q3[9 * i + 3 * q + k] = y[3 * q + k];
+                    pq[q] += (double)pl[9 * i + 3 * q + k] * y[3 * q + k];
+                }
+        }
+        wg_sum<3>(pq, red);
+        double alpha[3];
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            if (act[q] && !(pq[q] > 0.0)) {  // not positive definite along p_s: this solve stops
+                act[q] = false;
+                its[q] = it;
+            }
+            alpha[q] = act[q] ? rz[q] / pq[q] : 0.0;
+        }
+        double s2[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};  // r_s . r_s, r_s . z_s
+        for (int i = tid; i < nn; i += kRveThreads) {
+            double xv[9], rv[9], qv[9], mi[6];
+#pragma unroll
+            for (int t = 0; t < 9; ++t) {
+                xv[t] = x3[9 * i + t];
+                rv[t] = r3[9 * i + t];
+                qv[t] = q3[9 * i + t];
+            }
+#pragma unroll
+            for (int k = 0; k < 6; ++k) mi[k] = c.Minv[6 * i + k];
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+                double r[3], z[3];
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    x3[9 * i + 3 * q + k] = xv[3 * q + k] + alpha[q] * (double)pl[9 * i + 3 * q + k];
+                    r[k] = rv[3 * q + k] - alpha[q] * qv[3 * q + k];
+                    r3[9 * i + 3 * q + k] = r[k];
+                }
+                apply_minv(mi, r, z);
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    z3[9 * i + 3 * q + k] = z[k];
+                    s2[q] += r[k] * r[k];
+                    s2[3 + q] += r[k] * z[k];
+                }
+            }
+        }
+        wg_sum<6>(s2, red);
+#pragma unroll
+        for (int q = 0; q < 3; ++q)
+            if (act[q] && s2[q] <= tol2[q]) {
+                act[q] = false;
+                its[q] = it;
+            }
+        if (!(act[0] || act[1] || act[2])) break;
+        double cz[3] = {0.0, 0.0, 0.0};
+        if (coarse) {
+            rve_coarse_apply3(c, cv, cz);  // r complete: wg_sum ended on a barrier
+            wg_sum<3>(cz, red);
+        }
+        double beta[3];
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            const double rzn = s2[3 + q] + cz[q];
+            beta[q] = act[q] ? rzn / rz[q] : 0.0;
+            if (act[q]) rz[q] = rzn;
+        }
+        for (int i = tid; i < nn; i += kRveThreads) {
+            double zv[9];
+#pragma unroll
+            for (int t = 0; t < 9; ++t) zv[t] = z3[9 * i + t];
+            __builtin_amdgcn_sched_barrier(0);
+            const int a3 = (coarse && i != 0) ? rve_agg3(i, c.n, c.na) : 0;
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+                if (!act[q]) continue;  // frozen: its direction no longer moves x_s
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    const double yc = (coarse && i != 0) ? cv[3 * nc + q * nc + a3 + k] : 0.0;
+                    pl[9 * i + 3 * q + k] = (float)(zv[3 * q + k] + yc + beta[q] * (double)pl[9 * i + 3 * q + k]);
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < 3; ++q)
+        if (act[q]) its[q] = it;  // the iteration limit
+    __syncthreads();
+}
+
 // engineering strain (11,22,33,12,23,13) of grad_x Phi_I = e_i (x) g_J, I = 3J + i
 __device__ __forceinline__ void mode_strain(const double Fi[3][3], int I, double e[6]) {
     const int J = I / 3, i = I % 3;
@@ -1033,7 +1331,7 @@ template <int MODEL, typename MT, int G, bool NT>
 __global__ __launch_bounds__(kRveThreads) void k_rve_tangent(RveParams p) {
     __shared__ double lds[kRveLds];
     __shared__ double red[kRveWaves * 45];
-    __shared__ double cv[2 * kRveMaxCoarse];  // coarse residual and correction (setup: a pivot row)
+    __shared__ double cv[6 * kRveMaxCoarse];  // coarse residuals and corrections of three solves
     __shared__ int snode[kRveElemPerRound][8];
     __shared__ int sq;
     __shared__ double sFa[9];
@@ -1049,7 +1347,7 @@ __global__ __launch_bounds__(kRveThreads) void k_rve_tangent(RveParams p) {
         __syncthreads();
         if (qi >= p.n_rve) break;
         const int rid = p.order[qi];
-        if (p.info[4 * rid] != 0) continue;
+        if (!p.probe && p.info[4 * rid] != 0) continue;
         RveCtx<MT> c = rve_ctx<MT>(p, rid, slot);
         for (int k = tid; k < 3 * c.nn; k += kRveThreads) c.wc[k] = c.wconv[k];
         if (tid < 9) sFa[tid] = p.Fconv[9 * rid + 3 * (tid % 3) + tid / 3];
@@ -1072,26 +1370,42 @@ __global__ __launch_bounds__(kRveThreads) void k_rve_tangent(RveParams p) {
         __syncthreads();
         const long long nn = c.nn;
         int pcg_total = 0;
-        // condensed part b_k . v_m, v_m = K_red^-1 (-b_m)
+        // condensed part b_k . v_m, v_m = K_red^-1 (-b_m): the nine solves three at a time
         double* Aout = p.A + 81LL * rid;
-        for (int m = 0; m < 9; ++m) {
-            pcg_total += rve_pcg<MT, G, NT>(c, p, c.B + (long long)m * 3 * nn, -1.0, p.tangent_rtol, lds, red, cv,
-                                           coarse);
-            double dots[9];
+        if (p.probe) {
+            // tile probe: the first affine mode's solve with the single-right-hand-side CG of the
+            // Newton steps, whose vector sweeps the tile G shapes
+            rve_pcg<MT, G, NT>(c, p, c.B, -1.0, p.tangent_rtol, lds, red, cv, coarse);
+            continue;
+        }
+        for (int m0 = 0; m0 < 9; m0 += 3) {
+            int its[3];
+            rve_pcg3<MT, NT>(c, p, m0, p.tangent_rtol, reinterpret_cast<float*>(lds), red, cv, coarse, its);
+            pcg_total += its[0] + its[1] + its[2];
+            const double* x3 = rve_vec3(c, 0);
+            double dots[27];  // [9 s + k] = b_k . v_(m0+s)
 #pragma unroll
-            for (int k = 0; k < 9; ++k) dots[k] = 0.0;
+            for (int k = 0; k < 27; ++k) dots[k] = 0.0;
             for (int i = tid; i < c.nn; i += kRveThreads) {
 #pragma unroll
                 for (int a = 0; a < 3; ++a) {
-                    const double v = c.x[3 * i + a];
+                    double bk[9];
 #pragma unroll
-                    for (int k = 0; k < 9; ++k) dots[k] += c.B[(long long)k * 3 * nn + 3 * i + a] * v;
+                    for (int k = 0; k < 9; ++k) bk[k] = c.B[(long long)k * 3 * nn + 3 * i + a];
+#pragma unroll
+                    for (int q = 0; q < 3; ++q) {
+                        const double v = x3[9 * i + 3 * q + a];
+#pragma unroll
+                        for (int k = 0; k < 9; ++k) dots[9 * q + k] += bk[k] * v;
+                    }
                 }
             }
-            wg_sum<9>(dots, red);
+            wg_sum<27>(dots, red);
             if (tid == 0) {
 #pragma unroll
-                for (int k = 0; k < 9; ++k) sT[46 + 9 * k + m] = dots[k];
+                for (int q = 0; q < 3; ++q)
+#pragma unroll
+                    for (int k = 0; k < 9; ++k) sT[46 + 9 * k + m0 + q] = dots[9 * q + k];
             }
         }
         __syncthreads();

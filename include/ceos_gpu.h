@@ -435,25 +435,27 @@ int ceos_rve_set_inexact(ceos_rve_batch *b, int inexact);
  * The condensed tangent carries the storage rounding (about 1e-6 relative). */
 int ceos_rve_set_precision(ceos_rve_batch *b, int mode);
 
-/* CG tolerance of the nine condensation solves of the homogenized tangent (default: pcg_rtol). */
+/* CG tolerance of the nine condensation solves of the homogenized tangent (default: pcg_rtol).
+ * The nine solves run three at a time, each reading the reduced tangent once for its three
+ * right-hand sides (search directions stored FP32; residuals and tolerances FP64). */
 int ceos_rve_set_tangent_rtol(ceos_rve_batch *b, double rtol);
 
 /* Preconditioner of the RVE CG solves (Newton steps and condensation).  1 (default): two-level,
  * block Jacobi plus a coarse correction on the three translations of every aggregate of 4 x 4 x 4
  * torus nodes (node 0 left out; 192 coarse unknowns at side 16), z = D^-1 r + P A_c^-1 P^T r,
- * A_c = P^T K P assembled from the reduced tangent and inverted once per assembled tangent
- * (FP64, stored FP32).  0: block Jacobi alone.  The solves meet the same tolerances either way;
+ * A_c = P^T K P assembled from the reduced tangent and inverted at the first Newton step of
+ * every substep and once per condensation (FP64, stored FP32).  0: block Jacobi alone.  The solves meet the same tolerances either way;
  * only the iteration counts (info[2]) differ.  Replaces the PARDISO solve of the periodic system
  * (FEM/Modules/ModMultiscalePeriodicFEMSoE.f90:77-175) like the CG it preconditions; no
  * counterpart of its own in the reference. */
 int ceos_rve_set_coarse(ceos_rve_batch *b, int on);
 
-/* Tile of the CG vector sweeps: every thread loads the operands of g nodes (2, 4 or 8) before
- * computing them.  ceos_rve_autotune times the condensation of a sample of the batch's RVEs (up to
- * max_rves, largest first, at their current states, CG to 1e-4: the time per iteration is what is
- * compared; pure linear solves, the states are not changed, the sample's A and info entries are
- * overwritten) for each g, keeps the fastest and returns it (chosen) with the three times in
- * seconds (may be NULL).  Results agree to the solver tolerances across g. */
+/* Tile of the Newton steps' CG vector sweeps: every thread loads the operands of g nodes (2, 4 or
+ * 8) before computing them.  ceos_rve_autotune times, for each g, one linear solve of that CG per
+ * RVE of a sample of the batch (up to max_rves, largest first, at their converged states, to
+ * 1e-4: the time per iteration is what is compared; the states, P, A and info are not changed),
+ * keeps the fastest and returns it (chosen) with the three times in seconds (may be NULL).
+ * Results agree to the solver tolerances across g. */
 int ceos_rve_autotune(ceos_rve_batch *b, int max_rves, int *chosen, double *seconds);
 int ceos_rve_set_cg_group(ceos_rve_batch *b, int g);
 
