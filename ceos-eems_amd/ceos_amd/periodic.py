@@ -119,10 +119,20 @@ class PeriodicRve:
     """One periodic RVE on a general hex8 mesh: X (nn, 3), conn (ne, 8) 1-based, mat_id (ne,),
     materials {material id: (model, props)}.  solve(F) brings it to equilibrium under the macro
     deformation gradient F from its last committed state; commit() makes that state the
-    converged one (SwitchConvergedState)."""
+    converged one (SwitchConvergedState).
+
+    The reduced system is solved by PCG (in place of the reference's PARDISO solve of
+    K_red, ModMultiscalePeriodicFEMSoE.f90:77-175) with the library's preconditioner
+    `preconditioner`: by default the two-level one (block Jacobi plus the rigid-body modes of node
+    aggregates of the reduced mesh), whose coarse inverse serves `coarse_reuse` further solves, so
+    that the Newton steps of a solve and the nine condensation solves share one or two (a
+    preconditioner from a nearby tangent is still symmetric positive definite).  Every solve()
+    starts with a fresh coarse inverse, so a solve depends only on the state it starts from (a
+    restarted analysis repeats the uninterrupted one bitwise)."""
 
     def __init__(self, X, conn, mat_id=None, materials=None, nl=True, tol=1e-6, boundary=None, newton_rtol=1e-10,
-                 newton_atol=1e-13, newton_maxit=12, pcg_rtol=1e-10, pcg_maxit=20000):
+                 newton_atol=1e-13, newton_maxit=12, pcg_rtol=1e-10, pcg_maxit=20000,
+                 preconditioner=gpu.PREC_TWO_LEVEL, coarse_reuse=8):
         self.X = np.ascontiguousarray(X, dtype=np.float64).reshape(-1, 3)
         conn = np.ascontiguousarray(conn, dtype=np.int32).reshape(-1, 8)
         self.nn, self.ne = len(self.X), len(conn)
@@ -143,6 +153,8 @@ class PeriodicRve:
         c = int(nm[self.map.corners[0]])
         self.fixed = np.array([3 * c, 3 * c + 1, 3 * c + 2])
         self.red.set_bc([], self.fixed)
+        self.prec = (preconditioner, 0.0, coarse_reuse if preconditioner == gpu.PREC_TWO_LEVEL else 0)
+        self.red.set_preconditioner(*self.prec)
         self.per = gpu.Periodic(self.full, self.red, nm)
         self.nn_red, _, self.volume = self.per.info()
         self.newton = (newton_rtol, newton_atol, newton_maxit)
@@ -181,6 +193,7 @@ class PeriodicRve:
         s = self.stream.ptr
         rtol, atol, maxit = self.newton
         prt, pmax = self.pcg
+        self.red.set_preconditioner(*self.prec)  # a fresh coarse inverse at the first solve
         self.stream.synchronize()
         # the step starts from the committed U plus the affine increment
         self.dtmp.upload(self._affine(F - self.Fconv))
