@@ -1279,8 +1279,12 @@ __global__ __launch_bounds__(kRveThreads) void k_rve_solve(RveParams p) {
                 if (it == 0) coarse = p.coarse && rve_coarse_setup(c, lds, cv);
                 __syncthreads();
                 // inexact Newton: the linear solve only needs to beat the nonlinear residual
-                // reduction still to come (forcing 0.1 ||R|| / ||R_0||, within [pcg_rtol, 1e-2])
-                const double eta = p.inexact ? fmin(1e-2, fmax(p.pcg_rtol, 0.1 * nr / nr0)) : p.pcg_rtol;
+                // reduction still to come (forcing 0.1 ||R|| / ||R_0||), and never more than half the
+                // way to the Newton test itself (0.5 target / ||R||: the last step is not oversolved);
+                // within [pcg_rtol, 1e-2]
+                const double target = fmax(p.newton_rtol * nr0, p.newton_atol);
+                const double eta =
+                    p.inexact ? fmin(1e-2, fmax(p.pcg_rtol, fmax(0.1 * nr / nr0, 0.5 * target / nr))) : p.pcg_rtol;
                 pcg_total += rve_pcg<MT, G, NT>(c, p, c.R, -1.0, eta, lds, red, cv, coarse);
                 if (p.prof && tid == 0) p.prof[2 * rid + 1] += wall_clock64() - t1;
                 for (int k = tid; k < 3 * c.nn; k += kRveThreads) c.wc[k] += c.x[k];

@@ -424,8 +424,10 @@ int ceos_rve_set_solver(ceos_rve_batch *b, double newton_rtol, double newton_ato
                         double pcg_rtol, int pcg_maxit, int max_cutbacks);
 
 /* 1 (default): inexact Newton, the CG tolerance of a Newton step is
- * min(1e-2, max(pcg_rtol, 0.1 max|R| / max|R_0|)); 0: every CG solve to pcg_rtol.  The Newton
- * convergence test is the same either way; the tangent solves always use pcg_rtol. */
+ * min(1e-2, max(pcg_rtol, 0.1 max|R| / max|R_0|, 0.5 target / max|R|)), target =
+ * max(newton_rtol max|R_0|, newton_atol) (the last step is not solved past the Newton test);
+ * 0: every CG solve to pcg_rtol.  The Newton convergence test is the same either way; the
+ * tangent solves always use their own tolerance. */
 int ceos_rve_set_inexact(ceos_rve_batch *b, int inexact);
 
 /* Storage of the reduced tangent in the CG solves (Newton steps and condensation):
