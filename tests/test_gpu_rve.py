@@ -82,6 +82,21 @@ def test_condensed_tangent_matches_oracle():
         assert np.allclose(A[k], A[k].T, atol=1e-8 * np.abs(A[k]).max())  # hyperelastic: major symmetry
 
 
+def test_condensed_tangent_matches_oracle_at_side_12():
+    """The condensation at a workload side: three aggregates per axis in the coarse level, nine
+    solves in three groups of three with FP32 search directions; FP64 tangent storage."""
+    b, ph = _batch([12], [13])
+    b.set_solver(newton_rtol=1e-12, pcg_rtol=1e-12)
+    F = _F(22, 0.04)
+    P, A, info = b.solve(F[None], tangent=True)
+    print("info", info.tolist(), flush=True)
+    o = ro.OracleRVE(12, ph)
+    o.solve(F, rtol=1e-12)
+    Ao = o.tangent()
+    print("tangent rel", rel(A[0], Ao), flush=True)
+    assert rel(A[0], Ao) < 1e-7
+
+
 def test_incremental_solves_and_cut_back():
     """Two macro increments from the converged state, then a large stretch reached by the
     golden-ratio cut-back; the states match the oracle driven through the same increments."""
