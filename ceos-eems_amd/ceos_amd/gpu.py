@@ -66,7 +66,7 @@ EXPORTS = [
     "ceos_set_external_gp", "ceos_gp_deformation_gradient", "ceos_gp_deformation_gradient_device",
     "ceos_rve_solve_subset", "ceos_rve_to_external_device", "ceos_mlp_shape",
     "ceos_surrogate_create", "ceos_surrogate_destroy", "ceos_surrogate_features_device",
-    "ceos_surrogate_eval_device", "ceos_rve_set_precision", "ceos_rve_set_tangent_rtol", "ceos_rve_set_coarse",
+    "ceos_surrogate_eval_device", "ceos_rve_set_precision", "ceos_rve_set_tangent_rtol", "ceos_rve_set_coarse", "ceos_rve_set_warm_start",
     "ceos_mlp_gradients_weighted", "ceos_mlp_train_step_weighted", "ceos_mlp_set_training_precision",
     "ceos_set_pipeline", "ceos_rve_autotune", "ceos_rve_set_cg_group",
     "ceos_bc_set", "ceos_residual_bc", "ceos_residual_bc_device", "ceos_set_solver_precision",
@@ -146,6 +146,7 @@ def load_library(path=LIB_PATH):
     L.ceos_rve_set_precision.argtypes = [vp, i]
     L.ceos_rve_set_tangent_rtol.argtypes = [vp, d]
     L.ceos_rve_set_coarse.argtypes = [vp, i]
+    L.ceos_rve_set_warm_start.argtypes = [vp, i]
     L.ceos_rve_autotune.argtypes = [vp, i, C.POINTER(i), _dp]
     L.ceos_rve_set_cg_group.argtypes = [vp, i]
     L.ceos_rve_profile.argtypes = [vp, vp]
@@ -583,6 +584,10 @@ class RveBatch:
 
     def set_tangent_rtol(self, rtol):
         _check(load_library().ceos_rve_set_tangent_rtol(self._h, rtol), "ceos_rve_set_tangent_rtol")
+
+    def set_warm_start(self, on):
+        """Condensation solves from the RVE's last condensation solutions (on) or from zero."""
+        _check(load_library().ceos_rve_set_warm_start(self._h, 1 if on else 0), "ceos_rve_set_warm_start")
 
     def set_coarse(self, on):
         """Two-level CG preconditioner (aggregate translations) on or off (block Jacobi alone)."""

@@ -36,6 +36,7 @@ struct ceos_rve_batch {
     int cg_group = kCgGroupDefault;  // nodes per thread and operand batch in the CG sweeps
     double tangent_rtol = -1.0;   // < 0: pcg_rtol
     int coarse = 1;               // two-level CG preconditioner (aggregate translations)
+    int warm = 1;                 // condensation solves start from the last ones' solutions
     // device
     int *dorder = nullptr, *dside = nullptr, *dinfo = nullptr, *dqueue = nullptr;
     long long *dnode_off = nullptr, *dphase_off = nullptr;
@@ -43,6 +44,8 @@ struct ceos_rve_batch {
     double *dprops = nullptr, *dFbar = nullptr, *dFconv = nullptr, *dw = nullptr, *dP = nullptr, *dA = nullptr;
     double *dM = nullptr, *dV = nullptr;
     float* dC = nullptr;  // [slots][kRveMaxCoarse^2] coarse inverses
+    float* dVs = nullptr;  // [total nodes][27] the last condensation's solutions
+    int* dvvalid = nullptr;  // [n_rve]
     long long* dprof = nullptr;
     // subset solves: selected ids in queue order and in caller order, compact F / P / A / info
     int *dsel_q = nullptr, *dsel_u = nullptr, *dinfo_s = nullptr;
@@ -66,13 +69,14 @@ void free_all(ceos_rve_batch* b) {
     rfree(b->dorder); rfree(b->dside); rfree(b->dinfo); rfree(b->dqueue);
     rfree(b->dnode_off); rfree(b->dphase_off); rfree(b->dphase);
     rfree(b->dprops); rfree(b->dFbar); rfree(b->dFconv); rfree(b->dw); rfree(b->dP); rfree(b->dA);
-    rfree(b->dM); rfree(b->dV); rfree(b->dC); rfree(b->dprof);
+    rfree(b->dM); rfree(b->dV); rfree(b->dC); rfree(b->dVs); rfree(b->dvvalid); rfree(b->dprof);
     rfree(b->dsel_q); rfree(b->dsel_u); rfree(b->dinfo_s); rfree(b->dF_s); rfree(b->dP_s); rfree(b->dA_s);
 }
 
-__global__ void k_rve_reset(int n_rve, long long nodes3, double* Fconv, double* w, long long* prof) {
+__global__ void k_rve_reset(int n_rve, long long nodes3, double* Fconv, double* w, long long* prof, int* vvalid) {
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t < 2LL * n_rve) prof[t] = 0;
+    if (t < n_rve) vvalid[t] = 0;
     if (t < 9LL * n_rve) {
         const int k = (int)(t % 9);
         Fconv[t] = (k == 0 || k == 4 || k == 8) ? 1.0 : 0.0;
@@ -132,7 +136,7 @@ void launch_p(const RveParams& p, int grid, bool solve, bool tangent, bool f32, 
 int reset(ceos_rve_batch* b) {
     const long long n = std::max<long long>(9LL * b->n_rve, 3 * b->total_nodes);
     hipLaunchKernelGGL(k_rve_reset, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, b->stream, b->n_rve,
-                       3 * b->total_nodes, b->dFconv, b->dw, b->dprof);
+                       3 * b->total_nodes, b->dFconv, b->dw, b->dprof, b->dvvalid);
     RVE_HIP_OK(hipGetLastError());
     RVE_HIP_OK(hipStreamSynchronize(b->stream));
     return CEOS_OK;
@@ -180,6 +184,9 @@ int run(ceos_rve_batch* b, bool want_A, hipStream_t s, const int* order, int n_q
     p.prof = b->dprof;
     p.Cbuf = b->dC;
     p.coarse = b->coarse;
+    p.V = b->dVs;
+    p.vvalid = b->dvvalid;
+    p.warm = b->warm;
     hipLaunchKernelGGL(k_zero_queue, dim3(1), dim3(1), 0, s, b->dqueue);
     const int grid = std::min(n_queue, b->slots);
     const bool f32 = b->precision == CEOS_K_FP32_STORED;
@@ -253,6 +260,7 @@ int ceos_rve_batch_create(int n_rve, const int* side, const unsigned char* phase
         rmalloc(b->dM, (size_t)(243 * b->max_nodes * slots)) == hipSuccess &&
         rmalloc(b->dV, (size_t)(kRveVecPerNode * b->max_nodes * slots)) == hipSuccess &&
         rmalloc(b->dC, (size_t)kRveMaxCoarse * kRveMaxCoarse * slots) == hipSuccess &&
+        rmalloc(b->dVs, (size_t)27 * tot) == hipSuccess && rmalloc(b->dvvalid, n_rve) == hipSuccess &&
         rmalloc(b->dsel_q, n_rve) == hipSuccess && rmalloc(b->dsel_u, n_rve) == hipSuccess &&
         rmalloc(b->dinfo_s, 4LL * n_rve) == hipSuccess && rmalloc(b->dF_s, 9LL * n_rve) == hipSuccess &&
         rmalloc(b->dP_s, 9LL * n_rve) == hipSuccess && rmalloc(b->dA_s, 81LL * n_rve) == hipSuccess;
@@ -315,6 +323,12 @@ int ceos_rve_set_precision(ceos_rve_batch* b, int mode) {
 int ceos_rve_set_tangent_rtol(ceos_rve_batch* b, double rtol) {
     if (!b || !(rtol > 0.0 && rtol < 1.0)) return CEOS_ERR_ARG;
     b->tangent_rtol = rtol;
+    return CEOS_OK;
+}
+
+int ceos_rve_set_warm_start(ceos_rve_batch* b, int on) {
+    if (!b || (on != 0 && on != 1)) return CEOS_ERR_ARG;
+    b->warm = on;
     return CEOS_OK;
 }
 
@@ -459,16 +473,23 @@ int ceos_rve_state_save(ceos_rve_batch* b, const char* path) {
     RVE_HIP_OK(hipDeviceSynchronize());
     const long long n = b->n_rve, tot = b->total_nodes;
     std::vector<double> Fc(9 * n), w(3 * tot), P(9 * n), A(81 * n);
+    std::vector<int> vv(n);
+    std::vector<float> V(27 * tot);
     RVE_HIP_OK(hipMemcpy(Fc.data(), b->dFconv, sizeof(double) * Fc.size(), hipMemcpyDeviceToHost));
     RVE_HIP_OK(hipMemcpy(w.data(), b->dw, sizeof(double) * w.size(), hipMemcpyDeviceToHost));
     RVE_HIP_OK(hipMemcpy(P.data(), b->dP, sizeof(double) * P.size(), hipMemcpyDeviceToHost));
     RVE_HIP_OK(hipMemcpy(A.data(), b->dA, sizeof(double) * A.size(), hipMemcpyDeviceToHost));
+    RVE_HIP_OK(hipMemcpy(vv.data(), b->dvvalid, sizeof(int) * vv.size(), hipMemcpyDeviceToHost));
+    RVE_HIP_OK(hipMemcpy(V.data(), b->dVs, sizeof(float) * V.size(), hipMemcpyDeviceToHost));
     FILE* f = fopen(path, "wb");
     if (!f) return CEOS_ERR_ARG;
     const long long hdr[3] = {kRveMagic, n, tot};
     bool ok = fwrite(hdr, sizeof(hdr), 1, f) == 1 && fwrite(b->side.data(), sizeof(int), n, f) == (size_t)n;
     for (const auto* v : {&Fc, &w, &P, &A})
         ok = ok && fwrite(v->data(), sizeof(double), v->size(), f) == v->size();
+    // the warm-start solutions: a restarted batch then continues bitwise
+    ok = ok && fwrite(vv.data(), sizeof(int), vv.size(), f) == vv.size() &&
+         fwrite(V.data(), sizeof(float), V.size(), f) == V.size();
     ok = (fclose(f) == 0) && ok;
     return ok ? CEOS_OK : CEOS_ERR_ARG;
 }
@@ -484,14 +505,20 @@ int ceos_rve_state_load(ceos_rve_batch* b, const char* path) {
     bool ok = fread(hdr, sizeof(hdr), 1, f) == 1 && hdr[0] == kRveMagic && hdr[1] == n && hdr[2] == tot &&
               fread(side.data(), sizeof(int), n, f) == (size_t)n && side == b->side;
     std::vector<double> Fc(9 * n), w(3 * tot), P(9 * n), A(81 * n);
+    std::vector<int> vv(n);
+    std::vector<float> V(27 * tot);
     for (auto* v : {&Fc, &w, &P, &A})
         ok = ok && fread(v->data(), sizeof(double), v->size(), f) == v->size();
+    ok = ok && fread(vv.data(), sizeof(int), vv.size(), f) == vv.size() &&
+         fread(V.data(), sizeof(float), V.size(), f) == V.size();
     fclose(f);
     if (!ok) return CEOS_ERR_ARG;
     RVE_HIP_OK(hipMemcpy(b->dFconv, Fc.data(), sizeof(double) * Fc.size(), hipMemcpyHostToDevice));
     RVE_HIP_OK(hipMemcpy(b->dw, w.data(), sizeof(double) * w.size(), hipMemcpyHostToDevice));
     RVE_HIP_OK(hipMemcpy(b->dP, P.data(), sizeof(double) * P.size(), hipMemcpyHostToDevice));
     RVE_HIP_OK(hipMemcpy(b->dA, A.data(), sizeof(double) * A.size(), hipMemcpyHostToDevice));
+    RVE_HIP_OK(hipMemcpy(b->dvvalid, vv.data(), sizeof(int) * vv.size(), hipMemcpyHostToDevice));
+    RVE_HIP_OK(hipMemcpy(b->dVs, V.data(), sizeof(float) * V.size(), hipMemcpyHostToDevice));
     return CEOS_OK;
 }
 

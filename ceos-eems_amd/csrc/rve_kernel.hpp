@@ -85,6 +85,9 @@ struct RveParams {
                                   // solves (preconditioner set-up included), or nullptr
     float* Cbuf;                  // [slots][kRveMaxCoarse^2] inverse of the coarse matrix
     int coarse;                   // 1: two-level preconditioner (block Jacobi + aggregate translations)
+    float* V;                     // [nodes][27] the last condensation's solutions v_m (m = 3L + k), per node
+    int* vvalid;                  // [n_rve] 1: V holds the RVE's solutions of an earlier condensation
+    int warm;                     // 1: a condensation starts from V where it is valid
 };
 
 __device__ __forceinline__ double wave_sum(double v) {
@@ -855,7 +858,7 @@ __device__ void rve_coarse_apply3(const RveCtx<MT>& c, double* cv, double part[3
 // its[s] = iterations of solve s.
 template <typename MT, bool NT>
 __device__ void rve_pcg3(const RveCtx<MT>& c, const RveParams& p, int m0, double rtol, float* pl, double* red,
-                         double* cv, bool coarse, int its[3]) {
+                         double* cv, bool coarse, int its[3], const float* x0) {
     const int tid = threadIdx.x;
     const int nn = c.nn, nc = c.nc;
     const long long nn3 = 3LL * nn;
@@ -863,33 +866,67 @@ __device__ void rve_pcg3(const RveCtx<MT>& c, const RveParams& p, int m0, double
     double* r3 = rve_vec3(c, 1);
     double* q3 = rve_vec3(c, 2);
     double* z3 = rve_vec3(c, 3);
-    double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};  // r_s . z_s, r_s . r_s
+    // warm start (x0, FP32 per node [27 i + 3 m + k]): x = x0 exactly, also as the direction in LDS
+    // for r0 = -b - K x0; the stopping test stays relative to ||b||
+    if (x0) {
+        for (int i = tid; i < nn; i += kRveThreads) {
+#pragma unroll
+            for (int q = 0; q < 3; ++q)
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    const float v = i == 0 ? 0.0f : x0[27LL * i + 3 * (m0 + q) + k];
+                    pl[9 * i + 3 * q + k] = v;
+                    x3[9 * i + 3 * q + k] = (double)v;
+                }
+        }
+        __syncthreads();
+    }
+    double acc[9] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};  // r_s . z_s, r_s . r_s, b_s . b_s
     for (int i = tid; i < nn; i += kRveThreads) {
-        double b[9], mi[6];
+        double b[9], mi[6], y[9];
 #pragma unroll
         for (int q = 0; q < 3; ++q)
 #pragma unroll
             for (int k = 0; k < 3; ++k) b[3 * q + k] = c.B[(m0 + q) * nn3 + 3 * i + k];
 #pragma unroll
         for (int k = 0; k < 6; ++k) mi[k] = c.Minv[6 * i + k];
+        if (x0 && i != 0) {
+            rve_row3<MT, NT>(c, i, pl, y);
+        } else {
+#pragma unroll
+            for (int t = 0; t < 9; ++t) y[t] = 0.0;
+        }
         __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
         for (int q = 0; q < 3; ++q) {
             double rr[3], z[3];
 #pragma unroll
-            for (int k = 0; k < 3; ++k) rr[k] = i == 0 ? 0.0 : -b[3 * q + k];
+            for (int k = 0; k < 3; ++k) rr[k] = i == 0 ? 0.0 : -b[3 * q + k] - y[3 * q + k];
             apply_minv(mi, rr, z);
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
-                x3[9 * i + 3 * q + k] = 0.0;
+                if (!x0) {
+                    x3[9 * i + 3 * q + k] = 0.0;
+                    pl[9 * i + 3 * q + k] = (float)z[k];
+                } else {
+                    z3[9 * i + 3 * q + k] = z[k];  // p is still x0 for the other rows' products
+                }
                 r3[9 * i + 3 * q + k] = rr[k];
-                pl[9 * i + 3 * q + k] = (float)z[k];
                 acc[q] += rr[k] * z[k];
                 acc[3 + q] += rr[k] * rr[k];
+                const double bk = i == 0 ? 0.0 : b[3 * q + k];
+                acc[6 + q] += bk * bk;
             }
         }
     }
-    double sums[9] = {acc[0], acc[1], acc[2], acc[3], acc[4], acc[5], 0.0, 0.0, 0.0};
+    if (x0) {
+        __syncthreads();  // every row's product with x0 is done
+        for (int i = tid; i < nn; i += kRveThreads) {
+#pragma unroll
+            for (int t = 0; t < 9; ++t) pl[9 * i + t] = (float)z3[9 * i + t];
+        }
+    }
+    double sums[12] = {acc[0], acc[1], acc[2], acc[3], acc[4], acc[5], 0.0, 0.0, 0.0, acc[6], acc[7], acc[8]};
     if (coarse) {
         __syncthreads();  // r complete
         double pc[3];
@@ -898,14 +935,14 @@ __device__ void rve_pcg3(const RveCtx<MT>& c, const RveParams& p, int m0, double
         sums[7] = pc[1];
         sums[8] = pc[2];
     }
-    wg_sum<9>(sums, red);
+    wg_sum<12>(sums, red);
     double rz[3], tol2[3];
     bool act[3];
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
         rz[q] = sums[q] + sums[6 + q];
-        tol2[q] = rtol * rtol * sums[3 + q];
-        act[q] = sums[3 + q] > 0.0;
+        tol2[q] = rtol * rtol * sums[9 + q];
+        act[q] = sums[3 + q] > tol2[q];
         its[q] = 0;
     }
     if (coarse) {
@@ -1382,11 +1419,22 @@ __global__ __launch_bounds__(kRveThreads) void k_rve_tangent(RveParams p) {
             rve_pcg<MT, G, NT>(c, p, c.B, -1.0, p.tangent_rtol, lds, red, cv, coarse);
             continue;
         }
+        // the last condensation's solutions of this RVE start the solves (a nearby state: the
+        // initial residual is a fraction of b); this one's are kept for the next
+        float* Vr = p.V + 27LL * p.node_off[rid];
+        const bool warm = p.warm && p.vvalid[rid];
         for (int m0 = 0; m0 < 9; m0 += 3) {
             int its[3];
-            rve_pcg3<MT, NT>(c, p, m0, p.tangent_rtol, reinterpret_cast<float*>(lds), red, cv, coarse, its);
+            rve_pcg3<MT, NT>(c, p, m0, p.tangent_rtol, reinterpret_cast<float*>(lds), red, cv, coarse, its,
+                             warm ? Vr : nullptr);
             pcg_total += its[0] + its[1] + its[2];
             const double* x3 = rve_vec3(c, 0);
+            for (int i = tid; i < c.nn; i += kRveThreads) {
+#pragma unroll
+                for (int q = 0; q < 3; ++q)
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) Vr[27LL * i + 3 * (m0 + q) + k] = (float)x3[9 * i + 3 * q + k];
+            }
             double dots[27];  // [9 s + k] = b_k . v_(m0+s)
 #pragma unroll
             for (int k = 0; k < 27; ++k) dots[k] = 0.0;
@@ -1419,7 +1467,10 @@ __global__ __launch_bounds__(kRveThreads) void k_rve_tangent(RveParams p) {
             const int idx = lo * 9 - (lo * (lo - 1)) / 2 + (hi - lo);
             Aout[tid] = (sT[idx] + sT[46 + tid]) / sT[45];
         }
-        if (tid == 0) p.info[4 * rid + 2] += pcg_total;
+        if (tid == 0) {
+            p.info[4 * rid + 2] += pcg_total;
+            p.vvalid[rid] = 1;
+        }
         __syncthreads();
     }
 }

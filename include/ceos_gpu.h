@@ -452,6 +452,13 @@ int ceos_rve_set_tangent_rtol(ceos_rve_batch *b, double rtol);
  * counterpart of its own in the reference. */
 int ceos_rve_set_coarse(ceos_rve_batch *b, int on);
 
+/* 1 (default): the nine condensation solves of an RVE start from that RVE's solutions of its last
+ * condensation (kept per node in FP32; a nearby state, so the initial residual is a fraction of
+ * the right-hand side), 0: from zero.  The stopping test is ||r|| <= tangent_rtol ||b|| either
+ * way, so only the iteration counts (info[2]) differ.  ceos_rve_reset forgets the solutions;
+ * ceos_rve_state_save/load carry them.  No counterpart in the reference (PARDISO is direct). */
+int ceos_rve_set_warm_start(ceos_rve_batch *b, int on);
+
 /* Tile of the Newton steps' CG vector sweeps: every thread loads the operands of g nodes (2, 4 or
  * 8) before computing them.  ceos_rve_autotune times, for each g, one linear solve of that CG per
  * RVE of a sample of the batch (up to max_rves, largest first, at their converged states, to
@@ -485,7 +492,8 @@ int ceos_rve_profile(ceos_rve_batch *b, long long *ticks);
 /* Converged fluctuation of RVE `rve` (3 side^3 doubles, torus node x + s (y + s z)). */
 int ceos_rve_get_fluctuation(ceos_rve_batch *b, int rve, double *w);
 
-/* Converged state of the whole batch (F_conv, fluctuation, last homogenized P and dP/dF) to and
+/* Converged state of the whole batch (F_conv, fluctuation, last homogenized P and dP/dF, and the
+ * condensation solutions that warm-start the next condensation) to and
  * from a file, for restarting a multiscale analysis (FEM/ModFEMAnalysis.f90:2827-2956 rebuilds
  * the same state by replaying the end-of-step macro F through the RVE analysis).  The file carries
  * the RVE count and sides; loading into a batch of other RVEs returns CEOS_ERR_ARG.  Both calls

@@ -190,6 +190,31 @@ def test_two_level_preconditioner_agrees_and_cuts_iterations():
                 assert out[True][2][k, 2] < out[False][2][k, 2]
 
 
+def test_condensation_warm_start_agrees_and_saves_iterations():
+    """Condensation solves started from the RVE's last condensation solutions (default) give the
+    cold solves' tangent to the solver tolerance, the same P, and fewer CG iterations; FP32-stored
+    tangent as in the FE^2 workload."""
+    sides = [8, 12]
+    F1 = np.stack([_F(81, 0.04), _F(82, 0.04)])
+    F2 = F1 + 2e-3 * np.stack([np.eye(3) + 0.3 * np.ones((3, 3)), np.eye(3) - 0.2 * np.ones((3, 3))])
+    out = {}
+    for warm in (False, True):
+        b, _ = _batch(sides, [91, 92])
+        b.set_precision(gpu.K_FP32_STORED)
+        b.set_tangent_rtol(1e-8)
+        b.set_warm_start(warm)
+        b.solve(F1, tangent=True)
+        P, A, info = b.solve(F2, tangent=True)
+        print("warm", warm, info.tolist(), flush=True)
+        assert np.all(info[:, 0] == 0)
+        out[warm] = (P, A, info)
+    for k in range(len(sides)):
+        print(k, rel(out[True][1][k], out[False][1][k]), flush=True)
+        assert np.array_equal(out[True][0][k], out[False][0][k])
+        assert rel(out[True][1][k], out[False][1][k]) < 1e-6
+        assert out[True][2][k, 2] < out[False][2][k, 2]
+
+
 def test_autotune_picks_a_tile_and_every_tile_agrees():
     """The CG sweep tile changes only the order of independent loads: solves and condensed
     tangents with every tile agree to the solver tolerances; the autotuner times all three."""
@@ -220,7 +245,7 @@ def test_state_save_load_continues_bitwise(tmp_path):
     F1 = np.stack([_F(40 + k, 0.04) for k in range(3)])
     F2 = np.stack([_F(50 + k, 0.05) for k in range(3)])
     a, _ = _batch(sides, ids)
-    a.solve(F1)
+    a.solve(F1, tangent=True)  # the condensation's solutions warm-start the next one, and travel too
     path = str(tmp_path / "rve.state")
     a.state_save(path)
     P_a, A_a, info_a = a.solve(F2, tangent=True)
