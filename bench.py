@@ -352,7 +352,8 @@ def fe2_bench(no_cpu):
                    "rve_tangent_storage": "fp32 (FP64 residual, dot products, convergence tests)",
                    "rve_newton_rtol": 1e-10, "rve_cg_rtol": 1e-10, "rve_condensation_rtol": 1e-8,
                    "rve_condensation_start": "each RVE's solutions of its last condensation (from zero after a "
-                                             "reset: the first evaluation)",
+                                             "reset: the first evaluation); as dw/dF they also predict the "
+                                             "Newton solve's first iterate",
                    "macro_gp_stretch_range": [float(lam.min()), float(lam.max())],
                    "rve_cg_tile": {"chosen": tile, "probe_s": {str(g): float(t) for g, t in zip((2, 4, 8), tile_s)}},
                    "device_chains": "HIP graphs (captured at the first evaluation of each path)",

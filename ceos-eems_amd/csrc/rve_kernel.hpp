@@ -1270,6 +1270,25 @@ __global__ __launch_bounds__(kRveThreads) void k_rve_solve(RveParams p) {
         const int rid = p.order[qi];
         RveCtx<MT> c = rve_ctx<MT>(p, rid, slot);
         for (int k = tid; k < 3 * c.nn; k += kRveThreads) c.wc[k] = c.wconv[k];
+        if (p.warm && p.vvalid[rid]) {
+            // first-order predictor for the first attempt: the last condensation's solutions are
+            // dw/dF at the converged state, w += sum_m v_m (F - F_conv)_m (a cut-back restarts
+            // from w_conv); Newton then corrects to the same test
+            const float* Vr = p.V + 27LL * p.node_off[rid];
+            double dF[9];
+#pragma unroll
+            for (int m = 0; m < 9; ++m) dF[m] = p.Fbar[9 * rid + m] - p.Fconv[9 * rid + m];
+            for (int i = tid; i < c.nn; i += kRveThreads) {
+                if (i == 0) continue;
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    double dw = 0.0;
+#pragma unroll
+                    for (int m = 0; m < 9; ++m) dw += (double)Vr[27LL * i + 3 * m + k] * dF[m];
+                    c.wc[3 * i + k] += dw;
+                }
+            }
+        }
         __syncthreads();
 
         double alpha_min = 0.0, alpha = 1.0;

@@ -454,8 +454,10 @@ int ceos_rve_set_coarse(ceos_rve_batch *b, int on);
 
 /* 1 (default): the nine condensation solves of an RVE start from that RVE's solutions of its last
  * condensation (kept per node in FP32; a nearby state, so the initial residual is a fraction of
- * the right-hand side), 0: from zero.  The stopping test is ||r|| <= tangent_rtol ||b|| either
- * way, so only the iteration counts (info[2]) differ.  ceos_rve_reset forgets the solutions;
+ * the right-hand side), and, being dw/dF at the converged state, those solutions also predict
+ * the next solve's first iterate, w = w_conv + sum_m v_m (F - F_conv)_m (first attempt only);
+ * 0: condensation from zero, Newton from w_conv.  The stopping tests are unchanged, so results
+ * agree to the solver tolerances and only the iteration counts (info[1], info[2]) differ.  ceos_rve_reset forgets the solutions;
  * ceos_rve_state_save/load carry them.  No counterpart in the reference (PARDISO is direct). */
 int ceos_rve_set_warm_start(ceos_rve_batch *b, int on);
 
