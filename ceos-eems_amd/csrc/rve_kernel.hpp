@@ -9,7 +9,8 @@
 //   Newton (max-abs norm, relative test)  Tools/MathTools/NonlinearSolver/ModNewtonRaphsonFull.f90:74-163
 //   golden-ratio cut-back of the load      FEM/ModFEMAnalysis.f90:1270-1340
 //   homogenized first Piola stress        MultiScale/ModMultiscaleAnalysis.f90:140-329
-//   the PARDISO solve is replaced by block-Jacobi preconditioned CG.
+//   the PARDISO solve is replaced by preconditioned CG (block Jacobi plus a coarse level of
+//   aggregate translations).
 //
 // Formulation on the torus.  The periodic voxel mesh of side n has n^3 nodes (the master nodes
 // of the reference's T): node (i,j,k) = i + n (j + n k); element (i,j,k) joins the nodes
